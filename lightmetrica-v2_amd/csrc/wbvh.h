@@ -1,0 +1,57 @@
+// wbvh.h -- compressed 8-wide BVH layout shared by the host builder, the
+// host walker and the gfx950 traversal kernel.
+//
+// Replaces the reference's binary/4-wide BVH node types
+// (src/liblightmetrica/accel/accel_bvh_sahbin.cpp:36-55,
+//  src/liblightmetrica/accel/accel_qbvh.cpp:54-137).
+//
+// Node: 128 bytes = two 64-byte blocks, 128-byte aligned, so one node is one
+// L2 line on CDNA.  Child boxes are quantised to 16 bits per axis relative to
+// the node origin with a power-of-two scale per axis, rounded outward.  Up to
+// eight children per node; internal children of a node are stored
+// contiguously (child_base + rank), and the triangles of all leaf children of
+// a node are stored contiguously (tri_base + offset).  Each leaf holds 1..3
+// triangles.
+//
+// Child slots are assigned per node by octant (after Ylitie, Karras and
+// Laine, "Efficient Incoherent Ray Traversal on GPUs Through Compressed Wide
+// BVHs", HPG 2017): slot s holds the child that comes first along the
+// direction octant s, so visiting hit slots in the order (p ^ octant) for
+// p = 0..7 is approximately front to back for every ray.
+#pragma once
+#include <stdint.h>
+
+#define WBVH_WIDTH 8
+#define WBVH_MAX_LEAF_TRIS 3
+#define WBVH_STACK 32         // traversal stack entries per ray (node groups)
+
+// Triangle test selector (stored in the accel; picks the kernel variant).
+#define WBVH_ISECT_TRIACCEL 0     // Wald projection test, reference semantics
+#define WBVH_ISECT_WATERTIGHT 1   // Woop/Benthin/Wald watertight test
+
+struct alignas(128) WNode {
+    float ox, oy, oz;          // quantisation origin (lower corner)
+    uint8_t ex, ey, ez;        // per-axis scale = 2^(e - 127)
+    uint8_t imask;             // bit s set: slot s is an internal child
+    uint32_t child_base;       // index of the first internal child
+    uint32_t tri_base;         // index of the first triangle of leaf children
+    uint8_t meta[8];           // leaf slot: (count << 5) | offset; empty: 0
+    uint16_t qlo[3][8];        // quantised child lower bounds, per axis
+    uint16_t qhi[3][8];        // quantised child upper bounds, per axis
+};
+static_assert(sizeof(WNode) == 128, "WNode must be 128 bytes");
+
+// Triangle record, 48 bytes = 3 x float4.
+//   watertight: {v0.xyz, id} {v1.xyz, 0} {v2.xyz, 0}
+//   triaccel:   {k, n_u, n_v, n_d} {a_u, a_v, b_nu, b_nv} {c_nu, c_nv, id, 0}
+//   (the TriAccel fields follow include/lightmetrica/triaccel.h:35-48)
+struct alignas(16) WTri {
+    float f[12];
+};
+static_assert(sizeof(WTri) == 48, "WTri must be 48 bytes");
+
+// Per-ray record at the C boundary (32 B) and hit record (16 B).
+struct WRay { float o[3]; float tmin; float d[3]; float tmax; };
+struct WHit { float t, b0, b1; uint32_t id; };
+
+#define WBVH_MISS 0xffffffffu

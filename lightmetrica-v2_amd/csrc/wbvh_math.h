@@ -1,0 +1,199 @@
+// wbvh_math.h -- ray setup, child-box test and triangle tests, shared by the
+// host walker and the gfx950 traversal kernel so both take exactly the same
+// decisions (compiled with -ffp-contract=off; every FMA below is explicit).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include "wbvh.h"
+
+#define WB_HD __host__ __device__ __forceinline__
+
+// Relative widening of the far slab distance; with the world-space padding the
+// builder applies to child boxes it keeps the quantised box test conservative
+// under FP32 rounding.
+#define WBVH_FAR_SCALE 1.00000048f   // 1 + 2^-21
+
+WB_HD float wb_as_float(uint32_t u)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __uint_as_float(u);
+#else
+    union { uint32_t u; float f; } c; c.u = u; return c.f;
+#endif
+}
+WB_HD uint32_t wb_as_uint(float f)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __float_as_uint(f);
+#else
+    union { uint32_t u; float f; } c; c.f = f; return c.u;
+#endif
+}
+WB_HD int wb_popc(uint32_t x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __popc(x);
+#else
+    return __builtin_popcount(x);
+#endif
+}
+WB_HD int wb_ctz(uint32_t x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __ffs(x) - 1;
+#else
+    return __builtin_ctz(x);
+#endif
+}
+
+struct WRayPre {
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+    float ix, iy, iz;          // clamped reciprocal direction (box test)
+    uint32_t oct;              // direction octant: bit a set when d[a] < 0
+    // watertight shear constants (Woop, Benthin, Wald, JCGT 2013)
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+};
+
+WB_HD float wb_sel3(int k, float x, float y, float z)
+{
+    return k == 0 ? x : (k == 1 ? y : z);
+}
+
+WB_HD float wb_safe_inv(float d)
+{
+    const float tiny = 1e-20f;
+    const float dd = fabsf(d) < tiny ? copysignf(tiny, d) : d;
+    return 1.0f / dd;
+}
+
+WB_HD void wb_ray_setup(WRayPre& r, float ox, float oy, float oz, float tmin,
+                        float dx, float dy, float dz, float tmax)
+{
+    r.ox = ox; r.oy = oy; r.oz = oz; r.tmin = tmin;
+    r.dx = dx; r.dy = dy; r.dz = dz; r.tmax = tmax;
+    r.ix = wb_safe_inv(dx); r.iy = wb_safe_inv(dy); r.iz = wb_safe_inv(dz);
+    r.oct = (dx < 0.0f ? 1u : 0u) | (dy < 0.0f ? 2u : 0u) | (dz < 0.0f ? 4u : 0u);
+    int kz = 0;
+    if (fabsf(dy) > fabsf(dx)) kz = 1;
+    if (fabsf(dz) > fabsf(wb_sel3(kz, dx, dy, dz))) kz = 2;
+    int kx = kz + 1; if (kx == 3) kx = 0;
+    int ky = kx + 1; if (ky == 3) ky = 0;
+    const float dkz = wb_sel3(kz, dx, dy, dz);
+    if (dkz < 0.0f) { const int t = kx; kx = ky; ky = t; }
+    r.kx = kx; r.ky = ky; r.kz = kz;
+    r.Sx = wb_sel3(kx, dx, dy, dz) / dkz;
+    r.Sy = wb_sel3(ky, dx, dy, dz) / dkz;
+    r.Sz = 1.0f / dkz;
+}
+
+// Box test of all eight child slots of a node against [tmin, tmax].
+// Returns the hit mask of internal children permuted by the ray octant
+// (bit p <-> slot p ^ oct) and, through tri_mask, the triangles of hit leaf
+// children (bit i <-> triangle tri_base + i).
+WB_HD uint32_t wb_node_hits(const WNode& n, const WRayPre& r, float tmax, uint32_t& tri_mask)
+{
+    const float sx = wb_as_float((uint32_t)n.ex << 23) * r.ix;
+    const float sy = wb_as_float((uint32_t)n.ey << 23) * r.iy;
+    const float sz = wb_as_float((uint32_t)n.ez << 23) * r.iz;
+    const float bx = (n.ox - r.ox) * r.ix;
+    const float by = (n.oy - r.oy) * r.iy;
+    const float bz = (n.oz - r.oz) * r.iz;
+    uint32_t inner = 0, tris = 0;
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (int i = 0; i < 8; i++) {
+        const uint32_t meta = n.meta[i];
+        const bool internal = (n.imask >> i) & 1u;
+        if (!internal && meta == 0) continue;
+        const float x0 = fmaf((float)n.qlo[0][i], sx, bx), x1 = fmaf((float)n.qhi[0][i], sx, bx);
+        const float y0 = fmaf((float)n.qlo[1][i], sy, by), y1 = fmaf((float)n.qhi[1][i], sy, by);
+        const float z0 = fmaf((float)n.qlo[2][i], sz, bz), z1 = fmaf((float)n.qhi[2][i], sz, bz);
+        const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
+        const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+        if (!(tn <= tf * WBVH_FAR_SCALE)) continue;
+        if (internal) {
+            inner |= 1u << (i ^ r.oct);
+        } else {
+            const uint32_t cnt = meta >> 5, off = meta & 31u;
+            tris |= ((1u << cnt) - 1u) << off;
+        }
+    }
+    tri_mask = tris;
+    return inner;
+}
+
+// Kahan's difference of products a*b - c*d with two FMAs: exactly zero iff
+// the exact value is zero, otherwise of the exact value's sign.
+WB_HD float wb_dop(float a, float b, float c, float d)
+{
+    const float w = c * d;
+    const float e = fmaf(-c, d, w);
+    const float f = fmaf(a, b, -w);
+    return f + e;
+}
+
+// Watertight test.  On a hit with t in [tmin, tmax] writes t and the
+// unnormalised (V, W, det); b0 = V/det (2nd vertex), b1 = W/det (3rd vertex).
+WB_HD bool wb_tri_watertight(const WTri& tri, const WRayPre& r, float tmax,
+                             float& t, float& V, float& W, float& det)
+{
+    const float Ax0 = tri.f[0] - r.ox, Ay0 = tri.f[1] - r.oy, Az0 = tri.f[2] - r.oz;
+    const float Bx0 = tri.f[4] - r.ox, By0 = tri.f[5] - r.oy, Bz0 = tri.f[6] - r.oz;
+    const float Cx0 = tri.f[8] - r.ox, Cy0 = tri.f[9] - r.oy, Cz0 = tri.f[10] - r.oz;
+    const float Akz = wb_sel3(r.kz, Ax0, Ay0, Az0), Bkz = wb_sel3(r.kz, Bx0, By0, Bz0), Ckz = wb_sel3(r.kz, Cx0, Cy0, Cz0);
+    const float Ax = fmaf(-r.Sx, Akz, wb_sel3(r.kx, Ax0, Ay0, Az0));
+    const float Ay = fmaf(-r.Sy, Akz, wb_sel3(r.ky, Ax0, Ay0, Az0));
+    const float Bx = fmaf(-r.Sx, Bkz, wb_sel3(r.kx, Bx0, By0, Bz0));
+    const float By = fmaf(-r.Sy, Bkz, wb_sel3(r.ky, Bx0, By0, Bz0));
+    const float Cx = fmaf(-r.Sx, Ckz, wb_sel3(r.kx, Cx0, Cy0, Cz0));
+    const float Cy = fmaf(-r.Sy, Ckz, wb_sel3(r.ky, Cx0, Cy0, Cz0));
+    const float U = wb_dop(Cx, By, Cy, Bx);
+    const float Vv = wb_dop(Ax, Cy, Ay, Cx);
+    const float Ww = wb_dop(Bx, Ay, By, Ax);
+    if ((U < 0.0f || Vv < 0.0f || Ww < 0.0f) && (U > 0.0f || Vv > 0.0f || Ww > 0.0f)) return false;
+    const float dt = (U + Vv) + Ww;
+    if (dt == 0.0f) return false;
+    const float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
+    const float T = fmaf(U, Az, fmaf(Vv, Bz, Ww * Cz));
+    const float tt = T / dt;
+    if (!(tt >= r.tmin && tt <= tmax)) return false;
+    t = tt; V = Vv; W = Ww; det = dt;
+    return true;
+}
+
+// TriAccel test with the reference's operation order
+// (include/lightmetrica/triaccel.h:93-151); u = weight of the 2nd vertex,
+// v = weight of the 3rd vertex.
+WB_HD bool wb_tri_triaccel(const WTri& tri, const WRayPre& r, float tmax,
+                           float& t, float& u, float& v)
+{
+    const uint32_t k = wb_as_uint(tri.f[0]);
+    float o_u, o_v, o_k, d_u, d_v, d_k;
+    if (k == 0) { o_u = r.oy; o_v = r.oz; o_k = r.ox; d_u = r.dy; d_v = r.dz; d_k = r.dx; }
+    else if (k == 1) { o_u = r.oz; o_v = r.ox; o_k = r.oy; d_u = r.dz; d_v = r.dx; d_k = r.dy; }
+    else if (k == 2) { o_u = r.ox; o_v = r.oy; o_k = r.oz; d_u = r.dx; d_v = r.dy; d_k = r.dz; }
+    else return false;
+    const float n_u = tri.f[1], n_v = tri.f[2], n_d = tri.f[3];
+    const float a_u = tri.f[4], a_v = tri.f[5], b_nu = tri.f[6], b_nv = tri.f[7];
+    const float c_nu = tri.f[8], c_nv = tri.f[9];
+    const float demon = d_u * n_u + d_v * n_v + d_k;
+    if (demon == 0.0f) return false;
+    const float tt = (n_d - o_u * n_u - o_v * n_v - o_k) / demon;
+    if (tt < r.tmin || tt > tmax) return false;
+    const float hu = o_u + tt * d_u - a_u;
+    const float hv = o_v + tt * d_v - a_v;
+    const float uu = hv * b_nu + hu * b_nv;
+    const float vv = hu * c_nu + hv * c_nv;
+    if (!(uu >= 0.0f && vv >= 0.0f && uu + vv <= 1.0f)) return false;
+    t = tt; u = uu; v = vv;
+    return true;
+}
+
+WB_HD uint32_t wb_tri_id(const WTri& tri, int isect)
+{
+    return isect == WBVH_ISECT_TRIACCEL ? wb_as_uint(tri.f[10]) : wb_as_uint(tri.f[3]);
+}

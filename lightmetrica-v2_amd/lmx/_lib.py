@@ -1,0 +1,80 @@
+"""ctypes binding of liblmx.so (declarations mirror include/lmx.h)."""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "liblmx.so")
+
+
+class LmxError(RuntimeError):
+    """A call into liblmx returned an error code."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"liblmx error {code}: {msg}")
+        self.code = code
+
+
+class lmx_ray(ctypes.Structure):
+    _fields_ = [("o", ctypes.c_float * 3), ("tmin", ctypes.c_float),
+                ("d", ctypes.c_float * 3), ("tmax", ctypes.c_float)]
+
+
+class lmx_hit(ctypes.Structure):
+    _fields_ = [("t", ctypes.c_float), ("b0", ctypes.c_float), ("b1", ctypes.c_float),
+                ("id", ctypes.c_uint32)]
+
+
+class lmx_build_opts(ctypes.Structure):
+    _fields_ = [("isect", ctypes.c_int32), ("nthreads", ctypes.c_int32)]
+
+
+class lmx_accel_info(ctypes.Structure):
+    _fields_ = [("num_nodes", ctypes.c_uint32), ("num_tris", ctypes.c_uint32),
+                ("max_depth", ctypes.c_uint32), ("isect", ctypes.c_int32),
+                ("node_bytes", ctypes.c_uint64), ("tri_bytes", ctypes.c_uint64)]
+
+
+# Every symbol include/lmx.h declares, with its ctypes signature.
+SIGNATURES = {
+    "lmx_last_error": (ctypes.c_char_p, []),
+    "lmx_accel_build": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
+                                       ctypes.POINTER(lmx_build_opts), ctypes.POINTER(ctypes.c_void_p)]),
+    "lmx_accel_free": (None, [ctypes.c_void_p]),
+    "lmx_accel_info_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(lmx_accel_info)]),
+    "lmx_accel_upload": (ctypes.c_int, [ctypes.c_void_p]),
+    "lmx_accel_intersect": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                           ctypes.c_void_p, ctypes.c_void_p]),
+    "lmx_accel_intersect_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                                ctypes.c_void_p, ctypes.c_int32]),
+    "lmx_accel_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                       ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                       ctypes.c_int32]),
+}
+
+
+def _load():
+    # torch loads its bundled HIP runtime under the name libamdhip64.so with
+    # soname libamdhip64.so.7.  Importing torch first makes liblmx's
+    # dependency on libamdhip64.so.7 resolve to that same, already loaded
+    # runtime; loading liblmx first would bring in /opt/rocm's copy as well,
+    # and two HIP runtimes in one process do not share devices.
+    import torch  # noqa: F401
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python __graft_entry__.py build` "
+            "(the HIP extension is required; there is no CPU fallback)")
+    so = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(so, name)
+        fn.restype = res
+        fn.argtypes = args
+    return so
+
+
+lib = _load()
+
+
+def check(rc):
+    if rc != 0:
+        raise LmxError(rc, lib.lmx_last_error().decode(errors="replace"))
+    return rc

@@ -1,0 +1,172 @@
+"""Deterministic procedural scenes and ray sets for the benchmark configs.
+
+All generators are pure numpy and seeded; the same call yields the same
+arrays on every machine.  Triangles are returned in enumeration order
+(primitive order x face order), the order in which the reference numbers its
+triangles (src/liblightmetrica/accel/accel_qbvh.cpp:164-189).
+"""
+import numpy as np
+
+FLT_MAX = np.float32(np.finfo(np.float32).max)
+T_MIN = np.float32(1e-4)      # Math::EpsIsect(), include/lightmetrica/math.h:1668
+
+_ICO_T = (1.0 + 5.0 ** 0.5) / 2.0
+_ICO_V = np.array([
+    [-1, _ICO_T, 0], [1, _ICO_T, 0], [-1, -_ICO_T, 0], [1, -_ICO_T, 0],
+    [0, -1, _ICO_T], [0, 1, _ICO_T], [0, -1, -_ICO_T], [0, 1, -_ICO_T],
+    [_ICO_T, 0, -1], [_ICO_T, 0, 1], [-_ICO_T, 0, -1], [-_ICO_T, 0, 1]], dtype=np.float64)
+_ICO_F = np.array([
+    [0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11],
+    [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6], [7, 1, 8],
+    [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9],
+    [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], dtype=np.int64)
+
+
+def icosphere(subdivisions):
+    """Unit icosphere: the icosahedron split `subdivisions` times
+    (20 * 4**subdivisions triangles, outward counter-clockwise winding).
+    Returns float64 vertices (V, 3) and int64 faces (F, 3)."""
+    v = _ICO_V / np.linalg.norm(_ICO_V, axis=1, keepdims=True)
+    f = _ICO_F.copy()
+    for _ in range(subdivisions):
+        nv = v.shape[0]
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]], axis=0)
+        e.sort(axis=1)
+        key = e[:, 0] * nv + e[:, 1]
+        uniq, inv = np.unique(key, return_inverse=True)
+        a0 = uniq // nv
+        a1 = uniq % nv
+        mid = v[a0] + v[a1]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        v = np.concatenate([v, mid], axis=0)
+        nf = f.shape[0]
+        m01 = nv + inv[:nf]
+        m12 = nv + inv[nf:2 * nf]
+        m20 = nv + inv[2 * nf:]
+        f = np.concatenate([
+            np.stack([f[:, 0], m01, m20], axis=1),
+            np.stack([f[:, 1], m12, m01], axis=1),
+            np.stack([f[:, 2], m20, m12], axis=1),
+            np.stack([m01, m12, m20], axis=1)], axis=0)
+    return v, f
+
+
+def _hash3(ix, iy, iz, seed):
+    """32-bit integer hash of lattice coordinates -> uniform in [-1, 1]."""
+    h = (ix.astype(np.uint32) * np.uint32(0x8DA6B343)) ^ (iy.astype(np.uint32) * np.uint32(0xD8163841)) \
+        ^ (iz.astype(np.uint32) * np.uint32(0xCB1AB31F)) ^ np.uint32(seed * 0x9E3779B9 & 0xFFFFFFFF)
+    h ^= h >> np.uint32(16)
+    h *= np.uint32(0x7FEB352D)
+    h ^= h >> np.uint32(15)
+    h *= np.uint32(0x846CA68B)
+    h ^= h >> np.uint32(16)
+    return h.astype(np.float64) / 4294967295.0 * 2.0 - 1.0
+
+
+def value_noise(p, seed, octaves=3, base_freq=4.0):
+    """Value noise: hashed lattice values, smoothstep-trilinear interpolation,
+    `octaves` octaves of doubling frequency and halving amplitude, normalised
+    to [-1, 1]."""
+    total = np.zeros(p.shape[0])
+    amp_sum = 0.0
+    amp = 1.0
+    freq = base_freq
+    with np.errstate(over="ignore"):
+        for o in range(octaves):
+            q = p * freq
+            i0 = np.floor(q).astype(np.int64)
+            fr = q - i0
+            w = fr * fr * (3.0 - 2.0 * fr)
+            acc = np.zeros(p.shape[0])
+            for dx in (0, 1):
+                wx = w[:, 0] if dx else 1.0 - w[:, 0]
+                for dy in (0, 1):
+                    wy = w[:, 1] if dy else 1.0 - w[:, 1]
+                    for dz in (0, 1):
+                        wz = w[:, 2] if dz else 1.0 - w[:, 2]
+                        acc += wx * wy * wz * _hash3(i0[:, 0] + dx, i0[:, 1] + dy, i0[:, 2] + dz, seed + 7919 * o)
+            total += amp * acc
+            amp_sum += amp
+            amp *= 0.5
+            freq *= 2.0
+    return total / amp_sum
+
+
+def displaced_icosphere(subdivisions, amplitude=0.1, seed=1, center=(0.0, 0.0, 0.0), radius=1.0):
+    """Icosphere with vertices displaced radially by 3-octave value noise:
+    r = radius * (1 + amplitude * noise(unit position)).  float32 verts, uint32 tris."""
+    v, f = icosphere(subdivisions)
+    n = value_noise(v, seed)
+    v = v * (radius * (1.0 + amplitude * n))[:, None] + np.asarray(center, dtype=np.float64)[None, :]
+    return v.astype(np.float32), f.astype(np.uint32)
+
+
+def make_rays(o, d, tmin=T_MIN, tmax=FLT_MAX):
+    """Pack origins and directions into (n, 8) float32 ray records."""
+    n = o.shape[0]
+    r = np.empty((n, 8), dtype=np.float32)
+    r[:, 0:3] = o
+    r[:, 3] = tmin
+    r[:, 4:7] = d
+    r[:, 7] = tmax
+    return r
+
+
+def pinhole_rays(width, height, eye, center, up, fov_deg, tmin=T_MIN):
+    """Primary rays through pixel centres of a pinhole camera (vertical fov),
+    row 0 at the bottom as in the reference's raster convention
+    (src/liblightmetrica/asset/sensor/sensor_pinhole.cpp:79-90)."""
+    eye = np.asarray(eye, dtype=np.float64)
+    vz = eye - np.asarray(center, dtype=np.float64)
+    vz /= np.linalg.norm(vz)
+    vx = np.cross(np.asarray(up, dtype=np.float64), vz)
+    vx /= np.linalg.norm(vx)
+    vy = np.cross(vz, vx)
+    tf = np.tan(np.radians(fov_deg) * 0.5)
+    aspect = width / height
+    xs = (np.arange(width) + 0.5) / width * 2.0 - 1.0
+    ys = (np.arange(height) + 0.5) / height * 2.0 - 1.0
+    X, Y = np.meshgrid(xs, ys)
+    de = np.stack([aspect * tf * X.ravel(), tf * Y.ravel(), -np.ones(X.size)], axis=1)
+    de /= np.linalg.norm(de, axis=1, keepdims=True)
+    d = de[:, 0:1] * vx + de[:, 1:2] * vy + de[:, 2:3] * vz
+    o = np.broadcast_to(eye, d.shape)
+    return make_rays(o.astype(np.float32), d.astype(np.float32), tmin)
+
+
+def incoherent_rays(n, lo, hi, seed, tmin=T_MIN):
+    """Origins uniform in the box [lo, hi], directions uniform on the sphere
+    by the reference's mapping z = 1 - 2 u0, phi = 2 pi u1
+    (include/lightmetrica/sampler.h:79-85)."""
+    rng = np.random.default_rng(seed)
+    lo = np.asarray(lo, dtype=np.float64)
+    hi = np.asarray(hi, dtype=np.float64)
+    o = lo + (hi - lo) * rng.random((n, 3))
+    u = rng.random((n, 2))
+    z = 1.0 - 2.0 * u[:, 0]
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    phi = 2.0 * np.pi * u[:, 1]
+    d = np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)
+    return make_rays(o.astype(np.float32), d.astype(np.float32), tmin)
+
+
+# BVH traversal microbenchmark: 1,310,720-triangle displaced icosphere,
+# 2048x2048 coherent primary rays + as many incoherent rays.
+TRAVERSAL_SUBDIV = 8
+TRAVERSAL_SEED = 42
+TRAVERSAL_CAMERA = dict(eye=(0.0, 0.6, 3.2), center=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov_deg=40.0)
+
+
+def traversal_scene(subdiv=TRAVERSAL_SUBDIV):
+    return displaced_icosphere(subdiv, amplitude=0.1, seed=TRAVERSAL_SEED)
+
+
+def traversal_rays(verts, res=2048, n_incoherent=None, seed=TRAVERSAL_SEED + 1):
+    """(coherent, incoherent) ray sets of the traversal benchmark."""
+    coh = pinhole_rays(res, res, **TRAVERSAL_CAMERA)
+    if n_incoherent is None:
+        n_incoherent = res * res
+    lo = verts.min(axis=0)
+    hi = verts.max(axis=0)
+    inc = incoherent_rays(n_incoherent, lo, hi, seed)
+    return coh, inc

@@ -1,0 +1,114 @@
+/*
+ * orc.h -- CPU oracle for the Lightmetrica ray-tracing hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * product path is compared against; it is linked or loaded only by tests/,
+ * by __graft_entry__.smoke() and by the cpu_baseline leg of bench.py.  The
+ * product library never calls into it.
+ *
+ * Every function restates the reference algorithm it names (file:line under
+ * hi2p-perim/lightmetrica-v2).  Arithmetic is FP32, compiled with
+ * -ffp-contract=off so that every product and sum rounds exactly as written.
+ *
+ * Parity pins (what checks each function against something independent):
+ *   orc_triaccel_load / orc_triaccel_intersect
+ *       bit-exact against the reference's own triaccel.h compiled into
+ *       oracle/_ref (tests/test_oracle_ref.py), and the analytic quads of
+ *       src/lightmetrica-test/test_accel3.cpp:272-345 (tests/test_oracle.py).
+ *   orc_bound_intersect   bit-exact against the reference's bound.h via _ref.
+ *   orc_watertight_intersect
+ *       closed-form ray/plane hits, the FP64 arbiter (t and barycentrics of the
+ *       exact solution), and the watertightness invariant: a ray through a
+ *       shared edge or vertex of a closed mesh is never lost.
+ *   orc_closest_hit_naive brute force; qbvh traversal must equal it.
+ *   orc_qbvh_*            hit index equal to brute force on meshes of up to
+ *                         1e5 triangles (tests/test_oracle.py).
+ *   orc_sample_*          bit-exact against the reference's sampler.h via _ref.
+ */
+#ifndef LMX_ORC_H
+#define LMX_ORC_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Wald projection record; same field order and size (48 B) as
+ * include/lightmetrica/triaccel.h:35-48. */
+typedef struct orc_triaccel {
+    uint32_t k;
+    float n_u, n_v, n_d;
+    float a_u, a_v, b_nu, b_nv;
+    float c_nu, c_nv;
+    uint32_t face_index;
+    uint32_t prim_index;
+} orc_triaccel;
+
+/* Ray record used at the C boundary: origin, tmin, direction, tmax (32 B). */
+typedef struct orc_ray {
+    float o[3];
+    float tmin;
+    float d[3];
+    float tmax;
+} orc_ray;
+
+/* Hit record: t, b0 (weight of the 2nd vertex), b1 (weight of the 3rd
+ * vertex), global triangle id (0xffffffff = miss).  16 B. */
+typedef struct orc_hit {
+    float t, b0, b1;
+    uint32_t id;
+} orc_hit;
+
+/* Tie rule for equal t. */
+enum {
+    ORC_TIE_LAST_WINS = 0,   /* reference: t > maxT rejects only strictly greater */
+    ORC_TIE_MIN_ID = 1       /* canonical rule shared with the HIP kernel */
+};
+
+enum {
+    ORC_TEST_TRIACCEL = 0,
+    ORC_TEST_WATERTIGHT = 1
+};
+
+int orc_triaccel_load(const float A[3], const float B[3], const float C[3], orc_triaccel *out);
+int orc_triaccel_intersect(const orc_triaccel *tri, const float o[3], const float d[3],
+                           float mint, float maxt, float *u, float *v, float *t);
+int orc_watertight_intersect(const float v0[3], const float v1[3], const float v2[3],
+                             const float o[3], const float d[3], float mint, float maxt,
+                             float *b0, float *b1, float *t);
+int orc_bound_intersect(const float bmin[3], const float bmax[3], const float o[3],
+                        const float d[3], float tmin, float tmax);
+
+/* Brute-force closest hit over all triangles (accel_naive.cpp:92-124). */
+void orc_closest_hit_naive(const float *verts, const uint32_t *tris, uint32_t ntri,
+                           const orc_ray *rays, uint32_t nray, int test, int tie,
+                           orc_hit *hits, int nthreads);
+
+/* 4-wide QBVH (accel_qbvh.cpp) */
+typedef struct orc_qbvh orc_qbvh;
+orc_qbvh *orc_qbvh_build(const float *verts, const uint32_t *tris, uint32_t ntri);
+void orc_qbvh_free(orc_qbvh *q);
+uint32_t orc_qbvh_num_nodes(const orc_qbvh *q);
+void orc_qbvh_intersect(const orc_qbvh *q, const orc_ray *rays, uint32_t nray,
+                        orc_hit *hits, int nthreads);
+
+/* Binary binned-SAH BVH (accel_bvh_sahbin.cpp) */
+typedef struct orc_bvh orc_bvh;
+orc_bvh *orc_bvh_build(const float *verts, const uint32_t *tris, uint32_t ntri);
+void orc_bvh_free(orc_bvh *b);
+uint32_t orc_bvh_num_nodes(const orc_bvh *b);
+void orc_bvh_intersect(const orc_bvh *b, const orc_ray *rays, uint32_t nray,
+                       orc_hit *hits, int nthreads);
+
+/* Sampling routines of include/lightmetrica/sampler.h */
+void orc_sample_concentric_disk(const float u[2], float out[2]);
+void orc_sample_cosine_hemisphere(const float u[2], float out[3]);
+void orc_sample_uniform_sphere(const float u[2], float out[3]);
+void orc_sample_uniform_triangle(const float u[2], float out[2]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
