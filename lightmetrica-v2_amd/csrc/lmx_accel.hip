@@ -9,9 +9,15 @@
 //   * a per-lane stack of node groups in LDS (a group = the not yet visited
 //     hit children of one node: child base, hit mask, internal mask), so the
 //     stack never holds more entries than the tree has levels;
-//   * lanes whose ray finished are refilled from the work queue as a batch:
-//     one ballot counts them, one atomic per wave claims that many rays and
-//     mbcnt ranks give every idle lane its ray;
+//   * the next node is chosen right after a node's box test and its load is
+//     issued before that node's triangles are tested, so node latency
+//     overlaps triangle work; each lane loads exactly one node per loop
+//     iteration at one site, which keeps the 128-byte node in registers
+//     across iterations without copies;
+//   * lanes whose ray finished are refilled as a batch once at least
+//     LMX_REFILL_MIN lanes are idle: one ballot counts them, mbcnt ranks give
+//     every idle lane its ray, and the rays come from a per-wave pool that
+//     one atomic refills LMX_CLAIM_CHUNK rays at a time;
 //   * XCD-aware work split: workgroup b runs on XCD (b mod 8); the rays are
 //     cut into 8 contiguous regions and a wave first drains the region of
 //     its own XCD (neighbouring rays share BVH nodes in that XCD's L2), then
@@ -59,6 +65,8 @@ struct lmx_accel {
     WTri* d_tris = nullptr;
     uint32_t* d_counters = nullptr;   // 8 region cursors
     int grid = 0;
+    int stack_depth = 0;              // LDS stack entries per lane
+    size_t lds_bytes = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -137,7 +145,19 @@ void wbvh_walk(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* nv, uint
 // device kernels
 
 #define TRACE_WAVE 64
-#define REFILL_MIN 16   // refill when at least this many lanes are idle
+#ifndef LMX_REFILL_MIN
+#define LMX_REFILL_MIN 48      // refill when at least this many lanes of a wave are idle
+#endif
+#ifndef LMX_TRACE_WAVES_PER_EU
+#define LMX_TRACE_WAVES_PER_EU 1   // occupancy hint for the register allocator
+#endif
+#ifndef LMX_CLAIM_CHUNK
+#define LMX_CLAIM_CHUNK 128    // rays claimed from a region cursor per atomic
+#endif
+#ifndef LMX_TRI_BATCH
+#define LMX_TRI_BATCH 1        // triangle loads kept in flight together (1 or 2)
+#endif
+#define REFILL_MIN LMX_REFILL_MIN
 
 __global__ void k_init_cursors(uint32_t* cursors, uint32_t n)
 {
@@ -150,18 +170,53 @@ __device__ __forceinline__ uint32_t region_end(uint32_t j, uint32_t n)
     return (uint32_t)(((uint64_t)n * (j + 1u)) / 8u);
 }
 
+__device__ __forceinline__ void load_node(WNode& nd, const WNode* src)
+{
+    const uint4* s = reinterpret_cast<const uint4*>(src);
+    uint4* d = reinterpret_cast<uint4*>(&nd);
+#pragma unroll
+    for (int i = 0; i < 8; i++) d[i] = s[i];
+}
+
+__device__ __forceinline__ void load_tri(WTri& t, const WTri* src)
+{
+    const float4* s = reinterpret_cast<const float4*>(src);
+    float4* d = reinterpret_cast<float4*>(&t);
+    d[0] = s[0]; d[1] = s[1]; d[2] = s[2];
+}
+
+// Closest-hit update with the canonical tie rule (smallest id among equal t).
 template <int ISECT>
-__global__ __launch_bounds__(TRACE_WAVE) void k_trace(const WNode* __restrict__ nodes,
+__device__ __forceinline__ void test_tri(const WTri& tri, const WRayPre& r, float& best_t, uint32_t& best_id,
+                                         float& h0, float& h1, float& hdet)
+{
+    const uint32_t id = wb_tri_id(tri, ISECT);
+    float t, a0, a1, dt = 1.0f;
+    bool h;
+    if (ISECT == WBVH_ISECT_WATERTIGHT) h = wb_tri_watertight(tri, r, best_t, t, a0, a1, dt);
+    else h = wb_tri_triaccel(tri, r, best_t, t, a0, a1);
+    if (h && (t < best_t || id < best_id)) {
+        best_t = t; best_id = id; h0 = a0; h1 = a1; hdet = dt;
+    }
+}
+
+// The LDS stack is sized per launch to the BVH's depth (stack_depth entries
+// of 8 bytes per lane), which is what sets the kernel's occupancy.
+template <int ISECT>
+__global__ __launch_bounds__(TRACE_WAVE) __attribute__((amdgpu_waves_per_eu(LMX_TRACE_WAVES_PER_EU)))
+void k_trace(const WNode* __restrict__ nodes,
                                                       const WTri* __restrict__ tris,
                                                       const WRay* __restrict__ rays,
                                                       WHit* __restrict__ hits, uint32_t nray,
-                                                      uint32_t* __restrict__ cursors)
+                                                      uint32_t* __restrict__ cursors,
+                                                      int stack_depth, uint32_t refill_min)
 {
-    __shared__ uint2 stack[WBVH_STACK][TRACE_WAVE];
+    extern __shared__ uint2 stack[];     // [stack_depth][TRACE_WAVE]
     const uint32_t lane = threadIdx.x;
 
     uint32_t region = blockIdx.x & 7u;
     uint32_t regions_left = 8;          // regions not yet found exhausted
+    uint32_t pool_base = 0, pool_cnt = 0;   // claimed, not yet started rays
     bool valid = false;
     uint32_t ray_id = 0;
     WRayPre r;
@@ -169,105 +224,127 @@ __global__ __launch_bounds__(TRACE_WAVE) void k_trace(const WNode* __restrict__ 
     uint32_t best_id = WBVH_MISS;
     uint32_t g_base = 0, g_hits = 0, g_imask = 0;
     int sp = 0;
+    WNode nd;                           // node to test in the next iteration
+    load_node(nd, nodes);               // idle lanes always hold the root
 
     while (true) {
-        // ---- batch refill of idle lanes
+        // ---- batch refill of idle lanes, from this wave's pool of claimed
+        // rays first, then from a freshly claimed chunk
         const uint64_t idle = __ballot(!valid);
         const uint32_t nidle = (uint32_t)__popcll(idle);
-        if (regions_left != 0 && nidle >= REFILL_MIN) {
-            uint32_t base = 0, got = 0;
-            if (lane == 0) {
-                while (regions_left != 0) {
-                    const uint32_t end = region_end(region, nray);
-                    const uint32_t old = atomicAdd(&cursors[region], nidle);
-                    if (old < end) {
-                        base = old;
-                        got = min(nidle, end - old);
-                        break;
+        if ((regions_left != 0 || pool_cnt != 0) && (nidle >= refill_min || nidle == TRACE_WAVE)) {
+            const uint32_t take0 = min(nidle, pool_cnt);
+            const uint32_t base0 = pool_base;
+            pool_base += take0;
+            pool_cnt -= take0;
+            uint32_t base1 = 0, got1 = 0;
+            if (take0 < nidle && regions_left != 0) {
+                uint32_t cb = 0, cg = 0;
+                if (lane == 0) {
+                    while (regions_left != 0) {
+                        const uint32_t end = region_end(region, nray);
+                        const uint32_t old = atomicAdd(&cursors[region], (uint32_t)LMX_CLAIM_CHUNK);
+                        if (old < end) {
+                            cb = old;
+                            cg = min((uint32_t)LMX_CLAIM_CHUNK, end - old);
+                            break;
+                        }
+                        region = (region + 1u) & 7u;
+                        regions_left--;
                     }
-                    region = (region + 1u) & 7u;
-                    regions_left--;
                 }
+                cb = __shfl(cb, 0);
+                cg = __shfl(cg, 0);
+                regions_left = __shfl(regions_left, 0);
+                region = __shfl(region, 0);
+                got1 = min(nidle - take0, cg);
+                base1 = cb;
+                pool_base = cb + got1;
+                pool_cnt = cg - got1;
             }
-            base = __shfl(base, 0);
-            got = __shfl(got, 0);
-            regions_left = __shfl(regions_left, 0);
-            region = __shfl(region, 0);
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
                                    __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
-            if (!valid && rank < got) {
-                ray_id = base + rank;
+            if (!valid && rank < take0 + got1) {
+                ray_id = rank < take0 ? base0 + rank : base1 + (rank - take0);
                 const float4 a = reinterpret_cast<const float4*>(rays)[2 * ray_id];
                 const float4 b = reinterpret_cast<const float4*>(rays)[2 * ray_id + 1];
                 wb_ray_setup(r, a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w);
                 best_t = b.w; best_id = WBVH_MISS; h0 = h1 = 0.0f; hdet = 1.0f;
-                g_base = 0; g_hits = 1u << (0u ^ r.oct); g_imask = 1u;
-                sp = 0;
+                sp = 0;                        // nd already holds the root
                 valid = true;
             }
         }
-        if (__ballot(valid) == 0) {
-            if (regions_left == 0) break;
-            continue;
-        }
+        // single loop back-edge: with no ray in any lane and work left, the
+        // body below does nothing but reload the root
+        if (__ballot(valid) == 0 && regions_left == 0 && pool_cnt == 0) break;
 
+        // 1. box test of the node loaded in the previous iteration, and the
+        //    choice of the next node.  Every lane then loads exactly one node
+        //    per iteration at a single site (lanes without a next node reload
+        //    the root, which is what a refilled lane needs first), so the
+        //    node registers are carried around the loop without copies.
+        uint32_t tmask = 0, tbase = 0, child = 0;
+        bool more = false;
         if (valid) {
+            const uint32_t inner = wb_node_hits(nd, r, best_t, tmask);
+            tbase = nd.tri_base;
+            g_base = nd.child_base; g_hits = inner; g_imask = nd.imask;
+            more = true;
             if (g_hits == 0) {
                 if (sp == 0) {
-                    float4 o;
-                    if (best_id == WBVH_MISS) {
-                        o = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(WBVH_MISS));
-                    } else if (ISECT == WBVH_ISECT_WATERTIGHT) {
-                        o = make_float4(best_t, h0 / hdet, h1 / hdet, __uint_as_float(best_id));
-                    } else {
-                        o = make_float4(best_t, h0, h1, __uint_as_float(best_id));
-                    }
-                    reinterpret_cast<float4*>(hits)[ray_id] = o;
-                    valid = false;
+                    more = false;
                 } else {
                     sp--;
-                    const uint2 e = stack[sp][lane];
+                    const uint2 e = stack[sp * TRACE_WAVE + lane];
                     g_base = e.x; g_hits = e.y & 0xffu; g_imask = e.y >> 8;
                 }
             }
+            if (more) {
+                const uint32_t p = (uint32_t)(__ffs(g_hits) - 1);
+                g_hits &= g_hits - 1u;
+                const uint32_t s = p ^ r.oct;
+                child = g_base + (uint32_t)__popc(g_imask & ((1u << s) - 1u));
+                if (g_hits && sp < stack_depth) {
+                    stack[sp * TRACE_WAVE + lane] = make_uint2(g_base, g_hits | (g_imask << 8));
+                    sp++;
+                }
+            }
         }
+        // 2. start loading the next node; its latency overlaps the
+        //    triangle tests below
+        load_node(nd, nodes + child);
         if (valid) {
-            const uint32_t p = (uint32_t)(__ffs(g_hits) - 1);
-            g_hits &= g_hits - 1u;
-            const uint32_t s = p ^ r.oct;
-            const uint32_t child = g_base + (uint32_t)__popc(g_imask & ((1u << s) - 1u));
-            if (g_hits && sp < WBVH_STACK) {
-                stack[sp][lane] = make_uint2(g_base, g_hits | (g_imask << 8));
-                sp++;
-            }
-            WNode nd;
-            {
-                const uint4* src = reinterpret_cast<const uint4*>(nodes + child);
-                uint4* dst = reinterpret_cast<uint4*>(&nd);
-#pragma unroll
-                for (int i = 0; i < 8; i++) dst[i] = src[i];
-            }
-            uint32_t tmask = 0;
-            const uint32_t inner = wb_node_hits(nd, r, best_t, tmask);
-            g_base = nd.child_base; g_hits = inner; g_imask = nd.imask;
-            const uint32_t tbase = nd.tri_base;
+            // 3. triangles of the tested node
             while (tmask) {
-                const uint32_t k = (uint32_t)(__ffs(tmask) - 1);
+                const uint32_t k1 = (uint32_t)(__ffs(tmask) - 1);
                 tmask &= tmask - 1u;
-                WTri tri;
-                {
-                    const float4* src = reinterpret_cast<const float4*>(tris + tbase + k);
-                    float4* dst = reinterpret_cast<float4*>(&tri);
-                    dst[0] = src[0]; dst[1] = src[1]; dst[2] = src[2];
+#if LMX_TRI_BATCH == 2
+                const bool two = tmask != 0u;
+                const uint32_t k2 = two ? (uint32_t)(__ffs(tmask) - 1) : k1;
+                if (two) tmask &= tmask - 1u;
+                WTri t1, t2;
+                load_tri(t1, tris + tbase + k1);
+                load_tri(t2, tris + tbase + k2);
+                test_tri<ISECT>(t1, r, best_t, best_id, h0, h1, hdet);
+                if (two) test_tri<ISECT>(t2, r, best_t, best_id, h0, h1, hdet);
+#else
+                WTri t1;
+                load_tri(t1, tris + tbase + k1);
+                test_tri<ISECT>(t1, r, best_t, best_id, h0, h1, hdet);
+#endif
+            }
+            // 4. ray finished: write its hit and free the lane
+            if (!more) {
+                float4 o;
+                if (best_id == WBVH_MISS) {
+                    o = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(WBVH_MISS));
+                } else if (ISECT == WBVH_ISECT_WATERTIGHT) {
+                    o = make_float4(best_t, h0 / hdet, h1 / hdet, __uint_as_float(best_id));
+                } else {
+                    o = make_float4(best_t, h0, h1, __uint_as_float(best_id));
                 }
-                const uint32_t id = wb_tri_id(tri, ISECT);
-                float t, a0, a1, dt = 1.0f;
-                bool h;
-                if (ISECT == WBVH_ISECT_WATERTIGHT) h = wb_tri_watertight(tri, r, best_t, t, a0, a1, dt);
-                else h = wb_tri_triaccel(tri, r, best_t, t, a0, a1);
-                if (h && (t < best_t || id < best_id)) {
-                    best_t = t; best_id = id; h0 = a0; h1 = a1; hdet = dt;
-                }
+                reinterpret_cast<float4*>(hits)[ray_id] = o;
+                valid = false;
             }
         }
     }
@@ -355,11 +432,15 @@ int lmx_accel_upload(lmx_accel* a)
     int cus = 0;
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
         return hip_error(e, "lmx_accel_upload: device attribute");
+    // A node group is pushed only when descending one level, so the stack
+    // never holds more entries than the tree has levels below the root.
+    a->stack_depth = (int)std::max(1u, a->host.max_depth);
+    a->lds_bytes = (size_t)a->stack_depth * TRACE_WAVE * sizeof(uint2);
     int per_cu = 0;
     if (a->host.isect == WBVH_ISECT_WATERTIGHT)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<WBVH_ISECT_WATERTIGHT>, TRACE_WAVE, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<WBVH_ISECT_WATERTIGHT>, TRACE_WAVE, a->lds_bytes);
     else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<WBVH_ISECT_TRIACCEL>, TRACE_WAVE, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<WBVH_ISECT_TRIACCEL>, TRACE_WAVE, a->lds_bytes);
     if (e != hipSuccess) return hip_error(e, "lmx_accel_upload: occupancy query");
     a->grid = std::max(8, cus * std::max(1, per_cu));
     a->device = dev;
@@ -378,11 +459,13 @@ int lmx_accel_intersect(lmx_accel* a, const lmx_ray* rays, uint32_t n, lmx_hit* 
     const int grid = std::max(1, std::min(a->grid, (int)((n + TRACE_WAVE - 1) / TRACE_WAVE)));
     k_init_cursors<<<1, 64, 0, s>>>(a->d_counters, n);
     if (a->host.isect == WBVH_ISECT_WATERTIGHT)
-        k_trace<WBVH_ISECT_WATERTIGHT><<<grid, TRACE_WAVE, 0, s>>>(
-            a->d_nodes, a->d_tris, reinterpret_cast<const WRay*>(rays), reinterpret_cast<WHit*>(hits), n, a->d_counters);
+        k_trace<WBVH_ISECT_WATERTIGHT><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
+            a->d_nodes, a->d_tris, reinterpret_cast<const WRay*>(rays), reinterpret_cast<WHit*>(hits), n,
+            a->d_counters, a->stack_depth, REFILL_MIN);
     else
-        k_trace<WBVH_ISECT_TRIACCEL><<<grid, TRACE_WAVE, 0, s>>>(
-            a->d_nodes, a->d_tris, reinterpret_cast<const WRay*>(rays), reinterpret_cast<WHit*>(hits), n, a->d_counters);
+        k_trace<WBVH_ISECT_TRIACCEL><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
+            a->d_nodes, a->d_tris, reinterpret_cast<const WRay*>(rays), reinterpret_cast<WHit*>(hits), n,
+            a->d_counters, a->stack_depth, REFILL_MIN);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "lmx_accel_intersect: launch");
     return LMX_OK;

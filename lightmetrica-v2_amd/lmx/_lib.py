@@ -3,7 +3,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(HERE), "liblmx.so")
+# LMX_LIBRARY selects another build of the library (for comparing builds).
+LIB_PATH = os.environ.get("LMX_LIBRARY") or os.path.join(os.path.dirname(HERE), "liblmx.so")
 
 
 class LmxError(RuntimeError):

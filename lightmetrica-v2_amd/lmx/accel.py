@@ -38,10 +38,11 @@ class Accel:
         h = ctypes.c_void_p()
         check(lib.lmx_accel_build(_c(verts), self.nvert, _c(tris), self.ntri, ctypes.byref(opts), ctypes.byref(h)))
         self._h = h
+        self._free = lib.lmx_accel_free     # kept for use during interpreter shutdown
 
     def close(self):
         if getattr(self, "_h", None):
-            lib.lmx_accel_free(self._h)
+            self._free(self._h)
             self._h = None
 
     def __del__(self):

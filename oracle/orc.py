@@ -103,6 +103,7 @@ class QBVH:
         self._v = np.ascontiguousarray(verts, dtype=np.float32)
         self._t = np.ascontiguousarray(tris, dtype=np.uint32)
         self._h = lib().orc_qbvh_build(_p(self._v), _p(self._t), self._t.shape[0])
+        self._free = lib().orc_qbvh_free
 
     def intersect(self, rays, nthreads=8):
         r = np.ascontiguousarray(rays, dtype=np.float32)
@@ -115,7 +116,7 @@ class QBVH:
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().orc_qbvh_free(self._h)
+            self._free(self._h)
             self._h = None
 
 
@@ -126,6 +127,7 @@ class BVH:
         self._v = np.ascontiguousarray(verts, dtype=np.float32)
         self._t = np.ascontiguousarray(tris, dtype=np.uint32)
         self._h = lib().orc_bvh_build(_p(self._v), _p(self._t), self._t.shape[0])
+        self._free = lib().orc_bvh_free
 
     def intersect(self, rays, nthreads=8):
         r = np.ascontiguousarray(rays, dtype=np.float32)
@@ -138,7 +140,7 @@ class BVH:
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().orc_bvh_free(self._h)
+            self._free(self._h)
             self._h = None
 
 
