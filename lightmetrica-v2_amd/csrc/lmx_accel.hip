@@ -405,6 +405,14 @@ int lmx_accel_info_get(const lmx_accel* a, lmx_accel_info* info)
     return LMX_OK;
 }
 
+int lmx_accel_export(const lmx_accel* a, void* nodes, void* tris)
+{
+    if (!a) return set_error(LMX_EINVAL, "lmx_accel_export: NULL accel");
+    if (nodes) std::memcpy(nodes, a->host.nodes.data(), a->host.nodes.size() * sizeof(WNode));
+    if (tris) std::memcpy(tris, a->host.tris.data(), a->host.tris.size() * sizeof(WTri));
+    return LMX_OK;
+}
+
 int lmx_accel_upload(lmx_accel* a)
 {
     if (!a) return set_error(LMX_EINVAL, "lmx_accel_upload: NULL accel");

@@ -21,6 +21,14 @@ MISS = 0xFFFFFFFF
 FLT_MAX = np.float32(np.finfo(np.float32).max)
 
 
+# struct WNode (csrc/wbvh.h), 128 bytes
+WNODE_DTYPE = np.dtype([
+    ("o", np.float32, 3), ("e", np.uint8, 3), ("imask", np.uint8),
+    ("child_base", np.uint32), ("tri_base", np.uint32), ("meta", np.uint8, 8),
+    ("qlo", np.uint16, (3, 8)), ("qhi", np.uint16, (3, 8))])
+assert WNODE_DTYPE.itemsize == 128
+
+
 def _c(a):
     return ctypes.c_void_p(a.ctypes.data)
 
@@ -53,6 +61,14 @@ class Accel:
         check(lib.lmx_accel_info_get(self._h, ctypes.byref(i)))
         return {"num_nodes": i.num_nodes, "num_tris": i.num_tris, "max_depth": i.max_depth,
                 "isect": i.isect, "node_bytes": i.node_bytes, "tri_bytes": i.tri_bytes}
+
+    def export(self):
+        """(nodes, tris) as structured numpy arrays (layouts of csrc/wbvh.h)."""
+        info = self.info()
+        nodes = np.zeros(info["num_nodes"], dtype=WNODE_DTYPE)
+        tris = np.zeros((info["num_tris"], 12), dtype=np.float32)
+        check(lib.lmx_accel_export(self._h, _c(nodes), _c(tris)))
+        return nodes, tris
 
     def upload(self):
         check(lib.lmx_accel_upload(self._h))

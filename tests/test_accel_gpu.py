@@ -71,3 +71,24 @@ def test_full_size_traversal_benchmark(gpu):
     ids = accel.hit_ids(hg)
     assert (ids[: coh.shape[0]] != accel.MISS).mean() > 0.2
     assert (ids[coh.shape[0]:] != accel.MISS).mean() > 0.3
+
+
+@pytest.mark.parametrize("simple2", [False, True])
+@pytest.mark.parametrize("mode", [accel.ISECT_TRIACCEL, accel.ISECT_WATERTIGHT])
+def test_accel3_analytic_quads_on_gpu(gpu, simple2, mode):
+    """src/lightmetrica-test/test_accel3.cpp:272-345 through the HIP kernel:
+    p, gn and uv within EpsLarge = 1e-3 of the analytic answer."""
+    from test_oracle import (SIMPLE_F, SIMPLE_UV, SIMPLE_V, SIMPLE2_F, SIMPLE2_UV, SIMPLE2_V,
+                             hit_geometry, quad_rays)
+    v, f, uv = (SIMPLE2_V, SIMPLE2_F, SIMPLE2_UV) if simple2 else (SIMPLE_V, SIMPLE_F, SIMPLE_UV)
+    rays, expect = quad_rays(simple2)
+    a = accel.Accel(v, f, isect=mode)
+    hg = _gpu_hits(a, rays, gpu)
+    assert (accel.hit_ids(hg) != accel.MISS).all()
+    p, gn, tuv = hit_geometry(v, f, uv, rays, hg)
+    np.testing.assert_allclose(p, expect, atol=1e-3)
+    n_exp = np.array([0, 0, 1], np.float32) if not simple2 else np.array([1, 0, 1], np.float32) / np.sqrt(2)
+    np.testing.assert_allclose(gn, np.broadcast_to(n_exp, gn.shape), atol=1e-3)
+    np.testing.assert_allclose(tuv, expect[:, 0:2], atol=1e-3)
+    ho = orc.closest_hit_naive(v, f, rays, test=mode, tie=orc.TIE_MIN_ID)
+    np.testing.assert_array_equal(hg.view(np.uint32), ho.view(np.uint32))

@@ -1,0 +1,259 @@
+"""The CPU oracle, pinned before it is trusted.
+
+* Golden vectors written by the reference's own code (tests/golden/
+  reference_vectors.npz, from include/lightmetrica/{triaccel,bound,sampler}.h
+  compiled where they lie): the oracle must reproduce them bit for bit.
+* The reference's only traversal conformance test, the analytic quads of
+  src/lightmetrica-test/test_accel3.cpp:272-345 (p, gn and uv within
+  EpsLarge = 1e-3).
+* Invariants: the oracle's QBVH and binary BVH find the brute-force closest
+  hit; the watertight test loses no ray through shared edges or vertices of
+  a closed mesh; where the watertight and the TriAccel test disagree, an FP64
+  solve shows the ray grazes an edge (or the two hits tie in t).
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import orc
+
+with np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.npz")) as _z:
+    GOLD = {k: _z[k] for k in _z.files}
+FMAX = np.float32(np.finfo(np.float32).max)
+
+
+def _p(a):
+    # the caller keeps `a` alive across the call
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def test_triaccel_load_and_intersect_match_reference_vectors():
+    lib = orc.lib()
+    tri, o, d = _f32(GOLD["triaccel_tri"]), _f32(GOLD["triaccel_o"]), _f32(GOLD["triaccel_d"])
+    for i in range(tri.shape[0]):
+        rec = np.zeros(12, np.uint32)
+        rc = lib.orc_triaccel_load(_p(tri[i, 0]), _p(tri[i, 1]), _p(tri[i, 2]), _p(rec))
+        assert rc == GOLD["triaccel_load_rc"][i]
+        assert rec[0] == GOLD["triaccel_rec"][i][0]
+        if rec[0] < 3:
+            assert np.array_equal(rec[1:10], GOLD["triaccel_rec"][i][1:10]), i
+        u = np.zeros(1, np.float32); v = np.zeros(1, np.float32); t = np.zeros(1, np.float32)
+        h = lib.orc_triaccel_intersect(_p(rec), _p(o[i]), _p(d[i]), 0.0, FMAX, _p(u), _p(v), _p(t))
+        assert h == GOLD["triaccel_hit"][i], i
+        if h:
+            got = np.array([t[0], u[0], v[0]], np.float32)
+            assert np.array_equal(got.view(np.uint32), GOLD["triaccel_tuv"][i].view(np.uint32)), i
+
+
+def test_bound_intersect_matches_reference_vectors():
+    lib = orc.lib()
+    lo, hi, o, d = (_f32(GOLD[k]) for k in ("bound_lo", "bound_hi", "bound_o", "bound_d"))
+    for i in range(lo.shape[0]):
+        h = lib.orc_bound_intersect(_p(lo[i]), _p(hi[i]), _p(o[i]), _p(d[i]), 1e-4, FMAX)
+        assert h == GOLD["bound_hit"][i], i
+
+
+@pytest.mark.parametrize("fn,key,dim", [
+    ("orc_sample_concentric_disk", "sample_disk", 2),
+    ("orc_sample_cosine_hemisphere", "sample_hemi", 3),
+    ("orc_sample_uniform_sphere", "sample_sphere", 3),
+    ("orc_sample_uniform_triangle", "sample_tri", 2),
+])
+def test_samplers_match_reference_vectors(fn, key, dim):
+    f = getattr(orc.lib(), fn)
+    u = _f32(GOLD["sample_u"])
+    out = np.zeros((u.shape[0], dim), np.float32)
+    for i in range(u.shape[0]):
+        f(_p(u[i]), _p(out[i]))
+    np.testing.assert_array_equal(out.view(np.uint32), GOLD[key].view(np.uint32))
+
+
+# --- test_accel3.cpp:75-233 stub meshes ---------------------------------
+SIMPLE_V = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                     [0, 0, -1], [1, 0, -1], [1, 1, -1], [0, 1, -1]], np.float32)
+SIMPLE_F = np.array([[0, 1, 2], [0, 2, 3], [4, 5, 6], [4, 6, 7]], np.uint32)
+SIMPLE_UV = np.array([[0, 0], [1, 0], [1, 1], [0, 1]] * 2, np.float32)
+SIMPLE2_V = np.array([[0, 0, 0], [1, 0, -1], [1, 1, -1], [0, 1, 0]], np.float32)
+SIMPLE2_F = np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+SIMPLE2_UV = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32)
+
+
+def quad_rays(simple2):
+    """The 81 rays of test_accel3.cpp Simple (:287-296) / Simple2 (:329-338)."""
+    rays = []
+    expect = []
+    for i in range(1, 10):
+        y = np.float32(0.1) * np.float32(i)
+        for j in range(1, 10):
+            x = np.float32(0.1) * np.float32(j)
+            if simple2:
+                o = np.array([x, y, 1], np.float32)
+                d = np.array([0, 0, -1], np.float32)
+                expect.append((x, y, -x))
+            else:
+                o = np.array([0, 0, 1], np.float32)
+                d = np.array([x, y, -1], np.float64)
+                d = (d / np.linalg.norm(d)).astype(np.float32)
+                expect.append((x, y, 0))
+            rays.append(np.concatenate([o, [0.0], d, [FMAX]]).astype(np.float32))
+    return np.array(rays, np.float32), np.array(expect, np.float32)
+
+
+def hit_geometry(v, f, uv, rays, h):
+    """p = o + t d, gn = normalize(cross(p2-p1, p3-p1)), uv interpolated with
+    b0 on the 2nd and b1 on the 3rd vertex (intersectionutils.h:59-135)."""
+    ids = orc.hit_ids(h)
+    t, b0, b1 = h[:, 0:1], h[:, 1:2], h[:, 2:3]
+    p = rays[:, 0:3] + rays[:, 4:7] * t
+    tri = f[ids]
+    p1, p2, p3 = v[tri[:, 0]], v[tri[:, 1]], v[tri[:, 2]]
+    gn = np.cross(p2 - p1, p3 - p1)
+    gn /= np.linalg.norm(gn, axis=1, keepdims=True)
+    tuv = uv[tri[:, 0]] * (1 - b0 - b1) + uv[tri[:, 1]] * b0 + uv[tri[:, 2]] * b1
+    return p, gn, tuv
+
+
+# the oracle's QBVH and BVH restate the reference and use its TriAccel test
+@pytest.mark.parametrize("accel,test", [("naive", orc.TEST_TRIACCEL), ("naive", orc.TEST_WATERTIGHT),
+                                        ("qbvh", orc.TEST_TRIACCEL), ("bvh", orc.TEST_TRIACCEL)])
+@pytest.mark.parametrize("simple2", [False, True])
+def test_accel3_analytic_quads(test, simple2, accel):
+    v, f, uv = (SIMPLE2_V, SIMPLE2_F, SIMPLE2_UV) if simple2 else (SIMPLE_V, SIMPLE_F, SIMPLE_UV)
+    rays, expect = quad_rays(simple2)
+    if accel == "naive":
+        h = orc.closest_hit_naive(v, f, rays, test=test)
+    elif accel == "qbvh":
+        h = orc.QBVH(v, f).intersect(rays)
+    else:
+        h = orc.BVH(v, f).intersect(rays)
+    assert (orc.hit_ids(h) != orc.MISS).all()        # ASSERT_TRUE(accel->Intersect(...))
+    p, gn, tuv = hit_geometry(v, f, uv, rays, h)
+    eps = 1e-3                                        # Math::EpsLarge(), math.h:1665
+    np.testing.assert_allclose(p, expect, atol=eps)
+    n_exp = np.array([0, 0, 1], np.float32) if not simple2 else np.array([1, 0, 1], np.float32) / np.sqrt(2)
+    np.testing.assert_allclose(gn, np.broadcast_to(n_exp, gn.shape), atol=eps)
+    np.testing.assert_allclose(tuv, expect[:, 0:2], atol=eps)
+
+
+def displaced_sphere(sub, seed):
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "lightmetrica-v2_amd"))
+    from lmx import scenes
+    return scenes, scenes.displaced_icosphere(sub, seed=seed)
+
+
+def test_qbvh_and_bvh_equal_brute_force():
+    scenes, (v, f) = displaced_sphere(5, 7)            # 20,480 triangles
+    coh, inc = scenes.traversal_rays(v, res=48, n_incoherent=8000)
+    rays = np.concatenate([coh, inc])
+    hn = orc.closest_hit_naive(v, f, rays, test=orc.TEST_TRIACCEL, tie=orc.TIE_LAST_WINS)
+    for acc in (orc.QBVH(v, f), orc.BVH(v, f)):
+        h = acc.intersect(rays)
+        ids_n, ids = orc.hit_ids(hn), orc.hit_ids(h)
+        diff = ids != ids_n
+        # only exact ties in t may pick a different triangle (traversal order)
+        assert np.array_equal(h[diff, 0], hn[diff, 0])
+        assert diff.sum() <= 2
+        np.testing.assert_array_equal(h[~diff].view(np.uint32), hn[~diff].view(np.uint32))
+
+
+def _fp64_solve(v, f, rays, ids):
+    """Exact-ish hit data in FP64: t and barycentrics of each ray against the
+    given triangles (Cramer's rule)."""
+    tri = f[ids].astype(np.int64)
+    p0 = v[tri[:, 0]].astype(np.float64); p1 = v[tri[:, 1]].astype(np.float64); p2 = v[tri[:, 2]].astype(np.float64)
+    o = rays[:, 0:3].astype(np.float64); d = rays[:, 4:7].astype(np.float64)
+    e1, e2 = p1 - p0, p2 - p0
+    pv = np.cross(d, e2)
+    det = np.einsum("ij,ij->i", e1, pv)
+    tv = o - p0
+    u = np.einsum("ij,ij->i", tv, pv) / det
+    qv = np.cross(tv, e1)
+    w = np.einsum("ij,ij->i", d, qv) / det
+    t = np.einsum("ij,ij->i", e2, qv) / det
+    return t, u, w
+
+
+def test_watertight_vs_triaccel_differ_only_on_edges_or_ties():
+    scenes, (v, f) = displaced_sphere(6, 3)            # 81,920 triangles
+    coh, inc = scenes.traversal_rays(v, res=64, n_incoherent=12000)
+    rays = np.concatenate([coh, inc])
+    ha = orc.closest_hit_naive(v, f, rays, test=orc.TEST_TRIACCEL, tie=orc.TIE_MIN_ID)
+    hw = orc.closest_hit_naive(v, f, rays, test=orc.TEST_WATERTIGHT, tie=orc.TIE_MIN_ID)
+    ia, iw = orc.hit_ids(ha), orc.hit_ids(hw)
+    same = ia == iw
+    # agreeing rays: t and barycentrics within 1e-5 relative, measured against
+    # max(|t|, 1) (the scene scale): the TriAccel t is computed from absolute
+    # plane offsets, so rays that start close to the surface carry an absolute
+    # error of a few ulps of the scene scale.  The FP64 solve shows the
+    # watertight t is the more accurate of the two.
+    both = same & (ia != orc.MISS)
+    t64, b064, b164 = _fp64_solve(v, f, rays[both], ia[both])
+    scale = np.maximum(np.abs(t64), 1.0)
+    # the watertight hit is accurate: t within 1e-5 relative of the FP64
+    # solve, and the barycentrics locate the hit point on the triangle within
+    # 1e-5 of the scene scale (barycentric error x edge length: FP32 cannot
+    # resolve barycentrics of small, distant triangles better than that)
+    assert (np.abs(hw[both, 0] - t64) <= 1e-5 * np.abs(t64)).all()
+    tri = f[ia[both]]
+    e1 = np.linalg.norm(v[tri[:, 1]] - v[tri[:, 0]], axis=1)
+    e2 = np.linalg.norm(v[tri[:, 2]] - v[tri[:, 0]], axis=1)
+    assert (np.abs(hw[both, 1] - b064) * e1 <= 1e-5 * scale).all()
+    assert (np.abs(hw[both, 2] - b164) * e2 <= 1e-5 * scale).all()
+    # the two FP32 tests agree within 1e-5 of the scene scale, except where
+    # the TriAccel t itself is further than that from the FP64 solve (grazing
+    # rays: its plane offsets lose absolute precision)
+    ref_off = np.abs(ha[both, 0] - t64) > 1e-5 * scale
+    close = np.abs(hw[both, 0] - ha[both, 0]) <= 1e-5 * scale
+    assert (close | ref_off).all()
+    assert ref_off.sum() <= 1e-3 * both.sum()
+    assert (np.abs(hw[both, 1] - ha[both, 1]) * e1 <= 2e-5 * scale).all()
+    assert (np.abs(hw[both, 2] - ha[both, 2]) * e2 <= 2e-5 * scale).all()
+    # disagreeing rays: classified by the FP64 arbiter as grazing an edge of
+    # the triangle either test picked, or as a t tie
+    bad = np.nonzero(~same)[0]
+    assert len(bad) < 1e-3 * rays.shape[0]
+    for k in bad:
+        cands = [i for i in (ia[k], iw[k]) if i != orc.MISS]
+        edge = False
+        for c in cands:
+            t, b0, b1 = _fp64_solve(v, f, rays[k:k + 1], np.array([c]))
+            bmin = min(b0[0], b1[0], 1 - b0[0] - b1[0])
+            if abs(bmin) < 1e-5:
+                edge = True
+        tie = len(cands) == 2 and abs(ha[k, 0] - hw[k, 0]) <= 1e-6 * abs(ha[k, 0])
+        assert edge or tie, (k, ia[k], iw[k])
+
+
+def test_watertight_never_loses_rays_through_vertices_and_edges():
+    """Rays aimed exactly at mesh vertices and edge midpoints of a closed
+    mesh from outside must hit it (the TriAccel test may miss them)."""
+    scenes, (v, f) = displaced_sphere(3, 9)
+    targets = np.concatenate([v, 0.5 * (v[f[:, 0]] + v[f[:, 1]])]).astype(np.float32)
+    o = (targets * 3.0).astype(np.float32)
+    d = (targets - o)
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    rays = scenes.make_rays(o, d, tmin=np.float32(0.0))
+    hw = orc.closest_hit_naive(v, f, rays, test=orc.TEST_WATERTIGHT)
+    assert (orc.hit_ids(hw) != orc.MISS).all()
+
+
+def test_closed_form_plane_hit():
+    """Axis-aligned plane z = c hit from (x, y, 0) along +z: t = c exactly."""
+    v = np.array([[-10, -10, 2.5], [10, -10, 2.5], [0, 10, 2.5]], np.float32)
+    f = np.array([[0, 1, 2]], np.uint32)
+    o = np.array([[0.25, 0.5, 0.0], [1.0, -2.0, 0.0]], np.float32)
+    d = np.array([[0, 0, 1], [0, 0, 1]], np.float32)
+    from_ray = np.concatenate([o, np.zeros((2, 1), np.float32), d, np.full((2, 1), FMAX, np.float32)], axis=1)
+    for test in (orc.TEST_TRIACCEL, orc.TEST_WATERTIGHT):
+        h = orc.closest_hit_naive(v, f, from_ray, test=test)
+        np.testing.assert_array_equal(h[:, 0], [2.5, 2.5])
+        # barycentrics reproduce the hit point
+        p = v[0] * (1 - h[:, 1:2] - h[:, 2:3]) + v[1] * h[:, 1:2] + v[2] * h[:, 2:3]
+        np.testing.assert_allclose(p[:, :2], o[:, :2], atol=1e-5)
