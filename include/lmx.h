@@ -100,6 +100,86 @@ int lmx_accel_intersect_host(const lmx_accel *accel, const lmx_ray *rays, uint32
 int lmx_accel_count(const lmx_accel *accel, const lmx_ray *rays, uint32_t n,
                     uint64_t *node_visits, uint64_t *tri_tests, int32_t nthreads);
 
+/* ------------------------------------------------------------------------ */
+/* Path samplers: scene and renderers (Renderer::Render, renderer.h:81)       */
+
+#define LMX_MAT_NULL 0       /* bsdf::null: no scattering (light-only surfaces) */
+#define LMX_MAT_DIFFUSE 1    /* bsdf::diffuse (bsdf_diffuse.cpp) */
+#define LMX_MAT_PHONG 2      /* normalized Phong (no counterpart in the reference) */
+
+#define LMX_RASTER_RANDOM 0      /* raster position from the sample's uniforms, as the reference */
+#define LMX_RASTER_STRATIFIED 1  /* sample i -> pixel i mod (W H), jittered inside it */
+
+typedef struct lmx_material {
+    int32_t type;        /* LMX_MAT_* */
+    float R[3];          /* reflectance */
+    float exponent;      /* Phong exponent */
+    float Le[3];         /* emitted radiance; > 0 makes every triangle of the material an area light */
+} lmx_material;
+
+/* Pinhole sensor (sensor_pinhole.cpp:44-202): position, camera basis (the
+ * sensor looks along -vz), vertical field of view in degrees, film size. */
+typedef struct lmx_camera {
+    float pos[3], vx[3], vy[3], vz[3];
+    float fov_deg;
+    int32_t width, height;
+} lmx_camera;
+
+/* World-space scene (host arrays, copied).  Triangles in enumeration order;
+ * light_prims lists each light primitive as (first triangle, count). */
+typedef struct lmx_scene_desc {
+    const float *verts; uint32_t nvert;
+    const uint32_t *tris; uint32_t ntri;
+    const float *normals;            /* per vertex, or NULL (geometric normals) */
+    const uint32_t *tri_mat;         /* material index per triangle */
+    const lmx_material *mats; uint32_t nmat;
+    const uint32_t *light_prims; uint32_t nlight;
+    lmx_camera camera;
+} lmx_scene_desc;
+
+typedef struct lmx_scene lmx_scene;
+
+/* Per-sample uniform slots (shared by the generator and uniform tables):
+ * slot 0 selects the sensor, 1-2 place the raster position, 3-7 unused;
+ * path segment j (j = 0 is the camera ray) owns slots 8+4j .. 8+4j+3:
+ * direction 2-D sample, component sample, Russian roulette after the
+ * segment's hit.  Table width >= 8 + 4 (max_num_vertices - 1). */
+#define LMX_PT_SLOT_RASTER 1
+#define LMX_PT_SLOT_SEG0 8
+
+typedef struct lmx_render_params {
+    int32_t max_num_vertices;    /* -1: unbounded (renderer_pt.cpp:59) */
+    int32_t min_num_vertices;    /* (renderer_pt.cpp:60) */
+    int32_t rr_depth;            /* roulette after the hit of segment j when j + 1 >= rr_depth (1 = reference) */
+    float rr_prob;               /* continuation probability (0.5 in the reference) */
+    uint64_t seed;
+    uint64_t num_samples;        /* total camera samples = spp x W x H */
+    int32_t raster_mode;         /* LMX_RASTER_* */
+    uint32_t queue_capacity;     /* paths in flight; 0 = autotuned */
+    const float *uniform_table;  /* device, num_samples x table_width, or NULL (Philox) */
+    uint32_t table_width;
+    float *sample_out;           /* device, num_samples x 8 floats (raster x, y, C rgb, #splats, 0, 0), or NULL */
+} lmx_render_params;
+
+typedef struct lmx_render_stats {
+    uint64_t rays;               /* extension rays traced */
+    uint32_t iterations;         /* wavefront iterations */
+    uint32_t queue_capacity;     /* capacity used */
+    double ms;                   /* wall time of the render call */
+} lmx_render_stats;
+
+/* Builds the accel (watertight, 8-wide) and uploads shading data to the
+ * current device. */
+int lmx_scene_create(const lmx_scene_desc *desc, lmx_scene **out);
+void lmx_scene_free(lmx_scene *scene);
+
+/* renderer::pt (renderer_pt.cpp:64-242) as a wavefront pipeline: film_rgb is
+ * a device array of W x H x 3 floats (row 0 = bottom), overwritten with the
+ * image (sum of splats x W H / num_samples, Film::Rescale,
+ * scheduler.cpp:281-288).  Synchronous on `stream`; stats may be NULL. */
+int lmx_render_pt(lmx_scene *scene, const lmx_render_params *params, float *film_rgb, void *stream,
+                  lmx_render_stats *stats);
+
 /* Copies the host-side BVH out for inspection: num_nodes x 128-byte nodes and
  * num_tris x 48-byte triangle records (layouts in
  * lightmetrica-v2_amd/csrc/wbvh.h).  Either pointer may be NULL. */

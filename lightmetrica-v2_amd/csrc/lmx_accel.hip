@@ -63,7 +63,7 @@ struct lmx_accel {
     int device = -1;
     WNode* d_nodes = nullptr;
     WTri* d_tris = nullptr;
-    uint32_t* d_counters = nullptr;   // 8 region cursors
+    uint32_t* d_counters = nullptr;   // 8 region cursors + the ray count
     int grid = 0;
     int stack_depth = 0;              // LDS stack entries per lane
     size_t lds_bytes = 0;
@@ -159,10 +159,15 @@ void wbvh_walk(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* nv, uint
 #endif
 #define REFILL_MIN LMX_REFILL_MIN
 
-__global__ void k_init_cursors(uint32_t* cursors, uint32_t n)
+// Resets the 8 region cursors.  The ray count comes from n_dev when given
+// (a queue length written by an earlier kernel), else from n; the count the
+// trace kernel uses is stored in cursors[8].
+__global__ void k_init_cursors(uint32_t* cursors, uint32_t n, const uint32_t* n_dev)
 {
     const uint32_t j = threadIdx.x;
-    if (j < 8) cursors[j] = (uint32_t)(((uint64_t)n * j) / 8u);
+    const uint32_t nn = n_dev ? *n_dev : n;
+    if (j < 8) cursors[j] = (uint32_t)(((uint64_t)nn * j) / 8u);
+    if (j == 8) cursors[8] = nn;
 }
 
 __device__ __forceinline__ uint32_t region_end(uint32_t j, uint32_t n)
@@ -207,12 +212,13 @@ __global__ __launch_bounds__(TRACE_WAVE) __attribute__((amdgpu_waves_per_eu(LMX_
 void k_trace(const WNode* __restrict__ nodes,
                                                       const WTri* __restrict__ tris,
                                                       const WRay* __restrict__ rays,
-                                                      WHit* __restrict__ hits, uint32_t nray,
+                                                      WHit* __restrict__ hits,
                                                       uint32_t* __restrict__ cursors,
                                                       int stack_depth, uint32_t refill_min)
 {
     extern __shared__ uint2 stack[];     // [stack_depth][TRACE_WAVE]
     const uint32_t lane = threadIdx.x;
+    const uint32_t nray = cursors[8];
 
     uint32_t region = blockIdx.x & 7u;
     uint32_t regions_left = 8;          // regions not yet found exhausted
@@ -427,7 +433,7 @@ int lmx_accel_upload(lmx_accel* a)
         (void)hipFree(a->d_nodes); a->d_nodes = nullptr;
         return hip_error(e, "lmx_accel_upload: hipMalloc triangles");
     }
-    if ((e = hipMalloc(&a->d_counters, 8 * sizeof(uint32_t))) != hipSuccess) {
+    if ((e = hipMalloc(&a->d_counters, 16 * sizeof(uint32_t))) != hipSuccess) {
         (void)hipFree(a->d_nodes); (void)hipFree(a->d_tris); a->d_nodes = nullptr; a->d_tris = nullptr;
         return hip_error(e, "lmx_accel_upload: hipMalloc counters");
     }
@@ -455,28 +461,40 @@ int lmx_accel_upload(lmx_accel* a)
     return LMX_OK;
 }
 
+}  // extern "C"
+
+namespace lmx {
+
+int launch_trace(lmx_accel* a, const WRay* rays, uint32_t n, const uint32_t* n_dev, uint32_t n_max,
+                 WHit* hits, hipStream_t s)
+{
+    int rc = lmx_accel_upload(a);
+    if (rc != LMX_OK) return rc;
+    const int grid = std::max(1, std::min(a->grid, (int)((n_max + TRACE_WAVE - 1) / TRACE_WAVE)));
+    k_init_cursors<<<1, 64, 0, s>>>(a->d_counters, n, n_dev);
+    if (a->host.isect == WBVH_ISECT_WATERTIGHT)
+        k_trace<WBVH_ISECT_WATERTIGHT><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
+            a->d_nodes, a->d_tris, rays, hits, a->d_counters, a->stack_depth, REFILL_MIN);
+    else
+        k_trace<WBVH_ISECT_TRIACCEL><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
+            a->d_nodes, a->d_tris, rays, hits, a->d_counters, a->stack_depth, REFILL_MIN);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "trace launch");
+    return LMX_OK;
+}
+
+}  // namespace lmx
+
+extern "C" {
+
 int lmx_accel_intersect(lmx_accel* a, const lmx_ray* rays, uint32_t n, lmx_hit* hits, void* stream)
 {
     if (!a) return set_error(LMX_EINVAL, "lmx_accel_intersect: NULL accel");
     if (n == 0) return LMX_OK;
     if (!rays || !hits) return set_error(LMX_EINVAL, "lmx_accel_intersect: NULL ray or hit array");
     if (n > 0x7fffffffu) return set_error(LMX_EINVAL, "lmx_accel_intersect: too many rays");
-    int rc = lmx_accel_upload(a);
-    if (rc != LMX_OK) return rc;
-    hipStream_t s = (hipStream_t)stream;
-    const int grid = std::max(1, std::min(a->grid, (int)((n + TRACE_WAVE - 1) / TRACE_WAVE)));
-    k_init_cursors<<<1, 64, 0, s>>>(a->d_counters, n);
-    if (a->host.isect == WBVH_ISECT_WATERTIGHT)
-        k_trace<WBVH_ISECT_WATERTIGHT><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
-            a->d_nodes, a->d_tris, reinterpret_cast<const WRay*>(rays), reinterpret_cast<WHit*>(hits), n,
-            a->d_counters, a->stack_depth, REFILL_MIN);
-    else
-        k_trace<WBVH_ISECT_TRIACCEL><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
-            a->d_nodes, a->d_tris, reinterpret_cast<const WRay*>(rays), reinterpret_cast<WHit*>(hits), n,
-            a->d_counters, a->stack_depth, REFILL_MIN);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_error(e, "lmx_accel_intersect: launch");
-    return LMX_OK;
+    return launch_trace(a, reinterpret_cast<const WRay*>(rays), n, nullptr, n, reinterpret_cast<WHit*>(hits),
+                        (hipStream_t)stream);
 }
 
 int lmx_accel_intersect_host(const lmx_accel* a, const lmx_ray* rays, uint32_t n, lmx_hit* hits,

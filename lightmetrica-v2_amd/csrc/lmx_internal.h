@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <string>
+#include "lmx.h"
+#include "wbvh.h"
 
 namespace lmx {
 
@@ -9,5 +11,11 @@ namespace lmx {
 int set_error(int code, const std::string& msg);
 // Records a HIP runtime failure and returns LMX_EHIP.
 int hip_error(hipError_t e, const char* what);
+
+// Enqueues closest-hit traversal of rays[0, n) on stream s, where n is *n_dev
+// when n_dev is non-null (a count written by an earlier kernel; n_max bounds
+// it and sizes the grid), else n.
+int launch_trace(lmx_accel* a, const WRay* rays, uint32_t n, const uint32_t* n_dev, uint32_t n_max,
+                 WHit* hits, hipStream_t s);
 
 }  // namespace lmx

@@ -5,4 +5,4 @@ include/lmx.h).  This package loads it with ctypes and fails loudly when it
 is missing: there is no CPU fallback on the product path.
 """
 from ._lib import lib, LmxError, check, LIB_PATH  # noqa: F401
-from . import accel, scenes  # noqa: F401
+from . import accel, render, scenes  # noqa: F401

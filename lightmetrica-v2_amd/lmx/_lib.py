@@ -35,6 +35,40 @@ class lmx_accel_info(ctypes.Structure):
                 ("node_bytes", ctypes.c_uint64), ("tri_bytes", ctypes.c_uint64)]
 
 
+class lmx_material(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("R", ctypes.c_float * 3), ("exponent", ctypes.c_float),
+                ("Le", ctypes.c_float * 3)]
+
+
+class lmx_camera(ctypes.Structure):
+    _fields_ = [("pos", ctypes.c_float * 3), ("vx", ctypes.c_float * 3), ("vy", ctypes.c_float * 3),
+                ("vz", ctypes.c_float * 3), ("fov_deg", ctypes.c_float), ("width", ctypes.c_int32),
+                ("height", ctypes.c_int32)]
+
+
+class lmx_scene_desc(ctypes.Structure):
+    _fields_ = [("verts", ctypes.c_void_p), ("nvert", ctypes.c_uint32),
+                ("tris", ctypes.c_void_p), ("ntri", ctypes.c_uint32),
+                ("normals", ctypes.c_void_p), ("tri_mat", ctypes.c_void_p),
+                ("mats", ctypes.c_void_p), ("nmat", ctypes.c_uint32),
+                ("light_prims", ctypes.c_void_p), ("nlight", ctypes.c_uint32),
+                ("camera", lmx_camera)]
+
+
+class lmx_render_params(ctypes.Structure):
+    _fields_ = [("max_num_vertices", ctypes.c_int32), ("min_num_vertices", ctypes.c_int32),
+                ("rr_depth", ctypes.c_int32), ("rr_prob", ctypes.c_float),
+                ("seed", ctypes.c_uint64), ("num_samples", ctypes.c_uint64),
+                ("raster_mode", ctypes.c_int32), ("queue_capacity", ctypes.c_uint32),
+                ("uniform_table", ctypes.c_void_p), ("table_width", ctypes.c_uint32),
+                ("sample_out", ctypes.c_void_p)]
+
+
+class lmx_render_stats(ctypes.Structure):
+    _fields_ = [("rays", ctypes.c_uint64), ("iterations", ctypes.c_uint32),
+                ("queue_capacity", ctypes.c_uint32), ("ms", ctypes.c_double)]
+
+
 # Every symbol include/lmx.h declares, with its ctypes signature.
 SIGNATURES = {
     "lmx_last_error": (ctypes.c_char_p, []),
@@ -47,6 +81,10 @@ SIGNATURES = {
                                            ctypes.c_void_p, ctypes.c_void_p]),
     "lmx_accel_intersect_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                                 ctypes.c_void_p, ctypes.c_int32]),
+    "lmx_scene_create": (ctypes.c_int, [ctypes.POINTER(lmx_scene_desc), ctypes.POINTER(ctypes.c_void_p)]),
+    "lmx_scene_free": (None, [ctypes.c_void_p]),
+    "lmx_render_pt": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(lmx_render_params), ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.POINTER(lmx_render_stats)]),
     "lmx_accel_export": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "lmx_accel_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                        ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),

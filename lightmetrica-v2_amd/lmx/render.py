@@ -1,0 +1,115 @@
+"""Renderers over liblmx's wavefront pipeline.
+
+`Scene` uploads a renderer scene (the dicts of lmx.scenes: world-space mesh,
+materials, light primitives, pinhole camera); `Scene.render_pt` is the
+reference's renderer::pt (src/liblightmetrica/renderer/renderer_pt.cpp:64-242)
+with its parameters: max_num_vertices / min_num_vertices as in the reference
+(vertices, camera included), Russian roulette with continuation probability
+rr_prob after the hit of segment j once j + 1 >= rr_depth (rr_depth = 1,
+rr_prob = 0.5 is the reference's fixed rule, :207-215).
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import (check, lib, lmx_camera, lmx_material, lmx_render_params, lmx_render_stats,
+                   lmx_scene_desc)
+from .scenes import camera_frame
+
+RASTER_RANDOM = 0
+RASTER_STRATIFIED = 1
+SLOT_SEG0 = 8
+
+
+def table_width(max_num_vertices):
+    """Uniform-table width of renderer::pt for this path-length bound."""
+    return SLOT_SEG0 + 4 * (max_num_vertices - 1)
+
+
+class Scene:
+    def __init__(self, sc):
+        self._keep = []
+
+        def keep(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            self._keep.append(a)
+            return ctypes.c_void_p(a.ctypes.data)
+        mats = (lmx_material * len(sc["materials"]))()
+        for i, m in enumerate(sc["materials"]):
+            mats[i].type = m["type"]
+            mats[i].R[:] = list(m["R"])
+            mats[i].exponent = m["exponent"]
+            mats[i].Le[:] = list(m["Le"])
+        self._keep.append(mats)
+        d = lmx_scene_desc()
+        d.verts = keep(sc["verts"], np.float32)
+        d.nvert = sc["verts"].shape[0]
+        d.tris = keep(sc["tris"], np.uint32)
+        d.ntri = sc["tris"].shape[0]
+        d.normals = keep(sc["normals"], np.float32) if sc.get("normals") is not None else None
+        d.tri_mat = keep(sc["tri_mat"], np.uint32)
+        d.mats = ctypes.cast(mats, ctypes.c_void_p)
+        d.nmat = len(sc["materials"])
+        lp = np.array(sc["lights"], np.uint32).reshape(-1, 2)
+        d.light_prims = keep(lp, np.uint32)
+        d.nlight = lp.shape[0]
+        cam = sc["camera"]
+        pos, vx, vy, vz = camera_frame(cam)
+        c = lmx_camera()
+        c.pos[:] = list(pos); c.vx[:] = list(vx); c.vy[:] = list(vy); c.vz[:] = list(vz)
+        c.fov_deg = cam["fov_deg"]
+        c.width = cam["width"]
+        c.height = cam["height"]
+        d.camera = c
+        self.width, self.height = cam["width"], cam["height"]
+        h = ctypes.c_void_p()
+        check(lib.lmx_scene_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+        self._free = lib.lmx_scene_free
+        self._keep = []           # the library copied everything
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def render_pt(self, num_samples, max_num_vertices=9, min_num_vertices=0, rr_depth=1, rr_prob=0.5,
+                  seed=42, raster_mode=RASTER_STRATIFIED, queue_capacity=0, table=None, sample_out=False,
+                  stream=None):
+        """Returns (film as a torch tensor (H, W, 3) on the GPU, stats dict,
+        per-sample records (n, 8) or None).  `table`: torch float32 tensor
+        (num_samples, width) on the GPU, or None for Philox."""
+        import torch
+        dev = torch.device("cuda")
+        film = torch.empty((self.height, self.width, 3), dtype=torch.float32, device=dev)
+        so = None
+        p = lmx_render_params()
+        p.max_num_vertices = max_num_vertices
+        p.min_num_vertices = min_num_vertices
+        p.rr_depth = rr_depth
+        p.rr_prob = rr_prob
+        p.seed = seed
+        p.num_samples = num_samples
+        p.raster_mode = raster_mode
+        p.queue_capacity = queue_capacity
+        if table is not None:
+            assert table.is_cuda and table.dtype == torch.float32 and table.is_contiguous()
+            assert table.shape[0] == num_samples
+            p.uniform_table = ctypes.c_void_p(table.data_ptr())
+            p.table_width = table.shape[1]
+        if sample_out:
+            so = torch.zeros((num_samples, 8), dtype=torch.float32, device=dev)
+            p.sample_out = ctypes.c_void_p(so.data_ptr())
+        st = lmx_render_stats()
+        if stream is None:
+            # the renderer captures a hipGraph, which needs a non-legacy
+            # stream; order it after the work already queued on torch's
+            stream = torch.cuda.Stream(dev)
+            stream.wait_stream(torch.cuda.current_stream(dev))
+        check(lib.lmx_render_pt(self._h, ctypes.byref(p), ctypes.c_void_p(film.data_ptr()),
+                                ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st)))
+        stats = {"rays": st.rays, "iterations": st.iterations, "queue_capacity": st.queue_capacity, "ms": st.ms}
+        return film, stats, so

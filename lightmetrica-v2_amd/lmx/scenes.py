@@ -170,3 +170,104 @@ def traversal_rays(verts, res=2048, n_incoherent=None, seed=TRAVERSAL_SEED + 1):
     hi = verts.max(axis=0)
     inc = incoherent_rays(n_incoherent, lo, hi, seed)
     return coh, inc
+
+
+# ---------------------------------------------------------------------------
+# Renderer scenes.  A scene is a dict:
+#   verts (V,3) f32, tris (F,3) u32, normals (V,3) f32 or None,
+#   tri_mat (F,) u32, materials: list of dicts {type, R, exponent, Le},
+#   lights: list of (first triangle, count) -- one entry per light primitive,
+#   camera: dict(eye, center, up, fov_deg, width, height).
+MAT_NULL = 0
+MAT_DIFFUSE = 1
+MAT_PHONG = 2
+
+
+def _quad(p0, p1, p2, p3):
+    """Two triangles (p0,p1,p2), (p0,p2,p3); the face normal is
+    cross(p1-p0, p2-p0), the winding the reference's gn uses."""
+    return [p0, p1, p2, p3], [[0, 1, 2], [0, 2, 3]]
+
+
+def _box(center, size, angle_deg):
+    """Axis-aligned box of `size` rotated about +y, faces outward."""
+    cx, cy, cz = center
+    sx, sy, sz = (0.5 * s for s in size)
+    a = np.radians(angle_deg)
+    ca, sa = np.cos(a), np.sin(a)
+
+    def P(x, y, z):
+        return [cx + ca * x + sa * z, cy + y, cz - sa * x + ca * z]
+    c = [P(-sx, -sy, -sz), P(sx, -sy, -sz), P(sx, sy, -sz), P(-sx, sy, -sz),
+         P(-sx, -sy, sz), P(sx, -sy, sz), P(sx, sy, sz), P(-sx, sy, sz)]
+    faces = [(4, 5, 6, 7), (1, 0, 3, 2), (0, 4, 7, 3), (5, 1, 2, 6), (7, 6, 2, 3), (0, 1, 5, 4)]
+    v, t = [], []
+    for f in faces:
+        base = len(v)
+        v += [c[i] for i in f]
+        t += [[base, base + 1, base + 2], [base, base + 2, base + 3]]
+    return v, t
+
+
+def cornell_box(width=256, height=256):
+    """Procedural Cornell box: 5 wall quads (inward) + 2 boxes rotated
+    +-17 degrees (34 triangles) and a 0.25 x 0.25 emissive ceiling quad facing
+    down with Le = (17, 12, 4); Lambertian 0.73 grey / (0.65, 0.05, 0.05) red
+    / (0.12, 0.45, 0.15) green; pinhole camera with a 40 degree vertical fov.
+    The room is [-1, 1]^3, open towards +z."""
+    mats = [dict(type=MAT_DIFFUSE, R=(0.73, 0.73, 0.73), exponent=0.0, Le=(0, 0, 0)),
+            dict(type=MAT_DIFFUSE, R=(0.65, 0.05, 0.05), exponent=0.0, Le=(0, 0, 0)),
+            dict(type=MAT_DIFFUSE, R=(0.12, 0.45, 0.15), exponent=0.0, Le=(0, 0, 0)),
+            dict(type=MAT_NULL, R=(0, 0, 0), exponent=0.0, Le=(17.0, 12.0, 4.0))]
+    V, T, M = [], [], []
+
+    def add(vs, ts, m):
+        base = len(V)
+        V.extend(vs)
+        T.extend([[base + i for i in t] for t in ts])
+        M.extend([m] * len(ts))
+    add(*_quad([-1, -1, 1], [1, -1, 1], [1, -1, -1], [-1, -1, -1]), 0)     # floor, normal +y
+    add(*_quad([-1, 1, -1], [1, 1, -1], [1, 1, 1], [-1, 1, 1]), 0)         # ceiling, normal -y
+    add(*_quad([-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1]), 0)     # back, normal +z
+    add(*_quad([-1, -1, 1], [-1, -1, -1], [-1, 1, -1], [-1, 1, 1]), 1)     # left (red), normal +x
+    add(*_quad([1, -1, -1], [1, -1, 1], [1, 1, 1], [1, 1, -1]), 2)         # right (green), normal -x
+    add(*_box((-0.35, -0.4, -0.3), (0.6, 1.2, 0.6), 17.0), 0)              # tall box
+    add(*_box((0.35, -0.7, 0.3), (0.6, 0.6, 0.6), -17.0), 0)               # short box
+    first_light = len(T)
+    L = 0.125
+    add(*_quad([-L, 0.999, -L], [L, 0.999, -L], [L, 0.999, L], [-L, 0.999, L]), 3)   # light, normal -y
+    verts = np.array(V, np.float32)
+    tris = np.array(T, np.uint32)
+    # fix every face's winding so its normal points the intended way
+    return dict(verts=verts, tris=tris, normals=None, tri_mat=np.array(M, np.uint32), materials=mats,
+                lights=[(first_light, 2)],
+                camera=dict(eye=(0.0, 0.0, 3.8), center=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov_deg=40.0,
+                            width=width, height=height))
+
+
+def camera_frame(cam):
+    """Pinhole camera basis as the reference builds it from a lookat
+    transform (scene3.cpp:191-224): vz = normalize(eye - center),
+    vx = normalize(cross(up, vz)), vy = cross(vz, vx); the camera looks along
+    -vz."""
+    eye = np.asarray(cam["eye"], np.float64)
+    vz = eye - np.asarray(cam["center"], np.float64)
+    vz /= np.linalg.norm(vz)
+    vx = np.cross(np.asarray(cam["up"], np.float64), vz)
+    vx /= np.linalg.norm(vx)
+    vy = np.cross(vz, vx)
+    return eye.astype(np.float32), vx.astype(np.float32), vy.astype(np.float32), vz.astype(np.float32)
+
+
+def furnace_sphere(albedo=0.5, Le=1.0, subdiv=3, width=32, height=32):
+    """Closed sphere seen from inside, every triangle Lambertian with albedo
+    rho and emission Le (inward winding).  Every camera path sees radiance
+    Le * sum_{j < m-1} rho^j with m = max_num_vertices (Le / (1 - rho) when
+    unbounded): a closed form for the whole sampler."""
+    v, f = icosphere(subdiv)
+    f = f[:, [0, 2, 1]]                       # inward normals
+    mats = [dict(type=MAT_DIFFUSE, R=(albedo,) * 3, exponent=0.0, Le=(Le,) * 3)]
+    return dict(verts=v.astype(np.float32) * 2.0, tris=f.astype(np.uint32), normals=None,
+                tri_mat=np.zeros(f.shape[0], np.uint32), materials=mats, lights=[(0, f.shape[0])],
+                camera=dict(eye=(0.0, 0.0, 0.3), center=(0.0, 0.0, -1.0), up=(0.0, 1.0, 0.0), fov_deg=60.0,
+                            width=width, height=height))

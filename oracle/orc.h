@@ -93,6 +93,10 @@ void orc_qbvh_free(orc_qbvh *q);
 uint32_t orc_qbvh_num_nodes(const orc_qbvh *q);
 void orc_qbvh_intersect(const orc_qbvh *q, const orc_ray *rays, uint32_t nray,
                         orc_hit *hits, int nthreads);
+/* Same traversal with a chosen triangle test and tie rule; with
+ * ORC_TIE_MIN_ID the result equals orc_closest_hit_naive's. */
+void orc_qbvh_intersect_ex(const orc_qbvh *q, const orc_ray *rays, uint32_t nray,
+                           orc_hit *hits, int test, int tie, int nthreads);
 
 /* Binary binned-SAH BVH (accel_bvh_sahbin.cpp) */
 typedef struct orc_bvh orc_bvh;
@@ -107,6 +111,66 @@ void orc_sample_concentric_disk(const float u[2], float out[2]);
 void orc_sample_cosine_hemisphere(const float u[2], float out[3]);
 void orc_sample_uniform_sphere(const float u[2], float out[3]);
 void orc_sample_uniform_triangle(const float u[2], float out[2]);
+
+/* ---------------------------------------------------------------------- */
+/* Path samplers (orc_render.c) */
+
+/* Philox4x32-10 (Salmon et al., SC 2011) and the uniform addressed by
+ * (seed, sample, slot): block slot/4 of counter (sample lo, sample hi,
+ * slot/4, ORC_PHILOX_TAG) under key (seed lo, seed hi), top 24 bits. */
+#define ORC_PHILOX_TAG 0x6c6d7870u
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+float orc_uniform(uint64_t seed, uint64_t sample, uint32_t slot);
+
+/* Per-sample uniform slots of renderer::pt: slot 0 selects the sensor,
+ * 1-2 place the raster position, 3-4 are the sensor's second 2-D sample
+ * (unused by a pinhole), 5-7 padding; segment j (j = 0 is the camera ray)
+ * owns 8+4j .. 8+4j+3: direction 2-D sample, component sample, Russian
+ * roulette after the segment's hit.  Width = 8 + 4 (max_num_vertices - 1). */
+#define ORC_PT_RASTER 1u
+#define ORC_PT_SEG0 8u
+
+enum { ORC_MAT_NULL = 0, ORC_MAT_DIFFUSE = 1, ORC_MAT_PHONG = 2 };
+enum { ORC_RASTER_RANDOM = 0, ORC_RASTER_STRATIFIED = 1 };
+
+typedef struct orc_material {
+    int32_t type;        /* ORC_MAT_* */
+    float R[3];          /* reflectance */
+    float exponent;      /* Phong exponent */
+    float Le[3];         /* emitted radiance (area light when > 0) */
+} orc_material;
+
+typedef struct orc_scene_desc {
+    const float *verts; uint32_t nvert;
+    const uint32_t *tris; uint32_t ntri;
+    const float *normals;            /* per vertex, or NULL */
+    const uint32_t *tri_mat;         /* material index per triangle */
+    const orc_material *mats; uint32_t nmat;
+    const uint32_t *light_prims;     /* nlight x (first triangle, count) */
+    uint32_t nlight;
+    float cam_pos[3], cam_vx[3], cam_vy[3], cam_vz[3];
+    float fov_deg;                   /* vertical, degrees */
+    int32_t width, height;
+} orc_scene_desc;
+
+typedef struct orc_pt_params {
+    int32_t max_num_vertices;        /* -1: unbounded (renderer_pt.cpp:59) */
+    int32_t min_num_vertices;
+    int32_t rr_depth;                /* Russian roulette after hits of segments >= rr_depth-1 */
+    float rr_prob;                   /* continuation probability (0.5 in the reference) */
+    uint32_t seed_lo, seed_hi;
+    int32_t raster_mode;             /* ORC_RASTER_* */
+} orc_pt_params;
+
+typedef struct orc_scene orc_scene;
+orc_scene *orc_scene_new(const orc_scene_desc *d);
+void orc_scene_free(orc_scene *s);
+/* Samples [first, first+n): out[6 i ..] = raster x, y, C rgb, #splats.
+ * table: n rows of table_width uniforms (row 0 = sample `first`), or NULL
+ * for Philox. */
+void orc_pt_samples(const orc_scene *s, const orc_pt_params *p, const float *table, uint32_t table_width,
+                    uint64_t first_sample, uint64_t n, float *out, int nthreads);
+void orc_film_splat(const float *samples, uint64_t n, int32_t width, int32_t height, double *film);
 
 #ifdef __cplusplus
 }

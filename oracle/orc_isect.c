@@ -352,6 +352,7 @@ typedef struct {
     orc_triaccel *ta;     /* per triangle, in enumeration order */
     bnd *tb;              /* padded per-triangle bounds */
     int *indices;
+    float *tv;            /* per triangle: 9 floats, the three vertices */
 } sah_input;
 
 static void sah_input_init(sah_input *in, const float *verts, const uint32_t *tris, uint32_t ntri)
@@ -360,9 +361,13 @@ static void sah_input_init(sah_input *in, const float *verts, const uint32_t *tr
     in->ta = (orc_triaccel *)calloc(ntri ? ntri : 1, sizeof(orc_triaccel));
     in->tb = (bnd *)calloc(ntri ? ntri : 1, sizeof(bnd));
     in->indices = (int *)calloc(ntri ? ntri : 1, sizeof(int));
+    in->tv = (float *)calloc(ntri ? 9 * (size_t)ntri : 1, sizeof(float));
     for (uint32_t i = 0; i < ntri; i++) {
         const uint32_t *f = &tris[3 * i];
         const float *p1 = &verts[3 * f[0]], *p2 = &verts[3 * f[1]], *p3 = &verts[3 * f[2]];
+        memcpy(&in->tv[9 * (size_t)i], p1, 3 * sizeof(float));
+        memcpy(&in->tv[9 * (size_t)i + 3], p2, 3 * sizeof(float));
+        memcpy(&in->tv[9 * (size_t)i + 6], p3, 3 * sizeof(float));
         orc_triaccel_load(p1, p2, p3, &in->ta[i]);
         in->ta[i].face_index = i;
         in->ta[i].prim_index = 0;
@@ -377,7 +382,7 @@ static void sah_input_init(sah_input *in, const float *verts, const uint32_t *tr
 
 static void sah_input_free(sah_input *in)
 {
-    free(in->ta); free(in->tb); free(in->indices);
+    free(in->ta); free(in->tb); free(in->indices); free(in->tv);
 }
 
 #define NBINS 100
@@ -579,7 +584,7 @@ static inline int qnode_hit(const qnode *n, const __m128 o[3], const __m128 inv_
     return _mm_movemask_ps(_mm_cmpge_ps(maxT, minT));
 }
 
-typedef struct { const orc_qbvh *q; const orc_ray *rays; orc_hit *hits; } qbvh_ctx;
+typedef struct { const orc_qbvh *q; const orc_ray *rays; orc_hit *hits; int test, tie; } qbvh_ctx;
 
 static void qbvh_range(void *p, uint32_t b, uint32_t e)
 {
@@ -598,6 +603,8 @@ static void qbvh_range(void *p, uint32_t b, uint32_t e)
             inv_max[a] = _mm_set1_ps(ray->d[a] == 0.0f ? ORC_INF : 1.0f / ray->d[a]);
             sign[a] = ray->d[a] < 0.0f;
         }
+        wt_ray wr = {0, 0, 0, 0.0f, 0.0f, 0.0f};
+        if (c->test == ORC_TEST_WATERTIGHT) wt_setup(&wr, ray->d);
         int stack[64];                                         /* :429-434 */
         int sp = 0;
         stack[0] = 0;
@@ -610,9 +617,18 @@ static void qbvh_range(void *p, uint32_t b, uint32_t e)
                 for (unsigned i = offset; i < offset + size; i++) {
                     const int ti = q->in.indices[i];
                     float t, u, v;
-                    if (orc_triaccel_intersect(&q->in.ta[ti], ray->o, ray->d, minT, maxT, &u, &v, &t)) {
-                        hit = 1; maxT = t; min_index = ti; mb0 = u; mb1 = v;
+                    int h;
+                    if (c->test == ORC_TEST_TRIACCEL) {
+                        h = orc_triaccel_intersect(&q->in.ta[ti], ray->o, ray->d, minT, maxT, &u, &v, &t);
+                    } else {
+                        const float *tv = &q->in.tv[9 * (size_t)ti];
+                        float V, W, det;
+                        h = wt_test(&wr, ray->o, tv, tv + 3, tv + 6, minT, maxT, &t, &V, &W, &det);
+                        if (h) { u = V / det; v = W / det; }
                     }
+                    if (!h) continue;
+                    if (c->tie == ORC_TIE_MIN_ID && hit && t == maxT && ti > min_index) continue;
+                    hit = 1; maxT = t; min_index = ti; mb0 = u; mb1 = v;
                 }
             } else {
                 const qnode *n = &q->nodes[data];
@@ -632,7 +648,14 @@ static void qbvh_range(void *p, uint32_t b, uint32_t e)
 void orc_qbvh_intersect(const orc_qbvh *q, const orc_ray *rays, uint32_t nray,
                         orc_hit *hits, int nthreads)
 {
-    qbvh_ctx c = {q, rays, hits};
+    qbvh_ctx c = {q, rays, hits, ORC_TEST_TRIACCEL, ORC_TIE_LAST_WINS};
+    parallel_range(qbvh_range, &c, nray, nthreads);
+}
+
+void orc_qbvh_intersect_ex(const orc_qbvh *q, const orc_ray *rays, uint32_t nray,
+                           orc_hit *hits, int test, int tie, int nthreads)
+{
+    qbvh_ctx c = {q, rays, hits, test, tie};
     parallel_range(qbvh_range, &c, nray, nthreads);
 }
 
