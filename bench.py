@@ -62,6 +62,62 @@ def cpu_baseline(verts, tris, rays, threads):
     return rays.shape[0] / dt / 1e6, build_s, dt
 
 
+def pt_section(dev, cpu, threads):
+    """Wavefront renderer::pt on the sphere-grid scene (5,242,886 triangles,
+    1024x1024, 64 spp, max 17 vertices = depth 16, roulette from depth 3):
+    samples/s and Mrays/s (extension rays), beside the oracle's per-sample
+    renderer::pt restatement on host threads over a bounded prefix."""
+    import torch
+    from lmx import render, scenes
+    log("pt: generating the sphere-grid scene")
+    sc = scenes.sphere_grid()
+    rs = render.Scene(sc)
+    n = 1024 * 1024 * 64
+    kw = dict(max_num_vertices=17, rr_depth=3, rr_prob=0.5, seed=42)
+    rs.render_pt(1024 * 1024 * 4, **kw)               # autotune + warm-up
+    torch.cuda.synchronize()
+    times, rays = [], 0
+    for _ in range(3):
+        t0 = time.perf_counter()
+        film, st, _ = rs.render_pt(n, **kw)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        rays = st["rays"]
+    dt = min(times)
+    out = {
+        "workload": "pt: 16x16 displaced icospheres (5,242,886 tris), 1024x1024, 64 spp, depth 16, RR from depth 3",
+        "samples_per_s": round(n / dt, 1),
+        "msamples_per_s": round(n / dt / 1e6, 2),
+        "mrays_per_s": round(rays / dt / 1e6, 1),
+        "rays_per_sample": round(rays / n, 4),
+        "s_per_frame": round(dt, 4),
+        "iterations": st["iterations"],
+        "queue_capacity": st["queue_capacity"],
+        "image_mean": round(float(film.mean()), 5),
+        "cpu_baseline": None,
+    }
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import orc
+        log("pt: cpu baseline (oracle renderer::pt)")
+        t0 = time.perf_counter()
+        osc = orc.Scene(sc)
+        build = time.perf_counter() - t0
+        m = 1 << 19
+        osc.pt_samples(0, 4096, nthreads=threads, **kw)
+        t0 = time.perf_counter()
+        osc.pt_samples(0, m, nthreads=threads, **kw)
+        cdt = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "msamples_per_s": round(m / cdt / 1e6, 4),
+            "mrays_per_s_est": round(m / cdt * rays / n / 1e6, 3),
+            "cores": threads, "kind": "port",
+            "sample": f"first {m} samples of the same frame; oracle QBVH build {build:.1f}s not timed; "
+                      "Mrays/s estimated with the GPU's rays per sample"}
+        out["gpu_over_cpu_samples"] = round((n / dt) / (m / cdt), 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +125,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-pt", action="store_true", help="skip the wavefront path-tracer section")
     ap.add_argument("--no-per-set", action="store_true",
                     help="skip the separate coherent/incoherent timings (for profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -212,6 +269,8 @@ def main():
             "sample": f"every 4th ray of both sets ({sample.shape[0]} rays), same mesh; "
                       f"QBVH build {bsec:.2f}s (not timed), trace {tsec:.2f}s",
         }
+    if not args.no_pt:
+        out["pt"] = pt_section(dev, not args.no_cpu_baseline, args.cpu_threads or min(os.cpu_count() or 1, 16))
     print(json.dumps(out), flush=True)
 
 

@@ -503,6 +503,43 @@ int render_with_capacity(lmx_scene* sc, const RParams& rp, uint32_t cap, hipStre
     return rc;
 }
 
+// Queue-capacity autotuner: renders the same prefix of samples with each
+// candidate capacity (paths in flight) and keeps the fastest in samples per
+// second.  More paths in flight fill the GPU better but cost memory and make
+// the tail (the last iterations with few long paths) longer; the best point
+// depends on the scene's path-length distribution.  The result is kept per
+// scene.
+int autotune_capacity(lmx_scene* sc, const RParams& rp, hipStream_t st)
+{
+    static const uint32_t cands[] = {1u << 20, 1u << 21, 1u << 22, 1u << 23};
+    RParams probe = rp;
+    probe.sample_out = nullptr;
+    probe.total = (uint32_t)std::min<uint64_t>(rp.total, 1ull << 24);
+    uint32_t best = cands[1];
+    double best_rate = -1.0;
+    hipEvent_t e0, e1;
+    hipError_t e;
+    if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess)
+        return hip_error(e, "autotune events");
+    int rc = LMX_OK;
+    for (uint32_t cap : cands) {
+        if (cap > 4 * (uint64_t)probe.total && cap != cands[0]) break;
+        if ((rc = render_with_capacity(sc, probe, cap, st, nullptr, nullptr)) != LMX_OK) break;   // warm
+        (void)hipEventRecord(e0, st);
+        if ((rc = render_with_capacity(sc, probe, cap, st, nullptr, nullptr)) != LMX_OK) break;
+        (void)hipEventRecord(e1, st);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        const double rate = probe.total / (double)std::max(ms, 1e-3f);
+        if (rate > best_rate) { best_rate = rate; best = cap; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == LMX_OK) sc->tuned_cap = best;
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -600,12 +637,16 @@ int lmx_render_pt(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, vo
     rp.sample_out = p->sample_out;
     rp.film = film_rgb;
     const int zgrid = std::max(1, std::min((int)((3 * npix + 255) / 256), sc->cus * 8));
-    k_zero<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix);
-    uint32_t cap = p->queue_capacity ? p->queue_capacity : (sc->tuned_cap ? sc->tuned_cap : (1u << 21));
+    int rc = LMX_OK;
+    if (p->queue_capacity == 0 && sc->tuned_cap == 0) rc = autotune_capacity(sc, rp, st);
+    uint32_t cap = p->queue_capacity ? p->queue_capacity : sc->tuned_cap;
     cap = (uint32_t)std::min<uint64_t>(cap, std::max<uint64_t>(p->num_samples, 64));
     uint64_t rays = 0;
     uint32_t iters = 0;
-    int rc = render_with_capacity(sc, rp, cap, st, &rays, &iters);
+    if (rc == LMX_OK) {
+        k_zero<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix);
+        rc = render_with_capacity(sc, rp, cap, st, &rays, &iters);
+    }
     if (rc == LMX_OK) {
         k_scale<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix, (float)((double)npix / (double)p->num_samples));
         if ((e = hipStreamSynchronize(st)) != hipSuccess) rc = hip_error(e, "lmx_render_pt: sync");

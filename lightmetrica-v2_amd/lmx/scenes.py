@@ -209,7 +209,7 @@ def _box(center, size, angle_deg):
     return v, t
 
 
-def cornell_box(width=256, height=256):
+def cornell_box(width=256, height=256, phong_boxes=None):
     """Procedural Cornell box: 5 wall quads (inward) + 2 boxes rotated
     +-17 degrees (34 triangles) and a 0.25 x 0.25 emissive ceiling quad facing
     down with Le = (17, 12, 4); Lambertian 0.73 grey / (0.65, 0.05, 0.05) red
@@ -231,14 +231,17 @@ def cornell_box(width=256, height=256):
     add(*_quad([-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1]), 0)     # back, normal +z
     add(*_quad([-1, -1, 1], [-1, -1, -1], [-1, 1, -1], [-1, 1, 1]), 1)     # left (red), normal +x
     add(*_quad([1, -1, -1], [1, -1, 1], [1, 1, 1], [1, 1, -1]), 2)         # right (green), normal -x
-    add(*_box((-0.35, -0.4, -0.3), (0.6, 1.2, 0.6), 17.0), 0)              # tall box
-    add(*_box((0.35, -0.7, 0.3), (0.6, 0.6, 0.6), -17.0), 0)               # short box
+    box_mat = 0
+    if phong_boxes is not None:        # variant: glossy boxes, normalized Phong of this exponent
+        box_mat = len(mats)
+        mats.append(dict(type=MAT_PHONG, R=(0.8, 0.8, 0.8), exponent=float(phong_boxes), Le=(0, 0, 0)))
+    add(*_box((-0.35, -0.4, -0.3), (0.6, 1.2, 0.6), 17.0), box_mat)        # tall box
+    add(*_box((0.35, -0.7, 0.3), (0.6, 0.6, 0.6), -17.0), box_mat)         # short box
     first_light = len(T)
     L = 0.125
     add(*_quad([-L, 0.999, -L], [L, 0.999, -L], [L, 0.999, L], [-L, 0.999, L]), 3)   # light, normal -y
     verts = np.array(V, np.float32)
     tris = np.array(T, np.uint32)
-    # fix every face's winding so its normal points the intended way
     return dict(verts=verts, tris=tris, normals=None, tri_mat=np.array(M, np.uint32), materials=mats,
                 lights=[(first_light, 2)],
                 camera=dict(eye=(0.0, 0.0, 3.8), center=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov_deg=40.0,
@@ -270,4 +273,74 @@ def furnace_sphere(albedo=0.5, Le=1.0, subdiv=3, width=32, height=32):
     return dict(verts=v.astype(np.float32) * 2.0, tris=f.astype(np.uint32), normals=None,
                 tri_mat=np.zeros(f.shape[0], np.uint32), materials=mats, lights=[(0, f.shape[0])],
                 camera=dict(eye=(0.0, 0.0, 0.3), center=(0.0, 0.0, -1.0), up=(0.0, 1.0, 0.0), fov_deg=60.0,
+                            width=width, height=height))
+
+
+def vertex_normals(v, f):
+    """Area-weighted vertex normals (float32, unit length)."""
+    v64 = v.astype(np.float64)
+    fn = np.cross(v64[f[:, 1]] - v64[f[:, 0]], v64[f[:, 2]] - v64[f[:, 0]])
+    n = np.zeros_like(v64)
+    for k in range(3):
+        np.add.at(n, f[:, k], fn)
+    n /= np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-30)
+    return n.astype(np.float32)
+
+
+def sphere_grid(width=1024, height=1024, grid=16, subdiv=5, seed=7):
+    """Wavefront path-tracer benchmark scene: a grid x grid array of displaced
+    icospheres (subdiv 5: 20,480 triangles each; 5,242,880 for the 16 x 16
+    grid) on a ground plane, smooth vertex normals; per-sphere material from
+    `seed`: 70% Lambertian with albedo U[0.1, 0.9] per channel, 30% normalized
+    Phong with exponent log-uniform in [10, 1000]; two emissive quads facing
+    down."""
+    rng = np.random.default_rng(seed)
+    base_v, base_f = icosphere(subdiv)
+    nv, nf = base_v.shape[0], base_f.shape[0]
+    spacing, radius = 1.0, 0.38
+    half = (grid - 1) * spacing * 0.5
+    V = np.empty((grid * grid * nv + 12, 3), np.float32)
+    F = np.empty((grid * grid * nf + 6, 3), np.uint32)
+    M = np.empty(grid * grid * nf + 6, np.uint32)
+    mats = []
+    for gi in range(grid * grid):
+        gx, gz = gi % grid, gi // grid
+        c = np.array([gx * spacing - half, radius, gz * spacing - half])
+        noise = value_noise(base_v, seed * 1000 + gi)
+        v = base_v * (radius * (1.0 + 0.1 * noise))[:, None] + c
+        V[gi * nv:(gi + 1) * nv] = v
+        F[gi * nf:(gi + 1) * nf] = base_f + gi * nv
+        M[gi * nf:(gi + 1) * nf] = len(mats)
+        R = tuple(float(x) for x in rng.uniform(0.1, 0.9, 3))
+        if rng.random() < 0.7:
+            mats.append(dict(type=MAT_DIFFUSE, R=R, exponent=0.0, Le=(0, 0, 0)))
+        else:
+            e = float(np.exp(rng.uniform(np.log(10.0), np.log(1000.0))))
+            mats.append(dict(type=MAT_PHONG, R=R, exponent=e, Le=(0, 0, 0)))
+    normals = np.empty_like(V)
+    for gi in range(grid * grid):
+        normals[gi * nv:(gi + 1) * nv] = vertex_normals(V[gi * nv:(gi + 1) * nv], base_f)
+    o = grid * grid * nv
+    of = grid * grid * nf
+    G = half + 6.0
+    # ground (Lambertian 0.5), normal +y
+    V[o:o + 4] = [[-G, 0, G], [G, 0, G], [G, 0, -G], [-G, 0, -G]]
+    normals[o:o + 4] = [0, 1, 0]
+    F[of:of + 2] = [[o, o + 1, o + 2], [o, o + 2, o + 3]]
+    M[of:of + 2] = len(mats)
+    mats.append(dict(type=MAT_DIFFUSE, R=(0.5, 0.5, 0.5), exponent=0.0, Le=(0, 0, 0)))
+    # two area lights facing down
+    lights = []
+    for k, (cx, cz, le) in enumerate(((-4.0, -2.0, (14.0, 12.0, 10.0)), (4.5, 3.0, (8.0, 10.0, 14.0)))):
+        vo = o + 4 + 4 * k
+        L = 2.0
+        V[vo:vo + 4] = [[cx - L, 7.0, cz - L], [cx + L, 7.0, cz - L], [cx + L, 7.0, cz + L], [cx - L, 7.0, cz + L]]
+        normals[vo:vo + 4] = [0, -1, 0]
+        ft = of + 2 + 2 * k
+        F[ft:ft + 2] = [[vo, vo + 1, vo + 2], [vo, vo + 2, vo + 3]]
+        M[ft:ft + 2] = len(mats)
+        mats.append(dict(type=MAT_NULL, R=(0, 0, 0), exponent=0.0, Le=le))
+        lights.append((ft, 2))
+    return dict(verts=V, tris=F, normals=normals, tri_mat=M, materials=mats, lights=lights,
+                camera=dict(eye=(0.0, 7.5, 15.5), center=(0.0, 0.0, 0.5), up=(0.0, 1.0, 0.0), fov_deg=45.0,
                             width=width, height=height))

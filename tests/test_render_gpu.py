@@ -56,6 +56,28 @@ def test_pt_per_sample_parity_uniform_table(gpu, raster_mode):
     np.testing.assert_allclose(film.cpu().numpy(), f_cpu, rtol=2e-3, atol=2e-3)
 
 
+@pytest.mark.parametrize("exponent", [20.0, 400.0])
+def test_pt_per_sample_parity_phong(gpu, exponent):
+    """Cornell box with normalized-Phong boxes, and the displaced-sphere mesh
+    with smooth vertex normals (shading-normal correction in play)."""
+    sc = scenes.cornell_box(64, 64, phong_boxes=exponent)
+    n = 64 * 64 * 16
+    _, _, so = render.Scene(sc).render_pt(n, max_num_vertices=9, seed=77, sample_out=True)
+    c = orc.Scene(sc).pt_samples(0, n, max_num_vertices=9, seed=77)
+    ok = _compare(so.cpu().numpy(), c)
+    assert ok.mean() > 0.999, f"{(~ok).sum()} of {n} samples differ"
+
+
+def test_pt_per_sample_parity_sphere_grid_small(gpu):
+    sc = scenes.sphere_grid(96, 96, grid=3, subdiv=3, seed=3)
+    n = 96 * 96 * 8
+    _, _, so = render.Scene(sc).render_pt(n, max_num_vertices=17, rr_depth=3, seed=5, sample_out=True)
+    c = orc.Scene(sc).pt_samples(0, n, max_num_vertices=17, rr_depth=3, seed=5)
+    ok = _compare(so.cpu().numpy(), c)
+    assert (c[:, 5] > 0).sum() > 100
+    assert ok.mean() > 0.999, f"{(~ok).sum()} of {n} samples differ"
+
+
 def test_pt_per_sample_parity_philox(gpu):
     sc = scenes.cornell_box(64, 64)
     n = 64 * 64 * 16
