@@ -171,6 +171,11 @@ void orc_scene_free(orc_scene *s);
 void orc_pt_samples(const orc_scene *s, const orc_pt_params *p, const float *table, uint32_t table_width,
                     uint64_t first_sample, uint64_t n, float *out, int nthreads);
 void orc_film_splat(const float *samples, uint64_t n, int32_t width, int32_t height, double *film);
+/* BSDF test hooks on a flat frame with normal n */
+int orc_bsdf_query(const orc_material *m, const float n[3], const float wi[3], const float u[2],
+                   float wo[3], float *pdf, float f[3]);
+void orc_bsdf_eval(const orc_material *m, const float n[3], const float wi[3], const float wo[3],
+                   float *pdf, float f[3]);
 
 #ifdef __cplusplus
 }

@@ -195,6 +195,36 @@ def _render_sigs(so):
     so.orc_pt_samples.restype = None
     so.orc_film_splat.argtypes = [vp, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, vp]
     so.orc_film_splat.restype = None
+    so.orc_bsdf_query.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    so.orc_bsdf_eval.argtypes = [vp, vp, vp, vp, vp, vp]
+    so.orc_bsdf_eval.restype = None
+
+
+def _rlib():
+    so = lib()
+    if not hasattr(so, "_render_sigs"):
+        _render_sigs(so)
+        so._render_sigs = True
+    return so
+
+
+def bsdf_query(mat, n, wi, u):
+    """(ok, wo, pdf_projected, f) of one oracle BSDF sample on a flat frame."""
+    m = orc_material(mat["type"], (ctypes.c_float * 3)(*mat["R"]), mat["exponent"], (ctypes.c_float * 3)(*mat["Le"]))
+    n = np.ascontiguousarray(n, np.float32); wi = np.ascontiguousarray(wi, np.float32)
+    u = np.ascontiguousarray(u, np.float32)
+    wo = np.zeros(3, np.float32); pdf = np.zeros(1, np.float32); f = np.zeros(3, np.float32)
+    ok = _rlib().orc_bsdf_query(ctypes.byref(m), _p(n), _p(wi), _p(u), _p(wo), _p(pdf), _p(f))
+    return ok, wo, float(pdf[0]), f
+
+
+def bsdf_eval(mat, n, wi, wo):
+    m = orc_material(mat["type"], (ctypes.c_float * 3)(*mat["R"]), mat["exponent"], (ctypes.c_float * 3)(*mat["Le"]))
+    n = np.ascontiguousarray(n, np.float32); wi = np.ascontiguousarray(wi, np.float32)
+    wo = np.ascontiguousarray(wo, np.float32)
+    pdf = np.zeros(1, np.float32); f = np.zeros(3, np.float32)
+    _rlib().orc_bsdf_eval(ctypes.byref(m), _p(n), _p(wi), _p(wo), _p(pdf), _p(f))
+    return float(pdf[0]), f
 
 
 def philox(ctr, key):

@@ -432,3 +432,33 @@ void orc_film_splat(const float *samples, uint64_t n, int32_t width, int32_t hei
         d[0] += sm[2]; d[1] += sm[3]; d[2] += sm[4];
     }
 }
+
+/* Test hook: one BSDF query on a flat frame with normal n (gn = sn = n):
+ * samples wo from u, returns 1 and writes wo, the projected-solid-angle pdf
+ * and f; returns 0 when the BSDF gives no direction. */
+int orc_bsdf_query(const orc_material *m, const float n[3], const float wi[3], const float u[2],
+                   float wo[3], float *pdf, float f[3])
+{
+    frame fr;
+    fr.p = mk(0, 0, 0);
+    fr.gn = fr.sn = vnorm(ld3(n));
+    onb(fr.sn, &fr.u, &fr.v);
+    v3 w;
+    if (!bsdf_sample(m, &fr, ld3(wi), u, &w)) return 0;
+    wo[0] = w.x; wo[1] = w.y; wo[2] = w.z;
+    *pdf = bsdf_pdf(m, &fr, ld3(wi), w);
+    bsdf_eval(m, &fr, ld3(wi), w, f);
+    return 1;
+}
+
+/* pdf and f for a given pair of directions on the same flat frame */
+void orc_bsdf_eval(const orc_material *m, const float n[3], const float wi[3], const float wo[3],
+                   float *pdf, float f[3])
+{
+    frame fr;
+    fr.p = mk(0, 0, 0);
+    fr.gn = fr.sn = vnorm(ld3(n));
+    onb(fr.sn, &fr.u, &fr.v);
+    *pdf = bsdf_pdf(m, &fr, ld3(wi), ld3(wo));
+    bsdf_eval(m, &fr, ld3(wi), ld3(wo), f);
+}
