@@ -89,6 +89,12 @@ int lmx_accel_upload(lmx_accel *accel);
 int lmx_accel_intersect(lmx_accel *accel, const lmx_ray *rays, uint32_t n, lmx_hit *hits,
                         void *stream);
 
+/* Batched occlusion (the any-hit form behind Scene3::Visible,
+ * include/lightmetrica/scene3.h:107-116): hits[i].id is LMX_MISS when ray i
+ * is unoccluded on [tmin, tmax], otherwise the id of some triangle it hits
+ * there (not necessarily the closest; t, b0, b1 belong to that triangle). */
+int lmx_accel_occluded(lmx_accel *accel, const lmx_ray *rays, uint32_t n, lmx_hit *hits, void *stream);
+
 /* The same query on the host (the scalar Accel3::Intersect path), walking
  * the same wide BVH in the same order as the kernel.  Host arrays. */
 int lmx_accel_intersect_host(const lmx_accel *accel, const lmx_ray *rays, uint32_t n,
@@ -179,6 +185,22 @@ void lmx_scene_free(lmx_scene *scene);
  * scheduler.cpp:281-288).  Synchronous on `stream`; stats may be NULL. */
 int lmx_render_pt(lmx_scene *scene, const lmx_render_params *params, float *film_rgb, void *stream,
                   lmx_render_stats *stats);
+
+/* renderer::vcm with mode: bdpt (renderer_vcm.cpp:662-713): an eye and a
+ * light subpath of at most max_num_vertices vertices per sample, every
+ * (s, t) connection with s + t <= max_num_vertices, power-heuristic MIS over
+ * all strategies.  Uses max_num_vertices (2..32), min_num_vertices, seed,
+ * num_samples, raster_mode; queue_capacity is the batch size in samples
+ * (0 = 2^20).  Uniform table width >= 2 (8 + 4 (max_num_vertices - 1)): an
+ * eye block then a light block, each laid out as for lmx_render_pt except
+ * 1-2 emitter position, 3 emitter selection, 4-5 emitter direction.
+ * sample_out, when given, holds (max_num_vertices + 1)^2 records of 8 floats
+ * per sample at index t (max_num_vertices + 1) + s: raster x, y, C rgb
+ * (weighted, before the W H / N film scale), 1, MIS weight, 0.  The light
+ * position and emission direction use independent numbers (the reference
+ * draws both from one, light_area.cpp:72-80, which biases s >= 2). */
+int lmx_render_bdpt(lmx_scene *scene, const lmx_render_params *params, float *film_rgb, void *stream,
+                    lmx_render_stats *stats);
 
 /* Copies the host-side BVH out for inspection: num_nodes x 128-byte nodes and
  * num_tris x 48-byte triangle records (layouts in

@@ -207,7 +207,9 @@ __device__ __forceinline__ void test_tri(const WTri& tri, const WRayPre& r, floa
 
 // The LDS stack is sized per launch to the BVH's depth (stack_depth entries
 // of 8 bytes per lane), which is what sets the kernel's occupancy.
-template <int ISECT>
+// ANYHIT: stop at the first accepted triangle (shadow rays); the hit record
+// then names that triangle, not necessarily the closest.
+template <int ISECT, bool ANYHIT>
 __global__ __launch_bounds__(TRACE_WAVE) __attribute__((amdgpu_waves_per_eu(LMX_TRACE_WAVES_PER_EU)))
 void k_trace(const WNode* __restrict__ nodes,
                                                       const WTri* __restrict__ tris,
@@ -291,7 +293,7 @@ void k_trace(const WNode* __restrict__ nodes,
         //    node registers are carried around the loop without copies.
         uint32_t tmask = 0, tbase = 0, child = 0;
         bool more = false;
-        if (valid) {
+        if (valid && !(ANYHIT && best_id != WBVH_MISS)) {
             const uint32_t inner = wb_node_hits(nd, r, best_t, tmask);
             tbase = nd.tri_base;
             g_base = nd.child_base; g_hits = inner; g_imask = nd.imask;
@@ -339,7 +341,10 @@ void k_trace(const WNode* __restrict__ nodes,
                 test_tri<ISECT>(t1, r, best_t, best_id, h0, h1, hdet);
 #endif
             }
-            // 4. ray finished: write its hit and free the lane
+            // 4. ray finished: write its hit and free the lane.  An any-hit
+            //    ray that found a triangle stops one iteration later, after
+            //    the root has been loaded for its lane (the node loaded this
+            //    iteration is not the root).
             if (!more) {
                 float4 o;
                 if (best_id == WBVH_MISS) {
@@ -452,9 +457,9 @@ int lmx_accel_upload(lmx_accel* a)
     a->lds_bytes = (size_t)a->stack_depth * TRACE_WAVE * sizeof(uint2);
     int per_cu = 0;
     if (a->host.isect == WBVH_ISECT_WATERTIGHT)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<WBVH_ISECT_WATERTIGHT>, TRACE_WAVE, a->lds_bytes);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<WBVH_ISECT_WATERTIGHT, false>, TRACE_WAVE, a->lds_bytes);
     else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<WBVH_ISECT_TRIACCEL>, TRACE_WAVE, a->lds_bytes);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<WBVH_ISECT_TRIACCEL, false>, TRACE_WAVE, a->lds_bytes);
     if (e != hipSuccess) return hip_error(e, "lmx_accel_upload: occupancy query");
     a->grid = std::max(8, cus * std::max(1, per_cu));
     a->device = dev;
@@ -466,17 +471,24 @@ int lmx_accel_upload(lmx_accel* a)
 namespace lmx {
 
 int launch_trace(lmx_accel* a, const WRay* rays, uint32_t n, const uint32_t* n_dev, uint32_t n_max,
-                 WHit* hits, hipStream_t s)
+                 WHit* hits, hipStream_t s, bool anyhit)
 {
     int rc = lmx_accel_upload(a);
     if (rc != LMX_OK) return rc;
     const int grid = std::max(1, std::min(a->grid, (int)((n_max + TRACE_WAVE - 1) / TRACE_WAVE)));
     k_init_cursors<<<1, 64, 0, s>>>(a->d_counters, n, n_dev);
-    if (a->host.isect == WBVH_ISECT_WATERTIGHT)
-        k_trace<WBVH_ISECT_WATERTIGHT><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
+    const bool wt = a->host.isect == WBVH_ISECT_WATERTIGHT;
+    if (wt && !anyhit)
+        k_trace<WBVH_ISECT_WATERTIGHT, false><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
+            a->d_nodes, a->d_tris, rays, hits, a->d_counters, a->stack_depth, REFILL_MIN);
+    else if (wt)
+        k_trace<WBVH_ISECT_WATERTIGHT, true><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
+            a->d_nodes, a->d_tris, rays, hits, a->d_counters, a->stack_depth, REFILL_MIN);
+    else if (!anyhit)
+        k_trace<WBVH_ISECT_TRIACCEL, false><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
             a->d_nodes, a->d_tris, rays, hits, a->d_counters, a->stack_depth, REFILL_MIN);
     else
-        k_trace<WBVH_ISECT_TRIACCEL><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
+        k_trace<WBVH_ISECT_TRIACCEL, true><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
             a->d_nodes, a->d_tris, rays, hits, a->d_counters, a->stack_depth, REFILL_MIN);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "trace launch");
@@ -494,7 +506,17 @@ int lmx_accel_intersect(lmx_accel* a, const lmx_ray* rays, uint32_t n, lmx_hit* 
     if (!rays || !hits) return set_error(LMX_EINVAL, "lmx_accel_intersect: NULL ray or hit array");
     if (n > 0x7fffffffu) return set_error(LMX_EINVAL, "lmx_accel_intersect: too many rays");
     return launch_trace(a, reinterpret_cast<const WRay*>(rays), n, nullptr, n, reinterpret_cast<WHit*>(hits),
-                        (hipStream_t)stream);
+                        (hipStream_t)stream, false);
+}
+
+int lmx_accel_occluded(lmx_accel* a, const lmx_ray* rays, uint32_t n, lmx_hit* hits, void* stream)
+{
+    if (!a) return set_error(LMX_EINVAL, "lmx_accel_occluded: NULL accel");
+    if (n == 0) return LMX_OK;
+    if (!rays || !hits) return set_error(LMX_EINVAL, "lmx_accel_occluded: NULL ray or hit array");
+    if (n > 0x7fffffffu) return set_error(LMX_EINVAL, "lmx_accel_occluded: too many rays");
+    return launch_trace(a, reinterpret_cast<const WRay*>(rays), n, nullptr, n, reinterpret_cast<WHit*>(hits),
+                        (hipStream_t)stream, true);
 }
 
 int lmx_accel_intersect_host(const lmx_accel* a, const lmx_ray* rays, uint32_t n, lmx_hit* hits,

@@ -1,0 +1,705 @@
+// lmx_bdpt.hip -- the bidirectional path sampler of renderer::vcm in bdpt
+// mode (src/liblightmetrica/renderer/renderer_vcm.cpp:51-341,662-713, with
+// SubpathSampler::TraceSubpath_, subpathsampler.cpp:42-203) as a batched
+// wavefront pipeline on gfx950.
+//
+// Per batch of B samples:
+//   gen      -- the eye and light endpoints of every sample and their first
+//               rays, into one queue;
+//   (trace, shade, commit) x (max_vertices - 1)
+//            -- subpaths of both directions advance together in one queue, so
+//               paths of every length and direction share each launch; every
+//               vertex is stored with its throughput and with the area pdfs
+//               of being generated from either side (forward pf, reverse pr),
+//               evaluated from vertex positions as the reference does;
+//   connect  -- one thread per (sample, t) evaluates every s: the connection
+//               term, the power-heuristic weight from the stored pdfs in O(n)
+//               (ratio chain instead of the reference's O(n^2) pdf
+//               re-evaluation per strategy), and appends a shadow ray;
+//   trace    -- any-hit traversal of the shadow rays;
+//   splat    -- unoccluded contributions to the film.
+// The whole batch is captured once as a hipGraph and replayed per batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lmx.h"
+#include "lmx_internal.h"
+#include "rmath.h"
+#include "rshade.h"
+#include "wbvh.h"
+
+using namespace lmx;
+
+namespace {
+
+// Vertex record: 4 float4
+//   [0] p.xyz,  flags (bits 0-1 kind, bit 2 degenerate, bits 8+: light+1)
+//   [1] sn.xyz, pf  (area pdf of generating this vertex from its own side)
+//   [2] gn.xyz, pr  (area pdf of generating it from the opposite side, given
+//                    the next two vertices of its own subpath)
+//   [3] alpha rgb (subpath throughput up to the vertex), material id
+#define VK_CAMERA 1u
+#define VK_LIGHT 2u
+#define VK_SURFACE 3u
+#define VF_DEGENERATE 4u
+
+struct BParams {
+    int32_t maxv, minv;
+    uint2 key;
+    uint32_t total;
+    int32_t raster_mode;
+    const float* table;
+    uint32_t table_width;
+    uint32_t wsub;             // uniform block width of one subpath
+    uint32_t batch;            // samples per batch
+    float* sample_out;         // (maxv+1)^2 x 8 floats per sample, or null
+    float* film;
+};
+
+struct VB {
+    float4* v[2];              // eye, light
+    uint32_t* nsub[2];
+    uint32_t* epix;
+    float2* eraster;
+};
+
+struct Vtx {
+    f3 p, sn, gn;
+    uint32_t kind;
+    bool degenerate;
+    int light;
+    float pf, pr;
+    f3 alpha;
+    uint32_t mat;
+    Frame fr;
+};
+
+RD Vtx load_v(const VB& vb, const BParams& p, int kind, uint32_t i, uint32_t k)
+{
+    const float4* r = vb.v[kind] + ((size_t)i * (size_t)p.maxv + k) * 4;
+    const float4 a = r[0], b = r[1], c = r[2], d = r[3];
+    Vtx v;
+    v.p = mk3(a.x, a.y, a.z);
+    const uint32_t fl = __float_as_uint(a.w);
+    v.kind = fl & 3u;
+    v.degenerate = (fl & VF_DEGENERATE) != 0;
+    v.light = (int)(fl >> 8) - 1;
+    v.sn = mk3(b.x, b.y, b.z); v.pf = b.w;
+    v.gn = mk3(c.x, c.y, c.z); v.pr = c.w;
+    v.alpha = mk3(d.x, d.y, d.z);
+    v.mat = __float_as_uint(d.w);
+    v.fr.p = v.p; v.fr.sn = v.sn; v.fr.gn = v.gn;
+    if (!v.degenerate) onb(v.sn, v.fr.u, v.fr.v);
+    return v;
+}
+
+RD void store_v(const VB& vb, const BParams& p, int kind, uint32_t i, uint32_t k, f3 pos, uint32_t flags, f3 sn,
+                float pf, f3 gn, float pr, f3 alpha, uint32_t mat)
+{
+    float4* r = vb.v[kind] + ((size_t)i * (size_t)p.maxv + k) * 4;
+    r[0] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(flags));
+    r[1] = make_float4(sn.x, sn.y, sn.z, pf);
+    r[2] = make_float4(gn.x, gn.y, gn.z, pr);
+    r[3] = make_float4(alpha.x, alpha.y, alpha.z, __uint_as_float(mat));
+}
+
+RD void store_pr(const VB& vb, const BParams& p, int kind, uint32_t i, uint32_t k, float pr)
+{
+    float* r = reinterpret_cast<float*>(vb.v[kind] + ((size_t)i * (size_t)p.maxv + k) * 4 + 2);
+    r[3] = pr;
+}
+
+RD f3 dir_to(f3 from, f3 to) { return normalize(sub(to, from)); }
+
+// RenderUtils::GeometryTerm (renderutils.h:46-56)
+RD float gterm(f3 pa, f3 na, bool dega, f3 pb, f3 nb, bool degb)
+{
+    f3 d = sub(pb, pa);
+    const float l2 = dot(d, d);
+    const float l = sqrtf(l2);
+    d = mk3(d.x / l, d.y / l, d.z / l);
+    float t = 1.0f;
+    if (!dega) t *= fabsf(dot(na, d));
+    if (!degb) t *= fabsf(dot(nb, neg(d)));
+    return t / l2;
+}
+RD float gterm(const Vtx& a, const Vtx& b) { return gterm(a.p, a.sn, a.degenerate, b.p, b.sn, b.degenerate); }
+
+RD float pos_pdf_light(const RScene& s, int l) { return s.lights[l].inv_area * (1.0f / (float)s.nlight); }
+
+// Area-light emission: Le one-sided (light_area.cpp:105-110); direction pdf
+// 1/pi in projected solid angle when above the surface (:84-89).
+RD bool emits_towards(const Frame& f, f3 wo) { return to_local(f, wo).z > 0.0f; }
+
+// Light point: Distribution1D::SampleReuse + UniformSampleTriangle
+// (triangleutils.h:73-120)
+RD void sample_light_point(const RScene& s, int l, float u0, float u1, f3& p, f3& gn, uint32_t& tri)
+{
+    const RLight L = s.lights[l];
+    const float* cdf = s.light_cdf + L.cdf_off;
+    uint32_t lo = 0, hi = L.count + 1;          // upper_bound over cdf[0..count]
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cdf[mid] <= u0) lo = mid + 1; else hi = mid;
+    }
+    int i = (int)lo - 1;
+    i = i < 0 ? 0 : (i > (int)L.count - 1 ? (int)L.count - 1 : i);
+    const float u2 = (u0 - cdf[i]) / (cdf[i + 1] - cdf[i]);
+    float b0, b1;
+    uniform_triangle(u2, u1, b0, b1);
+    tri = L.first + (uint32_t)i;
+    const uint32_t i0 = s.tris[3 * tri], i1 = s.tris[3 * tri + 1], i2 = s.tris[3 * tri + 2];
+    const f3 p1 = ld3(s.verts + 3 * i0), p2 = ld3(s.verts + 3 * i1), p3 = ld3(s.verts + 3 * i2);
+    p = add(add(mul(p1, 1.0f - b0 - b1), mul(p2, b0)), mul(p3, b1));
+    gn = normalize(cross(sub(p2, p1), sub(p3, p1)));
+}
+
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_bd_gen(RScene s, BParams p, BCtl* __restrict__ ctl, VB vb, Queue q)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t sample = ctl->batch_first + i;
+    const bool valid = i < p.batch && sample < p.total;
+    bool eye_ray = false, light_ray = false;
+    f3 ewo = mk3(0, 0, 0), lwo = mk3(0, 0, 0), lp = mk3(0, 0, 0), la = mk3(0, 0, 0);
+    float epdf = 0.0f, lpdf = 0.0f;
+    if (valid) {
+        const Uniforms<BParams> U{p, sample};
+        // eye endpoint (pinhole; We = 1, pdf 1)
+        const float ux = U.one(4), uy = U.one(5);
+        uint32_t pix;
+        float rx, ry;
+        bool alive = true;
+        if (p.raster_mode == LMX_RASTER_STRATIFIED) {
+            const uint32_t npix = (uint32_t)s.width * (uint32_t)s.height;
+            pix = sample % npix;
+            const uint32_t px = pix % (uint32_t)s.width, py = pix / (uint32_t)s.width;
+            ewo = camera_dir(s, ((float)px + ux) / (float)s.width, ((float)py + uy) / (float)s.height);
+            rx = ((float)px + 0.5f) / (float)s.width;
+            ry = ((float)py + 0.5f) / (float)s.height;
+        } else {
+            ewo = camera_dir(s, ux, uy);
+            alive = camera_raster(s, ewo, rx, ry);
+            if (!alive) { rx = ry = 0.0f; }
+            pix = raster_pixel(s, rx, ry);
+        }
+        vb.epix[i] = pix;
+        vb.eraster[i] = make_float2(rx, ry);
+        store_v(vb, p, 0, i, 0, s.cam_pos, VK_CAMERA | VF_DEGENERATE, mk3(0, 0, 0), 1.0f, mk3(0, 0, 0), 0.0f,
+                mk3(1, 1, 1), 0u);
+        if (alive) {
+            epdf = importance(s, ewo);
+            eye_ray = epdf > 0.0f;
+        }
+        vb.nsub[0][i] = 1;
+        // light endpoint: primitive from uEP_C, point from uEP, direction from uED
+        if (s.nlight > 0) {
+            const uint32_t b = p.wsub;
+            const float e0 = U.one(b + 1), e1 = U.one(b + 2), ec = U.one(b + 3);
+            const float d0 = U.one(b + 4), d1 = U.one(b + 5);
+            int l = (int)(ec * (float)s.nlight);
+            l = l < 0 ? 0 : (l > (int)s.nlight - 1 ? (int)s.nlight - 1 : l);
+            f3 gn;
+            uint32_t tri;
+            sample_light_point(s, l, e0, e1, lp, gn, tri);
+            const float ppdf = pos_pdf_light(s, l);
+            const float a0 = 1.0f / ppdf;
+            const uint32_t mat = s.tri_mat[tri];
+            store_v(vb, p, 1, i, 0, lp, VK_LIGHT | ((uint32_t)(l + 1) << 8), gn, ppdf, gn, 0.0f, mk3(a0, a0, a0), mat);
+            vb.nsub[1][i] = 1;
+            Frame fr;
+            fr.p = lp; fr.gn = gn; fr.sn = gn;
+            onb(gn, fr.u, fr.v);
+            lwo = to_world(fr, cosine_hemisphere(d0, d1));
+            if (emits_towards(fr, lwo)) {
+                const RMat m = s.mats[mat];
+                lpdf = LMX_INV_PI_F;
+                la = mk3(a0 * (m.Le[0] / lpdf), a0 * (m.Le[1] / lpdf), a0 * (m.Le[2] / lpdf));
+                light_ray = !(m.Le[0] == 0.0f && m.Le[1] == 0.0f && m.Le[2] == 0.0f);
+            }
+        } else {
+            vb.nsub[1][i] = 0;
+        }
+    } else if (i < p.batch) {
+        vb.nsub[0][i] = 0;
+        vb.nsub[1][i] = 0;
+    }
+    const uint32_t se = wave_append(&ctl->n_trace, eye_ray);
+    if (eye_ray) {
+        q.ray[2 * se] = make_float4(s.cam_pos.x, s.cam_pos.y, s.cam_pos.z, LMX_EPS_ISECT);
+        q.ray[2 * se + 1] = make_float4(ewo.x, ewo.y, ewo.z, LMX_FLT_MAX);
+        q.thr[se] = make_float4(1.0f, 1.0f, 1.0f, epdf);
+        q.meta[se] = make_uint4(i, 0u, 1u, 0u);
+    }
+    const uint32_t sl = wave_append(&ctl->n_trace, light_ray);
+    if (light_ray) {
+        q.ray[2 * sl] = make_float4(lp.x, lp.y, lp.z, LMX_EPS_ISECT);
+        q.ray[2 * sl + 1] = make_float4(lwo.x, lwo.y, lwo.z, LMX_FLT_MAX);
+        q.thr[sl] = make_float4(la.x, la.y, la.z, lpdf);
+        q.meta[sl] = make_uint4(i, 1u, 1u, 0u);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bd_shade(RScene s, BParams p, BCtl* __restrict__ ctl, Queue in,
+                                                  const float4* __restrict__ hits, Queue out, VB vb)
+{
+    const uint32_t n = ctl->n_trace;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t j = base + threadIdx.x;
+        bool cont = false;
+        float4 o0, o1, ot;
+        uint4 om;
+        if (j < n) {
+            const uint4 meta = in.meta[j];
+            const float4 h = hits[j];
+            const uint32_t id = __float_as_uint(h.w);
+            if (id != WBVH_MISS) {
+                const uint32_t i = meta.x, k = meta.z;
+                const int kind = (int)meta.y;
+                const float4 r0 = in.ray[2 * j], r1 = in.ray[2 * j + 1], t4 = in.thr[j];
+                Frame fr;
+                surface_at(s, mk3(r0.x, r0.y, r0.z), mk3(r1.x, r1.y, r1.z), h.x, h.y, h.z, id, fr);
+                const uint32_t mat = s.tri_mat[id];
+                const int light = s.tri_light[id];
+                const Vtx prev = load_v(vb, p, kind, i, k - 1);
+                const float G = gterm(prev.p, prev.sn, prev.degenerate, fr.p, fr.sn, false);
+                const f3 alpha = mk3(t4.x, t4.y, t4.z);
+                // area pdfs from the vertex positions (as EvaluatePathPDF
+                // evaluates them, renderer_vcm.cpp:227-253): this vertex from
+                // its own side, and the vertex two back from the opposite side
+                const f3 dpn = dir_to(prev.p, fr.p);
+                float pf;
+                if (prev.kind == VK_CAMERA) {
+                    pf = importance(s, dpn) * G;
+                } else if (prev.kind == VK_LIGHT) {
+                    pf = emits_towards(prev.fr, dpn) ? LMX_INV_PI_F * G : 0.0f;
+                } else {
+                    const Vtx prev2 = load_v(vb, p, kind, i, k - 2);
+                    const RMat pm = s.mats[prev.mat];
+                    const f3 dpp = dir_to(prev.p, prev2.p);
+                    pf = bsdf_pdf(pm, prev.fr, dpp, dpn) * G;
+                    store_pr(vb, p, kind, i, k - 2, bsdf_pdf(pm, prev.fr, dpn, dpp) * gterm(prev, prev2));
+                }
+                store_v(vb, p, kind, i, k, fr.p, VK_SURFACE | ((uint32_t)(light + 1) << 8), fr.sn, pf, fr.gn,
+                        0.0f, alpha, mat);
+                vb.nsub[kind][i] = k + 1;
+                const RMat m = s.mats[mat];
+                if ((int)k + 1 < p.maxv && is_scatterer(m)) {
+                    const f3 wi = dir_to(fr.p, prev.p);
+                    const Uniforms<BParams> U{p, ctl->batch_first + i};
+                    const float4 u = U.block((kind ? p.wsub : 0u) + LMX_PT_SLOT_SEG0 + 4u * (k - 1u));
+                    f3 wo;
+                    if (bsdf_sample(m, fr, wi, u.x, u.y, wo)) {
+                        const f3 fs = bsdf_eval_t(m, fr, wi, wo, kind == 1);
+                        if (!(fs.x == 0.0f && fs.y == 0.0f && fs.z == 0.0f)) {
+                            const float pdf = bsdf_pdf(m, fr, wi, wo);
+                            const f3 a2 = mk3(alpha.x * (fs.x / pdf), alpha.y * (fs.y / pdf), alpha.z * (fs.z / pdf));
+                            o0 = make_float4(fr.p.x, fr.p.y, fr.p.z, LMX_EPS_ISECT);
+                            o1 = make_float4(wo.x, wo.y, wo.z, LMX_FLT_MAX);
+                            ot = make_float4(a2.x, a2.y, a2.z, pdf);
+                            om = make_uint4(i, (uint32_t)kind, k + 1u, 0u);
+                            cont = true;
+                        }
+                    }
+                }
+            }
+        }
+        const uint32_t slot = wave_append(&ctl->n_next, cont);
+        if (cont) {
+            out.ray[2 * slot] = o0;
+            out.ray[2 * slot + 1] = o1;
+            out.thr[slot] = ot;
+            out.meta[slot] = om;
+        }
+    }
+}
+
+__global__ void k_bd_commit(BCtl* ctl)
+{
+    if (threadIdx.x == 0) {
+        ctl->rays += ctl->n_trace;
+        ctl->n_trace = ctl->n_next;
+        ctl->n_next = 0;
+    }
+}
+
+struct ShadowQ {
+    float4* ray;               // 2 per connection
+    float4* c;                 // weighted contribution rgb, MIS weight
+    float4* aux;               // raster x, y
+    uint4* meta;               // sample, s, t, pixel
+};
+
+RD void record(const BParams& p, uint32_t sample, int s, int t, float rx, float ry, f3 C, float w)
+{
+    if (!p.sample_out) return;
+    float4* o = reinterpret_cast<float4*>(p.sample_out +
+        8 * ((size_t)sample * (size_t)((p.maxv + 1) * (p.maxv + 1)) + (size_t)(t * (p.maxv + 1) + s)));
+    o[0] = make_float4(rx, ry, C.x, C.y);
+    o[1] = make_float4(C.z, 1.0f, w, 0.0f);
+}
+
+RD void splat(const BParams& p, uint32_t pixel, f3 C)
+{
+    float* fp = p.film + 3 * (size_t)pixel;
+    atomicAdd(fp, C.x); atomicAdd(fp + 1, C.y); atomicAdd(fp + 2, C.z);
+}
+
+// One thread per (sample, t): every s of renderer_vcm.cpp:676-713.
+__global__ __launch_bounds__(256) void k_bd_connect(RScene s, BParams p, BCtl* __restrict__ ctl, VB vb, ShadowQ sq)
+{
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = gid / (uint32_t)p.maxv;
+    const int t = (int)(gid % (uint32_t)p.maxv) + 1;
+    const uint32_t sample = ctl->batch_first + i;
+    const bool valid = i < p.batch && sample < p.total && t <= (int)vb.nsub[0][i];
+    const int nE = valid ? (int)vb.nsub[0][i] : 0;
+    const int nL = valid ? (int)vb.nsub[1][i] : 0;
+    int minS = 2 - t; if (minS < 0) minS = 0;
+    if (p.minv - t > minS) minS = p.minv - t;
+    int maxS = p.maxv - t; if (nL < maxS) maxS = nL;
+    if (!valid) { minS = 1; maxS = 0; }
+    (void)nE;
+    Vtx vE, vEp;
+    if (valid) {
+        vE = load_v(vb, p, 0, i, (uint32_t)t - 1);
+        if (t >= 2) vEp = load_v(vb, p, 0, i, (uint32_t)t - 2);
+    }
+    const uint32_t epix = valid ? vb.epix[i] : 0u;
+    const float2 eras = valid ? vb.eraster[i] : make_float2(0.0f, 0.0f);
+    // the loop runs the wave's longest s range with every lane present, so
+    // shadow rays are appended with one atomic per wave per step
+    int s_lo = minS, s_hi = maxS;
+    for (int off = 32; off > 0; off >>= 1) {
+        s_lo = min(s_lo, __shfl_xor(s_lo, off));
+        s_hi = max(s_hi, __shfl_xor(s_hi, off));
+    }
+    for (int sv = s_lo; sv <= s_hi; sv++) {
+        bool emit = false;
+        f3 C = mk3(0, 0, 0);
+        float w = 0.0f, rx = 0.0f, ry = 0.0f;
+        uint32_t pixel = epix;
+        float4 r0, r1;
+        if (sv >= minS && sv <= maxS) {
+            const int n = sv + t;
+            if (sv == 0) {
+                // the eye subpath hit a light (t >= 2)
+                if (vE.light >= 0 && vE.kind == VK_SURFACE) {
+                    const f3 d = dir_to(vE.p, vEp.p);
+                    if (emits_towards(vE.fr, d)) {
+                        const RMat m = s.mats[vE.mat];
+                        const f3 Cs = mk3(vE.alpha.x * m.Le[0], vE.alpha.y * m.Le[1], vE.alpha.z * m.Le[2]);
+                        // MIS: strategies s' = 1 .. n-1 relative to s = 0
+                        float sum = 1.0f, r = 1.0f;
+                        bool ok = true;
+                        for (int jj = 0; jj <= n - 2 && ok; jj++) {
+                            const int m_ = n - 1 - jj;     // eye index of x_jj
+                            float pL, pE;
+                            if (jj == 0) {
+                                pL = pos_pdf_light(s, vE.light);
+                                pE = vE.pf;
+                            } else if (jj == 1) {
+                                pL = LMX_INV_PI_F * gterm(vE, vEp);
+                                pE = vEp.pf;
+                            } else {
+                                const Vtx x = load_v(vb, p, 0, i, (uint32_t)m_);
+                                pL = x.pr;
+                                pE = x.pf;
+                            }
+                            if (pE == 0.0f) { ok = false; break; }
+                            r *= pL / pE;
+                            sum += r * r;
+                        }
+                        if (ok && vE.pf > 0.0f) {
+                            w = 1.0f / sum;
+                            C = mk3(Cs.x * w, Cs.y * w, Cs.z * w);
+                            rx = eras.x; ry = eras.y;
+                            splat(p, pixel, C);
+                            record(p, sample, 0, t, rx, ry, C, w);
+                        }
+                    }
+                }
+            } else {
+                const Vtx vL = load_v(vb, p, 1, i, (uint32_t)sv - 1);
+                Vtx vLp;
+                if (sv >= 2) vLp = load_v(vb, p, 1, i, (uint32_t)sv - 2);
+                const f3 dLE = dir_to(vL.p, vE.p);
+                const f3 dEL = dir_to(vE.p, vL.p);
+                const float G = gterm(vL, vE);
+                f3 fsL = mk3(0, 0, 0), fsE = mk3(0, 0, 0);
+                float pLs = 0.0f, pEsm1 = 0.0f, pEsm2 = 0.0f, pLsp1 = 0.0f;
+                // light side of the connection
+                if (sv == 1) {
+                    if (emits_towards(vL.fr, dLE)) {
+                        const RMat m = s.mats[vL.mat];
+                        fsL = mk3(m.Le[0], m.Le[1], m.Le[2]);
+                        pLs = LMX_INV_PI_F * G;
+                    }
+                } else {
+                    const RMat m = s.mats[vL.mat];
+                    if (is_scatterer(m)) {
+                        const f3 wiL = dir_to(vL.p, vLp.p);
+                        fsL = bsdf_eval_t(m, vL.fr, wiL, dLE, true);
+                        pLs = bsdf_pdf(m, vL.fr, wiL, dLE) * G;
+                        pEsm2 = bsdf_pdf(m, vL.fr, dLE, wiL) * gterm(vL, vLp);
+                    }
+                }
+                // eye side
+                if (t == 1) {
+                    const float imp = importance(s, dEL);
+                    fsE = mk3(imp, imp, imp);
+                    pEsm1 = imp * G;
+                } else {
+                    const RMat m = s.mats[vE.mat];
+                    if (is_scatterer(m)) {
+                        const f3 wiE = dir_to(vE.p, vEp.p);
+                        fsE = bsdf_eval_t(m, vE.fr, wiE, dEL, false);
+                        pEsm1 = bsdf_pdf(m, vE.fr, wiE, dEL) * G;
+                        pLsp1 = bsdf_pdf(m, vE.fr, dEL, wiE) * gterm(vE, vEp);
+                    }
+                }
+                const f3 Cs = mk3(vL.alpha.x * fsL.x * G * fsE.x * vE.alpha.x,
+                                  vL.alpha.y * fsL.y * G * fsE.y * vE.alpha.y,
+                                  vL.alpha.z * fsL.z * G * fsE.z * vE.alpha.z);
+                bool ok = !(Cs.x == 0.0f && Cs.y == 0.0f && Cs.z == 0.0f);
+                if (ok && t == 1) {
+                    ok = camera_raster(s, dEL, rx, ry);
+                    pixel = raster_pixel(s, rx, ry);
+                } else {
+                    rx = eras.x; ry = eras.y;
+                }
+                // power heuristic from the area pdfs of both sides (O(n))
+                float sum = 1.0f, r = 1.0f;
+                for (int jj = sv - 1; jj >= 0 && ok; jj--) {
+                    float pE;
+                    const Vtx x = (jj == sv - 1) ? vL : (jj == sv - 2 ? vLp : load_v(vb, p, 1, i, (uint32_t)jj));
+                    if (jj == sv - 1) pE = pEsm1;
+                    else if (jj == sv - 2) pE = pEsm2;
+                    else pE = x.pr;
+                    if (x.pf == 0.0f) { ok = false; break; }
+                    r *= pE / x.pf;
+                    sum += r * r;
+                }
+                r = 1.0f;
+                for (int jj = sv; jj <= n - 2 && ok; jj++) {
+                    const int m_ = n - 1 - jj;
+                    const Vtx x = (m_ == t - 1) ? vE : (m_ == t - 2 ? vEp : load_v(vb, p, 0, i, (uint32_t)m_));
+                    float pL;
+                    if (jj == sv) pL = pLs;
+                    else if (jj == sv + 1) pL = pLsp1;
+                    else pL = x.pr;
+                    if (x.pf == 0.0f) { ok = false; break; }
+                    r *= pL / x.pf;
+                    sum += r * r;
+                }
+                if (ok) {
+                    w = 1.0f / sum;
+                    C = mk3(Cs.x * w, Cs.y * w, Cs.z * w);
+                    // Scene3::Visible (scene3.h:107-116)
+                    f3 d = sub(vE.p, vL.p);
+                    const float l = sqrtf(dot(d, d));
+                    d = mk3(d.x / l, d.y / l, d.z / l);
+                    r0 = make_float4(vL.p.x, vL.p.y, vL.p.z, LMX_EPS_ISECT);
+                    r1 = make_float4(d.x, d.y, d.z, l * (1.0f - LMX_EPS_ISECT));
+                    emit = true;
+                }
+            }
+        }
+        const uint32_t slot = wave_append(&ctl->n_shadow, emit);
+        if (emit) {
+            sq.ray[2 * slot] = r0;
+            sq.ray[2 * slot + 1] = r1;
+            sq.c[slot] = make_float4(C.x, C.y, C.z, w);
+            sq.aux[slot] = make_float4(rx, ry, 0.0f, 0.0f);
+            sq.meta[slot] = make_uint4(sample, (uint32_t)sv, (uint32_t)t, pixel);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bd_splat(BParams p, const BCtl* __restrict__ ctl, ShadowQ sq,
+                                                  const float4* __restrict__ hits)
+{
+    const uint32_t n = ctl->n_shadow;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        if (__float_as_uint(hits[j].w) != WBVH_MISS) continue;      // occluded
+        const float4 c = sq.c[j];
+        const uint4 m = sq.meta[j];
+        const float4 a = sq.aux[j];
+        splat(p, m.w, mk3(c.x, c.y, c.z));
+        record(p, m.x, (int)m.y, (int)m.z, a.x, a.y, mk3(c.x, c.y, c.z), c.w);
+    }
+}
+
+__global__ void k_bd_next_batch(BCtl* ctl, uint32_t batch)
+{
+    if (threadIdx.x == 0) {
+        ctl->shadow_rays += ctl->n_shadow;
+        ctl->batch_first += batch;
+        ctl->n_trace = 0; ctl->n_next = 0; ctl->n_shadow = 0;
+    }
+}
+
+__global__ void k_bd_init(BCtl* ctl)
+{
+    if (threadIdx.x == 0) {
+        ctl->n_trace = 0; ctl->n_next = 0; ctl->n_shadow = 0; ctl->batch_first = 0;
+        ctl->rays = 0; ctl->shadow_rays = 0;
+    }
+}
+
+__global__ void k_bd_zero(float* a, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = 0.0f;
+}
+
+__global__ void k_bd_scale(float* a, size_t n, float w)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] *= w;
+}
+
+int bd_buffers(lmx_scene* sc, uint32_t B, uint32_t maxv)
+{
+    if (sc->bd_batch == B && sc->bd_maxv == maxv) return LMX_OK;
+    hipError_t e;
+    const size_t nv = (size_t)B * maxv * 4;
+    const size_t nconn = (size_t)B * (size_t)(maxv * (maxv - 1) / 2 + 1);
+    for (int k = 0; k < 2; k++) {
+        if ((e = sc->bd_vert[k].alloc(nv)) != hipSuccess) return hip_error(e, "bdpt vertex buffers");
+        if ((e = sc->bd_nsub[k].alloc(B)) != hipSuccess) return hip_error(e, "bdpt buffers");
+        if ((e = sc->bd_ray[k].alloc(4 * (size_t)B)) != hipSuccess) return hip_error(e, "bdpt queues");
+        if ((e = sc->bd_thr[k].alloc(2 * (size_t)B)) != hipSuccess) return hip_error(e, "bdpt queues");
+        if ((e = sc->bd_meta[k].alloc(2 * (size_t)B)) != hipSuccess) return hip_error(e, "bdpt queues");
+    }
+    if ((e = sc->bd_epix.alloc(B)) != hipSuccess || (e = sc->bd_eraster.alloc(B)) != hipSuccess ||
+        (e = sc->bd_hits.alloc(2 * (size_t)B)) != hipSuccess || (e = sc->bd_sray.alloc(2 * nconn)) != hipSuccess ||
+        (e = sc->bd_sc.alloc(nconn)) != hipSuccess || (e = sc->bd_saux.alloc(nconn)) != hipSuccess ||
+        (e = sc->bd_shits.alloc(nconn)) != hipSuccess || (e = sc->bd_smeta.alloc(nconn)) != hipSuccess)
+        return hip_error(e, "bdpt buffers");
+    if (!sc->bd_ctl.p && (e = sc->bd_ctl.alloc(1)) != hipSuccess) return hip_error(e, "bdpt control block");
+    if (!sc->bd_hctl && (e = hipHostMalloc((void**)&sc->bd_hctl, sizeof(BCtl), hipHostMallocDefault)) != hipSuccess)
+        return hip_error(e, "bdpt pinned control block");
+    sc->bd_batch = B;
+    sc->bd_maxv = maxv;
+    return LMX_OK;
+}
+
+// Records one batch: generation, subpath iterations, connections, shadow
+// rays, splats, advance to the next batch.
+int bd_enqueue_batch(lmx_scene* sc, const BParams& bp, hipStream_t st)
+{
+    const uint32_t B = bp.batch;
+    VB vb;
+    vb.v[0] = sc->bd_vert[0].p; vb.v[1] = sc->bd_vert[1].p;
+    vb.nsub[0] = sc->bd_nsub[0].p; vb.nsub[1] = sc->bd_nsub[1].p;
+    vb.epix = sc->bd_epix.p; vb.eraster = sc->bd_eraster.p;
+    Queue q[2];
+    for (int k = 0; k < 2; k++) { q[k].ray = sc->bd_ray[k].p; q[k].thr = sc->bd_thr[k].p; q[k].meta = sc->bd_meta[k].p; }
+    BCtl* ctl = sc->bd_ctl.p;
+    const int cap_grid = sc->cus * 8;
+    k_bd_gen<<<(B + 255) / 256, 256, 0, st>>>(sc->rs, bp, ctl, vb, q[0]);
+    int a = 0;
+    for (int it = 0; it < bp.maxv - 1; it++) {
+        int rc = launch_trace(sc->accel, reinterpret_cast<const WRay*>(q[a].ray), 0, &ctl->n_trace, 2 * B,
+                              reinterpret_cast<WHit*>(sc->bd_hits.p), st, false);
+        if (rc != LMX_OK) return rc;
+        const int grid = std::max(1, std::min((int)((2 * B + 255) / 256), cap_grid));
+        k_bd_shade<<<grid, 256, 0, st>>>(sc->rs, bp, ctl, q[a], sc->bd_hits.p, q[1 - a], vb);
+        k_bd_commit<<<1, 64, 0, st>>>(ctl);
+        a = 1 - a;
+    }
+    ShadowQ sq{sc->bd_sray.p, sc->bd_sc.p, sc->bd_saux.p, sc->bd_smeta.p};
+    const uint32_t nthreads = B * (uint32_t)bp.maxv;
+    k_bd_connect<<<(nthreads + 255) / 256, 256, 0, st>>>(sc->rs, bp, ctl, vb, sq);
+    const uint32_t nconn = B * (uint32_t)(bp.maxv * (bp.maxv - 1) / 2 + 1);
+    int rc = launch_trace(sc->accel, reinterpret_cast<const WRay*>(sq.ray), 0, &ctl->n_shadow, nconn,
+                          reinterpret_cast<WHit*>(sc->bd_shits.p), st, true);
+    if (rc != LMX_OK) return rc;
+    k_bd_splat<<<std::max(1, std::min((int)((nconn + 255) / 256), cap_grid)), 256, 0, st>>>(bp, ctl, sq, sc->bd_shits.p);
+    k_bd_next_batch<<<1, 64, 0, st>>>(ctl, B);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? LMX_OK : hip_error(e, "bdpt launch");
+}
+
+}  // namespace
+
+extern "C" {
+
+int lmx_render_bdpt(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, void* stream, lmx_render_stats* stats)
+{
+    if (!sc || !p || !film_rgb) return set_error(LMX_EINVAL, "lmx_render_bdpt: NULL argument");
+    if (p->num_samples == 0 || p->num_samples > 0xffffffffull)
+        return set_error(LMX_EINVAL, "lmx_render_bdpt: num_samples must be in [1, 2^32)");
+    if (p->max_num_vertices < 2 || p->max_num_vertices > 32)
+        return set_error(LMX_EINVAL, "lmx_render_bdpt: max_num_vertices must be in [2, 32]");
+    const uint32_t maxv = (uint32_t)p->max_num_vertices;
+    const uint32_t wsub = LMX_PT_SLOT_SEG0 + 4u * (maxv - 1u);
+    if (p->uniform_table && p->table_width < 2 * wsub)
+        return set_error(LMX_EINVAL, "lmx_render_bdpt: uniform table narrower than 2 (8 + 4 (max_num_vertices - 1))");
+    if (p->raster_mode != LMX_RASTER_RANDOM && p->raster_mode != LMX_RASTER_STRATIFIED)
+        return set_error(LMX_EINVAL, "lmx_render_bdpt: unknown raster_mode");
+    hipStream_t st = (hipStream_t)stream;
+    hipStream_t own = nullptr;
+    hipError_t e;
+    if (!st) {
+        if ((e = hipStreamCreateWithFlags(&own, hipStreamNonBlocking)) != hipSuccess) return hip_error(e, "stream create");
+        st = own;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    uint32_t B = p->queue_capacity ? p->queue_capacity : (1u << 20);
+    B = (uint32_t)std::min<uint64_t>(B, p->num_samples);
+    int rc = bd_buffers(sc, B, maxv);
+    BParams bp;
+    bp.maxv = (int)maxv; bp.minv = p->min_num_vertices;
+    bp.key = make_uint2((uint32_t)p->seed, (uint32_t)(p->seed >> 32));
+    bp.total = (uint32_t)p->num_samples;
+    bp.raster_mode = p->raster_mode;
+    bp.table = p->uniform_table; bp.table_width = p->table_width;
+    bp.wsub = wsub;
+    bp.batch = B;
+    bp.sample_out = p->sample_out;
+    bp.film = film_rgb;
+    const size_t npix = (size_t)sc->width * (size_t)sc->height;
+    const int zgrid = std::max(1, std::min((int)((3 * npix + 255) / 256), sc->cus * 8));
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (rc == LMX_OK) {
+        k_bd_zero<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix);
+        k_bd_init<<<1, 64, 0, st>>>(sc->bd_ctl.p);
+        if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess) rc = hip_error(e, "capture");
+    }
+    if (rc == LMX_OK) {
+        rc = bd_enqueue_batch(sc, bp, st);
+        e = hipStreamEndCapture(st, &graph);
+        if (rc == LMX_OK && e != hipSuccess) rc = hip_error(e, "capture end");
+        if (rc == LMX_OK && (e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0)) != hipSuccess)
+            rc = hip_error(e, "graph instantiate");
+    }
+    const uint64_t nbatch = (p->num_samples + B - 1) / B;
+    for (uint64_t b = 0; rc == LMX_OK && b < nbatch; b++)
+        if ((e = hipGraphLaunch(exec, st)) != hipSuccess) rc = hip_error(e, "graph launch");
+    if (rc == LMX_OK) {
+        k_bd_scale<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix, (float)((double)npix / (double)p->num_samples));
+        if ((e = hipMemcpyAsync(sc->bd_hctl, sc->bd_ctl.p, sizeof(BCtl), hipMemcpyDeviceToHost, st)) != hipSuccess ||
+            (e = hipStreamSynchronize(st)) != hipSuccess)
+            rc = hip_error(e, "lmx_render_bdpt: sync");
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (own) (void)hipStreamDestroy(own);
+    if (stats) {
+        stats->rays = rc == LMX_OK ? sc->bd_hctl->rays + sc->bd_hctl->shadow_rays : 0;
+        stats->iterations = (uint32_t)nbatch;
+        stats->queue_capacity = B;
+        stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return rc;
+}
+
+}  // extern "C"

@@ -14,8 +14,9 @@ int hip_error(hipError_t e, const char* what);
 
 // Enqueues closest-hit traversal of rays[0, n) on stream s, where n is *n_dev
 // when n_dev is non-null (a count written by an earlier kernel; n_max bounds
-// it and sizes the grid), else n.
+// it and sizes the grid), else n.  With anyhit, a ray stops at the first
+// triangle it hits (occlusion queries).
 int launch_trace(lmx_accel* a, const WRay* rays, uint32_t n, const uint32_t* n_dev, uint32_t n_max,
-                 WHit* hits, hipStream_t s);
+                 WHit* hits, hipStream_t s, bool anyhit = false);
 
 }  // namespace lmx

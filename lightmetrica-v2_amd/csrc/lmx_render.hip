@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -29,30 +30,12 @@
 #include "lmx.h"
 #include "lmx_internal.h"
 #include "rmath.h"
+#include "rshade.h"
 #include "wbvh.h"
 
 using namespace lmx;
 
 namespace {
-
-struct RMat {
-    int32_t type;
-    float R[3];
-    float exponent;
-    float Le[3];
-};
-static_assert(sizeof(RMat) == 32, "RMat layout");
-
-struct RScene {
-    const uint32_t* tris;
-    const float* verts;
-    const float* normals;      // may be null
-    const uint32_t* tri_mat;
-    const RMat* mats;
-    f3 cam_pos, cam_vx, cam_vy, cam_vz;
-    float tan_fov, aspect;
-    int32_t width, height;
-};
 
 struct RParams {
     int32_t max_nv, min_nv, rr_depth;
@@ -66,170 +49,6 @@ struct RParams {
     float* film;               // W x H x 3 accumulation
 };
 
-// Device-side control block.
-struct Ctl {
-    uint32_t n_trace;          // rays in the queue traced next
-    uint32_t n_next;           // continuing paths appended by shade
-    uint32_t issued;           // camera samples started
-    uint32_t iterations;
-    unsigned long long rays;   // extension rays traced so far
-};
-
-struct Queue {
-    float4* ray;               // 2 per path: (o, tmin), (d, tmax)
-    float4* thr;               // throughput rgb
-    uint4* meta;               // sample, segment, pixel, flags
-};
-
-#define META_DEAD 1u
-
-// Uniform of (sample, slot) from the table or from Philox.
-struct Uniforms {
-    const RParams& p;
-    uint32_t sample;
-    RD float one(uint32_t slot) const
-    {
-        if (p.table) return p.table[(size_t)sample * p.table_width + slot];
-        const uint4 o = philox4x32_10(make_uint4(sample, 0u, slot >> 2, LMX_PHILOX_TAG), p.key);
-        const uint32_t w = (slot & 3) == 0 ? o.x : (slot & 3) == 1 ? o.y : (slot & 3) == 2 ? o.z : o.w;
-        return u24(w);
-    }
-    // the four slots of one aligned block (slot % 4 == 0)
-    RD float4 block(uint32_t slot) const
-    {
-        if (p.table) {
-            const float* t = p.table + (size_t)sample * p.table_width + slot;
-            return make_float4(t[0], t[1], t[2], t[3]);
-        }
-        const uint4 o = philox4x32_10(make_uint4(sample, 0u, slot >> 2, LMX_PHILOX_TAG), p.key);
-        return make_float4(u24(o.x), u24(o.y), u24(o.z), u24(o.w));
-    }
-};
-
-// Pinhole sensor: raster -> direction (sensor_pinhole.cpp:79-90) and its
-// inverse with the behind/outside rejection (:165-185).
-RD f3 camera_dir(const RScene& s, float rx, float ry)
-{
-    const float x = 2.0f * rx - 1.0f, y = 2.0f * ry - 1.0f;
-    const f3 we = normalize(mk3(s.aspect * s.tan_fov * x, s.tan_fov * y, -1.0f));
-    return add(add(mul(s.cam_vx, we.x), mul(s.cam_vy, we.y)), mul(s.cam_vz, we.z));
-}
-
-RD bool camera_raster(const RScene& s, f3 wo, float& rx, float& ry)
-{
-    const f3 we = mk3(dot(s.cam_vx, wo), dot(s.cam_vy, wo), dot(s.cam_vz, wo));
-    if (we.z >= 0.0f) return false;
-    rx = (-we.x / we.z / s.tan_fov / s.aspect + 1.0f) * 0.5f;
-    ry = (-we.y / we.z / s.tan_fov + 1.0f) * 0.5f;
-    return !(rx < 0.0f || rx > 1.0f || ry < 0.0f || ry > 1.0f);
-}
-
-// Film_HDR::Splat pixel (film_hdr.cpp:218-223)
-RD uint32_t raster_pixel(const RScene& s, float rx, float ry)
-{
-    int px = (int)(rx * (float)s.width), py = (int)(ry * (float)s.height);
-    px = px < 0 ? 0 : (px > s.width - 1 ? s.width - 1 : px);
-    py = py < 0 ? 0 : (py > s.height - 1 ? s.height - 1 : py);
-    return (uint32_t)py * (uint32_t)s.width + (uint32_t)px;
-}
-
-// IntersectionUtils::CreateTriangleIntersection (intersectionutils.h:59-135)
-RD void surface_at(const RScene& s, f3 o, f3 d, float t, float b0, float b1, uint32_t id, Frame& f)
-{
-    const uint32_t i0 = s.tris[3 * id], i1 = s.tris[3 * id + 1], i2 = s.tris[3 * id + 2];
-    f.p = add(o, mul(d, t));
-    const f3 p1 = ld3(s.verts + 3 * i0), p2 = ld3(s.verts + 3 * i1), p3 = ld3(s.verts + 3 * i2);
-    f.gn = normalize(cross(sub(p2, p1), sub(p3, p1)));
-    if (s.normals) {
-        const f3 n1 = ld3(s.normals + 3 * i0), n2 = ld3(s.normals + 3 * i1), n3 = ld3(s.normals + 3 * i2);
-        const float w0 = 1.0f - b0 - b1;
-        f.sn = normalize(add(add(mul(n1, w0), mul(n2, b0)), mul(n3, b1)));
-        if (isnan(f.sn.x) || isnan(f.sn.y) || isnan(f.sn.z)) f.sn = f.gn;
-    } else {
-        f.sn = f.gn;
-    }
-    onb(f.sn, f.u, f.v);
-}
-
-RD f3 reflect_sn(const Frame& f, f3 wi) { return sub(mul(f.sn, 2.0f * dot(wi, f.sn)), wi); }
-
-// BSDF queries split as in the reference (renderer_pt.cpp:129-160): sample a
-// direction, then pdf (projected solid angle) and f.  Lambertian:
-// bsdf_diffuse.cpp:69-117; normalized Phong: f = R (n+2)/(2 pi) cos^n(a),
-// lobe sampling cos(a) = u0^(1/(n+1)) about the mirror direction.
-RD bool bsdf_sample(const RMat& m, const Frame& f, f3 wi, float u0, float u1, f3& wo)
-{
-    const f3 li = to_local(f, wi);
-    if (li.z <= 0.0f) return false;
-    if (m.type == LMX_MAT_DIFFUSE) {
-        wo = to_world(f, cosine_hemisphere(u0, u1));
-        return true;
-    }
-    if (m.type == LMX_MAT_PHONG) {
-        const f3 r = reflect_sn(f, wi);
-        f3 ru, rv;
-        onb(r, ru, rv);
-        const float ca = powf(u0, 1.0f / (m.exponent + 1.0f));
-        const float sa = sqrtf(fmaxf(0.0f, 1.0f - ca * ca));
-        const float phi = 2.0f * LMX_PI_F * u1;
-        const f3 l = mk3(sa * cosf(phi), sa * sinf(phi), ca);
-        wo = add(add(mul(ru, l.x), mul(rv, l.y)), mul(r, l.z));
-        return true;
-    }
-    return false;
-}
-
-RD float bsdf_pdf(const RMat& m, const Frame& f, f3 wi, f3 wo)
-{
-    const f3 li = to_local(f, wi), lo = to_local(f, wo);
-    if (li.z <= 0.0f || lo.z <= 0.0f) return 0.0f;
-    if (m.type == LMX_MAT_DIFFUSE) return LMX_INV_PI_F;
-    if (m.type == LMX_MAT_PHONG) {
-        const float ca = fmaxf(0.0f, dot(reflect_sn(f, wi), wo));
-        const float pdf_sa = (m.exponent + 1.0f) * (0.5f * LMX_INV_PI_F) * powf(ca, m.exponent);
-        return pdf_sa / lo.z;
-    }
-    return 0.0f;
-}
-
-// BSDFUtils::ShadingNormalCorrection for transport EL (bsdfutils.h:55-66)
-RD float shading_correction(const Frame& f, f3 wi, f3 wo)
-{
-    const f3 li = to_local(f, wi), lo = to_local(f, wo);
-    const float wiDotNg = dot(wi, f.gn), woDotNg = dot(wo, f.gn);
-    if (wiDotNg * li.z <= 0.0f || woDotNg * lo.z <= 0.0f) return 0.0f;
-    return 1.0f;
-}
-
-RD f3 bsdf_eval(const RMat& m, const Frame& f, f3 wi, f3 wo)
-{
-    const f3 li = to_local(f, wi), lo = to_local(f, wo);
-    if (li.z <= 0.0f || lo.z <= 0.0f) return mk3(0.0f, 0.0f, 0.0f);
-    const float corr = shading_correction(f, wi, wo);
-    if (m.type == LMX_MAT_DIFFUSE)
-        return mk3(m.R[0] * LMX_INV_PI_F * corr, m.R[1] * LMX_INV_PI_F * corr, m.R[2] * LMX_INV_PI_F * corr);
-    if (m.type == LMX_MAT_PHONG) {
-        const float ca = fmaxf(0.0f, dot(reflect_sn(f, wi), wo));
-        const float k = (m.exponent + 2.0f) * (0.5f * LMX_INV_PI_F) * powf(ca, m.exponent);
-        return mk3(m.R[0] * k * corr, m.R[1] * k * corr, m.R[2] * k * corr);
-    }
-    return mk3(0.0f, 0.0f, 0.0f);
-}
-
-// Appends `take` lanes of this wave to a queue with one atomic per wave;
-// returns each taking lane's slot.
-RD uint32_t wave_append(uint32_t* counter, bool take)
-{
-    const uint64_t m = __ballot(take);
-    if (m == 0) return 0;                      // wave-uniform
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-    uint32_t base = 0;
-    const uint32_t first = (uint32_t)(__ffsll((unsigned long long)m) - 1);
-    if (lane == first) base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = __shfl(base, (int)first);
-    return base + rank;
-}
 
 // ---------------------------------------------------------------------------
 // kernels
@@ -270,7 +89,7 @@ __global__ __launch_bounds__(256) void k_shade(RScene s, RParams p, Ctl* __restr
                         }
                     }
                 }
-                const Uniforms U{p, sample};
+                const Uniforms<RParams> U{p, sample};
                 bool alive = true;
                 // Russian roulette (renderer_pt.cpp:207-215), from rr_depth on
                 if ((int)seg + 1 >= p.rr_depth) {
@@ -322,7 +141,7 @@ __global__ __launch_bounds__(256) void k_gen(RScene s, RParams p, const Ctl* __r
     const uint32_t base_slot = ctl->n_next, first = ctl->issued;
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < want; t += gridDim.x * blockDim.x) {
         const uint32_t sample = first + t, slot = base_slot + t;
-        const Uniforms U{p, sample};
+        const Uniforms<RParams> U{p, sample};
         const float ux = U.one(LMX_PT_SLOT_RASTER), uy = U.one(LMX_PT_SLOT_RASTER + 1);
         float rx, ry;
         f3 wo;
@@ -384,43 +203,8 @@ __global__ void k_scale(float* a, size_t n, float w)
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] *= w;
 }
 
-template <typename T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    ~DevBuf() { if (p) (void)hipFree(p); }
-    hipError_t alloc(size_t count)
-    {
-        if (p) { (void)hipFree(p); p = nullptr; }
-        n = count;
-        return hipMalloc(&p, std::max<size_t>(1, count) * sizeof(T));
-    }
-};
-
 }  // namespace
 
-struct lmx_scene {
-    lmx_accel* accel = nullptr;
-    DevBuf<uint32_t> tris, tri_mat;
-    DevBuf<float> verts, normals;
-    DevBuf<RMat> mats;
-    RScene rs{};
-    int width = 0, height = 0;
-    int cus = 0;
-    // wavefront buffers, reused across renders of the same capacity
-    uint32_t cap = 0;
-    DevBuf<float4> ray[2], thr[2];
-    DevBuf<uint4> meta[2];
-    DevBuf<float4> hits;
-    DevBuf<Ctl> ctl;
-    Ctl* h_ctl = nullptr;      // pinned
-    uint32_t tuned_cap = 0;
-    ~lmx_scene()
-    {
-        if (h_ctl) (void)hipHostFree(h_ctl);
-        lmx_accel_free(accel);
-    }
-};
 
 namespace {
 
@@ -578,6 +362,40 @@ int lmx_scene_create(const lmx_scene_desc* d, lmx_scene** out)
         return hip_error(e, "lmx_scene_create: upload");
     if (d->normals && (e = up(sc->normals, d->normals, 3 * (size_t)d->nvert)) != hipSuccess)
         return hip_error(e, "lmx_scene_create: upload normals");
+    // light primitives: per-triangle owner and area CDFs
+    // (TriangleUtils::CreateTriangleAreaDist, triangleutils.h:47-70)
+    std::vector<int32_t> tri_light(d->ntri, -1);
+    std::vector<RLight> lights(d->nlight);
+    std::vector<float> cdf;
+    for (uint32_t l = 0; l < d->nlight; l++) {
+        const uint32_t first = d->light_prims[2 * l], n = d->light_prims[2 * l + 1];
+        if (n == 0) return set_error(LMX_EINVAL, "lmx_scene_create: empty light primitive");
+        lights[l].first = first; lights[l].count = n; lights[l].cdf_off = (uint32_t)cdf.size();
+        const size_t base = cdf.size();
+        cdf.push_back(0.0f);
+        float sum = 0.0f;
+        for (uint32_t i = 0; i < n; i++) {
+            const uint32_t* t = d->tris + 3 * (size_t)(first + i);
+            tri_light[first + i] = (int32_t)l;
+            const float* a = d->verts + 3 * (size_t)t[0];
+            const float* b = d->verts + 3 * (size_t)t[1];
+            const float* c = d->verts + 3 * (size_t)t[2];
+            const float e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+            const float e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+            const float cr[3] = {e1[1] * e2[2] - e2[1] * e1[2], e1[2] * e2[0] - e2[2] * e1[0], e1[0] * e2[1] - e2[0] * e1[1]};
+            const float dd = (cr[0] * cr[0] + cr[1] * cr[1]) + cr[2] * cr[2];
+            const float area = std::sqrt(dd) * 0.5f;
+            cdf.push_back(cdf.back() + area);
+            sum += area;
+        }
+        const float inv = 1.0f / cdf.back();
+        for (size_t i = base; i < cdf.size(); i++) cdf[i] *= inv;
+        lights[l].inv_area = 1.0f / sum;
+    }
+    if ((e = up(sc->tri_light, tri_light.data(), tri_light.size())) != hipSuccess ||
+        (e = up(sc->lights, lights.data(), lights.size())) != hipSuccess ||
+        (e = up(sc->light_cdf, cdf.data(), cdf.size())) != hipSuccess)
+        return hip_error(e, "lmx_scene_create: upload lights");
     if ((e = sc->ctl.alloc(1)) != hipSuccess) return hip_error(e, "lmx_scene_create: control block");
     if ((e = hipHostMalloc((void**)&sc->h_ctl, sizeof(Ctl), hipHostMallocDefault)) != hipSuccess)
         return hip_error(e, "lmx_scene_create: pinned control block");
@@ -588,6 +406,8 @@ int lmx_scene_create(const lmx_scene_desc* d, lmx_scene** out)
     RScene& rs = sc->rs;
     rs.tris = sc->tris.p; rs.verts = sc->verts.p; rs.normals = d->normals ? sc->normals.p : nullptr;
     rs.tri_mat = sc->tri_mat.p; rs.mats = sc->mats.p;
+    rs.tri_light = sc->tri_light.p; rs.lights = sc->lights.p; rs.light_cdf = sc->light_cdf.p;
+    rs.nlight = d->nlight;
     const lmx_camera& c = d->camera;
     rs.cam_pos = mk3(c.pos[0], c.pos[1], c.pos[2]);
     rs.cam_vx = mk3(c.vx[0], c.vx[1], c.vx[2]);

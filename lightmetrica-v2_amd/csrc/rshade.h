@@ -1,0 +1,322 @@
+// rshade.h -- device-side scene, sampling and shading routines shared by
+// the wavefront path samplers (renderer::pt in lmx_render.hip and the
+// bidirectional sampler in lmx_bdpt.hip), and the lmx_scene object.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstddef>
+#include <cstdint>
+
+#include "lmx.h"
+#include "rmath.h"
+#include "wbvh.h"
+
+namespace lmx {
+
+struct RMat {
+    int32_t type;
+    float R[3];
+    float exponent;
+    float Le[3];
+};
+static_assert(sizeof(RMat) == 32, "RMat layout");
+
+// One light primitive: its triangles [first, first + count) carry an area
+// CDF at cdf[cdf_off .. cdf_off + count] (Distribution1D, dist.h:37-111).
+struct RLight {
+    uint32_t first, count, cdf_off;
+    float inv_area;
+};
+
+struct RScene {
+    const uint32_t* tris;
+    const float* verts;
+    const float* normals;      // may be null
+    const uint32_t* tri_mat;
+    const RMat* mats;
+    const int32_t* tri_light;  // light primitive of each triangle, -1 if none
+    const RLight* lights;
+    const float* light_cdf;
+    uint32_t nlight;
+    f3 cam_pos, cam_vx, cam_vy, cam_vz;
+    float tan_fov, aspect;
+    int32_t width, height;
+};
+
+// Uniform of (sample, slot) from the table or from Philox.
+// Device-side control block.
+struct Ctl {
+    uint32_t n_trace;          // rays in the queue traced next
+    uint32_t n_next;           // continuing paths appended by shade
+    uint32_t issued;           // camera samples started
+    uint32_t iterations;
+    unsigned long long rays;   // extension rays traced so far
+};
+
+struct Queue {
+    float4* ray;               // 2 per path: (o, tmin), (d, tmax)
+    float4* thr;               // throughput rgb
+    uint4* meta;               // sample, segment, pixel, flags
+};
+
+#define META_DEAD 1u
+
+template <class P>
+struct Uniforms {
+    const P& p;
+    uint32_t sample;
+    RD float one(uint32_t slot) const
+    {
+        if (p.table) return p.table[(size_t)sample * p.table_width + slot];
+        const uint4 o = philox4x32_10(make_uint4(sample, 0u, slot >> 2, LMX_PHILOX_TAG), p.key);
+        const uint32_t w = (slot & 3) == 0 ? o.x : (slot & 3) == 1 ? o.y : (slot & 3) == 2 ? o.z : o.w;
+        return u24(w);
+    }
+    // the four slots of one aligned block (slot % 4 == 0)
+    RD float4 block(uint32_t slot) const
+    {
+        if (p.table) {
+            const float* t = p.table + (size_t)sample * p.table_width + slot;
+            return make_float4(t[0], t[1], t[2], t[3]);
+        }
+        const uint4 o = philox4x32_10(make_uint4(sample, 0u, slot >> 2, LMX_PHILOX_TAG), p.key);
+        return make_float4(u24(o.x), u24(o.y), u24(o.z), u24(o.w));
+    }
+};
+
+// Pinhole sensor: raster -> direction (sensor_pinhole.cpp:79-90) and its
+// inverse with the behind/outside rejection (:165-185).
+RD f3 camera_dir(const RScene& s, float rx, float ry)
+{
+    const float x = 2.0f * rx - 1.0f, y = 2.0f * ry - 1.0f;
+    const f3 we = normalize(mk3(s.aspect * s.tan_fov * x, s.tan_fov * y, -1.0f));
+    return add(add(mul(s.cam_vx, we.x), mul(s.cam_vy, we.y)), mul(s.cam_vz, we.z));
+}
+
+RD bool camera_raster(const RScene& s, f3 wo, float& rx, float& ry)
+{
+    const f3 we = mk3(dot(s.cam_vx, wo), dot(s.cam_vy, wo), dot(s.cam_vz, wo));
+    if (we.z >= 0.0f) return false;
+    rx = (-we.x / we.z / s.tan_fov / s.aspect + 1.0f) * 0.5f;
+    ry = (-we.y / we.z / s.tan_fov + 1.0f) * 0.5f;
+    return !(rx < 0.0f || rx > 1.0f || ry < 0.0f || ry > 1.0f);
+}
+
+// Film_HDR::Splat pixel (film_hdr.cpp:218-223)
+RD uint32_t raster_pixel(const RScene& s, float rx, float ry)
+{
+    int px = (int)(rx * (float)s.width), py = (int)(ry * (float)s.height);
+    px = px < 0 ? 0 : (px > s.width - 1 ? s.width - 1 : px);
+    py = py < 0 ? 0 : (py > s.height - 1 ? s.height - 1 : py);
+    return (uint32_t)py * (uint32_t)s.width + (uint32_t)px;
+}
+
+// IntersectionUtils::CreateTriangleIntersection (intersectionutils.h:59-135)
+RD void surface_at(const RScene& s, f3 o, f3 d, float t, float b0, float b1, uint32_t id, Frame& f)
+{
+    const uint32_t i0 = s.tris[3 * id], i1 = s.tris[3 * id + 1], i2 = s.tris[3 * id + 2];
+    f.p = add(o, mul(d, t));
+    const f3 p1 = ld3(s.verts + 3 * i0), p2 = ld3(s.verts + 3 * i1), p3 = ld3(s.verts + 3 * i2);
+    f.gn = normalize(cross(sub(p2, p1), sub(p3, p1)));
+    if (s.normals) {
+        const f3 n1 = ld3(s.normals + 3 * i0), n2 = ld3(s.normals + 3 * i1), n3 = ld3(s.normals + 3 * i2);
+        const float w0 = 1.0f - b0 - b1;
+        f.sn = normalize(add(add(mul(n1, w0), mul(n2, b0)), mul(n3, b1)));
+        if (isnan(f.sn.x) || isnan(f.sn.y) || isnan(f.sn.z)) f.sn = f.gn;
+    } else {
+        f.sn = f.gn;
+    }
+    onb(f.sn, f.u, f.v);
+}
+
+RD f3 reflect_sn(const Frame& f, f3 wi) { return sub(mul(f.sn, 2.0f * dot(wi, f.sn)), wi); }
+
+// BSDF queries split as in the reference (renderer_pt.cpp:129-160): sample a
+// direction, then pdf (projected solid angle) and f.  Lambertian:
+// bsdf_diffuse.cpp:69-117; normalized Phong: f = R (n+2)/(2 pi) cos^n(a),
+// lobe sampling cos(a) = u0^(1/(n+1)) about the mirror direction.
+RD bool bsdf_sample(const RMat& m, const Frame& f, f3 wi, float u0, float u1, f3& wo)
+{
+    const f3 li = to_local(f, wi);
+    if (li.z <= 0.0f) return false;
+    if (m.type == LMX_MAT_DIFFUSE) {
+        wo = to_world(f, cosine_hemisphere(u0, u1));
+        return true;
+    }
+    if (m.type == LMX_MAT_PHONG) {
+        const f3 r = reflect_sn(f, wi);
+        f3 ru, rv;
+        onb(r, ru, rv);
+        const float ca = powf(u0, 1.0f / (m.exponent + 1.0f));
+        const float sa = sqrtf(fmaxf(0.0f, 1.0f - ca * ca));
+        const float phi = 2.0f * LMX_PI_F * u1;
+        const f3 l = mk3(sa * cosf(phi), sa * sinf(phi), ca);
+        wo = add(add(mul(ru, l.x), mul(rv, l.y)), mul(r, l.z));
+        return true;
+    }
+    return false;
+}
+
+RD float bsdf_pdf(const RMat& m, const Frame& f, f3 wi, f3 wo)
+{
+    const f3 li = to_local(f, wi), lo = to_local(f, wo);
+    if (li.z <= 0.0f || lo.z <= 0.0f) return 0.0f;
+    if (m.type == LMX_MAT_DIFFUSE) return LMX_INV_PI_F;
+    if (m.type == LMX_MAT_PHONG) {
+        const float ca = fmaxf(0.0f, dot(reflect_sn(f, wi), wo));
+        const float pdf_sa = (m.exponent + 1.0f) * (0.5f * LMX_INV_PI_F) * powf(ca, m.exponent);
+        return pdf_sa / lo.z;
+    }
+    return 0.0f;
+}
+
+// BSDFUtils::ShadingNormalCorrection for transport EL (bsdfutils.h:55-66)
+RD float shading_correction(const Frame& f, f3 wi, f3 wo)
+{
+    const f3 li = to_local(f, wi), lo = to_local(f, wo);
+    const float wiDotNg = dot(wi, f.gn), woDotNg = dot(wo, f.gn);
+    if (wiDotNg * li.z <= 0.0f || woDotNg * lo.z <= 0.0f) return 0.0f;
+    return 1.0f;
+}
+
+RD f3 bsdf_eval(const RMat& m, const Frame& f, f3 wi, f3 wo)
+{
+    const f3 li = to_local(f, wi), lo = to_local(f, wo);
+    if (li.z <= 0.0f || lo.z <= 0.0f) return mk3(0.0f, 0.0f, 0.0f);
+    const float corr = shading_correction(f, wi, wo);
+    if (m.type == LMX_MAT_DIFFUSE)
+        return mk3(m.R[0] * LMX_INV_PI_F * corr, m.R[1] * LMX_INV_PI_F * corr, m.R[2] * LMX_INV_PI_F * corr);
+    if (m.type == LMX_MAT_PHONG) {
+        const float ca = fmaxf(0.0f, dot(reflect_sn(f, wi), wo));
+        const float k = (m.exponent + 2.0f) * (0.5f * LMX_INV_PI_F) * powf(ca, m.exponent);
+        return mk3(m.R[0] * k * corr, m.R[1] * k * corr, m.R[2] * k * corr);
+    }
+    return mk3(0.0f, 0.0f, 0.0f);
+}
+
+// BSDF f for either transport direction: BSDFUtils::ShadingNormalCorrection
+// (bsdfutils.h:55-66) is 1 for EL and wiDotNs woDotNg / (woDotNs wiDotNg)
+// for LE.
+RD f3 bsdf_eval_t(const RMat& m, const Frame& f, f3 wi, f3 wo, bool le)
+{
+    const f3 li = to_local(f, wi), lo = to_local(f, wo);
+    if (li.z <= 0.0f || lo.z <= 0.0f) return mk3(0.0f, 0.0f, 0.0f);
+    const float wiDotNg = dot(wi, f.gn), woDotNg = dot(wo, f.gn);
+    const float wiDotNs = li.z, woDotNs = lo.z;
+    float corr;
+    if (wiDotNg * wiDotNs <= 0.0f || woDotNg * woDotNs <= 0.0f) corr = 0.0f;
+    else if (le) corr = wiDotNs * woDotNg / (woDotNs * wiDotNg);
+    else corr = 1.0f;
+    if (m.type == LMX_MAT_DIFFUSE)
+        return mk3(m.R[0] * LMX_INV_PI_F * corr, m.R[1] * LMX_INV_PI_F * corr, m.R[2] * LMX_INV_PI_F * corr);
+    if (m.type == LMX_MAT_PHONG) {
+        const float ca = fmaxf(0.0f, dot(reflect_sn(f, wi), wo));
+        const float k = (m.exponent + 2.0f) * (0.5f * LMX_INV_PI_F) * powf(ca, m.exponent);
+        return mk3(m.R[0] * k * corr, m.R[1] * k * corr, m.R[2] * k * corr);
+    }
+    return mk3(0.0f, 0.0f, 0.0f);
+}
+
+RD bool is_scatterer(const RMat& m) { return m.type == LMX_MAT_DIFFUSE || m.type == LMX_MAT_PHONG; }
+
+// Pinhole importance W_e(wo) = 1 / (A cos^3) with A = 4 tan^2(fov/2) aspect,
+// zero outside the film (sensor_pinhole.cpp:137-154).  It is also the
+// sensor's direction pdf in projected solid angle.
+RD float importance(const RScene& s, f3 wo)
+{
+    float rx, ry;
+    if (!camera_raster(s, wo, rx, ry)) return 0.0f;
+    const float cos_t = -dot(s.cam_vz, wo);
+    const float inv = 1.0f / cos_t;
+    const float A = s.tan_fov * s.tan_fov * s.aspect * 4.0f;
+    return inv * inv * inv / A;
+}
+
+// Appends `take` lanes of this wave to a queue with one atomic per wave;
+// returns each taking lane's slot.
+RD uint32_t wave_append(uint32_t* counter, bool take)
+{
+    const uint64_t m = __ballot(take);
+    if (m == 0) return 0;                      // wave-uniform
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    uint32_t base = 0;
+    const uint32_t first = (uint32_t)(__ffsll((unsigned long long)m) - 1);
+    if (lane == first) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, (int)first);
+    return base + rank;
+}
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    hipError_t alloc(size_t count)
+    {
+        if (p) { (void)hipFree(p); p = nullptr; }
+        n = count;
+        return hipMalloc(&p, std::max<size_t>(1, count) * sizeof(T));
+    }
+};
+
+}  // namespace lmx
+
+using lmx::Ctl;
+using lmx::DevBuf;
+using lmx::RLight;
+using lmx::RMat;
+using lmx::RScene;
+
+// Control block of the bidirectional sampler (one batch of samples at a time).
+struct BCtl {
+    uint32_t n_trace;          // subpath rays queued for the next trace
+    uint32_t n_next;           // continuations appended by the shade step
+    uint32_t n_shadow;         // connection (shadow) rays of the batch
+    uint32_t batch_first;      // first sample of the current batch
+    unsigned long long rays;   // extension rays traced
+    unsigned long long shadow_rays;
+};
+
+// The scene object behind the C ABI: accel, shading data on the device and
+// the wavefront buffers of the samplers (reused across renders).
+struct lmx_scene {
+    lmx_accel* accel = nullptr;
+    DevBuf<uint32_t> tris, tri_mat;
+    DevBuf<int32_t> tri_light;      // light primitive of each triangle, -1 if none
+    DevBuf<float> verts, normals;
+    DevBuf<RMat> mats;
+    DevBuf<RLight> lights;
+    DevBuf<float> light_cdf;
+    RScene rs{};
+    int width = 0, height = 0;
+    int cus = 0;
+    // wavefront buffers, reused across renders of the same capacity
+    uint32_t cap = 0;
+    DevBuf<float4> ray[2], thr[2];
+    DevBuf<uint4> meta[2];
+    DevBuf<float4> hits;
+    DevBuf<Ctl> ctl;
+    Ctl* h_ctl = nullptr;      // pinned
+    uint32_t tuned_cap = 0;
+    // bidirectional sampler buffers (sized for bd_batch samples, bd_maxv vertices)
+    uint32_t bd_batch = 0, bd_maxv = 0;
+    DevBuf<float4> bd_vert[2];          // per subpath vertex: 4 float4 (see lmx_bdpt.hip)
+    DevBuf<uint32_t> bd_nsub[2];
+    DevBuf<uint32_t> bd_epix;
+    DevBuf<float2> bd_eraster;
+    DevBuf<float4> bd_ray[2], bd_thr[2];
+    DevBuf<uint4> bd_meta[2];
+    DevBuf<float4> bd_hits;
+    DevBuf<float4> bd_sray, bd_sc, bd_saux, bd_shits;
+    DevBuf<uint4> bd_smeta;
+    DevBuf<BCtl> bd_ctl;
+    BCtl* bd_hctl = nullptr;      // pinned
+    ~lmx_scene()
+    {
+        if (h_ctl) (void)hipHostFree(h_ctl);
+        if (bd_hctl) (void)hipHostFree(bd_hctl);
+        lmx_accel_free(accel);
+    }
+};

@@ -79,12 +79,16 @@ SIGNATURES = {
     "lmx_accel_upload": (ctypes.c_int, [ctypes.c_void_p]),
     "lmx_accel_intersect": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                            ctypes.c_void_p, ctypes.c_void_p]),
+    "lmx_accel_occluded": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                          ctypes.c_void_p, ctypes.c_void_p]),
     "lmx_accel_intersect_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                                 ctypes.c_void_p, ctypes.c_int32]),
     "lmx_scene_create": (ctypes.c_int, [ctypes.POINTER(lmx_scene_desc), ctypes.POINTER(ctypes.c_void_p)]),
     "lmx_scene_free": (None, [ctypes.c_void_p]),
     "lmx_render_pt": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(lmx_render_params), ctypes.c_void_p,
                                      ctypes.c_void_p, ctypes.POINTER(lmx_render_stats)]),
+    "lmx_render_bdpt": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(lmx_render_params), ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.POINTER(lmx_render_stats)]),
     "lmx_accel_export": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "lmx_accel_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                        ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),

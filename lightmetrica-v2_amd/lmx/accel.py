@@ -90,6 +90,20 @@ class Accel:
                                       ctypes.c_void_p(hits.data_ptr()), ctypes.c_void_p(stream.cuda_stream)))
         return hits
 
+    def occluded(self, rays, hits=None, stream=None):
+        """GPU any-hit query: returns (and fills) a torch float32 tensor
+        (n, 4) whose id column is MISS exactly for unoccluded rays."""
+        import torch
+        assert rays.is_cuda and rays.dtype == torch.float32 and rays.is_contiguous()
+        n = rays.shape[0]
+        if hits is None:
+            hits = torch.empty((n, 4), dtype=torch.float32, device=rays.device)
+        if stream is None:
+            stream = torch.cuda.current_stream(rays.device)
+        check(lib.lmx_accel_occluded(self._h, ctypes.c_void_p(rays.data_ptr()), n,
+                                     ctypes.c_void_p(hits.data_ptr()), ctypes.c_void_p(stream.cuda_stream)))
+        return hits
+
     def intersect_host(self, rays, nthreads=0):
         """Host walker over the same BVH (numpy in, numpy out)."""
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)

@@ -26,6 +26,11 @@ def table_width(max_num_vertices):
     return SLOT_SEG0 + 4 * (max_num_vertices - 1)
 
 
+def bdpt_table_width(max_num_vertices):
+    """Uniform-table width of the bidirectional sampler (eye + light block)."""
+    return 2 * table_width(max_num_vertices)
+
+
 class Scene:
     def __init__(self, sc):
         self._keep = []
@@ -75,6 +80,46 @@ class Scene:
 
     def __del__(self):
         self.close()
+
+    def _params(self, num_samples, max_num_vertices, min_num_vertices, rr_depth, rr_prob, seed, raster_mode,
+                queue_capacity):
+        p = lmx_render_params()
+        p.max_num_vertices = max_num_vertices
+        p.min_num_vertices = min_num_vertices
+        p.rr_depth = rr_depth
+        p.rr_prob = rr_prob
+        p.seed = seed
+        p.num_samples = num_samples
+        p.raster_mode = raster_mode
+        p.queue_capacity = queue_capacity
+        return p
+
+    def render_bdpt(self, num_samples, max_num_vertices=10, min_num_vertices=0, seed=42,
+                    raster_mode=RASTER_STRATIFIED, batch=0, table=None, sample_out=False, stream=None):
+        """renderer::vcm in bdpt mode.  Returns (film (H, W, 3), stats,
+        per-sample records (n, (m+1)^2, 8) or None)."""
+        import torch
+        dev = torch.device("cuda")
+        film = torch.empty((self.height, self.width, 3), dtype=torch.float32, device=dev)
+        p = self._params(num_samples, max_num_vertices, min_num_vertices, 1, 0.5, seed, raster_mode, batch)
+        so = None
+        if table is not None:
+            assert table.is_cuda and table.dtype == torch.float32 and table.is_contiguous()
+            assert table.shape[0] == num_samples
+            p.uniform_table = ctypes.c_void_p(table.data_ptr())
+            p.table_width = table.shape[1]
+        if sample_out:
+            m = max_num_vertices
+            so = torch.zeros((num_samples, (m + 1) * (m + 1), 8), dtype=torch.float32, device=dev)
+            p.sample_out = ctypes.c_void_p(so.data_ptr())
+        st = lmx_render_stats()
+        if stream is None:
+            stream = torch.cuda.Stream(dev)
+            stream.wait_stream(torch.cuda.current_stream(dev))
+        check(lib.lmx_render_bdpt(self._h, ctypes.byref(p), ctypes.c_void_p(film.data_ptr()),
+                                  ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st)))
+        stats = {"rays": st.rays, "batches": st.iterations, "batch": st.queue_capacity, "ms": st.ms}
+        return film, stats, so
 
     def render_pt(self, num_samples, max_num_vertices=9, min_num_vertices=0, rr_depth=1, rr_prob=0.5,
                   seed=42, raster_mode=RASTER_STRATIFIED, queue_capacity=0, table=None, sample_out=False,

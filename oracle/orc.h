@@ -171,6 +171,19 @@ void orc_scene_free(orc_scene *s);
 void orc_pt_samples(const orc_scene *s, const orc_pt_params *p, const float *table, uint32_t table_width,
                     uint64_t first_sample, uint64_t n, float *out, int nthreads);
 void orc_film_splat(const float *samples, uint64_t n, int32_t width, int32_t height, double *film);
+/* renderer::vcm, mode bdpt (orc_render.c).  max_num_vertices >= 2 bounds
+ * both subpaths and every connected path.  Output per sample: (maxv+1)^2
+ * records of 8 floats indexed t (maxv+1) + s: raster x, y, C rgb (= f w / p,
+ * before the W H / N film scale), 1 if the strategy contributed, w, 0. */
+typedef struct orc_bdpt_params {
+    int32_t max_num_vertices;
+    int32_t min_num_vertices;
+    uint32_t seed_lo, seed_hi;
+    int32_t raster_mode;
+} orc_bdpt_params;
+void orc_bdpt_samples(const orc_scene *s, const orc_bdpt_params *p, const float *table, uint32_t table_width,
+                      uint64_t first_sample, uint64_t n, float *out, int nthreads);
+
 /* BSDF test hooks on a flat frame with normal n */
 int orc_bsdf_query(const orc_material *m, const float n[3], const float wi[3], const float u[2],
                    float wo[3], float *pdf, float f[3]);

@@ -180,6 +180,11 @@ class orc_pt_params(ctypes.Structure):
                 ("seed_lo", ctypes.c_uint32), ("seed_hi", ctypes.c_uint32), ("raster_mode", ctypes.c_int32)]
 
 
+class orc_bdpt_params(ctypes.Structure):
+    _fields_ = [("max_num_vertices", ctypes.c_int32), ("min_num_vertices", ctypes.c_int32),
+                ("seed_lo", ctypes.c_uint32), ("seed_hi", ctypes.c_uint32), ("raster_mode", ctypes.c_int32)]
+
+
 def _render_sigs(so):
     vp = ctypes.c_void_p
     so.orc_philox4x32_10.argtypes = [vp, vp, vp]
@@ -195,6 +200,9 @@ def _render_sigs(so):
     so.orc_pt_samples.restype = None
     so.orc_film_splat.argtypes = [vp, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, vp]
     so.orc_film_splat.restype = None
+    so.orc_bdpt_samples.argtypes = [vp, ctypes.POINTER(orc_bdpt_params), vp, ctypes.c_uint32, ctypes.c_uint64,
+                                    ctypes.c_uint64, vp, ctypes.c_int]
+    so.orc_bdpt_samples.restype = None
     so.orc_bsdf_query.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     so.orc_bsdf_eval.argtypes = [vp, vp, vp, vp, vp, vp]
     so.orc_bsdf_eval.restype = None
@@ -308,6 +316,21 @@ class Scene:
             tw = table.shape[1]
             tp = ctypes.c_void_p(table.ctypes.data)
         lib().orc_pt_samples(self._h, ctypes.byref(p), tp, tw, first, n, _p(out), nthreads)
+        return out
+
+    def bdpt_samples(self, first, n, max_num_vertices=10, min_num_vertices=0, seed=42,
+                     raster_mode=RASTER_STRATIFIED, table=None, nthreads=8):
+        """(n, (m+1)^2, 8) float32 per (t (m+1) + s): raster x, y, C rgb
+        (f w / p, before the W H / N film scale), contributed flag, w, 0."""
+        m = max_num_vertices
+        p = orc_bdpt_params(m, min_num_vertices, seed & 0xFFFFFFFF, seed >> 32, raster_mode)
+        out = np.zeros((n, (m + 1) * (m + 1), 8), np.float32)
+        tw, tp = 0, None
+        if table is not None:
+            table = np.ascontiguousarray(table, np.float32)
+            tw = table.shape[1]
+            tp = ctypes.c_void_p(table.ctypes.data)
+        lib().orc_bdpt_samples(self._h, ctypes.byref(p), tp, tw, first, n, _p(out), nthreads)
         return out
 
     def film(self, samples):

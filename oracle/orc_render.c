@@ -302,14 +302,15 @@ static int camera_raster(const camera *c, v3 wo, float *rx, float *ry)
 /* renderer::pt, one sample */
 
 typedef struct {
-    const float *table;
+    const float *table;      /* row 0 belongs to sample `first` */
     uint32_t width;
     uint64_t seed;
+    uint64_t first;
 } usrc;
 
 static inline float U(const usrc *s, uint64_t sample, uint32_t slot)
 {
-    if (s->table) return s->table[sample * s->width + slot];
+    if (s->table) return s->table[(sample - s->first) * s->width + slot];
     return orc_uniform(s->seed, sample, slot);
 }
 
@@ -403,6 +404,7 @@ void orc_pt_samples(const orc_scene *s, const orc_pt_params *pp, const float *ta
     pt_ctx c;
     c.s = s; c.pp = pp; c.us.table = table; c.us.width = table_width;
     c.us.seed = ((uint64_t)pp->seed_hi << 32) | pp->seed_lo;
+    c.us.first = first_sample;
     camera_init(&s->d, &c.cam);
     c.first = first_sample; c.out = out;
     if (nthreads < 1) nthreads = 1;
@@ -461,4 +463,402 @@ void orc_bsdf_eval(const orc_material *m, const float n[3], const float wi[3], c
     onb(fr.sn, &fr.u, &fr.v);
     *pdf = bsdf_pdf(m, &fr, ld3(wi), ld3(wo));
     bsdf_eval(m, &fr, ld3(wi), ld3(wo), f);
+}
+
+/* ====================================================================== */
+/* renderer::vcm in bdpt mode (renderer_vcm.cpp:51-341,662-713) with
+ * SubpathSampler::TraceSubpath_ (subpathsampler.cpp:42-203): an eye and a
+ * light subpath per sample, every (s,t) connection with s+t <= max vertices,
+ * contribution f w / p with the power heuristic over all strategies that can
+ * generate the path.  Restated literally: each connection builds the full
+ * vertex list and evaluates EvaluateF / EvaluatePathPDF for every strategy
+ * (path pdfs in double, so long paths neither overflow nor underflow).
+ *
+ * One deliberate difference: the reference samples an area light's position
+ * and its emission direction from the same 2-D number
+ * (light_area.cpp:72-80, SamplePositionAndDirection(uED, uEP) at
+ * subpathsampler.cpp:86), which correlates them and biases every strategy
+ * with s >= 2.  Here the position takes uEP and the direction uED.
+ *
+ * Uniform slots per sample: an eye block then a light block, each
+ * 8 + 4 (max_vertices - 1) wide.  Within a block: 1-2 position (uEP),
+ * 3 emitter selection (uEP_C), 4-5 direction (uED), 6 component (uED_C);
+ * direction sampling at subpath vertex j >= 1 uses 8 + 4 (j-1) + 0..2. */
+
+enum { BV_CAMERA = 1, BV_LIGHT = 2, BV_SURFACE = 3 };
+
+typedef struct {
+    int kind;                 /* BV_* role of this vertex */
+    int degenerate;           /* the pinhole's position has no surface */
+    frame fr;                 /* p, gn, sn, u, v */
+    const orc_material *mat;  /* surface material (BV_SURFACE, or a light surface) */
+    uint32_t tri;             /* triangle id (surfaces / light points) */
+    int light;                /* light primitive of the triangle, -1 if none */
+    float alpha[3];           /* subpath throughput up to this vertex */
+} bvert;
+
+typedef struct {
+    const orc_scene *s;
+    camera cam;
+    int *tri_light;           /* per triangle: light primitive index or -1 */
+    float *tri_cdf_base;      /* unused placeholder */
+} bctx;
+
+static float importance(const camera *c, v3 wo)
+{
+    float rx, ry;
+    if (!camera_raster(c, wo, &rx, &ry)) return 0.0f;
+    const v3 we = mk(vdot(c->vx, wo), vdot(c->vy, wo), vdot(c->vz, wo));
+    const float cos_t = -we.z;
+    const float inv = 1.0f / cos_t;
+    const float A = c->tan_fov * c->tan_fov * c->aspect * 4.0f;
+    return inv * inv * inv / A;                     /* sensor_pinhole.cpp:137-154 */
+}
+
+/* BSDF f with the transport direction's shading-normal correction
+ * (bsdfutils.h:55-66): EL -> 1, LE -> wiDotNs woDotNg / (woDotNs wiDotNg) */
+static void bsdf_eval_dir(const orc_material *m, const frame *f, v3 wi, v3 wo, int le, float out[3])
+{
+    out[0] = out[1] = out[2] = 0.0f;
+    const v3 li = to_local(f, wi), lo = to_local(f, wo);
+    if (li.z <= 0.0f || lo.z <= 0.0f) return;
+    const float wiDotNg = vdot(wi, f->gn), woDotNg = vdot(wo, f->gn);
+    const float wiDotNs = li.z, woDotNs = lo.z;
+    float corr;
+    if (wiDotNg * wiDotNs <= 0.0f || woDotNg * woDotNs <= 0.0f) corr = 0.0f;
+    else if (le) corr = wiDotNs * woDotNg / (woDotNs * wiDotNg);
+    else corr = 1.0f;
+    if (m->type == ORC_MAT_DIFFUSE) {
+        for (int c = 0; c < 3; c++) out[c] = m->R[c] * INV_PI_F * corr;
+    } else if (m->type == ORC_MAT_PHONG) {
+        const float ca = fmaxf(0.0f, vdot(reflect_sn(f, wi), wo));
+        const float k = (m->exponent + 2.0f) * (0.5f * INV_PI_F) * powf(ca, m->exponent);
+        for (int c = 0; c < 3; c++) out[c] = m->R[c] * k * corr;
+    }
+}
+
+/* Primitive::EvaluateDirection for a vertex in its role */
+static void v_eval(const bctx *b, const bvert *v, v3 wi, v3 wo, int le, float out[3])
+{
+    out[0] = out[1] = out[2] = 0.0f;
+    if (v->kind == BV_CAMERA) {
+        const float w = importance(&b->cam, wo);
+        out[0] = out[1] = out[2] = w;
+    } else if (v->kind == BV_LIGHT) {                 /* light_area.cpp:105-110 */
+        if (to_local(&v->fr, wo).z <= 0.0f) return;
+        for (int c = 0; c < 3; c++) out[c] = v->mat->Le[c];
+    } else if (v->mat && (v->mat->type == ORC_MAT_DIFFUSE || v->mat->type == ORC_MAT_PHONG)) {
+        bsdf_eval_dir(v->mat, &v->fr, wi, wo, le, out);
+    }
+}
+
+/* Primitive::EvaluateDirectionPDF, projected solid angle */
+static float v_pdf(const bctx *b, const bvert *v, v3 wi, v3 wo)
+{
+    if (v->kind == BV_CAMERA) return importance(&b->cam, wo);
+    if (v->kind == BV_LIGHT) return to_local(&v->fr, wo).z <= 0.0f ? 0.0f : INV_PI_F;
+    if (v->mat && (v->mat->type == ORC_MAT_DIFFUSE || v->mat->type == ORC_MAT_PHONG))
+        return bsdf_pdf(v->mat, &v->fr, wi, wo);
+    return 0.0f;
+}
+
+/* RenderUtils::GeometryTerm (renderutils.h:46-56) */
+static float gterm(const bvert *a, const bvert *c)
+{
+    v3 d = vsub(c->fr.p, a->fr.p);
+    const float l2 = vdot(d, d);
+    const float l = sqrtf(l2);
+    d = mk(d.x / l, d.y / l, d.z / l);
+    float t = 1.0f;
+    if (!a->degenerate) t *= fabsf(vdot(a->fr.sn, d));
+    if (!c->degenerate) t *= fabsf(vdot(c->fr.sn, vneg(d)));
+    return t / l2;
+}
+
+static v3 dir_to(const bvert *from, const bvert *to) { return vnorm(vsub(to->fr.p, from->fr.p)); }
+
+/* EvaluatePositionGivenDirectionPDF x EvaluateEmitterPDF of an endpoint */
+static float pos_pdf(const bctx *b, const bvert *v)
+{
+    if (v->kind == BV_CAMERA) return 1.0f;
+    if (v->light < 0) return 0.0f;
+    return b->s->inv_area[v->light] * (1.0f / (float)b->s->d.nlight);
+}
+
+/* Area light sample (TriangleUtils::SampleTriangleMesh, triangleutils.h:73-120) */
+static void sample_light_point(const bctx *b, int l, const float u[2], bvert *v)
+{
+    const orc_scene *s = b->s;
+    const uint32_t first = s->d.light_prims[2 * l], n = s->d.light_prims[2 * l + 1];
+    const float *cdf = s->cdf[l];
+    /* Distribution1D::SampleReuse (dist.h:71-77) */
+    int lo = 0;
+    for (uint32_t k = 1; k <= n; k++)
+        if (cdf[k] <= u[0]) lo = (int)k; else break;
+    int i = lo > (int)n - 1 ? (int)n - 1 : lo;
+    float u2[2] = {(u[0] - cdf[i]) / (cdf[i + 1] - cdf[i]), u[1]};
+    float bb[2];
+    orc_sample_uniform_triangle(u2, bb);
+    const uint32_t tri = first + (uint32_t)i;
+    const uint32_t *t = &s->d.tris[3 * tri];
+    const v3 p1 = ld3(&s->d.verts[3 * t[0]]), p2 = ld3(&s->d.verts[3 * t[1]]), p3 = ld3(&s->d.verts[3 * t[2]]);
+    v->fr.p = vadd(vadd(vmul(p1, 1.0f - bb[0] - bb[1]), vmul(p2, bb[0])), vmul(p3, bb[1]));
+    v->fr.gn = vnorm(vcross(vsub(p2, p1), vsub(p3, p1)));
+    v->fr.sn = v->fr.gn;
+    onb(v->fr.sn, &v->fr.u, &v->fr.v);
+    v->kind = BV_LIGHT; v->degenerate = 0; v->tri = tri; v->light = l;
+    v->mat = &s->d.mats[s->d.tri_mat[tri]];
+}
+
+/* TraceSubpath_ for one direction; returns the number of vertices */
+static int trace_subpath(const bctx *b, const orc_bdpt_params *pp, const usrc *us, uint64_t sample,
+                         uint32_t block, int light_dir, bvert *vs)
+{
+    const orc_scene *s = b->s;
+    const int maxv = pp->max_num_vertices;
+    int nv = 0;
+    v3 init_wo = mk(0, 0, 0);
+    for (int step = 0; maxv == -1 || step < maxv; step++) {
+        if (step == 0) {
+            bvert v;
+            memset(&v, 0, sizeof v);
+            const float uEP[2] = {U(us, sample, block + 1), U(us, sample, block + 2)};
+            const float uEPC = U(us, sample, block + 3);
+            const float uED[2] = {U(us, sample, block + 4), U(us, sample, block + 5)};
+            if (light_dir) {
+                if (s->d.nlight == 0) return 0;
+                int l = (int)(uEPC * (float)s->d.nlight);       /* scene3.cpp:510-513 */
+                l = l < 0 ? 0 : (l > (int)s->d.nlight - 1 ? (int)s->d.nlight - 1 : l);
+                sample_light_point(b, l, uEP, &v);
+                float lw[3];
+                orc_sample_cosine_hemisphere(uED, lw);
+                init_wo = to_world(&v.fr, mk(lw[0], lw[1], lw[2]));
+                const float a = 1.0f / pos_pdf(b, &v);
+                v.alpha[0] = v.alpha[1] = v.alpha[2] = a;
+            } else {
+                v.kind = BV_CAMERA; v.degenerate = 1; v.light = -1;
+                v.fr.p = b->cam.pos;
+                if (pp->raster_mode == ORC_RASTER_STRATIFIED) {
+                    const uint64_t npix = (uint64_t)s->d.width * (uint64_t)s->d.height;
+                    const uint32_t pix = (uint32_t)(sample % npix);
+                    const uint32_t px = pix % (uint32_t)s->d.width, py = pix / (uint32_t)s->d.width;
+                    init_wo = camera_dir(&b->cam, ((float)px + uED[0]) / (float)s->d.width,
+                                         ((float)py + uED[1]) / (float)s->d.height);
+                } else {
+                    init_wo = camera_dir(&b->cam, uED[0], uED[1]);
+                }
+                v.alpha[0] = v.alpha[1] = v.alpha[2] = 1.0f;
+            }
+            vs[nv++] = v;
+            continue;
+        }
+        bvert *pv = &vs[nv - 1];
+        v3 wi, wo;
+        if (step == 1) {
+            wi = mk(0, 0, 0);
+            wo = init_wo;
+        } else {
+            const bvert *ppv = &vs[nv - 2];
+            wi = dir_to(pv, ppv);
+            const uint32_t base = block + ORC_PT_SEG0 + 4u * (uint32_t)(step - 2);
+            const float uD[2] = {U(us, sample, base), U(us, sample, base + 1)};
+            if (pv->kind != BV_SURFACE || !bsdf_sample(pv->mat, &pv->fr, wi, uD, &wo)) break;
+        }
+        float fs[3];
+        v_eval(b, pv, wi, wo, light_dir, fs);
+        if (fs[0] == 0.0f && fs[1] == 0.0f && fs[2] == 0.0f) break;
+        const float pdf = v_pdf(b, pv, wi, wo);
+        orc_hit h;
+        if (!scene_intersect(s, pv->fr.p, wo, &h)) break;
+        bvert v;
+        memset(&v, 0, sizeof v);
+        surface_at(s, pv->fr.p, wo, &h, &v.fr);
+        v.kind = BV_SURFACE; v.degenerate = 0; v.tri = h.id;
+        v.mat = &s->d.mats[s->d.tri_mat[h.id]];
+        v.light = b->tri_light[h.id];
+        for (int c = 0; c < 3; c++) v.alpha[c] = pv->alpha[c] * (fs[c] / pdf);
+        vs[nv++] = v;
+    }
+    return nv;
+}
+
+/* The full path x_0 (light end) .. x_{n-1} (camera end) of strategy s. */
+static int connect(const bctx *b, const bvert *L, int nL, const bvert *E, int nE, int s, int t, bvert *x)
+{
+    (void)nL; (void)nE;
+    int n = 0;
+    for (int i = 0; i < s; i++) x[n++] = L[i];
+    for (int i = t - 1; i >= 0; i--) x[n++] = E[i];
+    if (s == 0) {
+        if (x[0].light < 0) return 0;                            /* :83-87 */
+        x[0].kind = BV_LIGHT;
+    } else if (t > 0) {
+        /* Scene3::Visible (scene3.h:107-116) */
+        const bvert *vL = &L[s - 1], *vE = &E[t - 1];
+        v3 d = vsub(vE->fr.p, vL->fr.p);
+        const float l = sqrtf(vdot(d, d));
+        d = mk(d.x / l, d.y / l, d.z / l);
+        orc_ray r;
+        r.o[0] = vL->fr.p.x; r.o[1] = vL->fr.p.y; r.o[2] = vL->fr.p.z; r.tmin = EPS_ISECT;
+        r.d[0] = d.x; r.d[1] = d.y; r.d[2] = d.z; r.tmax = l * (1.0f - EPS_ISECT);
+        orc_hit h;
+        orc_qbvh_intersect_ex(b->s->q, &r, 1, &h, ORC_TEST_WATERTIGHT, ORC_TIE_MIN_ID, 1);
+        if (h.id != 0xffffffffu) return 0;
+    }
+    return n;
+}
+
+/* VCMPath::EvaluateF (renderer_vcm.cpp:116-200) */
+static void eval_f(const bctx *b, const bvert *x, int n, int s, float out[3])
+{
+    const int t = n - s;
+    float fL[3] = {1, 1, 1}, fE[3] = {1, 1, 1}, cst[3] = {0, 0, 0}, tmp[3];
+    if (s > 0) {
+        /* EvaluatePosition of the light endpoint = 1 */
+        for (int i = 0; i < s - 1; i++) {
+            const v3 wi = i >= 1 ? dir_to(&x[i], &x[i - 1]) : mk(0, 0, 0);
+            const v3 wo = dir_to(&x[i], &x[i + 1]);
+            v_eval(b, &x[i], wi, wo, 1, tmp);
+            const float G = gterm(&x[i], &x[i + 1]);
+            for (int c = 0; c < 3; c++) fL[c] *= tmp[c] * G;
+        }
+    }
+    if (fL[0] == 0.0f && fL[1] == 0.0f && fL[2] == 0.0f) { out[0] = out[1] = out[2] = 0.0f; return; }
+    if (t > 0) {
+        for (int i = n - 1; i > s; i--) {
+            const v3 wi = i < n - 1 ? dir_to(&x[i], &x[i + 1]) : mk(0, 0, 0);
+            const v3 wo = dir_to(&x[i], &x[i - 1]);
+            v_eval(b, &x[i], wi, wo, 0, tmp);
+            const float G = gterm(&x[i], &x[i - 1]);
+            for (int c = 0; c < 3; c++) fE[c] *= tmp[c] * G;
+        }
+    }
+    if (fE[0] == 0.0f && fE[1] == 0.0f && fE[2] == 0.0f) { out[0] = out[1] = out[2] = 0.0f; return; }
+    if (s == 0) {
+        v_eval(b, &x[0], mk(0, 0, 0), dir_to(&x[0], &x[1]), 0, cst);
+    } else {
+        float fsL[3], fsE[3];
+        const bvert *vL = &x[s - 1], *vE = &x[s];
+        v_eval(b, vL, s >= 2 ? dir_to(vL, &x[s - 2]) : mk(0, 0, 0), dir_to(vL, vE), 1, fsL);
+        v_eval(b, vE, s + 1 < n ? dir_to(vE, &x[s + 1]) : mk(0, 0, 0), dir_to(vE, vL), 0, fsE);
+        const float G = gterm(vL, vE);
+        for (int c = 0; c < 3; c++) cst[c] = fsL[c] * G * fsE[c];
+    }
+    for (int c = 0; c < 3; c++) out[c] = fL[c] * cst[c] * fE[c];
+}
+
+/* VCMPath::EvaluatePathPDF (renderer_vcm.cpp:202-256), vertex connection only */
+static double path_pdf(const bctx *b, const bvert *x, int n, int s)
+{
+    const int t = n - s;
+    if (s == 0 && t > 0) { if (x[0].kind == BV_CAMERA) return 0.0; }        /* delta position */
+    else if (s > 0 && t == 0) { if (x[n - 1].kind == BV_CAMERA) return 0.0; }
+    double pdf = 1.0;
+    if (s > 0) {
+        bvert x0 = x[0];
+        if (x0.kind != BV_LIGHT) { if (x0.light < 0) return 0.0; x0.kind = BV_LIGHT; }
+        pdf *= (double)pos_pdf(b, &x0);
+        for (int i = 0; i < s - 1; i++) {
+            const bvert *vi = i == 0 ? &x0 : &x[i];
+            const v3 wi = i >= 1 ? dir_to(&x[i], &x[i - 1]) : mk(0, 0, 0);
+            pdf *= (double)v_pdf(b, vi, wi, dir_to(&x[i], &x[i + 1])) * (double)gterm(&x[i], &x[i + 1]);
+        }
+    }
+    if (t > 0) {
+        pdf *= (double)pos_pdf(b, &x[n - 1]);
+        for (int i = n - 1; i >= s + 1; i--) {
+            const v3 wi = i + 1 < n ? dir_to(&x[i], &x[i + 1]) : mk(0, 0, 0);
+            pdf *= (double)v_pdf(b, &x[i], wi, dir_to(&x[i], &x[i - 1])) * (double)gterm(&x[i], &x[i - 1]);
+        }
+    }
+    return pdf;
+}
+
+static void bdpt_sample(const bctx *b, const orc_bdpt_params *pp, const usrc *us, uint64_t sample, float *out)
+{
+    const int maxv = pp->max_num_vertices;
+    const int stride = (maxv + 1) * (maxv + 1);
+    memset(out, 0, sizeof(float) * 8 * (size_t)stride);
+    bvert E[64], L[64], x[128];
+    const uint32_t wsub = ORC_PT_SEG0 + 4u * (uint32_t)(maxv - 1);
+    const int nE = trace_subpath(b, pp, us, sample, 0, 0, E);           /* eye first, :671-672 */
+    const int nL = trace_subpath(b, pp, us, sample, wsub, 1, L);
+    const int npix = b->s->d.width * b->s->d.height;
+    for (int t = 1; t <= nE; t++) {
+        int minS = 2 - t; if (minS < 0) minS = 0;
+        if (pp->min_num_vertices - t > minS) minS = pp->min_num_vertices - t;
+        int maxS = maxv - t; if (nL < maxS) maxS = nL;
+        for (int s = minS; s <= maxS; s++) {
+            const int n = connect(b, L, nL, E, nE, s, t, x);
+            if (n == 0) continue;
+            float f[3];
+            eval_f(b, x, n, s, f);
+            if (f[0] == 0.0f && f[1] == 0.0f && f[2] == 0.0f) continue;
+            const double ps = path_pdf(b, x, n, s);
+            if (ps == 0.0) continue;
+            double invw = 0.0;
+            for (int k = 0; k <= n; k++) {
+                const double pk = path_pdf(b, x, n, k);
+                if (pk > 0.0) { const double r = pk / ps; invw += r * r; }
+            }
+            const double w = 1.0 / invw;
+            /* raster position of the path's last edge (renderer_vcm.cpp:343-350) */
+            float rx = 0, ry = 0;
+            if (t == 1) {
+                if (!camera_raster(&b->cam, dir_to(&x[n - 1], &x[n - 2]), &rx, &ry)) continue;
+            } else if (pp->raster_mode == ORC_RASTER_STRATIFIED) {
+                const uint32_t pix = (uint32_t)(sample % (uint64_t)npix);
+                rx = ((float)(pix % (uint32_t)b->s->d.width) + 0.5f) / (float)b->s->d.width;
+                ry = ((float)(pix / (uint32_t)b->s->d.width) + 0.5f) / (float)b->s->d.height;
+            } else {
+                if (!camera_raster(&b->cam, dir_to(&x[n - 1], &x[n - 2]), &rx, &ry)) continue;
+            }
+            float *o = &out[8 * (size_t)(t * (maxv + 1) + s)];
+            o[0] = rx; o[1] = ry;
+            for (int c = 0; c < 3; c++) o[2 + c] = (float)((double)f[c] * w / ps);
+            o[5] = 1.0f; o[6] = (float)w;
+        }
+    }
+}
+
+typedef struct { bctx *b; const orc_bdpt_params *pp; usrc us; uint64_t first, n; float *out; int stride; } bd_job_ctx;
+typedef struct { bd_job_ctx *c; uint64_t b, e; } bd_job;
+
+static void *bd_run(void *p)
+{
+    bd_job *j = (bd_job *)p;
+    for (uint64_t i = j->b; i < j->e; i++)
+        bdpt_sample(j->c->b, j->c->pp, &j->c->us, j->c->first + i, &j->c->out[(size_t)8 * j->c->stride * i]);
+    return NULL;
+}
+
+void orc_bdpt_samples(const orc_scene *s, const orc_bdpt_params *pp, const float *table, uint32_t table_width,
+                      uint64_t first_sample, uint64_t n, float *out, int nthreads)
+{
+    bctx b;
+    b.s = s;
+    camera_init(&s->d, &b.cam);
+    b.tri_light = (int *)malloc(sizeof(int) * (s->d.ntri ? s->d.ntri : 1));
+    for (uint32_t i = 0; i < s->d.ntri; i++) b.tri_light[i] = -1;
+    for (uint32_t l = 0; l < s->d.nlight; l++)
+        for (uint32_t k = 0; k < s->d.light_prims[2 * l + 1]; k++) b.tri_light[s->d.light_prims[2 * l] + k] = (int)l;
+    b.tri_cdf_base = NULL;
+    bd_job_ctx c;
+    c.b = &b; c.pp = pp; c.us.table = table; c.us.width = table_width;
+    c.us.seed = ((uint64_t)pp->seed_hi << 32) | pp->seed_lo;
+    c.us.first = first_sample;
+    c.first = first_sample; c.n = n; c.out = out;
+    c.stride = (pp->max_num_vertices + 1) * (pp->max_num_vertices + 1);
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256];
+    bd_job jobs[256];
+    for (int t = 0; t < nthreads; t++) {
+        jobs[t].c = &c;
+        jobs[t].b = n * (uint64_t)t / (uint64_t)nthreads;
+        jobs[t].e = n * (uint64_t)(t + 1) / (uint64_t)nthreads;
+        pthread_create(&th[t], NULL, bd_run, &jobs[t]);
+    }
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    free(b.tri_light);
 }

@@ -92,3 +92,33 @@ def test_accel3_analytic_quads_on_gpu(gpu, simple2, mode):
     np.testing.assert_allclose(tuv, expect[:, 0:2], atol=1e-3)
     ho = orc.closest_hit_naive(v, f, rays, test=mode, tie=orc.TIE_MIN_ID)
     np.testing.assert_array_equal(hg.view(np.uint32), ho.view(np.uint32))
+
+
+@pytest.mark.parametrize("tmax_scale", [0.5, 1.0, 2.0])
+def test_occlusion_matches_closest_hit(gpu, tmax_scale):
+    """Any-hit traversal: a ray is occluded on [tmin, tmax] exactly when the
+    closest hit lies in that range, and the reported triangle really is hit
+    there.  Mixed lengths exercise lanes that finish early and are refilled."""
+    import torch
+    v, f = scenes.displaced_icosphere(5, seed=13)
+    coh, inc = scenes.traversal_rays(v, res=96, n_incoherent=40000, seed=21)
+    rays = np.concatenate([coh, inc]).copy()
+    hc = orc.closest_hit_naive(v, f, rays, test=orc.TEST_WATERTIGHT, tie=orc.TIE_MIN_ID)
+    ids = orc.hit_ids(hc)
+    rng = np.random.default_rng(4)
+    # finite tmax around the closest hit distance (and beyond for misses)
+    t_ref = np.where(ids != orc.MISS, hc[:, 0], 3.0)
+    rays[:, 7] = (t_ref * tmax_scale * rng.uniform(0.7, 1.3, rays.shape[0])).astype(np.float32)
+    a = accel.Accel(v, f)
+    hg = a.occluded(torch.from_numpy(rays).to(gpu))
+    torch.cuda.synchronize()
+    hg = hg.cpu().numpy()
+    occ_expect = (ids != orc.MISS) & (hc[:, 0] <= rays[:, 7])
+    occ = accel.hit_ids(hg) != accel.MISS
+    np.testing.assert_array_equal(occ, occ_expect)
+    # the reported triangle is hit by that ray within range
+    k = np.nonzero(occ)[0][:2000]
+    for i in k[:200]:
+        tri = int(accel.hit_ids(hg)[i])
+        one = orc.closest_hit_naive(v, f[tri:tri + 1], rays[i:i + 1], test=orc.TEST_WATERTIGHT)
+        assert orc.hit_ids(one)[0] == 0
