@@ -118,6 +118,65 @@ def pt_section(dev, cpu, threads):
     return out
 
 
+def bdpt_section(dev, cpu, threads):
+    """Batched renderer::vcm (bdpt mode) on the sphere-grid scene (5,242,886
+    triangles, 1024x1024, 16 spp, subpaths of up to 10 vertices, every (s, t)
+    strategy with power-heuristic MIS): samples/s and Mrays/s (subpath
+    extension rays plus shadow rays), beside the oracle's per-sample
+    restatement on host threads over a bounded prefix."""
+    import torch
+    from lmx import render, scenes
+    log("bdpt: generating the sphere-grid scene")
+    sc = scenes.sphere_grid()
+    rs = render.Scene(sc)
+    n = 1024 * 1024 * 16
+    kw = dict(max_num_vertices=10, seed=42)
+    rs.render_bdpt(1 << 22, **kw)                     # autotune + warm-up
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        film, st, _ = rs.render_bdpt(n, **kw)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    dt = min(times)
+    rays = st["rays"]
+    out = {
+        "workload": "bdpt: 16x16 displaced icospheres (5,242,886 tris), 1024x1024, 16 spp, "
+                    "subpaths of up to 10 vertices, all (s,t) strategies, power heuristic",
+        "samples_per_s": round(n / dt, 1),
+        "msamples_per_s": round(n / dt / 1e6, 2),
+        "mrays_per_s": round(rays / dt / 1e6, 1),
+        "rays_per_sample": round(rays / n, 4),
+        "s_per_frame": round(dt, 4),
+        "batches": st["batches"],
+        "batch": st["batch"],
+        "image_mean": round(float(film.mean()), 5),
+        "cpu_baseline": None,
+    }
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import orc
+        log("bdpt: cpu baseline (oracle renderer::vcm bdpt)")
+        t0 = time.perf_counter()
+        osc = orc.Scene(sc)
+        build = time.perf_counter() - t0
+        osc.bdpt_samples(0, 1024, nthreads=threads, **kw)
+        done, chunk = 0, 8192
+        t0 = time.perf_counter()
+        while done < (1 << 18) and time.perf_counter() - t0 < 10.0:
+            osc.bdpt_samples(done, chunk, nthreads=threads, **kw)
+            done += chunk
+        cdt = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "msamples_per_s": round(done / cdt / 1e6, 5),
+            "cores": threads, "kind": "port",
+            "sample": f"first {done} samples of the same frame (about 10 s); oracle QBVH build "
+                      f"{build:.1f}s not timed"}
+        out["gpu_over_cpu_samples"] = round((n / dt) / (done / cdt), 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +185,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-pt", action="store_true", help="skip the wavefront path-tracer section")
+    ap.add_argument("--no-bdpt", action="store_true", help="skip the bidirectional section")
+    ap.add_argument("--only", choices=["pt", "bdpt"], default=None,
+                    help="run one renderer section alone and print its JSON (for profiling runs)")
     ap.add_argument("--no-per-set", action="store_true",
                     help="skip the separate coherent/incoherent timings (for profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -153,6 +215,11 @@ def main():
     from lmx import accel, scenes
 
     dev = torch.device("cuda:0")
+    if args.only:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        fn = pt_section if args.only == "pt" else bdpt_section
+        print(json.dumps({args.only: fn(dev, not args.no_cpu_baseline, threads)}), flush=True)
+        return
     log("generating scene")
     verts, tris = scenes.traversal_scene()
     coh, inc = scenes.traversal_rays(verts)
@@ -271,6 +338,8 @@ def main():
         }
     if not args.no_pt:
         out["pt"] = pt_section(dev, not args.no_cpu_baseline, args.cpu_threads or min(os.cpu_count() or 1, 16))
+    if not args.no_bdpt:
+        out["bdpt"] = bdpt_section(dev, not args.no_cpu_baseline, args.cpu_threads or min(os.cpu_count() or 1, 16))
     print(json.dumps(out), flush=True)
 
 

@@ -38,11 +38,18 @@ using namespace lmx;
 namespace {
 
 // Vertex record: 4 float4
-//   [0] p.xyz,  flags (bits 0-1 kind, bit 2 degenerate, bits 8+: light+1)
+//   [0] p.xyz,  flags (bits 0-1 kind, bit 2 degenerate, bits 3-15 light+1,
+//                      bits 16-31 material id)
 //   [1] sn.xyz, pf  (area pdf of generating this vertex from its own side)
 //   [2] gn.xyz, pr  (area pdf of generating it from the opposite side, given
 //                    the next two vertices of its own subpath)
-//   [3] alpha rgb (subpath throughput up to the vertex), material id
+//   [3] alpha rgb (subpath throughput up to the vertex), U
+// U of vertex k is S_{k-2}, where S_j = sum over i <= j of the products
+// prod_{m=i..j} (pr_m / pf_m)^2 -- the part of the power-heuristic
+// denominator that lies beyond the connection's two nearest vertices on this
+// side (the camera vertex is excluded: a pinhole is never hit).  It obeys
+// S_j = (pr_j / pf_j)^2 (1 + S_{j-1}), so every connection's MIS weight
+// takes O(1) work; U < 0 marks a subpath with a zero pf in that range.
 #define VK_CAMERA 1u
 #define VK_LIGHT 2u
 #define VK_SURFACE 3u
@@ -75,9 +82,18 @@ struct Vtx {
     int light;
     float pf, pr;
     f3 alpha;
+    float U;
     uint32_t mat;
     Frame fr;
 };
+
+#define VB_MAX_MATERIALS 65536u
+#define VB_MAX_LIGHTS 8190u
+
+RD uint32_t vflags(uint32_t kind, bool degenerate, int light, uint32_t mat)
+{
+    return kind | (degenerate ? VF_DEGENERATE : 0u) | ((uint32_t)(light + 1) << 3) | (mat << 16);
+}
 
 RD Vtx load_v(const VB& vb, const BParams& p, int kind, uint32_t i, uint32_t k)
 {
@@ -88,24 +104,25 @@ RD Vtx load_v(const VB& vb, const BParams& p, int kind, uint32_t i, uint32_t k)
     const uint32_t fl = __float_as_uint(a.w);
     v.kind = fl & 3u;
     v.degenerate = (fl & VF_DEGENERATE) != 0;
-    v.light = (int)(fl >> 8) - 1;
+    v.light = (int)((fl >> 3) & 0x1fffu) - 1;
+    v.mat = fl >> 16;
     v.sn = mk3(b.x, b.y, b.z); v.pf = b.w;
     v.gn = mk3(c.x, c.y, c.z); v.pr = c.w;
     v.alpha = mk3(d.x, d.y, d.z);
-    v.mat = __float_as_uint(d.w);
+    v.U = d.w;
     v.fr.p = v.p; v.fr.sn = v.sn; v.fr.gn = v.gn;
     if (!v.degenerate) onb(v.sn, v.fr.u, v.fr.v);
     return v;
 }
 
 RD void store_v(const VB& vb, const BParams& p, int kind, uint32_t i, uint32_t k, f3 pos, uint32_t flags, f3 sn,
-                float pf, f3 gn, float pr, f3 alpha, uint32_t mat)
+                float pf, f3 gn, float pr, f3 alpha, float U)
 {
     float4* r = vb.v[kind] + ((size_t)i * (size_t)p.maxv + k) * 4;
     r[0] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(flags));
     r[1] = make_float4(sn.x, sn.y, sn.z, pf);
     r[2] = make_float4(gn.x, gn.y, gn.z, pr);
-    r[3] = make_float4(alpha.x, alpha.y, alpha.z, __uint_as_float(mat));
+    r[3] = make_float4(alpha.x, alpha.y, alpha.z, U);
 }
 
 RD void store_pr(const VB& vb, const BParams& p, int kind, uint32_t i, uint32_t k, float pr)
@@ -129,6 +146,17 @@ RD float gterm(f3 pa, f3 na, bool dega, f3 pb, f3 nb, bool degb)
     return t / l2;
 }
 RD float gterm(const Vtx& a, const Vtx& b) { return gterm(a.p, a.sn, a.degenerate, b.p, b.sn, b.degenerate); }
+
+// a^2 (1 + b^2 (1 + U)): the sum over one side of the squared pdf ratios of
+// the strategies that move the junction 1, 2, 3, ... vertices into it; a
+// zero factor ends the chain (the later ratios contain it).
+RD float chain_sq(float a, float b, float U)
+{
+    if (a == 0.0f) return 0.0f;
+    const float a2 = a * a;
+    if (b == 0.0f) return a2;
+    return a2 * (1.0f + b * b * (1.0f + U));
+}
 
 RD float pos_pdf_light(const RScene& s, int l) { return s.lights[l].inv_area * (1.0f / (float)s.nlight); }
 
@@ -191,8 +219,8 @@ __global__ __launch_bounds__(256) void k_bd_gen(RScene s, BParams p, BCtl* __res
         }
         vb.epix[i] = pix;
         vb.eraster[i] = make_float2(rx, ry);
-        store_v(vb, p, 0, i, 0, s.cam_pos, VK_CAMERA | VF_DEGENERATE, mk3(0, 0, 0), 1.0f, mk3(0, 0, 0), 0.0f,
-                mk3(1, 1, 1), 0u);
+        store_v(vb, p, 0, i, 0, s.cam_pos, vflags(VK_CAMERA, true, -1, 0u), mk3(0, 0, 0), 1.0f, mk3(0, 0, 0), 0.0f,
+                mk3(1, 1, 1), 0.0f);
         if (alive) {
             epdf = importance(s, ewo);
             eye_ray = epdf > 0.0f;
@@ -211,7 +239,7 @@ __global__ __launch_bounds__(256) void k_bd_gen(RScene s, BParams p, BCtl* __res
             const float ppdf = pos_pdf_light(s, l);
             const float a0 = 1.0f / ppdf;
             const uint32_t mat = s.tri_mat[tri];
-            store_v(vb, p, 1, i, 0, lp, VK_LIGHT | ((uint32_t)(l + 1) << 8), gn, ppdf, gn, 0.0f, mk3(a0, a0, a0), mat);
+            store_v(vb, p, 1, i, 0, lp, vflags(VK_LIGHT, false, l, mat), gn, ppdf, gn, 0.0f, mk3(a0, a0, a0), 0.0f);
             vb.nsub[1][i] = 1;
             Frame fr;
             fr.p = lp; fr.gn = gn; fr.sn = gn;
@@ -230,14 +258,15 @@ __global__ __launch_bounds__(256) void k_bd_gen(RScene s, BParams p, BCtl* __res
         vb.nsub[0][i] = 0;
         vb.nsub[1][i] = 0;
     }
-    const uint32_t se = wave_append(&ctl->n_trace, eye_ray);
+    __shared__ uint32_t lds[5];
+    const uint32_t se = block_reserve<4>(&ctl->n_trace, (uint32_t)eye_ray + (uint32_t)light_ray, lds);
+    const uint32_t sl = se + (uint32_t)eye_ray;
     if (eye_ray) {
         q.ray[2 * se] = make_float4(s.cam_pos.x, s.cam_pos.y, s.cam_pos.z, LMX_EPS_ISECT);
         q.ray[2 * se + 1] = make_float4(ewo.x, ewo.y, ewo.z, LMX_FLT_MAX);
         q.thr[se] = make_float4(1.0f, 1.0f, 1.0f, epdf);
         q.meta[se] = make_uint4(i, 0u, 1u, 0u);
     }
-    const uint32_t sl = wave_append(&ctl->n_trace, light_ray);
     if (light_ray) {
         q.ray[2 * sl] = make_float4(lp.x, lp.y, lp.z, LMX_EPS_ISECT);
         q.ray[2 * sl + 1] = make_float4(lwo.x, lwo.y, lwo.z, LMX_FLT_MAX);
@@ -249,9 +278,13 @@ __global__ __launch_bounds__(256) void k_bd_gen(RScene s, BParams p, BCtl* __res
 __global__ __launch_bounds__(256) void k_bd_shade(RScene s, BParams p, BCtl* __restrict__ ctl, Queue in,
                                                   const float4* __restrict__ hits, Queue out, VB vb)
 {
+    // one thread per queue entry (the grid covers the queue's capacity;
+    // workgroups past the current count leave at once)
+    __shared__ uint32_t lds[5];
     const uint32_t n = ctl->n_trace;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+    {
+        const uint32_t base = blockIdx.x * blockDim.x;
+        if (base >= n) return;
         const uint32_t j = base + threadIdx.x;
         bool cont = false;
         float4 o0, o1, ot;
@@ -275,7 +308,7 @@ __global__ __launch_bounds__(256) void k_bd_shade(RScene s, BParams p, BCtl* __r
                 // evaluates them, renderer_vcm.cpp:227-253): this vertex from
                 // its own side, and the vertex two back from the opposite side
                 const f3 dpn = dir_to(prev.p, fr.p);
-                float pf;
+                float pf, U = 0.0f;
                 if (prev.kind == VK_CAMERA) {
                     pf = importance(s, dpn) * G;
                 } else if (prev.kind == VK_LIGHT) {
@@ -285,10 +318,17 @@ __global__ __launch_bounds__(256) void k_bd_shade(RScene s, BParams p, BCtl* __r
                     const RMat pm = s.mats[prev.mat];
                     const f3 dpp = dir_to(prev.p, prev2.p);
                     pf = bsdf_pdf(pm, prev.fr, dpp, dpn) * G;
-                    store_pr(vb, p, kind, i, k - 2, bsdf_pdf(pm, prev.fr, dpn, dpp) * gterm(prev, prev2));
+                    const float pr2 = bsdf_pdf(pm, prev.fr, dpn, dpp) * gterm(prev, prev2);
+                    store_pr(vb, p, kind, i, k - 2, pr2);
+                    // U_k = S_{k-2}; on the eye side S_0 (the camera) is excluded
+                    const bool bad = prev2.pf == 0.0f || prev.U < 0.0f;
+                    const float rho = bad ? 0.0f : pr2 / prev2.pf;
+                    if (bad && (k > 2 || kind == 1)) U = -1.0f;
+                    else if (k == 2) U = kind == 1 ? rho * rho : 0.0f;
+                    else U = rho == 0.0f ? 0.0f : rho * rho * (1.0f + prev.U);
                 }
-                store_v(vb, p, kind, i, k, fr.p, VK_SURFACE | ((uint32_t)(light + 1) << 8), fr.sn, pf, fr.gn,
-                        0.0f, alpha, mat);
+                store_v(vb, p, kind, i, k, fr.p, vflags(VK_SURFACE, false, light, mat), fr.sn, pf, fr.gn,
+                        0.0f, alpha, U);
                 vb.nsub[kind][i] = k + 1;
                 const RMat m = s.mats[mat];
                 if ((int)k + 1 < p.maxv && is_scatterer(m)) {
@@ -311,7 +351,7 @@ __global__ __launch_bounds__(256) void k_bd_shade(RScene s, BParams p, BCtl* __r
                 }
             }
         }
-        const uint32_t slot = wave_append(&ctl->n_next, cont);
+        const uint32_t slot = block_reserve<4>(&ctl->n_next, cont ? 1u : 0u, lds);
         if (cont) {
             out.ray[2 * slot] = o0;
             out.ray[2 * slot + 1] = o1;
@@ -353,194 +393,193 @@ RD void splat(const BParams& p, uint32_t pixel, f3 C)
 }
 
 // One thread per (sample, t): every s of renderer_vcm.cpp:676-713.
-__global__ __launch_bounds__(256) void k_bd_connect(RScene s, BParams p, BCtl* __restrict__ ctl, VB vb, ShadowQ sq)
+// Connection plan: one thread per sample.  Strategies s = 0 (the eye
+// subpath hit a light) are evaluated and splatted here; every structurally
+// possible s >= 1 gets a shadow-queue slot, reserved with one atomic per
+// workgroup, whose (sample, s, t) the evaluation step reads.
+__global__ __launch_bounds__(256) void k_bd_plan(RScene s, BParams p, BCtl* __restrict__ ctl, VB vb, ShadowQ sq)
 {
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t i = gid / (uint32_t)p.maxv;
-    const int t = (int)(gid % (uint32_t)p.maxv) + 1;
+    __shared__ uint32_t lds[5];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t sample = ctl->batch_first + i;
-    const bool valid = i < p.batch && sample < p.total && t <= (int)vb.nsub[0][i];
+    const bool valid = i < p.batch && sample < p.total;
     const int nE = valid ? (int)vb.nsub[0][i] : 0;
     const int nL = valid ? (int)vb.nsub[1][i] : 0;
-    int minS = 2 - t; if (minS < 0) minS = 0;
-    if (p.minv - t > minS) minS = p.minv - t;
-    int maxS = p.maxv - t; if (nL < maxS) maxS = nL;
-    if (!valid) { minS = 1; maxS = 0; }
-    (void)nE;
-    Vtx vE, vEp;
-    if (valid) {
-        vE = load_v(vb, p, 0, i, (uint32_t)t - 1);
-        if (t >= 2) vEp = load_v(vb, p, 0, i, (uint32_t)t - 2);
+    // s range of each t: [max(2 - t, minv - t, 0), min(maxv - t, nL)]
+    uint32_t cnt = 0;
+    for (int t = 1; t <= nE; t++) {
+        const int s1 = max(max(1, 2 - t), p.minv - t);
+        const int s2 = min(p.maxv - t, nL);
+        if (s2 >= s1) cnt += (uint32_t)(s2 - s1 + 1);
     }
-    const uint32_t epix = valid ? vb.epix[i] : 0u;
-    const float2 eras = valid ? vb.eraster[i] : make_float2(0.0f, 0.0f);
-    // the loop runs the wave's longest s range with every lane present, so
-    // shadow rays are appended with one atomic per wave per step
-    int s_lo = minS, s_hi = maxS;
-    for (int off = 32; off > 0; off >>= 1) {
-        s_lo = min(s_lo, __shfl_xor(s_lo, off));
-        s_hi = max(s_hi, __shfl_xor(s_hi, off));
+    uint32_t slot = block_reserve<4>(&ctl->n_shadow, cnt, lds);
+    for (int t = 1; t <= nE; t++) {
+        const int s1 = max(max(1, 2 - t), p.minv - t);
+        const int s2 = min(p.maxv - t, nL);
+        for (int sv = s1; sv <= s2; sv++) sq.meta[slot++] = make_uint4(i, (uint32_t)sv, (uint32_t)t, 0u);
     }
-    for (int sv = s_lo; sv <= s_hi; sv++) {
-        bool emit = false;
-        f3 C = mk3(0, 0, 0);
+    if (!valid) return;
+    const uint32_t pixel = vb.epix[i];
+    const float2 eras = vb.eraster[i];
+    for (int t = 2; t <= nE; t++) {
+        if (max(0, p.minv - t) > 0) continue;          // s = 0 excluded by min_num_vertices
+        const Vtx vE = load_v(vb, p, 0, i, (uint32_t)t - 1);
+        if (!(vE.light >= 0 && vE.kind == VK_SURFACE)) continue;
+        const Vtx vEp = load_v(vb, p, 0, i, (uint32_t)t - 2);
         float w = 0.0f, rx = 0.0f, ry = 0.0f;
-        uint32_t pixel = epix;
-        float4 r0, r1;
-        if (sv >= minS && sv <= maxS) {
-            const int n = sv + t;
-            if (sv == 0) {
-                // the eye subpath hit a light (t >= 2)
-                if (vE.light >= 0 && vE.kind == VK_SURFACE) {
-                    const f3 d = dir_to(vE.p, vEp.p);
-                    if (emits_towards(vE.fr, d)) {
-                        const RMat m = s.mats[vE.mat];
-                        const f3 Cs = mk3(vE.alpha.x * m.Le[0], vE.alpha.y * m.Le[1], vE.alpha.z * m.Le[2]);
-                        // MIS: strategies s' = 1 .. n-1 relative to s = 0
-                        float sum = 1.0f, r = 1.0f;
-                        bool ok = true;
-                        for (int jj = 0; jj <= n - 2 && ok; jj++) {
-                            const int m_ = n - 1 - jj;     // eye index of x_jj
-                            float pL, pE;
-                            if (jj == 0) {
-                                pL = pos_pdf_light(s, vE.light);
-                                pE = vE.pf;
-                            } else if (jj == 1) {
-                                pL = LMX_INV_PI_F * gterm(vE, vEp);
-                                pE = vEp.pf;
-                            } else {
-                                const Vtx x = load_v(vb, p, 0, i, (uint32_t)m_);
-                                pL = x.pr;
-                                pE = x.pf;
-                            }
-                            if (pE == 0.0f) { ok = false; break; }
-                            r *= pL / pE;
-                            sum += r * r;
-                        }
-                        if (ok && vE.pf > 0.0f) {
-                            w = 1.0f / sum;
-                            C = mk3(Cs.x * w, Cs.y * w, Cs.z * w);
-                            rx = eras.x; ry = eras.y;
-                            splat(p, pixel, C);
-                            record(p, sample, 0, t, rx, ry, C, w);
-                        }
-                    }
-                }
-            } else {
-                const Vtx vL = load_v(vb, p, 1, i, (uint32_t)sv - 1);
-                Vtx vLp;
-                if (sv >= 2) vLp = load_v(vb, p, 1, i, (uint32_t)sv - 2);
-                const f3 dLE = dir_to(vL.p, vE.p);
-                const f3 dEL = dir_to(vE.p, vL.p);
-                const float G = gterm(vL, vE);
-                f3 fsL = mk3(0, 0, 0), fsE = mk3(0, 0, 0);
-                float pLs = 0.0f, pEsm1 = 0.0f, pEsm2 = 0.0f, pLsp1 = 0.0f;
-                // light side of the connection
-                if (sv == 1) {
-                    if (emits_towards(vL.fr, dLE)) {
-                        const RMat m = s.mats[vL.mat];
-                        fsL = mk3(m.Le[0], m.Le[1], m.Le[2]);
-                        pLs = LMX_INV_PI_F * G;
-                    }
-                } else {
-                    const RMat m = s.mats[vL.mat];
-                    if (is_scatterer(m)) {
-                        const f3 wiL = dir_to(vL.p, vLp.p);
-                        fsL = bsdf_eval_t(m, vL.fr, wiL, dLE, true);
-                        pLs = bsdf_pdf(m, vL.fr, wiL, dLE) * G;
-                        pEsm2 = bsdf_pdf(m, vL.fr, dLE, wiL) * gterm(vL, vLp);
-                    }
-                }
-                // eye side
-                if (t == 1) {
-                    const float imp = importance(s, dEL);
-                    fsE = mk3(imp, imp, imp);
-                    pEsm1 = imp * G;
-                } else {
+        f3 C;
+        {
+            // the eye subpath hit a light (t >= 2)
+            if (vE.light >= 0 && vE.kind == VK_SURFACE) {
+                const f3 d = dir_to(vE.p, vEp.p);
+                if (emits_towards(vE.fr, d)) {
                     const RMat m = s.mats[vE.mat];
-                    if (is_scatterer(m)) {
-                        const f3 wiE = dir_to(vE.p, vEp.p);
-                        fsE = bsdf_eval_t(m, vE.fr, wiE, dEL, false);
-                        pEsm1 = bsdf_pdf(m, vE.fr, wiE, dEL) * G;
-                        pLsp1 = bsdf_pdf(m, vE.fr, dEL, wiE) * gterm(vE, vEp);
+                    const f3 Cs = mk3(vE.alpha.x * m.Le[0], vE.alpha.y * m.Le[1], vE.alpha.z * m.Le[2]);
+                    // MIS: strategies s' = 1 .. n-1 relative to s = 0
+                    const bool ok = vE.pf > 0.0f && (t < 3 || (vEp.pf > 0.0f && vE.U >= 0.0f));
+                    const float sum = ok ? 1.0f + chain_sq(pos_pdf_light(s, vE.light) / vE.pf,
+                                                           t >= 3 ? LMX_INV_PI_F * gterm(vE, vEp) / vEp.pf : 0.0f,
+                                                           vE.U) : 0.0f;
+                    if (ok) {
+                        w = 1.0f / sum;
+                        C = mk3(Cs.x * w, Cs.y * w, Cs.z * w);
+                        rx = eras.x; ry = eras.y;
+                        splat(p, pixel, C);
+                        record(p, sample, 0, t, rx, ry, C, w);
                     }
-                }
-                const f3 Cs = mk3(vL.alpha.x * fsL.x * G * fsE.x * vE.alpha.x,
-                                  vL.alpha.y * fsL.y * G * fsE.y * vE.alpha.y,
-                                  vL.alpha.z * fsL.z * G * fsE.z * vE.alpha.z);
-                bool ok = !(Cs.x == 0.0f && Cs.y == 0.0f && Cs.z == 0.0f);
-                if (ok && t == 1) {
-                    ok = camera_raster(s, dEL, rx, ry);
-                    pixel = raster_pixel(s, rx, ry);
-                } else {
-                    rx = eras.x; ry = eras.y;
-                }
-                // power heuristic from the area pdfs of both sides (O(n))
-                float sum = 1.0f, r = 1.0f;
-                for (int jj = sv - 1; jj >= 0 && ok; jj--) {
-                    float pE;
-                    const Vtx x = (jj == sv - 1) ? vL : (jj == sv - 2 ? vLp : load_v(vb, p, 1, i, (uint32_t)jj));
-                    if (jj == sv - 1) pE = pEsm1;
-                    else if (jj == sv - 2) pE = pEsm2;
-                    else pE = x.pr;
-                    if (x.pf == 0.0f) { ok = false; break; }
-                    r *= pE / x.pf;
-                    sum += r * r;
-                }
-                r = 1.0f;
-                for (int jj = sv; jj <= n - 2 && ok; jj++) {
-                    const int m_ = n - 1 - jj;
-                    const Vtx x = (m_ == t - 1) ? vE : (m_ == t - 2 ? vEp : load_v(vb, p, 0, i, (uint32_t)m_));
-                    float pL;
-                    if (jj == sv) pL = pLs;
-                    else if (jj == sv + 1) pL = pLsp1;
-                    else pL = x.pr;
-                    if (x.pf == 0.0f) { ok = false; break; }
-                    r *= pL / x.pf;
-                    sum += r * r;
-                }
-                if (ok) {
-                    w = 1.0f / sum;
-                    C = mk3(Cs.x * w, Cs.y * w, Cs.z * w);
-                    // Scene3::Visible (scene3.h:107-116)
-                    f3 d = sub(vE.p, vL.p);
-                    const float l = sqrtf(dot(d, d));
-                    d = mk3(d.x / l, d.y / l, d.z / l);
-                    r0 = make_float4(vL.p.x, vL.p.y, vL.p.z, LMX_EPS_ISECT);
-                    r1 = make_float4(d.x, d.y, d.z, l * (1.0f - LMX_EPS_ISECT));
-                    emit = true;
                 }
             }
-        }
-        const uint32_t slot = wave_append(&ctl->n_shadow, emit);
-        if (emit) {
-            sq.ray[2 * slot] = r0;
-            sq.ray[2 * slot + 1] = r1;
-            sq.c[slot] = make_float4(C.x, C.y, C.z, w);
-            sq.aux[slot] = make_float4(rx, ry, 0.0f, 0.0f);
-            sq.meta[slot] = make_uint4(sample, (uint32_t)sv, (uint32_t)t, pixel);
         }
     }
 }
 
-__global__ __launch_bounds__(256) void k_bd_splat(BParams p, const BCtl* __restrict__ ctl, ShadowQ sq,
+// Connection evaluation: one thread per shadow-queue slot.  Writes the
+// shadow ray with the weighted contribution, or a ray that cannot hit
+// (tmax < tmin) with s = 0 when the connection carries nothing.
+__global__ __launch_bounds__(256) void k_bd_connect(RScene s, BParams p, BCtl* __restrict__ ctl, VB vb, ShadowQ sq)
+{
+    const uint32_t nslot = ctl->n_shadow;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nslot; j += gridDim.x * blockDim.x) {
+        const uint4 pm = sq.meta[j];
+        const uint32_t i = pm.x;
+        const int sv = (int)pm.y, t = (int)pm.z;
+        const uint32_t sample = ctl->batch_first + i;
+        const Vtx vE = load_v(vb, p, 0, i, (uint32_t)t - 1);
+        Vtx vEp;
+        if (t >= 2) vEp = load_v(vb, p, 0, i, (uint32_t)t - 2);
+        uint32_t pixel = vb.epix[i];
+        const float2 eras = vb.eraster[i];
+        bool emit = false;
+        f3 C = mk3(0, 0, 0);
+        float w = 0.0f, rx = 0.0f, ry = 0.0f;
+        float4 r0, r1;
+        const Vtx vL = load_v(vb, p, 1, i, (uint32_t)sv - 1);
+        Vtx vLp;
+        if (sv >= 2) vLp = load_v(vb, p, 1, i, (uint32_t)sv - 2);
+        const f3 dLE = dir_to(vL.p, vE.p);
+        const f3 dEL = dir_to(vE.p, vL.p);
+        const float G = gterm(vL, vE);
+        f3 fsL = mk3(0, 0, 0), fsE = mk3(0, 0, 0);
+        float pLs = 0.0f, pEsm1 = 0.0f, pEsm2 = 0.0f, pLsp1 = 0.0f;
+        // light side of the connection
+        if (sv == 1) {
+            if (emits_towards(vL.fr, dLE)) {
+                const RMat m = s.mats[vL.mat];
+                fsL = mk3(m.Le[0], m.Le[1], m.Le[2]);
+                pLs = LMX_INV_PI_F * G;
+            }
+        } else {
+            const RMat m = s.mats[vL.mat];
+            if (is_scatterer(m)) {
+                const f3 wiL = dir_to(vL.p, vLp.p);
+                fsL = bsdf_eval_t(m, vL.fr, wiL, dLE, true);
+                pLs = bsdf_pdf(m, vL.fr, wiL, dLE) * G;
+                pEsm2 = bsdf_pdf(m, vL.fr, dLE, wiL) * gterm(vL, vLp);
+            }
+        }
+        // eye side
+        if (t == 1) {
+            const float imp = importance(s, dEL);
+            fsE = mk3(imp, imp, imp);
+            pEsm1 = imp * G;
+        } else {
+            const RMat m = s.mats[vE.mat];
+            if (is_scatterer(m)) {
+                const f3 wiE = dir_to(vE.p, vEp.p);
+                fsE = bsdf_eval_t(m, vE.fr, wiE, dEL, false);
+                pEsm1 = bsdf_pdf(m, vE.fr, wiE, dEL) * G;
+                pLsp1 = bsdf_pdf(m, vE.fr, dEL, wiE) * gterm(vE, vEp);
+            }
+        }
+        const f3 Cs = mk3(vL.alpha.x * fsL.x * G * fsE.x * vE.alpha.x,
+                          vL.alpha.y * fsL.y * G * fsE.y * vE.alpha.y,
+                          vL.alpha.z * fsL.z * G * fsE.z * vE.alpha.z);
+        bool ok = !(Cs.x == 0.0f && Cs.y == 0.0f && Cs.z == 0.0f);
+        if (ok && t == 1) {
+            ok = camera_raster(s, dEL, rx, ry);
+            pixel = raster_pixel(s, rx, ry);
+        } else {
+            rx = eras.x; ry = eras.y;
+        }
+        // power heuristic from the area pdfs of both sides: the two
+        // nearest vertices of each side from the junction pdfs, the
+        // rest from the running sums U
+        if (vL.pf == 0.0f || (sv >= 2 && (vLp.pf == 0.0f || vL.U < 0.0f))) ok = false;
+        if (t >= 2 && (vE.pf == 0.0f || (t >= 3 && (vEp.pf == 0.0f || vE.U < 0.0f)))) ok = false;
+        float sum = 1.0f;
+        if (ok) {
+            sum += chain_sq(pEsm1 / vL.pf, sv >= 2 ? pEsm2 / vLp.pf : 0.0f, vL.U);
+            if (t >= 2) sum += chain_sq(pLs / vE.pf, t >= 3 ? pLsp1 / vEp.pf : 0.0f, vE.U);
+        }
+        if (ok) {
+            w = 1.0f / sum;
+            C = mk3(Cs.x * w, Cs.y * w, Cs.z * w);
+            // Scene3::Visible (scene3.h:107-116)
+            f3 d = sub(vE.p, vL.p);
+            const float l = sqrtf(dot(d, d));
+            d = mk3(d.x / l, d.y / l, d.z / l);
+            r0 = make_float4(vL.p.x, vL.p.y, vL.p.z, LMX_EPS_ISECT);
+            r1 = make_float4(d.x, d.y, d.z, l * (1.0f - LMX_EPS_ISECT));
+            emit = true;
+        }
+        if (emit) {
+            sq.ray[2 * j] = r0;
+            sq.ray[2 * j + 1] = r1;
+            sq.c[j] = make_float4(C.x, C.y, C.z, w);
+            sq.aux[j] = make_float4(rx, ry, 0.0f, 0.0f);
+            sq.meta[j] = make_uint4(sample, (uint32_t)sv, (uint32_t)t, pixel);
+        } else {
+            sq.ray[2 * j] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+            sq.ray[2 * j + 1] = make_float4(0.0f, 0.0f, 1.0f, -1.0f);
+            sq.meta[j] = make_uint4(sample, 0u, (uint32_t)t, 0u);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bd_splat(BParams p, BCtl* __restrict__ ctl, ShadowQ sq,
                                                   const float4* __restrict__ hits)
 {
     const uint32_t n = ctl->n_shadow;
+    uint32_t live = 0;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint4 m = sq.meta[j];
+        if (m.y == 0) continue;                                      // no contribution
+        live++;
         if (__float_as_uint(hits[j].w) != WBVH_MISS) continue;      // occluded
         const float4 c = sq.c[j];
-        const uint4 m = sq.meta[j];
         const float4 a = sq.aux[j];
         splat(p, m.w, mk3(c.x, c.y, c.z));
         record(p, m.x, (int)m.y, (int)m.z, a.x, a.y, mk3(c.x, c.y, c.z), c.w);
     }
+    // live shadow rays, one atomic per wave
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) live += __shfl_xor(live, off);
+    if ((threadIdx.x & 63u) == 0 && live) atomicAdd(&ctl->shadow_rays, (unsigned long long)live);
 }
 
 __global__ void k_bd_next_batch(BCtl* ctl, uint32_t batch)
 {
     if (threadIdx.x == 0) {
-        ctl->shadow_rays += ctl->n_shadow;
         ctl->batch_first += batch;
         ctl->n_trace = 0; ctl->n_next = 0; ctl->n_shadow = 0;
     }
@@ -564,9 +603,23 @@ __global__ void k_bd_scale(float* a, size_t n, float w)
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] *= w;
 }
 
+// Device bytes of one batch slot: two subpaths of vertex records, their
+// queues and hits, and the shadow-ray records of every connection.
+size_t bd_bytes_per_sample(uint32_t maxv)
+{
+    const size_t conn = (size_t)(maxv * (maxv - 1) / 2 + 1);
+    return 2 * ((size_t)maxv * 64 + 4 + 4 * 16 + 2 * 16 + 2 * 16) + 4 + 8 + 2 * 16 + conn * (2 * 16 + 4 * 16);
+}
+
+// Buffers for batches of B samples of up to maxv vertices; kept when the
+// current ones are large enough (the layouts only need B maxv and B conn(maxv)
+// slots).
 int bd_buffers(lmx_scene* sc, uint32_t B, uint32_t maxv)
 {
-    if (sc->bd_batch == B && sc->bd_maxv == maxv) return LMX_OK;
+    if (B <= sc->bd_batch && maxv <= sc->bd_maxv) return LMX_OK;
+    B = std::max(B, sc->bd_batch);
+    maxv = std::max(maxv, sc->bd_maxv);
+    sc->bd_batch = sc->bd_maxv = 0;
     hipError_t e;
     const size_t nv = (size_t)B * maxv * 4;
     const size_t nconn = (size_t)B * (size_t)(maxv * (maxv - 1) / 2 + 1);
@@ -609,14 +662,13 @@ int bd_enqueue_batch(lmx_scene* sc, const BParams& bp, hipStream_t st)
         int rc = launch_trace(sc->accel, reinterpret_cast<const WRay*>(q[a].ray), 0, &ctl->n_trace, 2 * B,
                               reinterpret_cast<WHit*>(sc->bd_hits.p), st, false);
         if (rc != LMX_OK) return rc;
-        const int grid = std::max(1, std::min((int)((2 * B + 255) / 256), cap_grid));
-        k_bd_shade<<<grid, 256, 0, st>>>(sc->rs, bp, ctl, q[a], sc->bd_hits.p, q[1 - a], vb);
+        k_bd_shade<<<(2 * B + 255) / 256, 256, 0, st>>>(sc->rs, bp, ctl, q[a], sc->bd_hits.p, q[1 - a], vb);
         k_bd_commit<<<1, 64, 0, st>>>(ctl);
         a = 1 - a;
     }
     ShadowQ sq{sc->bd_sray.p, sc->bd_sc.p, sc->bd_saux.p, sc->bd_smeta.p};
-    const uint32_t nthreads = B * (uint32_t)bp.maxv;
-    k_bd_connect<<<(nthreads + 255) / 256, 256, 0, st>>>(sc->rs, bp, ctl, vb, sq);
+    k_bd_plan<<<(B + 255) / 256, 256, 0, st>>>(sc->rs, bp, ctl, vb, sq);
+    k_bd_connect<<<cap_grid, 256, 0, st>>>(sc->rs, bp, ctl, vb, sq);
     const uint32_t nconn = B * (uint32_t)(bp.maxv * (bp.maxv - 1) / 2 + 1);
     int rc = launch_trace(sc->accel, reinterpret_cast<const WRay*>(sq.ray), 0, &ctl->n_shadow, nconn,
                           reinterpret_cast<WHit*>(sc->bd_shits.p), st, true);
@@ -625,6 +677,72 @@ int bd_enqueue_batch(lmx_scene* sc, const BParams& bp, hipStream_t st)
     k_bd_next_batch<<<1, 64, 0, st>>>(ctl, B);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? LMX_OK : hip_error(e, "bdpt launch");
+}
+
+// Captures one batch as a graph.
+int bd_graph(lmx_scene* sc, const BParams& bp, hipStream_t st, hipGraphExec_t* exec)
+{
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_error(e, "capture");
+    int rc = bd_enqueue_batch(sc, bp, st);
+    e = hipStreamEndCapture(st, &graph);
+    if (rc == LMX_OK && e != hipSuccess) rc = hip_error(e, "capture end");
+    if (rc == LMX_OK && (e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0)) != hipSuccess)
+        rc = hip_error(e, "graph instantiate");
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+}
+
+// Batch-size autotuner: times one batch of each candidate size on this scene
+// (after one untimed batch that warms the graph and the caches) and keeps the
+// size with the highest samples/s for this max_num_vertices.  Candidates are
+// bounded by a quarter of the free device memory.  The probe batches splat
+// into `film`, which the caller clears afterwards.
+int bd_autotune(lmx_scene* sc, BParams bp, float* film, hipStream_t st)
+{
+    static const uint32_t cands[] = {1u << 19, 1u << 20, 1u << 21, 1u << 22};
+    size_t free_b = 0, total_b = 0;
+    hipError_t e = hipMemGetInfo(&free_b, &total_b);
+    if (e != hipSuccess) return hip_error(e, "bdpt autotune: memory info");
+    const size_t budget = (free_b + (size_t)sc->bd_batch * bd_bytes_per_sample(sc->bd_maxv)) / 4;
+    const size_t per = bd_bytes_per_sample((uint32_t)bp.maxv);
+    bp.sample_out = nullptr;
+    bp.table = nullptr;                        // probes draw from the generator
+    bp.film = film;
+    hipEvent_t e0, e1;
+    if ((e = hipEventCreate(&e0)) != hipSuccess) return hip_error(e, "autotune events");
+    if ((e = hipEventCreate(&e1)) != hipSuccess) { (void)hipEventDestroy(e0); return hip_error(e, "autotune events"); }
+    uint32_t best = cands[0];
+    double best_rate = -1.0;
+    int rc = LMX_OK;
+    for (uint32_t B : cands) {
+        if (B != cands[0] && (size_t)B * per > budget) break;
+        bp.batch = B;
+        bp.total = 2u * B;
+        if ((rc = bd_buffers(sc, B, (uint32_t)bp.maxv)) != LMX_OK) break;
+        hipGraphExec_t exec = nullptr;
+        if ((rc = bd_graph(sc, bp, st, &exec)) != LMX_OK) break;
+        k_bd_init<<<1, 64, 0, st>>>(sc->bd_ctl.p);
+        (void)hipGraphLaunch(exec, st);
+        (void)hipEventRecord(e0, st);
+        e = hipGraphLaunch(exec, st);
+        (void)hipEventRecord(e1, st);
+        const hipError_t es = hipEventSynchronize(e1);
+        (void)hipGraphExecDestroy(exec);
+        if (e != hipSuccess || es != hipSuccess) { rc = hip_error(e != hipSuccess ? e : es, "bdpt autotune"); break; }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        const double rate = B / (double)std::max(ms, 1e-3f);
+        if (rate > best_rate) { best_rate = rate; best = B; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == LMX_OK) {
+        sc->bd_tuned_batch = best;
+        sc->bd_tuned_maxv = (uint32_t)bp.maxv;
+    }
+    return rc;
 }
 
 }  // namespace
@@ -638,6 +756,8 @@ int lmx_render_bdpt(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, 
         return set_error(LMX_EINVAL, "lmx_render_bdpt: num_samples must be in [1, 2^32)");
     if (p->max_num_vertices < 2 || p->max_num_vertices > 32)
         return set_error(LMX_EINVAL, "lmx_render_bdpt: max_num_vertices must be in [2, 32]");
+    if (sc->mats.n > VB_MAX_MATERIALS || sc->lights.n > VB_MAX_LIGHTS)
+        return set_error(LMX_EINVAL, "lmx_render_bdpt: at most 65536 materials and 8190 lights");
     const uint32_t maxv = (uint32_t)p->max_num_vertices;
     const uint32_t wsub = LMX_PT_SLOT_SEG0 + 4u * (maxv - 1u);
     if (p->uniform_table && p->table_width < 2 * wsub)
@@ -652,9 +772,6 @@ int lmx_render_bdpt(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, 
         st = own;
     }
     const auto t0 = std::chrono::steady_clock::now();
-    uint32_t B = p->queue_capacity ? p->queue_capacity : (1u << 20);
-    B = (uint32_t)std::min<uint64_t>(B, p->num_samples);
-    int rc = bd_buffers(sc, B, maxv);
     BParams bp;
     bp.maxv = (int)maxv; bp.minv = p->min_num_vertices;
     bp.key = make_uint2((uint32_t)p->seed, (uint32_t)(p->seed >> 32));
@@ -662,24 +779,29 @@ int lmx_render_bdpt(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, 
     bp.raster_mode = p->raster_mode;
     bp.table = p->uniform_table; bp.table_width = p->table_width;
     bp.wsub = wsub;
-    bp.batch = B;
     bp.sample_out = p->sample_out;
     bp.film = film_rgb;
+    int rc = LMX_OK;
+    // batch size: the caller's, else the autotuned one (tuned once per scene
+    // and max_num_vertices, for renders of more than the smallest candidate)
+    uint32_t B = p->queue_capacity;
+    if (!B) {
+        if (p->num_samples <= (1u << 19)) B = (uint32_t)p->num_samples;
+        else {
+            if (sc->bd_tuned_maxv != maxv) rc = bd_autotune(sc, bp, film_rgb, st);
+            B = sc->bd_tuned_batch;
+        }
+    }
+    B = (uint32_t)std::min<uint64_t>(B, p->num_samples);
+    bp.batch = B;
+    if (rc == LMX_OK) rc = bd_buffers(sc, B, maxv);
     const size_t npix = (size_t)sc->width * (size_t)sc->height;
     const int zgrid = std::max(1, std::min((int)((3 * npix + 255) / 256), sc->cus * 8));
-    hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     if (rc == LMX_OK) {
         k_bd_zero<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix);
         k_bd_init<<<1, 64, 0, st>>>(sc->bd_ctl.p);
-        if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess) rc = hip_error(e, "capture");
-    }
-    if (rc == LMX_OK) {
-        rc = bd_enqueue_batch(sc, bp, st);
-        e = hipStreamEndCapture(st, &graph);
-        if (rc == LMX_OK && e != hipSuccess) rc = hip_error(e, "capture end");
-        if (rc == LMX_OK && (e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0)) != hipSuccess)
-            rc = hip_error(e, "graph instantiate");
+        rc = bd_graph(sc, bp, st, &exec);
     }
     const uint64_t nbatch = (p->num_samples + B - 1) / B;
     for (uint64_t b = 0; rc == LMX_OK && b < nbatch; b++)
@@ -691,7 +813,6 @@ int lmx_render_bdpt(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, 
             rc = hip_error(e, "lmx_render_bdpt: sync");
     }
     if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
     if (own) (void)hipStreamDestroy(own);
     if (stats) {
         stats->rays = rc == LMX_OK ? sc->bd_hctl->rays + sc->bd_hctl->shadow_rays : 0;

@@ -56,6 +56,7 @@ struct RParams {
 __global__ __launch_bounds__(256) void k_shade(RScene s, RParams p, Ctl* __restrict__ ctl, Queue in,
                                                const float4* __restrict__ hits, Queue out)
 {
+    __shared__ uint32_t lds[5];
     const uint32_t n = ctl->n_trace;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -118,7 +119,7 @@ __global__ __launch_bounds__(256) void k_shade(RScene s, RParams p, Ctl* __restr
                 }
             }
         }
-        const uint32_t slot = wave_append(&ctl->n_next, cont);
+        const uint32_t slot = block_reserve<4>(&ctl->n_next, cont ? 1u : 0u, lds);
         if (cont) {
             out.ray[2 * slot] = o_ray0;
             out.ray[2 * slot + 1] = o_ray1;

@@ -233,19 +233,31 @@ RD float importance(const RScene& s, f3 wo)
     return inv * inv * inv / A;
 }
 
-// Appends `take` lanes of this wave to a queue with one atomic per wave;
-// returns each taking lane's slot.
-RD uint32_t wave_append(uint32_t* counter, bool take)
+// Reserves `count` consecutive queue slots per thread of a workgroup of NW
+// waves with one atomic per workgroup: exclusive prefix sum over each wave,
+// the wave totals combined in LDS, their sum added to the counter by thread 0.
+// Every thread of the workgroup must call it; `lds` holds NW + 1 words.
+template <int NW>
+RD uint32_t block_reserve(uint32_t* counter, uint32_t count, uint32_t* lds)
 {
-    const uint64_t m = __ballot(take);
-    if (m == 0) return 0;                      // wave-uniform
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-    uint32_t base = 0;
-    const uint32_t first = (uint32_t)(__ffsll((unsigned long long)m) - 1);
-    if (lane == first) base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = __shfl(base, (int)first);
-    return base + rank;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t incl = count;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off);
+        if (lane >= (uint32_t)off) incl += v;
+    }
+    if (lane == 63u) lds[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < NW; w++) { const uint32_t c = lds[w]; lds[w] = tot; tot += c; }
+        lds[NW] = tot ? atomicAdd(counter, tot) : 0u;
+    }
+    __syncthreads();
+    const uint32_t r = lds[NW] + lds[wave] + incl - count;
+    __syncthreads();
+    return r;
 }
 
 template <typename T>
@@ -273,10 +285,10 @@ using lmx::RScene;
 struct BCtl {
     uint32_t n_trace;          // subpath rays queued for the next trace
     uint32_t n_next;           // continuations appended by the shade step
-    uint32_t n_shadow;         // connection (shadow) rays of the batch
+    uint32_t n_shadow;         // shadow-queue slots of the batch (live and dead)
     uint32_t batch_first;      // first sample of the current batch
     unsigned long long rays;   // extension rays traced
-    unsigned long long shadow_rays;
+    unsigned long long shadow_rays;   // live connection (shadow) rays traced
 };
 
 // The scene object behind the C ABI: accel, shading data on the device and
@@ -302,6 +314,7 @@ struct lmx_scene {
     uint32_t tuned_cap = 0;
     // bidirectional sampler buffers (sized for bd_batch samples, bd_maxv vertices)
     uint32_t bd_batch = 0, bd_maxv = 0;
+    uint32_t bd_tuned_batch = 0, bd_tuned_maxv = 0;   // autotuned batch for one max_num_vertices
     DevBuf<float4> bd_vert[2];          // per subpath vertex: 4 float4 (see lmx_bdpt.hip)
     DevBuf<uint32_t> bd_nsub[2];
     DevBuf<uint32_t> bd_epix;
