@@ -37,17 +37,6 @@ using namespace lmx;
 
 namespace {
 
-struct RParams {
-    int32_t max_nv, min_nv, rr_depth;
-    float rr_prob;
-    uint2 key;
-    uint32_t total;
-    int32_t raster_mode;
-    const float* table;
-    uint32_t table_width;
-    float* sample_out;         // 8 floats per sample, or null
-    float* film;               // W x H x 3 accumulation
-};
 
 
 // ---------------------------------------------------------------------------
@@ -232,15 +221,20 @@ int enqueue_iteration(lmx_scene* sc, const RParams& rp, int a, int b, hipStream_
                           reinterpret_cast<WHit*>(sc->hits.p), st);
     if (rc != LMX_OK) return rc;
     const int grid = std::max(1, std::min((int)((sc->cap + 255) / 256), sc->cus * 8));
-    k_shade<<<grid, 256, 0, st>>>(sc->rs, rp, sc->ctl.p, qa, sc->hits.p, qb);
+    if (rp.guide) {
+        if ((rc = launch_guided_shade(sc, rp, qa, qb, st)) != LMX_OK) return rc;
+    } else {
+        k_shade<<<grid, 256, 0, st>>>(sc->rs, rp, sc->ctl.p, qa, sc->hits.p, qb);
+    }
     k_gen<<<grid, 256, 0, st>>>(sc->rs, rp, sc->ctl.p, sc->cap, qb);
     k_commit<<<1, 64, 0, st>>>(sc->ctl.p, sc->cap, rp.total);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? LMX_OK : hip_error(e, "wavefront launch");
 }
 
-int render_with_capacity(lmx_scene* sc, const RParams& rp, uint32_t cap, hipStream_t st, uint64_t* rays,
-                         uint32_t* iters)
+}  // namespace
+
+int wavefront_render(lmx_scene* sc, const RParams& rp, uint32_t cap, hipStream_t st, uint64_t* rays, uint32_t* iters)
 {
     int rc = ensure_queues(sc, cap);
     if (rc != LMX_OK) return rc;
@@ -288,6 +282,8 @@ int render_with_capacity(lmx_scene* sc, const RParams& rp, uint32_t cap, hipStre
     return rc;
 }
 
+namespace {
+
 // Queue-capacity autotuner: renders the same prefix of samples with each
 // candidate capacity (paths in flight) and keeps the fastest in samples per
 // second.  More paths in flight fill the GPU better but cost memory and make
@@ -309,9 +305,9 @@ int autotune_capacity(lmx_scene* sc, const RParams& rp, hipStream_t st)
     int rc = LMX_OK;
     for (uint32_t cap : cands) {
         if (cap > 4 * (uint64_t)probe.total && cap != cands[0]) break;
-        if ((rc = render_with_capacity(sc, probe, cap, st, nullptr, nullptr)) != LMX_OK) break;   // warm
+        if ((rc = wavefront_render(sc, probe, cap, st, nullptr, nullptr)) != LMX_OK) break;   // warm
         (void)hipEventRecord(e0, st);
-        if ((rc = render_with_capacity(sc, probe, cap, st, nullptr, nullptr)) != LMX_OK) break;
+        if ((rc = wavefront_render(sc, probe, cap, st, nullptr, nullptr)) != LMX_OK) break;
         (void)hipEventRecord(e1, st);
         (void)hipEventSynchronize(e1);
         float ms = 0.0f;
@@ -419,6 +415,12 @@ int lmx_scene_create(const lmx_scene_desc* d, lmx_scene** out)
     rs.aspect = (float)c.width / (float)c.height;
     rs.width = c.width; rs.height = c.height;
     sc->width = c.width; sc->height = c.height;
+    for (int a = 0; a < 3; a++) { sc->bounds[a] = 3.4e38f; sc->bounds[3 + a] = -3.4e38f; }
+    for (uint32_t v = 0; v < d->nvert; v++)
+        for (int a = 0; a < 3; a++) {
+            sc->bounds[a] = std::min(sc->bounds[a], d->verts[3 * v + a]);
+            sc->bounds[3 + a] = std::max(sc->bounds[3 + a], d->verts[3 * v + a]);
+        }
     *out = sc.release();
     return LMX_OK;
 }
@@ -457,6 +459,7 @@ int lmx_render_pt(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, vo
     rp.table = p->uniform_table; rp.table_width = p->table_width;
     rp.sample_out = p->sample_out;
     rp.film = film_rgb;
+    rp.guide = nullptr;
     const int zgrid = std::max(1, std::min((int)((3 * npix + 255) / 256), sc->cus * 8));
     int rc = LMX_OK;
     if (p->queue_capacity == 0 && sc->tuned_cap == 0) rc = autotune_capacity(sc, rp, st);
@@ -466,7 +469,7 @@ int lmx_render_pt(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, vo
     uint32_t iters = 0;
     if (rc == LMX_OK) {
         k_zero<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix);
-        rc = render_with_capacity(sc, rp, cap, st, &rays, &iters);
+        rc = wavefront_render(sc, rp, cap, st, &rays, &iters);
     }
     if (rc == LMX_OK) {
         k_scale<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix, (float)((double)npix / (double)p->num_samples));

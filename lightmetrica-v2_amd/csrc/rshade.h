@@ -62,6 +62,22 @@ struct Queue {
 
 #define META_DEAD 1u
 
+struct GuideLaunch;            // the learned sampler's state for a shading launch
+
+// Parameters of the unidirectional wavefront sampler (plain and guided).
+struct RParams {
+    int32_t max_nv, min_nv, rr_depth;
+    float rr_prob;
+    uint2 key;
+    uint32_t total;
+    int32_t raster_mode;
+    const float* table;
+    uint32_t table_width;
+    float* sample_out;         // 8 floats per sample, or null
+    float* film;               // W x H x 3 accumulation
+    const GuideLaunch* guide;  // null: BSDF sampling only
+};
+
 template <class P>
 struct Uniforms {
     const P& p;
@@ -233,6 +249,19 @@ RD float importance(const RScene& s, f3 wo)
     return inv * inv * inv / A;
 }
 
+}  // namespace lmx
+
+struct lmx_scene;
+// The unidirectional wavefront loop (lmx_render.hip): generation, trace,
+// shade (guided when rp.guide is set), graph-replayed until every sample is
+// done.  `cap` paths in flight.
+int wavefront_render(lmx_scene* sc, const lmx::RParams& rp, uint32_t cap, hipStream_t st, uint64_t* rays,
+                     uint32_t* iters);
+// The guided shading step of one iteration (lmx_guide.hip).
+int launch_guided_shade(lmx_scene* sc, const lmx::RParams& rp, lmx::Queue in, lmx::Queue out, hipStream_t st);
+
+namespace lmx {
+
 // Reserves `count` consecutive queue slots per thread of a workgroup of NW
 // waves with one atomic per workgroup: exclusive prefix sum over each wave,
 // the wave totals combined in LDS, their sum added to the counter by thread 0.
@@ -303,6 +332,7 @@ struct lmx_scene {
     DevBuf<float> light_cdf;
     RScene rs{};
     int width = 0, height = 0;
+    float bounds[6] = {0, 0, 0, 0, 0, 0};   // scene AABB: min xyz, max xyz
     int cus = 0;
     // wavefront buffers, reused across renders of the same capacity
     uint32_t cap = 0;

@@ -70,6 +70,18 @@ class lmx_render_stats(ctypes.Structure):
 
 
 # Every symbol include/lmx.h declares, with its ctypes signature.
+class lmx_guide_opts(ctypes.Structure):
+    _fields_ = [("lr", ctypes.c_float), ("beta1", ctypes.c_float), ("beta2", ctypes.c_float),
+                ("eps", ctypes.c_float), ("batch", ctypes.c_uint32), ("steps_per_pass", ctypes.c_uint32),
+                ("wclamp", ctypes.c_float), ("seed", ctypes.c_uint64)]
+
+
+class lmx_guide_stats(ctypes.Structure):
+    _fields_ = [("records", ctypes.c_uint32), ("records_nonzero", ctypes.c_uint32), ("steps", ctypes.c_uint32),
+                ("loss", ctypes.c_float),
+                ("render_ms", ctypes.c_double), ("train_ms", ctypes.c_double)]
+
+
 SIGNATURES = {
     "lmx_last_error": (ctypes.c_char_p, []),
     "lmx_accel_build": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32,
@@ -89,6 +101,20 @@ SIGNATURES = {
                                      ctypes.c_void_p, ctypes.POINTER(lmx_render_stats)]),
     "lmx_render_bdpt": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(lmx_render_params), ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.POINTER(lmx_render_stats)]),
+    "lmx_guide_default_opts": (None, [ctypes.POINTER(lmx_guide_opts)]),
+    "lmx_guide_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(lmx_guide_opts),
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    "lmx_guide_free": (None, [ctypes.c_void_p]),
+    "lmx_guide_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    "lmx_render_guided": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(lmx_render_params),
+                                         ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.POINTER(lmx_guide_stats), ctypes.POINTER(lmx_render_stats)]),
+    "lmx_guide_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                      ctypes.c_void_p]),
+    "lmx_guide_grad": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "lmx_guide_mix_sample": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                            ctypes.c_void_p]),
     "lmx_accel_export": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "lmx_accel_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                        ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),

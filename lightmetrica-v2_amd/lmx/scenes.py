@@ -344,3 +344,66 @@ def sphere_grid(width=1024, height=1024, grid=16, subdiv=5, seed=7):
     return dict(verts=V, tris=F, normals=normals, tri_mat=M, materials=mats, lights=lights,
                 camera=dict(eye=(0.0, 7.5, 15.5), center=(0.0, 0.0, 0.5), up=(0.0, 1.0, 0.0), fov_deg=45.0,
                             width=width, height=height))
+
+
+def occluded_room(width=1024, height=1024, n_spheres=96, subdiv=5, seed=11):
+    """Path-guiding benchmark scene: a room of 12 quads whose only emitter
+    sits in a side compartment behind a full-height dividing wall with a
+    0.1-wide gap, so light reaches the main room only through the gap; the
+    main room holds n_spheres displaced icospheres (subdiv 5: 20,480
+    triangles each, 1,966,080 for 96) of which 40% are normalized Phong with
+    exponent 200 and the rest Lambertian.  Quads: floor, ceiling, 4 walls
+    (inward), the divider's two pieces with a face on each side (4), the
+    emitter (facing the gap) and a shelf in the compartment."""
+    rng = np.random.default_rng(seed)
+    X0, X1, XD, Y1, Z = -1.5, 1.5, 0.9, 2.0, 1.5
+    gap = 0.05
+    mats = [dict(type=MAT_DIFFUSE, R=(0.7, 0.7, 0.7), exponent=0.0, Le=(0, 0, 0)),     # 0 walls
+            dict(type=MAT_DIFFUSE, R=(0.6, 0.55, 0.45), exponent=0.0, Le=(0, 0, 0)),   # 1 divider
+            dict(type=MAT_NULL, R=(0, 0, 0), exponent=0.0, Le=(60.0, 52.0, 44.0))]     # 2 emitter
+    V, T, M, N = [], [], [], []
+
+    def add(vs, ts, m):
+        base = len(V)
+        V.extend(vs)
+        T.extend([[base + i for i in t] for t in ts])
+        M.extend([m] * len(ts))
+        n = np.cross(np.subtract(vs[1], vs[0]), np.subtract(vs[2], vs[0]))
+        N.extend([list(n / np.linalg.norm(n))] * len(vs))
+    add(*_quad([X0, 0, Z], [X1, 0, Z], [X1, 0, -Z], [X0, 0, -Z]), 0)           # floor, +y
+    add(*_quad([X0, Y1, -Z], [X1, Y1, -Z], [X1, Y1, Z], [X0, Y1, Z]), 0)       # ceiling, -y
+    add(*_quad([X0, 0, -Z], [X1, 0, -Z], [X1, Y1, -Z], [X0, Y1, -Z]), 0)       # back, +z
+    add(*_quad([X1, 0, Z], [X0, 0, Z], [X0, Y1, Z], [X1, Y1, Z]), 0)           # front, -z
+    add(*_quad([X0, 0, Z], [X0, 0, -Z], [X0, Y1, -Z], [X0, Y1, Z]), 0)         # left, +x
+    add(*_quad([X1, 0, -Z], [X1, 0, Z], [X1, Y1, Z], [X1, Y1, -Z]), 0)         # right, -x
+    for z0, z1 in ((-Z, -gap), (gap, Z)):                                      # divider pieces
+        add(*_quad([XD, 0, z0], [XD, 0, z1], [XD, Y1, z1], [XD, Y1, z0]), 1)   # main-room face, -x
+        add(*_quad([XD, 0, z1], [XD, 0, z0], [XD, Y1, z0], [XD, Y1, z1]), 1)   # compartment face, +x
+    first_light = len(T)
+    xl = X1 - 0.001
+    add(*_quad([xl, 0.8, -0.25], [xl, 0.8, 0.25], [xl, 1.3, 0.25], [xl, 1.3, -0.25]), 2)   # emitter, -x
+    add(*_quad([XD, 0.5, Z], [X1, 0.5, Z], [X1, 0.5, -Z], [XD, 0.5, -Z]), 0)   # shelf, +y
+    base_v, base_f = icosphere(subdiv)
+    nv = base_v.shape[0]
+    Vs, Fs, Ms, Ns = [np.array(V, np.float32)], [np.array(T, np.uint32)], [np.array(M, np.uint32)], [np.array(N, np.float32)]
+    off = len(V)
+    for k in range(n_spheres):
+        r = rng.uniform(0.08, 0.16)
+        c = np.array([rng.uniform(X0 + 0.2, XD - 0.2), rng.uniform(r, 1.4), rng.uniform(-Z + 0.2, Z - 0.2)])
+        noise = value_noise(base_v, seed * 1000 + k)
+        v = (base_v * (r * (1.0 + 0.1 * noise))[:, None] + c).astype(np.float32)
+        Vs.append(v)
+        Fs.append((base_f + off).astype(np.uint32))
+        Ns.append(vertex_normals(v, base_f))
+        if rng.random() < 0.4:
+            mats.append(dict(type=MAT_PHONG, R=tuple(float(x) for x in rng.uniform(0.5, 0.9, 3)),
+                             exponent=200.0, Le=(0, 0, 0)))
+        else:
+            mats.append(dict(type=MAT_DIFFUSE, R=tuple(float(x) for x in rng.uniform(0.2, 0.8, 3)),
+                             exponent=0.0, Le=(0, 0, 0)))
+        Ms.append(np.full(base_f.shape[0], len(mats) - 1, np.uint32))
+        off += nv
+    return dict(verts=np.concatenate(Vs), tris=np.concatenate(Fs), normals=np.concatenate(Ns),
+                tri_mat=np.concatenate(Ms), materials=mats, lights=[(first_light, 2)],
+                camera=dict(eye=(-1.35, 1.1, 1.2), center=(0.9, 0.55, -0.3), up=(0.0, 1.0, 0.0), fov_deg=62.0,
+                            width=width, height=height))

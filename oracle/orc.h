@@ -190,6 +190,48 @@ int orc_bsdf_query(const orc_material *m, const float n[3], const float wi[3], c
 void orc_bsdf_eval(const orc_material *m, const float n[3], const float wi[3], const float wo[3],
                    float *pdf, float f[3]);
 
+/* Learned directional sampler (no counterpart in the reference; orc_guide.c).
+ * FP64 restatement of the MLP, the vMF mixture and the training loss:
+ *   input x[7] in [0,1]: position (scene bounds), normal (theta/pi,
+ *   phi/2pi), direction to the previous vertex (theta/pi, phi/2pi);
+ *   features sin/cos(2^f pi x_d), f < 12, index d*24 + 2f (+1 for cos);
+ *   168 -> 64 -> 64 -> 64 -> 64 (ReLU) -> 41;
+ *   y[3k..3k+2] lobe mean (normalised), y[24+k] log kappa (clamped to
+ *   [-4, 9]), y[32+k] lobe weight logits (softmax), y[40] BSDF selection
+ *   probability 0.1 + 0.8 sigmoid(y40);
+ *   parameters (canonical flat layout, W[k][n] = input k -> output n):
+ *   W1[169][64] (row 168 = bias), W2[64][64], b2[64], W3, b3, W4, b4,
+ *   W5[64][41], b5[41] -- ORC_GUIDE_NPARAM values.
+ * bf16 != 0 rounds weights (except the biases of layers 2-5) and every
+ * layer's input activations to bfloat16 (round to nearest even), as the
+ * device kernels do; accumulation stays FP64.
+ * Pinned by: vMF normalisation and sampling in closed form (Jakob 2012),
+ * central finite differences of the loss, and the Adam update formula. */
+#define ORC_GUIDE_NIN 7
+#define ORC_GUIDE_NFEAT 168
+#define ORC_GUIDE_H 64
+#define ORC_GUIDE_NOUT 41
+#define ORC_GUIDE_NLOBE 8
+#define ORC_GUIDE_NPARAM 25961
+void orc_guide_features(const double x[7], double feat[168]);
+void orc_guide_forward(const double *params, const double x[7], double y[41], int bf16);
+void orc_guide_forward_batch(const double *params, const double *x, uint64_t n, double *y, int bf16);
+double orc_guide_vmf_pdf(const double mu[3], double kappa, const double w[3]);
+void orc_guide_vmf_sample(const double mu[3], double kappa, double u1, double u2, double out[3]);
+/* vMF mixture of the outputs y: pdf of direction w (solid angle), and the
+ * BSDF selection probability */
+double orc_guide_mix_pdf(const double y[41], const double w[3], double *alpha);
+/* a direction from the mixture: lobe by u (cumulative weights), then the lobe
+ * sampled with (u1, u2); returns the lobe */
+int orc_guide_mix_sample(const double y[41], double u, double u1, double u2, double out[3]);
+/* mean over n records of -w log(alpha pb + (1 - alpha) p_vmf(dir)); grad
+ * (ORC_GUIDE_NPARAM, may be NULL) receives its gradient */
+double orc_guide_loss(const double *params, uint64_t n, const double *x, const double *dir, const double *pb,
+                      const double *w, double *grad, int bf16);
+void orc_guide_adam(double *params, double *m, double *v, const double *grad, uint64_t n, int step, double lr,
+                    double beta1, double beta2, double eps);
+double orc_bf16_round(double x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -338,3 +338,101 @@ class Scene:
         s = np.ascontiguousarray(samples, np.float32)
         lib().orc_film_splat(_p(s), s.shape[0], self.width, self.height, _p(f))
         return f
+
+
+# ---------------------------------------------------------------------------
+# learned directional sampler (orc_guide.c)
+
+GUIDE_NPARAM = 25961
+GUIDE_NOUT = 41
+_guide_ready = False
+
+
+def _glib():
+    global _guide_ready
+    so = lib()
+    if not _guide_ready:
+        vp, u64, i32, f64 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_double
+        so.orc_guide_features.argtypes = [vp, vp]
+        so.orc_guide_features.restype = None
+        so.orc_guide_forward_batch.argtypes = [vp, vp, u64, vp, i32]
+        so.orc_guide_forward_batch.restype = None
+        so.orc_guide_vmf_pdf.argtypes = [vp, f64, vp]
+        so.orc_guide_vmf_pdf.restype = f64
+        so.orc_guide_vmf_sample.argtypes = [vp, f64, f64, f64, vp]
+        so.orc_guide_vmf_sample.restype = None
+        so.orc_guide_mix_pdf.argtypes = [vp, vp, vp]
+        so.orc_guide_mix_pdf.restype = f64
+        so.orc_guide_mix_sample.argtypes = [vp, f64, f64, f64, vp]
+        so.orc_guide_mix_sample.restype = i32
+        so.orc_guide_loss.argtypes = [vp, u64, vp, vp, vp, vp, vp, i32]
+        so.orc_guide_loss.restype = f64
+        so.orc_guide_adam.argtypes = [vp, vp, vp, vp, u64, i32, f64, f64, f64, f64]
+        so.orc_guide_adam.restype = None
+        so.orc_bf16_round.argtypes = [f64]
+        so.orc_bf16_round.restype = f64
+        _guide_ready = True
+    return so
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def guide_features(x):
+    x = _d(x)
+    f = np.zeros(168)
+    _glib().orc_guide_features(_p(x), _p(f))
+    return f
+
+
+def guide_forward(params, x, bf16=False):
+    """(n, 7) inputs -> (n, 41) outputs."""
+    params, x = _d(params), _d(np.atleast_2d(x))
+    y = np.zeros((x.shape[0], GUIDE_NOUT))
+    _glib().orc_guide_forward_batch(_p(params), _p(x), x.shape[0], _p(y), int(bf16))
+    return y
+
+
+def vmf_pdf(mu, kappa, w):
+    mu, w = _d(mu), _d(w)
+    return _glib().orc_guide_vmf_pdf(_p(mu), float(kappa), _p(w))
+
+
+def vmf_sample(mu, kappa, u1, u2):
+    mu = _d(mu)
+    o = np.zeros(3)
+    _glib().orc_guide_vmf_sample(_p(mu), float(kappa), float(u1), float(u2), _p(o))
+    return o
+
+
+def mix_pdf(y, w):
+    """(p_vmf(w), alpha) of the mixture encoded by outputs y."""
+    y, w = _d(y), _d(w)
+    a = np.zeros(1)
+    p = _glib().orc_guide_mix_pdf(_p(y), _p(w), _p(a))
+    return p, float(a[0])
+
+
+def mix_sample(y, u, u1, u2):
+    y = _d(y)
+    o = np.zeros(3)
+    k = _glib().orc_guide_mix_sample(_p(y), float(u), float(u1), float(u2), _p(o))
+    return o, k
+
+
+def guide_loss(params, x, dirs, pb, w, grad=True, bf16=False):
+    params, x, dirs, pb, w = _d(params), _d(x), _d(dirs), _d(pb), _d(w)
+    g = np.zeros(GUIDE_NPARAM) if grad else None
+    L = _glib().orc_guide_loss(_p(params), x.shape[0], _p(x), _p(dirs), _p(pb), _p(w),
+                               _p(g) if grad else None, int(bf16))
+    return (L, g) if grad else L
+
+
+def guide_adam(params, m, v, grad, step, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+    """In place on float64 arrays params, m, v."""
+    _glib().orc_guide_adam(_p(params), _p(m), _p(v), _p(_d(grad)), params.shape[0], int(step), lr, beta1, beta2, eps)
+
+
+def bf16_round(x):
+    return _glib().orc_bf16_round(float(x))
