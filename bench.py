@@ -177,6 +177,128 @@ def bdpt_section(dev, cpu, threads):
     return out
 
 
+MLP_FLOP_FWD = 2 * (168 * 64 + 3 * 64 * 64 + 64 * 41)     # 51,328 algorithmic FLOP per query
+MFMA_BF16_PEAK = 2.5e15
+
+
+def guide_section(dev, ref_spp=16384, spp=256, train_passes=16, steps=16, batch=1 << 15, lr=5e-3, cache=None):
+    """Learned path guiding on the occluded-room scene (12 quads + 96
+    displaced spheres = 1,966,104 tris, 40% Phong e=200, emitter behind a wall
+    with a 0.1-wide gap), 1024x1024: train online on the first 16 spp (1 spp
+    per pass, Adam steps after each), render 256 spp guided; plain PT gets the
+    same wall-clock time; both MSEs against a ref_spp plain reference.  Both
+    samplers use the throughput roulette from depth 3 and at most 17
+    vertices.  Also: MFMA utilisation of the MLP (inference and training
+    kernels, timed with HIP events on their stream)."""
+    import numpy as np
+    import torch
+    from lmx import guide, render, scenes
+    kw = dict(max_num_vertices=17, rr_depth=3, rr_prob=0.5, rr_mode=render.RR_THROUGHPUT)
+    if cache and "ref" in cache:
+        sc, rs, ref, t_ref = cache["sc"], cache["rs"], cache["ref"], cache["t_ref"]
+        W, H = sc["camera"]["width"], sc["camera"]["height"]
+        npix = W * H
+    else:
+        log("guide: generating the occluded-room scene")
+        sc = scenes.occluded_room()
+        rs = render.Scene(sc)
+        W, H = sc["camera"]["width"], sc["camera"]["height"]
+        npix = W * H
+        rs.render_pt(npix * 4, seed=1, **kw)               # autotune + warm-up
+        log(f"guide: plain reference, {ref_spp} spp")
+        t0 = time.perf_counter()
+        ref = torch.zeros((H, W, 3), dtype=torch.float64, device=dev)
+        chunk, done = 64, 0
+        while done < ref_spp:
+            c = min(chunk, ref_spp - done)
+            f, _, _ = rs.render_pt(npix * c, seed=1_000_000 + done, **kw)
+            ref += f.double() * c
+            done += c
+        ref /= done
+        torch.cuda.synchronize()
+        t_ref = time.perf_counter() - t0
+        if cache is not None:
+            cache.update(sc=sc, rs=rs, ref=ref, t_ref=t_ref)
+    # plain sampling rate (for the equal-time budget)
+    rs.render_pt(npix * 16, seed=3, **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rs.render_pt(npix * 64, seed=4, **kw)
+    torch.cuda.synchronize()
+    rate = npix * 64 / (time.perf_counter() - t0)
+    # guided: online training then the render, one wall-clock budget
+    g = guide.Guide(rs, lr=lr, steps_per_pass=steps, batch=batch)
+    g.render(npix // 16, seed=5, **kw)                    # kernels and buffers warm
+    g = guide.Guide(rs, lr=lr, steps_per_pass=steps, batch=batch)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    recs = nz = 0
+    for k in range(train_passes):
+        _, _, gs = g.render(npix, seed=100 + k, train=True, **kw)
+        recs += gs["records"]
+        nz += gs["records_nonzero"]
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    fg, stg, _ = g.render(npix * spp, seed=777, **kw)
+    torch.cuda.synchronize()
+    t_render = time.perf_counter() - t1
+    t_guided = t_train + t_render
+    n_plain = max(1, int(t_guided * rate / npix))
+    t0 = time.perf_counter()
+    fp, stp, _ = rs.render_pt(npix * n_plain, seed=999, **kw)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    mse_g = float(((fg.double() - ref) ** 2).mean())
+    mse_p = float(((fp.double() - ref) ** 2).mean())
+    fq, _, _ = rs.render_pt(npix * spp, seed=998, **kw)
+    mse_p_same = float(((fq.double() - ref) ** 2).mean())
+    # MFMA utilisation: inference over 2^22 queries, training over 2^16 records
+    x = torch.rand((1 << 22, 7), device=dev)
+    g.eval(x)
+    st = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(5):
+        g.eval(x)
+    e1.record(st)
+    torch.cuda.synchronize()
+    t_eval = e0.elapsed_time(e1) / 5 / 1e3
+    r = np.random.default_rng(1)
+    xs = r.random((1 << 16, 7)); d = r.normal(size=(1 << 16, 3)); d /= np.linalg.norm(d, axis=1, keepdims=True)
+    recs_t = torch.from_numpy(guide.Guide.records(xs, d, r.uniform(0.05, 1.0, 1 << 16), r.uniform(0, 2, 1 << 16))).to(dev)
+    g.grad(recs_t)
+    e0.record(st)
+    for _ in range(5):
+        g.grad(recs_t)
+    e1.record(st)
+    torch.cuda.synchronize()
+    t_grad = e0.elapsed_time(e1) / 5 / 1e3
+    inf_tflops = (1 << 22) * MLP_FLOP_FWD / t_eval / 1e12
+    trn_tflops = (1 << 16) * 3 * MLP_FLOP_FWD / t_grad / 1e12
+    return {
+        "workload": "guide: occluded room (12 quads + 96 displaced spheres, 1,966,104 tris, 40% Phong e=200, "
+                    "emitter behind a 0.1-wide gap), 1024x1024, MLP 168-64x4-41 bf16 -> 8 vMF lobes + BSDF mix, "
+                    "online training on the first 16 spp, 256 spp guided vs plain PT at equal time",
+        "metric": "equal-time MSE ratio, guided vs plain GPU PT",
+        "mse_ratio_equal_time": round(mse_p / mse_g, 3),
+        "mse_ratio_equal_spp": round(mse_p_same / mse_g, 3),
+        "mse_guided": mse_g, "mse_plain_equal_time": mse_p, "mse_plain_256spp": mse_p_same,
+        "guided_s": round(t_guided, 3), "train_s": round(t_train, 3), "render_s": round(t_render, 3),
+        "plain_spp_equal_time": n_plain, "plain_s": round(t_plain, 3),
+        "guided_msamples_per_s": round(npix * spp / t_render / 1e6, 1),
+        "plain_msamples_per_s": round(rate / 1e6, 1),
+        "train_records": recs, "train_records_nonzero": nz,
+        "reference": {"spp": ref_spp, "s": round(t_ref, 1), "mean": round(float(ref.mean()), 6)},
+        "training": {"passes": train_passes, "steps_per_pass": steps, "batch": batch, "lr": lr},
+        "mfma": {"peak_tflops_bf16": MFMA_BF16_PEAK / 1e12,
+                 "inference_tflops": round(inf_tflops, 1), "inference_frac": round(inf_tflops * 1e12 / MFMA_BF16_PEAK, 4),
+                 "inference_mqueries_per_s": round((1 << 22) / t_eval / 1e6, 1),
+                 "training_tflops": round(trn_tflops, 2), "training_frac": round(trn_tflops * 1e12 / MFMA_BF16_PEAK, 5),
+                 "flop_per_query": MLP_FLOP_FWD},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,7 +308,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-pt", action="store_true", help="skip the wavefront path-tracer section")
     ap.add_argument("--no-bdpt", action="store_true", help="skip the bidirectional section")
-    ap.add_argument("--only", choices=["pt", "bdpt"], default=None,
+    ap.add_argument("--no-guide", action="store_true", help="skip the path-guiding section")
+    ap.add_argument("--guide-ref-spp", type=int, default=16384,
+                    help="spp of the plain reference image of the path-guiding section")
+    ap.add_argument("--only", choices=["pt", "bdpt", "guide"], default=None,
                     help="run one renderer section alone and print its JSON (for profiling runs)")
     ap.add_argument("--no-per-set", action="store_true",
                     help="skip the separate coherent/incoherent timings (for profiling runs)")
@@ -206,7 +331,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(rays shard across GPUs with no exchange, but this build runs on one GPU only)",
+              "(rays shard across GPUs with no exchange; this benchmark runs on one GPU)",
               file=sys.stderr)
         sys.exit(2)
 
@@ -217,8 +342,12 @@ def main():
     dev = torch.device("cuda:0")
     if args.only:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        fn = pt_section if args.only == "pt" else bdpt_section
-        print(json.dumps({args.only: fn(dev, not args.no_cpu_baseline, threads)}), flush=True)
+        if args.only == "guide":
+            out = guide_section(dev, args.guide_ref_spp)
+        else:
+            fn = pt_section if args.only == "pt" else bdpt_section
+            out = fn(dev, not args.no_cpu_baseline, threads)
+        print(json.dumps({args.only: out}), flush=True)
         return
     log("generating scene")
     verts, tris = scenes.traversal_scene()
@@ -340,6 +469,8 @@ def main():
         out["pt"] = pt_section(dev, not args.no_cpu_baseline, args.cpu_threads or min(os.cpu_count() or 1, 16))
     if not args.no_bdpt:
         out["bdpt"] = bdpt_section(dev, not args.no_cpu_baseline, args.cpu_threads or min(os.cpu_count() or 1, 16))
+    if not args.no_guide:
+        out["guide"] = guide_section(dev, args.guide_ref_spp)
     print(json.dumps(out), flush=True)
 
 

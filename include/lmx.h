@@ -152,6 +152,12 @@ typedef struct lmx_scene lmx_scene;
  * segment's hit.  Table width >= 8 + 4 (max_num_vertices - 1). */
 #define LMX_PT_SLOT_RASTER 1
 #define LMX_PT_SLOT_SEG0 8
+/* Russian roulette of the unidirectional samplers: continue with rr_prob
+ * (renderer_pt.cpp:207-215), or -- an extension without a reference
+ * counterpart -- with min(1, largest throughput channel), which keeps the
+ * throughput of surviving paths from growing */
+#define LMX_RR_FIXED 0
+#define LMX_RR_THROUGHPUT 1
 
 typedef struct lmx_render_params {
     int32_t max_num_vertices;    /* -1: unbounded (renderer_pt.cpp:59) */
@@ -165,6 +171,7 @@ typedef struct lmx_render_params {
     const float *uniform_table;  /* device, num_samples x table_width, or NULL (Philox) */
     uint32_t table_width;
     float *sample_out;           /* device, num_samples x 8 floats (raster x, y, C rgb, #splats, 0, 0), or NULL */
+    int32_t rr_mode;             /* LMX_RR_FIXED (reference) or LMX_RR_THROUGHPUT */
 } lmx_render_params;
 
 typedef struct lmx_render_stats {
@@ -210,13 +217,13 @@ int lmx_render_bdpt(lmx_scene *scene, const lmx_render_params *params, float *fi
  * to the previous vertex as theta/pi, phi/2pi; 12-frequency sin/cos
  * encoding; 4 hidden layers of 64 ReLU units; BF16 weights and activations,
  * FP32 accumulation) gives 8 von Mises-Fisher lobes and a BSDF selection
- * probability alpha in [0.1, 0.9]; the next direction is drawn from
+ * probability alpha in [0.5, 0.9]; the next direction is drawn from
  * alpha BSDF + (1 - alpha) lobes (one-sample MIS, so the image stays
  * unbiased).  Parameters: LMX_GUIDE_NPARAM floats, layout
  * W1[169][64] (row 168 = bias), W2[64][64], b2[64], W3, b3, W4, b4,
  * W5[64][41], b5[41], with W[k][n] the weight from input k to output n.
  * Outputs y[41]: lobe means y[3k..3k+2] (normalised), log kappa y[24+k]
- * (clamped to [-4, 9]), lobe weight logits y[32+k], alpha = 0.1 + 0.8
+ * (clamped to [-4, 9]), lobe weight logits y[32+k], alpha = 0.5 + 0.4
  * sigmoid(y[40]).  Trained online with Adam on the loss
  * -w log(alpha p_bsdf + (1 - alpha) p_vmf), w = radiance x BSDF x cosine /
  * sampling density of each recorded direction, normalised to mean one over
@@ -236,7 +243,7 @@ typedef struct lmx_guide_stats {
     float loss;                    /* mean loss of the last step */
     double render_ms, train_ms;
 } lmx_guide_stats;
-void lmx_guide_default_opts(lmx_guide_opts *opts);   /* lr 1e-3, 0.9, 0.999, 1e-8, 2^16, 8, 50, 7 */
+void lmx_guide_default_opts(lmx_guide_opts *opts);   /* lr 5e-3, 0.9, 0.999, 1e-8, 2^15, 16, 50, 7 */
 int lmx_guide_create(lmx_scene *scene, const lmx_guide_opts *opts, lmx_guide **out);
 void lmx_guide_free(lmx_guide *guide);
 /* host copy of the FP32 parameters in (set != 0) or out */

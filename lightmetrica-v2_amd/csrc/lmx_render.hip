@@ -83,8 +83,9 @@ __global__ __launch_bounds__(256) void k_shade(RScene s, RParams p, Ctl* __restr
                 bool alive = true;
                 // Russian roulette (renderer_pt.cpp:207-215), from rr_depth on
                 if ((int)seg + 1 >= p.rr_depth) {
-                    if (U.one(LMX_PT_SLOT_SEG0 + 4u * seg + 3u) > p.rr_prob) alive = false;
-                    else thr = mk3(thr.x / p.rr_prob, thr.y / p.rr_prob, thr.z / p.rr_prob);
+                    const float q = rr_continue(p, thr);
+                    if (!(q > 0.0f) || U.one(LMX_PT_SLOT_SEG0 + 4u * seg + 3u) > q) alive = false;
+                    else thr = mk3(thr.x / q, thr.y / q, thr.z / q);
                 }
                 nv++;
                 if (alive && p.max_nv != -1 && nv >= p.max_nv) alive = false;
@@ -439,6 +440,8 @@ int lmx_render_pt(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, vo
     if (p->uniform_table && p->table_width < (uint32_t)(LMX_PT_SLOT_SEG0 + 4 * (p->max_num_vertices - 1)))
         return set_error(LMX_EINVAL, "lmx_render_pt: uniform table narrower than 8 + 4 (max_num_vertices - 1)");
     if (!(p->rr_prob > 0.0f && p->rr_prob <= 1.0f)) return set_error(LMX_EINVAL, "lmx_render_pt: rr_prob must be in (0, 1]");
+    if (p->rr_mode != LMX_RR_FIXED && p->rr_mode != LMX_RR_THROUGHPUT)
+        return set_error(LMX_EINVAL, "lmx_render_pt: unknown rr_mode");
     if (p->raster_mode != LMX_RASTER_RANDOM && p->raster_mode != LMX_RASTER_STRATIFIED)
         return set_error(LMX_EINVAL, "lmx_render_pt: unknown raster_mode");
     hipStream_t st = (hipStream_t)stream;
@@ -460,6 +463,7 @@ int lmx_render_pt(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, vo
     rp.sample_out = p->sample_out;
     rp.film = film_rgb;
     rp.guide = nullptr;
+    rp.rr_mode = p->rr_mode;
     const int zgrid = std::max(1, std::min((int)((3 * npix + 255) / 256), sc->cus * 8));
     int rc = LMX_OK;
     if (p->queue_capacity == 0 && sc->tuned_cap == 0) rc = autotune_capacity(sc, rp, st);

@@ -76,7 +76,14 @@ struct RParams {
     float* sample_out;         // 8 floats per sample, or null
     float* film;               // W x H x 3 accumulation
     const GuideLaunch* guide;  // null: BSDF sampling only
+    int32_t rr_mode;           // LMX_RR_*
 };
+
+// Russian-roulette continuation probability of a path with throughput thr
+RD float rr_continue(const RParams& p, f3 thr)
+{
+    return p.rr_mode == LMX_RR_THROUGHPUT ? fminf(1.0f, fmaxf(thr.x, fmaxf(thr.y, thr.z))) : p.rr_prob;
+}
 
 template <class P>
 struct Uniforms {

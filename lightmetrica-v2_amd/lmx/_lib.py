@@ -61,7 +61,7 @@ class lmx_render_params(ctypes.Structure):
                 ("seed", ctypes.c_uint64), ("num_samples", ctypes.c_uint64),
                 ("raster_mode", ctypes.c_int32), ("queue_capacity", ctypes.c_uint32),
                 ("uniform_table", ctypes.c_void_p), ("table_width", ctypes.c_uint32),
-                ("sample_out", ctypes.c_void_p)]
+                ("sample_out", ctypes.c_void_p), ("rr_mode", ctypes.c_int32)]
 
 
 class lmx_render_stats(ctypes.Structure):

@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 
 from ._lib import check, lib, lmx_guide_opts, lmx_guide_stats, lmx_render_params, lmx_render_stats
-from .render import RASTER_STRATIFIED
+from .render import RASTER_STRATIFIED, RR_FIXED
 
 NPARAM = 25961
 NOUT = 41
@@ -110,7 +110,8 @@ class Guide:
         return out
 
     def render(self, num_samples, max_num_vertices=9, min_num_vertices=0, rr_depth=1, rr_prob=0.5, seed=42,
-               raster_mode=RASTER_STRATIFIED, train=False, queue_capacity=0, table=None, stream=None):
+               raster_mode=RASTER_STRATIFIED, train=False, queue_capacity=0, table=None, stream=None,
+               rr_mode=RR_FIXED):
         """Guided renderer::pt.  Returns (film (H, W, 3) torch, render stats,
         guide stats)."""
         import torch
@@ -126,6 +127,7 @@ class Guide:
         p.num_samples = num_samples
         p.raster_mode = raster_mode
         p.queue_capacity = queue_capacity
+        p.rr_mode = rr_mode
         if table is not None:
             assert table.is_cuda and table.dtype == torch.float32 and table.is_contiguous()
             assert table.shape[0] == num_samples

@@ -18,6 +18,8 @@ from .scenes import camera_frame
 
 RASTER_RANDOM = 0
 RASTER_STRATIFIED = 1
+RR_FIXED = 0
+RR_THROUGHPUT = 1
 SLOT_SEG0 = 8
 
 
@@ -123,7 +125,7 @@ class Scene:
 
     def render_pt(self, num_samples, max_num_vertices=9, min_num_vertices=0, rr_depth=1, rr_prob=0.5,
                   seed=42, raster_mode=RASTER_STRATIFIED, queue_capacity=0, table=None, sample_out=False,
-                  stream=None):
+                  stream=None, rr_mode=RR_FIXED):
         """Returns (film as a torch tensor (H, W, 3) on the GPU, stats dict,
         per-sample records (n, 8) or None).  `table`: torch float32 tensor
         (num_samples, width) on the GPU, or None for Philox."""
@@ -140,6 +142,7 @@ class Scene:
         p.num_samples = num_samples
         p.raster_mode = raster_mode
         p.queue_capacity = queue_capacity
+        p.rr_mode = rr_mode
         if table is not None:
             assert table.is_cuda and table.dtype == torch.float32 and table.is_contiguous()
             assert table.shape[0] == num_samples

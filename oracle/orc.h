@@ -160,6 +160,7 @@ typedef struct orc_pt_params {
     float rr_prob;                   /* continuation probability (0.5 in the reference) */
     uint32_t seed_lo, seed_hi;
     int32_t raster_mode;             /* ORC_RASTER_* */
+    int32_t rr_mode;                 /* 0: rr_prob; 1: min(1, max throughput channel) */
 } orc_pt_params;
 
 typedef struct orc_scene orc_scene;
@@ -198,7 +199,7 @@ void orc_bsdf_eval(const orc_material *m, const float n[3], const float wi[3], c
  *   168 -> 64 -> 64 -> 64 -> 64 (ReLU) -> 41;
  *   y[3k..3k+2] lobe mean (normalised), y[24+k] log kappa (clamped to
  *   [-4, 9]), y[32+k] lobe weight logits (softmax), y[40] BSDF selection
- *   probability 0.1 + 0.8 sigmoid(y40);
+ *   probability 0.5 + 0.4 sigmoid(y40);
  *   parameters (canonical flat layout, W[k][n] = input k -> output n):
  *   W1[169][64] (row 168 = bias), W2[64][64], b2[64], W3, b3, W4, b4,
  *   W5[64][41], b5[41] -- ORC_GUIDE_NPARAM values.

@@ -177,7 +177,8 @@ class orc_scene_desc(ctypes.Structure):
 class orc_pt_params(ctypes.Structure):
     _fields_ = [("max_num_vertices", ctypes.c_int32), ("min_num_vertices", ctypes.c_int32),
                 ("rr_depth", ctypes.c_int32), ("rr_prob", ctypes.c_float),
-                ("seed_lo", ctypes.c_uint32), ("seed_hi", ctypes.c_uint32), ("raster_mode", ctypes.c_int32)]
+                ("seed_lo", ctypes.c_uint32), ("seed_hi", ctypes.c_uint32), ("raster_mode", ctypes.c_int32),
+                ("rr_mode", ctypes.c_int32)]
 
 
 class orc_bdpt_params(ctypes.Structure):
@@ -304,10 +305,10 @@ class Scene:
             self._h = None
 
     def pt_samples(self, first, n, max_num_vertices=9, min_num_vertices=0, rr_depth=1, rr_prob=0.5,
-                   seed=42, raster_mode=RASTER_STRATIFIED, table=None, nthreads=8):
+                   seed=42, raster_mode=RASTER_STRATIFIED, table=None, nthreads=8, rr_mode=0):
         """(n, 6) float32: raster x, y, C rgb, number of splats."""
         p = orc_pt_params(max_num_vertices, min_num_vertices, rr_depth, rr_prob, seed & 0xFFFFFFFF,
-                          seed >> 32, raster_mode)
+                          seed >> 32, raster_mode, rr_mode)
         out = np.zeros((n, 6), np.float32)
         tw = 0
         tp = None

@@ -40,8 +40,8 @@
 
 #define LOGK_MIN (-4.0)
 #define LOGK_MAX 9.0
-#define ALPHA_MIN 0.1
-#define ALPHA_SPAN 0.8
+#define ALPHA_MIN 0.5
+#define ALPHA_SPAN 0.4
 static const double PI = 3.14159265358979323846;
 
 double orc_bf16_round(double x)

@@ -375,8 +375,14 @@ static void pt_sample(const orc_scene *s, const camera *cam, const orc_pt_params
         }
         /* Russian roulette (:207-215), from depth rr_depth on */
         if (seg + 1 >= pp->rr_depth) {
-            if (U(us, sample, base + 3) > pp->rr_prob) break;
-            for (int c = 0; c < 3; c++) thr[c] /= pp->rr_prob;
+            /* rr_mode 1: throughput roulette, an extension without a reference counterpart */
+            float q = pp->rr_prob;
+            if (pp->rr_mode == 1) {
+                q = fmaxf(thr[0], fmaxf(thr[1], thr[2]));
+                q = q < 1.0f ? q : 1.0f;
+            }
+            if (!(q > 0.0f) || U(us, sample, base + 3) > q) break;
+            for (int c = 0; c < 3; c++) thr[c] /= q;
         }
         wi = vneg(wo);
         num_vertices++;

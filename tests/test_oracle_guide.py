@@ -80,7 +80,7 @@ def test_mixture_normalised_and_sampled():
     freq = np.bincount(lobes, minlength=8) / len(lobes)
     assert np.abs(freq - pi).max() < 0.015
     _, alpha = orc.mix_pdf(y, [0, 0, 1.0])
-    assert abs(alpha - (0.1 + 0.8 / (1 + np.exp(-y[40])))) < 1e-12
+    assert abs(alpha - (0.5 + 0.4 / (1 + np.exp(-y[40])))) < 1e-12
 
 
 def _batch(seed, n):

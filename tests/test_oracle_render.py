@@ -109,3 +109,14 @@ def test_uniform_table_and_philox_agree():
     a = s.pt_samples(0, n, max_num_vertices=maxv, seed=42)
     b = s.pt_samples(0, n, max_num_vertices=maxv, table=tab)
     np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("rr_depth", [1, 3])
+def test_pt_furnace_throughput_roulette(rr_depth):
+    """The throughput roulette (survive with min(1, max channel)) is unbiased:
+    the furnace closed form holds with it too."""
+    rho, Le, maxv = 0.5, 1.0, 7
+    sc = scenes.furnace_sphere(albedo=rho, Le=Le, width=32, height=32)
+    out = orc.Scene(sc).pt_samples(0, 32 * 32 * 256, max_num_vertices=maxv, rr_depth=rr_depth, seed=6, rr_mode=1)
+    expect = Le * sum(rho ** j for j in range(maxv - 1))
+    assert abs(out[:, 2:5].mean() - expect) < 0.01 * expect

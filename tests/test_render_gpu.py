@@ -78,12 +78,13 @@ def test_pt_per_sample_parity_sphere_grid_small(gpu):
     assert ok.mean() > 0.999, f"{(~ok).sum()} of {n} samples differ"
 
 
-def test_pt_per_sample_parity_philox(gpu):
+@pytest.mark.parametrize("rr_mode", [render.RR_FIXED, render.RR_THROUGHPUT])
+def test_pt_per_sample_parity_philox(gpu, rr_mode):
     sc = scenes.cornell_box(64, 64)
     n = 64 * 64 * 16
     rs = render.Scene(sc)
-    _, _, so = rs.render_pt(n, max_num_vertices=9, seed=1234, sample_out=True)
-    c = orc.Scene(sc).pt_samples(0, n, max_num_vertices=9, seed=1234)
+    _, _, so = rs.render_pt(n, max_num_vertices=9, seed=1234, sample_out=True, rr_mode=rr_mode, rr_depth=2)
+    c = orc.Scene(sc).pt_samples(0, n, max_num_vertices=9, seed=1234, rr_mode=rr_mode, rr_depth=2)
     ok = _compare(so.cpu().numpy(), c)
     assert ok.mean() > 0.999, f"{(~ok).sum()} of {n} samples differ"
 
