@@ -193,6 +193,22 @@ void lmx_scene_free(lmx_scene *scene);
 int lmx_render_pt(lmx_scene *scene, const lmx_render_params *params, float *film_rgb, void *stream,
                   lmx_render_stats *stats);
 
+/* renderer::ptdirect (renderer_ptdirect.cpp:72-297) and renderer::ptmis
+ * (renderer_ptmis.cpp:69-326): renderer::pt plus next-event estimation at
+ * every vertex, the camera's included (a light by uniform selection, a point
+ * by area, one shadow ray); ptmis weighs it and the BSDF-sampled light hits
+ * by the balance heuristic, ptdirect counts light sampling only.  Same
+ * parameters as lmx_render_pt (ptdirect ignores min_num_vertices as the
+ * reference does); uniform blocks of 8 slots per vertex v at 8 + 8 v:
+ * direction 0-1, unused 2, roulette after the hit of segment v 3, light
+ * selection 4, light position 5-6.  sample_out records have 16 floats: the
+ * 8 of lmx_render_pt, then raster x, y, C rgb, count of the camera-vertex
+ * connection (splatted where the light point projects). */
+int lmx_render_ptdirect(lmx_scene *scene, const lmx_render_params *params, float *film_rgb, void *stream,
+                        lmx_render_stats *stats);
+int lmx_render_ptmis(lmx_scene *scene, const lmx_render_params *params, float *film_rgb, void *stream,
+                     lmx_render_stats *stats);
+
 /* renderer::vcm with mode: bdpt (renderer_vcm.cpp:662-713): an eye and a
  * light subpath of at most max_num_vertices vertices per sample, every
  * (s, t) connection with s + t <= max_num_vertices, power-heuristic MIS over

@@ -131,20 +131,6 @@ RD void store_pr(const VB& vb, const BParams& p, int kind, uint32_t i, uint32_t 
     r[3] = pr;
 }
 
-RD f3 dir_to(f3 from, f3 to) { return normalize(sub(to, from)); }
-
-// RenderUtils::GeometryTerm (renderutils.h:46-56)
-RD float gterm(f3 pa, f3 na, bool dega, f3 pb, f3 nb, bool degb)
-{
-    f3 d = sub(pb, pa);
-    const float l2 = dot(d, d);
-    const float l = sqrtf(l2);
-    d = mk3(d.x / l, d.y / l, d.z / l);
-    float t = 1.0f;
-    if (!dega) t *= fabsf(dot(na, d));
-    if (!degb) t *= fabsf(dot(nb, neg(d)));
-    return t / l2;
-}
 RD float gterm(const Vtx& a, const Vtx& b) { return gterm(a.p, a.sn, a.degenerate, b.p, b.sn, b.degenerate); }
 
 // a^2 (1 + b^2 (1 + U)): the sum over one side of the squared pdf ratios of
@@ -156,35 +142,6 @@ RD float chain_sq(float a, float b, float U)
     const float a2 = a * a;
     if (b == 0.0f) return a2;
     return a2 * (1.0f + b * b * (1.0f + U));
-}
-
-RD float pos_pdf_light(const RScene& s, int l) { return s.lights[l].inv_area * (1.0f / (float)s.nlight); }
-
-// Area-light emission: Le one-sided (light_area.cpp:105-110); direction pdf
-// 1/pi in projected solid angle when above the surface (:84-89).
-RD bool emits_towards(const Frame& f, f3 wo) { return to_local(f, wo).z > 0.0f; }
-
-// Light point: Distribution1D::SampleReuse + UniformSampleTriangle
-// (triangleutils.h:73-120)
-RD void sample_light_point(const RScene& s, int l, float u0, float u1, f3& p, f3& gn, uint32_t& tri)
-{
-    const RLight L = s.lights[l];
-    const float* cdf = s.light_cdf + L.cdf_off;
-    uint32_t lo = 0, hi = L.count + 1;          // upper_bound over cdf[0..count]
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (cdf[mid] <= u0) lo = mid + 1; else hi = mid;
-    }
-    int i = (int)lo - 1;
-    i = i < 0 ? 0 : (i > (int)L.count - 1 ? (int)L.count - 1 : i);
-    const float u2 = (u0 - cdf[i]) / (cdf[i + 1] - cdf[i]);
-    float b0, b1;
-    uniform_triangle(u2, u1, b0, b1);
-    tri = L.first + (uint32_t)i;
-    const uint32_t i0 = s.tris[3 * tri], i1 = s.tris[3 * tri + 1], i2 = s.tris[3 * tri + 2];
-    const f3 p1 = ld3(s.verts + 3 * i0), p2 = ld3(s.verts + 3 * i1), p3 = ld3(s.verts + 3 * i2);
-    p = add(add(mul(p1, 1.0f - b0 - b1), mul(p2, b0)), mul(p3, b1));
-    gn = normalize(cross(sub(p2, p1), sub(p3, p1)));
 }
 
 // ---------------------------------------------------------------------------

@@ -1141,6 +1141,9 @@ int lmx_render_guided(lmx_scene* sc, lmx_guide* g, const lmx_render_params* p, i
     rp.film = film_rgb;
     rp.guide = &gl;
     rp.rr_mode = p->rr_mode;
+    rp.nee = NEE_NONE;
+    rp.seg_stride = 4;
+    rp.so_stride = 8;
     uint32_t cap = p->queue_capacity ? p->queue_capacity : (sc->tuned_cap ? sc->tuned_cap : (1u << 22));
     cap = (uint32_t)std::min<uint64_t>(cap, std::max<uint64_t>(p->num_samples, 64));
     uint64_t rays = 0;

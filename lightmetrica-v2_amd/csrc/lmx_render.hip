@@ -42,6 +42,82 @@ namespace {
 // ---------------------------------------------------------------------------
 // kernels
 
+// Next-event estimation from one vertex (renderer_ptmis.cpp:130-195,
+// renderer_ptdirect.cpp:109-150): a light by u[0] (uniform over the area
+// lights, Scene3::SampleEmitter), a point on it by u[1..2]; contribution
+// thr fs G Le / (p_light p_area), weighted in MIS mode by the balance
+// heuristic against BSDF sampling of the same direction.  Camera vertex:
+// fs = the sensor importance, raster position of the connection returned.
+struct NeeOut {
+    bool emit;
+    float4 r0, r1;
+    f3 C;
+    float rx, ry;
+};
+
+RD NeeOut nee_connect(const RScene& s, const RParams& p, bool camera, const Frame& fr, const RMat& m, f3 wi, f3 thr,
+                      float u0, float u1, float u2)
+{
+    NeeOut o;
+    o.emit = false;
+    o.rx = o.ry = 0.0f;
+    if (s.nlight == 0) return o;
+    int l = (int)(u0 * (float)s.nlight);
+    l = l < 0 ? 0 : (l > (int)s.nlight - 1 ? (int)s.nlight - 1 : l);
+    f3 lp, lgn;
+    uint32_t tri;
+    sample_light_point(s, l, u1, u2, lp, lgn, tri);
+    const f3 pv = camera ? s.cam_pos : fr.p;
+    const f3 ppL = dir_to(pv, lp);
+    const RMat lm = s.mats[s.tri_mat[tri]];
+    if (!(dot(lgn, neg(ppL)) > 0.0f)) return o;                      // one-sided emitter
+    f3 fsE;
+    float pB;
+    if (camera) {
+        const float W = importance(s, ppL);
+        if (!(W > 0.0f) || !camera_raster(s, ppL, o.rx, o.ry)) return o;
+        fsE = mk3(W, W, W);
+        pB = W;
+    } else {
+        fsE = bsdf_eval(m, fr, wi, ppL);
+        pB = bsdf_pdf(m, fr, wi, ppL);
+    }
+    const float G = gterm(pv, fr.sn, camera, lp, lgn, false);
+    const float inv_pl = (float)s.nlight / s.lights[l].inv_area;       // 1 / (p_light p_area)
+    f3 C = mk3(thr.x * fsE.x * G * lm.Le[0] * inv_pl, thr.y * fsE.y * G * lm.Le[1] * inv_pl,
+               thr.z * fsE.z * G * lm.Le[2] * inv_pl);
+    // a black contribution is dropped before the weight (renderer_ptmis.cpp:171),
+    // which also keeps G = 0 (an infinite light density) out of it
+    if (C.x == 0.0f && C.y == 0.0f && C.z == 0.0f) return o;
+    if (p.nee == NEE_MIS) {
+        const float pL = s.lights[l].inv_area / G * (1.0f / (float)s.nlight);
+        const float w = pL / (pL + pB);
+        C = mk3(C.x * w, C.y * w, C.z * w);
+    }
+    // Scene3::Visible (scene3.h:107-116)
+    f3 d = sub(lp, pv);
+    const float len = sqrtf(dot(d, d));
+    d = mk3(d.x / len, d.y / len, d.z / len);
+    o.r0 = make_float4(pv.x, pv.y, pv.z, LMX_EPS_ISECT);
+    o.r1 = make_float4(d.x, d.y, d.z, len * (1.0f - LMX_EPS_ISECT));
+    o.C = C;
+    o.emit = true;
+    return o;
+}
+
+RD void nee_append(const RParams& p, Ctl* ctl, uint32_t* lds, const NeeOut& o, uint32_t sample, uint32_t pixel,
+                   bool camera)
+{
+    const uint32_t slot = block_reserve<4>(&ctl->n_shadow, o.emit ? 1u : 0u, lds);
+    if (o.emit) {
+        p.nq.ray[2 * slot] = o.r0;
+        p.nq.ray[2 * slot + 1] = o.r1;
+        p.nq.c[slot] = make_float4(o.C.x, o.C.y, o.C.z, 0.0f);
+        p.nq.meta[slot] = make_uint4(sample, pixel, camera ? 1u : 0u, __float_as_uint(o.rx));
+        p.nq.ry[slot] = o.ry;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_shade(RScene s, RParams p, Ctl* __restrict__ ctl, Queue in,
                                                const float4* __restrict__ hits, Queue out)
 {
@@ -53,6 +129,9 @@ __global__ __launch_bounds__(256) void k_shade(RScene s, RParams p, Ctl* __restr
         bool cont = false;
         float4 o_ray0, o_ray1, o_thr;
         uint4 o_meta;
+        NeeOut no;
+        no.emit = false;
+        uint32_t no_sample = 0, no_pixel = 0;
         if (i < n) {
             const uint4 meta = in.meta[i];
             const float4 h = hits[i];
@@ -67,14 +146,23 @@ __global__ __launch_bounds__(256) void k_shade(RScene s, RParams p, Ctl* __restr
                 surface_at(s, o, d, h.x, h.y, h.z, id, fr);
                 const RMat m = s.mats[s.tri_mat[id]];
                 int nv = (int)seg + 1;                  // numVertices when this segment was traced
-                // hit with a light (renderer_pt.cpp:183-194)
-                if (m.Le[0] > 0.0f || m.Le[1] > 0.0f || m.Le[2] > 0.0f) {
-                    if (nv + 1 >= p.min_nv && to_local(fr, neg(d)).z > 0.0f) {
-                        const f3 C = mk3(thr.x * m.Le[0], thr.y * m.Le[1], thr.z * m.Le[2]);
+                // hit with a light (renderer_pt.cpp:183-194; renderer_ptmis.cpp:229-257
+                // weighs it against light sampling; renderer::ptdirect ignores it)
+                if (p.nee != NEE_DIRECT && (m.Le[0] > 0.0f || m.Le[1] > 0.0f || m.Le[2] > 0.0f)) {
+                    const float cl = to_local(fr, neg(d)).z;
+                    if (nv + 1 >= p.min_nv && cl > 0.0f) {
+                        float w = 1.0f;
+                        if (p.nee == NEE_MIS) {
+                            // t4.w: the BSDF density of d at the previous vertex times its cosine there
+                            const int l = s.tri_light[id];
+                            const float pL = s.lights[l].inv_area * h.x * h.x / cl * (1.0f / (float)s.nlight);
+                            w = t4.w / (t4.w + pL);
+                        }
+                        const f3 C = mk3(thr.x * m.Le[0] * w, thr.y * m.Le[1] * w, thr.z * m.Le[2] * w);
                         float* fp = p.film + 3 * (size_t)pixel;
                         atomicAdd(fp, C.x); atomicAdd(fp + 1, C.y); atomicAdd(fp + 2, C.z);
                         if (p.sample_out) {
-                            float* so = p.sample_out + 8 * (size_t)sample;
+                            float* so = p.sample_out + p.so_stride * (size_t)sample;
                             so[2] += C.x; so[3] += C.y; so[4] += C.z; so[5] += 1.0f;
                         }
                     }
@@ -84,15 +172,21 @@ __global__ __launch_bounds__(256) void k_shade(RScene s, RParams p, Ctl* __restr
                 // Russian roulette (renderer_pt.cpp:207-215), from rr_depth on
                 if ((int)seg + 1 >= p.rr_depth) {
                     const float q = rr_continue(p, thr);
-                    if (!(q > 0.0f) || U.one(LMX_PT_SLOT_SEG0 + 4u * seg + 3u) > q) alive = false;
+                    if (!(q > 0.0f) || U.one(LMX_PT_SLOT_SEG0 + p.seg_stride * seg + 3u) > q) alive = false;
                     else thr = mk3(thr.x / q, thr.y / q, thr.z / q);
                 }
                 nv++;
                 if (alive && p.max_nv != -1 && nv >= p.max_nv) alive = false;
                 if (alive && (m.type == LMX_MAT_DIFFUSE || m.type == LMX_MAT_PHONG)) {
                     const uint32_t seg2 = seg + 1u;
-                    const float4 u = U.block(LMX_PT_SLOT_SEG0 + 4u * seg2);
                     const f3 wi = neg(d);
+                    if (p.nee != NEE_NONE && (p.nee == NEE_DIRECT || nv + 1 >= p.min_nv)) {
+                        const float4 un = U.block(LMX_PT_SLOT_SEG0 + p.seg_stride * seg2 + 4u);
+                        no = nee_connect(s, p, false, fr, m, wi, thr, un.x, un.y, un.z);
+                        no_sample = sample;
+                        no_pixel = pixel;
+                    }
+                    const float4 u = U.block(LMX_PT_SLOT_SEG0 + p.seg_stride * seg2);
                     f3 wo;
                     if (bsdf_sample(m, fr, wi, u.x, u.y, wo)) {
                         const float pdf = bsdf_pdf(m, fr, wi, wo);
@@ -101,7 +195,7 @@ __global__ __launch_bounds__(256) void k_shade(RScene s, RParams p, Ctl* __restr
                             thr = mk3(thr.x * (fs.x / pdf), thr.y * (fs.y / pdf), thr.z * (fs.z / pdf));
                             o_ray0 = make_float4(fr.p.x, fr.p.y, fr.p.z, LMX_EPS_ISECT);
                             o_ray1 = make_float4(wo.x, wo.y, wo.z, LMX_FLT_MAX);
-                            o_thr = make_float4(thr.x, thr.y, thr.z, 0.0f);
+                            o_thr = make_float4(thr.x, thr.y, thr.z, pdf * fabsf(to_local(fr, wo).z));
                             o_meta = make_uint4(sample, seg2, pixel, 0u);
                             cont = true;
                         }
@@ -116,6 +210,7 @@ __global__ __launch_bounds__(256) void k_shade(RScene s, RParams p, Ctl* __restr
             out.thr[slot] = o_thr;
             out.meta[slot] = o_meta;
         }
+        if (p.nee != NEE_NONE) nee_append(p, ctl, lds, no, no_sample, no_pixel, false);
     }
 }
 
@@ -126,42 +221,85 @@ RD uint32_t gen_count(const Ctl* ctl, uint32_t cap, uint32_t total)
     return free_slots < left ? free_slots : left;
 }
 
-__global__ __launch_bounds__(256) void k_gen(RScene s, RParams p, const Ctl* __restrict__ ctl, uint32_t cap, Queue out)
+__global__ __launch_bounds__(256) void k_gen(RScene s, RParams p, Ctl* __restrict__ ctl, uint32_t cap, Queue out)
 {
+    __shared__ uint32_t lds[5];
     const uint32_t want = gen_count(ctl, cap, p.total);
     const uint32_t base_slot = ctl->n_next, first = ctl->issued;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < want; t += gridDim.x * blockDim.x) {
-        const uint32_t sample = first + t, slot = base_slot + t;
-        const Uniforms<RParams> U{p, sample};
-        const float ux = U.one(LMX_PT_SLOT_RASTER), uy = U.one(LMX_PT_SLOT_RASTER + 1);
-        float rx, ry;
-        f3 wo;
-        uint32_t pixel, flags = 0;
-        float out_rx, out_ry;
-        if (p.raster_mode == LMX_RASTER_STRATIFIED) {
-            const uint32_t npix = (uint32_t)s.width * (uint32_t)s.height;
-            pixel = sample % npix;
-            const uint32_t px = pixel % (uint32_t)s.width, py = pixel / (uint32_t)s.width;
-            rx = ((float)px + ux) / (float)s.width;
-            ry = ((float)py + uy) / (float)s.height;
-            wo = camera_dir(s, rx, ry);
-            out_rx = ((float)px + 0.5f) / (float)s.width;
-            out_ry = ((float)py + 0.5f) / (float)s.height;
-        } else {
-            wo = camera_dir(s, ux, uy);
-            if (!camera_raster(s, wo, rx, ry)) { flags = META_DEAD; rx = ry = 0.0f; }
-            pixel = raster_pixel(s, rx, ry);
-            out_rx = rx; out_ry = ry;
+    // workgroup-uniform loop: the camera-vertex connections append per workgroup
+    for (uint32_t base = blockIdx.x * blockDim.x; base < want; base += gridDim.x * blockDim.x) {
+        const uint32_t t = base + threadIdx.x;
+        NeeOut no;
+        no.emit = false;
+        uint32_t sample = 0, pixel = 0;
+        if (t < want) {
+            sample = first + t;
+            const uint32_t slot = base_slot + t;
+            const Uniforms<RParams> U{p, sample};
+            const float ux = U.one(LMX_PT_SLOT_RASTER), uy = U.one(LMX_PT_SLOT_RASTER + 1);
+            float rx, ry;
+            f3 wo;
+            uint32_t flags = 0;
+            float out_rx, out_ry;
+            if (p.raster_mode == LMX_RASTER_STRATIFIED) {
+                const uint32_t npix = (uint32_t)s.width * (uint32_t)s.height;
+                pixel = sample % npix;
+                const uint32_t px = pixel % (uint32_t)s.width, py = pixel / (uint32_t)s.width;
+                rx = ((float)px + ux) / (float)s.width;
+                ry = ((float)py + uy) / (float)s.height;
+                wo = camera_dir(s, rx, ry);
+                out_rx = ((float)px + 0.5f) / (float)s.width;
+                out_ry = ((float)py + 0.5f) / (float)s.height;
+            } else {
+                wo = camera_dir(s, ux, uy);
+                if (!camera_raster(s, wo, rx, ry)) { flags = META_DEAD; rx = ry = 0.0f; }
+                pixel = raster_pixel(s, rx, ry);
+                out_rx = rx; out_ry = ry;
+            }
+            if (flags & META_DEAD) { out_rx = 0.0f; out_ry = 0.0f; }
+            // thr.w: the direction's density at the camera (importance), for MIS at a light hit
+            out.ray[2 * slot] = make_float4(s.cam_pos.x, s.cam_pos.y, s.cam_pos.z, LMX_EPS_ISECT);
+            out.ray[2 * slot + 1] = make_float4(wo.x, wo.y, wo.z, (flags & META_DEAD) ? -1.0f : LMX_FLT_MAX);
+            out.thr[slot] = make_float4(1.0f, 1.0f, 1.0f, p.nee == NEE_MIS ? importance(s, wo) : 0.0f);
+            out.meta[slot] = make_uint4(sample, 0u, pixel, flags);
+            if (p.sample_out) {
+                float4* so = reinterpret_cast<float4*>(p.sample_out + p.so_stride * (size_t)sample);
+                so[0] = make_float4(out_rx, out_ry, 0.0f, 0.0f);
+                so[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (p.so_stride == 16) { so[2] = so[1]; so[3] = so[1]; }
+            }
+            // next-event estimation at the camera vertex
+            // (none for a sample whose raster position failed, renderer_ptmis.cpp:95-103)
+            if (p.nee != NEE_NONE && !(flags & META_DEAD) && (p.nee == NEE_DIRECT || 2 >= p.min_nv)) {
+                const float4 un = U.block(LMX_PT_SLOT_SEG0 + 4u);
+                Frame fr;
+                RMat m;
+                no = nee_connect(s, p, true, fr, m, mk3(0, 0, 0), mk3(1.0f, 1.0f, 1.0f), un.x, un.y, un.z);
+                if (no.emit) pixel = raster_pixel(s, no.rx, no.ry);
+            }
         }
-        if (flags & META_DEAD) { out_rx = 0.0f; out_ry = 0.0f; }
-        out.ray[2 * slot] = make_float4(s.cam_pos.x, s.cam_pos.y, s.cam_pos.z, LMX_EPS_ISECT);
-        out.ray[2 * slot + 1] = make_float4(wo.x, wo.y, wo.z, (flags & META_DEAD) ? -1.0f : LMX_FLT_MAX);
-        out.thr[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-        out.meta[slot] = make_uint4(sample, 0u, pixel, flags);
+        if (p.nee != NEE_NONE) nee_append(p, ctl, lds, no, sample, pixel, true);
+    }
+}
+
+// Splats the unoccluded next-event connections of the iteration.
+__global__ __launch_bounds__(256) void k_nee_splat(RParams p, Ctl* __restrict__ ctl)
+{
+    const uint32_t n = ctl->n_shadow;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        if (__float_as_uint(p.nq.hits[j].w) != WBVH_MISS) continue;          // occluded
+        const float4 c = p.nq.c[j];
+        const uint4 m = p.nq.meta[j];
+        float* fp = p.film + 3 * (size_t)m.y;
+        atomicAdd(fp, c.x); atomicAdd(fp + 1, c.y); atomicAdd(fp + 2, c.z);
         if (p.sample_out) {
-            float4* so = reinterpret_cast<float4*>(p.sample_out + 8 * (size_t)sample);
-            so[0] = make_float4(out_rx, out_ry, 0.0f, 0.0f);
-            so[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            float* so = p.sample_out + p.so_stride * (size_t)m.x;
+            if (m.z) {
+                so[8] = __uint_as_float(m.w); so[9] = p.nq.ry[j];
+                so[10] += c.x; so[11] += c.y; so[12] += c.z; so[13] += 1.0f;
+            } else {
+                so[2] += c.x; so[3] += c.y; so[4] += c.z; so[5] += 1.0f;
+            }
         }
     }
 }
@@ -170,6 +308,8 @@ __global__ void k_commit(Ctl* ctl, uint32_t cap, uint32_t total)
 {
     if (threadIdx.x != 0) return;
     const uint32_t want = gen_count(ctl, cap, total);
+    ctl->shadow_rays += ctl->n_shadow;
+    ctl->n_shadow = 0;
     ctl->rays += ctl->n_trace;
     ctl->n_trace = ctl->n_next + want;
     ctl->n_next = 0;
@@ -181,6 +321,7 @@ __global__ void k_init_ctl(Ctl* ctl)
 {
     if (threadIdx.x == 0) {
         ctl->n_trace = 0; ctl->n_next = 0; ctl->issued = 0; ctl->iterations = 0; ctl->rays = 0;
+        ctl->n_shadow = 0; ctl->shadow_rays = 0;
     }
 }
 
@@ -213,6 +354,29 @@ int ensure_queues(lmx_scene* sc, uint32_t cap)
     return LMX_OK;
 }
 
+int ensure_nee(lmx_scene* sc, uint32_t cap)
+{
+    if (sc->nee_cap >= cap) return LMX_OK;
+    hipError_t e;
+    const size_t n = 2 * (size_t)cap;
+    if ((e = sc->nee_ray.alloc(2 * n)) != hipSuccess || (e = sc->nee_c.alloc(n)) != hipSuccess ||
+        (e = sc->nee_hits.alloc(n)) != hipSuccess || (e = sc->nee_meta.alloc(n)) != hipSuccess ||
+        (e = sc->nee_ry.alloc(n)) != hipSuccess)
+        return hip_error(e, "shadow queue alloc");
+    sc->nee_cap = cap;
+    return LMX_OK;
+}
+
+// The next-event connections of one iteration: any-hit trace, then splat.
+int enqueue_nee(lmx_scene* sc, const RParams& rp, hipStream_t st)
+{
+    int rc = launch_trace(sc->accel, reinterpret_cast<const WRay*>(rp.nq.ray), 0, &sc->ctl.p->n_shadow,
+                          2 * sc->nee_cap, reinterpret_cast<WHit*>(rp.nq.hits), st, true);
+    if (rc != LMX_OK) return rc;
+    k_nee_splat<<<std::max(1, std::min((int)((2 * sc->nee_cap + 255) / 256), sc->cus * 8)), 256, 0, st>>>(rp, sc->ctl.p);
+    return LMX_OK;
+}
+
 // Records one wavefront iteration tracing queue `a` and filling queue `b`.
 int enqueue_iteration(lmx_scene* sc, const RParams& rp, int a, int b, hipStream_t st)
 {
@@ -228,6 +392,7 @@ int enqueue_iteration(lmx_scene* sc, const RParams& rp, int a, int b, hipStream_
         k_shade<<<grid, 256, 0, st>>>(sc->rs, rp, sc->ctl.p, qa, sc->hits.p, qb);
     }
     k_gen<<<grid, 256, 0, st>>>(sc->rs, rp, sc->ctl.p, sc->cap, qb);
+    if (rp.nee != NEE_NONE && (rc = enqueue_nee(sc, rp, st)) != LMX_OK) return rc;
     k_commit<<<1, 64, 0, st>>>(sc->ctl.p, sc->cap, rp.total);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? LMX_OK : hip_error(e, "wavefront launch");
@@ -235,14 +400,22 @@ int enqueue_iteration(lmx_scene* sc, const RParams& rp, int a, int b, hipStream_
 
 }  // namespace
 
-int wavefront_render(lmx_scene* sc, const RParams& rp, uint32_t cap, hipStream_t st, uint64_t* rays, uint32_t* iters)
+int wavefront_render(lmx_scene* sc, const RParams& rp_in, uint32_t cap, hipStream_t st, uint64_t* rays,
+                     uint32_t* iters)
 {
     int rc = ensure_queues(sc, cap);
     if (rc != LMX_OK) return rc;
+    RParams rp = rp_in;
+    if (rp.nee != NEE_NONE) {
+        if ((rc = ensure_nee(sc, cap)) != LMX_OK) return rc;
+        rp.nq = NeeQ{sc->nee_ray.p, sc->nee_c.p, sc->nee_meta.p, sc->nee_ry.p, sc->nee_hits.p};
+    }
     k_init_ctl<<<1, 64, 0, st>>>(sc->ctl.p);
-    // the first generation: queue 0 gets camera paths, then the counts commit
+    // the first generation: queue 0 gets camera paths (and their camera-vertex
+    // connections), then the counts commit
     k_gen<<<std::max(1, std::min((int)((cap + 255) / 256), sc->cus * 8)), 256, 0, st>>>(
         sc->rs, rp, sc->ctl.p, cap, Queue{sc->ray[0].p, sc->thr[0].p, sc->meta[0].p});
+    if (rp.nee != NEE_NONE && (rc = enqueue_nee(sc, rp, st)) != LMX_OK) return rc;
     k_commit<<<1, 64, 0, st>>>(sc->ctl.p, cap, rp.total);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "wavefront start");
@@ -278,7 +451,7 @@ int wavefront_render(lmx_scene* sc, const RParams& rp, uint32_t cap, hipStream_t
     }
     (void)hipGraphExecDestroy(exec);
     (void)hipGraphDestroy(graph);
-    if (rays) *rays = sc->h_ctl->rays;
+    if (rays) *rays = sc->h_ctl->rays + sc->h_ctl->shadow_rays;
     if (iters) *iters = sc->h_ctl->iterations;
     return rc;
 }
@@ -323,6 +496,74 @@ int autotune_capacity(lmx_scene* sc, const RParams& rp, hipStream_t st)
 }
 
 }  // namespace
+
+static int render_unidir(lmx_scene* sc, const lmx_render_params* p, int nee, const char* name, float* film_rgb, void* stream,
+                         lmx_render_stats* stats)
+{
+    if (!sc || !p || !film_rgb) return set_error(LMX_EINVAL, std::string(name) + ": NULL argument");
+    if (p->num_samples == 0 || p->num_samples > 0xffffffffull)
+        return set_error(LMX_EINVAL, std::string(name) + ": num_samples must be in [1, 2^32)");
+    if (p->max_num_vertices != -1 && p->max_num_vertices < 2)
+        return set_error(LMX_EINVAL, std::string(name) + ": max_num_vertices must be -1 or >= 2");
+    if (p->max_num_vertices == -1 && p->uniform_table)
+        return set_error(LMX_EINVAL, std::string(name) + ": a uniform table needs a bounded max_num_vertices");
+    const uint32_t stride = nee == NEE_NONE ? 4u : 8u;
+    if (p->uniform_table && p->table_width < (uint32_t)(LMX_PT_SLOT_SEG0 + stride * (p->max_num_vertices - 1)))
+        return set_error(LMX_EINVAL, std::string(name) + ": uniform table narrower than 8 + " + std::to_string(stride) +
+                                     " (max_num_vertices - 1)");
+    if (!(p->rr_prob > 0.0f && p->rr_prob <= 1.0f)) return set_error(LMX_EINVAL, std::string(name) + ": rr_prob must be in (0, 1]");
+    if (p->rr_mode != LMX_RR_FIXED && p->rr_mode != LMX_RR_THROUGHPUT)
+        return set_error(LMX_EINVAL, std::string(name) + ": unknown rr_mode");
+    if (p->raster_mode != LMX_RASTER_RANDOM && p->raster_mode != LMX_RASTER_STRATIFIED)
+        return set_error(LMX_EINVAL, std::string(name) + ": unknown raster_mode");
+    hipStream_t st = (hipStream_t)stream;
+    hipStream_t own = nullptr;
+    hipError_t e;
+    if (!st) {   // graph capture needs a non-legacy stream
+        if ((e = hipStreamCreateWithFlags(&own, hipStreamNonBlocking)) != hipSuccess) return hip_error(e, "stream create");
+        st = own;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t npix = (size_t)sc->width * (size_t)sc->height;
+    RParams rp;
+    rp.max_nv = p->max_num_vertices; rp.min_nv = p->min_num_vertices; rp.rr_depth = p->rr_depth;
+    rp.rr_prob = p->rr_prob;
+    rp.key = make_uint2((uint32_t)p->seed, (uint32_t)(p->seed >> 32));
+    rp.total = (uint32_t)p->num_samples;
+    rp.raster_mode = p->raster_mode;
+    rp.table = p->uniform_table; rp.table_width = p->table_width;
+    rp.sample_out = p->sample_out;
+    rp.film = film_rgb;
+    rp.guide = nullptr;
+    rp.rr_mode = p->rr_mode;
+    rp.nee = nee;
+    rp.seg_stride = stride;
+    rp.so_stride = nee == NEE_NONE ? 8u : 16u;
+    const int zgrid = std::max(1, std::min((int)((3 * npix + 255) / 256), sc->cus * 8));
+    int rc = LMX_OK;
+    if (p->queue_capacity == 0 && sc->tuned_cap == 0) rc = autotune_capacity(sc, rp, st);
+    uint32_t cap = p->queue_capacity ? p->queue_capacity : sc->tuned_cap;
+    cap = (uint32_t)std::min<uint64_t>(cap, std::max<uint64_t>(p->num_samples, 64));
+    uint64_t rays = 0;
+    uint32_t iters = 0;
+    if (rc == LMX_OK) {
+        k_zero<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix);
+        rc = wavefront_render(sc, rp, cap, st, &rays, &iters);
+    }
+    if (rc == LMX_OK) {
+        k_scale<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix, (float)((double)npix / (double)p->num_samples));
+        if ((e = hipStreamSynchronize(st)) != hipSuccess) rc = hip_error(e, "render sync");
+    }
+    if (own) (void)hipStreamDestroy(own);
+    if (stats) {
+        stats->rays = rays;
+        stats->iterations = iters;
+        stats->queue_capacity = cap;
+        stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return rc;
+}
+
 
 extern "C" {
 
@@ -428,65 +669,21 @@ int lmx_scene_create(const lmx_scene_desc* d, lmx_scene** out)
 
 void lmx_scene_free(lmx_scene* s) { delete s; }
 
+
 int lmx_render_pt(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, void* stream, lmx_render_stats* stats)
 {
-    if (!sc || !p || !film_rgb) return set_error(LMX_EINVAL, "lmx_render_pt: NULL argument");
-    if (p->num_samples == 0 || p->num_samples > 0xffffffffull)
-        return set_error(LMX_EINVAL, "lmx_render_pt: num_samples must be in [1, 2^32)");
-    if (p->max_num_vertices != -1 && p->max_num_vertices < 2)
-        return set_error(LMX_EINVAL, "lmx_render_pt: max_num_vertices must be -1 or >= 2");
-    if (p->max_num_vertices == -1 && p->uniform_table)
-        return set_error(LMX_EINVAL, "lmx_render_pt: a uniform table needs a bounded max_num_vertices");
-    if (p->uniform_table && p->table_width < (uint32_t)(LMX_PT_SLOT_SEG0 + 4 * (p->max_num_vertices - 1)))
-        return set_error(LMX_EINVAL, "lmx_render_pt: uniform table narrower than 8 + 4 (max_num_vertices - 1)");
-    if (!(p->rr_prob > 0.0f && p->rr_prob <= 1.0f)) return set_error(LMX_EINVAL, "lmx_render_pt: rr_prob must be in (0, 1]");
-    if (p->rr_mode != LMX_RR_FIXED && p->rr_mode != LMX_RR_THROUGHPUT)
-        return set_error(LMX_EINVAL, "lmx_render_pt: unknown rr_mode");
-    if (p->raster_mode != LMX_RASTER_RANDOM && p->raster_mode != LMX_RASTER_STRATIFIED)
-        return set_error(LMX_EINVAL, "lmx_render_pt: unknown raster_mode");
-    hipStream_t st = (hipStream_t)stream;
-    hipStream_t own = nullptr;
-    hipError_t e;
-    if (!st) {   // graph capture needs a non-legacy stream
-        if ((e = hipStreamCreateWithFlags(&own, hipStreamNonBlocking)) != hipSuccess) return hip_error(e, "stream create");
-        st = own;
-    }
-    const auto t0 = std::chrono::steady_clock::now();
-    const size_t npix = (size_t)sc->width * (size_t)sc->height;
-    RParams rp;
-    rp.max_nv = p->max_num_vertices; rp.min_nv = p->min_num_vertices; rp.rr_depth = p->rr_depth;
-    rp.rr_prob = p->rr_prob;
-    rp.key = make_uint2((uint32_t)p->seed, (uint32_t)(p->seed >> 32));
-    rp.total = (uint32_t)p->num_samples;
-    rp.raster_mode = p->raster_mode;
-    rp.table = p->uniform_table; rp.table_width = p->table_width;
-    rp.sample_out = p->sample_out;
-    rp.film = film_rgb;
-    rp.guide = nullptr;
-    rp.rr_mode = p->rr_mode;
-    const int zgrid = std::max(1, std::min((int)((3 * npix + 255) / 256), sc->cus * 8));
-    int rc = LMX_OK;
-    if (p->queue_capacity == 0 && sc->tuned_cap == 0) rc = autotune_capacity(sc, rp, st);
-    uint32_t cap = p->queue_capacity ? p->queue_capacity : sc->tuned_cap;
-    cap = (uint32_t)std::min<uint64_t>(cap, std::max<uint64_t>(p->num_samples, 64));
-    uint64_t rays = 0;
-    uint32_t iters = 0;
-    if (rc == LMX_OK) {
-        k_zero<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix);
-        rc = wavefront_render(sc, rp, cap, st, &rays, &iters);
-    }
-    if (rc == LMX_OK) {
-        k_scale<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix, (float)((double)npix / (double)p->num_samples));
-        if ((e = hipStreamSynchronize(st)) != hipSuccess) rc = hip_error(e, "lmx_render_pt: sync");
-    }
-    if (own) (void)hipStreamDestroy(own);
-    if (stats) {
-        stats->rays = rays;
-        stats->iterations = iters;
-        stats->queue_capacity = cap;
-        stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    }
-    return rc;
+    return render_unidir(sc, p, NEE_NONE, "lmx_render_pt", film_rgb, stream, stats);
+}
+
+int lmx_render_ptdirect(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, void* stream,
+                        lmx_render_stats* stats)
+{
+    return render_unidir(sc, p, NEE_DIRECT, "lmx_render_ptdirect", film_rgb, stream, stats);
+}
+
+int lmx_render_ptmis(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, void* stream, lmx_render_stats* stats)
+{
+    return render_unidir(sc, p, NEE_MIS, "lmx_render_ptmis", film_rgb, stream, stats);
 }
 
 }  // extern "C"

@@ -52,6 +52,9 @@ struct Ctl {
     uint32_t issued;           // camera samples started
     uint32_t iterations;
     unsigned long long rays;   // extension rays traced so far
+    uint32_t n_shadow;         // next-event shadow rays of the iteration
+    uint32_t pad;
+    unsigned long long shadow_rays;
 };
 
 struct Queue {
@@ -63,6 +66,19 @@ struct Queue {
 #define META_DEAD 1u
 
 struct GuideLaunch;            // the learned sampler's state for a shading launch
+
+// Next-event estimation: shadow rays of one iteration (at most one per path
+// shaded plus one per camera path started)
+struct NeeQ {
+    float4* ray;               // 2 per shadow ray
+    float4* c;                 // weighted contribution rgb, 0
+    uint4* meta;               // sample, pixel, camera-vertex flag, raster x bits
+    float* ry;                 // raster y of camera-vertex connections
+    float4* hits;
+};
+#define NEE_NONE 0
+#define NEE_DIRECT 1           // renderer::ptdirect: light sampling only
+#define NEE_MIS 2              // renderer::ptmis: light and BSDF sampling, balance heuristic
 
 // Parameters of the unidirectional wavefront sampler (plain and guided).
 struct RParams {
@@ -77,6 +93,10 @@ struct RParams {
     float* film;               // W x H x 3 accumulation
     const GuideLaunch* guide;  // null: BSDF sampling only
     int32_t rr_mode;           // LMX_RR_*
+    int32_t nee;               // NEE_*
+    uint32_t seg_stride;       // uniform slots per vertex (4; 8 with next-event estimation)
+    uint32_t so_stride;        // floats per sample_out record (8; 16 with next-event estimation)
+    NeeQ nq;
 };
 
 // Russian-roulette continuation probability of a path with throughput thr
@@ -256,6 +276,49 @@ RD float importance(const RScene& s, f3 wo)
     return inv * inv * inv / A;
 }
 
+RD f3 dir_to(f3 from, f3 to) { return normalize(sub(to, from)); }
+
+// RenderUtils::GeometryTerm (renderutils.h:46-56)
+RD float gterm(f3 pa, f3 na, bool dega, f3 pb, f3 nb, bool degb)
+{
+    f3 d = sub(pb, pa);
+    const float l2 = dot(d, d);
+    const float l = sqrtf(l2);
+    d = mk3(d.x / l, d.y / l, d.z / l);
+    float t = 1.0f;
+    if (!dega) t *= fabsf(dot(na, d));
+    if (!degb) t *= fabsf(dot(nb, neg(d)));
+    return t / l2;
+}
+RD float pos_pdf_light(const RScene& s, int l) { return s.lights[l].inv_area * (1.0f / (float)s.nlight); }
+
+// Area-light emission: Le one-sided (light_area.cpp:105-110); direction pdf
+// 1/pi in projected solid angle when above the surface (:84-89).
+RD bool emits_towards(const Frame& f, f3 wo) { return to_local(f, wo).z > 0.0f; }
+
+// Light point: Distribution1D::SampleReuse + UniformSampleTriangle
+// (triangleutils.h:73-120)
+RD void sample_light_point(const RScene& s, int l, float u0, float u1, f3& p, f3& gn, uint32_t& tri)
+{
+    const RLight L = s.lights[l];
+    const float* cdf = s.light_cdf + L.cdf_off;
+    uint32_t lo = 0, hi = L.count + 1;          // upper_bound over cdf[0..count]
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cdf[mid] <= u0) lo = mid + 1; else hi = mid;
+    }
+    int i = (int)lo - 1;
+    i = i < 0 ? 0 : (i > (int)L.count - 1 ? (int)L.count - 1 : i);
+    const float u2 = (u0 - cdf[i]) / (cdf[i + 1] - cdf[i]);
+    float b0, b1;
+    uniform_triangle(u2, u1, b0, b1);
+    tri = L.first + (uint32_t)i;
+    const uint32_t i0 = s.tris[3 * tri], i1 = s.tris[3 * tri + 1], i2 = s.tris[3 * tri + 2];
+    const f3 p1 = ld3(s.verts + 3 * i0), p2 = ld3(s.verts + 3 * i1), p3 = ld3(s.verts + 3 * i2);
+    p = add(add(mul(p1, 1.0f - b0 - b1), mul(p2, b0)), mul(p3, b1));
+    gn = normalize(cross(sub(p2, p1), sub(p3, p1)));
+}
+
 }  // namespace lmx
 
 struct lmx_scene;
@@ -346,6 +409,11 @@ struct lmx_scene {
     DevBuf<float4> ray[2], thr[2];
     DevBuf<uint4> meta[2];
     DevBuf<float4> hits;
+    // next-event shadow queue (2 x cap entries), allocated on first use
+    DevBuf<float4> nee_ray, nee_c, nee_hits;
+    DevBuf<uint4> nee_meta;
+    DevBuf<float> nee_ry;
+    uint32_t nee_cap = 0;
     DevBuf<Ctl> ctl;
     Ctl* h_ctl = nullptr;      // pinned
     uint32_t tuned_cap = 0;

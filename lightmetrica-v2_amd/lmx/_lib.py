@@ -101,6 +101,10 @@ SIGNATURES = {
                                      ctypes.c_void_p, ctypes.POINTER(lmx_render_stats)]),
     "lmx_render_bdpt": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(lmx_render_params), ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.POINTER(lmx_render_stats)]),
+    "lmx_render_ptdirect": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(lmx_render_params), ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.POINTER(lmx_render_stats)]),
+    "lmx_render_ptmis": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(lmx_render_params), ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.POINTER(lmx_render_stats)]),
     "lmx_guide_default_opts": (None, [ctypes.POINTER(lmx_guide_opts)]),
     "lmx_guide_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(lmx_guide_opts),
                                         ctypes.POINTER(ctypes.c_void_p)]),

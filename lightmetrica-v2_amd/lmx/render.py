@@ -23,9 +23,10 @@ RR_THROUGHPUT = 1
 SLOT_SEG0 = 8
 
 
-def table_width(max_num_vertices):
-    """Uniform-table width of renderer::pt for this path-length bound."""
-    return SLOT_SEG0 + 4 * (max_num_vertices - 1)
+def table_width(max_num_vertices, nee=False):
+    """Uniform-table width of renderer::pt (nee: ptdirect / ptmis) for this
+    path-length bound."""
+    return SLOT_SEG0 + (8 if nee else 4) * (max_num_vertices - 1)
 
 
 def bdpt_table_width(max_num_vertices):
@@ -123,9 +124,18 @@ class Scene:
         stats = {"rays": st.rays, "batches": st.iterations, "batch": st.queue_capacity, "ms": st.ms}
         return film, stats, so
 
+    def render_ptdirect(self, num_samples, **kw):
+        """renderer::ptdirect: renderer::pt with light sampling at every
+        vertex (records (n, 16) with sample_out)."""
+        return self.render_pt(num_samples, _entry="lmx_render_ptdirect", **kw)
+
+    def render_ptmis(self, num_samples, **kw):
+        """renderer::ptmis: light and BSDF sampling, balance heuristic."""
+        return self.render_pt(num_samples, _entry="lmx_render_ptmis", **kw)
+
     def render_pt(self, num_samples, max_num_vertices=9, min_num_vertices=0, rr_depth=1, rr_prob=0.5,
                   seed=42, raster_mode=RASTER_STRATIFIED, queue_capacity=0, table=None, sample_out=False,
-                  stream=None, rr_mode=RR_FIXED):
+                  stream=None, rr_mode=RR_FIXED, _entry="lmx_render_pt"):
         """Returns (film as a torch tensor (H, W, 3) on the GPU, stats dict,
         per-sample records (n, 8) or None).  `table`: torch float32 tensor
         (num_samples, width) on the GPU, or None for Philox."""
@@ -149,7 +159,7 @@ class Scene:
             p.uniform_table = ctypes.c_void_p(table.data_ptr())
             p.table_width = table.shape[1]
         if sample_out:
-            so = torch.zeros((num_samples, 8), dtype=torch.float32, device=dev)
+            so = torch.zeros((num_samples, 8 if _entry == "lmx_render_pt" else 16), dtype=torch.float32, device=dev)
             p.sample_out = ctypes.c_void_p(so.data_ptr())
         st = lmx_render_stats()
         if stream is None:
@@ -157,7 +167,7 @@ class Scene:
             # stream; order it after the work already queued on torch's
             stream = torch.cuda.Stream(dev)
             stream.wait_stream(torch.cuda.current_stream(dev))
-        check(lib.lmx_render_pt(self._h, ctypes.byref(p), ctypes.c_void_p(film.data_ptr()),
-                                ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st)))
+        check(getattr(lib, _entry)(self._h, ctypes.byref(p), ctypes.c_void_p(film.data_ptr()),
+                                   ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st)))
         stats = {"rays": st.rays, "iterations": st.iterations, "queue_capacity": st.queue_capacity, "ms": st.ms}
         return film, stats, so

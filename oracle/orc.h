@@ -172,6 +172,10 @@ void orc_scene_free(orc_scene *s);
 void orc_pt_samples(const orc_scene *s, const orc_pt_params *p, const float *table, uint32_t table_width,
                     uint64_t first_sample, uint64_t n, float *out, int nthreads);
 void orc_film_splat(const float *samples, uint64_t n, int32_t width, int32_t height, double *film);
+/* renderer::ptdirect (mode 1) and renderer::ptmis (mode 2): n x 16 floats,
+ * layout in orc_render.c (nee_sample) */
+void orc_ptnee_samples(const orc_scene *s, const orc_pt_params *p, int mode, const float *table,
+                       uint32_t table_width, uint64_t first_sample, uint64_t n, float *out, int nthreads);
 /* renderer::vcm, mode bdpt (orc_render.c).  max_num_vertices >= 2 bounds
  * both subpaths and every connected path.  Output per sample: (maxv+1)^2
  * records of 8 floats indexed t (maxv+1) + s: raster x, y, C rgb (= f w / p,

@@ -204,6 +204,9 @@ def _render_sigs(so):
     so.orc_bdpt_samples.argtypes = [vp, ctypes.POINTER(orc_bdpt_params), vp, ctypes.c_uint32, ctypes.c_uint64,
                                     ctypes.c_uint64, vp, ctypes.c_int]
     so.orc_bdpt_samples.restype = None
+    so.orc_ptnee_samples.argtypes = [vp, ctypes.POINTER(orc_pt_params), ctypes.c_int, vp, ctypes.c_uint32,
+                                     ctypes.c_uint64, ctypes.c_uint64, vp, ctypes.c_int]
+    so.orc_ptnee_samples.restype = None
     so.orc_bsdf_query.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     so.orc_bsdf_eval.argtypes = [vp, vp, vp, vp, vp, vp]
     so.orc_bsdf_eval.restype = None
@@ -317,6 +320,22 @@ class Scene:
             tw = table.shape[1]
             tp = ctypes.c_void_p(table.ctypes.data)
         lib().orc_pt_samples(self._h, ctypes.byref(p), tp, tw, first, n, _p(out), nthreads)
+        return out
+
+    def nee_samples(self, first, n, mode, max_num_vertices=9, min_num_vertices=0, rr_depth=1, rr_prob=0.5,
+                    seed=42, raster_mode=RASTER_STRATIFIED, table=None, nthreads=8, rr_mode=0):
+        """renderer::ptdirect (mode 1) / renderer::ptmis (mode 2): (n, 16)
+        float32 -- raster x, y, C rgb, count, 0, 0, then the camera-vertex
+        connection's raster x, y, C rgb, count, 0, 0."""
+        p = orc_pt_params(max_num_vertices, min_num_vertices, rr_depth, rr_prob, seed & 0xFFFFFFFF,
+                          seed >> 32, raster_mode, rr_mode)
+        out = np.zeros((n, 16), np.float32)
+        tw, tp = 0, None
+        if table is not None:
+            table = np.ascontiguousarray(table, np.float32)
+            tw = table.shape[1]
+            tp = ctypes.c_void_p(table.ctypes.data)
+        lib().orc_ptnee_samples(self._h, ctypes.byref(p), int(mode), tp, tw, first, n, _p(out), nthreads)
         return out
 
     def bdpt_samples(self, first, n, max_num_vertices=10, min_num_vertices=0, seed=42,

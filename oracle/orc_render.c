@@ -868,3 +868,192 @@ void orc_bdpt_samples(const orc_scene *s, const orc_bdpt_params *pp, const float
     for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
     free(b.tri_light);
 }
+
+/* ---------------------------------------------------------------------------
+ * renderer::ptdirect (mode 1, renderer_ptdirect.cpp:72-297) and
+ * renderer::ptmis (mode 2, renderer_ptmis.cpp:69-326): renderer::pt with
+ * next-event estimation at every vertex, the camera's included.  Uniform
+ * block of vertex v at 8 + 8 v: direction 0-1, roulette after the hit of
+ * segment v 3, light selection 4, light position 5-6.  out[16]: raster x, y,
+ * C rgb, count (connections and hits splatted at the sample's pixel), 0, 0,
+ * raster x, y, C rgb, count of the camera-vertex connection, 0, 0. */
+static void nee_sample(const bctx *b, const orc_pt_params *pp, int mode, const usrc *us, uint64_t sample,
+                       float out[16])
+{
+    const orc_scene *s = b->s;
+    const camera *cam = &b->cam;
+    memset(out, 0, 16 * sizeof(float));
+    float rx, ry;
+    v3 wo;
+    const float ux = U(us, sample, ORC_PT_RASTER), uy = U(us, sample, ORC_PT_RASTER + 1);
+    if (pp->raster_mode == ORC_RASTER_STRATIFIED) {
+        const uint64_t npix = (uint64_t)s->d.width * (uint64_t)s->d.height;
+        const uint32_t pix = (uint32_t)(sample % npix);
+        const uint32_t px = pix % (uint32_t)s->d.width, py = pix / (uint32_t)s->d.width;
+        rx = ((float)px + ux) / (float)s->d.width;
+        ry = ((float)py + uy) / (float)s->d.height;
+        wo = camera_dir(cam, rx, ry);
+        out[0] = ((float)px + 0.5f) / (float)s->d.width;
+        out[1] = ((float)py + 0.5f) / (float)s->d.height;
+    } else {
+        wo = camera_dir(cam, ux, uy);
+        if (!camera_raster(cam, wo, &rx, &ry)) return;           /* renderer_ptmis.cpp:95-103 */
+        out[0] = rx; out[1] = ry;
+    }
+    float thr[3] = {1.0f, 1.0f, 1.0f};
+    bvert cur;
+    memset(&cur, 0, sizeof cur);
+    cur.kind = BV_CAMERA; cur.degenerate = 1; cur.fr.p = cam->pos; cur.light = -1;
+    v3 wi = mk(0, 0, 0);
+    int nv = 1;
+    for (int v = 0;; v++) {
+        if (pp->max_num_vertices != -1 && nv >= pp->max_num_vertices) break;
+        const uint32_t base = ORC_PT_SEG0 + 8u * (uint32_t)v;
+        const int scat = cur.kind == BV_CAMERA ||
+                         (cur.mat && (cur.mat->type == ORC_MAT_DIFFUSE || cur.mat->type == ORC_MAT_PHONG));
+        if (!scat) break;
+        /* direct light sampling (ptmis :130-195, ptdirect :109-150) */
+        if (s->d.nlight > 0 && (mode == 1 || nv + 1 >= pp->min_num_vertices)) {
+            const float uL = U(us, sample, base + 4);
+            const float up[2] = {U(us, sample, base + 5), U(us, sample, base + 6)};
+            int l = (int)(uL * (float)s->d.nlight);
+            l = l < 0 ? 0 : (l > (int)s->d.nlight - 1 ? (int)s->d.nlight - 1 : l);
+            bvert L;
+            memset(&L, 0, sizeof L);
+            sample_light_point(b, l, up, &L);
+            const v3 ppL = dir_to(&cur, &L);
+            float fsE[3], pB;
+            if (cur.kind == BV_CAMERA) {
+                const float W = importance(cam, ppL);
+                fsE[0] = fsE[1] = fsE[2] = W;
+                pB = W;
+            } else {
+                bsdf_eval_dir(cur.mat, &cur.fr, wi, ppL, 0, fsE);
+                pB = bsdf_pdf(cur.mat, &cur.fr, wi, ppL);
+            }
+            const float G = gterm(&cur, &L);
+            const float inv_pl = (float)s->d.nlight / s->inv_area[l];
+            float C[3];
+            const int emits = vdot(L.fr.gn, vneg(ppL)) > 0.0f;
+            for (int c = 0; c < 3; c++) C[c] = emits ? thr[c] * fsE[c] * G * L.mat->Le[c] * inv_pl : 0.0f;
+            /* black contributions are dropped before the weight (:171) */
+            if (mode == 2 && !(C[0] == 0.0f && C[1] == 0.0f && C[2] == 0.0f)) {
+                const float pL = s->inv_area[l] / G * (1.0f / (float)s->d.nlight);
+                const float w = pL / (pL + pB);
+                for (int c = 0; c < 3; c++) C[c] *= w;
+            }
+            if (!(C[0] == 0.0f && C[1] == 0.0f && C[2] == 0.0f)) {
+                float r2x = 0.0f, r2y = 0.0f;
+                int ok = 1;
+                if (cur.kind == BV_CAMERA) ok = camera_raster(cam, ppL, &r2x, &r2y);
+                /* Scene3::Visible (scene3.h:107-116) */
+                v3 d = vsub(L.fr.p, cur.fr.p);
+                const float len = sqrtf(vdot(d, d));
+                d = mk(d.x / len, d.y / len, d.z / len);
+                orc_ray r;
+                r.o[0] = cur.fr.p.x; r.o[1] = cur.fr.p.y; r.o[2] = cur.fr.p.z; r.tmin = EPS_ISECT;
+                r.d[0] = d.x; r.d[1] = d.y; r.d[2] = d.z; r.tmax = len * (1.0f - EPS_ISECT);
+                orc_hit h;
+                orc_qbvh_intersect_ex(s->q, &r, 1, &h, ORC_TEST_WATERTIGHT, ORC_TIE_MIN_ID, 1);
+                if (ok && h.id == 0xffffffffu) {
+                    if (cur.kind == BV_CAMERA) {
+                        out[8] = r2x; out[9] = r2y;
+                        for (int c = 0; c < 3; c++) out[10 + c] += C[c];
+                        out[13] += 1.0f;
+                    } else {
+                        for (int c = 0; c < 3; c++) out[2 + c] += C[c];
+                        out[5] += 1.0f;
+                    }
+                }
+            }
+        }
+        /* next direction; qB = its density (projected solid angle) times the
+         * cosine at this vertex (1 at the pinhole) */
+        float qB;
+        if (cur.kind == BV_CAMERA) {
+            qB = importance(cam, wo);
+        } else {
+            const float u2[2] = {U(us, sample, base), U(us, sample, base + 1)};
+            if (!bsdf_sample(cur.mat, &cur.fr, wi, u2, &wo)) break;
+            const float pdf = bsdf_pdf(cur.mat, &cur.fr, wi, wo);
+            float fs[3];
+            bsdf_eval(cur.mat, &cur.fr, wi, wo, fs);
+            if (fs[0] == 0.0f && fs[1] == 0.0f && fs[2] == 0.0f) break;
+            for (int c = 0; c < 3; c++) thr[c] *= fs[c] / pdf;
+            qB = pdf * fabsf(to_local(&cur.fr, wo).z);
+        }
+        orc_hit h;
+        if (!scene_intersect(s, cur.fr.p, wo, &h)) break;
+        frame fr;
+        surface_at(s, cur.fr.p, wo, &h, &fr);
+        const orc_material *mat = &s->d.mats[s->d.tri_mat[h.id]];
+        /* light hit (ptmis :229-257; ptdirect ignores it) */
+        if (mode == 2 && (mat->Le[0] > 0.0f || mat->Le[1] > 0.0f || mat->Le[2] > 0.0f) &&
+            nv + 1 >= pp->min_num_vertices) {
+            const float cl = to_local(&fr, vneg(wo)).z;
+            if (cl > 0.0f) {
+                const int l = b->tri_light[h.id];
+                const float pL = s->inv_area[l] * h.t * h.t / cl * (1.0f / (float)s->d.nlight);
+                const float w = qB / (qB + pL);
+                for (int c = 0; c < 3; c++) out[2 + c] += thr[c] * mat->Le[c] * w;
+                out[5] += 1.0f;
+            }
+        }
+        /* Russian roulette (ptmis :261-269), from depth rr_depth on */
+        if (v + 1 >= pp->rr_depth) {
+            float q = pp->rr_prob;
+            if (pp->rr_mode == 1) {
+                q = fmaxf(thr[0], fmaxf(thr[1], thr[2]));
+                q = q < 1.0f ? q : 1.0f;
+            }
+            if (!(q > 0.0f) || U(us, sample, base + 3) > q) break;
+            for (int c = 0; c < 3; c++) thr[c] /= q;
+        }
+        wi = vneg(wo);
+        cur.kind = BV_SURFACE; cur.degenerate = 0; cur.fr = fr; cur.mat = mat; cur.tri = h.id;
+        cur.light = b->tri_light[h.id];
+        nv++;
+    }
+}
+
+typedef struct { bctx *b; const orc_pt_params *pp; int mode; usrc us; uint64_t first; float *out; } nee_job_ctx;
+typedef struct { nee_job_ctx *c; uint64_t b, e; } nee_job;
+
+static void *nee_run(void *p)
+{
+    nee_job *j = (nee_job *)p;
+    for (uint64_t i = j->b; i < j->e; i++)
+        nee_sample(j->c->b, j->c->pp, j->c->mode, &j->c->us, j->c->first + i, &j->c->out[(size_t)16 * i]);
+    return NULL;
+}
+
+void orc_ptnee_samples(const orc_scene *s, const orc_pt_params *pp, int mode, const float *table,
+                       uint32_t table_width, uint64_t first_sample, uint64_t n, float *out, int nthreads)
+{
+    bctx b;
+    b.s = s;
+    camera_init(&s->d, &b.cam);
+    b.tri_light = (int *)malloc(sizeof(int) * (s->d.ntri ? s->d.ntri : 1));
+    for (uint32_t i = 0; i < s->d.ntri; i++) b.tri_light[i] = -1;
+    for (uint32_t l = 0; l < s->d.nlight; l++)
+        for (uint32_t k = 0; k < s->d.light_prims[2 * l + 1]; k++) b.tri_light[s->d.light_prims[2 * l] + k] = (int)l;
+    b.tri_cdf_base = NULL;
+    nee_job_ctx c;
+    c.b = &b; c.pp = pp; c.mode = mode;
+    c.us.table = table; c.us.width = table_width;
+    c.us.seed = ((uint64_t)pp->seed_hi << 32) | pp->seed_lo;
+    c.us.first = first_sample;
+    c.first = first_sample; c.out = out;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256];
+    nee_job jobs[256];
+    for (int t = 0; t < nthreads; t++) {
+        jobs[t].c = &c;
+        jobs[t].b = n * (uint64_t)t / (uint64_t)nthreads;
+        jobs[t].e = n * (uint64_t)(t + 1) / (uint64_t)nthreads;
+        pthread_create(&th[t], NULL, nee_run, &jobs[t]);
+    }
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    free(b.tri_light);
+}

@@ -120,3 +120,32 @@ def test_pt_furnace_throughput_roulette(rr_depth):
     out = orc.Scene(sc).pt_samples(0, 32 * 32 * 256, max_num_vertices=maxv, rr_depth=rr_depth, seed=6, rr_mode=1)
     expect = Le * sum(rho ** j for j in range(maxv - 1))
     assert abs(out[:, 2:5].mean() - expect) < 0.01 * expect
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("maxv,rr_depth", [(4, 1), (7, 3)])
+def test_nee_furnace_closed_form(mode, maxv, rr_depth):
+    """renderer::ptdirect (1) and renderer::ptmis (2) inside the emitting,
+    reflecting sphere: every path length k carries Le rho^(k-2), found by
+    light sampling (and, for ptmis, by BSDF hits weighted against it)."""
+    rho, Le = 0.5, 1.0
+    sc = scenes.furnace_sphere(albedo=rho, Le=Le, width=32, height=32)
+    out = orc.Scene(sc).nee_samples(0, 32 * 32 * 128, mode, max_num_vertices=maxv, rr_depth=rr_depth, seed=8)
+    expect = Le * sum(rho ** j for j in range(maxv - 1))
+    img = out[:, 2:5].mean() + out[:, 10:13].mean()
+    assert abs(img - expect) < 0.01 * expect, (img, expect)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_nee_cornell_matches_pt(mode):
+    """Next-event estimation and plain path tracing estimate the same image."""
+    sc = scenes.cornell_box(32, 32)
+    o = orc.Scene(sc)
+    n = 32 * 32 * 256
+    a = o.nee_samples(0, n, mode, max_num_vertices=9, seed=3)
+    ma = (a[:, 2:5].astype(np.float64) + a[:, 10:13]).mean(axis=1)
+    b = o.pt_samples(0, 32 * 32 * 4096, max_num_vertices=9, seed=4)[:, 2:5].astype(np.float64).mean(axis=1)
+    se = np.hypot(ma.std() / np.sqrt(len(ma)), b.std() / np.sqrt(len(b)))
+    assert abs(ma.mean() - b.mean()) < 3.5 * se, (ma.mean(), b.mean(), se)
+    # and light sampling is far less noisy than BSDF sampling here
+    assert ma.std() < 0.5 * b.std()

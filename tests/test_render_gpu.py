@@ -121,3 +121,66 @@ def test_pt_cornell_image_within_3_sigma(gpu):
     lit = (mg + mc) > 0
     assert lit.mean() > 0.5
     assert (z[lit] <= 3.0).mean() > 0.99, (z[lit] > 3).mean()
+
+
+def _compare_nee(g, c, rtol=1e-4):
+    """(n, 16) records of renderer::ptdirect / ptmis: the pixel part (0-5)
+    and the camera-vertex connection (8-13) must agree like _compare's."""
+    ok = _compare(g[:, 0:8], c[:, 0:8], rtol)
+    cam_g, cam_c = g[:, 13] > 0, c[:, 13] > 0
+    same = cam_g == cam_c
+    close = np.all(np.abs(g[:, 10:13] - c[:, 10:13]) <= rtol * np.maximum(np.abs(c[:, 10:13]), 1e-6), axis=1)
+    ras = np.all(~cam_c[:, None] | (np.abs(g[:, 8:10] - c[:, 8:10]) <= 1e-5), axis=1)
+    return ok & same & close & ras
+
+
+@pytest.mark.parametrize("entry,mode", [("ptdirect", 1), ("ptmis", 2)])
+@pytest.mark.parametrize("raster_mode", [render.RASTER_STRATIFIED, render.RASTER_RANDOM])
+def test_nee_per_sample_parity_uniform_table(gpu, entry, mode, raster_mode):
+    import torch
+    sc = scenes.cornell_box(32, 32, phong_boxes=20.0)
+    n, m = 32 * 32 * 16, 9
+    tab = np.random.default_rng(21).random((n, render.table_width(m, nee=True)), dtype=np.float32)
+    rs = render.Scene(sc)
+    _, _, so = getattr(rs, "render_" + entry)(n, max_num_vertices=m, raster_mode=raster_mode,
+                                              table=torch.from_numpy(tab).to(gpu), sample_out=True)
+    c = orc.Scene(sc).nee_samples(0, n, mode, max_num_vertices=m, raster_mode=raster_mode, table=tab)
+    assert (c[:, 5] > 0).sum() > 1000 and (c[:, 13] > 0).sum() > 10
+    ok = _compare_nee(so.cpu().numpy(), c)
+    assert ok.mean() > 0.998, f"{(~ok).sum()} of {n} samples differ"
+
+
+@pytest.mark.parametrize("entry,mode", [("ptdirect", 1), ("ptmis", 2)])
+def test_nee_per_sample_parity_philox(gpu, entry, mode):
+    sc = scenes.cornell_box(64, 64)
+    n = 64 * 64 * 8
+    _, _, so = getattr(render.Scene(sc), "render_" + entry)(n, max_num_vertices=9, seed=77, sample_out=True,
+                                                            rr_depth=2)
+    c = orc.Scene(sc).nee_samples(0, n, mode, max_num_vertices=9, seed=77, rr_depth=2)
+    ok = _compare_nee(so.cpu().numpy(), c)
+    assert ok.mean() > 0.998, f"{(~ok).sum()} of {n} samples differ"
+
+
+@pytest.mark.parametrize("entry", ["ptdirect", "ptmis"])
+def test_nee_furnace_closed_form(gpu, entry):
+    rho, Le, maxv = 0.5, 1.0, 6
+    sc = scenes.furnace_sphere(albedo=rho, Le=Le, width=32, height=32)
+    film, _, _ = getattr(render.Scene(sc), "render_" + entry)(32 * 32 * 256, max_num_vertices=maxv, rr_depth=2,
+                                                              seed=9)
+    expect = Le * sum(rho ** j for j in range(maxv - 1))
+    assert abs(float(film.mean()) - expect) < 0.01 * expect
+
+
+def test_nee_images_agree_with_pt(gpu):
+    sc = scenes.cornell_box(32, 32)
+    rs = render.Scene(sc)
+    def batches(fn, k):
+        a = np.stack([fn(s).cpu().numpy().astype(np.float64) for s in range(k)])
+        return a.mean(0), a.std(0, ddof=1) / np.sqrt(k)
+    mp, sp = batches(lambda s: rs.render_pt(32 * 32 * 4096, max_num_vertices=9, seed=50 + s)[0], 16)
+    for entry in ("ptdirect", "ptmis"):
+        mn, sn = batches(lambda s: getattr(rs, "render_" + entry)(32 * 32 * 256, max_num_vertices=9, seed=90 + s)[0], 16)
+        z = np.abs(mn - mp) / np.sqrt(sn ** 2 + sp ** 2 + 1e-12)
+        lit = (mn + mp) > 1e-3
+        assert (z[lit] <= 3.0).mean() > 0.97, (entry, (z[lit] > 3).mean())
+        assert abs(mn.mean() - mp.mean()) < 0.01 * mp.mean(), entry
