@@ -112,6 +112,7 @@ int lmx_accel_count(const lmx_accel *accel, const lmx_ray *rays, uint32_t n,
 #define LMX_MAT_NULL 0       /* bsdf::null: no scattering (light-only surfaces) */
 #define LMX_MAT_DIFFUSE 1    /* bsdf::diffuse (bsdf_diffuse.cpp) */
 #define LMX_MAT_PHONG 2      /* normalized Phong (no counterpart in the reference) */
+#define LMX_MAT_GGX 3        /* bsdf::cooktorrance (bsdf_cooktorrance.cpp, GGX distribution, conductor Fresnel) */
 
 #define LMX_RASTER_RANDOM 0      /* raster position from the sample's uniforms, as the reference */
 #define LMX_RASTER_STRATIFIED 1  /* sample i -> pixel i mod (W H), jittered inside it */
@@ -121,6 +122,9 @@ typedef struct lmx_material {
     float R[3];          /* reflectance */
     float exponent;      /* Phong exponent */
     float Le[3];         /* emitted radiance; > 0 makes every triangle of the material an area light */
+    float roughness;     /* GGX roughness (bsdf_cooktorrance.cpp: default 0.1) */
+    float eta[3], k[3];  /* conductor index (default 0.14, 0.129, 0.1585; k 4.58625, 3.348125, 2.329375) */
+    float reserved;
 } lmx_material;
 
 /* Pinhole sensor (sensor_pinhole.cpp:44-202): position, camera basis (the

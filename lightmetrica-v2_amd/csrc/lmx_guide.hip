@@ -808,7 +808,7 @@ __global__ __launch_bounds__(256) void k_guided_shade(RScene s, RParams p, Guide
                 }
                 nv++;
                 if (alive && p.max_nv != -1 && nv >= p.max_nv) alive = false;
-                if (alive && (m.type == LMX_MAT_DIFFUSE || m.type == LMX_MAT_PHONG)) {
+                if (alive && is_scatterer(m)) {
                     guided = true;
                     wi = neg(d);
                     guide_inputs(gl.dev, fr.p, fr.sn, wi, x);

@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void k_shade(RScene s, RParams p, Ctl* __restr
                 }
                 nv++;
                 if (alive && p.max_nv != -1 && nv >= p.max_nv) alive = false;
-                if (alive && (m.type == LMX_MAT_DIFFUSE || m.type == LMX_MAT_PHONG)) {
+                if (alive && is_scatterer(m)) {
                     const uint32_t seg2 = seg + 1u;
                     const f3 wi = neg(d);
                     if (p.nee != NEE_NONE && (p.nee == NEE_DIRECT || nv + 1 >= p.min_nv)) {

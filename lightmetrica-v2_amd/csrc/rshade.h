@@ -19,8 +19,11 @@ struct RMat {
     float R[3];
     float exponent;
     float Le[3];
+    float roughness;
+    float eta[3], k[3];
+    float reserved;
 };
-static_assert(sizeof(RMat) == 32, "RMat layout");
+static_assert(sizeof(RMat) == 64, "RMat layout");
 
 // One light primitive: its triangles [first, first + count) carry an area
 // CDF at cdf[cdf_off .. cdf_off + count] (Distribution1D, dist.h:37-111).
@@ -179,6 +182,63 @@ RD f3 reflect_sn(const Frame& f, f3 wi) { return sub(mul(f.sn, 2.0f * dot(wi, f.
 // direction, then pdf (projected solid angle) and f.  Lambertian:
 // bsdf_diffuse.cpp:69-117; normalized Phong: f = R (n+2)/(2 pi) cos^n(a),
 // lobe sampling cos(a) = u0^(1/(n+1)) about the mirror direction.
+// BSDF_CookTorrance with the GGX distribution (bsdf_cooktorrance.cpp:36,
+// 184-226, 256-292), in the shading frame.  The shadowing term uses
+// |wo.H| for both directions as the reference writes it (:281-282); for the
+// half vector wi.H = wo.H, so only rounding can tell the two apart.
+RD float ggx_local_tan(f3 v)
+{
+    const float t = 1.0f - v.z * v.z;
+    return t <= 0.0f ? 0.0f : sqrtf(t) / v.z;
+}
+RD float ggx_D(float r, f3 H)
+{
+    const float cosH = H.z;
+    const float tanH = ggx_local_tan(H);
+    if (cosH <= 0.0f) return 0.0f;
+    const float t1 = r * r;
+    const float t = r * r + tanH * tanH;
+    const float t2 = LMX_PI_F * cosH * cosH * cosH * cosH * t * t;
+    return t1 / t2;
+}
+RD f3 ggx_sample_H(float r, float u0, float u1)
+{
+    const float eps = 1e-7f;                                   // Math::Eps()
+    const float a = (1.0f - eps) * u0 + eps;                   // ToOpenClosed
+    const float b = (1.0f - 2.0f * eps) * u1 + eps;            // ToOpenOpen
+    const float den = sqrtf(1.0f - (1.0f - r * r) * a);
+    const float cosT = sqrtf(1.0f - a) / den;
+    const float sinT = r * (sqrtf(a) / den);
+    const float phi = LMX_PI_F * (2.0f * b - 1.0f);
+    return mk3(sinT * cosf(phi), sinT * sinf(phi), cosT);
+}
+RD float ggx_G(f3 li, f3 lo, f3 H)
+{
+    const float nH = H.z, no = lo.z, ni = li.z;
+    const float woH = fabsf(dot(lo, H)), wiH = fabsf(dot(lo, H));
+    return fminf(1.0f, fminf(2.0f * nH * no / woH, 2.0f * nH * ni / wiH));
+}
+RD float ggx_F(float eta, float k, float c)
+{
+    const float tmp = (eta * eta + k * k) * (c * c);
+    const float rParl2 = (tmp - eta * (2.0f * c) + 1.0f) / (tmp + eta * (2.0f * c) + 1.0f);
+    const float tmpF = eta * eta + k * k;
+    const float rPerp2 = (tmpF - eta * (2.0f * c) + c * c) / (tmpF + eta * (2.0f * c) + c * c);
+    return (rParl2 + rPerp2) * 0.5f;
+}
+RD f3 ggx_f(const RMat& m, f3 li, f3 lo)
+{
+    const f3 H = normalize(add(li, lo));
+    const float D = ggx_D(m.roughness, H);
+    const float G = ggx_G(li, lo, H);
+    const float c = dot(li, H);
+    f3 out;
+    out.x = m.R[0] * D * G * ggx_F(m.eta[0], m.k[0], c) / (4.0f * li.z) / lo.z;
+    out.y = m.R[1] * D * G * ggx_F(m.eta[1], m.k[1], c) / (4.0f * li.z) / lo.z;
+    out.z = m.R[2] * D * G * ggx_F(m.eta[2], m.k[2], c) / (4.0f * li.z) / lo.z;
+    return out;
+}
+
 RD bool bsdf_sample(const RMat& m, const Frame& f, f3 wi, float u0, float u1, f3& wo)
 {
     const f3 li = to_local(f, wi);
@@ -198,6 +258,14 @@ RD bool bsdf_sample(const RMat& m, const Frame& f, f3 wi, float u0, float u1, f3
         wo = add(add(mul(ru, l.x), mul(rv, l.y)), mul(r, l.z));
         return true;
     }
+    if (m.type == LMX_MAT_GGX) {
+        const f3 H = ggx_sample_H(m.roughness, u0, u1);
+        const f3 ni = neg(li);
+        const f3 lo = sub(ni, mul(H, 2.0f * dot(ni, H)));
+        if (lo.z <= 0.0f) return false;
+        wo = to_world(f, lo);
+        return true;
+    }
     return false;
 }
 
@@ -210,6 +278,11 @@ RD float bsdf_pdf(const RMat& m, const Frame& f, f3 wi, f3 wo)
         const float ca = fmaxf(0.0f, dot(reflect_sn(f, wi), wo));
         const float pdf_sa = (m.exponent + 1.0f) * (0.5f * LMX_INV_PI_F) * powf(ca, m.exponent);
         return pdf_sa / lo.z;
+    }
+    if (m.type == LMX_MAT_GGX) {
+        const f3 H = normalize(add(li, lo));
+        const float D = ggx_D(m.roughness, H);
+        return D * H.z / (4.0f * dot(lo, H)) / lo.z;
     }
     return 0.0f;
 }
@@ -235,6 +308,10 @@ RD f3 bsdf_eval(const RMat& m, const Frame& f, f3 wi, f3 wo)
         const float k = (m.exponent + 2.0f) * (0.5f * LMX_INV_PI_F) * powf(ca, m.exponent);
         return mk3(m.R[0] * k * corr, m.R[1] * k * corr, m.R[2] * k * corr);
     }
+    if (m.type == LMX_MAT_GGX) {
+        const f3 v = ggx_f(m, li, lo);
+        return mk3(v.x * corr, v.y * corr, v.z * corr);
+    }
     return mk3(0.0f, 0.0f, 0.0f);
 }
 
@@ -258,10 +335,17 @@ RD f3 bsdf_eval_t(const RMat& m, const Frame& f, f3 wi, f3 wo, bool le)
         const float k = (m.exponent + 2.0f) * (0.5f * LMX_INV_PI_F) * powf(ca, m.exponent);
         return mk3(m.R[0] * k * corr, m.R[1] * k * corr, m.R[2] * k * corr);
     }
+    if (m.type == LMX_MAT_GGX) {
+        const f3 v = ggx_f(m, li, lo);
+        return mk3(v.x * corr, v.y * corr, v.z * corr);
+    }
     return mk3(0.0f, 0.0f, 0.0f);
 }
 
-RD bool is_scatterer(const RMat& m) { return m.type == LMX_MAT_DIFFUSE || m.type == LMX_MAT_PHONG; }
+RD bool is_scatterer(const RMat& m)
+{
+    return m.type == LMX_MAT_DIFFUSE || m.type == LMX_MAT_PHONG || m.type == LMX_MAT_GGX;
+}
 
 // Pinhole importance W_e(wo) = 1 / (A cos^3) with A = 4 tan^2(fov/2) aspect,
 // zero outside the film (sensor_pinhole.cpp:137-154).  It is also the

@@ -37,7 +37,8 @@ class lmx_accel_info(ctypes.Structure):
 
 class lmx_material(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int32), ("R", ctypes.c_float * 3), ("exponent", ctypes.c_float),
-                ("Le", ctypes.c_float * 3)]
+                ("Le", ctypes.c_float * 3), ("roughness", ctypes.c_float), ("eta", ctypes.c_float * 3),
+                ("k", ctypes.c_float * 3), ("reserved", ctypes.c_float)]
 
 
 class lmx_camera(ctypes.Structure):

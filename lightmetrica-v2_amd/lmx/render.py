@@ -20,6 +20,8 @@ RASTER_RANDOM = 0
 RASTER_STRATIFIED = 1
 RR_FIXED = 0
 RR_THROUGHPUT = 1
+# bsdf::cooktorrance defaults (bsdf_cooktorrance.cpp:60-63)
+GGX_DEFAULT = dict(roughness=0.1, eta=(0.14, 0.129, 0.1585), k=(4.58625, 3.348125, 2.329375))
 SLOT_SEG0 = 8
 
 
@@ -48,6 +50,9 @@ class Scene:
             mats[i].R[:] = list(m["R"])
             mats[i].exponent = m["exponent"]
             mats[i].Le[:] = list(m["Le"])
+            mats[i].roughness = m.get("roughness", GGX_DEFAULT["roughness"])
+            mats[i].eta[:] = list(m.get("eta", GGX_DEFAULT["eta"]))
+            mats[i].k[:] = list(m.get("k", GGX_DEFAULT["k"]))
         self._keep.append(mats)
         d = lmx_scene_desc()
         d.verts = keep(sc["verts"], np.float32)

@@ -181,6 +181,7 @@ def traversal_rays(verts, res=2048, n_incoherent=None, seed=TRAVERSAL_SEED + 1):
 MAT_NULL = 0
 MAT_DIFFUSE = 1
 MAT_PHONG = 2
+MAT_GGX = 3
 
 
 def _quad(p0, p1, p2, p3):
@@ -209,7 +210,7 @@ def _box(center, size, angle_deg):
     return v, t
 
 
-def cornell_box(width=256, height=256, phong_boxes=None):
+def cornell_box(width=256, height=256, phong_boxes=None, ggx_boxes=None):
     """Procedural Cornell box: 5 wall quads (inward) + 2 boxes rotated
     +-17 degrees (34 triangles) and a 0.25 x 0.25 emissive ceiling quad facing
     down with Le = (17, 12, 4); Lambertian 0.73 grey / (0.65, 0.05, 0.05) red
@@ -235,6 +236,9 @@ def cornell_box(width=256, height=256, phong_boxes=None):
     if phong_boxes is not None:        # variant: glossy boxes, normalized Phong of this exponent
         box_mat = len(mats)
         mats.append(dict(type=MAT_PHONG, R=(0.8, 0.8, 0.8), exponent=float(phong_boxes), Le=(0, 0, 0)))
+    if ggx_boxes is not None:          # variant: metal boxes, bsdf::cooktorrance of this roughness
+        box_mat = len(mats)
+        mats.append(dict(type=MAT_GGX, R=(0.9, 0.8, 0.7), exponent=0.0, Le=(0, 0, 0), roughness=float(ggx_boxes)))
     add(*_box((-0.35, -0.4, -0.3), (0.6, 1.2, 0.6), 17.0), box_mat)        # tall box
     add(*_box((0.35, -0.7, 0.3), (0.6, 0.6, 0.6), -17.0), box_mat)         # short box
     first_light = len(T)

@@ -130,7 +130,7 @@ float orc_uniform(uint64_t seed, uint64_t sample, uint32_t slot);
 #define ORC_PT_RASTER 1u
 #define ORC_PT_SEG0 8u
 
-enum { ORC_MAT_NULL = 0, ORC_MAT_DIFFUSE = 1, ORC_MAT_PHONG = 2 };
+enum { ORC_MAT_NULL = 0, ORC_MAT_DIFFUSE = 1, ORC_MAT_PHONG = 2, ORC_MAT_GGX = 3 };
 enum { ORC_RASTER_RANDOM = 0, ORC_RASTER_STRATIFIED = 1 };
 
 typedef struct orc_material {
@@ -138,6 +138,9 @@ typedef struct orc_material {
     float R[3];          /* reflectance */
     float exponent;      /* Phong exponent */
     float Le[3];         /* emitted radiance (area light when > 0) */
+    float roughness;     /* GGX (bsdf_cooktorrance.cpp) */
+    float eta[3], k[3];
+    float reserved;
 } orc_material;
 
 typedef struct orc_scene_desc {

@@ -160,7 +160,23 @@ PHILOX_TAG = 0x6C6D7870
 
 class orc_material(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int32), ("R", ctypes.c_float * 3), ("exponent", ctypes.c_float),
-                ("Le", ctypes.c_float * 3)]
+                ("Le", ctypes.c_float * 3), ("roughness", ctypes.c_float), ("eta", ctypes.c_float * 3),
+                ("k", ctypes.c_float * 3), ("reserved", ctypes.c_float)]
+
+
+GGX_DEFAULT = dict(roughness=0.1, eta=(0.14, 0.129, 0.1585), k=(4.58625, 3.348125, 2.329375))
+
+
+def _material(mat):
+    m = orc_material()
+    m.type = mat["type"]
+    m.R[:] = list(mat["R"])
+    m.exponent = mat.get("exponent", 0.0)
+    m.Le[:] = list(mat.get("Le", (0, 0, 0)))
+    m.roughness = mat.get("roughness", GGX_DEFAULT["roughness"])
+    m.eta[:] = list(mat.get("eta", GGX_DEFAULT["eta"]))
+    m.k[:] = list(mat.get("k", GGX_DEFAULT["k"]))
+    return m
 
 
 class orc_scene_desc(ctypes.Structure):
@@ -222,7 +238,7 @@ def _rlib():
 
 def bsdf_query(mat, n, wi, u):
     """(ok, wo, pdf_projected, f) of one oracle BSDF sample on a flat frame."""
-    m = orc_material(mat["type"], (ctypes.c_float * 3)(*mat["R"]), mat["exponent"], (ctypes.c_float * 3)(*mat["Le"]))
+    m = _material(mat)
     n = np.ascontiguousarray(n, np.float32); wi = np.ascontiguousarray(wi, np.float32)
     u = np.ascontiguousarray(u, np.float32)
     wo = np.zeros(3, np.float32); pdf = np.zeros(1, np.float32); f = np.zeros(3, np.float32)
@@ -231,7 +247,7 @@ def bsdf_query(mat, n, wi, u):
 
 
 def bsdf_eval(mat, n, wi, wo):
-    m = orc_material(mat["type"], (ctypes.c_float * 3)(*mat["R"]), mat["exponent"], (ctypes.c_float * 3)(*mat["Le"]))
+    m = _material(mat)
     n = np.ascontiguousarray(n, np.float32); wi = np.ascontiguousarray(wi, np.float32)
     wo = np.ascontiguousarray(wo, np.float32)
     pdf = np.zeros(1, np.float32); f = np.zeros(3, np.float32)
@@ -271,10 +287,7 @@ class Scene:
             return ctypes.c_void_p(a.ctypes.data)
         mats = (orc_material * len(sc["materials"]))()
         for i, m in enumerate(sc["materials"]):
-            mats[i].type = m["type"]
-            mats[i].R[:] = list(m["R"])
-            mats[i].exponent = m["exponent"]
-            mats[i].Le[:] = list(m["Le"])
+            mats[i] = _material(m)
         self._keep.append(mats)
         d = orc_scene_desc()
         d.verts = keep(sc["verts"], np.float32)

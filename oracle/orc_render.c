@@ -209,6 +209,53 @@ static inline v3 reflect_sn(const frame *f, v3 wi)
     return vsub(vmul(f->sn, 2.0f * vdot(wi, f->sn)), wi);
 }
 
+/* BSDF_CookTorrance, GGX distribution (bsdf_cooktorrance.cpp:36,184-226,
+ * 256-292), in the shading frame; the shadowing term takes |wo.H| for both
+ * directions as written there (:281-282). */
+static float ggx_tan(v3 v)
+{
+    const float t = 1.0f - v.z * v.z;                          /* Math::LocalTan, math.h:1903-1907 */
+    return t <= 0.0f ? 0.0f : sqrtf(t) / v.z;
+}
+static float ggx_D(float r, v3 H)
+{
+    const float cosH = H.z, tanH = ggx_tan(H);
+    if (cosH <= 0.0f) return 0.0f;
+    const float t1 = r * r;
+    const float t = r * r + tanH * tanH;
+    const float t2 = PI_F * cosH * cosH * cosH * cosH * t * t;
+    return t1 / t2;
+}
+static v3 ggx_sample_H(float r, const float u[2])
+{
+    const float eps = 1e-7f;
+    const float a = (1.0f - eps) * u[0] + eps, b = (1.0f - 2.0f * eps) * u[1] + eps;
+    const float den = sqrtf(1.0f - (1.0f - r * r) * a);
+    const float cosT = sqrtf(1.0f - a) / den, sinT = r * (sqrtf(a) / den);
+    const float phi = PI_F * (2.0f * b - 1.0f);
+    return mk(sinT * cosf(phi), sinT * sinf(phi), cosT);
+}
+static float ggx_G(v3 li, v3 lo, v3 H)
+{
+    const float nH = H.z, no = lo.z, ni = li.z;
+    const float woH = fabsf(vdot(lo, H)), wiH = fabsf(vdot(lo, H));
+    return fminf(1.0f, fminf(2.0f * nH * no / woH, 2.0f * nH * ni / wiH));
+}
+static float ggx_F(float eta, float k, float c)
+{
+    const float tmp = (eta * eta + k * k) * (c * c);
+    const float rParl2 = (tmp - eta * (2.0f * c) + 1.0f) / (tmp + eta * (2.0f * c) + 1.0f);
+    const float tmpF = eta * eta + k * k;
+    const float rPerp2 = (tmpF - eta * (2.0f * c) + c * c) / (tmpF + eta * (2.0f * c) + c * c);
+    return (rParl2 + rPerp2) * 0.5f;
+}
+static void ggx_f(const orc_material *m, v3 li, v3 lo, float out[3])
+{
+    const v3 H = vnorm(vadd(li, lo));
+    const float D = ggx_D(m->roughness, H), G = ggx_G(li, lo, H), c = vdot(li, H);
+    for (int k = 0; k < 3; k++) out[k] = m->R[k] * D * G * ggx_F(m->eta[k], m->k[k], c) / (4.0f * li.z) / lo.z;
+}
+
 /* Samples wo; returns 0 when the BSDF gives no direction (the reference's
  * SampleDirection returns without writing wo, bsdf_diffuse.cpp:69-79). */
 static int bsdf_sample(const orc_material *m, const frame *f, v3 wi, const float u[2], v3 *wo)
@@ -233,6 +280,14 @@ static int bsdf_sample(const orc_material *m, const frame *f, v3 wi, const float
         *wo = vadd(vadd(vmul(ru, l.x), vmul(rv, l.y)), vmul(r, l.z));
         return 1;
     }
+    if (m->type == ORC_MAT_GGX) {
+        const v3 H = ggx_sample_H(m->roughness, u);
+        const v3 ni = vneg(li);
+        const v3 lo = vsub(ni, vmul(H, 2.0f * vdot(ni, H)));
+        if (lo.z <= 0.0f) return 0;
+        *wo = to_world(f, lo);
+        return 1;
+    }
     return 0;
 }
 
@@ -246,6 +301,10 @@ static float bsdf_pdf(const orc_material *m, const frame *f, v3 wi, v3 wo)
         const float ca = fmaxf(0.0f, vdot(reflect_sn(f, wi), wo));
         const float pdf_sa = (m->exponent + 1.0f) * (0.5f * INV_PI_F) * powf(ca, m->exponent);
         return pdf_sa / lo.z;
+    }
+    if (m->type == ORC_MAT_GGX) {
+        const v3 H = vnorm(vadd(li, lo));
+        return ggx_D(m->roughness, H) * H.z / (4.0f * vdot(lo, H)) / lo.z;
     }
     return 0.0f;
 }
@@ -262,6 +321,9 @@ static void bsdf_eval(const orc_material *m, const frame *f, v3 wi, v3 wo, float
         const float ca = fmaxf(0.0f, vdot(reflect_sn(f, wi), wo));
         const float k = (m->exponent + 2.0f) * (0.5f * INV_PI_F) * powf(ca, m->exponent);
         for (int c = 0; c < 3; c++) out[c] = m->R[c] * k * corr;
+    } else if (m->type == ORC_MAT_GGX) {
+        ggx_f(m, li, lo, out);
+        for (int c = 0; c < 3; c++) out[c] *= corr;
     }
 }
 
@@ -349,7 +411,7 @@ static void pt_sample(const orc_scene *s, const camera *cam, const orc_pt_params
         if (pp->max_num_vertices != -1 && num_vertices >= pp->max_num_vertices) break;
         const uint32_t base = ORC_PT_SEG0 + 4u * (uint32_t)seg;
         if (seg > 0) {
-            if (!mat || (mat->type != ORC_MAT_DIFFUSE && mat->type != ORC_MAT_PHONG)) break;
+            if (!mat || (mat->type != ORC_MAT_DIFFUSE && mat->type != ORC_MAT_PHONG && mat->type != ORC_MAT_GGX)) break;
             const float u2[2] = {U(us, sample, base), U(us, sample, base + 1)};
             if (!bsdf_sample(mat, &fr, wi, u2, &wo)) break;
             const float pdf = bsdf_pdf(mat, &fr, wi, wo);
@@ -540,6 +602,9 @@ static void bsdf_eval_dir(const orc_material *m, const frame *f, v3 wi, v3 wo, i
         const float ca = fmaxf(0.0f, vdot(reflect_sn(f, wi), wo));
         const float k = (m->exponent + 2.0f) * (0.5f * INV_PI_F) * powf(ca, m->exponent);
         for (int c = 0; c < 3; c++) out[c] = m->R[c] * k * corr;
+    } else if (m->type == ORC_MAT_GGX) {
+        ggx_f(m, li, lo, out);
+        for (int c = 0; c < 3; c++) out[c] *= corr;
     }
 }
 
@@ -553,7 +618,8 @@ static void v_eval(const bctx *b, const bvert *v, v3 wi, v3 wo, int le, float ou
     } else if (v->kind == BV_LIGHT) {                 /* light_area.cpp:105-110 */
         if (to_local(&v->fr, wo).z <= 0.0f) return;
         for (int c = 0; c < 3; c++) out[c] = v->mat->Le[c];
-    } else if (v->mat && (v->mat->type == ORC_MAT_DIFFUSE || v->mat->type == ORC_MAT_PHONG)) {
+    } else if (v->mat && (v->mat->type == ORC_MAT_DIFFUSE || v->mat->type == ORC_MAT_PHONG ||
+                          v->mat->type == ORC_MAT_GGX)) {
         bsdf_eval_dir(v->mat, &v->fr, wi, wo, le, out);
     }
 }
@@ -563,7 +629,7 @@ static float v_pdf(const bctx *b, const bvert *v, v3 wi, v3 wo)
 {
     if (v->kind == BV_CAMERA) return importance(&b->cam, wo);
     if (v->kind == BV_LIGHT) return to_local(&v->fr, wo).z <= 0.0f ? 0.0f : INV_PI_F;
-    if (v->mat && (v->mat->type == ORC_MAT_DIFFUSE || v->mat->type == ORC_MAT_PHONG))
+    if (v->mat && (v->mat->type == ORC_MAT_DIFFUSE || v->mat->type == ORC_MAT_PHONG || v->mat->type == ORC_MAT_GGX))
         return bsdf_pdf(v->mat, &v->fr, wi, wo);
     return 0.0f;
 }
@@ -910,7 +976,8 @@ static void nee_sample(const bctx *b, const orc_pt_params *pp, int mode, const u
         if (pp->max_num_vertices != -1 && nv >= pp->max_num_vertices) break;
         const uint32_t base = ORC_PT_SEG0 + 8u * (uint32_t)v;
         const int scat = cur.kind == BV_CAMERA ||
-                         (cur.mat && (cur.mat->type == ORC_MAT_DIFFUSE || cur.mat->type == ORC_MAT_PHONG));
+                         (cur.mat && (cur.mat->type == ORC_MAT_DIFFUSE || cur.mat->type == ORC_MAT_PHONG ||
+                                      cur.mat->type == ORC_MAT_GGX));
         if (!scat) break;
         /* direct light sampling (ptmis :130-195, ptdirect :109-150) */
         if (s->d.nlight > 0 && (mode == 1 || nv + 1 >= pp->min_num_vertices)) {

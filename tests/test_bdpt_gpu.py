@@ -119,3 +119,14 @@ def test_bdpt_and_pt_images_agree_within_3_sigma(gpu):
     assert (z[lit] <= 3.0).mean() > 0.97, (z[lit] > 3).mean()
     # and the image means agree to 1%
     assert abs(mb.mean() - mp.mean()) < 0.01 * mp.mean()
+
+
+def test_bdpt_per_strategy_parity_ggx(gpu):
+    import torch
+    sc = scenes.cornell_box(32, 32, ggx_boxes=0.3)
+    n, m = 32 * 32 * 8, 8
+    tab = np.random.default_rng(12).random((n, render.bdpt_table_width(m)), dtype=np.float32)
+    _, _, so = render.Scene(sc).render_bdpt(n, max_num_vertices=m, table=torch.from_numpy(tab).to(gpu),
+                                            sample_out=True, batch=4096)
+    c = orc.Scene(sc).bdpt_samples(0, n, max_num_vertices=m, table=tab)
+    _check(so.cpu().numpy(), c, tight=0.99, loose=0.999)

@@ -149,3 +149,49 @@ def test_nee_cornell_matches_pt(mode):
     assert abs(ma.mean() - b.mean()) < 3.5 * se, (ma.mean(), b.mean(), se)
     # and light sampling is far less noisy than BSDF sampling here
     assert ma.std() < 0.5 * b.std()
+
+
+@pytest.mark.parametrize("roughness", [0.05, 0.3, 0.8])
+def test_ggx_distribution_normalised(roughness):
+    """bsdf::cooktorrance's GGX D integrates to one against cos(theta_h):
+    D cos = r^2 / (pi cos^3 (r^2 + tan^2)^2), integrated over theta_h in
+    closed form (substituting t = tan^2)."""
+    mat = dict(type=scenes.MAT_GGX, R=(1, 1, 1), exponent=0.0, Le=(0, 0, 0), roughness=roughness,
+               eta=(1.0, 1.0, 1.0), k=(0.0, 0.0, 0.0))
+    n = np.array([0, 0, 1], np.float32)
+    # at normal incidence H = normalize(wi + wo); pdf_proj * cos_o = D cos_h / (4 wo.H)
+    wi = np.array([0, 0, 1], np.float32)
+    k = 40000
+    tot = 0.0
+    for c in (np.arange(k) + 0.5) / k:
+        x = np.array([np.sqrt(1 - c * c), 0.0, c], np.float32)
+        pdf, f = orc.bsdf_eval(mat, n, wi, x)
+        tot += pdf * c * (2 * np.pi / k)
+    # every reflected direction of a normal-incidence ray stays above the
+    # surface for H within 45 degrees; the pdf integrates to the probability
+    # mass of D cos over those H, in closed form 1 / (1 + r^2)
+    assert abs(tot - 1.0 / (1.0 + roughness ** 2)) < 3e-3, (tot, 1.0 / (1.0 + roughness ** 2))
+
+
+@pytest.mark.parametrize("roughness", [0.1, 0.5])
+def test_ggx_sampling_matches_pdf_and_is_reciprocal(roughness):
+    """E[1 / p_sa] over the sampler's draws is the solid angle of the
+    directions it can reach (2 pi: the whole hemisphere), which fails if the
+    half-vector Jacobian 1 / (4 wo.H) is off; f is symmetric in wi, wo; and
+    E[f cos / p] is the directional albedo, at most R."""
+    mat = dict(type=scenes.MAT_GGX, R=(0.9, 0.9, 0.9), exponent=0.0, Le=(0, 0, 0), roughness=roughness)
+    n = np.array([0, 0, 1], np.float32)
+    wi = np.array([0.5, 0.1, 0.86], np.float32); wi /= np.linalg.norm(wi)
+    rng = np.random.default_rng(11)
+    m = 40000
+    inv, alb = 0.0, np.zeros(3)
+    for u in rng.random((m, 2), dtype=np.float32):
+        ok, wo, pdf, f = orc.bsdf_query(mat, n, wi, u)
+        if ok and pdf > 0:
+            inv += 1.0 / (pdf * wo[2])
+            alb += f / pdf
+            if rng.random() < 0.02:
+                _, f2 = orc.bsdf_eval(mat, n, wo, wi)
+                np.testing.assert_allclose(f2, f, rtol=2e-4)
+    assert abs(inv / m - 2 * np.pi) < 0.05 * 2 * np.pi, inv / m
+    assert np.all(alb / m <= 0.9 * 1.001) and np.all(alb / m > 0.5)

@@ -184,3 +184,22 @@ def test_nee_images_agree_with_pt(gpu):
         lit = (mn + mp) > 1e-3
         assert (z[lit] <= 3.0).mean() > 0.97, (entry, (z[lit] > 3).mean())
         assert abs(mn.mean() - mp.mean()) < 0.01 * mp.mean(), entry
+
+
+@pytest.mark.parametrize("entry,mode", [("pt", 0), ("ptmis", 2)])
+def test_ggx_per_sample_parity(gpu, entry, mode):
+    """bsdf::cooktorrance (GGX) boxes: GPU and oracle per sample."""
+    import torch
+    sc = scenes.cornell_box(32, 32, ggx_boxes=0.3)
+    n, m = 32 * 32 * 16, 9
+    rs = render.Scene(sc)
+    tab = np.random.default_rng(31).random((n, render.table_width(m, nee=mode > 0)), dtype=np.float32)
+    _, _, so = getattr(rs, "render_" + entry)(n, max_num_vertices=m, table=torch.from_numpy(tab).to(gpu),
+                                              sample_out=True)
+    if mode == 0:
+        c = orc.Scene(sc).pt_samples(0, n, max_num_vertices=m, table=tab)
+        ok = _compare(so.cpu().numpy(), c)
+    else:
+        c = orc.Scene(sc).nee_samples(0, n, mode, max_num_vertices=m, table=tab)
+        ok = _compare_nee(so.cpu().numpy(), c)
+    assert ok.mean() > 0.995, f"{(~ok).sum()} of {n} samples differ"
