@@ -96,6 +96,18 @@ def pt_section(dev, cpu, threads):
         "image_mean": round(float(film.mean()), 5),
         "cpu_baseline": None,
     }
+    # renderer::ptdirect / ptmis on the same frame: next-event estimation
+    # adds one shadow ray per vertex
+    for entry in ("ptdirect", "ptmis"):
+        fn = getattr(rs, "render_" + entry)
+        fn(1024 * 1024 * 4, **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, st2, _ = fn(n, **kw)
+        torch.cuda.synchronize()
+        dt2 = time.perf_counter() - t0
+        out[entry] = {"msamples_per_s": round(n / dt2 / 1e6, 2), "mrays_per_s": round(st2["rays"] / dt2 / 1e6, 1),
+                      "rays_per_sample": round(st2["rays"] / n, 4)}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import orc
