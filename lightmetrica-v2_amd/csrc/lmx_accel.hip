@@ -17,7 +17,7 @@
 //   * lanes whose ray finished are refilled as a batch once at least
 //     LMX_REFILL_MIN lanes are idle: one ballot counts them, mbcnt ranks give
 //     every idle lane its ray, and the rays come from a per-wave pool that
-//     one atomic refills LMX_CLAIM_CHUNK rays at a time;
+//     one atomic refills up to LMX_CLAIM_CHUNK rays at a time;
 //   * XCD-aware work split: workgroup b runs on XCD (b mod 8); the rays are
 //     cut into 8 contiguous regions and a wave first drains the region of
 //     its own XCD (neighbouring rays share BVH nodes in that XCD's L2), then
@@ -221,6 +221,10 @@ void k_trace(const WNode* __restrict__ nodes,
     extern __shared__ uint2 stack[];     // [stack_depth][TRACE_WAVE]
     const uint32_t lane = threadIdx.x;
     const uint32_t nray = cursors[8];
+    // claim size: LMX_CLAIM_CHUNK for full queues; short queues (the tail of
+    // a wavefront render) are spread over every wave of the grid, so that
+    // their time is that of a few rays per wave, not of whole chunks in turn
+    const uint32_t claim = min((uint32_t)LMX_CLAIM_CHUNK, max(1u, (nray + gridDim.x - 1u) / gridDim.x));
 
     uint32_t region = blockIdx.x & 7u;
     uint32_t regions_left = 8;          // regions not yet found exhausted
@@ -251,10 +255,10 @@ void k_trace(const WNode* __restrict__ nodes,
                 if (lane == 0) {
                     while (regions_left != 0) {
                         const uint32_t end = region_end(region, nray);
-                        const uint32_t old = atomicAdd(&cursors[region], (uint32_t)LMX_CLAIM_CHUNK);
+                        const uint32_t old = atomicAdd(&cursors[region], claim);
                         if (old < end) {
                             cb = old;
-                            cg = min((uint32_t)LMX_CLAIM_CHUNK, end - old);
+                            cg = min(claim, end - old);
                             break;
                         }
                         region = (region + 1u) & 7u;

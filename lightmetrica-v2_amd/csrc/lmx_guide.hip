@@ -212,12 +212,19 @@ RD bf16x8 encode_chunk(const float* xs, int kc, int g, int r, int mt, float& sx,
     return b;
 }
 
+// Observer of the forward pass: sees every operand of the encoding (chunk kc
+// of tile mt) and every hidden activation (layer L = 0..3, tile mt, chunk c),
+// in operand layout.  Inference passes NoSink.
+struct NoSink {
+    RD void enc(int, int, const bf16x8&) {}
+    RD void hidden(int, int, int, const bf16x8&) {}
+};
+
 // Forward pass of MT tiles of 16 queries (inputs xs[MT*16][8]) for one wave;
-// outputs to ys[MT*16][YS].  With KEEP the layer inputs h1..h4 (operand
-// layout) are kept for the backward pass.
-template <int MT, bool KEEP>
+// outputs to ys[MT*16][YS].
+template <int MT, class Sink>
 RD void mlp_forward(const bf16x8* __restrict__ W, const float* __restrict__ bias, const float* xs, float* ys,
-                    bf16x8 (&hk)[4][MT][2])
+                    Sink& sink)
 {
     const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
     f32x4 acc[4][MT];
@@ -231,7 +238,10 @@ RD void mlp_forward(const bf16x8* __restrict__ W, const float* __restrict__ bias
     for (int kc = 0; kc < 6; kc++) {
         bf16x8 b[MT];
 #pragma unroll
-        for (int mt = 0; mt < MT; mt++) b[mt] = encode_chunk(xs, kc, g, r, mt, sx[mt], cx[mt]);
+        for (int mt = 0; mt < MT; mt++) {
+            b[mt] = encode_chunk(xs, kc, g, r, mt, sx[mt], cx[mt]);
+            sink.enc(kc, mt, b[mt]);
+        }
 #pragma unroll
         for (int nt = 0; nt < 4; nt++) {
             const bf16x8 a = W[OFF_A1 + (nt * 6 + kc) * 64 + lane];
@@ -245,7 +255,7 @@ RD void mlp_forward(const bf16x8* __restrict__ W, const float* __restrict__ bias
 #pragma unroll
         for (int c = 0; c < 2; c++) {
             h[mt][c] = relu_pack(acc[2 * c][mt], acc[2 * c + 1][mt]);
-            if (KEEP) hk[0][mt][c] = h[mt][c];
+            sink.hidden(0, mt, c, h[mt][c]);
         }
 #pragma unroll
     for (int L = 0; L < 3; L++) {
@@ -271,7 +281,7 @@ RD void mlp_forward(const bf16x8* __restrict__ W, const float* __restrict__ bias
 #pragma unroll
             for (int c = 0; c < 2; c++) {
                 h[mt][c] = relu_pack(acc[2 * c][mt], acc[2 * c + 1][mt]);
-                if (KEEP) hk[L + 1][mt][c] = h[mt][c];
+                sink.hidden(L + 1, mt, c, h[mt][c]);
             }
     }
     f32x4 o[3][MT];
@@ -432,13 +442,13 @@ __global__ __launch_bounds__(256) void k_eval(GuideDev G, const float* __restric
     for (int t = threadIdx.x; t < N_BIAS; t += blockDim.x) bl[t] = G.bias[t];
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    bf16x8 dummy[4][4][2];
+    NoSink dummy;
     for (uint32_t base = (blockIdx.x * 4 + wave) * 64u; base < n; base += gridDim.x * 256u) {
         const uint32_t q = base + lane;
 #pragma unroll
         for (int d = 0; d < 7; d++) xs[wave][lane * 8 + d] = q < n ? x[(size_t)q * 7 + d] : 0.0f;
         wave_sync();
-        mlp_forward<4, false>(wl, bl, xs[wave], ys[wave], dummy);
+        mlp_forward<4>(wl, bl, xs[wave], ys[wave], dummy);
         wave_sync();
         if (q < n)
             for (int o = 0; o < 41; o++) y[(size_t)q * 41 + o] = ys[wave][lane * YS + o];
@@ -474,68 +484,69 @@ __global__ void k_mix_sample(const float* __restrict__ y, const float* __restric
 // throughput at the vertex, the loss weight, and the path's contribution so
 // far (the weight is finished once the path has ended).
 
-constexpr int TR_WAVES = 2;           // waves per training workgroup
-constexpr int SROW = 40;              // bf16 per staged unit row (32 records + bank padding)
+// Two kernels per step.  k_train_fb: forward (keeping the layer inputs),
+// the loss gradient and the backward pass, 32 records per wave pass; it
+// writes every layer input A_l and every layer-output delta D_l as bf16 rows
+// [unit][record] to a scratch array.  k_wgrad: dW_l^T = D_l A_{l-1}^T as a
+// split-K GEMM over records, each workgroup staging 32-record slices of all
+// rows in LDS and its waves accumulating all 108 output tiles in registers;
+// one gradient partial per workgroup, summed by k_adam.
+//
+// Scratch layout: one block of NROWS x 32 bf16 per 32-record pass,
+// [pass][row][record], so that a pass's rows are one contiguous 47 KB span:
+constexpr int ROW_A0 = 0;              // encoding, 192 slots (layer-1 operand order)
+constexpr int ROW_A1 = 192;            // h1..h4, 64 units each, natural order
+constexpr int ROW_D1 = ROW_A1 + 256;   // deltas of layers 1..4 (pre-activation), 64 each
+constexpr int ROW_D5 = ROW_D1 + 256;   // output gradient, 48 (41 used)
+constexpr int NROWS = ROW_D5 + 48;     // 752
+constexpr int FB_WAVES = 4;
+constexpr int WG_WAVES = 4;
+constexpr int SROW = 40;               // bf16 per staged row (32 records + bank padding)
 
-// the staged operands are read 16 bytes per lane: they come first, aligned
-struct TrainLds {
-    alignas(16) __bf16 sh[TR_WAVES][192 * SROW];  // layer inputs, [unit][record]
-    alignas(16) __bf16 sd[TR_WAVES][64 * SROW];   // deltas, [unit][record]
-    alignas(16) float xs[TR_WAVES][32 * 8];
-    alignas(16) float ys[TR_WAVES][32 * YS];
-    alignas(16) float grad[GP_N];
+// one operand-layout chunk (tile mt, chunk kc: units unit_of_slot(kc, g, j))
+// -> rows [row0 + unit][record] of a pass block
+RD void store_chunk(__bf16* blk, int row0, int mt, int kc, const bf16x8& v, int lane)
+{
+    const int g = lane >> 4, r = lane & 15;
+    __bf16* base = blk + mt * 16 + r;
+#pragma unroll
+    for (int j = 0; j < 8; j++) base[(row0 + unit_of_slot(kc, g, j)) * 32] = v[j];
+}
+
+// ReLU mask of an operand-layout activation: bit 16 mt + 8 c + j
+RD uint32_t relu_bits(int mt, int c, const bf16x8& h)
+{
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) m |= ((float)h[j] > 0.0f ? 1u : 0u) << (16 * mt + 8 * c + j);
+    return m;
+}
+
+// forward observer of training: the encoding (layer-1 slot order) and
+// h1..h4 go to the scratch rows; the ReLU masks stay in registers
+struct TrainSink {
+    __bf16* blk;
+    int lane;
+    uint32_t mask[4];
+    RD void enc(int kc, int mt, const bf16x8& b)
+    {
+        const int g = lane >> 4, r = lane & 15;
+        __bf16* base = blk + mt * 16 + r;
+#pragma unroll
+        for (int j = 0; j < 8; j++) base[(ROW_A0 + kc * 32 + 8 * g + j) * 32] = b[j];
+    }
+    RD void hidden(int L, int mt, int c, const bf16x8& h)
+    {
+        store_chunk(blk, ROW_A1 + 64 * L, mt, c, h, lane);
+        const uint32_t m = relu_bits(mt, c, h);
+        mask[L] = (mt == 0 && c == 0) ? m : (mask[L] | m);
+    }
 };
-static_assert((192 * SROW * 2) % 16 == 0 && (64 * SROW * 2) % 16 == 0, "staging rows keep 16-byte alignment");
-
-// stage an operand-layout activation (or delta) of 2 tiles x 2 chunks into
-// [unit][record]
-RD void stage_units(__bf16* st, const bf16x8 (&v)[2][2], int lane)
-{
-    const int g = lane >> 4, r = lane & 15;
-#pragma unroll
-    for (int mt = 0; mt < 2; mt++)
-#pragma unroll
-        for (int kc = 0; kc < 2; kc++)
-#pragma unroll
-            for (int j = 0; j < 8; j++) st[unit_of_slot(kc, g, j) * SROW + mt * 16 + r] = v[mt][kc][j];
-}
-
-// dW^T tiles: D(n, k) += sum over the 32 staged records of d[n][rec] h[k][rec];
-// added to the LDS gradient at W + k * ldw + n (bias: + sum of d at B + n)
-RD void wgrad(float* grad, const __bf16* sd, const __bf16* sh, int ntn, int ntk, int nvalid, int W, int ldw, int lane,
-              const int* krow /* null: identity */)
-{
-    const int g = lane >> 4, r = lane & 15;
-    for (int nt = 0; nt < ntn; nt++) {
-        const bf16x8 a = *reinterpret_cast<const bf16x8*>(sd + (nt * 16 + r) * SROW + 8 * g);
-        for (int kt = 0; kt < ntk; kt++) {
-            const bf16x8 b = *reinterpret_cast<const bf16x8*>(sh + (kt * 16 + r) * SROW + 8 * g);
-            f32x4 d = MFMA16(a, b, (f32x4{0.0f, 0.0f, 0.0f, 0.0f}));
-            const int k = kt * 16 + r;
-            const int row = krow ? krow[k] : k;
-            if (row < 0) continue;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int n = nt * 16 + 4 * g + i;
-                if (n < nvalid) atomicAdd(&grad[W + row * ldw + n], d[i]);
-            }
-        }
-    }
-}
-
-RD void bgrad(float* grad, const __bf16* sd, int nvalid, int B, int lane)
-{
-    if (lane < nvalid) {
-        float s = 0.0f;
-        for (int q = 0; q < 32; q++) s += (float)sd[lane * SROW + q];
-        atomicAdd(&grad[B + lane], s);
-    }
-}
 
 // one backward layer: delta of the layer input = W (natural rows) * delta,
-// masked by the ReLU of that input (h, operand layout)
-RD void back_layer(const bf16x8* __restrict__ Wb, const bf16x8 (&din)[2][2], const bf16x8 (&h)[2][2],
-                   bf16x8 (&dout)[2][2], int lane)
+// masked by the ReLU of that input (bits of relu_bits)
+RD void back_layer(const bf16x8* __restrict__ Wb, const bf16x8 (&din)[2][2], uint32_t mask, bf16x8 (&dout)[2][2],
+                   int lane)
 {
     f32x4 acc[4][2];
 #pragma unroll
@@ -556,34 +567,29 @@ RD void back_layer(const bf16x8* __restrict__ Wb, const bf16x8 (&din)[2][2], con
         for (int c = 0; c < 2; c++)
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                dout[mt][c][i] = (float)h[mt][c][i] > 0.0f ? (__bf16)acc[2 * c][mt][i] : (__bf16)0.0f;
-                dout[mt][c][4 + i] = (float)h[mt][c][4 + i] > 0.0f ? (__bf16)acc[2 * c + 1][mt][i] : (__bf16)0.0f;
+                const int bit = 16 * mt + 8 * c;
+                dout[mt][c][i] = (mask >> (bit + i)) & 1u ? (__bf16)acc[2 * c][mt][i] : (__bf16)0.0f;
+                dout[mt][c][4 + i] = (mask >> (bit + 4 + i)) & 1u ? (__bf16)acc[2 * c + 1][mt][i] : (__bf16)0.0f;
             }
 }
 
 // Forward, loss gradient and backward over `n` records (indices idx, or
-// 0..n-1), 32 per wave pass; per-workgroup gradient sums to partial[block].
-__global__ __launch_bounds__(TR_WAVES * 64) void k_train(GuideDev G, const float4* __restrict__ recs,
-                                                        const uint32_t* __restrict__ idx, uint32_t n, float wnorm,
-                                                        float wclamp, float inv_n, float* __restrict__ partial,
-                                                        float* __restrict__ loss)
+// 0..n-1), padded to whole 32-record passes (padding records have weight 0);
+// writes the scratch blocks of all passes.
+__global__ __launch_bounds__(FB_WAVES * 64) void k_train_fb(GuideDev G, const float4* __restrict__ recs,
+                                                           const uint32_t* __restrict__ idx, uint32_t n,
+                                                           float wnorm, float wclamp, float inv_n,
+                                                           __bf16* __restrict__ act, float* __restrict__ loss)
 {
-    __shared__ __attribute__((aligned(16))) TrainLds S;
-    for (int t = threadIdx.x; t < GP_N; t += blockDim.x) S.grad[t] = 0.0f;
-    __shared__ int krow1[192];
-    for (int t = threadIdx.x; t < 192; t += blockDim.x) {
-        const int kc = t >> 5, s = t & 31;
-        krow1[t] = feat_of_slot(kc, s >> 3, s & 7);
-    }
-    __syncthreads();
+    __shared__ float xs_all[FB_WAVES][32 * 8];
+    __shared__ float ys_all[FB_WAVES][32 * YS];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
-    float* xs = S.xs[wave];
-    float* ys = S.ys[wave];
-    __bf16* sh = S.sh[wave];
-    __bf16* sd = S.sd[wave];
+    float* xs = xs_all[wave];
+    float* ys = ys_all[wave];
     float lsum = 0.0f;
     const uint32_t npass = (n + 31) / 32;
-    for (uint32_t pass = blockIdx.x * TR_WAVES + wave; pass < npass; pass += gridDim.x * TR_WAVES) {
+    for (uint32_t pass = blockIdx.x * FB_WAVES + wave; pass < npass; pass += gridDim.x * FB_WAVES) {
+        __bf16* blk = act + (size_t)pass * NROWS * 32;
         const uint32_t j = pass * 32 + (uint32_t)lane;
         const bool mine = lane < 32 && j < n;
         f3 dir = mk3(0, 0, 0);
@@ -601,26 +607,22 @@ __global__ __launch_bounds__(TR_WAVES * 64) void k_train(GuideDev G, const float
             for (int k = 0; k < 7; k++) xs[lane * 8 + k] = x[k];
         }
         wave_sync();
-        bf16x8 hk[4][2][2];
-        mlp_forward<2, true>(G.pack, G.bias, xs, ys, hk);
+        TrainSink sink{blk, lane, {0u, 0u, 0u, 0u}};
+        mlp_forward<2>(G.pack, G.bias, xs, ys, sink);
         wave_sync();
-        // output gradient (rows of ys, in place), pre-scaled by 1/n
+        // output gradient (rows of ys, in place; pre-scaled by 1/n) and D5
         if (lane < 32) {
             float gy[41];
             if (mine) lsum += out_grad(ys + lane * YS, dir, pb, w, gy);
             else for (int o = 0; o < 41; o++) gy[o] = 0.0f;
             for (int o = 0; o < 41; o++) {
-                const float v = gy[o];
-                ys[lane * YS + o] = v;
-                sd[o * SROW + lane] = (__bf16)v;
+                ys[lane * YS + o] = gy[o];
+                blk[(ROW_D5 + o) * 32 + lane] = (__bf16)gy[o];
             }
-            for (int o = 41; o < 48; o++) sd[o * SROW + lane] = (__bf16)0.0f;
+            for (int o = 41; o < 48; o++) blk[(ROW_D5 + o) * 32 + lane] = (__bf16)0.0f;
         }
-        stage_units(sh, hk[3], lane);
         wave_sync();
-        wgrad(S.grad, sd, sh, 3, 4, 41, GP_W5, 41, lane, nullptr);
-        bgrad(S.grad, sd, 41, GP_B5, lane);
-        // delta of h4: W5 (natural output order) * dY
+        // backward: W5 (natural output order) * dY, then W4, W3, W2
         bf16x8 dy[2][2], d[2][2], dn[2][2];
 #pragma unroll
         for (int mt = 0; mt < 2; mt++)
@@ -631,43 +633,113 @@ __global__ __launch_bounds__(TR_WAVES * 64) void k_train(GuideDev G, const float
                     const int o = 32 * kc + 8 * g + jj;
                     dy[mt][kc][jj] = o < 41 ? (__bf16)ys[(mt * 16 + r) * YS + o] : (__bf16)0.0f;
                 }
-        back_layer(G.pack + OFF_B5, dy, hk[3], d, lane);
-        // layers 4, 3, 2: weight gradient from (delta, input), then delta of the input
-        static constexpr int WL[3] = {GP_W4, GP_W3, GP_W2}, BL[3] = {GP_B4, GP_B3, GP_B2};
+        back_layer(G.pack + OFF_B5, dy, sink.mask[3], d, lane);     // D4
 #pragma unroll
-        for (int L = 0; L < 3; L++) {
-            wave_sync();
-            stage_units(sd, d, lane);
-            stage_units(sh, hk[2 - L], lane);
-            wave_sync();
-            wgrad(S.grad, sd, sh, 4, 4, 64, WL[L], 64, lane, nullptr);
-            bgrad(S.grad, sd, 64, BL[L], lane);
-            back_layer(G.pack + OFF_B4 + L * N_AH, d, hk[2 - L], dn, lane);
+        for (int L = 3; L >= 0; L--) {                              // store D_L+1, make D_L
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+                for (int kc = 0; kc < 2; kc++) store_chunk(blk, ROW_D1 + 64 * L, mt, kc, d[mt][kc], lane);
+            if (L == 0) break;
+            back_layer(G.pack + OFF_B4 + (3 - L) * N_AH, d, sink.mask[L - 1], dn, lane);
 #pragma unroll
             for (int mt = 0; mt < 2; mt++)
 #pragma unroll
                 for (int kc = 0; kc < 2; kc++) d[mt][kc] = dn[mt][kc];
         }
-        // layer 1: its input is the encoding, re-made into [slot][record]
-        wave_sync();
-        stage_units(sd, d, lane);
-#pragma unroll
-        for (int mt = 0; mt < 2; mt++) {
-            float sx = 0.0f, cx = 0.0f;
-            for (int kc = 0; kc < 6; kc++) {
-                const bf16x8 b = encode_chunk(xs, kc, g, r, mt, sx, cx);
-#pragma unroll
-                for (int jj = 0; jj < 8; jj++) sh[(kc * 32 + 8 * g + jj) * SROW + mt * 16 + r] = b[jj];
-            }
-        }
-        wave_sync();
-        wgrad(S.grad, sd, sh, 4, 12, 64, GP_W1, 64, lane, krow1);
         wave_sync();
     }
-    __syncthreads();
-    for (int t = threadIdx.x; t < GP_N; t += blockDim.x) partial[(size_t)blockIdx.x * GP_N + t] = S.grad[t];
     for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
     if (lane == 0 && loss) atomicAdd(loss, lsum);
+}
+
+// The 108 weight-gradient tiles of the network (16 x 16 each, rows n of a
+// layer's outputs, columns k of its inputs): layer 1 4 x 12, layers 2-4
+// 4 x 4 each, layer 5 3 x 4.  Tile t: its layer, the first scratch row of its
+// delta rows and of its input rows.
+struct WTile {
+    int layer, nt, kt, drow, arow;
+};
+RD WTile tile_of(int t)
+{
+    WTile T;
+    if (t < 48) {
+        T.layer = 1; T.nt = t / 12; T.kt = t % 12;
+        T.drow = ROW_D1 + 16 * T.nt; T.arow = ROW_A0 + 16 * T.kt;
+    } else if (t < 96) {
+        const int u = t - 48;
+        T.layer = 2 + u / 16; T.nt = (u % 16) / 4; T.kt = u % 4;
+        T.drow = ROW_D1 + 64 * (T.layer - 1) + 16 * T.nt; T.arow = ROW_A1 + 64 * (T.layer - 2) + 16 * T.kt;
+    } else {
+        const int u = t - 96;
+        T.layer = 5; T.nt = u / 4; T.kt = u % 4;
+        T.drow = ROW_D5 + 16 * T.nt; T.arow = ROW_A1 + 192 + 16 * T.kt;
+    }
+    return T;
+}
+constexpr int TILES_PER_WAVE = 27;
+static_assert(TILES_PER_WAVE * WG_WAVES == 108, "every tile has one wave");
+
+// Split-K weight gradients: workgroup b sums the passes [b*chunk, (b+1)*chunk)
+// of the scratch into partial[b] (every parameter written).
+__global__ __launch_bounds__(WG_WAVES * 64) void k_wgrad(const __bf16* __restrict__ act, uint32_t npass,
+                                                         uint32_t chunk, float* __restrict__ partial)
+{
+    __shared__ alignas(16) __bf16 st[NROWS * SROW];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+    f32x4 acc[TILES_PER_WAVE];
+#pragma unroll
+    for (int t = 0; t < TILES_PER_WAVE; t++) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    float bsum = 0.0f;                                   // thread t < 240: bias row t of D2..D5
+    const uint32_t p_begin = blockIdx.x * chunk;
+    const uint32_t p_end = min(npass, p_begin + chunk);
+    for (uint32_t pass = p_begin; pass < p_end; pass++) {
+        __syncthreads();
+        // stage one pass block: 752 rows x 64 B, 16 B per thread per step
+        const uint4* src = reinterpret_cast<const uint4*>(act + (size_t)pass * NROWS * 32);
+        for (int t = threadIdx.x; t < NROWS * 4; t += blockDim.x) {
+            const int row = t >> 2, part = t & 3;
+            *reinterpret_cast<uint4*>(&st[row * SROW + part * 8]) = src[t];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < TILES_PER_WAVE; i++) {
+            const WTile T = tile_of(wave * TILES_PER_WAVE + i);
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(&st[(T.drow + r) * SROW + 8 * g]);
+            const bf16x8 b = *reinterpret_cast<const bf16x8*>(&st[(T.arow + r) * SROW + 8 * g]);
+            acc[i] = MFMA16(a, b, acc[i]);
+        }
+        if (threadIdx.x < 240) {
+            const int row = ROW_D1 + 64 + threadIdx.x;      // D2, D3, D4 (64 each), D5 (48)
+            for (int q = 0; q < 32; q++) bsum += (float)st[row * SROW + q];
+        }
+    }
+    float* P = partial + (size_t)blockIdx.x * GP_N;
+#pragma unroll
+    for (int i = 0; i < TILES_PER_WAVE; i++) {
+        const WTile T = tile_of(wave * TILES_PER_WAVE + i);
+        const int k = T.kt * 16 + r;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int nn = T.nt * 16 + 4 * g + e;
+            if (T.layer == 1) {
+                const int row = feat_of_slot(k >> 5, (k & 31) >> 3, k & 7);
+                if (row >= 0) P[GP_W1 + row * 64 + nn] = acc[i][e];
+            } else if (T.layer == 5) {
+                if (nn < 41) P[GP_W5 + k * 41 + nn] = acc[i][e];
+            } else {
+                const int W = T.layer == 2 ? GP_W2 : (T.layer == 3 ? GP_W3 : GP_W4);
+                P[W + k * 64 + nn] = acc[i][e];
+            }
+        }
+    }
+    if (threadIdx.x < 240) {
+        const int t = threadIdx.x;
+        if (t < 64) P[GP_B2 + t] = bsum;
+        else if (t < 128) P[GP_B3 + t - 64] = bsum;
+        else if (t < 192) P[GP_B4 + t - 128] = bsum;
+        else if (t - 192 < 41) P[GP_B5 + t - 192] = bsum;
+    }
 }
 
 // Loss weights of the records of one pass: w = mean over channels of
@@ -754,7 +826,7 @@ __global__ __launch_bounds__(256) void k_guided_shade(RScene s, RParams p, Guide
     for (int t = threadIdx.x; t < N_BIAS; t += blockDim.x) bl[t] = gl.dev.bias[t];
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    bf16x8 dummy[4][4][2];
+    NoSink dummy;
     const uint32_t n = ctl->n_trace;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -818,7 +890,7 @@ __global__ __launch_bounds__(256) void k_guided_shade(RScene s, RParams p, Guide
 #pragma unroll
         for (int k = 0; k < 7; k++) xs[wave][lane * 8 + k] = x[k];
         __syncthreads();
-        if (__ballot(guided)) mlp_forward<4, false>(wl, bl, xs[wave], ys[wave], dummy);   // wave-uniform
+        if (__ballot(guided)) mlp_forward<4>(wl, bl, xs[wave], ys[wave], dummy);   // wave-uniform
         __syncthreads();
         if (guided) {
             Mix mx;
@@ -898,7 +970,7 @@ struct lmx_guide {
     lmx_scene* scene = nullptr;
     lmx_guide_opts opts;
     DevBuf<float> P, m, v, grad, partial, loss, wsum;
-    DevBuf<__bf16> pack;
+    DevBuf<__bf16> pack, act;
     DevBuf<float> bias;
     DevBuf<float4> recs;
     DevBuf<uint32_t> rec_count, idx, nz, nz_count;
@@ -931,11 +1003,21 @@ int guide_pack(lmx_guide* g, hipStream_t st)
 int guide_train_step(lmx_guide* g, const float4* recs, const uint32_t* idx, uint32_t n, float wnorm, float wclamp,
                      bool adam, float* grad_out, hipStream_t st)
 {
-    const int nwg = std::max(1, std::min((int)((n + 31) / 32 + TR_WAVES - 1) / TR_WAVES, g->nparts));
     hipError_t e;
+    const uint32_t npass = std::max<uint32_t>((n + 31) / 32, 1);
+    if (g->act.n < (size_t)NROWS * 32 * npass) {
+        if ((e = hipStreamSynchronize(st)) != hipSuccess || (e = g->act.alloc((size_t)NROWS * 32 * npass)) != hipSuccess)
+            return hip_error(e, "guide training scratch");
+    }
+    // forward/backward: one wave per 32-record pass, up to 8 waves per CU
+    const int fb_wg = std::max(1, std::min((int)((npass + FB_WAVES - 1) / FB_WAVES), 2 * g->nparts));
+    // weight gradients: at least 8 passes per workgroup
+    const int nwg = std::max(1, std::min((int)((npass + 7) / 8), g->nparts));
+    const uint32_t chunk = (npass + nwg - 1) / nwg;
     k_fill<<<1, 64, 0, st>>>(g->loss.p, 1, 0.0f);
-    k_train<<<nwg, TR_WAVES * 64, 0, st>>>(g->dev(), recs, idx, n, wnorm, wclamp, 1.0f / (float)n, g->partial.p,
-                                           g->loss.p);
+    k_train_fb<<<fb_wg, FB_WAVES * 64, 0, st>>>(g->dev(), recs, idx, n, wnorm, wclamp, 1.0f / (float)n, g->act.p,
+                                                g->loss.p);
+    k_wgrad<<<nwg, WG_WAVES * 64, 0, st>>>(g->act.p, npass, chunk, g->partial.p);
     float lr = 0.0f, c1 = 1.0f, c2 = 1.0f;
     if (adam) {
         g->step++;
@@ -994,7 +1076,8 @@ int lmx_guide_create(lmx_scene* sc, const lmx_guide_opts* o, lmx_guide** out)
         (e = g->partial.alloc((size_t)g->nparts * GP_N)) != hipSuccess || (e = g->loss.alloc(1)) != hipSuccess ||
         (e = g->wsum.alloc(1)) != hipSuccess || (e = g->pack.alloc((size_t)N_PACK * 8)) != hipSuccess ||
         (e = g->bias.alloc(N_BIAS)) != hipSuccess || (e = g->rec_count.alloc(1)) != hipSuccess ||
-        (e = g->idx.alloc(g->opts.batch)) != hipSuccess || (e = g->nz_count.alloc(1)) != hipSuccess)
+        (e = g->idx.alloc(g->opts.batch)) != hipSuccess || (e = g->nz_count.alloc(1)) != hipSuccess ||
+        (e = g->act.alloc((size_t)NROWS * ((g->opts.batch + 31) / 32 * 32))) != hipSuccess)
         return hip_error(e, "lmx_guide_create: device memory");
     std::vector<float> P;
     init_params(P, g->opts.seed);
