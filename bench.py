@@ -143,12 +143,12 @@ def bdpt_section(dev, cpu, threads):
     rs = render.Scene(sc)
     n = 1024 * 1024 * 16
     kw = dict(max_num_vertices=10, seed=42)
-    rs.render_bdpt(1 << 22, **kw)                     # autotune + warm-up
+    rs.render_vcm_bdpt(1 << 22, **kw)                     # autotune + warm-up
     torch.cuda.synchronize()
     times = []
     for _ in range(3):
         t0 = time.perf_counter()
-        film, st, _ = rs.render_bdpt(n, **kw)
+        film, st, _ = rs.render_vcm_bdpt(n, **kw)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
     dt = min(times)

@@ -162,6 +162,10 @@ typedef struct lmx_scene lmx_scene;
  * throughput of surviving paths from growing */
 #define LMX_RR_FIXED 0
 #define LMX_RR_THROUGHPUT 1
+/* Multiple-importance weights of renderer::bdpt (misweight::powerheuristics,
+ * misweight::simple; renderer_bdpt.cpp:704-787) */
+#define LMX_MIS_POWER 0
+#define LMX_MIS_SIMPLE 1
 
 typedef struct lmx_render_params {
     int32_t max_num_vertices;    /* -1: unbounded (renderer_pt.cpp:59) */
@@ -176,6 +180,7 @@ typedef struct lmx_render_params {
     uint32_t table_width;
     float *sample_out;           /* device, num_samples x 8 floats (raster x, y, C rgb, #splats, 0, 0), or NULL */
     int32_t rr_mode;             /* LMX_RR_FIXED (reference) or LMX_RR_THROUGHPUT */
+    int32_t mis;                 /* renderer::bdpt: LMX_MIS_POWER (reference default) or LMX_MIS_SIMPLE */
 } lmx_render_params;
 
 typedef struct lmx_render_stats {
@@ -227,6 +232,28 @@ int lmx_render_ptmis(lmx_scene *scene, const lmx_render_params *params, float *f
  * (weighted, before the W H / N film scale), 1, MIS weight, 0.  The light
  * position and emission direction use independent numbers (the reference
  * draws both from one, light_area.cpp:72-80, which biases s >= 2). */
+int lmx_render_vcm_bdpt(lmx_scene *scene, const lmx_render_params *params, float *film_rgb, void *stream,
+                        lmx_render_stats *stats);
+
+/* renderer::bdpt (renderer_bdpt.cpp:814-1000): an eye and a light subpath per
+ * sample with Russian roulette 0.5 after every vertex but the endpoint, each
+ * subpath vertex also sampling a "direct" vertex on an emitter of the other
+ * kind; every strategy (s, t) plus the direct ones (s = 0: a light point
+ * sampled from eye vertex t - 2) of every length n, weighed by `mis` over
+ * all of them and divided by the roulette's selection probability.  The
+ * light subpath's direct vertices (points on a pinhole sensor) carry nothing
+ * and are not sampled.  max_num_vertices: -1 (unbounded, the reference's
+ * default; subpaths are cut at 32 vertices, reached with probability 2^-30)
+ * or 2..32; min_num_vertices applies only when max_num_vertices is given
+ * (renderer_bdpt.cpp:887).  queue_capacity as for lmx_render_vcm_bdpt.
+ * Uniform table width >= 2 (8 + 8 (M - 1)), M = max_num_vertices or 32: an
+ * eye block then a light block; within a block 1-2 endpoint position,
+ * 3 emitter selection, 4-5 endpoint direction, and step j >= 1 (subpath
+ * vertex j) owns 8 + 8 (j - 1) .. +7: 0-1 direction sampled at vertex j - 1
+ * (j >= 2), 3 roulette after vertex j, 4 light selection and 5-6 light
+ * position of the direct vertex sampled from eye vertex j - 1.  sample_out,
+ * when given, holds 2 (M + 1)^2 records of 8 floats per sample at index
+ * 2 (t (M + 1) + s) + direct, laid out as lmx_render_vcm_bdpt's. */
 int lmx_render_bdpt(lmx_scene *scene, const lmx_render_params *params, float *film_rgb, void *stream,
                     lmx_render_stats *stats);
 

@@ -1,7 +1,19 @@
-// lmx_bdpt.hip -- the bidirectional path sampler of renderer::vcm in bdpt
-// mode (src/liblightmetrica/renderer/renderer_vcm.cpp:51-341,662-713, with
-// SubpathSampler::TraceSubpath_, subpathsampler.cpp:42-203) as a batched
-// wavefront pipeline on gfx950.
+// lmx_bdpt.hip -- the bidirectional path samplers as one batched wavefront
+// pipeline on gfx950, in two variants:
+//   vcm  -- renderer::vcm in bdpt mode (src/liblightmetrica/renderer/
+//           renderer_vcm.cpp:51-341,662-713, with SubpathSampler::TraceSubpath_,
+//           subpathsampler.cpp:42-203): subpaths without roulette;
+//   bdpt -- renderer::bdpt (renderer_bdpt.cpp:50-1002): roulette 0.5 after
+//           every subpath vertex but the endpoint (compensated by
+//           1 / SelectionPDF, here a factor 2 per survived roulette in the
+//           subpath throughput), plus the "direct" strategy of every eye
+//           vertex: a light point sampled from it (next-event estimation,
+//           s = 0 with a direct vertex), with a power-heuristic or
+//           1 / #strategies weight over all strategies including those.
+//           The light subpath's direct vertices (a point on the sensor) are
+//           not sampled: with a pinhole, whose position is a delta, their
+//           strategies carry nothing (EvaluateCst evaluates the sensor's
+//           position with evalDelta and gets 0, :389-393).
 //
 // Per batch of B samples:
 //   gen      -- the eye and light endpoints of every sample and their first
@@ -23,6 +35,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -50,12 +63,23 @@ namespace {
 // side (the camera vertex is excluded: a pinhole is never hit).  It obeys
 // S_j = (pr_j / pf_j)^2 (1 + S_{j-1}), so every connection's MIS weight
 // takes O(1) work; U < 0 marks a subpath with a zero pf in that range.
+// With the simple weight the square becomes the indicator (ratio != 0), and
+// the same recursion counts the strategies with a nonzero pdf.
+// The bdpt variant also keeps, per light vertex k >= 2, V_k = prod over
+// m = 1 .. k-2 of (pr_m / pf_m)^2: with it the ratio of the direct
+// strategy's pdf (equal to that of s = 1) to any s is O(1) as well.
 #define VK_CAMERA 1u
 #define VK_LIGHT 2u
 #define VK_SURFACE 3u
 #define VF_DEGENERATE 4u
 
+#define BD_VCM 0
+#define BD_BDPT 1
+
 struct BParams {
+    int32_t variant;           // BD_VCM or BD_BDPT
+    int32_t mis;               // LMX_MIS_POWER or LMX_MIS_SIMPLE (bdpt)
+    int32_t bounded;           // bdpt: max_num_vertices given (else capped at maxv)
     int32_t maxv, minv;
     uint2 key;
     uint32_t total;
@@ -73,6 +97,7 @@ struct VB {
     uint32_t* nsub[2];
     uint32_t* epix;
     float2* eraster;
+    float* V;                  // bdpt: V_k of the light vertices
 };
 
 struct Vtx {
@@ -133,16 +158,24 @@ RD void store_pr(const VB& vb, const BParams& p, int kind, uint32_t i, uint32_t 
 
 RD float gterm(const Vtx& a, const Vtx& b) { return gterm(a.p, a.sn, a.degenerate, b.p, b.sn, b.degenerate); }
 
+// the weight heuristic's transform of one pdf ratio: its square (power
+// heuristic) or whether it is nonzero (simple weight)
+RD float sqm(float x, int mis) { return mis == LMX_MIS_SIMPLE ? (x != 0.0f ? 1.0f : 0.0f) : x * x; }
+
 // a^2 (1 + b^2 (1 + U)): the sum over one side of the squared pdf ratios of
 // the strategies that move the junction 1, 2, 3, ... vertices into it; a
 // zero factor ends the chain (the later ratios contain it).
-RD float chain_sq(float a, float b, float U)
+RD float chain_sq(float a, float b, float U, int mis)
 {
     if (a == 0.0f) return 0.0f;
-    const float a2 = a * a;
+    const float a2 = sqm(a, mis);
     if (b == 0.0f) return a2;
-    return a2 * (1.0f + b * b * (1.0f + U));
+    return a2 * (1.0f + sqm(b, mis) * (1.0f + U));
 }
+
+// uniform slots of the bdpt variant (within a subpath block): step j >= 1
+// owns 8 + 8 (j - 1) .. +7
+RD uint32_t rb_step(uint32_t j) { return LMX_PT_SLOT_SEG0 + 8u * (j - 1u); }
 
 // ---------------------------------------------------------------------------
 
@@ -281,23 +314,35 @@ __global__ __launch_bounds__(256) void k_bd_shade(RScene s, BParams p, BCtl* __r
                     const bool bad = prev2.pf == 0.0f || prev.U < 0.0f;
                     const float rho = bad ? 0.0f : pr2 / prev2.pf;
                     if (bad && (k > 2 || kind == 1)) U = -1.0f;
-                    else if (k == 2) U = kind == 1 ? rho * rho : 0.0f;
-                    else U = rho == 0.0f ? 0.0f : rho * rho * (1.0f + prev.U);
+                    else if (k == 2) U = kind == 1 ? sqm(rho, p.mis) : 0.0f;
+                    else U = rho == 0.0f ? 0.0f : sqm(rho, p.mis) * (1.0f + prev.U);
+                    // V_k = V_{k-1} (pr_{k-2} / pf_{k-2})^2, V_2 = 1
+                    if (p.variant == BD_BDPT && kind == 1) {
+                        float* Vi = vb.V + (size_t)i * (size_t)p.maxv;
+                        Vi[k] = k == 2 ? 1.0f : (bad ? 0.0f : Vi[k - 1] * sqm(rho, p.mis));
+                    }
                 }
                 store_v(vb, p, kind, i, k, fr.p, vflags(VK_SURFACE, false, light, mat), fr.sn, pf, fr.gn,
                         0.0f, alpha, U);
                 vb.nsub[kind][i] = k + 1;
                 const RMat m = s.mats[mat];
-                if ((int)k + 1 < p.maxv && is_scatterer(m)) {
+                const Uniforms<BParams> Un{p, ctl->batch_first + i};
+                const uint32_t blk = kind ? p.wsub : 0u;
+                // bdpt: roulette after the vertex (renderer_bdpt.cpp:205-211),
+                // the survivors' throughput doubled (1 / SelectionPDF)
+                const bool rr_ok = p.variant != BD_BDPT || Un.one(blk + rb_step(k) + 3u) <= 0.5f;
+                if ((int)k + 1 < p.maxv && is_scatterer(m) && rr_ok) {
                     const f3 wi = dir_to(fr.p, prev.p);
-                    const Uniforms<BParams> U{p, ctl->batch_first + i};
-                    const float4 u = U.block((kind ? p.wsub : 0u) + LMX_PT_SLOT_SEG0 + 4u * (k - 1u));
+                    const float4 u = Un.block(blk + (p.variant == BD_BDPT ? rb_step(k + 1u)
+                                                                           : LMX_PT_SLOT_SEG0 + 4u * (k - 1u)));
                     f3 wo;
                     if (bsdf_sample(m, fr, wi, u.x, u.y, wo)) {
                         const f3 fs = bsdf_eval_t(m, fr, wi, wo, kind == 1);
                         if (!(fs.x == 0.0f && fs.y == 0.0f && fs.z == 0.0f)) {
                             const float pdf = bsdf_pdf(m, fr, wi, wo);
-                            const f3 a2 = mk3(alpha.x * (fs.x / pdf), alpha.y * (fs.y / pdf), alpha.z * (fs.z / pdf));
+                            const float sel = p.variant == BD_BDPT ? 2.0f : 1.0f;
+                            const f3 a2 = mk3(alpha.x * (fs.x / pdf) * sel, alpha.y * (fs.y / pdf) * sel,
+                                              alpha.z * (fs.z / pdf) * sel);
                             o0 = make_float4(fr.p.x, fr.p.y, fr.p.z, LMX_EPS_ISECT);
                             o1 = make_float4(wo.x, wo.y, wo.z, LMX_FLT_MAX);
                             ot = make_float4(a2.x, a2.y, a2.z, pdf);
@@ -334,11 +379,16 @@ struct ShadowQ {
     uint4* meta;               // sample, s, t, pixel
 };
 
-RD void record(const BParams& p, uint32_t sample, int s, int t, float rx, float ry, f3 C, float w)
+// sample_out record of strategy (s, t, direct): vcm (M+1)^2 records at
+// t (M+1) + s, bdpt 2 (M+1)^2 at 2 (t (M+1) + s) + direct
+RD void record(const BParams& p, uint32_t sample, int s, int t, int direct, float rx, float ry, f3 C, float w)
 {
     if (!p.sample_out) return;
-    float4* o = reinterpret_cast<float4*>(p.sample_out +
-        8 * ((size_t)sample * (size_t)((p.maxv + 1) * (p.maxv + 1)) + (size_t)(t * (p.maxv + 1) + s)));
+    const size_t M1 = (size_t)(p.maxv + 1);
+    const size_t idx = p.variant == BD_BDPT ? 2 * ((size_t)t * M1 + (size_t)s) + (size_t)direct
+                                            : (size_t)t * M1 + (size_t)s;
+    const size_t per = p.variant == BD_BDPT ? 2 * M1 * M1 : M1 * M1;
+    float4* o = reinterpret_cast<float4*>(p.sample_out + 8 * ((size_t)sample * per + idx));
     o[0] = make_float4(rx, ry, C.x, C.y);
     o[1] = make_float4(C.z, 1.0f, w, 0.0f);
 }
@@ -349,11 +399,28 @@ RD void splat(const BParams& p, uint32_t pixel, f3 C)
     atomicAdd(fp, C.x); atomicAdd(fp + 1, C.y); atomicAdd(fp + 2, C.z);
 }
 
-// One thread per (sample, t): every s of renderer_vcm.cpp:676-713.
+// the smallest path length a strategy may have (renderer_bdpt.cpp:887 applies
+// min_num_vertices only when max_num_vertices is given)
+RD int min_len(const BParams& p) { return (p.variant == BD_BDPT && !p.bounded) ? 0 : p.minv; }
+
+// bdpt: does eye vertex k sample a direct (light) vertex?  Step k + 1 runs
+// (k + 1 < maxv, and the roulette after vertex k >= 1 passed), and the
+// vertex scatters (the camera's importance for k = 0).
+RD bool eye_direct(const RScene& s, const BParams& p, const VB& vb, uint32_t i, uint32_t sample, int k)
+{
+    if (s.nlight == 0 || k + 1 >= p.maxv || k + 2 < min_len(p)) return false;
+    if (k == 0) return true;
+    const Uniforms<BParams> Un{p, sample};
+    if (!(Un.one(rb_step((uint32_t)k) + 3u) <= 0.5f)) return false;
+    const float4 a = vb.v[0][((size_t)i * (size_t)p.maxv + (size_t)k) * 4];
+    return is_scatterer(s.mats[__float_as_uint(a.w) >> 16]);
+}
+
 // Connection plan: one thread per sample.  Strategies s = 0 (the eye
 // subpath hit a light) are evaluated and splatted here; every structurally
-// possible s >= 1 gets a shadow-queue slot, reserved with one atomic per
-// workgroup, whose (sample, s, t) the evaluation step reads.
+// possible s >= 1 gets a shadow-queue slot, and so does (bdpt) every direct
+// strategy, reserved with one atomic per workgroup, whose (sample, s, t,
+// direct) the evaluation step reads.
 __global__ __launch_bounds__(256) void k_bd_plan(RScene s, BParams p, BCtl* __restrict__ ctl, VB vb, ShadowQ sq)
 {
     __shared__ uint32_t lds[5];
@@ -362,51 +429,54 @@ __global__ __launch_bounds__(256) void k_bd_plan(RScene s, BParams p, BCtl* __re
     const bool valid = i < p.batch && sample < p.total;
     const int nE = valid ? (int)vb.nsub[0][i] : 0;
     const int nL = valid ? (int)vb.nsub[1][i] : 0;
-    // s range of each t: [max(2 - t, minv - t, 0), min(maxv - t, nL)]
+    const int minv = min_len(p);
+    const bool bdpt = p.variant == BD_BDPT;
+    // s range of each t: [max(2 - t, minv - t, 1), min(maxv - t, nL)]; the
+    // unbounded bdpt variant takes every s <= nL
     uint32_t cnt = 0;
     for (int t = 1; t <= nE; t++) {
-        const int s1 = max(max(1, 2 - t), p.minv - t);
-        const int s2 = min(p.maxv - t, nL);
+        const int s1 = max(max(1, 2 - t), minv - t);
+        const int s2 = (bdpt && !p.bounded) ? nL : min(p.maxv - t, nL);
         if (s2 >= s1) cnt += (uint32_t)(s2 - s1 + 1);
     }
-    uint32_t slot = block_reserve<4>(&ctl->n_shadow, cnt, lds);
+    uint32_t ndirect = 0;
+    if (bdpt)
+        for (int k = 0; k < nE; k++) ndirect += eye_direct(s, p, vb, i, sample, k) ? 1u : 0u;
+    uint32_t slot = block_reserve<4>(&ctl->n_shadow, cnt + ndirect, lds);
     for (int t = 1; t <= nE; t++) {
-        const int s1 = max(max(1, 2 - t), p.minv - t);
-        const int s2 = min(p.maxv - t, nL);
+        const int s1 = max(max(1, 2 - t), minv - t);
+        const int s2 = (bdpt && !p.bounded) ? nL : min(p.maxv - t, nL);
         for (int sv = s1; sv <= s2; sv++) sq.meta[slot++] = make_uint4(i, (uint32_t)sv, (uint32_t)t, 0u);
     }
+    // a direct strategy is evaluated as the s = 1 connection of a fresh light
+    // point with eye vertex k (its pdf equals that of s = 1); flag in .w
+    if (ndirect)
+        for (int k = 0; k < nE; k++)
+            if (eye_direct(s, p, vb, i, sample, k)) sq.meta[slot++] = make_uint4(i, 1u, (uint32_t)k + 1u, 1u);
     if (!valid) return;
     const uint32_t pixel = vb.epix[i];
     const float2 eras = vb.eraster[i];
     for (int t = 2; t <= nE; t++) {
-        if (max(0, p.minv - t) > 0) continue;          // s = 0 excluded by min_num_vertices
+        if (t < minv) continue;                        // s = 0 excluded by min_num_vertices
         const Vtx vE = load_v(vb, p, 0, i, (uint32_t)t - 1);
         if (!(vE.light >= 0 && vE.kind == VK_SURFACE)) continue;
         const Vtx vEp = load_v(vb, p, 0, i, (uint32_t)t - 2);
-        float w = 0.0f, rx = 0.0f, ry = 0.0f;
-        f3 C;
-        {
-            // the eye subpath hit a light (t >= 2)
-            if (vE.light >= 0 && vE.kind == VK_SURFACE) {
-                const f3 d = dir_to(vE.p, vEp.p);
-                if (emits_towards(vE.fr, d)) {
-                    const RMat m = s.mats[vE.mat];
-                    const f3 Cs = mk3(vE.alpha.x * m.Le[0], vE.alpha.y * m.Le[1], vE.alpha.z * m.Le[2]);
-                    // MIS: strategies s' = 1 .. n-1 relative to s = 0
-                    const bool ok = vE.pf > 0.0f && (t < 3 || (vEp.pf > 0.0f && vE.U >= 0.0f));
-                    const float sum = ok ? 1.0f + chain_sq(pos_pdf_light(s, vE.light) / vE.pf,
-                                                           t >= 3 ? LMX_INV_PI_F * gterm(vE, vEp) / vEp.pf : 0.0f,
-                                                           vE.U) : 0.0f;
-                    if (ok) {
-                        w = 1.0f / sum;
-                        C = mk3(Cs.x * w, Cs.y * w, Cs.z * w);
-                        rx = eras.x; ry = eras.y;
-                        splat(p, pixel, C);
-                        record(p, sample, 0, t, rx, ry, C, w);
-                    }
-                }
-            }
-        }
+        // the eye subpath hit a light (t >= 2)
+        const f3 d = dir_to(vE.p, vEp.p);
+        if (!emits_towards(vE.fr, d)) continue;
+        const RMat m = s.mats[vE.mat];
+        const f3 Cs = mk3(vE.alpha.x * m.Le[0], vE.alpha.y * m.Le[1], vE.alpha.z * m.Le[2]);
+        // MIS: strategies s' = 1 .. n-1 relative to s = 0, and (bdpt) the
+        // direct strategy, whose pdf is that of s' = 1
+        const bool ok = vE.pf > 0.0f && (t < 3 || (vEp.pf > 0.0f && vE.U >= 0.0f));
+        if (!ok) continue;
+        const float a = pos_pdf_light(s, vE.light) / vE.pf;
+        float sum = 1.0f + chain_sq(a, t >= 3 ? LMX_INV_PI_F * gterm(vE, vEp) / vEp.pf : 0.0f, vE.U, p.mis);
+        if (bdpt) sum += sqm(a, p.mis);
+        const float w = 1.0f / sum;
+        const f3 C = mk3(Cs.x * w, Cs.y * w, Cs.z * w);
+        splat(p, pixel, C);
+        record(p, sample, 0, t, 0, eras.x, eras.y, C, w);
     }
 }
 
@@ -420,6 +490,7 @@ __global__ __launch_bounds__(256) void k_bd_connect(RScene s, BParams p, BCtl* _
         const uint4 pm = sq.meta[j];
         const uint32_t i = pm.x;
         const int sv = (int)pm.y, t = (int)pm.z;
+        const bool direct = pm.w == 1u;                  // bdpt: light point sampled from eye vertex t - 1
         const uint32_t sample = ctl->batch_first + i;
         const Vtx vE = load_v(vb, p, 0, i, (uint32_t)t - 1);
         Vtx vEp;
@@ -430,7 +501,29 @@ __global__ __launch_bounds__(256) void k_bd_connect(RScene s, BParams p, BCtl* _
         f3 C = mk3(0, 0, 0);
         float w = 0.0f, rx = 0.0f, ry = 0.0f;
         float4 r0, r1;
-        const Vtx vL = load_v(vb, p, 1, i, (uint32_t)sv - 1);
+        Vtx vL;
+        if (direct) {
+            // the direct vertex of step t (renderer_bdpt.cpp:166-190): light by
+            // uniform selection, point by area (light_area.cpp:67-70)
+            const Uniforms<BParams> Un{p, sample};
+            const uint32_t b = rb_step((uint32_t)t);
+            int l = (int)(Un.one(b + 4u) * (float)s.nlight);
+            l = l < 0 ? 0 : (l > (int)s.nlight - 1 ? (int)s.nlight - 1 : l);
+            f3 gn;
+            uint32_t tri;
+            sample_light_point(s, l, Un.one(b + 5u), Un.one(b + 6u), vL.p, gn, tri);
+            vL.sn = gn; vL.gn = gn;
+            vL.kind = VK_LIGHT; vL.degenerate = false; vL.light = l;
+            vL.mat = s.tri_mat[tri];
+            vL.pf = pos_pdf_light(s, l); vL.pr = 0.0f;
+            const float a0 = 1.0f / vL.pf;
+            vL.alpha = mk3(a0, a0, a0);
+            vL.U = 0.0f;
+            vL.fr.p = vL.p; vL.fr.sn = gn; vL.fr.gn = gn;
+            onb(gn, vL.fr.u, vL.fr.v);
+        } else {
+            vL = load_v(vb, p, 1, i, (uint32_t)sv - 1);
+        }
         Vtx vLp;
         if (sv >= 2) vLp = load_v(vb, p, 1, i, (uint32_t)sv - 2);
         const f3 dLE = dir_to(vL.p, vE.p);
@@ -485,12 +578,23 @@ __global__ __launch_bounds__(256) void k_bd_connect(RScene s, BParams p, BCtl* _
         if (t >= 2 && (vE.pf == 0.0f || (t >= 3 && (vEp.pf == 0.0f || vE.U < 0.0f)))) ok = false;
         float sum = 1.0f;
         if (ok) {
-            sum += chain_sq(pEsm1 / vL.pf, sv >= 2 ? pEsm2 / vLp.pf : 0.0f, vL.U);
-            if (t >= 2) sum += chain_sq(pLs / vE.pf, t >= 3 ? pLsp1 / vEp.pf : 0.0f, vE.U);
+            const float aL = pEsm1 / vL.pf, bL = sv >= 2 ? pEsm2 / vLp.pf : 0.0f;
+            sum += chain_sq(aL, bL, vL.U, p.mis);
+            if (t >= 2) sum += chain_sq(pLs / vE.pf, t >= 3 ? pLsp1 / vEp.pf : 0.0f, vE.U, p.mis);
+            // bdpt: the direct strategy, whose pdf equals that of s = 1
+            // (p_1 / p_s = the light side's ratios down to vertex 1)
+            if (p.variant == BD_BDPT) {
+                if (sv == 1) sum += 1.0f;
+                else if (sv == 2) sum += sqm(aL, p.mis);
+                else sum += sqm(aL, p.mis) * sqm(bL, p.mis) * vb.V[(size_t)i * (size_t)p.maxv + (size_t)(sv - 1)];
+            }
         }
         if (ok) {
             w = 1.0f / sum;
-            C = mk3(Cs.x * w, Cs.y * w, Cs.z * w);
+            // a direct strategy from eye vertex t - 1 >= 1 ran one roulette
+            // more than that vertex's throughput holds
+            const float sel = (direct && t >= 2) ? 2.0f : 1.0f;
+            C = mk3(Cs.x * w * sel, Cs.y * w * sel, Cs.z * w * sel);
             // Scene3::Visible (scene3.h:107-116)
             f3 d = sub(vE.p, vL.p);
             const float l = sqrtf(dot(d, d));
@@ -504,7 +608,7 @@ __global__ __launch_bounds__(256) void k_bd_connect(RScene s, BParams p, BCtl* _
             sq.ray[2 * j + 1] = r1;
             sq.c[j] = make_float4(C.x, C.y, C.z, w);
             sq.aux[j] = make_float4(rx, ry, 0.0f, 0.0f);
-            sq.meta[j] = make_uint4(sample, (uint32_t)sv, (uint32_t)t, pixel);
+            sq.meta[j] = make_uint4(sample, (uint32_t)sv, (uint32_t)t | (direct ? 0x10000u : 0u), pixel);
         } else {
             sq.ray[2 * j] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
             sq.ray[2 * j + 1] = make_float4(0.0f, 0.0f, 1.0f, -1.0f);
@@ -526,7 +630,10 @@ __global__ __launch_bounds__(256) void k_bd_splat(BParams p, BCtl* __restrict__ 
         const float4 c = sq.c[j];
         const float4 a = sq.aux[j];
         splat(p, m.w, mk3(c.x, c.y, c.z));
-        record(p, m.x, (int)m.y, (int)m.z, a.x, a.y, mk3(c.x, c.y, c.z), c.w);
+        // a direct strategy's path has the light point as vertex 0 and the
+        // eye vertices 0 .. t-1: recorded as (s = 0, t + 1, direct)
+        const int t = (int)(m.z & 0xffffu), direct = (int)(m.z >> 16);
+        record(p, m.x, direct ? 0 : (int)m.y, direct ? t + 1 : t, direct, a.x, a.y, mk3(c.x, c.y, c.z), c.w);
     }
     // live shadow rays, one atomic per wave
 #pragma unroll
@@ -560,26 +667,35 @@ __global__ void k_bd_scale(float* a, size_t n, float w)
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] *= w;
 }
 
-// Device bytes of one batch slot: two subpaths of vertex records, their
-// queues and hits, and the shadow-ray records of every connection.
-size_t bd_bytes_per_sample(uint32_t maxv)
+// Shadow-queue slots one sample may need: the (s >= 1, t >= 1) connections
+// with s + t <= maxv (vcm and bounded bdpt) or s, t <= maxv (unbounded bdpt),
+// plus one s = 0 marker (vcm) or up to maxv - 1 direct strategies (bdpt).
+uint32_t bd_conn_per_sample(uint32_t maxv, int variant, bool bounded)
 {
-    const size_t conn = (size_t)(maxv * (maxv - 1) / 2 + 1);
-    return 2 * ((size_t)maxv * 64 + 4 + 4 * 16 + 2 * 16 + 2 * 16) + 4 + 8 + 2 * 16 + conn * (2 * 16 + 4 * 16);
+    if (variant == BD_VCM) return maxv * (maxv - 1) / 2 + 1;
+    return (bounded ? maxv * (maxv - 1) / 2 : maxv * maxv) + maxv;
 }
 
-// Buffers for batches of B samples of up to maxv vertices; kept when the
-// current ones are large enough (the layouts only need B maxv and B conn(maxv)
-// slots).
-int bd_buffers(lmx_scene* sc, uint32_t B, uint32_t maxv)
+// Device bytes of one batch slot: two subpaths of vertex records, their
+// queues and hits, and the shadow-ray records of every connection.
+size_t bd_bytes_per_sample(uint32_t maxv, uint32_t conn)
 {
-    if (B <= sc->bd_batch && maxv <= sc->bd_maxv) return LMX_OK;
+    return 2 * ((size_t)maxv * 64 + 4 + 4 * 16 + 2 * 16 + 2 * 16) + (size_t)maxv * 4 + 4 + 8 + 2 * 16 +
+           (size_t)conn * (2 * 16 + 4 * 16);
+}
+
+// Buffers for batches of B samples of up to maxv vertices and conn
+// connections each; kept when the current ones are large enough.
+int bd_buffers(lmx_scene* sc, uint32_t B, uint32_t maxv, uint32_t conn)
+{
+    if (B <= sc->bd_batch && maxv <= sc->bd_maxv && conn <= sc->bd_conn) return LMX_OK;
     B = std::max(B, sc->bd_batch);
     maxv = std::max(maxv, sc->bd_maxv);
-    sc->bd_batch = sc->bd_maxv = 0;
+    conn = std::max(conn, sc->bd_conn);
+    sc->bd_batch = sc->bd_maxv = sc->bd_conn = 0;
     hipError_t e;
     const size_t nv = (size_t)B * maxv * 4;
-    const size_t nconn = (size_t)B * (size_t)(maxv * (maxv - 1) / 2 + 1);
+    const size_t nconn = (size_t)B * conn;
     for (int k = 0; k < 2; k++) {
         if ((e = sc->bd_vert[k].alloc(nv)) != hipSuccess) return hip_error(e, "bdpt vertex buffers");
         if ((e = sc->bd_nsub[k].alloc(B)) != hipSuccess) return hip_error(e, "bdpt buffers");
@@ -587,7 +703,8 @@ int bd_buffers(lmx_scene* sc, uint32_t B, uint32_t maxv)
         if ((e = sc->bd_thr[k].alloc(2 * (size_t)B)) != hipSuccess) return hip_error(e, "bdpt queues");
         if ((e = sc->bd_meta[k].alloc(2 * (size_t)B)) != hipSuccess) return hip_error(e, "bdpt queues");
     }
-    if ((e = sc->bd_epix.alloc(B)) != hipSuccess || (e = sc->bd_eraster.alloc(B)) != hipSuccess ||
+    if ((e = sc->bd_V.alloc((size_t)B * maxv)) != hipSuccess || (e = sc->bd_epix.alloc(B)) != hipSuccess ||
+        (e = sc->bd_eraster.alloc(B)) != hipSuccess ||
         (e = sc->bd_hits.alloc(2 * (size_t)B)) != hipSuccess || (e = sc->bd_sray.alloc(2 * nconn)) != hipSuccess ||
         (e = sc->bd_sc.alloc(nconn)) != hipSuccess || (e = sc->bd_saux.alloc(nconn)) != hipSuccess ||
         (e = sc->bd_shits.alloc(nconn)) != hipSuccess || (e = sc->bd_smeta.alloc(nconn)) != hipSuccess)
@@ -597,6 +714,7 @@ int bd_buffers(lmx_scene* sc, uint32_t B, uint32_t maxv)
         return hip_error(e, "bdpt pinned control block");
     sc->bd_batch = B;
     sc->bd_maxv = maxv;
+    sc->bd_conn = conn;
     return LMX_OK;
 }
 
@@ -609,6 +727,7 @@ int bd_enqueue_batch(lmx_scene* sc, const BParams& bp, hipStream_t st)
     vb.v[0] = sc->bd_vert[0].p; vb.v[1] = sc->bd_vert[1].p;
     vb.nsub[0] = sc->bd_nsub[0].p; vb.nsub[1] = sc->bd_nsub[1].p;
     vb.epix = sc->bd_epix.p; vb.eraster = sc->bd_eraster.p;
+    vb.V = sc->bd_V.p;
     Queue q[2];
     for (int k = 0; k < 2; k++) { q[k].ray = sc->bd_ray[k].p; q[k].thr = sc->bd_thr[k].p; q[k].meta = sc->bd_meta[k].p; }
     BCtl* ctl = sc->bd_ctl.p;
@@ -626,7 +745,7 @@ int bd_enqueue_batch(lmx_scene* sc, const BParams& bp, hipStream_t st)
     ShadowQ sq{sc->bd_sray.p, sc->bd_sc.p, sc->bd_saux.p, sc->bd_smeta.p};
     k_bd_plan<<<(B + 255) / 256, 256, 0, st>>>(sc->rs, bp, ctl, vb, sq);
     k_bd_connect<<<cap_grid, 256, 0, st>>>(sc->rs, bp, ctl, vb, sq);
-    const uint32_t nconn = B * (uint32_t)(bp.maxv * (bp.maxv - 1) / 2 + 1);
+    const uint32_t nconn = B * bd_conn_per_sample((uint32_t)bp.maxv, bp.variant, bp.bounded != 0);
     int rc = launch_trace(sc->accel, reinterpret_cast<const WRay*>(sq.ray), 0, &ctl->n_shadow, nconn,
                           reinterpret_cast<WHit*>(sc->bd_shits.p), st, true);
     if (rc != LMX_OK) return rc;
@@ -651,6 +770,12 @@ int bd_graph(lmx_scene* sc, const BParams& bp, hipStream_t st, hipGraphExec_t* e
     return rc;
 }
 
+// autotuner key: variant, max_num_vertices, bounded
+uint32_t bd_key(const BParams& bp)
+{
+    return 1u + (uint32_t)bp.maxv + 64u * (uint32_t)bp.variant + 128u * (uint32_t)bp.bounded;
+}
+
 // Batch-size autotuner: times one batch of each candidate size on this scene
 // (after one untimed batch that warms the graph and the caches) and keeps the
 // size with the highest samples/s for this max_num_vertices.  Candidates are
@@ -662,8 +787,9 @@ int bd_autotune(lmx_scene* sc, BParams bp, float* film, hipStream_t st)
     size_t free_b = 0, total_b = 0;
     hipError_t e = hipMemGetInfo(&free_b, &total_b);
     if (e != hipSuccess) return hip_error(e, "bdpt autotune: memory info");
-    const size_t budget = (free_b + (size_t)sc->bd_batch * bd_bytes_per_sample(sc->bd_maxv)) / 4;
-    const size_t per = bd_bytes_per_sample((uint32_t)bp.maxv);
+    const size_t budget = (free_b + (size_t)sc->bd_batch * bd_bytes_per_sample(sc->bd_maxv, sc->bd_conn)) / 4;
+    const uint32_t conn = bd_conn_per_sample((uint32_t)bp.maxv, bp.variant, bp.bounded != 0);
+    const size_t per = bd_bytes_per_sample((uint32_t)bp.maxv, conn);
     bp.sample_out = nullptr;
     bp.table = nullptr;                        // probes draw from the generator
     bp.film = film;
@@ -677,7 +803,7 @@ int bd_autotune(lmx_scene* sc, BParams bp, float* film, hipStream_t st)
         if (B != cands[0] && (size_t)B * per > budget) break;
         bp.batch = B;
         bp.total = 2u * B;
-        if ((rc = bd_buffers(sc, B, (uint32_t)bp.maxv)) != LMX_OK) break;
+        if ((rc = bd_buffers(sc, B, (uint32_t)bp.maxv, conn)) != LMX_OK) break;
         hipGraphExec_t exec = nullptr;
         if ((rc = bd_graph(sc, bp, st, &exec)) != LMX_OK) break;
         k_bd_init<<<1, 64, 0, st>>>(sc->bd_ctl.p);
@@ -697,30 +823,40 @@ int bd_autotune(lmx_scene* sc, BParams bp, float* film, hipStream_t st)
     (void)hipEventDestroy(e1);
     if (rc == LMX_OK) {
         sc->bd_tuned_batch = best;
-        sc->bd_tuned_maxv = (uint32_t)bp.maxv;
+        sc->bd_tuned_key = bd_key(bp);
     }
     return rc;
 }
 
-}  // namespace
-
-extern "C" {
-
-int lmx_render_bdpt(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, void* stream, lmx_render_stats* stats)
+// One render of either variant: batches of B samples, each replayed from one
+// captured graph; the film scaled by W H / N at the end.
+int render_bidir(lmx_scene* sc, const lmx_render_params* p, int variant, const char* name, float* film_rgb,
+                 void* stream, lmx_render_stats* stats)
 {
-    if (!sc || !p || !film_rgb) return set_error(LMX_EINVAL, "lmx_render_bdpt: NULL argument");
+    char msg[160];
+    if (!sc || !p || !film_rgb) return set_error(LMX_EINVAL, (std::string(name) + ": NULL argument").c_str());
     if (p->num_samples == 0 || p->num_samples > 0xffffffffull)
-        return set_error(LMX_EINVAL, "lmx_render_bdpt: num_samples must be in [1, 2^32)");
-    if (p->max_num_vertices < 2 || p->max_num_vertices > 32)
-        return set_error(LMX_EINVAL, "lmx_render_bdpt: max_num_vertices must be in [2, 32]");
+        return set_error(LMX_EINVAL, (std::string(name) + ": num_samples must be in [1, 2^32)").c_str());
+    const bool bounded = p->max_num_vertices != -1;
+    if (bounded && (p->max_num_vertices < 2 || p->max_num_vertices > 32)) {
+        snprintf(msg, sizeof msg, "%s: max_num_vertices must be in [2, 32]%s", name,
+                 variant == BD_BDPT ? " or -1" : "");
+        return set_error(LMX_EINVAL, msg);
+    }
+    if (!bounded && variant == BD_VCM)
+        return set_error(LMX_EINVAL, (std::string(name) + ": max_num_vertices must be in [2, 32]").c_str());
+    if (variant == BD_BDPT && p->mis != LMX_MIS_POWER && p->mis != LMX_MIS_SIMPLE)
+        return set_error(LMX_EINVAL, (std::string(name) + ": unknown mis").c_str());
     if (sc->mats.n > VB_MAX_MATERIALS || sc->lights.n > VB_MAX_LIGHTS)
-        return set_error(LMX_EINVAL, "lmx_render_bdpt: at most 65536 materials and 8190 lights");
-    const uint32_t maxv = (uint32_t)p->max_num_vertices;
-    const uint32_t wsub = LMX_PT_SLOT_SEG0 + 4u * (maxv - 1u);
-    if (p->uniform_table && p->table_width < 2 * wsub)
-        return set_error(LMX_EINVAL, "lmx_render_bdpt: uniform table narrower than 2 (8 + 4 (max_num_vertices - 1))");
+        return set_error(LMX_EINVAL, (std::string(name) + ": at most 65536 materials and 8190 lights").c_str());
+    const uint32_t maxv = bounded ? (uint32_t)p->max_num_vertices : 32u;
+    const uint32_t wsub = LMX_PT_SLOT_SEG0 + (variant == BD_BDPT ? 8u : 4u) * (maxv - 1u);
+    if (p->uniform_table && p->table_width < 2 * wsub) {
+        snprintf(msg, sizeof msg, "%s: uniform table narrower than %u", name, 2 * wsub);
+        return set_error(LMX_EINVAL, msg);
+    }
     if (p->raster_mode != LMX_RASTER_RANDOM && p->raster_mode != LMX_RASTER_STRATIFIED)
-        return set_error(LMX_EINVAL, "lmx_render_bdpt: unknown raster_mode");
+        return set_error(LMX_EINVAL, (std::string(name) + ": unknown raster_mode").c_str());
     hipStream_t st = (hipStream_t)stream;
     hipStream_t own = nullptr;
     hipError_t e;
@@ -730,6 +866,9 @@ int lmx_render_bdpt(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, 
     }
     const auto t0 = std::chrono::steady_clock::now();
     BParams bp;
+    bp.variant = variant;
+    bp.mis = variant == BD_BDPT ? p->mis : LMX_MIS_POWER;
+    bp.bounded = bounded ? 1 : 0;
     bp.maxv = (int)maxv; bp.minv = p->min_num_vertices;
     bp.key = make_uint2((uint32_t)p->seed, (uint32_t)(p->seed >> 32));
     bp.total = (uint32_t)p->num_samples;
@@ -739,19 +878,20 @@ int lmx_render_bdpt(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, 
     bp.sample_out = p->sample_out;
     bp.film = film_rgb;
     int rc = LMX_OK;
-    // batch size: the caller's, else the autotuned one (tuned once per scene
-    // and max_num_vertices, for renders of more than the smallest candidate)
+    // batch size: the caller's, else the autotuned one (tuned once per scene,
+    // variant and max_num_vertices, for renders of more than the smallest
+    // candidate)
     uint32_t B = p->queue_capacity;
     if (!B) {
         if (p->num_samples <= (1u << 19)) B = (uint32_t)p->num_samples;
         else {
-            if (sc->bd_tuned_maxv != maxv) rc = bd_autotune(sc, bp, film_rgb, st);
+            if (sc->bd_tuned_key != bd_key(bp)) rc = bd_autotune(sc, bp, film_rgb, st);
             B = sc->bd_tuned_batch;
         }
     }
     B = (uint32_t)std::min<uint64_t>(B, p->num_samples);
     bp.batch = B;
-    if (rc == LMX_OK) rc = bd_buffers(sc, B, maxv);
+    if (rc == LMX_OK) rc = bd_buffers(sc, B, maxv, bd_conn_per_sample(maxv, variant, bounded));
     const size_t npix = (size_t)sc->width * (size_t)sc->height;
     const int zgrid = std::max(1, std::min((int)((3 * npix + 255) / 256), sc->cus * 8));
     hipGraphExec_t exec = nullptr;
@@ -767,7 +907,7 @@ int lmx_render_bdpt(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, 
         k_bd_scale<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix, (float)((double)npix / (double)p->num_samples));
         if ((e = hipMemcpyAsync(sc->bd_hctl, sc->bd_ctl.p, sizeof(BCtl), hipMemcpyDeviceToHost, st)) != hipSuccess ||
             (e = hipStreamSynchronize(st)) != hipSuccess)
-            rc = hip_error(e, "lmx_render_bdpt: sync");
+            rc = hip_error(e, (std::string(name) + ": sync").c_str());
     }
     if (exec) (void)hipGraphExecDestroy(exec);
     if (own) (void)hipStreamDestroy(own);
@@ -778,6 +918,21 @@ int lmx_render_bdpt(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, 
         stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lmx_render_vcm_bdpt(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, void* stream,
+                        lmx_render_stats* stats)
+{
+    return render_bidir(sc, p, BD_VCM, "lmx_render_vcm_bdpt", film_rgb, stream, stats);
+}
+
+int lmx_render_bdpt(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, void* stream, lmx_render_stats* stats)
+{
+    return render_bidir(sc, p, BD_BDPT, "lmx_render_bdpt", film_rgb, stream, stats);
 }
 
 }  // extern "C"

@@ -502,9 +502,10 @@ struct lmx_scene {
     Ctl* h_ctl = nullptr;      // pinned
     uint32_t tuned_cap = 0;
     // bidirectional sampler buffers (sized for bd_batch samples, bd_maxv vertices)
-    uint32_t bd_batch = 0, bd_maxv = 0;
-    uint32_t bd_tuned_batch = 0, bd_tuned_maxv = 0;   // autotuned batch for one max_num_vertices
+    uint32_t bd_batch = 0, bd_maxv = 0, bd_conn = 0;   // bd_conn: shadow-queue slots per sample
+    uint32_t bd_tuned_batch = 0, bd_tuned_key = 0;      // autotuned batch for one (variant, max_num_vertices)
     DevBuf<float4> bd_vert[2];          // per subpath vertex: 4 float4 (see lmx_bdpt.hip)
+    DevBuf<float> bd_V;                 // per light subpath vertex (bdpt variant)
     DevBuf<uint32_t> bd_nsub[2];
     DevBuf<uint32_t> bd_epix;
     DevBuf<float2> bd_eraster;

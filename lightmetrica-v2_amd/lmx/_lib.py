@@ -62,7 +62,7 @@ class lmx_render_params(ctypes.Structure):
                 ("seed", ctypes.c_uint64), ("num_samples", ctypes.c_uint64),
                 ("raster_mode", ctypes.c_int32), ("queue_capacity", ctypes.c_uint32),
                 ("uniform_table", ctypes.c_void_p), ("table_width", ctypes.c_uint32),
-                ("sample_out", ctypes.c_void_p), ("rr_mode", ctypes.c_int32)]
+                ("sample_out", ctypes.c_void_p), ("rr_mode", ctypes.c_int32), ("mis", ctypes.c_int32)]
 
 
 class lmx_render_stats(ctypes.Structure):
@@ -102,6 +102,8 @@ SIGNATURES = {
                                      ctypes.c_void_p, ctypes.POINTER(lmx_render_stats)]),
     "lmx_render_bdpt": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(lmx_render_params), ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.POINTER(lmx_render_stats)]),
+    "lmx_render_vcm_bdpt": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(lmx_render_params), ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.POINTER(lmx_render_stats)]),
     "lmx_render_ptdirect": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(lmx_render_params), ctypes.c_void_p,
                                            ctypes.c_void_p, ctypes.POINTER(lmx_render_stats)]),
     "lmx_render_ptmis": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(lmx_render_params), ctypes.c_void_p,

@@ -20,6 +20,8 @@ RASTER_RANDOM = 0
 RASTER_STRATIFIED = 1
 RR_FIXED = 0
 RR_THROUGHPUT = 1
+MIS_POWER = 0
+MIS_SIMPLE = 1
 # bsdf::cooktorrance defaults (bsdf_cooktorrance.cpp:60-63)
 GGX_DEFAULT = dict(roughness=0.1, eta=(0.14, 0.129, 0.1585), k=(4.58625, 3.348125, 2.329375))
 SLOT_SEG0 = 8
@@ -31,9 +33,15 @@ def table_width(max_num_vertices, nee=False):
     return SLOT_SEG0 + (8 if nee else 4) * (max_num_vertices - 1)
 
 
-def bdpt_table_width(max_num_vertices):
-    """Uniform-table width of the bidirectional sampler (eye + light block)."""
+def vcm_bdpt_table_width(max_num_vertices):
+    """Uniform-table width of renderer::vcm's bdpt mode (eye + light block)."""
     return 2 * table_width(max_num_vertices)
+
+
+def bdpt_table_width(max_num_vertices):
+    """Uniform-table width of renderer::bdpt (eye + light block; -1 = 32)."""
+    m = 32 if max_num_vertices == -1 else max_num_vertices
+    return 2 * table_width(m, nee=True)
 
 
 class Scene:
@@ -102,14 +110,31 @@ class Scene:
         p.queue_capacity = queue_capacity
         return p
 
-    def render_bdpt(self, num_samples, max_num_vertices=10, min_num_vertices=0, seed=42,
-                    raster_mode=RASTER_STRATIFIED, batch=0, table=None, sample_out=False, stream=None):
+    def render_vcm_bdpt(self, num_samples, max_num_vertices=10, min_num_vertices=0, seed=42,
+                        raster_mode=RASTER_STRATIFIED, batch=0, table=None, sample_out=False, stream=None):
         """renderer::vcm in bdpt mode.  Returns (film (H, W, 3), stats,
         per-sample records (n, (m+1)^2, 8) or None)."""
+        return self._bidir("lmx_render_vcm_bdpt", (max_num_vertices + 1) ** 2, num_samples, max_num_vertices,
+                           min_num_vertices, seed, raster_mode, batch, table, sample_out, stream, MIS_POWER)
+
+    def render_bdpt(self, num_samples, max_num_vertices=-1, min_num_vertices=0, seed=42,
+                    raster_mode=RASTER_STRATIFIED, batch=0, table=None, sample_out=False, stream=None,
+                    mis=MIS_POWER):
+        """renderer::bdpt (roulette subpaths, direct light vertices, power or
+        simple weights).  Returns (film (H, W, 3), stats, per-sample records
+        (n, 2 (M+1)^2, 8) at 2 (t (M+1) + s) + direct, or None); M =
+        max_num_vertices, 32 when unbounded (-1)."""
+        m = 32 if max_num_vertices == -1 else max_num_vertices
+        return self._bidir("lmx_render_bdpt", 2 * (m + 1) ** 2, num_samples, max_num_vertices,
+                           min_num_vertices, seed, raster_mode, batch, table, sample_out, stream, mis)
+
+    def _bidir(self, entry, nrec, num_samples, max_num_vertices, min_num_vertices, seed, raster_mode, batch,
+               table, sample_out, stream, mis):
         import torch
         dev = torch.device("cuda")
         film = torch.empty((self.height, self.width, 3), dtype=torch.float32, device=dev)
         p = self._params(num_samples, max_num_vertices, min_num_vertices, 1, 0.5, seed, raster_mode, batch)
+        p.mis = mis
         so = None
         if table is not None:
             assert table.is_cuda and table.dtype == torch.float32 and table.is_contiguous()
@@ -117,14 +142,13 @@ class Scene:
             p.uniform_table = ctypes.c_void_p(table.data_ptr())
             p.table_width = table.shape[1]
         if sample_out:
-            m = max_num_vertices
-            so = torch.zeros((num_samples, (m + 1) * (m + 1), 8), dtype=torch.float32, device=dev)
+            so = torch.zeros((num_samples, nrec, 8), dtype=torch.float32, device=dev)
             p.sample_out = ctypes.c_void_p(so.data_ptr())
         st = lmx_render_stats()
         if stream is None:
             stream = torch.cuda.Stream(dev)
             stream.wait_stream(torch.cuda.current_stream(dev))
-        check(lib.lmx_render_bdpt(self._h, ctypes.byref(p), ctypes.c_void_p(film.data_ptr()),
+        check(getattr(lib, entry)(self._h, ctypes.byref(p), ctypes.c_void_p(film.data_ptr()),
                                   ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st)))
         stats = {"rays": st.rays, "batches": st.iterations, "batch": st.queue_capacity, "ms": st.ms}
         return film, stats, so

@@ -192,6 +192,22 @@ typedef struct orc_bdpt_params {
 void orc_bdpt_samples(const orc_scene *s, const orc_bdpt_params *p, const float *table, uint32_t table_width,
                       uint64_t first_sample, uint64_t n, float *out, int nthreads);
 
+/* renderer::bdpt (orc_render.c).  max_num_vertices: -1 (unbounded) or >= 2;
+ * min_num_vertices applies only when bounded (renderer_bdpt.cpp:887);
+ * mis: 0 power heuristic, 1 simple.  Output per sample: 2 (M+1)^2 records of
+ * 8 floats (M = max_num_vertices, 32 when unbounded) indexed
+ * 2 (t (M+1) + s) + direct: raster x, y, C rgb (= C* w / SelectionPDF,
+ * before the W H / N film scale), 1 if the strategy contributed, w, 0. */
+typedef struct orc_bdptref_params {
+    int32_t max_num_vertices;
+    int32_t min_num_vertices;
+    uint32_t seed_lo, seed_hi;
+    int32_t raster_mode;
+    int32_t mis;
+} orc_bdptref_params;
+void orc_bdptref_samples(const orc_scene *s, const orc_bdptref_params *p, const float *table, uint32_t table_width,
+                         uint64_t first_sample, uint64_t n, float *out, int nthreads);
+
 /* BSDF test hooks on a flat frame with normal n */
 int orc_bsdf_query(const orc_material *m, const float n[3], const float wi[3], const float u[2],
                    float wo[3], float *pdf, float f[3]);

@@ -202,6 +202,12 @@ class orc_bdpt_params(ctypes.Structure):
                 ("seed_lo", ctypes.c_uint32), ("seed_hi", ctypes.c_uint32), ("raster_mode", ctypes.c_int32)]
 
 
+class orc_bdptref_params(ctypes.Structure):
+    _fields_ = [("max_num_vertices", ctypes.c_int32), ("min_num_vertices", ctypes.c_int32),
+                ("seed_lo", ctypes.c_uint32), ("seed_hi", ctypes.c_uint32), ("raster_mode", ctypes.c_int32),
+                ("mis", ctypes.c_int32)]
+
+
 def _render_sigs(so):
     vp = ctypes.c_void_p
     so.orc_philox4x32_10.argtypes = [vp, vp, vp]
@@ -220,6 +226,9 @@ def _render_sigs(so):
     so.orc_bdpt_samples.argtypes = [vp, ctypes.POINTER(orc_bdpt_params), vp, ctypes.c_uint32, ctypes.c_uint64,
                                     ctypes.c_uint64, vp, ctypes.c_int]
     so.orc_bdpt_samples.restype = None
+    so.orc_bdptref_samples.argtypes = [vp, ctypes.POINTER(orc_bdptref_params), vp, ctypes.c_uint32, ctypes.c_uint64,
+                                       ctypes.c_uint64, vp, ctypes.c_int]
+    so.orc_bdptref_samples.restype = None
     so.orc_ptnee_samples.argtypes = [vp, ctypes.POINTER(orc_pt_params), ctypes.c_int, vp, ctypes.c_uint32,
                                      ctypes.c_uint64, ctypes.c_uint64, vp, ctypes.c_int]
     so.orc_ptnee_samples.restype = None
@@ -364,6 +373,23 @@ class Scene:
             tw = table.shape[1]
             tp = ctypes.c_void_p(table.ctypes.data)
         lib().orc_bdpt_samples(self._h, ctypes.byref(p), tp, tw, first, n, _p(out), nthreads)
+        return out
+
+    def bdptref_samples(self, first, n, max_num_vertices=-1, min_num_vertices=0, seed=42,
+                        raster_mode=RASTER_STRATIFIED, mis=0, table=None, nthreads=8):
+        """renderer::bdpt: (n, 2 (M+1)^2, 8) float32 per 2 (t (M+1) + s) + direct
+        (M = max_num_vertices, 32 when unbounded): raster x, y, C rgb
+        (C* w / SelectionPDF, before the W H / N film scale), contributed
+        flag, w, 0.  mis: 0 power heuristic, 1 simple."""
+        M = 32 if max_num_vertices == -1 else max_num_vertices
+        p = orc_bdptref_params(max_num_vertices, min_num_vertices, seed & 0xFFFFFFFF, seed >> 32, raster_mode, mis)
+        out = np.zeros((n, 2 * (M + 1) * (M + 1), 8), np.float32)
+        tw, tp = 0, None
+        if table is not None:
+            table = np.ascontiguousarray(table, np.float32)
+            tw = table.shape[1]
+            tp = ctypes.c_void_p(table.ctypes.data)
+        lib().orc_bdptref_samples(self._h, ctypes.byref(p), tp, tw, first, n, _p(out), nthreads)
         return out
 
     def film(self, samples):

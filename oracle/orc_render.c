@@ -936,6 +936,409 @@ void orc_bdpt_samples(const orc_scene *s, const orc_bdpt_params *pp, const float
 }
 
 /* ---------------------------------------------------------------------------
+ * renderer::bdpt (renderer_bdpt.cpp:50-1002), restated literally: per sample
+ * an eye and a light subpath (Subpath::Sample, :80-220) whose vertices hold a
+ * BSDF-sampled vertex `sv` and a `direct` vertex sampled on an emitter of the
+ * other kind from the previous vertex; Russian roulette with probability 0.5
+ * after every vertex but the endpoint; every strategy (s, t, direct) of every
+ * length n (:885-950) built by Path::Connect (:233-333), weighed by the power
+ * heuristic or the simple 1/#strategies weight over EvaluatePDF (:649-699,
+ * :704-784) and divided by SelectionPDF (:348-368).
+ *
+ * Deliberate differences, as for renderer::vcm: the area light's position and
+ * emission direction take independent numbers; a strategy whose own pdf is
+ * zero contributes nothing.  max_num_vertices = -1 (unbounded, the
+ * reference's default) keeps at most 64 vertices per subpath here.
+ *
+ * Uniform slots per sample: an eye block then a light block, each
+ * 8 + 8 (M - 1) wide (M = max_num_vertices, 32 when unbounded).  Within a
+ * block: 1-2 endpoint position, 3 emitter selection, 4-5 endpoint direction;
+ * step j >= 1 (vertex j) owns 8 + 8 (j - 1) .. +7: 0-1 direction sampled at
+ * vertex j - 1 (j >= 2), 3 roulette after vertex j, 4 emitter selection and
+ * 5-6 position of the direct vertex sampled from vertex j - 1. */
+
+#define RB_CAP 64
+
+typedef struct { bvert sv, direct; int has_sv, has_direct; } rsub;
+
+/* Scene3::Visible (scene3.h:107-116) */
+static int visible(const bctx *b, v3 p1, v3 p2)
+{
+    v3 d = vsub(p2, p1);
+    const float l = sqrtf(vdot(d, d));
+    d = mk(d.x / l, d.y / l, d.z / l);
+    orc_ray r;
+    r.o[0] = p1.x; r.o[1] = p1.y; r.o[2] = p1.z; r.tmin = EPS_ISECT;
+    r.d[0] = d.x; r.d[1] = d.y; r.d[2] = d.z; r.tmax = l * (1.0f - EPS_ISECT);
+    orc_hit h;
+    orc_qbvh_intersect_ex(b->s->q, &r, 1, &h, ORC_TEST_WATERTIGHT, ORC_TIE_MIN_ID, 1);
+    return h.id == 0xffffffffu;
+}
+
+/* Emitter::EvaluatePosition (light_area.cpp:112-115, sensor_pinhole.cpp:120-123) */
+static float ep_position(const bvert *v, int eval_delta)
+{
+    if (v->kind == BV_CAMERA) return eval_delta ? 0.0f : 1.0f;
+    return 1.0f;
+}
+
+/* EvaluatePositionGivenDirectionPDF = EvaluatePositionGivenPreviousPositionPDF
+ * for both emitters (light_area.cpp:92-100, sensor_pinhole.cpp:105-113) */
+static float ep_pos_pdf(const bctx *b, const bvert *v, int eval_delta)
+{
+    if (v->kind == BV_CAMERA) return eval_delta ? 0.0f : 1.0f;
+    return v->light < 0 ? 0.0f : b->s->inv_area[v->light];
+}
+
+/* Scene3::EvaluateEmitterPDF (scene3.cpp:524-540) */
+static float emitter_pdf(const bctx *b, const bvert *v)
+{
+    return v->kind == BV_CAMERA ? 1.0f : 1.0f / (float)b->s->d.nlight;
+}
+
+static int rb_subpath(const bctx *b, const orc_bdptref_params *pp, const usrc *us, uint64_t sample, uint32_t block,
+                      int light_dir, rsub *vs)
+{
+    const orc_scene *s = b->s;
+    const int maxv = pp->max_num_vertices;
+    int nv = 0;
+    v3 init_wo = mk(0, 0, 0);
+    for (int step = 0; (maxv == -1 || step < maxv) && step < RB_CAP; step++) {
+        if (step == 0) {
+            /* an emitter, a point on it and the initial direction (:89-107) */
+            rsub e;
+            memset(&e, 0, sizeof e);
+            bvert *v = &e.sv;
+            const float uP[2] = {U(us, sample, block + 1), U(us, sample, block + 2)};
+            const float uC = U(us, sample, block + 3);
+            const float uD[2] = {U(us, sample, block + 4), U(us, sample, block + 5)};
+            if (light_dir) {
+                if (s->d.nlight == 0) return 0;
+                int l = (int)(uC * (float)s->d.nlight);
+                l = l < 0 ? 0 : (l > (int)s->d.nlight - 1 ? (int)s->d.nlight - 1 : l);
+                sample_light_point(b, l, uP, v);
+                float lw[3];
+                orc_sample_cosine_hemisphere(uD, lw);
+                init_wo = to_world(&v->fr, mk(lw[0], lw[1], lw[2]));
+            } else {
+                v->kind = BV_CAMERA; v->degenerate = 1; v->light = -1;
+                v->fr.p = b->cam.pos;
+                if (pp->raster_mode == ORC_RASTER_STRATIFIED) {
+                    const uint64_t npix = (uint64_t)s->d.width * (uint64_t)s->d.height;
+                    const uint32_t pix = (uint32_t)(sample % npix);
+                    const uint32_t px = pix % (uint32_t)s->d.width, py = pix / (uint32_t)s->d.width;
+                    init_wo = camera_dir(&b->cam, ((float)px + uD[0]) / (float)s->d.width,
+                                         ((float)py + uD[1]) / (float)s->d.height);
+                } else {
+                    init_wo = camera_dir(&b->cam, uD[0], uD[1]);
+                }
+            }
+            e.has_sv = 1;
+            vs[nv++] = e;
+            continue;
+        }
+        const uint32_t sb = block + ORC_PT_SEG0 + 8u * (uint32_t)(step - 1);
+        const bvert *pv = &vs[nv - 1].sv;
+        const bvert *ppv = nv > 1 ? &vs[nv - 2].sv : NULL;
+        rsub nx;
+        memset(&nx, 0, sizeof nx);
+        /* the vertex sampled with the BSDF (:117-160) */
+        {
+            const v3 wi = ppv ? dir_to(pv, ppv) : mk(0, 0, 0);
+            v3 wo = init_wo;
+            int ok = 1;
+            if (step > 1) {
+                const float uD[2] = {U(us, sample, sb), U(us, sample, sb + 1)};
+                ok = pv->kind == BV_SURFACE && bsdf_sample(pv->mat, &pv->fr, wi, uD, &wo);
+            }
+            if (ok) {
+                float fs[3];
+                v_eval(b, pv, wi, wo, light_dir, fs);
+                orc_hit h;
+                if (!(fs[0] == 0.0f && fs[1] == 0.0f && fs[2] == 0.0f) && scene_intersect(s, pv->fr.p, wo, &h)) {
+                    bvert *v = &nx.sv;
+                    surface_at(s, pv->fr.p, wo, &h, &v->fr);
+                    v->kind = BV_SURFACE; v->degenerate = 0; v->tri = h.id;
+                    v->mat = &s->d.mats[s->d.tri_mat[h.id]];
+                    v->light = b->tri_light[h.id];
+                    nx.has_sv = 1;
+                }
+            }
+        }
+        /* the direct vertex: an emitter of the other kind, sampled from the
+         * previous vertex, if visible from it (:166-190) */
+        {
+            bvert *v = &nx.direct;
+            int have = 0;
+            if (light_dir) {
+                v->kind = BV_CAMERA; v->degenerate = 1; v->light = -1;
+                v->fr.p = b->cam.pos;                    /* sensor_pinhole.cpp:73-77 */
+                have = 1;
+            } else if (s->d.nlight > 0) {
+                int l = (int)(U(us, sample, sb + 4) * (float)s->d.nlight);
+                l = l < 0 ? 0 : (l > (int)s->d.nlight - 1 ? (int)s->d.nlight - 1 : l);
+                const float uP[2] = {U(us, sample, sb + 5), U(us, sample, sb + 6)};
+                sample_light_point(b, l, uP, v);
+                have = 1;
+            }
+            nx.has_direct = have && visible(b, pv->fr.p, v->fr.p);
+        }
+        if (nx.has_sv || nx.has_direct) vs[nv++] = nx;
+        if (!nx.has_sv) break;
+        if (U(us, sample, sb + 3) > 0.5f) break;           /* :205-211 */
+    }
+    return nv;
+}
+
+/* Path::Connect (:233-333); returns the number of vertices, 0 if the
+ * strategy does not apply to these subpaths */
+static int rb_connect(const bctx *b, const rsub *L, const rsub *E, int s, int t, int direct, bvert *x)
+{
+    int n = 0;
+    if (s == 0 && t > 0) {
+        if (!direct) {
+            if (!E[t - 1].has_sv || E[t - 1].sv.light < 0) return 0;
+            for (int i = t - 1; i >= 0; i--) x[n++] = E[i].sv;
+        } else {
+            if (!E[t - 1].has_direct) return 0;
+            x[n++] = E[t - 1].direct;
+            for (int i = t - 2; i >= 0; i--) x[n++] = E[i].sv;
+        }
+        x[0].kind = BV_LIGHT;
+    } else if (s > 0 && t == 0) {
+        if (!direct) {
+            /* the light subpath would have to hit the sensor: a pinhole is
+             * not part of the scene's surfaces */
+            return 0;
+        }
+        if (!L[s - 1].has_direct) return 0;
+        for (int i = 0; i < s - 1; i++) x[n++] = L[i].sv;
+        x[n++] = L[s - 1].direct;
+        x[n - 1].kind = BV_CAMERA;
+    } else {
+        if (!L[s - 1].has_sv || !E[t - 1].has_sv) return 0;
+        if (!visible(b, L[s - 1].sv.fr.p, E[t - 1].sv.fr.p)) return 0;
+        for (int i = 0; i < s; i++) x[n++] = L[i].sv;
+        for (int i = t - 1; i >= 0; i--) x[n++] = E[i].sv;
+    }
+    return n;
+}
+
+/* Path::EvaluateCst (:378-412) */
+static void rb_cst(const bctx *b, const bvert *x, int n, int s, float out[3])
+{
+    const int t = n - s;
+    out[0] = out[1] = out[2] = 0.0f;
+    float e[3];
+    if (s == 0 && t > 0) {
+        const float P = ep_position(&x[0], 1);
+        v_eval(b, &x[0], mk(0, 0, 0), dir_to(&x[0], &x[1]), 0, e);
+        for (int c = 0; c < 3; c++) out[c] = P * e[c];
+    } else if (s > 0 && t == 0) {
+        const float P = ep_position(&x[n - 1], 1);
+        v_eval(b, &x[n - 1], mk(0, 0, 0), dir_to(&x[n - 1], &x[n - 2]), 1, e);
+        for (int c = 0; c < 3; c++) out[c] = P * e[c];
+    } else {
+        const bvert *vL = &x[s - 1], *vE = &x[s];
+        float fsL[3], fsE[3];
+        v_eval(b, vL, s >= 2 ? dir_to(vL, &x[s - 2]) : mk(0, 0, 0), dir_to(vL, vE), 1, fsL);
+        v_eval(b, vE, s + 1 < n ? dir_to(vE, &x[s + 1]) : mk(0, 0, 0), dir_to(vE, vL), 0, fsE);
+        const float G = gterm(vL, vE);
+        for (int c = 0; c < 3; c++) out[c] = fsL[c] * G * fsE[c];
+    }
+}
+
+static int black(const float v[3]) { return v[0] == 0.0f && v[1] == 0.0f && v[2] == 0.0f; }
+
+/* Path::EvaluateUnweightContribution (:489-597): alpha_L c_st alpha_E */
+static void rb_unweighted(const bctx *b, const bvert *x, int n, int s, int direct, float out[3])
+{
+    const int t = n - s;
+    float aL[3] = {1, 1, 1}, aE[3] = {1, 1, 1}, fs[3];
+    out[0] = out[1] = out[2] = 0.0f;
+    if (s > 0) {
+        const float a = ep_position(&x[0], 0) / ep_pos_pdf(b, &x[0], 0) / emitter_pdf(b, &x[0]);
+        aL[0] = aL[1] = aL[2] = a;
+        for (int i = 0; i < s - 1; i++) {
+            const v3 wi = i >= 1 ? dir_to(&x[i], &x[i - 1]) : mk(0, 0, 0);
+            const v3 wo = dir_to(&x[i], &x[i + 1]);
+            v_eval(b, &x[i], wi, wo, 1, fs);
+            if (black(fs)) return;
+            const float d = (t == 0 && i == s - 2 && direct)
+                ? ep_pos_pdf(b, &x[i + 1], 0) / gterm(&x[i + 1], &x[i]) * emitter_pdf(b, &x[i + 1])
+                : v_pdf(b, &x[i], wi, wo);
+            for (int c = 0; c < 3; c++) aL[c] *= fs[c] / d;
+        }
+    }
+    if (black(aL)) return;
+    if (t > 0) {
+        const float a = ep_position(&x[n - 1], 0) / ep_pos_pdf(b, &x[n - 1], 0) / emitter_pdf(b, &x[n - 1]);
+        aE[0] = aE[1] = aE[2] = a;
+        for (int i = n - 1; i > s; i--) {
+            const v3 wi = i < n - 1 ? dir_to(&x[i], &x[i + 1]) : mk(0, 0, 0);
+            const v3 wo = dir_to(&x[i], &x[i - 1]);
+            v_eval(b, &x[i], wi, wo, 0, fs);
+            if (black(fs)) return;
+            const float d = (s == 0 && i == 1 && direct)
+                ? ep_pos_pdf(b, &x[i - 1], 0) / gterm(&x[i - 1], &x[i]) * emitter_pdf(b, &x[i - 1])
+                : v_pdf(b, &x[i], wi, wo);
+            for (int c = 0; c < 3; c++) aE[c] *= fs[c] / d;
+        }
+    }
+    if (black(aE)) return;
+    float cst[3];
+    rb_cst(b, x, n, s, cst);
+    if (black(cst)) return;
+    for (int c = 0; c < 3; c++) out[c] = aL[c] * cst[c] * aE[c];
+}
+
+/* Path::Samplable + Path::EvaluatePDF (:599-699), product area measure; in
+ * double, so long paths neither overflow nor underflow.  No BSDF here is a
+ * delta distribution; the pinhole's position is one. */
+static double rb_pdf(const bctx *b, const bvert *x, int n, int s, int direct)
+{
+    const int t = n - s;
+    if (s > 0 && t > 0 && direct) return 0.0;
+    if (s > 0 && t == 0 && x[n - 1].kind == BV_CAMERA) return 0.0;
+    if (s == 0 && t > 0 && x[0].kind == BV_CAMERA) return 0.0;
+    double pdf = 1.0;
+    if (s > 0) {
+        pdf *= (double)ep_pos_pdf(b, &x[0], 0) * (double)emitter_pdf(b, &x[0]);
+        for (int i = 0; i < s - 1; i++) {
+            if (t == 0 && i == s - 2 && direct) {
+                pdf *= (double)ep_pos_pdf(b, &x[i + 1], 0) * (double)emitter_pdf(b, &x[i + 1]);
+            } else {
+                const v3 wi = i >= 1 ? dir_to(&x[i], &x[i - 1]) : mk(0, 0, 0);
+                pdf *= (double)v_pdf(b, &x[i], wi, dir_to(&x[i], &x[i + 1])) * (double)gterm(&x[i], &x[i + 1]);
+            }
+        }
+    }
+    if (t > 0) {
+        pdf *= (double)ep_pos_pdf(b, &x[n - 1], 0) * (double)emitter_pdf(b, &x[n - 1]);
+        for (int i = n - 1; i >= s + 1; i--) {
+            if (s == 0 && i == 1 && direct) {
+                pdf *= (double)ep_pos_pdf(b, &x[0], 0) * (double)emitter_pdf(b, &x[0]);
+            } else {
+                const v3 wi = i + 1 < n ? dir_to(&x[i], &x[i + 1]) : mk(0, 0, 0);
+                pdf *= (double)v_pdf(b, &x[i], wi, dir_to(&x[i], &x[i - 1])) * (double)gterm(&x[i], &x[i - 1]);
+            }
+        }
+    }
+    return pdf;
+}
+
+/* MISWeight_PowerHeuristics / MISWeight_Simple (:704-784) */
+static double rb_mis(const bctx *b, const bvert *x, int n, int s0, int d0, int simple)
+{
+    const double ps = rb_pdf(b, x, n, s0, d0);
+    double inv = 0.0;
+    int nonzero = 0;
+    for (int s = 0; s <= n; s++)
+        for (int d = 0; d < 2; d++) {
+            if (s > 0 && n - s > 0 && d) continue;
+            const double pi = rb_pdf(b, x, n, s, d);
+            if (pi > 0.0) { const double r = pi / ps; inv += r * r; nonzero++; }
+        }
+    return simple ? 1.0 / (double)nonzero : 1.0 / inv;
+}
+
+static void bdptref_sample(const bctx *b, const orc_bdptref_params *pp, const usrc *us, uint64_t sample, float *out)
+{
+    const int M = pp->max_num_vertices == -1 ? 32 : pp->max_num_vertices;
+    const int stride = 2 * (M + 1) * (M + 1);
+    memset(out, 0, sizeof(float) * 8 * (size_t)stride);
+    rsub L[RB_CAP], E[RB_CAP];
+    bvert x[2 * RB_CAP];
+    const uint32_t wsub = ORC_PT_SEG0 + 8u * (uint32_t)(M - 1);
+    const int nL = rb_subpath(b, pp, us, sample, wsub, 1, L);      /* light first (:866-867) */
+    const int nE = rb_subpath(b, pp, us, sample, 0, 0, E);
+    const int npix = b->s->d.width * b->s->d.height;
+    for (int n = 2; n <= nE + nL; n++) {
+        if (pp->max_num_vertices != -1 && (n > pp->max_num_vertices || n < pp->min_num_vertices)) continue;
+        const int minS = n - nE > 0 ? n - nE : 0;
+        const int maxS = nL < n ? nL : n;
+        for (int s = minS; s <= maxS; s++)
+            for (int d = 0; d < 2; d++) {
+                const int t = n - s;
+                if (s > 0 && t > 0 && d) continue;
+                if (s > M || t > M) continue;                    /* beyond the record layout */
+                const int nv = rb_connect(b, L, E, s, t, d, x);
+                if (nv == 0) continue;
+                float cs[3];
+                rb_unweighted(b, x, nv, s, d, cs);
+                if (black(cs)) continue;
+                if (rb_pdf(b, x, nv, s, d) == 0.0) continue;
+                const double w = rb_mis(b, x, nv, s, d, pp->mis);
+                /* Path::SelectionPDF (:348-368) */
+                float sel = 1.0f;
+                for (int i = 1; i < s - 1; i++) sel *= 0.5f;
+                for (int i = 1; i < t - 1; i++) sel *= 0.5f;
+                float C[3];
+                for (int c = 0; c < 3; c++) C[c] = (float)((double)cs[c] * w) / sel;
+                if (black(C)) continue;
+                /* Path::RasterPosition (:370-376); the eye subpath's own first
+                 * edge gives the sample's pixel in stratified mode */
+                float rx = 0.0f, ry = 0.0f;
+                const int eye_edge = t >= 2 && !(s == 0 && d == 1 && t == 2);
+                if (eye_edge && pp->raster_mode == ORC_RASTER_STRATIFIED) {
+                    const uint32_t pix = (uint32_t)(sample % (uint64_t)npix);
+                    rx = ((float)(pix % (uint32_t)b->s->d.width) + 0.5f) / (float)b->s->d.width;
+                    ry = ((float)(pix / (uint32_t)b->s->d.width) + 0.5f) / (float)b->s->d.height;
+                } else if (!camera_raster(&b->cam, dir_to(&x[nv - 1], &x[nv - 2]), &rx, &ry)) {
+                    continue;
+                }
+                float *o = &out[8 * (size_t)(2 * (t * (M + 1) + s) + d)];
+                o[0] = rx; o[1] = ry;
+                for (int c = 0; c < 3; c++) o[2 + c] = C[c];
+                o[5] = 1.0f; o[6] = (float)w;
+            }
+    }
+}
+
+typedef struct { bctx *b; const orc_bdptref_params *pp; usrc us; float *out; int stride; } rb_job_ctx;
+typedef struct { rb_job_ctx *c; uint64_t b, e; } rb_job;
+
+static void *rb_run(void *p)
+{
+    rb_job *j = (rb_job *)p;
+    for (uint64_t i = j->b; i < j->e; i++)
+        bdptref_sample(j->c->b, j->c->pp, &j->c->us, j->c->us.first + i, &j->c->out[(size_t)8 * j->c->stride * i]);
+    return NULL;
+}
+
+void orc_bdptref_samples(const orc_scene *s, const orc_bdptref_params *pp, const float *table, uint32_t table_width,
+                         uint64_t first_sample, uint64_t n, float *out, int nthreads)
+{
+    bctx b;
+    b.s = s;
+    camera_init(&s->d, &b.cam);
+    b.tri_light = (int *)malloc(sizeof(int) * (s->d.ntri ? s->d.ntri : 1));
+    for (uint32_t i = 0; i < s->d.ntri; i++) b.tri_light[i] = -1;
+    for (uint32_t l = 0; l < s->d.nlight; l++)
+        for (uint32_t k = 0; k < s->d.light_prims[2 * l + 1]; k++) b.tri_light[s->d.light_prims[2 * l] + k] = (int)l;
+    b.tri_cdf_base = NULL;
+    const int M = pp->max_num_vertices == -1 ? 32 : pp->max_num_vertices;
+    rb_job_ctx c;
+    c.b = &b; c.pp = pp; c.us.table = table; c.us.width = table_width;
+    c.us.seed = ((uint64_t)pp->seed_hi << 32) | pp->seed_lo;
+    c.us.first = first_sample;
+    c.out = out;
+    c.stride = 2 * (M + 1) * (M + 1);
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256];
+    rb_job jobs[256];
+    for (int t = 0; t < nthreads; t++) {
+        jobs[t].c = &c;
+        jobs[t].b = n * (uint64_t)t / (uint64_t)nthreads;
+        jobs[t].e = n * (uint64_t)(t + 1) / (uint64_t)nthreads;
+        pthread_create(&th[t], NULL, rb_run, &jobs[t]);
+    }
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    free(b.tri_light);
+}
+
+/* ---------------------------------------------------------------------------
  * renderer::ptdirect (mode 1, renderer_ptdirect.cpp:72-297) and
  * renderer::ptmis (mode 2, renderer_ptmis.cpp:69-326): renderer::pt with
  * next-event estimation at every vertex, the camera's included.  Uniform

@@ -60,10 +60,10 @@ def test_bdpt_per_strategy_parity_uniform_table(gpu, raster_mode, phong):
     import torch
     sc = scenes.cornell_box(32, 32, phong_boxes=phong)
     n, m = 32 * 32 * 8, 8
-    tab = np.random.default_rng(11).random((n, render.bdpt_table_width(m)), dtype=np.float32)
+    tab = np.random.default_rng(11).random((n, render.vcm_bdpt_table_width(m)), dtype=np.float32)
     rs = render.Scene(sc)
-    film, stats, so = rs.render_bdpt(n, max_num_vertices=m, raster_mode=raster_mode,
-                                     table=torch.from_numpy(tab).to(gpu), sample_out=True, batch=4096)
+    film, stats, so = rs.render_vcm_bdpt(n, max_num_vertices=m, raster_mode=raster_mode,
+                                         table=torch.from_numpy(tab).to(gpu), sample_out=True, batch=4096)
     g = so.cpu().numpy()
     c = orc.Scene(sc).bdpt_samples(0, n, max_num_vertices=m, raster_mode=raster_mode, table=tab)
     if phong is None:
@@ -78,7 +78,7 @@ def test_bdpt_per_strategy_parity_uniform_table(gpu, raster_mode, phong):
 def test_bdpt_per_strategy_parity_philox(gpu):
     sc = scenes.sphere_grid(48, 48, grid=3, subdiv=3, seed=3)
     n, m = 48 * 48 * 4, 10
-    _, _, so = render.Scene(sc).render_bdpt(n, max_num_vertices=m, seed=7, sample_out=True, batch=3000)
+    _, _, so = render.Scene(sc).render_vcm_bdpt(n, max_num_vertices=m, seed=7, sample_out=True, batch=3000)
     c = orc.Scene(sc).bdpt_samples(0, n, max_num_vertices=m, seed=7)
     _check(so.cpu().numpy(), c, tight=0.999, loose=0.9999, min_both=1000)
 
@@ -88,7 +88,7 @@ def test_bdpt_furnace_closed_form(gpu, maxv):
     rho, Le = 0.5, 1.0
     sc = scenes.furnace_sphere(albedo=rho, Le=Le, subdiv=2, width=16, height=16)
     n = 16 * 16 * 1024
-    _, _, so = render.Scene(sc).render_bdpt(n, max_num_vertices=maxv, seed=3, sample_out=True)
+    _, _, so = render.Scene(sc).render_vcm_bdpt(n, max_num_vertices=maxv, seed=3, sample_out=True)
     rec = so.cpu().numpy().astype(np.float64)
     m = maxv
     for k in range(2, maxv + 1):               # path length in vertices
@@ -111,7 +111,7 @@ def test_bdpt_and_pt_images_agree_within_3_sigma(gpu):
         imgs = [fn(seed).cpu().numpy().astype(np.float64) for seed in range(k)]
         a = np.stack(imgs)
         return a.mean(axis=0), a.std(axis=0, ddof=1) / np.sqrt(k)
-    mb, sb = batches(lambda sd: rs.render_bdpt(32 * 32 * 256, max_num_vertices=9, seed=100 + sd)[0], 16)
+    mb, sb = batches(lambda sd: rs.render_vcm_bdpt(32 * 32 * 256, max_num_vertices=9, seed=100 + sd)[0], 16)
     mp, sp = batches(lambda sd: rs.render_pt(32 * 32 * 4096, max_num_vertices=9, seed=200 + sd)[0], 16)
     z = np.abs(mb - mp) / np.sqrt(sb ** 2 + sp ** 2 + 1e-12)
     lit = (mb + mp) > 1e-3
@@ -125,8 +125,8 @@ def test_bdpt_per_strategy_parity_ggx(gpu):
     import torch
     sc = scenes.cornell_box(32, 32, ggx_boxes=0.3)
     n, m = 32 * 32 * 8, 8
-    tab = np.random.default_rng(12).random((n, render.bdpt_table_width(m)), dtype=np.float32)
-    _, _, so = render.Scene(sc).render_bdpt(n, max_num_vertices=m, table=torch.from_numpy(tab).to(gpu),
-                                            sample_out=True, batch=4096)
+    tab = np.random.default_rng(12).random((n, render.vcm_bdpt_table_width(m)), dtype=np.float32)
+    _, _, so = render.Scene(sc).render_vcm_bdpt(n, max_num_vertices=m, table=torch.from_numpy(tab).to(gpu),
+                                                sample_out=True, batch=4096)
     c = orc.Scene(sc).bdpt_samples(0, n, max_num_vertices=m, table=tab)
     _check(so.cpu().numpy(), c, tight=0.99, loose=0.999)
