@@ -135,7 +135,8 @@ def bdpt_section(dev, cpu, threads):
     triangles, 1024x1024, 16 spp, subpaths of up to 10 vertices, every (s, t)
     strategy with power-heuristic MIS): samples/s and Mrays/s (subpath
     extension rays plus shadow rays), beside the oracle's per-sample
-    restatement on host threads over a bounded prefix."""
+    restatement on host threads over a bounded prefix; renderer::bdpt on the
+    same frame (at most 10 vertices, and unbounded) beside it."""
     import torch
     from lmx import render, scenes
     log("bdpt: generating the sphere-grid scene")
@@ -166,6 +167,22 @@ def bdpt_section(dev, cpu, threads):
         "image_mean": round(float(film.mean()), 5),
         "cpu_baseline": None,
     }
+    # renderer::bdpt on the same frame: roulette subpaths with direct light
+    # vertices, power heuristic; bounded at 10 vertices and unbounded
+    for key, m in (("renderer_bdpt", 10), ("renderer_bdpt_unbounded", -1)):
+        kb = dict(max_num_vertices=m, seed=42)
+        rs.render_bdpt(1 << 22, **kb)                    # autotune + warm-up
+        torch.cuda.synchronize()
+        tb = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            fb, sb, _ = rs.render_bdpt(n, **kb)
+            torch.cuda.synchronize()
+            tb.append(time.perf_counter() - t0)
+        db = min(tb)
+        out[key] = {"max_num_vertices": m, "msamples_per_s": round(n / db / 1e6, 2),
+                    "mrays_per_s": round(sb["rays"] / db / 1e6, 1), "rays_per_sample": round(sb["rays"] / n, 4),
+                    "s_per_frame": round(db, 4), "batch": sb["batch"], "image_mean": round(float(fb.mean()), 5)}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import orc
@@ -186,6 +203,16 @@ def bdpt_section(dev, cpu, threads):
             "sample": f"first {done} samples of the same frame (about 10 s); oracle QBVH build "
                       f"{build:.1f}s not timed"}
         out["gpu_over_cpu_samples"] = round((n / dt) / (done / cdt), 1)
+        log("bdpt: cpu baseline (oracle renderer::bdpt)")
+        osc.bdptref_samples(0, 1024, max_num_vertices=10, nthreads=threads)
+        done, t0 = 0, time.perf_counter()
+        while done < (1 << 18) and time.perf_counter() - t0 < 5.0:
+            osc.bdptref_samples(done, chunk, max_num_vertices=10, nthreads=threads)
+            done += chunk
+        rdt = time.perf_counter() - t0
+        out["renderer_bdpt"]["cpu_baseline"] = {
+            "msamples_per_s": round(done / rdt / 1e6, 5), "cores": threads, "kind": "port",
+            "sample": f"first {done} samples of the same frame (about 5 s)"}
     return out
 
 

@@ -63,7 +63,7 @@ struct lmx_accel {
     int device = -1;
     WNode* d_nodes = nullptr;
     WTri* d_tris = nullptr;
-    uint32_t* d_counters = nullptr;   // 8 region cursors + the ray count
+    uint32_t* d_counters = nullptr;   // 8 region cursors + the ray count, CURSOR_STRIDE words apart
     int grid = 0;
     int stack_depth = 0;              // LDS stack entries per lane
     size_t lds_bytes = 0;
@@ -146,28 +146,34 @@ void wbvh_walk(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* nv, uint
 
 #define TRACE_WAVE 64
 #ifndef LMX_REFILL_MIN
-#define LMX_REFILL_MIN 48      // refill when at least this many lanes of a wave are idle
+#define LMX_REFILL_MIN 16      // refill when at least this many lanes of a wave are idle
 #endif
 #ifndef LMX_TRACE_WAVES_PER_EU
 #define LMX_TRACE_WAVES_PER_EU 1   // occupancy hint for the register allocator
 #endif
 #ifndef LMX_CLAIM_CHUNK
-#define LMX_CLAIM_CHUNK 128    // rays claimed from a region cursor per atomic
+#define LMX_CLAIM_CHUNK 64     // rays claimed from a region cursor per atomic
 #endif
 #ifndef LMX_TRI_BATCH
 #define LMX_TRI_BATCH 1        // triangle loads kept in flight together (1 or 2)
 #endif
 #define REFILL_MIN LMX_REFILL_MIN
+// words between two region cursors: each cursor sits in a 256-byte line of
+// its own, so the atomics of different regions do not serialise on one line
+#ifndef LMX_CURSOR_STRIDE
+#define LMX_CURSOR_STRIDE 64
+#endif
+#define CURSOR_STRIDE LMX_CURSOR_STRIDE
 
 // Resets the 8 region cursors.  The ray count comes from n_dev when given
 // (a queue length written by an earlier kernel), else from n; the count the
-// trace kernel uses is stored in cursors[8].
+// trace kernel uses is stored after the cursors.
 __global__ void k_init_cursors(uint32_t* cursors, uint32_t n, const uint32_t* n_dev)
 {
     const uint32_t j = threadIdx.x;
     const uint32_t nn = n_dev ? *n_dev : n;
-    if (j < 8) cursors[j] = (uint32_t)(((uint64_t)nn * j) / 8u);
-    if (j == 8) cursors[8] = nn;
+    if (j < 8) cursors[j * CURSOR_STRIDE] = (uint32_t)(((uint64_t)nn * j) / 8u);
+    if (j == 8) cursors[8 * CURSOR_STRIDE] = nn;
 }
 
 __device__ __forceinline__ uint32_t region_end(uint32_t j, uint32_t n)
@@ -220,7 +226,8 @@ void k_trace(const WNode* __restrict__ nodes,
 {
     extern __shared__ uint2 stack[];     // [stack_depth][TRACE_WAVE]
     const uint32_t lane = threadIdx.x;
-    const uint32_t nray = cursors[8];
+    const uint32_t nray = cursors[8 * CURSOR_STRIDE];
+    if (nray == 0) return;                  // uniform over the grid
     // claim size: LMX_CLAIM_CHUNK for full queues; short queues (the tail of
     // a wavefront render) are spread over every wave of the grid, so that
     // their time is that of a few rays per wave, not of whole chunks in turn
@@ -255,7 +262,7 @@ void k_trace(const WNode* __restrict__ nodes,
                 if (lane == 0) {
                     while (regions_left != 0) {
                         const uint32_t end = region_end(region, nray);
-                        const uint32_t old = atomicAdd(&cursors[region], claim);
+                        const uint32_t old = atomicAdd(&cursors[region * CURSOR_STRIDE], claim);
                         if (old < end) {
                             cb = old;
                             cg = min(claim, end - old);
@@ -442,7 +449,7 @@ int lmx_accel_upload(lmx_accel* a)
         (void)hipFree(a->d_nodes); a->d_nodes = nullptr;
         return hip_error(e, "lmx_accel_upload: hipMalloc triangles");
     }
-    if ((e = hipMalloc(&a->d_counters, 16 * sizeof(uint32_t))) != hipSuccess) {
+    if ((e = hipMalloc(&a->d_counters, (9 * CURSOR_STRIDE) * sizeof(uint32_t))) != hipSuccess) {
         (void)hipFree(a->d_nodes); (void)hipFree(a->d_tris); a->d_nodes = nullptr; a->d_tris = nullptr;
         return hip_error(e, "lmx_accel_upload: hipMalloc counters");
     }

@@ -685,14 +685,19 @@ size_t bd_bytes_per_sample(uint32_t maxv, uint32_t conn)
 }
 
 // Buffers for batches of B samples of up to maxv vertices and conn
-// connections each; kept when the current ones are large enough.
+// connections each; kept when the current ones are large enough, else
+// replaced by buffers of exactly this shape (the old ones are freed first).
 int bd_buffers(lmx_scene* sc, uint32_t B, uint32_t maxv, uint32_t conn)
 {
     if (B <= sc->bd_batch && maxv <= sc->bd_maxv && conn <= sc->bd_conn) return LMX_OK;
-    B = std::max(B, sc->bd_batch);
-    maxv = std::max(maxv, sc->bd_maxv);
-    conn = std::max(conn, sc->bd_conn);
     sc->bd_batch = sc->bd_maxv = sc->bd_conn = 0;
+    for (int k = 0; k < 2; k++) {
+        sc->bd_vert[k].release(); sc->bd_nsub[k].release(); sc->bd_ray[k].release();
+        sc->bd_thr[k].release(); sc->bd_meta[k].release();
+    }
+    sc->bd_V.release(); sc->bd_epix.release(); sc->bd_eraster.release(); sc->bd_hits.release();
+    sc->bd_sray.release(); sc->bd_sc.release(); sc->bd_saux.release(); sc->bd_shits.release();
+    sc->bd_smeta.release();
     hipError_t e;
     const size_t nv = (size_t)B * maxv * 4;
     const size_t nconn = (size_t)B * conn;
