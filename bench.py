@@ -216,6 +216,41 @@ def bdpt_section(dev, cpu, threads):
     return out
 
 
+def image_section(dev, cpu):
+    """lmx_film_save of a 1024x1024 film in each format (GPU encode, one
+    device-to-host copy, host run-length / container write to a temporary
+    directory): wall ms per save, median of 5; beside it the oracle's
+    per-pixel RGBE and PNG transforms on one host thread."""
+    import tempfile
+    import numpy as np
+    import torch
+    from lmx import image
+    rng = np.random.default_rng(1)
+    f = (rng.random((1024, 1024, 3)) * np.exp(rng.normal(0, 2, (1024, 1024, 1)))).astype(np.float32)
+    t = torch.from_numpy(f).to(dev)
+    out = {"workload": "film output: 1024x1024 RGB float film -> .hdr (RLE RGBE), .exr (float), .png, .pfm"}
+    with tempfile.TemporaryDirectory() as d:
+        for name, ty in (("radiancehdr", image.RADIANCEHDR), ("openexr", image.OPENEXR), ("png", image.PNG),
+                         ("pfm", image.PFM)):
+            image.save(t, os.path.join(d, "w"), ty)
+            ts = []
+            for k in range(5):
+                t0 = time.perf_counter()
+                fn = image.save(t, os.path.join(d, f"f{k}"), ty)
+                ts.append(time.perf_counter() - t0)
+            out[name] = {"ms": round(1e3 * sorted(ts)[2], 2), "bytes": os.path.getsize(fn)}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import orc
+        t0 = time.perf_counter()
+        orc.film_rgbe(f)
+        orc.film_png(f)
+        out["cpu_baseline"] = {"ms_rgbe_plus_png_transforms": round(1e3 * (time.perf_counter() - t0), 2),
+                               "cores": 1, "kind": "port",
+                               "sample": "the oracle's per-pixel RGBE and PNG transforms of the same film"}
+    return out
+
+
 MLP_FLOP_FWD = 2 * (168 * 64 + 3 * 64 * 64 + 64 * 41)     # 51,328 algorithmic FLOP per query
 MFMA_BF16_PEAK = 2.5e15
 
@@ -348,9 +383,10 @@ def main():
     ap.add_argument("--no-pt", action="store_true", help="skip the wavefront path-tracer section")
     ap.add_argument("--no-bdpt", action="store_true", help="skip the bidirectional section")
     ap.add_argument("--no-guide", action="store_true", help="skip the path-guiding section")
+    ap.add_argument("--no-image", action="store_true", help="skip the film-output section")
     ap.add_argument("--guide-ref-spp", type=int, default=16384,
                     help="spp of the plain reference image of the path-guiding section")
-    ap.add_argument("--only", choices=["pt", "bdpt", "guide"], default=None,
+    ap.add_argument("--only", choices=["pt", "bdpt", "guide", "image"], default=None,
                     help="run one renderer section alone and print its JSON (for profiling runs)")
     ap.add_argument("--no-per-set", action="store_true",
                     help="skip the separate coherent/incoherent timings (for profiling runs)")
@@ -383,6 +419,8 @@ def main():
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         if args.only == "guide":
             out = guide_section(dev, args.guide_ref_spp)
+        elif args.only == "image":
+            out = image_section(dev, not args.no_cpu_baseline)
         else:
             fn = pt_section if args.only == "pt" else bdpt_section
             out = fn(dev, not args.no_cpu_baseline, threads)
@@ -510,6 +548,8 @@ def main():
         out["bdpt"] = bdpt_section(dev, not args.no_cpu_baseline, args.cpu_threads or min(os.cpu_count() or 1, 16))
     if not args.no_guide:
         out["guide"] = guide_section(dev, args.guide_ref_spp)
+    if not args.no_image:
+        out["image"] = image_section(dev, not args.no_cpu_baseline)
     print(json.dumps(out), flush=True)
 
 

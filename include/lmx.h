@@ -24,6 +24,7 @@ extern "C" {
 #define LMX_ENOMEM (-2)     /* host or device allocation failed */
 #define LMX_EHIP (-3)       /* HIP runtime error (no device, launch failure) */
 #define LMX_EBUILD (-4)     /* BVH construction failed */
+#define LMX_EIO (-5)        /* file could not be read or written */
 
 /* Triangle test used by traversal. */
 #define LMX_ISECT_TRIACCEL 0     /* Wald projection test, reference semantics */
@@ -311,6 +312,29 @@ int lmx_guide_grad(lmx_guide *guide, const float *records, uint32_t n, float *gr
 /* test hook: from outputs y (n x 41) and uniforms u (n x 3: lobe, u1, u2)
  * draw a mixture direction; out n x 8: dir xyz, p_vmf(dir), alpha, lobe */
 int lmx_guide_mix_sample(const float *y, const float *u, uint32_t n, float *out, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Film output (Film_HDR::Save, film_hdr.cpp:260-280 and SaveImage :47-164,
+ * without FreeImage).  film_rgb is a device film of W x H x 3 floats (row 0 =
+ * bottom, as the renderers write it); the file is path + ".hdr" / ".exr" /
+ * ".png" (the reference appends the type's suffix) / ".pfm".  radiancehdr:
+ * RGBE with run-length encoded scanlines; openexr: uncompressed 32-bit float
+ * B, G, R scanlines; png: 8-bit RGB of (int)(v^(1/2.2) 255) clamped to
+ * [0, 255]; radiancehdr and openexr clamp negative values to 0, and all three
+ * store the top row first.  pfm (no reference counterpart): little-endian
+ * "PF", rows bottom to top, values as rendered.  The pixels are encoded on
+ * `stream`; the call returns once the file is written. */
+#define LMX_IMAGE_RADIANCEHDR 0
+#define LMX_IMAGE_OPENEXR 1
+#define LMX_IMAGE_PNG 2
+#define LMX_IMAGE_PFM 3
+int lmx_film_save(const float *film_rgb, int32_t width, int32_t height, int32_t type, const char *path,
+                  void *stream);
+/* Reads a .hdr (flat or run-length scanlines, "-Y H +X W"), an .exr as
+ * lmx_film_save writes it, or a little-endian .pfm into rgb (host,
+ * W x H x 3 floats, row 0 = bottom); rgb NULL: only the size.  No reference
+ * counterpart: for exchanging golden images. */
+int lmx_image_load(const char *path, float *rgb, int32_t *width, int32_t *height);
 
 /* Copies the host-side BVH out for inspection: num_nodes x 128-byte nodes and
  * num_tris x 48-byte triangle records (layouts in

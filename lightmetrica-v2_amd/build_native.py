@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "liblmx.so")
 
-SOURCES = ["lmx_accel.hip", "lmx_render.hip", "lmx_bdpt.hip", "lmx_guide.hip", "wbvh_build.cpp"]
+SOURCES = ["lmx_accel.hip", "lmx_render.hip", "lmx_bdpt.hip", "lmx_guide.hip", "lmx_image.hip", "wbvh_build.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

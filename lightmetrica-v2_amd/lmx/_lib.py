@@ -104,6 +104,10 @@ SIGNATURES = {
                                        ctypes.c_void_p, ctypes.POINTER(lmx_render_stats)]),
     "lmx_render_vcm_bdpt": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(lmx_render_params), ctypes.c_void_p,
                                            ctypes.c_void_p, ctypes.POINTER(lmx_render_stats)]),
+    "lmx_film_save": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_char_p,
+                                     ctypes.c_void_p]),
+    "lmx_image_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.POINTER(ctypes.c_int32)]),
     "lmx_render_ptdirect": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(lmx_render_params), ctypes.c_void_p,
                                            ctypes.c_void_p, ctypes.POINTER(lmx_render_stats)]),
     "lmx_render_ptmis": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(lmx_render_params), ctypes.c_void_p,

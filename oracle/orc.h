@@ -208,6 +208,11 @@ typedef struct orc_bdptref_params {
 void orc_bdptref_samples(const orc_scene *s, const orc_bdptref_params *p, const float *table, uint32_t table_width,
                          uint64_t first_sample, uint64_t n, float *out, int nthreads);
 
+/* Film output transforms of Film_HDR::Save (orc_image.c), file row order:
+ * RGBE bytes (4 per pixel) and PNG tone-mapped bytes (3 per pixel). */
+void orc_film_rgbe(const float *film, int w, int h, uint8_t *out);
+void orc_film_png(const float *film, int w, int h, uint8_t *out);
+
 /* BSDF test hooks on a flat frame with normal n */
 int orc_bsdf_query(const orc_material *m, const float n[3], const float wi[3], const float u[2],
                    float wo[3], float *pdf, float f[3]);

@@ -229,6 +229,10 @@ def _render_sigs(so):
     so.orc_bdptref_samples.argtypes = [vp, ctypes.POINTER(orc_bdptref_params), vp, ctypes.c_uint32, ctypes.c_uint64,
                                        ctypes.c_uint64, vp, ctypes.c_int]
     so.orc_bdptref_samples.restype = None
+    so.orc_film_rgbe.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    so.orc_film_rgbe.restype = None
+    so.orc_film_png.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    so.orc_film_png.restype = None
     so.orc_ptnee_samples.argtypes = [vp, ctypes.POINTER(orc_pt_params), ctypes.c_int, vp, ctypes.c_uint32,
                                      ctypes.c_uint64, ctypes.c_uint64, vp, ctypes.c_int]
     so.orc_ptnee_samples.restype = None
@@ -495,3 +499,22 @@ def guide_adam(params, m, v, grad, step, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e
 
 def bf16_round(x):
     return _glib().orc_bf16_round(float(x))
+
+
+def film_rgbe(film):
+    """Film_HDR::Save's RGBE bytes of an (H, W, 3) film (row 0 = bottom), in
+    file row order (top row first): (H, W, 4) uint8."""
+    f = np.ascontiguousarray(film, np.float32)
+    h, w = f.shape[:2]
+    out = np.zeros((h, w, 4), np.uint8)
+    lib().orc_film_rgbe(_p(f), w, h, _p(out))
+    return out
+
+
+def film_png(film):
+    """Film_HDR::Save's PNG tone-mapped bytes, file row order: (H, W, 3)."""
+    f = np.ascontiguousarray(film, np.float32)
+    h, w = f.shape[:2]
+    out = np.zeros((h, w, 3), np.uint8)
+    lib().orc_film_png(_p(f), w, h, _p(out))
+    return out
