@@ -432,26 +432,40 @@ RD float out_grad(const float* y, f3 dir, float pb, float w, float* g)
 // ---------------------------------------------------------------------------
 // kernels: evaluation and test hooks
 
-__global__ __launch_bounds__(256) void k_eval(GuideDev G, const float* __restrict__ x, uint32_t n, float* __restrict__ y)
+// Inference workgroups: EVAL_WAVES waves sharing the weights in LDS
+// (55 KB); each wave's inputs (64 x 8 floats) live at the start of its
+// output rows (64 x YS floats), which layer 1 has read before layer 5
+// writes them.  156.6 KB of LDS: one workgroup, 8 waves, per CU.
+constexpr int EVAL_WAVES = 8;
+static_assert(N_FWD * 16 + N_BIAS * 4 + EVAL_WAVES * 64 * YS * 4 <= 163840, "inference LDS");
+
+__global__ __launch_bounds__(EVAL_WAVES * 64) void k_eval(GuideDev G, const float* __restrict__ x, uint32_t n,
+                                                          float* __restrict__ y)
 {
     __shared__ bf16x8 wl[N_FWD];
     __shared__ float bl[N_BIAS];
-    __shared__ float xs[4][64 * 8];
-    __shared__ float ys[4][64 * YS];
+    __shared__ float ys[EVAL_WAVES][64 * YS];
     for (int t = threadIdx.x; t < N_FWD; t += blockDim.x) wl[t] = G.pack[t];
     for (int t = threadIdx.x; t < N_BIAS; t += blockDim.x) bl[t] = G.bias[t];
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float* yw = ys[wave];
     NoSink dummy;
-    for (uint32_t base = (blockIdx.x * 4 + wave) * 64u; base < n; base += gridDim.x * 256u) {
-        const uint32_t q = base + lane;
-#pragma unroll
-        for (int d = 0; d < 7; d++) xs[wave][lane * 8 + d] = q < n ? x[(size_t)q * 7 + d] : 0.0f;
+    for (uint32_t base = (blockIdx.x * EVAL_WAVES + wave) * 64u; base < n; base += gridDim.x * EVAL_WAVES * 64u) {
+        // the wave's 64 x 7 inputs, coalesced
+        const uint32_t nq = min(64u, n - base);
+        for (uint32_t k = (uint32_t)lane; k < 64u * 7u; k += 64u) {
+            const uint32_t q = k / 7u, d = k - 7u * q;
+            yw[q * 8u + d] = q < nq ? x[(size_t)base * 7 + k] : 0.0f;
+        }
         wave_sync();
-        mlp_forward<4>(wl, bl, xs[wave], ys[wave], dummy);
+        mlp_forward<4>(wl, bl, yw, yw, dummy);
         wave_sync();
-        if (q < n)
-            for (int o = 0; o < 41; o++) y[(size_t)q * 41 + o] = ys[wave][lane * YS + o];
+        // the wave's 64 x 41 outputs, coalesced
+        for (uint32_t k = (uint32_t)lane; k < nq * 41u; k += 64u) {
+            const uint32_t q = k / 41u, o = k - 41u * q;
+            y[(size_t)base * 41 + k] = yw[q * YS + o];
+        }
         wave_sync();
     }
 }
@@ -814,14 +828,15 @@ struct GuideLaunch {
 
 namespace {
 
-__global__ __launch_bounds__(256) void k_guided_shade(RScene s, RParams p, GuideLaunch gl, Ctl* __restrict__ ctl,
-                                                      Queue in, const float4* __restrict__ hits, Queue out)
+// EVAL_WAVES waves per workgroup share the weights in LDS, as k_eval's do
+__global__ __launch_bounds__(EVAL_WAVES * 64) void k_guided_shade(RScene s, RParams p, GuideLaunch gl,
+                                                                  Ctl* __restrict__ ctl, Queue in,
+                                                                  const float4* __restrict__ hits, Queue out)
 {
     __shared__ bf16x8 wl[N_FWD];
     __shared__ float bl[N_BIAS];
-    __shared__ float xs[4][64 * 8];
-    __shared__ float ys[4][64 * YS];
-    __shared__ uint32_t lds[5];
+    __shared__ float ys[EVAL_WAVES][64 * YS];       // inputs at the start of each wave's rows
+    __shared__ uint32_t lds[EVAL_WAVES + 1];
     for (int t = threadIdx.x; t < N_FWD; t += blockDim.x) wl[t] = gl.dev.pack[t];
     for (int t = threadIdx.x; t < N_BIAS; t += blockDim.x) bl[t] = gl.dev.bias[t];
     __syncthreads();
@@ -888,9 +903,9 @@ __global__ __launch_bounds__(256) void k_guided_shade(RScene s, RParams p, Guide
             }
         }
 #pragma unroll
-        for (int k = 0; k < 7; k++) xs[wave][lane * 8 + k] = x[k];
+        for (int k = 0; k < 7; k++) ys[wave][lane * 8 + k] = x[k];
         __syncthreads();
-        if (__ballot(guided)) mlp_forward<4>(wl, bl, xs[wave], ys[wave], dummy);   // wave-uniform
+        if (__ballot(guided)) mlp_forward<4>(wl, bl, ys[wave], ys[wave], dummy);   // wave-uniform
         __syncthreads();
         if (guided) {
             Mix mx;
@@ -932,7 +947,7 @@ __global__ __launch_bounds__(256) void k_guided_shade(RScene s, RParams p, Guide
                 }
             }
         }
-        const uint32_t slot = block_reserve<4>(&ctl->n_next, cont ? 1u : 0u, lds);
+        const uint32_t slot = block_reserve<EVAL_WAVES>(&ctl->n_next, cont ? 1u : 0u, lds);
         if (cont) {
             out.ray[2 * slot] = o_ray0;
             out.ray[2 * slot + 1] = o_ray1;
@@ -940,7 +955,7 @@ __global__ __launch_bounds__(256) void k_guided_shade(RScene s, RParams p, Guide
             out.meta[slot] = o_meta;
         }
         if (gl.recs) {
-            const uint32_t r = block_reserve<4>(gl.rec_count, rec ? 1u : 0u, lds);
+            const uint32_t r = block_reserve<EVAL_WAVES>(gl.rec_count, rec ? 1u : 0u, lds);
             if (rec && r < gl.rec_cap) {
                 float4* R = gl.recs + 5 * (size_t)r;
                 R[0] = make_float4(x[0], x[1], x[2], x[3]);
@@ -957,8 +972,8 @@ __global__ __launch_bounds__(256) void k_guided_shade(RScene s, RParams p, Guide
 
 int launch_guided_shade(lmx_scene* sc, const RParams& rp, Queue qa, Queue qb, hipStream_t st)
 {
-    const int grid = std::max(1, std::min((int)((sc->cap + 255) / 256), sc->cus * 2));
-    k_guided_shade<<<grid, 256, 0, st>>>(sc->rs, rp, *rp.guide, sc->ctl.p, qa, sc->hits.p, qb);
+    const int grid = std::max(1, std::min((int)((sc->cap + EVAL_WAVES * 64 - 1) / (EVAL_WAVES * 64)), sc->cus));
+    k_guided_shade<<<grid, EVAL_WAVES * 64, 0, st>>>(sc->rs, rp, *rp.guide, sc->ctl.p, qa, sc->hits.p, qb);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? LMX_OK : hip_error(e, "guided shade launch");
 }
@@ -1134,8 +1149,8 @@ int lmx_guide_eval(lmx_guide* g, const float* x, uint32_t n, float* y, void* str
     if (!g || (n && (!x || !y))) return set_error(LMX_EINVAL, "lmx_guide_eval: NULL argument");
     if (n == 0) return LMX_OK;
     hipStream_t st = (hipStream_t)stream;
-    const int grid = std::max(1, std::min((int)((n + 255) / 256), g->scene->cus));
-    k_eval<<<grid, 256, 0, st>>>(g->dev(), x, n, y);
+    const int grid = std::max(1, std::min((int)((n + EVAL_WAVES * 64 - 1) / (EVAL_WAVES * 64)), g->scene->cus));
+    k_eval<<<grid, EVAL_WAVES * 64, 0, st>>>(g->dev(), x, n, y);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? LMX_OK : hip_error(e, "lmx_guide_eval");
 }
