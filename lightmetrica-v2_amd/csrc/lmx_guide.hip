@@ -72,7 +72,8 @@ constexpr int OFF_B5 = N_FWD;         // backward output layer: W5 as [hidden][o
 constexpr int OFF_B4 = OFF_B5 + N_AH; // backward layers 4, 3, 2: W as [input][output]
 constexpr int N_PACK = OFF_B4 + 3 * N_AH;
 constexpr int N_BIAS = 3 * 64 + 48;   // b2, b3, b4, b5 (zero-padded to 48)
-constexpr int YS = 49;                // LDS row stride of the 41 outputs (bank spread)
+constexpr int YS = 41;                // LDS row stride of the 41 outputs (9 banks apart: conflict-free rows)
+constexpr int XS = 7;                 // LDS row stride of the 7 inputs
 
 constexpr float LOGK_MIN = -4.0f, LOGK_MAX = 9.0f;
 // BSDF selection probability alpha in [0.5, 0.9]: a direction the BSDF draws
@@ -178,15 +179,26 @@ RD void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+typedef float f32x2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x4v __attribute__((ext_vector_type(4)));
+typedef short s16x2v __attribute__((ext_vector_type(2)));
+
+// two values rounded to bf16 as one packed pair, then the ReLU on the pair
+// as two 16-bit integers: a set sign bit (negative, or -0) is a negative
+// integer and becomes +0, the rest keep their bits
+RD bf16x2v relu2(float a, float b)
+{
+    const bf16x2v p = __builtin_convertvector((f32x2v){a, b}, bf16x2v);
+    const s16x2v v = __builtin_elementwise_max(__builtin_bit_cast(s16x2v, p), (s16x2v){0, 0});
+    return __builtin_bit_cast(bf16x2v, v);
+}
+
 RD bf16x8 relu_pack(f32x4 lo, f32x4 hi)
 {
-    bf16x8 r;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        r[i] = (__bf16)fmaxf(lo[i], 0.0f);
-        r[4 + i] = (__bf16)fmaxf(hi[i], 0.0f);
-    }
-    return r;
+    const bf16x4v l = __builtin_shufflevector(relu2(lo[0], lo[1]), relu2(lo[2], lo[3]), 0, 1, 2, 3);
+    const bf16x4v h = __builtin_shufflevector(relu2(hi[0], hi[1]), relu2(hi[2], hi[3]), 0, 1, 2, 3);
+    return __builtin_shufflevector(l, h, 0, 1, 2, 3, 4, 5, 6, 7);
 }
 
 // Operand of the encoding for k chunk kc, for this lane's query of tile mt.
@@ -196,12 +208,13 @@ RD bf16x8 encode_chunk(const float* xs, int kc, int g, int r, int mt, float& sx,
     bf16x8 b;
     const int d = kc < 3 ? g : g + 4;
     if (d < 7) {
-        if (kc == 0 || kc == 3) sincospif(xs[(mt * 16 + r) * 8 + d], &sx, &cx);
+        if (kc == 0 || kc == 3) sincospif(xs[(mt * 16 + r) * XS + d], &sx, &cx);
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             b[2 * q] = (__bf16)sx;
             b[2 * q + 1] = (__bf16)cx;
-            const float s2 = 2.0f * sx * cx, c2 = cx * cx - sx * sx;
+            const float sc = sx * cx;
+            const float s2 = sc + sc, c2 = fmaf(cx, cx, -(sx * sx));
             sx = s2;
             cx = c2;
         }
@@ -305,7 +318,8 @@ RD void mlp_forward(const bf16x8* __restrict__ W, const float* __restrict__ bias
 #pragma unroll
         for (int mt = 0; mt < MT; mt++)
 #pragma unroll
-            for (int i = 0; i < 4; i++) ys[(mt * 16 + r) * YS + nt * 16 + 4 * g + i] = o[nt][mt][i];
+            for (int i = 0; i < 4; i++)
+                if (nt * 16 + 4 * g + i < 41) ys[(mt * 16 + r) * YS + nt * 16 + 4 * g + i] = o[nt][mt][i];
 }
 
 // Network inputs of a shading point: position in the scene bounds, shading
@@ -433,18 +447,21 @@ RD float out_grad(const float* y, f3 dir, float pb, float w, float* g)
 // kernels: evaluation and test hooks
 
 // Inference workgroups: EVAL_WAVES waves sharing the weights in LDS
-// (55 KB); each wave's inputs (64 x 8 floats) live at the start of its
-// output rows (64 x YS floats), which layer 1 has read before layer 5
-// writes them.  156.6 KB of LDS: one workgroup, 8 waves, per CU.
-constexpr int EVAL_WAVES = 8;
+// (55 KB); each wave's inputs (64 x 7 floats) live at the start of its
+// output rows (64 x 41 floats), which layer 1 has read before layer 5
+// writes them.  140 KB of LDS: one workgroup, 8 waves, per CU.
+#ifndef LMX_EVAL_WAVES
+#define LMX_EVAL_WAVES 8
+#endif
+constexpr int EVAL_WAVES = LMX_EVAL_WAVES;
 static_assert(N_FWD * 16 + N_BIAS * 4 + EVAL_WAVES * 64 * YS * 4 <= 163840, "inference LDS");
 
 __global__ __launch_bounds__(EVAL_WAVES * 64) void k_eval(GuideDev G, const float* __restrict__ x, uint32_t n,
-                                                          float* __restrict__ y)
+                                                          float* __restrict__ y, int vec)
 {
     __shared__ bf16x8 wl[N_FWD];
     __shared__ float bl[N_BIAS];
-    __shared__ float ys[EVAL_WAVES][64 * YS];
+    __shared__ __attribute__((aligned(16))) float ys[EVAL_WAVES][64 * YS];
     for (int t = threadIdx.x; t < N_FWD; t += blockDim.x) wl[t] = G.pack[t];
     for (int t = threadIdx.x; t < N_BIAS; t += blockDim.x) bl[t] = G.bias[t];
     __syncthreads();
@@ -452,19 +469,23 @@ __global__ __launch_bounds__(EVAL_WAVES * 64) void k_eval(GuideDev G, const floa
     float* yw = ys[wave];
     NoSink dummy;
     for (uint32_t base = (blockIdx.x * EVAL_WAVES + wave) * 64u; base < n; base += gridDim.x * EVAL_WAVES * 64u) {
-        // the wave's 64 x 7 inputs, coalesced
+        // the wave's 64 x 7 inputs and 64 x 41 outputs are contiguous blocks
+        // in both memories: 16-byte copies for a full block
         const uint32_t nq = min(64u, n - base);
-        for (uint32_t k = (uint32_t)lane; k < 64u * 7u; k += 64u) {
-            const uint32_t q = k / 7u, d = k - 7u * q;
-            yw[q * 8u + d] = q < nq ? x[(size_t)base * 7 + k] : 0.0f;
+        if (nq == 64u && vec) {
+            const float4* src = reinterpret_cast<const float4*>(x + (size_t)base * 7);
+            for (uint32_t k = (uint32_t)lane; k < 112u; k += 64u) reinterpret_cast<float4*>(yw)[k] = src[k];
+        } else {
+            for (uint32_t k = (uint32_t)lane; k < 64u * 7u; k += 64u) yw[k] = k < nq * 7u ? x[(size_t)base * 7 + k] : 0.0f;
         }
         wave_sync();
         mlp_forward<4>(wl, bl, yw, yw, dummy);
         wave_sync();
-        // the wave's 64 x 41 outputs, coalesced
-        for (uint32_t k = (uint32_t)lane; k < nq * 41u; k += 64u) {
-            const uint32_t q = k / 41u, o = k - 41u * q;
-            y[(size_t)base * 41 + k] = yw[q * YS + o];
+        if (nq == 64u && vec) {
+            float4* dst = reinterpret_cast<float4*>(y + (size_t)base * 41);
+            for (uint32_t k = (uint32_t)lane; k < 656u; k += 64u) dst[k] = reinterpret_cast<const float4*>(yw)[k];
+        } else {
+            for (uint32_t k = (uint32_t)lane; k < nq * 41u; k += 64u) y[(size_t)base * 41 + k] = yw[k];
         }
         wave_sync();
     }
@@ -618,7 +639,7 @@ __global__ __launch_bounds__(FB_WAVES * 64) void k_train_fb(GuideDev G, const fl
                 dir = mk3(c.x, c.y, c.z);
                 w = fminf(d.w * wnorm, wclamp) * inv_n;
             }
-            for (int k = 0; k < 7; k++) xs[lane * 8 + k] = x[k];
+            for (int k = 0; k < 7; k++) xs[lane * XS + k] = x[k];
         }
         wave_sync();
         TrainSink sink{blk, lane, {0u, 0u, 0u, 0u}};
@@ -828,15 +849,17 @@ struct GuideLaunch {
 
 namespace {
 
-// EVAL_WAVES waves per workgroup share the weights in LDS, as k_eval's do
-__global__ __launch_bounds__(EVAL_WAVES * 64) void k_guided_shade(RScene s, RParams p, GuideLaunch gl,
+// GS_WAVES waves per workgroup share the weights in LDS, as k_eval's do
+// (224 VGPRs: two waves per SIMD)
+constexpr int GS_WAVES = 8;
+__global__ __launch_bounds__(GS_WAVES * 64) void k_guided_shade(RScene s, RParams p, GuideLaunch gl,
                                                                   Ctl* __restrict__ ctl, Queue in,
                                                                   const float4* __restrict__ hits, Queue out)
 {
     __shared__ bf16x8 wl[N_FWD];
     __shared__ float bl[N_BIAS];
-    __shared__ float ys[EVAL_WAVES][64 * YS];       // inputs at the start of each wave's rows
-    __shared__ uint32_t lds[EVAL_WAVES + 1];
+    __shared__ float ys[GS_WAVES][64 * YS];       // inputs at the start of each wave's rows
+    __shared__ uint32_t lds[GS_WAVES + 1];
     for (int t = threadIdx.x; t < N_FWD; t += blockDim.x) wl[t] = gl.dev.pack[t];
     for (int t = threadIdx.x; t < N_BIAS; t += blockDim.x) bl[t] = gl.dev.bias[t];
     __syncthreads();
@@ -903,7 +926,7 @@ __global__ __launch_bounds__(EVAL_WAVES * 64) void k_guided_shade(RScene s, RPar
             }
         }
 #pragma unroll
-        for (int k = 0; k < 7; k++) ys[wave][lane * 8 + k] = x[k];
+        for (int k = 0; k < 7; k++) ys[wave][lane * XS + k] = x[k];
         __syncthreads();
         if (__ballot(guided)) mlp_forward<4>(wl, bl, ys[wave], ys[wave], dummy);   // wave-uniform
         __syncthreads();
@@ -947,7 +970,7 @@ __global__ __launch_bounds__(EVAL_WAVES * 64) void k_guided_shade(RScene s, RPar
                 }
             }
         }
-        const uint32_t slot = block_reserve<EVAL_WAVES>(&ctl->n_next, cont ? 1u : 0u, lds);
+        const uint32_t slot = block_reserve<GS_WAVES>(&ctl->n_next, cont ? 1u : 0u, lds);
         if (cont) {
             out.ray[2 * slot] = o_ray0;
             out.ray[2 * slot + 1] = o_ray1;
@@ -955,7 +978,7 @@ __global__ __launch_bounds__(EVAL_WAVES * 64) void k_guided_shade(RScene s, RPar
             out.meta[slot] = o_meta;
         }
         if (gl.recs) {
-            const uint32_t r = block_reserve<EVAL_WAVES>(gl.rec_count, rec ? 1u : 0u, lds);
+            const uint32_t r = block_reserve<GS_WAVES>(gl.rec_count, rec ? 1u : 0u, lds);
             if (rec && r < gl.rec_cap) {
                 float4* R = gl.recs + 5 * (size_t)r;
                 R[0] = make_float4(x[0], x[1], x[2], x[3]);
@@ -972,8 +995,8 @@ __global__ __launch_bounds__(EVAL_WAVES * 64) void k_guided_shade(RScene s, RPar
 
 int launch_guided_shade(lmx_scene* sc, const RParams& rp, Queue qa, Queue qb, hipStream_t st)
 {
-    const int grid = std::max(1, std::min((int)((sc->cap + EVAL_WAVES * 64 - 1) / (EVAL_WAVES * 64)), sc->cus));
-    k_guided_shade<<<grid, EVAL_WAVES * 64, 0, st>>>(sc->rs, rp, *rp.guide, sc->ctl.p, qa, sc->hits.p, qb);
+    const int grid = std::max(1, std::min((int)((sc->cap + GS_WAVES * 64 - 1) / (GS_WAVES * 64)), sc->cus));
+    k_guided_shade<<<grid, GS_WAVES * 64, 0, st>>>(sc->rs, rp, *rp.guide, sc->ctl.p, qa, sc->hits.p, qb);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? LMX_OK : hip_error(e, "guided shade launch");
 }
@@ -1150,7 +1173,8 @@ int lmx_guide_eval(lmx_guide* g, const float* x, uint32_t n, float* y, void* str
     if (n == 0) return LMX_OK;
     hipStream_t st = (hipStream_t)stream;
     const int grid = std::max(1, std::min((int)((n + EVAL_WAVES * 64 - 1) / (EVAL_WAVES * 64)), g->scene->cus));
-    k_eval<<<grid, EVAL_WAVES * 64, 0, st>>>(g->dev(), x, n, y);
+    const int vec = ((uintptr_t)x % 16 == 0 && (uintptr_t)y % 16 == 0) ? 1 : 0;    // 16-byte block copies
+    k_eval<<<grid, EVAL_WAVES * 64, 0, st>>>(g->dev(), x, n, y, vec);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? LMX_OK : hip_error(e, "lmx_guide_eval");
 }
