@@ -75,7 +75,9 @@ class Scene:
         d.light_prims = keep(lp, np.uint32)
         d.nlight = lp.shape[0]
         cam = sc["camera"]
-        pos, vx, vy, vz = camera_frame(cam)
+        # eye / center / up, or the sensor's frame (position and the three
+        # columns of its transform) as a scene file gives it
+        pos, vx, vy, vz = cam["frame"] if "frame" in cam else camera_frame(cam)
         c = lmx_camera()
         c.pos[:] = list(pos); c.vx[:] = list(vx); c.vy[:] = list(vy); c.vz[:] = list(vz)
         c.fov_deg = cam["fov_deg"]

@@ -1,0 +1,73 @@
+"""A reference-format scene file of the Cornell box (raw meshes, area light,
+diffuse BSDFs, lookat pinhole, film asset, renderer node) renders the same
+paths as the same scene built directly: per-sample contributions from one
+Philox stream agree, and render_file writes the film with its type's suffix."""
+import numpy as np
+import pytest
+
+from lmx import image, meshfile, render, scenefile, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _cornell_yaml(sc, spp):
+    cam = sc["camera"]
+    lines = ["lightmetrica:", "  version: 1.1.0", "  assets:", "    type: assets3", "    params:",
+             "      film_1:", "        interface: film", "        type: hdr",
+             f"        params: {{w: {cam['width']}, h: {cam['height']}, type: radiancehdr}}",
+             "      sensor_1:", "        interface: sensor", "        type: pinhole",
+             f"        params: {{film: film_1, fov: {cam['fov_deg']}}}"]
+    nodes = ["        - id: cam", "          sensor: sensor_1", "          transform:", "            lookat:",
+             "              eye: %g %g %g" % cam["eye"], "              center: %g %g %g" % cam["center"],
+             "              up: %g %g %g" % cam["up"]]
+    light_tris = set()
+    for first, count in sc["lights"]:
+        light_tris.update(range(first, first + count))
+    for mi, m in enumerate(sc["materials"]):
+        sel = np.nonzero(sc["tri_mat"] == mi)[0]
+        if sel.size == 0:
+            continue
+        f = sc["tris"][sel]
+        used, inv = np.unique(f.reshape(-1), return_inverse=True)
+        mesh = meshfile.dump_raw_mesh(sc["verts"][used], inv.reshape(-1, 3))
+        lines += [f"      mesh_{mi}:", "        interface: trianglemesh", "        type: raw", "        params:"]
+        lines += ["          " + l for l in mesh.splitlines()]
+        nodes += [f"        - id: n{mi}", f"          mesh: mesh_{mi}"]
+        if m["type"] == scenes.MAT_DIFFUSE:
+            lines += [f"      bsdf_{mi}:", "        interface: bsdf", "        type: diffuse",
+                      "        params: {R: '%r %r %r'}" % tuple(m["R"])]
+            nodes.append(f"          bsdf: bsdf_{mi}")
+        if any(m["Le"]):
+            lines += [f"      light_{mi}:", "        interface: light", "        type: area",
+                      "        params: {Le: '%r %r %r'}" % tuple(m["Le"])]
+            nodes.append(f"          light: light_{mi}")
+    n = cam["width"] * cam["height"] * spp
+    lines += ["  scene:", "    type: scene3", "    params:", "      sensor: cam", "      nodes:"] + nodes
+    lines += ["  renderer:", "    type: pt", "    params:", f"      num_samples: {n}", "      max_num_vertices: 9"]
+    return "\n".join(lines) + "\n"
+
+
+def test_scene_file_renders_the_same_paths(gpu, tmp_path):
+    sc = scenes.cornell_box(32, 32)
+    text = _cornell_yaml(sc, 16)
+    loaded = scenefile.load_scene(text)
+    s2 = loaded["scene"]
+    assert s2["tris"].shape == sc["tris"].shape and len(s2["lights"]) == 1
+    n = 32 * 32 * 16
+    a = render.Scene(sc).render_pt(n, max_num_vertices=9, seed=5, sample_out=True)[2].cpu().numpy()
+    b = render.Scene(s2).render_pt(n, max_num_vertices=9, seed=5, sample_out=True)[2].cpu().numpy()
+    same = (a[:, 5] == b[:, 5]) & np.all(np.abs(a[:, 2:5] - b[:, 2:5]) <= 1e-5 * np.maximum(np.abs(a[:, 2:5]), 1e-3), axis=1)
+    assert (a[:, 5] > 0).sum() > 20
+    assert same.mean() > 0.999, (~same).sum()
+    # the whole command: renderer node, film type, suffix
+    path = tmp_path / "cornell.yml"
+    path.write_text(text)
+    out = scenefile.render_file(str(path), str(tmp_path / "result"), seed=7)
+    assert out == str(tmp_path / "result") + ".hdr"
+    img = image.load(out)
+    ref = render.Scene(sc).render_pt(n, max_num_vertices=9, seed=7, raster_mode=render.RASTER_RANDOM)[0].cpu().numpy()
+    # RGBE keeps 8 mantissa bits of the largest channel
+    assert img.shape == ref.shape
+    np.testing.assert_allclose(img.mean(), ref.mean(), rtol=1e-2)
+    m = ref.max(axis=2, keepdims=True)
+    assert (np.abs(img - ref) <= m * 2.0 ** -7 + 1e-6).mean() > 0.99
