@@ -106,3 +106,11 @@ def test_renderer_bdpt_and_pt_images_agree_within_3_sigma(gpu):
     assert lit.mean() > 0.5
     assert (z[lit] <= 3.0).mean() > 0.97, (z[lit] > 3).mean()
     assert abs(mb.mean() - mp.mean()) < 0.01 * mp.mean()
+
+
+def test_renderer_bdpt_per_strategy_parity_ggx(gpu):
+    """bsdf::cooktorrance boxes: connections evaluate the GGX lobe from both
+    sides and the direct strategies at glossy eye vertices."""
+    sc = scenes.cornell_box(32, 32, ggx_boxes=0.3)
+    g, c = _run(sc, 32 * 32 * 8, 8, gpu, table_seed=17, batch=4096)
+    _check(g, c, tight=0.99, loose=0.999, min_both=3000)
