@@ -192,6 +192,31 @@ typedef struct orc_bdpt_params {
 void orc_bdpt_samples(const orc_scene *s, const orc_bdpt_params *p, const float *table, uint32_t table_width,
                       uint64_t first_sample, uint64_t n, float *out, int nthreads);
 
+/* renderer::vcm, modes vcm and bdpm, one pass with merge radius `radius`
+ * over `num_photons` photon subpaths (orc_render.c).  Eye uniforms as
+ * orc_bdpt_samples (table, or Philox with seed); photon subpath i uses the
+ * light block of sample i of its own source (ptable, or Philox with pseed).
+ * Output per eye sample: 2 (maxv+1)^2 records of 8 floats at
+ * 2 (t (maxv+1) + s) + merge: raster x, y, C rgb (f w / p before the
+ * W H / N film scale; merges summed over their photon vertices), count, sum
+ * of w, 0.  A merge of eye vertex t-1 with photon vertex j is recorded at
+ * s = j + 1. */
+#define ORC_VCM_VCM 0
+#define ORC_VCM_BDPM 2
+typedef struct orc_vcm_params {
+    int32_t max_num_vertices;
+    int32_t min_num_vertices;
+    uint32_t seed_lo, seed_hi;
+    uint32_t pseed_lo, pseed_hi;
+    int32_t raster_mode;
+    int32_t mode;
+    uint32_t num_photons;
+    float radius;
+} orc_vcm_params;
+void orc_vcm_samples(const orc_scene *s, const orc_vcm_params *p, const float *table, uint32_t table_width,
+                     const float *ptable, uint32_t ptable_width, uint64_t first_sample, uint64_t n, float *out,
+                     int nthreads);
+
 /* renderer::bdpt (orc_render.c).  max_num_vertices: -1 (unbounded) or >= 2;
  * min_num_vertices applies only when bounded (renderer_bdpt.cpp:887);
  * mis: 0 power heuristic, 1 simple.  Output per sample: 2 (M+1)^2 records of

@@ -208,6 +208,17 @@ class orc_bdptref_params(ctypes.Structure):
                 ("mis", ctypes.c_int32)]
 
 
+class orc_vcm_params(ctypes.Structure):
+    _fields_ = [("max_num_vertices", ctypes.c_int32), ("min_num_vertices", ctypes.c_int32),
+                ("seed_lo", ctypes.c_uint32), ("seed_hi", ctypes.c_uint32),
+                ("pseed_lo", ctypes.c_uint32), ("pseed_hi", ctypes.c_uint32), ("raster_mode", ctypes.c_int32),
+                ("mode", ctypes.c_int32), ("num_photons", ctypes.c_uint32), ("radius", ctypes.c_float)]
+
+
+VCM_VCM = 0
+VCM_BDPM = 2
+
+
 def _render_sigs(so):
     vp = ctypes.c_void_p
     so.orc_philox4x32_10.argtypes = [vp, vp, vp]
@@ -229,6 +240,9 @@ def _render_sigs(so):
     so.orc_bdptref_samples.argtypes = [vp, ctypes.POINTER(orc_bdptref_params), vp, ctypes.c_uint32, ctypes.c_uint64,
                                        ctypes.c_uint64, vp, ctypes.c_int]
     so.orc_bdptref_samples.restype = None
+    so.orc_vcm_samples.argtypes = [vp, ctypes.POINTER(orc_vcm_params), vp, ctypes.c_uint32, vp, ctypes.c_uint32,
+                                   ctypes.c_uint64, ctypes.c_uint64, vp, ctypes.c_int]
+    so.orc_vcm_samples.restype = None
     so.orc_film_rgbe.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
     so.orc_film_rgbe.restype = None
     so.orc_film_png.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
@@ -394,6 +408,26 @@ class Scene:
             tw = table.shape[1]
             tp = ctypes.c_void_p(table.ctypes.data)
         lib().orc_bdptref_samples(self._h, ctypes.byref(p), tp, tw, first, n, _p(out), nthreads)
+        return out
+
+    def vcm_samples(self, first, n, radius, num_photons, mode=VCM_VCM, max_num_vertices=10, min_num_vertices=0,
+                    seed=42, pseed=43, raster_mode=RASTER_STRATIFIED, table=None, ptable=None, nthreads=8):
+        """renderer::vcm (vcm or bdpm) for one pass: (n, 2 (m+1)^2, 8) float32 per
+        2 (t (m+1) + s) + merge: raster x, y, C rgb (merges summed over
+        photon vertices), count, sum of w, 0."""
+        m = max_num_vertices
+        p = orc_vcm_params(m, min_num_vertices, seed & 0xFFFFFFFF, seed >> 32, pseed & 0xFFFFFFFF, pseed >> 32,
+                           raster_mode, mode, num_photons, radius)
+        out = np.zeros((n, 2 * (m + 1) * (m + 1), 8), np.float32)
+        tw = ptw = 0
+        tp = pp = None
+        if table is not None:
+            table = np.ascontiguousarray(table, np.float32)
+            tw, tp = table.shape[1], ctypes.c_void_p(table.ctypes.data)
+        if ptable is not None:
+            ptable = np.ascontiguousarray(ptable, np.float32)
+            ptw, pp = ptable.shape[1], ctypes.c_void_p(ptable.ctypes.data)
+        lib().orc_vcm_samples(self._h, ctypes.byref(p), tp, tw, pp, ptw, first, n, _p(out), nthreads)
         return out
 
     def film(self, samples):

@@ -936,6 +936,309 @@ void orc_bdpt_samples(const orc_scene *s, const orc_bdpt_params *pp, const float
 }
 
 /* ---------------------------------------------------------------------------
+ * renderer::vcm in modes vcm and bdpm (renderer_vcm.cpp:102-114 MergeSubpaths,
+ * :116-200 EvaluateF, :202-256 EvaluatePathPDF, :258-340 the weights,
+ * :580-770 one pass), restated literally for ONE pass with merge radius r:
+ * N photon subpaths (light subpaths of the photon uniform source) give the
+ * merge candidates, every light vertex j >= 1 of them; each eye sample
+ * traces its eye and light subpaths as in bdpt mode, makes every connection
+ * (mode vcm) and, at every eye vertex t-1 >= 1, merges with every photon
+ * vertex j closer than r: the path is photon vertices 0..j-1 followed by eye
+ * vertices t-1..0 (the photon vertex itself is replaced by the eye vertex),
+ * its pdf carries pi r^2, and the weight is the power heuristic over every
+ * connection and merge strategy with merge terms counted N times
+ * (EvaluateMISWeight_VCM) or over the merges alone (EvaluateMISWeight_BDPM).
+ * The candidates come from a uniform grid (the reference's kd-tree returns
+ * the same set).  Uniform slots as orc_bdpt_samples; photon subpath i takes
+ * the light block of sample i of the photon source. */
+
+/* EvaluateF with merging (:116-200) */
+static void vcm_f(const bctx *b, const bvert *x, int n, int s, int merge, float out[3])
+{
+    const int t = n - s;
+    float fL[3] = {1, 1, 1}, fE[3] = {1, 1, 1}, cst[3] = {0, 0, 0}, tmp[3];
+    out[0] = out[1] = out[2] = 0.0f;
+    if (s > 0)
+        for (int i = 0; i < (merge ? s : s - 1); i++) {
+            const v3 wi = i >= 1 ? dir_to(&x[i], &x[i - 1]) : mk(0, 0, 0);
+            v_eval(b, &x[i], wi, dir_to(&x[i], &x[i + 1]), 1, tmp);
+            const float G = gterm(&x[i], &x[i + 1]);
+            for (int c = 0; c < 3; c++) fL[c] *= tmp[c] * G;
+        }
+    if (fL[0] == 0.0f && fL[1] == 0.0f && fL[2] == 0.0f) return;
+    if (t > 0)
+        for (int i = n - 1; i > s; i--) {
+            const v3 wi = i < n - 1 ? dir_to(&x[i], &x[i + 1]) : mk(0, 0, 0);
+            v_eval(b, &x[i], wi, dir_to(&x[i], &x[i - 1]), 0, tmp);
+            const float G = gterm(&x[i], &x[i - 1]);
+            for (int c = 0; c < 3; c++) fE[c] *= tmp[c] * G;
+        }
+    if (fE[0] == 0.0f && fE[1] == 0.0f && fE[2] == 0.0f) return;
+    if (!merge) {
+        eval_f(b, x, n, s, out);
+        return;
+    }
+    /* the merged vertex scatters from the eye side towards the light side */
+    v_eval(b, &x[s], dir_to(&x[s], &x[s + 1]), dir_to(&x[s], &x[s - 1]), 0, cst);
+    for (int c = 0; c < 3; c++) out[c] = fL[c] * cst[c] * fE[c];
+}
+
+/* EvaluatePathPDF with merging (:202-256), product area measure in double */
+static double vcm_pdf(const bctx *b, const bvert *x, int n, int s, int merge, float r)
+{
+    const int t = n - s;
+    if (!merge) return path_pdf(b, x, n, s);
+    if (s == 0 || t == 0) return 0.0;
+    if (x[s].kind == BV_CAMERA) return 0.0;             /* delta position */
+    double pdf = 1.0;
+    bvert x0 = x[0];
+    if (x0.kind != BV_LIGHT) { if (x0.light < 0) return 0.0; x0.kind = BV_LIGHT; }
+    pdf *= (double)pos_pdf(b, &x0);
+    for (int i = 0; i < s; i++) {
+        const bvert *vi = i == 0 ? &x0 : &x[i];
+        const v3 wi = i >= 1 ? dir_to(&x[i], &x[i - 1]) : mk(0, 0, 0);
+        pdf *= (double)v_pdf(b, vi, wi, dir_to(&x[i], &x[i + 1])) * (double)gterm(&x[i], &x[i + 1]);
+    }
+    pdf *= (double)pos_pdf(b, &x[n - 1]);
+    for (int i = n - 1; i >= s + 1; i--) {
+        const v3 wi = i + 1 < n ? dir_to(&x[i], &x[i + 1]) : mk(0, 0, 0);
+        pdf *= (double)v_pdf(b, &x[i], wi, dir_to(&x[i], &x[i - 1])) * (double)gterm(&x[i], &x[i - 1]);
+    }
+    return pdf * (double)(PI_F * r * r);
+}
+
+/* EvaluateMISWeight_VCM / _BDPM (:258-340) */
+static double vcm_weight(const bctx *b, const bvert *x, int n, int s0, int merge0, float r, uint32_t np, int bdpm)
+{
+    const double ps = vcm_pdf(b, x, n, s0, merge0, r);
+    double inv = 0.0;
+    for (int s = 0; s <= n; s++)
+        for (int ty = bdpm ? 1 : 0; ty < 2; ty++) {
+            const double pi = vcm_pdf(b, x, n, s, ty, r);
+            if (pi > 0.0) { const double q = pi / ps; inv += q * q * (ty ? (double)np : 1.0); }
+        }
+    return 1.0 / inv;
+}
+
+typedef struct {
+    const bctx *b;
+    const orc_vcm_params *pp;
+    usrc us;
+    const bvert *ph;          /* photon vertices, np x maxv */
+    const int *ph_n;          /* vertices per photon subpath */
+    /* grid over the photon vertices j >= 1 (cell size r) */
+    float lo[3], cell;
+    int dim[3];
+    int *cstart, *items;      /* items: photon * maxv + j */
+    float *out;
+    int stride;
+    uint64_t first, n;
+} vcm_ctx;
+
+static void vcm_sample(const vcm_ctx *c, uint64_t sample, float *out)
+{
+    const bctx *b = c->b;
+    const orc_vcm_params *pp = c->pp;
+    const int maxv = pp->max_num_vertices;
+    const int M1 = maxv + 1;
+    const int bdpm = pp->mode == ORC_VCM_BDPM;
+    const float r = pp->radius;
+    memset(out, 0, sizeof(float) * 8 * (size_t)c->stride);
+    orc_bdpt_params bp = {pp->max_num_vertices, pp->min_num_vertices, pp->seed_lo, pp->seed_hi, pp->raster_mode};
+    bvert E[64], L[64], x[128];
+    const uint32_t wsub = ORC_PT_SEG0 + 4u * (uint32_t)(maxv - 1);
+    const int nE = trace_subpath(b, &bp, &c->us, sample, 0, 0, E);
+    const int nL = trace_subpath(b, &bp, &c->us, sample, wsub, 1, L);
+    const int npix = b->s->d.width * b->s->d.height;
+    for (int t = 1; t <= nE; t++) {
+        float rx = 0.0f, ry = 0.0f;
+        if (pp->raster_mode == ORC_RASTER_STRATIFIED) {
+            const uint32_t pix = (uint32_t)(sample % (uint64_t)npix);
+            rx = ((float)(pix % (uint32_t)b->s->d.width) + 0.5f) / (float)b->s->d.width;
+            ry = ((float)(pix / (uint32_t)b->s->d.width) + 0.5f) / (float)b->s->d.height;
+        }
+        /* vertex connection (mode vcm) */
+        if (!bdpm) {
+            int minS = 2 - t; if (minS < 0) minS = 0;
+            if (pp->min_num_vertices - t > minS) minS = pp->min_num_vertices - t;
+            int maxS = maxv - t; if (nL < maxS) maxS = nL;
+            for (int s = minS; s <= maxS; s++) {
+                const int n = connect(b, L, nL, E, nE, s, t, x);
+                if (n == 0) continue;
+                float f[3];
+                eval_f(b, x, n, s, f);
+                if (f[0] == 0.0f && f[1] == 0.0f && f[2] == 0.0f) continue;
+                const double ps = vcm_pdf(b, x, n, s, 0, r);
+                if (ps == 0.0) continue;
+                const double w = vcm_weight(b, x, n, s, 0, r, pp->num_photons, 0);
+                float px = rx, py = ry;
+                if (t == 1 || pp->raster_mode != ORC_RASTER_STRATIFIED)
+                    if (!camera_raster(&b->cam, dir_to(&x[n - 1], &x[n - 2]), &px, &py)) continue;
+                float *o = &out[8 * (size_t)(2 * (t * M1 + s))];
+                o[0] = px; o[1] = py;
+                for (int k = 0; k < 3; k++) o[2 + k] = (float)((double)f[k] * w / ps);
+                o[5] = 1.0f; o[6] = (float)w;
+            }
+        }
+        /* vertex merging at eye vertex t-1 (not the pinhole) */
+        const bvert *vE = &E[t - 1];
+        if (vE->kind == BV_CAMERA) continue;
+        float px = rx, py = ry;
+        if (pp->raster_mode != ORC_RASTER_STRATIFIED)
+            if (!camera_raster(&b->cam, dir_to(&E[0], &E[1]), &px, &py)) continue;
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+            const float pa = a == 0 ? vE->fr.p.x : (a == 1 ? vE->fr.p.y : vE->fr.p.z);
+            lo[a] = (int)floorf((pa - r - c->lo[a]) / c->cell);
+            hi[a] = (int)floorf((pa + r - c->lo[a]) / c->cell);
+            if (lo[a] < 0) lo[a] = 0;
+            if (hi[a] > c->dim[a] - 1) hi[a] = c->dim[a] - 1;
+        }
+        for (int gz = lo[2]; gz <= hi[2]; gz++)
+            for (int gy = lo[1]; gy <= hi[1]; gy++)
+                for (int gx = lo[0]; gx <= hi[0]; gx++) {
+                    const int cidx = (gz * c->dim[1] + gy) * c->dim[0] + gx;
+                    for (int k = c->cstart[cidx]; k < c->cstart[cidx + 1]; k++) {
+                        const int it = c->items[k];
+                        const int pi = it / maxv, vi = it % maxv;
+                        const bvert *pv = &c->ph[(size_t)pi * maxv + vi];
+                        const v3 d = vsub(pv->fr.p, vE->fr.p);
+                        if (!(vdot(d, d) < r * r)) continue;
+                        const int n = vi + t;
+                        if (n < pp->min_num_vertices || maxv < n) continue;
+                        /* MergeSubpaths(L, E, vi, t) */
+                        const bvert *Lp = &c->ph[(size_t)pi * maxv];
+                        int m = 0;
+                        for (int i = 0; i < vi; i++) x[m++] = Lp[i];
+                        for (int i = t - 1; i >= 0; i--) x[m++] = E[i];
+                        float f[3];
+                        vcm_f(b, x, n, vi, 1, f);
+                        if (f[0] == 0.0f && f[1] == 0.0f && f[2] == 0.0f) continue;
+                        const double ps = vcm_pdf(b, x, n, vi, 1, r);
+                        if (ps == 0.0) continue;
+                        const double w = vcm_weight(b, x, n, vi, 1, r, pp->num_photons, bdpm);
+                        float *o = &out[8 * (size_t)(2 * (t * M1 + vi + 1) + 1)];
+                        o[0] = px; o[1] = py;
+                        for (int q = 0; q < 3; q++) o[2 + q] += (float)((double)f[q] * w / ps);
+                        o[5] += 1.0f; o[6] += (float)w;
+                    }
+                }
+    }
+}
+
+typedef struct { vcm_ctx *c; uint64_t b, e; } vcm_job;
+
+static void *vcm_run(void *p)
+{
+    vcm_job *j = (vcm_job *)p;
+    for (uint64_t i = j->b; i < j->e; i++)
+        vcm_sample(j->c, j->c->first + i, &j->c->out[(size_t)8 * j->c->stride * i]);
+    return NULL;
+}
+
+void orc_vcm_samples(const orc_scene *s, const orc_vcm_params *pp, const float *table, uint32_t table_width,
+                     const float *ptable, uint32_t ptable_width, uint64_t first_sample, uint64_t n, float *out,
+                     int nthreads)
+{
+    bctx b;
+    b.s = s;
+    camera_init(&s->d, &b.cam);
+    b.tri_light = (int *)malloc(sizeof(int) * (s->d.ntri ? s->d.ntri : 1));
+    for (uint32_t i = 0; i < s->d.ntri; i++) b.tri_light[i] = -1;
+    for (uint32_t l = 0; l < s->d.nlight; l++)
+        for (uint32_t k = 0; k < s->d.light_prims[2 * l + 1]; k++) b.tri_light[s->d.light_prims[2 * l] + k] = (int)l;
+    b.tri_cdf_base = NULL;
+    const int maxv = pp->max_num_vertices;
+    /* photon subpaths */
+    const uint32_t np = pp->num_photons;
+    bvert *ph = (bvert *)calloc((size_t)np * maxv + 1, sizeof(bvert));
+    int *ph_n = (int *)calloc(np + 1, sizeof(int));
+    usrc pu;
+    pu.table = ptable; pu.width = ptable_width;
+    pu.seed = ((uint64_t)pp->pseed_hi << 32) | pp->pseed_lo;
+    pu.first = 0;
+    orc_bdpt_params bp = {pp->max_num_vertices, pp->min_num_vertices, pp->pseed_lo, pp->pseed_hi, pp->raster_mode};
+    const uint32_t wsub = ORC_PT_SEG0 + 4u * (uint32_t)(maxv - 1);
+    float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
+    int count = 0;
+    for (uint32_t i = 0; i < np; i++) {
+        bvert tmp[64];
+        ph_n[i] = trace_subpath(&b, &bp, &pu, i, wsub, 1, tmp);
+        for (int j = 0; j < ph_n[i]; j++) {
+            ph[(size_t)i * maxv + j] = tmp[j];
+            if (j >= 1) {
+                const float q[3] = {tmp[j].fr.p.x, tmp[j].fr.p.y, tmp[j].fr.p.z};
+                for (int a = 0; a < 3; a++) { if (q[a] < lo[a]) lo[a] = q[a]; if (q[a] > hi[a]) hi[a] = q[a]; }
+                count++;
+            }
+        }
+    }
+    vcm_ctx c;
+    c.b = &b; c.pp = pp; c.ph = ph; c.ph_n = ph_n;
+    c.cell = pp->radius;
+    long long cells = 1;
+    for (int a = 0; a < 3; a++) {
+        if (count == 0) { lo[a] = 0.0f; hi[a] = 0.0f; }
+        c.lo[a] = lo[a];
+        c.dim[a] = (int)floorf((hi[a] - lo[a]) / c.cell) + 1;
+        if (c.dim[a] > 512) { c.dim[a] = 512; }
+        cells *= c.dim[a];
+    }
+    /* cells beyond 512 per axis: widen the cell (the set found is the same) */
+    float span = 0.0f;
+    for (int a = 0; a < 3; a++) if (hi[a] - lo[a] > span) span = hi[a] - lo[a];
+    if (span / c.cell > 511.0f) {
+        c.cell = span / 511.0f;
+        cells = 1;
+        for (int a = 0; a < 3; a++) { c.dim[a] = (int)floorf((hi[a] - lo[a]) / c.cell) + 1; cells *= c.dim[a]; }
+    }
+    c.cstart = (int *)calloc((size_t)cells + 1, sizeof(int));
+    c.items = (int *)malloc(sizeof(int) * (size_t)(count + 1));
+    int *cellof = (int *)malloc(sizeof(int) * (size_t)((size_t)np * maxv + 1));
+    for (uint32_t i = 0; i < np; i++)
+        for (int j = 1; j < ph_n[i]; j++) {
+            const bvert *v = &ph[(size_t)i * maxv + j];
+            const float q[3] = {v->fr.p.x, v->fr.p.y, v->fr.p.z};
+            int g[3];
+            for (int a = 0; a < 3; a++) {
+                g[a] = (int)floorf((q[a] - c.lo[a]) / c.cell);
+                if (g[a] < 0) g[a] = 0;
+                if (g[a] > c.dim[a] - 1) g[a] = c.dim[a] - 1;
+            }
+            const int ci = (g[2] * c.dim[1] + g[1]) * c.dim[0] + g[0];
+            cellof[(size_t)i * maxv + j] = ci;
+            c.cstart[ci + 1]++;
+        }
+    for (long long k = 0; k < cells; k++) c.cstart[k + 1] += c.cstart[k];
+    int *fill = (int *)calloc((size_t)cells, sizeof(int));
+    for (uint32_t i = 0; i < np; i++)
+        for (int j = 1; j < ph_n[i]; j++) {
+            const int ci = cellof[(size_t)i * maxv + j];
+            c.items[c.cstart[ci] + fill[ci]++] = (int)i * maxv + j;
+        }
+    free(fill);
+    free(cellof);
+    c.us.table = table; c.us.width = table_width;
+    c.us.seed = ((uint64_t)pp->seed_hi << 32) | pp->seed_lo;
+    c.us.first = first_sample;
+    c.out = out;
+    c.stride = 2 * (maxv + 1) * (maxv + 1);
+    c.first = first_sample; c.n = n;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256];
+    vcm_job jobs[256];
+    for (int t = 0; t < nthreads; t++) {
+        jobs[t].c = &c;
+        jobs[t].b = n * (uint64_t)t / (uint64_t)nthreads;
+        jobs[t].e = n * (uint64_t)(t + 1) / (uint64_t)nthreads;
+        pthread_create(&th[t], NULL, vcm_run, &jobs[t]);
+    }
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    free(c.cstart); free(c.items); free(ph); free(ph_n); free(b.tri_light);
+}
+
+/* ---------------------------------------------------------------------------
  * renderer::bdpt (renderer_bdpt.cpp:50-1002), restated literally: per sample
  * an eye and a light subpath (Subpath::Sample, :80-220) whose vertices hold a
  * BSDF-sampled vertex `sv` and a `direct` vertex sampled on an emitter of the
