@@ -258,6 +258,42 @@ int lmx_render_vcm_bdpt(lmx_scene *scene, const lmx_render_params *params, float
 int lmx_render_bdpt(lmx_scene *scene, const lmx_render_params *params, float *film_rgb, void *stream,
                     lmx_render_stats *stats);
 
+/* renderer::vcm in modes vcm and bdpm (renderer_vcm.cpp:580-770, parameters
+ * :560-575): num_iteration_pass passes; pass q traces
+ * num_photon_trace_samples photon (light) subpaths, gathers their vertices
+ * past the light into a hash grid, then params->num_samples eye samples
+ * (num_eye_trace_samples), each with (mode vcm) every connection of
+ * lmx_render_vcm_bdpt and, at every eye vertex past the pinhole, a merge with
+ * every photon vertex within radius r_q (r_0 = initial_radius,
+ * r_q = sqrt((alpha + q) / (1 + q)) r_{q-1}); power-heuristic MIS over
+ * connections and merges (merges counted num_photon_trace_samples times),
+ * merges alone in mode bdpm.  film_rgb is the mean of the passes' images.
+ * Eye sample i of pass q uses uniform index q num_samples + i of `seed`,
+ * photon i index q num_photon_trace_samples + i of photon_seed, each laid
+ * out as lmx_render_vcm_bdpt's sample (the photon uses the light block).
+ * With uniform tables, photon_table (num_photon_trace_samples rows of
+ * photon_table_width >= 2 (8 + 4 (max_num_vertices - 1)) floats) or
+ * sample_out, num_iteration_pass must be 1.  sample_out holds
+ * 2 (M + 1)^2 records of 8 floats per eye sample at
+ * 2 (t (M + 1) + s) + merge; a merge of eye vertex t - 1 with photon
+ * vertex j is recorded at s = j + 1 (its path is that of the connection
+ * (j, t)), summed over the photons: raster x, y, C rgb, count, sum of
+ * weights, 0.  The render zeroes sample_out first. */
+#define LMX_VCM_MODE_VCM 0
+#define LMX_VCM_MODE_BDPM 2
+typedef struct lmx_vcm_params {
+    int32_t mode;                        /* LMX_VCM_MODE_VCM (reference default) or LMX_VCM_MODE_BDPM */
+    uint32_t num_iteration_pass;         /* reference default 100 */
+    uint32_t num_photon_trace_samples;   /* 1 .. 2^26 (reference default 10000) */
+    float initial_radius;                /* > 0 (reference default 0.1) */
+    float alpha;                         /* radius reduction, > 0 (reference default 0.7) */
+    uint64_t photon_seed;
+    const float *photon_table;           /* device, or NULL (Philox with photon_seed) */
+    uint32_t photon_table_width;
+} lmx_vcm_params;
+int lmx_render_vcm(lmx_scene *scene, const lmx_render_params *params, const lmx_vcm_params *vcm, float *film_rgb,
+                   void *stream, lmx_render_stats *stats);
+
 /* ---------------------------------------------------------------------------
  * Learned directional sampler.  No counterpart in the reference: a third
  * renderer beside renderer::pt and renderer::vcm, for the callers of

@@ -13,7 +13,23 @@
 //           The light subpath's direct vertices (a point on the sensor) are
 //           not sampled: with a pinhole, whose position is a delta, their
 //           strategies carry nothing (EvaluateCst evaluates the sensor's
-//           position with evalDelta and gets 0, :389-393).
+//           position with evalDelta and gets 0, :389-393);
+//   vcm merging -- renderer::vcm in modes vcm and bdpm (renderer_vcm.cpp:
+//           580-770): per pass, N photon subpaths (light subpaths of their
+//           own uniform source) traced by the same gen/trace/shade kernels,
+//           a hash grid over their vertices past the light, then the eye
+//           batches with (mode vcm) every connection and, at every eye
+//           vertex past the pinhole, a merge with every photon vertex within
+//           the pass's radius.  A merge of photon vertex j with eye vertex
+//           t-1 makes the path of the connection (s = j, t) between photon
+//           vertex j-1 and eye vertex t-1, so it is evaluated as that
+//           connection without a shadow ray: its pdf is the connection's
+//           times pf_L(eye vertex) pi r^2, and every strategy's term of the
+//           power-heuristic denominator (connection s' and merge at x_s')
+//           folds into mu_s' = c_conn + N (pi r^2 pf_L(x_s'))^2 times the
+//           connection ratio, c_conn 1 (vcm) or 0 (bdpm).  The O(1) ratio
+//           chains below carry mu instead of 1; with N = 0 they are the
+//           bdpt-mode chains.
 //
 // Per batch of B samples:
 //   gen      -- the eye and light endpoints of every sample and their first
@@ -34,6 +50,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -63,6 +80,10 @@ namespace {
 // side (the camera vertex is excluded: a pinhole is never hit).  It obeys
 // S_j = (pr_j / pf_j)^2 (1 + S_{j-1}), so every connection's MIS weight
 // takes O(1) work; U < 0 marks a subpath with a zero pf in that range.
+// With vertex merging the 1 becomes mu of the vertex where that strategy's
+// merge would happen: on the light side S_j = rho_j^2 (mu(x_j) + S_{j-1}),
+// on the eye side S_j = rho_j^2 (mu(x_{j-1}) + S_{j-1}), mu of the camera and
+// of the light endpoint being c_conn (neither can be merged at).
 // With the simple weight the square becomes the indicator (ratio != 0), and
 // the same recursion counts the strategies with a nonzero pdf.
 // The bdpt variant also keeps, per light vertex k >= 2, V_k = prod over
@@ -78,6 +99,10 @@ namespace {
 
 struct BParams {
     int32_t variant;           // BD_VCM or BD_BDPT
+    int32_t merging;           // vcm/bdpm: merge records, photon grid, merge kernel
+    int32_t connect;           // connections and s = 0 strategies evaluated (all but bdpm)
+    int32_t phase;             // merging: 0 eye samples, 1 photon subpaths
+    float c_conn;              // mu's connection term: 1, or 0 (bdpm)
     int32_t mis;               // LMX_MIS_POWER or LMX_MIS_SIMPLE (bdpt)
     int32_t bounded;           // bdpt: max_num_vertices given (else capped at maxv)
     int32_t maxv, minv;
@@ -156,21 +181,34 @@ RD void store_pr(const VB& vb, const BParams& p, int kind, uint32_t i, uint32_t 
     r[3] = pr;
 }
 
+RD float load_pr(const VB& vb, const BParams& p, int kind, uint32_t i, uint32_t k)
+{
+    return vb.v[kind][((size_t)i * (size_t)p.maxv + k) * 4 + 2].w;
+}
+
 RD float gterm(const Vtx& a, const Vtx& b) { return gterm(a.p, a.sn, a.degenerate, b.p, b.sn, b.degenerate); }
 
 // the weight heuristic's transform of one pdf ratio: its square (power
 // heuristic) or whether it is nonzero (simple weight)
 RD float sqm(float x, int mis) { return mis == LMX_MIS_SIMPLE ? (x != 0.0f ? 1.0f : 0.0f) : x * x; }
 
-// a^2 (1 + b^2 (1 + U)): the sum over one side of the squared pdf ratios of
-// the strategies that move the junction 1, 2, 3, ... vertices into it; a
-// zero factor ends the chain (the later ratios contain it).
-RD float chain_sq(float a, float b, float U, int mis)
+// mu of a path vertex: c_conn, plus N (pi r^2 pf_L)^2 where a merge can
+// happen there
+RD float mu_of(float c_conn, float c_merge, bool mergeable, float pfL)
+{
+    return (mergeable && c_merge != 0.0f) ? c_conn + c_merge * (pfL * pfL) : c_conn;
+}
+
+// a^2 (m1 + b^2 (m2 + U)): the sum over one side of the squared pdf ratios
+// of the strategies that move the junction 1, 2, 3, ... vertices into it,
+// each times mu of its merge vertex (m1, m2 for the first two, the rest
+// inside U); a zero factor ends the chain (the later ratios contain it).
+RD float chain_mu(float a, float b, float m1, float m2, float U, int mis)
 {
     if (a == 0.0f) return 0.0f;
     const float a2 = sqm(a, mis);
-    if (b == 0.0f) return a2;
-    return a2 * (1.0f + sqm(b, mis) * (1.0f + U));
+    if (b == 0.0f) return a2 * m1;
+    return a2 * (m1 + sqm(b, mis) * (m2 + U));
 }
 
 // uniform slots of the bdpt variant (within a subpath block): step j >= 1
@@ -183,41 +221,45 @@ __global__ __launch_bounds__(256) void k_bd_gen(RScene s, BParams p, BCtl* __res
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t sample = ctl->batch_first + i;
-    const bool valid = i < p.batch && sample < p.total;
+    const bool valid = i < p.batch && sample - ctl->pass_first < p.total;
+    const bool eye = p.phase == 0;             // the photon phase traces light subpaths only
+    const bool lsub = p.connect != 0 || p.phase == 1;   // bdpm's eye samples use no light subpath
     bool eye_ray = false, light_ray = false;
     f3 ewo = mk3(0, 0, 0), lwo = mk3(0, 0, 0), lp = mk3(0, 0, 0), la = mk3(0, 0, 0);
     float epdf = 0.0f, lpdf = 0.0f;
     if (valid) {
         const Uniforms<BParams> U{p, sample};
         // eye endpoint (pinhole; We = 1, pdf 1)
-        const float ux = U.one(4), uy = U.one(5);
-        uint32_t pix;
-        float rx, ry;
-        bool alive = true;
-        if (p.raster_mode == LMX_RASTER_STRATIFIED) {
-            const uint32_t npix = (uint32_t)s.width * (uint32_t)s.height;
-            pix = sample % npix;
-            const uint32_t px = pix % (uint32_t)s.width, py = pix / (uint32_t)s.width;
-            ewo = camera_dir(s, ((float)px + ux) / (float)s.width, ((float)py + uy) / (float)s.height);
-            rx = ((float)px + 0.5f) / (float)s.width;
-            ry = ((float)py + 0.5f) / (float)s.height;
-        } else {
-            ewo = camera_dir(s, ux, uy);
-            alive = camera_raster(s, ewo, rx, ry);
-            if (!alive) { rx = ry = 0.0f; }
-            pix = raster_pixel(s, rx, ry);
+        if (eye) {
+            const float ux = U.one(4), uy = U.one(5);
+            uint32_t pix;
+            float rx, ry;
+            bool alive = true;
+            if (p.raster_mode == LMX_RASTER_STRATIFIED) {
+                const uint32_t npix = (uint32_t)s.width * (uint32_t)s.height;
+                pix = sample % npix;
+                const uint32_t px = pix % (uint32_t)s.width, py = pix / (uint32_t)s.width;
+                ewo = camera_dir(s, ((float)px + ux) / (float)s.width, ((float)py + uy) / (float)s.height);
+                rx = ((float)px + 0.5f) / (float)s.width;
+                ry = ((float)py + 0.5f) / (float)s.height;
+            } else {
+                ewo = camera_dir(s, ux, uy);
+                alive = camera_raster(s, ewo, rx, ry);
+                if (!alive) { rx = ry = 0.0f; }
+                pix = raster_pixel(s, rx, ry);
+            }
+            vb.epix[i] = pix;
+            vb.eraster[i] = make_float2(rx, ry);
+            store_v(vb, p, 0, i, 0, s.cam_pos, vflags(VK_CAMERA, true, -1, 0u), mk3(0, 0, 0), 1.0f, mk3(0, 0, 0), 0.0f,
+                    mk3(1, 1, 1), 0.0f);
+            if (alive) {
+                epdf = importance(s, ewo);
+                eye_ray = epdf > 0.0f;
+            }
+            vb.nsub[0][i] = 1;
         }
-        vb.epix[i] = pix;
-        vb.eraster[i] = make_float2(rx, ry);
-        store_v(vb, p, 0, i, 0, s.cam_pos, vflags(VK_CAMERA, true, -1, 0u), mk3(0, 0, 0), 1.0f, mk3(0, 0, 0), 0.0f,
-                mk3(1, 1, 1), 0.0f);
-        if (alive) {
-            epdf = importance(s, ewo);
-            eye_ray = epdf > 0.0f;
-        }
-        vb.nsub[0][i] = 1;
         // light endpoint: primitive from uEP_C, point from uEP, direction from uED
-        if (s.nlight > 0) {
+        if (s.nlight > 0 && lsub) {
             const uint32_t b = p.wsub;
             const float e0 = U.one(b + 1), e1 = U.one(b + 2), ec = U.one(b + 3);
             const float d0 = U.one(b + 4), d1 = U.one(b + 5);
@@ -245,7 +287,7 @@ __global__ __launch_bounds__(256) void k_bd_gen(RScene s, BParams p, BCtl* __res
             vb.nsub[1][i] = 0;
         }
     } else if (i < p.batch) {
-        vb.nsub[0][i] = 0;
+        if (eye) vb.nsub[0][i] = 0;
         vb.nsub[1][i] = 0;
     }
     __shared__ uint32_t lds[5];
@@ -310,12 +352,21 @@ __global__ __launch_bounds__(256) void k_bd_shade(RScene s, BParams p, BCtl* __r
                     pf = bsdf_pdf(pm, prev.fr, dpp, dpn) * G;
                     const float pr2 = bsdf_pdf(pm, prev.fr, dpn, dpp) * gterm(prev, prev2);
                     store_pr(vb, p, kind, i, k - 2, pr2);
-                    // U_k = S_{k-2}; on the eye side S_0 (the camera) is excluded
+                    // U_k = S_{k-2}; on the eye side S_0 (the camera) is excluded.
+                    // mu of the strategy's merge vertex: light side x_{k-2}
+                    // (the endpoint for k = 2), eye side x_{k-3} (the camera
+                    // for k = 3)
                     const bool bad = prev2.pf == 0.0f || prev.U < 0.0f;
                     const float rho = bad ? 0.0f : pr2 / prev2.pf;
+                    const float cm = p.merging ? ctl->c_merge : 0.0f;
+                    float mu = p.c_conn;
+                    if (cm != 0.0f) {
+                        if (kind == 1) mu = mu_of(p.c_conn, cm, k >= 3, prev2.pf);
+                        else if (k >= 4) mu = mu_of(p.c_conn, cm, true, load_pr(vb, p, 0, i, k - 3));
+                    }
                     if (bad && (k > 2 || kind == 1)) U = -1.0f;
-                    else if (k == 2) U = kind == 1 ? sqm(rho, p.mis) : 0.0f;
-                    else U = rho == 0.0f ? 0.0f : sqm(rho, p.mis) * (1.0f + prev.U);
+                    else if (k == 2) U = kind == 1 ? sqm(rho, p.mis) * p.c_conn : 0.0f;
+                    else U = rho == 0.0f ? 0.0f : sqm(rho, p.mis) * (mu + prev.U);
                     // V_k = V_{k-1} (pr_{k-2} / pf_{k-2})^2, V_2 = 1
                     if (p.variant == BD_BDPT && kind == 1) {
                         float* Vi = vb.V + (size_t)i * (size_t)p.maxv;
@@ -379,18 +430,34 @@ struct ShadowQ {
     uint4* meta;               // sample, s, t, pixel
 };
 
-// sample_out record of strategy (s, t, direct): vcm (M+1)^2 records at
-// t (M+1) + s, bdpt 2 (M+1)^2 at 2 (t (M+1) + s) + direct
+// sample_out record of strategy (s, t, direct or merge): vcm (M+1)^2
+// records at t (M+1) + s; bdpt and vcm merging 2 (M+1)^2 at
+// 2 (t (M+1) + s) + direct / merge
+RD float4* record_at(const BParams& p, uint32_t sample, int s, int t, int flag)
+{
+    const size_t M1 = (size_t)(p.maxv + 1);
+    const bool two = p.variant == BD_BDPT || p.merging;
+    const size_t idx = two ? 2 * ((size_t)t * M1 + (size_t)s) + (size_t)flag : (size_t)t * M1 + (size_t)s;
+    const size_t per = two ? 2 * M1 * M1 : M1 * M1;
+    return reinterpret_cast<float4*>(p.sample_out + 8 * ((size_t)sample * per + idx));
+}
+
 RD void record(const BParams& p, uint32_t sample, int s, int t, int direct, float rx, float ry, f3 C, float w)
 {
     if (!p.sample_out) return;
-    const size_t M1 = (size_t)(p.maxv + 1);
-    const size_t idx = p.variant == BD_BDPT ? 2 * ((size_t)t * M1 + (size_t)s) + (size_t)direct
-                                            : (size_t)t * M1 + (size_t)s;
-    const size_t per = p.variant == BD_BDPT ? 2 * M1 * M1 : M1 * M1;
-    float4* o = reinterpret_cast<float4*>(p.sample_out + 8 * ((size_t)sample * per + idx));
+    float4* o = record_at(p, sample, s, t, direct);
     o[0] = make_float4(rx, ry, C.x, C.y);
     o[1] = make_float4(C.z, 1.0f, w, 0.0f);
+}
+
+// merges of one (sample, t, s) add up: C, count and weight sums
+RD void record_merge(const BParams& p, uint32_t sample, int s, int t, float rx, float ry, f3 C, float w)
+{
+    if (!p.sample_out) return;
+    float4* o = record_at(p, sample, s, t, 1);
+    const float4 a = o[0], b = o[1];
+    o[0] = make_float4(rx, ry, a.z + C.x, a.w + C.y);
+    o[1] = make_float4(b.x + C.z, b.y + 1.0f, b.z + w, 0.0f);
 }
 
 RD void splat(const BParams& p, uint32_t pixel, f3 C)
@@ -416,6 +483,86 @@ RD bool eye_direct(const RScene& s, const BParams& p, const VB& vb, uint32_t i, 
     return is_scatterer(s.mats[__float_as_uint(a.w) >> 16]);
 }
 
+// One junction of light vertex vL (index sv - 1 of its subpath, vLp the one
+// before) with eye vertex vE (index t - 1, vEp the one before): the
+// unweighted contribution alpha_L f_L G f_E alpha_E, the area pdf pLs of vE
+// from the light side, and the power-heuristic denominator over every
+// strategy of the path relative to this connection, the two nearest vertices
+// of each side from the junction pdfs and the rest from the running sums U.
+// With merging (cm = N (pi r^2)^2 != 0) every term carries mu of its merge
+// vertex; prE3 is the stored pr of eye vertex t - 3 (its pdf from the light
+// side), used when t >= 4.
+struct Junction {
+    f3 Cs;
+    float pLs;
+    float aL, bL;      // the light side's ratios at vertices sv - 1, sv - 2
+    float sum;
+    bool ok;           // every pdf the sum divides by is nonzero
+};
+
+RD Junction junction(const RScene& s, const BParams& p, float cm, const Vtx& vL, const Vtx& vLp, int sv,
+                     const Vtx& vE, const Vtx& vEp, int t, float prE3)
+{
+    Junction J;
+    const f3 dLE = dir_to(vL.p, vE.p);
+    const f3 dEL = dir_to(vE.p, vL.p);
+    const float G = gterm(vL, vE);
+    f3 fsL = mk3(0, 0, 0), fsE = mk3(0, 0, 0);
+    float pLs = 0.0f, pEsm1 = 0.0f, pEsm2 = 0.0f, pLsp1 = 0.0f;
+    // light side of the connection
+    if (sv == 1) {
+        if (emits_towards(vL.fr, dLE)) {
+            const RMat m = s.mats[vL.mat];
+            fsL = mk3(m.Le[0], m.Le[1], m.Le[2]);
+            pLs = LMX_INV_PI_F * G;
+        }
+    } else {
+        const RMat m = s.mats[vL.mat];
+        if (is_scatterer(m)) {
+            const f3 wiL = dir_to(vL.p, vLp.p);
+            fsL = bsdf_eval_t(m, vL.fr, wiL, dLE, true);
+            pLs = bsdf_pdf(m, vL.fr, wiL, dLE) * G;
+            pEsm2 = bsdf_pdf(m, vL.fr, dLE, wiL) * gterm(vL, vLp);
+        }
+    }
+    // eye side
+    if (t == 1) {
+        const float imp = importance(s, dEL);
+        fsE = mk3(imp, imp, imp);
+        pEsm1 = imp * G;
+    } else {
+        const RMat m = s.mats[vE.mat];
+        if (is_scatterer(m)) {
+            const f3 wiE = dir_to(vE.p, vEp.p);
+            fsE = bsdf_eval_t(m, vE.fr, wiE, dEL, false);
+            pEsm1 = bsdf_pdf(m, vE.fr, wiE, dEL) * G;
+            pLsp1 = bsdf_pdf(m, vE.fr, dEL, wiE) * gterm(vE, vEp);
+        }
+    }
+    J.Cs = mk3(vL.alpha.x * fsL.x * G * fsE.x * vE.alpha.x, vL.alpha.y * fsL.y * G * fsE.y * vE.alpha.y,
+               vL.alpha.z * fsL.z * G * fsE.z * vE.alpha.z);
+    J.pLs = pLs;
+    J.ok = true;
+    if (vL.pf == 0.0f || (sv >= 2 && (vLp.pf == 0.0f || vL.U < 0.0f))) J.ok = false;
+    if (t >= 2 && (vE.pf == 0.0f || (t >= 3 && (vEp.pf == 0.0f || vE.U < 0.0f)))) J.ok = false;
+    J.aL = J.bL = 0.0f;
+    J.sum = p.c_conn;
+    if (J.ok) {
+        J.aL = pEsm1 / vL.pf;
+        J.bL = sv >= 2 ? pEsm2 / vLp.pf : 0.0f;
+        // this strategy's own term: mu of vE (no merge at the pinhole)
+        J.sum = mu_of(p.c_conn, cm, t >= 2, pLs);
+        // light side: mu of light vertices sv - 1, sv - 2 (the endpoint is no merge)
+        J.sum += chain_mu(J.aL, J.bL, mu_of(p.c_conn, cm, sv >= 2, vL.pf),
+                          mu_of(p.c_conn, cm, sv >= 3, sv >= 3 ? vLp.pf : 0.0f), vL.U, p.mis);
+        // eye side: mu of eye vertices t - 2, t - 3 (the camera is no merge)
+        if (t >= 2)
+            J.sum += chain_mu(pLs / vE.pf, t >= 3 ? pLsp1 / vEp.pf : 0.0f, mu_of(p.c_conn, cm, t >= 3, pLsp1),
+                              mu_of(p.c_conn, cm, t >= 4, prE3), vE.U, p.mis);
+    }
+    return J;
+}
+
 // Connection plan: one thread per sample.  Strategies s = 0 (the eye
 // subpath hit a light) are evaluated and splatted here; every structurally
 // possible s >= 1 gets a shadow-queue slot, and so does (bdpt) every direct
@@ -426,7 +573,7 @@ __global__ __launch_bounds__(256) void k_bd_plan(RScene s, BParams p, BCtl* __re
     __shared__ uint32_t lds[5];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t sample = ctl->batch_first + i;
-    const bool valid = i < p.batch && sample < p.total;
+    const bool valid = i < p.batch && sample - ctl->pass_first < p.total;
     const int nE = valid ? (int)vb.nsub[0][i] : 0;
     const int nL = valid ? (int)vb.nsub[1][i] : 0;
     const int minv = min_len(p);
@@ -453,9 +600,10 @@ __global__ __launch_bounds__(256) void k_bd_plan(RScene s, BParams p, BCtl* __re
     if (ndirect)
         for (int k = 0; k < nE; k++)
             if (eye_direct(s, p, vb, i, sample, k)) sq.meta[slot++] = make_uint4(i, 1u, (uint32_t)k + 1u, 1u);
-    if (!valid) return;
+    if (!valid || !p.connect) return;
     const uint32_t pixel = vb.epix[i];
     const float2 eras = vb.eraster[i];
+    const float cm = p.merging ? ctl->c_merge : 0.0f;
     for (int t = 2; t <= nE; t++) {
         if (t < minv) continue;                        // s = 0 excluded by min_num_vertices
         const Vtx vE = load_v(vb, p, 0, i, (uint32_t)t - 1);
@@ -471,7 +619,12 @@ __global__ __launch_bounds__(256) void k_bd_plan(RScene s, BParams p, BCtl* __re
         const bool ok = vE.pf > 0.0f && (t < 3 || (vEp.pf > 0.0f && vE.U >= 0.0f));
         if (!ok) continue;
         const float a = pos_pdf_light(s, vE.light) / vE.pf;
-        float sum = 1.0f + chain_sq(a, t >= 3 ? LMX_INV_PI_F * gterm(vE, vEp) / vEp.pf : 0.0f, vE.U, p.mis);
+        // with merging: mu of eye vertices t-2 (from the light's emission
+        // pdf) and t-3 (its stored pr); the light point itself is no merge
+        const float pL1 = t >= 3 ? LMX_INV_PI_F * gterm(vE, vEp) : 0.0f;
+        const float m1 = mu_of(p.c_conn, cm, t >= 3, pL1);
+        const float m2 = mu_of(p.c_conn, cm, t >= 4, t >= 4 && cm != 0.0f ? load_pr(vb, p, 0, i, t - 3) : 0.0f);
+        float sum = p.c_conn + chain_mu(a, t >= 3 ? pL1 / vEp.pf : 0.0f, m1, m2, vE.U, p.mis);
         if (bdpt) sum += sqm(a, p.mis);
         const float w = 1.0f / sum;
         const f3 C = mk3(Cs.x * w, Cs.y * w, Cs.z * w);
@@ -526,70 +679,27 @@ __global__ __launch_bounds__(256) void k_bd_connect(RScene s, BParams p, BCtl* _
         }
         Vtx vLp;
         if (sv >= 2) vLp = load_v(vb, p, 1, i, (uint32_t)sv - 2);
-        const f3 dLE = dir_to(vL.p, vE.p);
-        const f3 dEL = dir_to(vE.p, vL.p);
-        const float G = gterm(vL, vE);
-        f3 fsL = mk3(0, 0, 0), fsE = mk3(0, 0, 0);
-        float pLs = 0.0f, pEsm1 = 0.0f, pEsm2 = 0.0f, pLsp1 = 0.0f;
-        // light side of the connection
-        if (sv == 1) {
-            if (emits_towards(vL.fr, dLE)) {
-                const RMat m = s.mats[vL.mat];
-                fsL = mk3(m.Le[0], m.Le[1], m.Le[2]);
-                pLs = LMX_INV_PI_F * G;
-            }
-        } else {
-            const RMat m = s.mats[vL.mat];
-            if (is_scatterer(m)) {
-                const f3 wiL = dir_to(vL.p, vLp.p);
-                fsL = bsdf_eval_t(m, vL.fr, wiL, dLE, true);
-                pLs = bsdf_pdf(m, vL.fr, wiL, dLE) * G;
-                pEsm2 = bsdf_pdf(m, vL.fr, dLE, wiL) * gterm(vL, vLp);
-            }
-        }
-        // eye side
-        if (t == 1) {
-            const float imp = importance(s, dEL);
-            fsE = mk3(imp, imp, imp);
-            pEsm1 = imp * G;
-        } else {
-            const RMat m = s.mats[vE.mat];
-            if (is_scatterer(m)) {
-                const f3 wiE = dir_to(vE.p, vEp.p);
-                fsE = bsdf_eval_t(m, vE.fr, wiE, dEL, false);
-                pEsm1 = bsdf_pdf(m, vE.fr, wiE, dEL) * G;
-                pLsp1 = bsdf_pdf(m, vE.fr, dEL, wiE) * gterm(vE, vEp);
-            }
-        }
-        const f3 Cs = mk3(vL.alpha.x * fsL.x * G * fsE.x * vE.alpha.x,
-                          vL.alpha.y * fsL.y * G * fsE.y * vE.alpha.y,
-                          vL.alpha.z * fsL.z * G * fsE.z * vE.alpha.z);
+        const float cm = p.merging ? ctl->c_merge : 0.0f;
+        const float prE3 = (t >= 4 && cm != 0.0f) ? load_pr(vb, p, 0, i, (uint32_t)t - 3) : 0.0f;
+        const Junction J = junction(s, p, cm, vL, vLp, sv, vE, vEp, t, prE3);
+        const f3 Cs = J.Cs;
         bool ok = !(Cs.x == 0.0f && Cs.y == 0.0f && Cs.z == 0.0f);
         if (ok && t == 1) {
-            ok = camera_raster(s, dEL, rx, ry);
+            ok = camera_raster(s, dir_to(vE.p, vL.p), rx, ry);
             pixel = raster_pixel(s, rx, ry);
         } else {
             rx = eras.x; ry = eras.y;
         }
-        // power heuristic from the area pdfs of both sides: the two
-        // nearest vertices of each side from the junction pdfs, the
-        // rest from the running sums U
-        if (vL.pf == 0.0f || (sv >= 2 && (vLp.pf == 0.0f || vL.U < 0.0f))) ok = false;
-        if (t >= 2 && (vE.pf == 0.0f || (t >= 3 && (vEp.pf == 0.0f || vE.U < 0.0f)))) ok = false;
-        float sum = 1.0f;
+        ok = ok && J.ok;
         if (ok) {
-            const float aL = pEsm1 / vL.pf, bL = sv >= 2 ? pEsm2 / vLp.pf : 0.0f;
-            sum += chain_sq(aL, bL, vL.U, p.mis);
-            if (t >= 2) sum += chain_sq(pLs / vE.pf, t >= 3 ? pLsp1 / vEp.pf : 0.0f, vE.U, p.mis);
+            float sum = J.sum;
             // bdpt: the direct strategy, whose pdf equals that of s = 1
             // (p_1 / p_s = the light side's ratios down to vertex 1)
             if (p.variant == BD_BDPT) {
                 if (sv == 1) sum += 1.0f;
-                else if (sv == 2) sum += sqm(aL, p.mis);
-                else sum += sqm(aL, p.mis) * sqm(bL, p.mis) * vb.V[(size_t)i * (size_t)p.maxv + (size_t)(sv - 1)];
+                else if (sv == 2) sum += sqm(J.aL, p.mis);
+                else sum += sqm(J.aL, p.mis) * sqm(J.bL, p.mis) * vb.V[(size_t)i * (size_t)p.maxv + (size_t)(sv - 1)];
             }
-        }
-        if (ok) {
             w = 1.0f / sum;
             // a direct strategy from eye vertex t - 1 >= 1 ran one roulette
             // more than that vertex's throughput holds
@@ -654,7 +764,208 @@ __global__ void k_bd_init(BCtl* ctl)
     if (threadIdx.x == 0) {
         ctl->n_trace = 0; ctl->n_next = 0; ctl->n_shadow = 0; ctl->batch_first = 0;
         ctl->rays = 0; ctl->shadow_rays = 0;
+        ctl->pass_first = 0; ctl->radius = 0.0f; ctl->cell = 0.0f; ctl->c_merge = 0.0f; ctl->merges = 0;
     }
+}
+
+// start of a vertex-merging phase: its first sample index, the pass's merge
+// radius and N (pi r^2)^2
+__global__ void k_bd_phase(BCtl* ctl, uint32_t first, float radius, float cell, float c_merge)
+{
+    if (threadIdx.x == 0) {
+        ctl->n_trace = 0; ctl->n_next = 0; ctl->n_shadow = 0;
+        ctl->batch_first = first; ctl->pass_first = first;
+        ctl->radius = radius; ctl->cell = cell; ctl->c_merge = c_merge;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Vertex merging.  A hash grid of cell edge 2r over the photon vertices past
+// the light: count per bucket, exclusive scan, scatter of (position, photon
+// * maxv + vertex) records.  A query at x visits the buckets of the (at most
+// 2 x 2 x 2) cells that [x - r, x + r] touches and keeps the records of those
+// very cells (two cells may share a bucket) within r of x: the set is that of
+// the reference's kd-tree range query (renderer_vcm.cpp:356-505).
+
+struct Grid {
+    float4* items;
+    uint32_t* start;           // H + 1 bucket starts after the scan
+    uint32_t* cursor;          // H fill counters
+    uint32_t* part;            // scan partial sums
+    uint32_t H;                // power of two
+};
+
+RD int cell_of(float x, float cell)
+{
+    const float c = floorf(x / cell);
+    return (int)fminf(fmaxf(c, -1.0e9f), 1.0e9f);
+}
+
+RD uint32_t bucket_of(int cx, int cy, int cz, uint32_t H)
+{
+    return (((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u)) & (H - 1u);
+}
+
+__global__ void k_vc_grid_clear(Grid g)
+{
+    for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h <= g.H; h += gridDim.x * blockDim.x) {
+        g.start[h] = 0u;
+        if (h < g.H) g.cursor[h] = 0u;
+    }
+}
+
+// one thread per (photon, vertex >= 1) slot; phase 0 counts, phase 1 scatters
+template <int PHASE>
+__global__ __launch_bounds__(256) void k_vc_grid_fill(BParams p, const BCtl* __restrict__ ctl, VB ph, Grid g)
+{
+    const uint32_t np = p.batch;
+    const size_t n = (size_t)np * (size_t)(p.maxv - 1);
+    const float cell = ctl->cell;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t i = (uint32_t)(idx % np), j = 1u + (uint32_t)(idx / np);
+        if (j >= ph.nsub[1][i]) continue;
+        const float4 a = ph.v[1][((size_t)i * (size_t)p.maxv + j) * 4];
+        const uint32_t h = bucket_of(cell_of(a.x, cell), cell_of(a.y, cell), cell_of(a.z, cell), g.H);
+        if (PHASE == 0) {
+            atomicAdd(&g.start[h], 1u);
+        } else {
+            const uint32_t slot = g.start[h] + atomicAdd(&g.cursor[h], 1u);
+            g.items[slot] = make_float4(a.x, a.y, a.z, __uint_as_float(i * (uint32_t)p.maxv + j));
+        }
+    }
+}
+
+// exclusive scan of a 256-thread block's values; `tot` gets the block total
+RD uint32_t block_scan(uint32_t v, uint32_t* lds, uint32_t& tot)
+{
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= (uint32_t)off) x += y;
+    }
+    if (lane == 63u) lds[w] = x;
+    __syncthreads();
+    uint32_t base = 0;
+    for (uint32_t q = 0; q < w; q++) base += lds[q];
+    tot = lds[0] + lds[1] + lds[2] + lds[3];
+    __syncthreads();
+    return base + x - v;
+}
+
+#define SCAN_PER 16u
+#define SCAN_TILE (256u * SCAN_PER)
+
+// tile-local exclusive scan of a[0, n), tile totals into part
+__global__ __launch_bounds__(256) void k_scan_tiles(uint32_t* a, uint32_t n, uint32_t* part)
+{
+    __shared__ uint32_t lds[4];
+    const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER;
+    uint32_t v[SCAN_PER], sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN_PER; k++) {
+        v[k] = base + k < n ? a[base + k] : 0u;
+        sum += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = block_scan(sum, lds, tot);
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN_PER; k++) {
+        if (base + k < n) a[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// exclusive scan of the tile totals (one workgroup; nt <= 256 * 64)
+__global__ __launch_bounds__(256) void k_scan_part(uint32_t* part, uint32_t nt)
+{
+    __shared__ uint32_t lds[4];
+    const uint32_t per = (nt + 255u) / 256u;
+    const uint32_t base = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < per; k++) sum += base + k < nt ? part[base + k] : 0u;
+    uint32_t tot;
+    uint32_t run = block_scan(sum, lds, tot);
+    for (uint32_t k = 0; k < per; k++)
+        if (base + k < nt) {
+            const uint32_t v = part[base + k];
+            part[base + k] = run;
+            run += v;
+        }
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(uint32_t* a, uint32_t n, const uint32_t* part)
+{
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+        a[j] += part[j / SCAN_TILE];
+}
+
+// Merges: one thread per (eye sample, eye vertex k = t - 1 >= 1).  Every
+// photon vertex j within r makes the path of the connection (s = j, t)
+// between photon vertex j - 1 and eye vertex t - 1, kept when its length is
+// within [min, max] vertices; its contribution f w / p is the connection's
+// unweighted term times pLs pi r^2 / D and its weight (pi r^2 pLs)^2 / D,
+// D the mu-weighted denominator (EvaluateF / EvaluatePathPDF with merging and
+// EvaluateMISWeight_VCM / _BDPM, renderer_vcm.cpp:116-340).
+__global__ __launch_bounds__(256) void k_vc_merge(RScene s, BParams p, BCtl* __restrict__ ctl, VB vb, VB ph,
+                                                  Grid g)
+{
+    const uint32_t nk = (uint32_t)p.maxv - 1u;
+    const size_t n = (size_t)p.batch * nk;
+    const float r = ctl->radius, cell = ctl->cell, cm = ctl->c_merge;
+    const float r2 = r * r, area = LMX_PI_F * r * r;
+    uint32_t nmerge = 0;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t i = (uint32_t)(idx % p.batch), k = 1u + (uint32_t)(idx / p.batch);
+        const uint32_t sample = ctl->batch_first + i;
+        if (sample - ctl->pass_first >= p.total) continue;
+        if (k >= vb.nsub[0][i]) continue;
+        const int t = (int)k + 1;
+        const Vtx vE = load_v(vb, p, 0, i, k);
+        const Vtx vEp = load_v(vb, p, 0, i, k - 1u);
+        const float prE3 = t >= 4 ? load_pr(vb, p, 0, i, (uint32_t)t - 3) : 0.0f;
+        const float2 eras = vb.eraster[i];
+        int lo[3], hi[3];
+        const float q[3] = {vE.p.x, vE.p.y, vE.p.z};
+#pragma unroll
+        for (int a = 0; a < 3; a++) { lo[a] = cell_of(q[a] - r, cell); hi[a] = cell_of(q[a] + r, cell); }
+        f3 Csum = mk3(0, 0, 0);
+        for (int cz = lo[2]; cz <= hi[2]; cz++)
+            for (int cy = lo[1]; cy <= hi[1]; cy++)
+                for (int cx = lo[0]; cx <= hi[0]; cx++) {
+                    const uint32_t h = bucket_of(cx, cy, cz, g.H);
+                    const uint32_t e = g.start[h + 1];
+                    for (uint32_t it = g.start[h]; it < e; it++) {
+                        const float4 c = g.items[it];
+                        if (cell_of(c.x, cell) != cx || cell_of(c.y, cell) != cy || cell_of(c.z, cell) != cz) continue;
+                        const f3 d = sub(mk3(c.x, c.y, c.z), vE.p);
+                        if (!(dot(d, d) < r2)) continue;
+                        const uint32_t id = __float_as_uint(c.w);
+                        const uint32_t pi = id / (uint32_t)p.maxv;
+                        const int j = (int)(id % (uint32_t)p.maxv);
+                        const int len = j + t;
+                        if (len < p.minv || len > p.maxv) continue;
+                        const Vtx vL = load_v(ph, p, 1, pi, (uint32_t)j - 1u);
+                        Vtx vLp;
+                        if (j >= 2) vLp = load_v(ph, p, 1, pi, (uint32_t)j - 2u);
+                        const Junction J = junction(s, p, cm, vL, vLp, j, vE, vEp, t, prE3);
+                        if (!J.ok || J.pLs == 0.0f) continue;
+                        if (J.Cs.x == 0.0f && J.Cs.y == 0.0f && J.Cs.z == 0.0f) continue;
+                        const float f = J.pLs * area / J.sum;
+                        const f3 C = mk3(J.Cs.x * f, J.Cs.y * f, J.Cs.z * f);
+                        const float pm = area * J.pLs;
+                        Csum = add(Csum, C);
+                        nmerge++;
+                        record_merge(p, sample, j + 1, t, eras.x, eras.y, C, pm * pm / J.sum);
+                    }
+                }
+        if (!(Csum.x == 0.0f && Csum.y == 0.0f && Csum.z == 0.0f)) splat(p, vb.epix[i], Csum);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) nmerge += __shfl_xor(nmerge, off);
+    if ((threadIdx.x & 63u) == 0 && nmerge) atomicAdd(&ctl->merges, (unsigned long long)nmerge);
 }
 
 __global__ void k_bd_zero(float* a, size_t n)
@@ -670,8 +981,9 @@ __global__ void k_bd_scale(float* a, size_t n, float w)
 // Shadow-queue slots one sample may need: the (s >= 1, t >= 1) connections
 // with s + t <= maxv (vcm and bounded bdpt) or s, t <= maxv (unbounded bdpt),
 // plus one s = 0 marker (vcm) or up to maxv - 1 direct strategies (bdpt).
-uint32_t bd_conn_per_sample(uint32_t maxv, int variant, bool bounded)
+uint32_t bd_conn_per_sample(uint32_t maxv, int variant, bool bounded, bool connect = true)
 {
+    if (!connect) return 1;
     if (variant == BD_VCM) return maxv * (maxv - 1) / 2 + 1;
     return (bounded ? maxv * (maxv - 1) / 2 : maxv * maxv) + maxv;
 }
@@ -750,11 +1062,19 @@ int bd_enqueue_batch(lmx_scene* sc, const BParams& bp, hipStream_t st)
     ShadowQ sq{sc->bd_sray.p, sc->bd_sc.p, sc->bd_saux.p, sc->bd_smeta.p};
     k_bd_plan<<<(B + 255) / 256, 256, 0, st>>>(sc->rs, bp, ctl, vb, sq);
     k_bd_connect<<<cap_grid, 256, 0, st>>>(sc->rs, bp, ctl, vb, sq);
-    const uint32_t nconn = B * bd_conn_per_sample((uint32_t)bp.maxv, bp.variant, bp.bounded != 0);
+    const uint32_t nconn = B * bd_conn_per_sample((uint32_t)bp.maxv, bp.variant, bp.bounded != 0, bp.connect != 0);
     int rc = launch_trace(sc->accel, reinterpret_cast<const WRay*>(sq.ray), 0, &ctl->n_shadow, nconn,
                           reinterpret_cast<WHit*>(sc->bd_shits.p), st, true);
     if (rc != LMX_OK) return rc;
     k_bd_splat<<<std::max(1, std::min((int)((nconn + 255) / 256), cap_grid)), 256, 0, st>>>(bp, ctl, sq, sc->bd_shits.p);
+    if (bp.merging) {
+        VB ph = vb;
+        ph.v[1] = sc->vc_vert.p; ph.nsub[1] = sc->vc_nsub.p;
+        const Grid g{sc->vc_items.p, sc->vc_start.p, sc->vc_cursor.p, sc->vc_part.p, sc->vc_H};
+        const size_t nm = (size_t)B * (size_t)(bp.maxv - 1);
+        k_vc_merge<<<std::max(1, (int)std::min<size_t>((nm + 255) / 256, (size_t)cap_grid)), 256, 0, st>>>(
+            sc->rs, bp, ctl, vb, ph, g);
+    }
     k_bd_next_batch<<<1, 64, 0, st>>>(ctl, B);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? LMX_OK : hip_error(e, "bdpt launch");
@@ -872,6 +1192,7 @@ int render_bidir(lmx_scene* sc, const lmx_render_params* p, int variant, const c
     const auto t0 = std::chrono::steady_clock::now();
     BParams bp;
     bp.variant = variant;
+    bp.merging = 0; bp.connect = 1; bp.phase = 0; bp.c_conn = 1.0f;
     bp.mis = variant == BD_BDPT ? p->mis : LMX_MIS_POWER;
     bp.bounded = bounded ? 1 : 0;
     bp.maxv = (int)maxv; bp.minv = p->min_num_vertices;
@@ -925,6 +1246,202 @@ int render_bidir(lmx_scene* sc, const lmx_render_params* p, int variant, const c
     return rc;
 }
 
+// Photon-subpath and grid buffers for np photons of up to maxv vertices;
+// kept when large enough, else replaced (the old ones freed first).
+int vc_buffers(lmx_scene* sc, uint32_t np, uint32_t maxv)
+{
+    if (np <= sc->vc_np && maxv <= sc->vc_maxv) return LMX_OK;
+    sc->vc_np = sc->vc_maxv = sc->vc_H = 0;
+    sc->vc_vert.release(); sc->vc_nsub.release(); sc->vc_items.release();
+    sc->vc_start.release(); sc->vc_cursor.release(); sc->vc_part.release();
+    const size_t items = (size_t)np * (maxv - 1);
+    uint32_t H = 1024;
+    while (H < items && H < (1u << 25)) H <<= 1;
+    hipError_t e;
+    if ((e = sc->vc_vert.alloc((size_t)np * maxv * 4)) != hipSuccess || (e = sc->vc_nsub.alloc(np)) != hipSuccess ||
+        (e = sc->vc_items.alloc(items)) != hipSuccess || (e = sc->vc_start.alloc((size_t)H + 1)) != hipSuccess ||
+        (e = sc->vc_cursor.alloc(H)) != hipSuccess ||
+        (e = sc->vc_part.alloc((H + SCAN_TILE) / SCAN_TILE)) != hipSuccess)
+        return hip_error(e, "vcm photon buffers");
+    sc->vc_np = np;
+    sc->vc_maxv = maxv;
+    sc->vc_H = H;
+    return LMX_OK;
+}
+
+// Records one photon phase: the photon subpaths (light subpaths traced by the
+// gen / trace / shade kernels into the photon vertex buffer) and the hash
+// grid over their vertices.
+int vc_enqueue_photons(lmx_scene* sc, const BParams& pp, hipStream_t st)
+{
+    const uint32_t np = pp.batch;
+    VB ph;
+    ph.v[0] = sc->bd_vert[0].p; ph.v[1] = sc->vc_vert.p;
+    ph.nsub[0] = sc->bd_nsub[0].p; ph.nsub[1] = sc->vc_nsub.p;
+    ph.epix = sc->bd_epix.p; ph.eraster = sc->bd_eraster.p;
+    ph.V = sc->bd_V.p;
+    Queue q[2];
+    for (int k = 0; k < 2; k++) { q[k].ray = sc->bd_ray[k].p; q[k].thr = sc->bd_thr[k].p; q[k].meta = sc->bd_meta[k].p; }
+    BCtl* ctl = sc->bd_ctl.p;
+    const int cap_grid = sc->cus * 8;
+    k_bd_gen<<<(np + 255) / 256, 256, 0, st>>>(sc->rs, pp, ctl, ph, q[0]);
+    int a = 0;
+    for (int it = 0; it < pp.maxv - 1; it++) {
+        int rc = launch_trace(sc->accel, reinterpret_cast<const WRay*>(q[a].ray), 0, &ctl->n_trace, np,
+                              reinterpret_cast<WHit*>(sc->bd_hits.p), st, false);
+        if (rc != LMX_OK) return rc;
+        k_bd_shade<<<(np + 255) / 256, 256, 0, st>>>(sc->rs, pp, ctl, q[a], sc->bd_hits.p, q[1 - a], ph);
+        k_bd_commit<<<1, 64, 0, st>>>(ctl);
+        a = 1 - a;
+    }
+    const Grid g{sc->vc_items.p, sc->vc_start.p, sc->vc_cursor.p, sc->vc_part.p, sc->vc_H};
+    const uint32_t n1 = g.H + 1u, ntile = (n1 + SCAN_TILE - 1) / SCAN_TILE;
+    const size_t nslot = (size_t)np * (size_t)(pp.maxv - 1);
+    const int fgrid = std::max(1, (int)std::min<size_t>((nslot + 255) / 256, (size_t)cap_grid));
+    k_vc_grid_clear<<<std::max(1, std::min((int)((n1 + 255) / 256), cap_grid)), 256, 0, st>>>(g);
+    k_vc_grid_fill<0><<<fgrid, 256, 0, st>>>(pp, ctl, ph, g);
+    k_scan_tiles<<<ntile, 256, 0, st>>>(g.start, n1, g.part);
+    k_scan_part<<<1, 256, 0, st>>>(g.part, ntile);
+    k_scan_add<<<std::max(1, std::min((int)((n1 + 255) / 256), cap_grid)), 256, 0, st>>>(g.start, n1, g.part);
+    k_vc_grid_fill<1><<<fgrid, 256, 0, st>>>(pp, ctl, ph, g);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? LMX_OK : hip_error(e, "vcm photon launch");
+}
+
+// Captures a phase (photons, or one eye batch) as a graph.
+template <typename F>
+int capture(hipStream_t st, hipGraphExec_t* exec, F&& record)
+{
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_error(e, "capture");
+    int rc = record();
+    e = hipStreamEndCapture(st, &graph);
+    if (rc == LMX_OK && e != hipSuccess) rc = hip_error(e, "capture end");
+    if (rc == LMX_OK && (e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0)) != hipSuccess)
+        rc = hip_error(e, "graph instantiate");
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+}
+
+// renderer::vcm, modes vcm and bdpm: per pass the photon graph, then the eye
+// batches (each one graph: subpaths, connections, shadow rays, merges); the
+// film holds the sum of every pass's splats, scaled at the end by
+// W H / (N_eye passes) -- the mean of the per-pass images that the
+// reference's running rescale (renderer_vcm.cpp:752-757) keeps.
+int render_vcm(lmx_scene* sc, const lmx_render_params* p, const lmx_vcm_params* v, float* film_rgb, void* stream,
+               lmx_render_stats* stats)
+{
+    const char* name = "lmx_render_vcm";
+    char msg[160];
+    if (!sc || !p || !v || !film_rgb) return set_error(LMX_EINVAL, "lmx_render_vcm: NULL argument");
+    if (p->num_samples == 0 || p->num_samples > 0xffffffffull)
+        return set_error(LMX_EINVAL, "lmx_render_vcm: num_samples must be in [1, 2^32)");
+    if (p->max_num_vertices < 2 || p->max_num_vertices > 32)
+        return set_error(LMX_EINVAL, "lmx_render_vcm: max_num_vertices must be in [2, 32]");
+    if (v->mode != LMX_VCM_MODE_VCM && v->mode != LMX_VCM_MODE_BDPM)
+        return set_error(LMX_EINVAL, "lmx_render_vcm: unknown mode");
+    if (v->num_iteration_pass == 0) return set_error(LMX_EINVAL, "lmx_render_vcm: num_iteration_pass must be >= 1");
+    if (v->num_photon_trace_samples == 0 || v->num_photon_trace_samples > (1u << 26))
+        return set_error(LMX_EINVAL, "lmx_render_vcm: num_photon_trace_samples must be in [1, 2^26]");
+    if (!(v->initial_radius > 0.0f) || !(v->initial_radius < 3.0e38f) || !(v->alpha > 0.0f) ||
+        !(v->alpha < 3.0e38f))
+        return set_error(LMX_EINVAL, "lmx_render_vcm: initial_radius and alpha must be positive and finite");
+    if ((uint64_t)v->num_iteration_pass * p->num_samples > 0xffffffffull ||
+        (uint64_t)v->num_iteration_pass * v->num_photon_trace_samples > 0xffffffffull)
+        return set_error(LMX_EINVAL, "lmx_render_vcm: passes x samples must stay below 2^32");
+    if ((p->uniform_table || v->photon_table || p->sample_out) && v->num_iteration_pass != 1)
+        return set_error(LMX_EINVAL, "lmx_render_vcm: uniform tables and sample_out need num_iteration_pass 1");
+    if (sc->mats.n > VB_MAX_MATERIALS || sc->lights.n > VB_MAX_LIGHTS)
+        return set_error(LMX_EINVAL, "lmx_render_vcm: at most 65536 materials and 8190 lights");
+    if (p->raster_mode != LMX_RASTER_RANDOM && p->raster_mode != LMX_RASTER_STRATIFIED)
+        return set_error(LMX_EINVAL, "lmx_render_vcm: unknown raster_mode");
+    const uint32_t maxv = (uint32_t)p->max_num_vertices;
+    const uint32_t wsub = LMX_PT_SLOT_SEG0 + 4u * (maxv - 1u);
+    if ((p->uniform_table && p->table_width < 2 * wsub) || (v->photon_table && v->photon_table_width < 2 * wsub)) {
+        snprintf(msg, sizeof msg, "%s: uniform table narrower than %u", name, 2 * wsub);
+        return set_error(LMX_EINVAL, msg);
+    }
+    hipStream_t st = (hipStream_t)stream;
+    hipStream_t own = nullptr;
+    hipError_t e;
+    if (!st) {
+        if ((e = hipStreamCreateWithFlags(&own, hipStreamNonBlocking)) != hipSuccess) return hip_error(e, "stream create");
+        st = own;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    const bool connect = v->mode == LMX_VCM_MODE_VCM;
+    const uint32_t Ne = (uint32_t)p->num_samples, np = v->num_photon_trace_samples;
+    BParams bp;
+    bp.variant = BD_VCM;
+    bp.merging = 1; bp.connect = connect ? 1 : 0; bp.phase = 0; bp.c_conn = connect ? 1.0f : 0.0f;
+    bp.mis = LMX_MIS_POWER;
+    bp.bounded = 1;
+    bp.maxv = (int)maxv; bp.minv = p->min_num_vertices;
+    bp.key = make_uint2((uint32_t)p->seed, (uint32_t)(p->seed >> 32));
+    bp.total = Ne;
+    bp.raster_mode = p->raster_mode;
+    bp.table = p->uniform_table; bp.table_width = p->table_width;
+    bp.wsub = wsub;
+    bp.sample_out = p->sample_out;
+    bp.film = film_rgb;
+    uint32_t B = p->queue_capacity ? p->queue_capacity : (1u << 20);
+    B = std::min(B, Ne);
+    bp.batch = B;
+    BParams pp = bp;                     // the photon phase
+    pp.phase = 1;
+    pp.batch = np; pp.total = np;
+    pp.key = make_uint2((uint32_t)v->photon_seed, (uint32_t)(v->photon_seed >> 32));
+    pp.table = v->photon_table; pp.table_width = v->photon_table_width;
+    pp.sample_out = nullptr;
+    // the photon phase runs in the same queues: one ray per photon
+    int rc = bd_buffers(sc, std::max(B, (np + 1) / 2), maxv, bd_conn_per_sample(maxv, BD_VCM, true, connect));
+    if (rc == LMX_OK) rc = vc_buffers(sc, np, maxv);
+    const size_t npix = (size_t)sc->width * (size_t)sc->height;
+    const int zgrid = std::max(1, std::min((int)((3 * npix + 255) / 256), sc->cus * 8));
+    hipGraphExec_t gph = nullptr, geye = nullptr;
+    if (rc == LMX_OK) {
+        k_bd_zero<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix);
+        if (p->sample_out) {
+            const size_t ns = (size_t)Ne * 16 * (size_t)(maxv + 1) * (size_t)(maxv + 1);
+            k_bd_zero<<<std::max(1, (int)std::min<size_t>((ns + 255) / 256, (size_t)sc->cus * 8)), 256, 0, st>>>(
+                p->sample_out, ns);
+        }
+        k_bd_init<<<1, 64, 0, st>>>(sc->bd_ctl.p);
+        rc = capture(st, &gph, [&] { return vc_enqueue_photons(sc, pp, st); });
+    }
+    if (rc == LMX_OK) rc = capture(st, &geye, [&] { return bd_enqueue_batch(sc, bp, st); });
+    const uint64_t nbatch = ((uint64_t)Ne + B - 1) / B;
+    double r = v->initial_radius;
+    for (uint32_t q = 0; rc == LMX_OK && q < v->num_iteration_pass; q++) {
+        if (q > 0) r = std::sqrt((v->alpha + (double)q) / (1.0 + (double)q)) * r;
+        const double a = 3.14159265358979323846 * r * r;
+        const float rf = (float)r, cell = (float)(2.0 * r), cm = (float)((double)np * a * a);
+        k_bd_phase<<<1, 64, 0, st>>>(sc->bd_ctl.p, q * np, rf, cell, cm);
+        if ((e = hipGraphLaunch(gph, st)) != hipSuccess) { rc = hip_error(e, "graph launch"); break; }
+        k_bd_phase<<<1, 64, 0, st>>>(sc->bd_ctl.p, q * Ne, rf, cell, cm);
+        for (uint64_t b = 0; rc == LMX_OK && b < nbatch; b++)
+            if ((e = hipGraphLaunch(geye, st)) != hipSuccess) rc = hip_error(e, "graph launch");
+    }
+    if (rc == LMX_OK) {
+        k_bd_scale<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix,
+                                          (float)((double)npix / ((double)Ne * (double)v->num_iteration_pass)));
+        if ((e = hipMemcpyAsync(sc->bd_hctl, sc->bd_ctl.p, sizeof(BCtl), hipMemcpyDeviceToHost, st)) != hipSuccess ||
+            (e = hipStreamSynchronize(st)) != hipSuccess)
+            rc = hip_error(e, "lmx_render_vcm: sync");
+    }
+    if (gph) (void)hipGraphExecDestroy(gph);
+    if (geye) (void)hipGraphExecDestroy(geye);
+    if (own) (void)hipStreamDestroy(own);
+    if (stats) {
+        stats->rays = rc == LMX_OK ? sc->bd_hctl->rays + sc->bd_hctl->shadow_rays : 0;
+        stats->iterations = v->num_iteration_pass;
+        stats->queue_capacity = B;
+        stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -938,6 +1455,12 @@ int lmx_render_vcm_bdpt(lmx_scene* sc, const lmx_render_params* p, float* film_r
 int lmx_render_bdpt(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, void* stream, lmx_render_stats* stats)
 {
     return render_bidir(sc, p, BD_BDPT, "lmx_render_bdpt", film_rgb, stream, stats);
+}
+
+int lmx_render_vcm(lmx_scene* sc, const lmx_render_params* p, const lmx_vcm_params* v, float* film_rgb, void* stream,
+                   lmx_render_stats* stats)
+{
+    return render_vcm(sc, p, v, film_rgb, stream, stats);
 }
 
 }  // extern "C"

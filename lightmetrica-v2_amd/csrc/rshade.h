@@ -478,6 +478,11 @@ struct BCtl {
     uint32_t batch_first;      // first sample of the current batch
     unsigned long long rays;   // extension rays traced
     unsigned long long shadow_rays;   // live connection (shadow) rays traced
+    uint32_t pass_first;       // first sample index of the current pass (photons or eye samples)
+    float radius;              // vertex merging: merge radius of the pass
+    float cell;                // vertex merging: grid cell edge (2 radius)
+    float c_merge;             // vertex merging: N (pi radius^2)^2
+    unsigned long long merges; // vertex merging: merged paths evaluated
 };
 
 // The scene object behind the C ABI: accel, shading data on the device and
@@ -522,6 +527,12 @@ struct lmx_scene {
     DevBuf<uint4> bd_smeta;
     DevBuf<BCtl> bd_ctl;
     BCtl* bd_hctl = nullptr;      // pinned
+    // vertex merging: photon subpaths and the hash grid over their vertices
+    uint32_t vc_np = 0, vc_maxv = 0, vc_H = 0;
+    DevBuf<float4> vc_vert;             // photon subpath vertices, as bd_vert[1]
+    DevBuf<uint32_t> vc_nsub;
+    DevBuf<float4> vc_items;            // position, photon * maxv + vertex
+    DevBuf<uint32_t> vc_start, vc_cursor, vc_part;
     ~lmx_scene()
     {
         if (h_ctl) (void)hipHostFree(h_ctl);

@@ -13,7 +13,7 @@ import ctypes
 import numpy as np
 
 from ._lib import (check, lib, lmx_camera, lmx_material, lmx_render_params, lmx_render_stats,
-                   lmx_scene_desc)
+                   lmx_scene_desc, lmx_vcm_params)
 from .scenes import camera_frame
 
 RASTER_RANDOM = 0
@@ -22,6 +22,9 @@ RR_FIXED = 0
 RR_THROUGHPUT = 1
 MIS_POWER = 0
 MIS_SIMPLE = 1
+# renderer::vcm modes with merging (renderer_vcm.cpp:563-570)
+VCM_MODE_VCM = 0
+VCM_MODE_BDPM = 2
 # bsdf::cooktorrance defaults (bsdf_cooktorrance.cpp:60-63)
 GGX_DEFAULT = dict(roughness=0.1, eta=(0.14, 0.129, 0.1585), k=(4.58625, 3.348125, 2.329375))
 SLOT_SEG0 = 8
@@ -129,6 +132,47 @@ class Scene:
         m = 32 if max_num_vertices == -1 else max_num_vertices
         return self._bidir("lmx_render_bdpt", 2 * (m + 1) ** 2, num_samples, max_num_vertices,
                            min_num_vertices, seed, raster_mode, batch, table, sample_out, stream, mis)
+
+    def render_vcm(self, num_samples, mode=VCM_MODE_VCM, num_iteration_pass=100, num_photon_trace_samples=10000,
+                   initial_radius=0.1, alpha=0.7, max_num_vertices=10, min_num_vertices=0, seed=42,
+                   photon_seed=None, raster_mode=RASTER_STRATIFIED, batch=0, table=None, photon_table=None,
+                   sample_out=False, stream=None):
+        """renderer::vcm in mode vcm or bdpm (defaults: renderer_vcm.cpp:
+        560-575); num_samples = num_eye_trace_samples per pass.  Returns (film
+        (H, W, 3), the mean of the passes' images; stats; per-sample records
+        (n, 2 (m+1)^2, 8) at 2 (t (m+1) + s) + merge, or None)."""
+        import torch
+        dev = torch.device("cuda")
+        film = torch.empty((self.height, self.width, 3), dtype=torch.float32, device=dev)
+        p = self._params(num_samples, max_num_vertices, min_num_vertices, 1, 0.5, seed, raster_mode, batch)
+        v = lmx_vcm_params()
+        v.mode = mode
+        v.num_iteration_pass = num_iteration_pass
+        v.num_photon_trace_samples = num_photon_trace_samples
+        v.initial_radius = initial_radius
+        v.alpha = alpha
+        v.photon_seed = (seed + 1) if photon_seed is None else photon_seed
+        so = None
+        for t, n in ((table, num_samples), (photon_table, num_photon_trace_samples)):
+            if t is not None:
+                assert t.is_cuda and t.dtype == torch.float32 and t.is_contiguous() and t.shape[0] == n
+        if table is not None:
+            p.uniform_table = ctypes.c_void_p(table.data_ptr())
+            p.table_width = table.shape[1]
+        if photon_table is not None:
+            v.photon_table = ctypes.c_void_p(photon_table.data_ptr())
+            v.photon_table_width = photon_table.shape[1]
+        if sample_out:
+            so = torch.empty((num_samples, 2 * (max_num_vertices + 1) ** 2, 8), dtype=torch.float32, device=dev)
+            p.sample_out = ctypes.c_void_p(so.data_ptr())
+        st = lmx_render_stats()
+        if stream is None:
+            stream = torch.cuda.Stream(dev)
+            stream.wait_stream(torch.cuda.current_stream(dev))
+        check(lib.lmx_render_vcm(self._h, ctypes.byref(p), ctypes.byref(v), ctypes.c_void_p(film.data_ptr()),
+                                 ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st)))
+        stats = {"rays": st.rays, "passes": st.iterations, "batch": st.queue_capacity, "ms": st.ms}
+        return film, stats, so
 
     def _bidir(self, entry, nrec, num_samples, max_num_vertices, min_num_vertices, seed, raster_mode, batch,
                table, sample_out, stream, mis):
