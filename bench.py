@@ -216,6 +216,67 @@ def bdpt_section(dev, cpu, threads):
     return out
 
 
+def vcm_section(dev, cpu, threads):
+    """renderer::vcm with vertex merging on the sphere-grid frame of the
+    bidirectional section (1024x1024, subpaths of up to 10 vertices): 8 passes
+    of 2 spp of eye samples and 2^20 photon subpaths each, initial radius 0.05
+    (about 3.5 pixels at the grid's distance), alpha 0.7; mode vcm
+    (connections and merges) and mode bdpm (merges alone).  Eye samples/s of
+    the whole render (photon passes, grids, eye subpaths, connections, shadow
+    rays and merges), beside the oracle's restatement of one pass on host
+    threads over a bounded prefix of eye samples, its photon pass (one host
+    thread) timed whole."""
+    import torch
+    from lmx import render, scenes
+    log("vcm: generating the sphere-grid scene")
+    sc = scenes.sphere_grid()
+    rs = render.Scene(sc)
+    npix = 1024 * 1024
+    passes, ne, nph, r0 = 8, 2 * npix, 1 << 20, 0.05
+    kw = dict(num_iteration_pass=passes, num_photon_trace_samples=nph, initial_radius=r0, alpha=0.7,
+              max_num_vertices=10, seed=42)
+    out = {"workload": "vcm: 16x16 displaced icospheres (5,242,886 tris), 1024x1024, 8 passes x 2 spp, "
+                       "2^20 photon subpaths per pass, radius 0.05 (alpha 0.7), subpaths of up to 10 vertices"}
+    for key, mode in (("vcm", render.VCM_MODE_VCM), ("bdpm", render.VCM_MODE_BDPM)):
+        rs.render_vcm(npix // 4, mode=mode, **dict(kw, num_iteration_pass=1))        # warm-up
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            film, st, _ = rs.render_vcm(ne, mode=mode, **kw)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        dt = min(times)
+        out[key] = {"msamples_per_s": round(passes * ne / dt / 1e6, 2),
+                    "mrays_per_s": round(st["rays"] / dt / 1e6, 1),
+                    "rays_per_sample": round(st["rays"] / (passes * ne), 4),
+                    "ms_per_pass": round(dt / passes * 1e3, 2), "s_per_frame": round(dt, 4),
+                    "image_mean": round(float(film.mean()), 5), "cpu_baseline": None}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import orc
+        log("vcm: cpu baseline (oracle renderer::vcm, one pass)")
+        osc = orc.Scene(sc)
+        m = dict(radius=r0, num_photons=nph, mode=orc.VCM_VCM, max_num_vertices=10, seed=42, pseed=43)
+        tph = []
+        for _ in range(2):                                       # photon pass + grid + one eye sample
+            t0 = time.perf_counter()
+            osc.vcm_samples(0, 1, nthreads=threads, **m)
+            tph.append(time.perf_counter() - t0)
+        tph = min(tph)
+        chunk = 65536
+        t0 = time.perf_counter()
+        osc.vcm_samples(0, chunk, nthreads=threads, **m)
+        teye = max(time.perf_counter() - t0 - tph, 1e-6) / chunk
+        rate = ne / (tph + ne * teye)
+        out["vcm"]["cpu_baseline"] = {
+            "msamples_per_s": round(rate / 1e6, 5), "cores": threads, "kind": "port",
+            "sample": f"one pass on {threads} threads: photon pass of 2^20 subpaths with its grid ({tph:.2f}s) and "
+                      f"{chunk} eye samples ({teye * 1e6:.1f} us each), scaled to the pass's {ne} eye samples"}
+        out["vcm"]["gpu_over_cpu_samples"] = round(out["vcm"]["msamples_per_s"] * 1e6 / rate, 1)
+    return out
+
+
 def image_section(dev, cpu):
     """lmx_film_save of a 1024x1024 film in each format (GPU encode, one
     device-to-host copy, host run-length / container write to a temporary
@@ -395,9 +456,10 @@ def main():
     ap.add_argument("--no-bdpt", action="store_true", help="skip the bidirectional section")
     ap.add_argument("--no-guide", action="store_true", help="skip the path-guiding section")
     ap.add_argument("--no-image", action="store_true", help="skip the film-output section")
+    ap.add_argument("--no-vcm", action="store_true", help="skip the vertex-merging section")
     ap.add_argument("--guide-ref-spp", type=int, default=16384,
                     help="spp of the plain reference image of the path-guiding section")
-    ap.add_argument("--only", choices=["pt", "bdpt", "guide", "image"], default=None,
+    ap.add_argument("--only", choices=["pt", "bdpt", "vcm", "guide", "image"], default=None,
                     help="run one renderer section alone and print its JSON (for profiling runs)")
     ap.add_argument("--no-per-set", action="store_true",
                     help="skip the separate coherent/incoherent timings (for profiling runs)")
@@ -432,6 +494,8 @@ def main():
             out = guide_section(dev, args.guide_ref_spp)
         elif args.only == "image":
             out = image_section(dev, not args.no_cpu_baseline)
+        elif args.only == "vcm":
+            out = vcm_section(dev, not args.no_cpu_baseline, threads)
         else:
             fn = pt_section if args.only == "pt" else bdpt_section
             out = fn(dev, not args.no_cpu_baseline, threads)
@@ -557,6 +621,8 @@ def main():
         out["pt"] = pt_section(dev, not args.no_cpu_baseline, args.cpu_threads or min(os.cpu_count() or 1, 16))
     if not args.no_bdpt:
         out["bdpt"] = bdpt_section(dev, not args.no_cpu_baseline, args.cpu_threads or min(os.cpu_count() or 1, 16))
+    if not args.no_vcm:
+        out["vcm"] = vcm_section(dev, not args.no_cpu_baseline, args.cpu_threads or min(os.cpu_count() or 1, 16))
     if not args.no_guide:
         out["guide"] = guide_section(dev, args.guide_ref_spp)
     if not args.no_image:

@@ -123,6 +123,7 @@ struct VB {
     uint32_t* epix;
     float2* eraster;
     float* V;                  // bdpt: V_k of the light vertices
+    uint32_t* mq;              // merging: queue of eye vertices to merge at (sample * 32 + vertex)
 };
 
 struct Vtx {
@@ -600,6 +601,18 @@ __global__ __launch_bounds__(256) void k_bd_plan(RScene s, BParams p, BCtl* __re
     if (ndirect)
         for (int k = 0; k < nE; k++)
             if (eye_direct(s, p, vb, i, sample, k)) sq.meta[slot++] = make_uint4(i, 1u, (uint32_t)k + 1u, 1u);
+    if (p.merging) {
+        // eye vertices k >= 1 that scatter and leave room for a merged path
+        // of at most maxv vertices (photon vertex j >= 1, length j + k + 1)
+        uint32_t mmask = 0;
+        for (int k = 1; k < nE && k + 2 <= p.maxv; k++) {
+            const float4 a = vb.v[0][((size_t)i * (size_t)p.maxv + (size_t)k) * 4];
+            if (is_scatterer(s.mats[__float_as_uint(a.w) >> 16])) mmask |= 1u << k;
+        }
+        uint32_t mslot = block_reserve<4>(&ctl->n_mq, (uint32_t)__popc(mmask), lds);
+        for (int k = 1; k < 32; k++)
+            if (mmask & (1u << k)) vb.mq[mslot++] = i * 32u + (uint32_t)k;
+    }
     if (!valid || !p.connect) return;
     const uint32_t pixel = vb.epix[i];
     const float2 eras = vb.eraster[i];
@@ -755,7 +768,7 @@ __global__ void k_bd_next_batch(BCtl* ctl, uint32_t batch)
 {
     if (threadIdx.x == 0) {
         ctl->batch_first += batch;
-        ctl->n_trace = 0; ctl->n_next = 0; ctl->n_shadow = 0;
+        ctl->n_trace = 0; ctl->n_next = 0; ctl->n_shadow = 0; ctl->n_mq = 0;
     }
 }
 
@@ -765,6 +778,7 @@ __global__ void k_bd_init(BCtl* ctl)
         ctl->n_trace = 0; ctl->n_next = 0; ctl->n_shadow = 0; ctl->batch_first = 0;
         ctl->rays = 0; ctl->shadow_rays = 0;
         ctl->pass_first = 0; ctl->radius = 0.0f; ctl->cell = 0.0f; ctl->c_merge = 0.0f; ctl->merges = 0;
+        ctl->n_mq = 0;
     }
 }
 
@@ -773,7 +787,7 @@ __global__ void k_bd_init(BCtl* ctl)
 __global__ void k_bd_phase(BCtl* ctl, uint32_t first, float radius, float cell, float c_merge)
 {
     if (threadIdx.x == 0) {
-        ctl->n_trace = 0; ctl->n_next = 0; ctl->n_shadow = 0;
+        ctl->n_trace = 0; ctl->n_next = 0; ctl->n_shadow = 0; ctl->n_mq = 0;
         ctl->batch_first = first; ctl->pass_first = first;
         ctl->radius = radius; ctl->cell = cell; ctl->c_merge = c_merge;
     }
@@ -902,65 +916,92 @@ __global__ __launch_bounds__(256) void k_scan_add(uint32_t* a, uint32_t n, const
         a[j] += part[j / SCAN_TILE];
 }
 
-// Merges: one thread per (eye sample, eye vertex k = t - 1 >= 1).  Every
-// photon vertex j within r makes the path of the connection (s = j, t)
-// between photon vertex j - 1 and eye vertex t - 1, kept when its length is
-// within [min, max] vertices; its contribution f w / p is the connection's
-// unweighted term times pLs pi r^2 / D and its weight (pi r^2 pLs)^2 / D,
-// D the mu-weighted denominator (EvaluateF / EvaluatePathPDF with merging and
-// EvaluateMISWeight_VCM / _BDPM, renderer_vcm.cpp:116-340).
+// Merges: one thread per queued eye vertex k = t - 1 >= 1.  Every photon
+// vertex j within r makes the path of the connection (s = j, t) between
+// photon vertex j - 1 and eye vertex t - 1, kept when its length is within
+// [min, max] vertices; its contribution f w / p is the connection's
+// unweighted term times pLs pi r^2 / D and its weight (pi r^2 pLs)^2 / D, D
+// the mu-weighted denominator (EvaluateF / EvaluatePathPDF with merging and
+// EvaluateMISWeight_VCM / _BDPM, renderer_vcm.cpp:116-340).  The thread
+// alternates a gather of up to MERGE_LIST photon vertices within r (cheap
+// distance tests) into its LDS list with the evaluation of the list, so the
+// lanes of a wave evaluate together instead of idling behind each other's
+// candidate tests.
+#define MERGE_LIST 16
+
 __global__ __launch_bounds__(256) void k_vc_merge(RScene s, BParams p, BCtl* __restrict__ ctl, VB vb, VB ph,
                                                   Grid g)
 {
-    const uint32_t nk = (uint32_t)p.maxv - 1u;
-    const size_t n = (size_t)p.batch * nk;
+    __shared__ uint32_t hl[MERGE_LIST * 256];          // [hit][thread]
+    const uint32_t nq = ctl->n_mq;
     const float r = ctl->radius, cell = ctl->cell, cm = ctl->c_merge;
     const float r2 = r * r, area = LMX_PI_F * r * r;
     uint32_t nmerge = 0;
-    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
-        const uint32_t i = (uint32_t)(idx % p.batch), k = 1u + (uint32_t)(idx / p.batch);
+    for (uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x; qi < nq; qi += gridDim.x * blockDim.x) {
+        const uint32_t e = vb.mq[qi];
+        const uint32_t i = e >> 5, k = e & 31u;
         const uint32_t sample = ctl->batch_first + i;
-        if (sample - ctl->pass_first >= p.total) continue;
-        if (k >= vb.nsub[0][i]) continue;
         const int t = (int)k + 1;
         const Vtx vE = load_v(vb, p, 0, i, k);
         const Vtx vEp = load_v(vb, p, 0, i, k - 1u);
         const float prE3 = t >= 4 ? load_pr(vb, p, 0, i, (uint32_t)t - 3) : 0.0f;
         const float2 eras = vb.eraster[i];
-        int lo[3], hi[3];
+        int lo[3], nc[3];
         const float q[3] = {vE.p.x, vE.p.y, vE.p.z};
 #pragma unroll
-        for (int a = 0; a < 3; a++) { lo[a] = cell_of(q[a] - r, cell); hi[a] = cell_of(q[a] + r, cell); }
+        for (int a = 0; a < 3; a++) {
+            lo[a] = cell_of(q[a] - r, cell);
+            nc[a] = cell_of(q[a] + r, cell) - lo[a] + 1;
+        }
+        const int ncell = nc[0] * nc[1] * nc[2];
+        // photon vertex j may merge only if the merged path fits: j in [minv - t, maxv - t]
+        const int jmax = p.maxv - t, jmin = p.minv - t;
+        int c = 0;
+        uint32_t it = 0, end = 0;
+        bool fresh = true;                               // `it` not yet set for cell c
         f3 Csum = mk3(0, 0, 0);
-        for (int cz = lo[2]; cz <= hi[2]; cz++)
-            for (int cy = lo[1]; cy <= hi[1]; cy++)
-                for (int cx = lo[0]; cx <= hi[0]; cx++) {
+        while (c < ncell) {
+            // gather
+            uint32_t nh = 0;
+            while (c < ncell && nh < MERGE_LIST) {
+                const int cx = lo[0] + c % nc[0], cy = lo[1] + (c / nc[0]) % nc[1], cz = lo[2] + c / (nc[0] * nc[1]);
+                if (fresh) {
                     const uint32_t h = bucket_of(cx, cy, cz, g.H);
-                    const uint32_t e = g.start[h + 1];
-                    for (uint32_t it = g.start[h]; it < e; it++) {
-                        const float4 c = g.items[it];
-                        if (cell_of(c.x, cell) != cx || cell_of(c.y, cell) != cy || cell_of(c.z, cell) != cz) continue;
-                        const f3 d = sub(mk3(c.x, c.y, c.z), vE.p);
-                        if (!(dot(d, d) < r2)) continue;
-                        const uint32_t id = __float_as_uint(c.w);
-                        const uint32_t pi = id / (uint32_t)p.maxv;
-                        const int j = (int)(id % (uint32_t)p.maxv);
-                        const int len = j + t;
-                        if (len < p.minv || len > p.maxv) continue;
-                        const Vtx vL = load_v(ph, p, 1, pi, (uint32_t)j - 1u);
-                        Vtx vLp;
-                        if (j >= 2) vLp = load_v(ph, p, 1, pi, (uint32_t)j - 2u);
-                        const Junction J = junction(s, p, cm, vL, vLp, j, vE, vEp, t, prE3);
-                        if (!J.ok || J.pLs == 0.0f) continue;
-                        if (J.Cs.x == 0.0f && J.Cs.y == 0.0f && J.Cs.z == 0.0f) continue;
-                        const float f = J.pLs * area / J.sum;
-                        const f3 C = mk3(J.Cs.x * f, J.Cs.y * f, J.Cs.z * f);
-                        const float pm = area * J.pLs;
-                        Csum = add(Csum, C);
-                        nmerge++;
-                        record_merge(p, sample, j + 1, t, eras.x, eras.y, C, pm * pm / J.sum);
-                    }
+                    it = g.start[h];
+                    end = g.start[h + 1];
+                    fresh = false;
                 }
+                for (; it < end && nh < MERGE_LIST; it++) {
+                    const float4 cd = g.items[it];
+                    const int j = (int)(__float_as_uint(cd.w) % (uint32_t)p.maxv);
+                    if (j > jmax || j < jmin) continue;
+                    if (cell_of(cd.x, cell) != cx || cell_of(cd.y, cell) != cy || cell_of(cd.z, cell) != cz) continue;
+                    const f3 d = sub(mk3(cd.x, cd.y, cd.z), vE.p);
+                    if (!(dot(d, d) < r2)) continue;
+                    hl[nh * 256u + threadIdx.x] = __float_as_uint(cd.w);
+                    nh++;
+                }
+                if (it >= end) { c++; fresh = true; }
+            }
+            // evaluate
+            for (uint32_t h = 0; h < nh; h++) {
+                const uint32_t id = hl[h * 256u + threadIdx.x];
+                const uint32_t pi = id / (uint32_t)p.maxv;
+                const int j = (int)(id % (uint32_t)p.maxv);
+                const Vtx vL = load_v(ph, p, 1, pi, (uint32_t)j - 1u);
+                Vtx vLp;
+                if (j >= 2) vLp = load_v(ph, p, 1, pi, (uint32_t)j - 2u);
+                const Junction J = junction(s, p, cm, vL, vLp, j, vE, vEp, t, prE3);
+                if (!J.ok || J.pLs == 0.0f) continue;
+                if (J.Cs.x == 0.0f && J.Cs.y == 0.0f && J.Cs.z == 0.0f) continue;
+                const float f = J.pLs * area / J.sum;
+                const f3 C = mk3(J.Cs.x * f, J.Cs.y * f, J.Cs.z * f);
+                const float pm = area * J.pLs;
+                Csum = add(Csum, C);
+                nmerge++;
+                record_merge(p, sample, j + 1, t, eras.x, eras.y, C, pm * pm / J.sum);
+            }
+        }
         if (!(Csum.x == 0.0f && Csum.y == 0.0f && Csum.z == 0.0f)) splat(p, vb.epix[i], Csum);
     }
 #pragma unroll
@@ -1045,6 +1086,7 @@ int bd_enqueue_batch(lmx_scene* sc, const BParams& bp, hipStream_t st)
     vb.nsub[0] = sc->bd_nsub[0].p; vb.nsub[1] = sc->bd_nsub[1].p;
     vb.epix = sc->bd_epix.p; vb.eraster = sc->bd_eraster.p;
     vb.V = sc->bd_V.p;
+    vb.mq = sc->vc_mq.p;
     Queue q[2];
     for (int k = 0; k < 2; k++) { q[k].ray = sc->bd_ray[k].p; q[k].thr = sc->bd_thr[k].p; q[k].meta = sc->bd_meta[k].p; }
     BCtl* ctl = sc->bd_ctl.p;
@@ -1072,7 +1114,7 @@ int bd_enqueue_batch(lmx_scene* sc, const BParams& bp, hipStream_t st)
         ph.v[1] = sc->vc_vert.p; ph.nsub[1] = sc->vc_nsub.p;
         const Grid g{sc->vc_items.p, sc->vc_start.p, sc->vc_cursor.p, sc->vc_part.p, sc->vc_H};
         const size_t nm = (size_t)B * (size_t)(bp.maxv - 1);
-        k_vc_merge<<<std::max(1, (int)std::min<size_t>((nm + 255) / 256, (size_t)cap_grid)), 256, 0, st>>>(
+        k_vc_merge<<<std::max(1, (int)std::min<size_t>((nm + 255) / 256, (size_t)cap_grid * 2)), 256, 0, st>>>(
             sc->rs, bp, ctl, vb, ph, g);
     }
     k_bd_next_batch<<<1, 64, 0, st>>>(ctl, B);
@@ -1248,8 +1290,12 @@ int render_bidir(lmx_scene* sc, const lmx_render_params* p, int variant, const c
 
 // Photon-subpath and grid buffers for np photons of up to maxv vertices;
 // kept when large enough, else replaced (the old ones freed first).
-int vc_buffers(lmx_scene* sc, uint32_t np, uint32_t maxv)
+int vc_buffers(lmx_scene* sc, uint32_t np, uint32_t maxv, uint32_t B)
 {
+    if ((size_t)B * (maxv - 1) > sc->vc_mq.n) {
+        hipError_t e = sc->vc_mq.alloc((size_t)B * (maxv - 1));
+        if (e != hipSuccess) return hip_error(e, "vcm merge queue");
+    }
     if (np <= sc->vc_np && maxv <= sc->vc_maxv) return LMX_OK;
     sc->vc_np = sc->vc_maxv = sc->vc_H = 0;
     sc->vc_vert.release(); sc->vc_nsub.release(); sc->vc_items.release();
@@ -1280,6 +1326,7 @@ int vc_enqueue_photons(lmx_scene* sc, const BParams& pp, hipStream_t st)
     ph.nsub[0] = sc->bd_nsub[0].p; ph.nsub[1] = sc->vc_nsub.p;
     ph.epix = sc->bd_epix.p; ph.eraster = sc->bd_eraster.p;
     ph.V = sc->bd_V.p;
+    ph.mq = nullptr;
     Queue q[2];
     for (int k = 0; k < 2; k++) { q[k].ray = sc->bd_ray[k].p; q[k].thr = sc->bd_thr[k].p; q[k].meta = sc->bd_meta[k].p; }
     BCtl* ctl = sc->bd_ctl.p;
@@ -1396,7 +1443,7 @@ int render_vcm(lmx_scene* sc, const lmx_render_params* p, const lmx_vcm_params* 
     pp.sample_out = nullptr;
     // the photon phase runs in the same queues: one ray per photon
     int rc = bd_buffers(sc, std::max(B, (np + 1) / 2), maxv, bd_conn_per_sample(maxv, BD_VCM, true, connect));
-    if (rc == LMX_OK) rc = vc_buffers(sc, np, maxv);
+    if (rc == LMX_OK) rc = vc_buffers(sc, np, maxv, B);
     const size_t npix = (size_t)sc->width * (size_t)sc->height;
     const int zgrid = std::max(1, std::min((int)((3 * npix + 255) / 256), sc->cus * 8));
     hipGraphExec_t gph = nullptr, geye = nullptr;

@@ -483,6 +483,7 @@ struct BCtl {
     float cell;                // vertex merging: grid cell edge (2 radius)
     float c_merge;             // vertex merging: N (pi radius^2)^2
     unsigned long long merges; // vertex merging: merged paths evaluated
+    uint32_t n_mq;             // vertex merging: eye vertices queued for merging in the batch
 };
 
 // The scene object behind the C ABI: accel, shading data on the device and
@@ -533,6 +534,7 @@ struct lmx_scene {
     DevBuf<uint32_t> vc_nsub;
     DevBuf<float4> vc_items;            // position, photon * maxv + vertex
     DevBuf<uint32_t> vc_start, vc_cursor, vc_part;
+    DevBuf<uint32_t> vc_mq;             // merge queue: eye sample * 32 + vertex, bd_batch x (bd_maxv - 1)
     ~lmx_scene()
     {
         if (h_ctl) (void)hipHostFree(h_ctl);

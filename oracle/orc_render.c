@@ -1128,6 +1128,29 @@ static void vcm_sample(const vcm_ctx *c, uint64_t sample, float *out)
 
 typedef struct { vcm_ctx *c; uint64_t b, e; } vcm_job;
 
+/* photon subpaths [b, e) */
+typedef struct {
+    const bctx *b;
+    const orc_bdpt_params *bp;
+    const usrc *pu;
+    bvert *ph;
+    int *ph_n;
+    int maxv;
+    uint32_t wsub;
+    uint32_t lo, hi;
+} vcm_pjob;
+
+static void *vcm_photons(void *p)
+{
+    vcm_pjob *j = (vcm_pjob *)p;
+    bvert tmp[64];
+    for (uint32_t i = j->lo; i < j->hi; i++) {
+        j->ph_n[i] = trace_subpath(j->b, j->bp, j->pu, i, j->wsub, 1, tmp);
+        for (int k = 0; k < j->ph_n[i]; k++) j->ph[(size_t)i * j->maxv + k] = tmp[k];
+    }
+    return NULL;
+}
+
 static void *vcm_run(void *p)
 {
     vcm_job *j = (vcm_job *)p;
@@ -1159,20 +1182,29 @@ void orc_vcm_samples(const orc_scene *s, const orc_vcm_params *pp, const float *
     pu.first = 0;
     orc_bdpt_params bp = {pp->max_num_vertices, pp->min_num_vertices, pp->pseed_lo, pp->pseed_hi, pp->raster_mode};
     const uint32_t wsub = ORC_PT_SEG0 + 4u * (uint32_t)(maxv - 1);
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    {
+        pthread_t pth[256];
+        vcm_pjob pj[256];
+        for (int t = 0; t < nthreads; t++) {
+            pj[t].b = &b; pj[t].bp = &bp; pj[t].pu = &pu; pj[t].ph = ph; pj[t].ph_n = ph_n;
+            pj[t].maxv = maxv; pj[t].wsub = wsub;
+            pj[t].lo = (uint32_t)((uint64_t)np * (uint64_t)t / (uint64_t)nthreads);
+            pj[t].hi = (uint32_t)((uint64_t)np * (uint64_t)(t + 1) / (uint64_t)nthreads);
+            pthread_create(&pth[t], NULL, vcm_photons, &pj[t]);
+        }
+        for (int t = 0; t < nthreads; t++) pthread_join(pth[t], NULL);
+    }
     float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
     int count = 0;
-    for (uint32_t i = 0; i < np; i++) {
-        bvert tmp[64];
-        ph_n[i] = trace_subpath(&b, &bp, &pu, i, wsub, 1, tmp);
-        for (int j = 0; j < ph_n[i]; j++) {
-            ph[(size_t)i * maxv + j] = tmp[j];
-            if (j >= 1) {
-                const float q[3] = {tmp[j].fr.p.x, tmp[j].fr.p.y, tmp[j].fr.p.z};
-                for (int a = 0; a < 3; a++) { if (q[a] < lo[a]) lo[a] = q[a]; if (q[a] > hi[a]) hi[a] = q[a]; }
-                count++;
-            }
+    for (uint32_t i = 0; i < np; i++)
+        for (int j = 1; j < ph_n[i]; j++) {
+            const bvert *v = &ph[(size_t)i * maxv + j];
+            const float q[3] = {v->fr.p.x, v->fr.p.y, v->fr.p.z};
+            for (int a = 0; a < 3; a++) { if (q[a] < lo[a]) lo[a] = q[a]; if (q[a] > hi[a]) hi[a] = q[a]; }
+            count++;
         }
-    }
     vcm_ctx c;
     c.b = &b; c.pp = pp; c.ph = ph; c.ph_n = ph_n;
     c.cell = pp->radius;
@@ -1224,8 +1256,6 @@ void orc_vcm_samples(const orc_scene *s, const orc_vcm_params *pp, const float *
     c.out = out;
     c.stride = 2 * (maxv + 1) * (maxv + 1);
     c.first = first_sample; c.n = n;
-    if (nthreads < 1) nthreads = 1;
-    if (nthreads > 256) nthreads = 256;
     pthread_t th[256];
     vcm_job jobs[256];
     for (int t = 0; t < nthreads; t++) {
