@@ -258,8 +258,8 @@ int lmx_render_vcm_bdpt(lmx_scene *scene, const lmx_render_params *params, float
 int lmx_render_bdpt(lmx_scene *scene, const lmx_render_params *params, float *film_rgb, void *stream,
                     lmx_render_stats *stats);
 
-/* renderer::vcm in modes vcm and bdpm (renderer_vcm.cpp:580-770, parameters
- * :560-575): num_iteration_pass passes; pass q traces
+/* renderer::vcm in modes vcm and bdpm (renderer_vcm.cpp:584-790, parameters
+ * :558-577): num_iteration_pass passes; pass q traces
  * num_photon_trace_samples photon (light) subpaths, gathers their vertices
  * past the light into a hash grid, then params->num_samples eye samples
  * (num_eye_trace_samples), each with (mode vcm) every connection of

@@ -15,7 +15,7 @@
 //           strategies carry nothing (EvaluateCst evaluates the sensor's
 //           position with evalDelta and gets 0, :389-393);
 //   vcm merging -- renderer::vcm in modes vcm and bdpm (renderer_vcm.cpp:
-//           580-770): per pass, N photon subpaths (light subpaths of their
+//           584-790): per pass, N photon subpaths (light subpaths of their
 //           own uniform source) traced by the same gen/trace/shade kernels,
 //           a hash grid over their vertices past the light, then the eye
 //           batches with (mode vcm) every connection and, at every eye

@@ -138,7 +138,7 @@ class Scene:
                    photon_seed=None, raster_mode=RASTER_STRATIFIED, batch=0, table=None, photon_table=None,
                    sample_out=False, stream=None):
         """renderer::vcm in mode vcm or bdpm (defaults: renderer_vcm.cpp:
-        560-575); num_samples = num_eye_trace_samples per pass.  Returns (film
+        558-577); num_samples = num_eye_trace_samples per pass.  Returns (film
         (H, W, 3), the mean of the passes' images; stats; per-sample records
         (n, 2 (m+1)^2, 8) at 2 (t (m+1) + s) + merge, or None)."""
         import torch

@@ -356,7 +356,8 @@ def load_scene(source, base_dir=None):
 
 def render(loaded, seed=42, raster_mode=None):
     """Renders a load_scene result with its renderer node (pt, ptdirect,
-    ptmis, bdpt; vcm in bdpt mode) and the reference's parameter defaults.
+    ptmis, bdpt; vcm in its bdpt, vcm and bdpm modes) and the reference's
+    parameter defaults.
     Returns (film tensor, stats)."""
     from . import render as R
     rs = R.Scene(loaded["scene"])
@@ -372,10 +373,25 @@ def render(loaded, seed=42, raster_mode=None):
         film, st, _ = rs.render_bdpt(n, max_num_vertices=int(p.get("max_num_vertices", -1)),
                                      min_num_vertices=int(p.get("min_num_vertices", 0)), seed=seed, raster_mode=rm,
                                      mis=mis)
-    elif t == "vcm" and str(p.get("mode", "vcm")) == "bdpt":
-        film, st, _ = rs.render_vcm_bdpt(n, max_num_vertices=int(p.get("max_num_vertices", 10)),
-                                         min_num_vertices=int(p.get("min_num_vertices", 0)), seed=seed,
-                                         raster_mode=rm)
+    elif t == "vcm":
+        # renderer_vcm.cpp:558-577: every mode runs num_iteration_pass passes
+        # of num_eye_trace_samples eye samples and keeps their mean image
+        mode = str(p.get("mode", "vcm"))
+        passes = int(p.get("num_iteration_pass", 100))
+        ne = int(p.get("num_eye_trace_samples", 10000))
+        common = dict(max_num_vertices=int(p.get("max_num_vertices", 10)),
+                      min_num_vertices=int(p.get("min_num_vertices", 0)), seed=seed, raster_mode=rm)
+        if mode == "bdpt":
+            # independent samples: one render of passes x eye samples is the passes' mean
+            film, st, _ = rs.render_vcm_bdpt(passes * ne, **common)
+        elif mode in ("vcm", "bdpm"):
+            film, st, _ = rs.render_vcm(ne, mode=R.VCM_MODE_VCM if mode == "vcm" else R.VCM_MODE_BDPM,
+                                        num_iteration_pass=passes,
+                                        num_photon_trace_samples=int(p.get("num_photon_trace_samples", 10000)),
+                                        initial_radius=float(p.get("initial_radius", 0.1)),
+                                        alpha=float(p.get("alpha", 0.7)), **common)
+        else:
+            raise SceneFileError(f"renderer::vcm mode {mode!r} is unknown")
     else:
         raise SceneFileError(f"renderer::{t} is not supported here")
     return film, st

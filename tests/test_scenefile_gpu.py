@@ -10,7 +10,7 @@ from lmx import image, meshfile, render, scenefile, scenes
 pytestmark = pytest.mark.gpu
 
 
-def _cornell_yaml(sc, spp):
+def _cornell_yaml(sc, spp, renderer=None):
     cam = sc["camera"]
     lines = ["lightmetrica:", "  version: 1.1.0", "  assets:", "    type: assets3", "    params:",
              "      film_1:", "        interface: film", "        type: hdr",
@@ -43,7 +43,8 @@ def _cornell_yaml(sc, spp):
             nodes.append(f"          light: light_{mi}")
     n = cam["width"] * cam["height"] * spp
     lines += ["  scene:", "    type: scene3", "    params:", "      sensor: cam", "      nodes:"] + nodes
-    lines += ["  renderer:", "    type: pt", "    params:", f"      num_samples: {n}", "      max_num_vertices: 9"]
+    lines += renderer or ["  renderer:", "    type: pt", "    params:", f"      num_samples: {n}",
+                          "      max_num_vertices: 9"]
     return "\n".join(lines) + "\n"
 
 
@@ -71,3 +72,27 @@ def test_scene_file_renders_the_same_paths(gpu, tmp_path):
     np.testing.assert_allclose(img.mean(), ref.mean(), rtol=1e-2)
     m = ref.max(axis=2, keepdims=True)
     assert (np.abs(img - ref) <= m * 2.0 ** -7 + 1e-6).mean() > 0.99
+
+
+@pytest.mark.parametrize("mode", ["vcm", "bdpm", "bdpt"])
+def test_scene_file_vcm_renderer_node(gpu, mode):
+    """renderer::vcm's node (renderer_vcm.cpp:558-577): every mode renders
+    num_iteration_pass passes of num_eye_trace_samples eye samples."""
+    sc = scenes.cornell_box(32, 32)
+    node = ["  renderer:", "    type: vcm", "    params:", f"      mode: {mode}", "      num_iteration_pass: 3",
+            "      num_eye_trace_samples: 8192", "      num_photon_trace_samples: 5000",
+            "      initial_radius: 0.05", "      alpha: 0.5", "      max_num_vertices: 6"]
+    loaded = scenefile.load_scene(_cornell_yaml(sc, 1, node))
+    film, st = scenefile.render(loaded, seed=11)
+    rs = render.Scene(sc)
+    kw = dict(max_num_vertices=6, seed=11, raster_mode=render.RASTER_RANDOM)
+    if mode == "bdpt":
+        ref = rs.render_vcm_bdpt(3 * 8192, **kw)[0]
+    else:
+        ref = rs.render_vcm(8192, mode=render.VCM_MODE_VCM if mode == "vcm" else render.VCM_MODE_BDPM,
+                            num_iteration_pass=3, num_photon_trace_samples=5000, initial_radius=0.05, alpha=0.5,
+                            **kw)[0]
+    a, b = film.cpu().numpy(), ref.cpu().numpy()
+    assert a.mean() > 0
+    # the same samples; splat order differs between the two renders
+    np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-6 * float(b.max()))
