@@ -217,6 +217,25 @@ void orc_vcm_samples(const orc_scene *s, const orc_vcm_params *p, const float *t
                      const float *ptable, uint32_t ptable_width, uint64_t first_sample, uint64_t n, float *out,
                      int nthreads);
 
+/* renderer::pm (orc_render.c): num_photons photon subpaths (uniform rows of
+ * ptable, or Philox with pseed), then eye samples first_sample .. +n (table
+ * or seed; renderer::vcm's eye layout).  max_num_vertices -1 (unbounded) or
+ * >= 2.  Output per eye sample 16 floats: raster x, y, emission C rgb,
+ * emission count, density-estimate C rgb, photons gathered, vertex count at
+ * the gather vertex (0: none), 0 x 5 (before the W H / N film scale). */
+typedef struct orc_pm_params {
+    int32_t max_num_vertices;
+    uint32_t seed_lo, seed_hi;
+    uint32_t pseed_lo, pseed_hi;
+    int32_t raster_mode;
+    int32_t finalgather;
+    uint32_t num_photons;
+    float radius;
+} orc_pm_params;
+void orc_pm_samples(const orc_scene *s, const orc_pm_params *p, const float *table, uint32_t table_width,
+                    const float *ptable, uint32_t ptable_width, uint64_t first_sample, uint64_t n, float *out,
+                    int nthreads);
+
 /* renderer::bdpt (orc_render.c).  max_num_vertices: -1 (unbounded) or >= 2;
  * min_num_vertices applies only when bounded (renderer_bdpt.cpp:887);
  * mis: 0 power heuristic, 1 simple.  Output per sample: 2 (M+1)^2 records of

@@ -215,6 +215,12 @@ class orc_vcm_params(ctypes.Structure):
                 ("mode", ctypes.c_int32), ("num_photons", ctypes.c_uint32), ("radius", ctypes.c_float)]
 
 
+class orc_pm_params(ctypes.Structure):
+    _fields_ = [("max_num_vertices", ctypes.c_int32), ("seed_lo", ctypes.c_uint32), ("seed_hi", ctypes.c_uint32),
+                ("pseed_lo", ctypes.c_uint32), ("pseed_hi", ctypes.c_uint32), ("raster_mode", ctypes.c_int32),
+                ("finalgather", ctypes.c_int32), ("num_photons", ctypes.c_uint32), ("radius", ctypes.c_float)]
+
+
 VCM_VCM = 0
 VCM_BDPM = 2
 
@@ -243,6 +249,9 @@ def _render_sigs(so):
     so.orc_vcm_samples.argtypes = [vp, ctypes.POINTER(orc_vcm_params), vp, ctypes.c_uint32, vp, ctypes.c_uint32,
                                    ctypes.c_uint64, ctypes.c_uint64, vp, ctypes.c_int]
     so.orc_vcm_samples.restype = None
+    so.orc_pm_samples.argtypes = [vp, ctypes.POINTER(orc_pm_params), vp, ctypes.c_uint32, vp, ctypes.c_uint32,
+                                  ctypes.c_uint64, ctypes.c_uint64, vp, ctypes.c_int]
+    so.orc_pm_samples.restype = None
     so.orc_film_rgbe.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
     so.orc_film_rgbe.restype = None
     so.orc_film_png.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
@@ -428,6 +437,25 @@ class Scene:
             ptable = np.ascontiguousarray(ptable, np.float32)
             ptw, pp = ptable.shape[1], ctypes.c_void_p(ptable.ctypes.data)
         lib().orc_vcm_samples(self._h, ctypes.byref(p), tp, tw, pp, ptw, first, n, _p(out), nthreads)
+        return out
+
+    def pm_samples(self, first, n, radius, num_photons, max_num_vertices=-1, finalgather=1, seed=42, pseed=43,
+                   raster_mode=RASTER_STRATIFIED, table=None, ptable=None, nthreads=8):
+        """renderer::pm: (n, 16) float32 per eye sample: raster x, y, emission
+        C rgb, emission count, density C rgb, photons gathered, vertex count
+        at the gather vertex, 0 x 5."""
+        p = orc_pm_params(max_num_vertices, seed & 0xFFFFFFFF, seed >> 32, pseed & 0xFFFFFFFF, pseed >> 32,
+                          raster_mode, finalgather, num_photons, radius)
+        out = np.zeros((n, 16), np.float32)
+        tw = ptw = 0
+        tp = pp = None
+        if table is not None:
+            table = np.ascontiguousarray(table, np.float32)
+            tw, tp = table.shape[1], ctypes.c_void_p(table.ctypes.data)
+        if ptable is not None:
+            ptable = np.ascontiguousarray(ptable, np.float32)
+            ptw, pp = ptable.shape[1], ctypes.c_void_p(ptable.ctypes.data)
+        lib().orc_pm_samples(self._h, ctypes.byref(p), tp, tw, pp, ptw, first, n, _p(out), nthreads)
         return out
 
     def film(self, samples):

@@ -1860,3 +1860,303 @@ void orc_ptnee_samples(const orc_scene *s, const orc_pt_params *pp, int mode, co
     for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
     free(b.tri_light);
 }
+
+/* ---------------------------------------------------------------------------
+ * renderer::pm (renderer_pm.cpp:61-220) with photonmap::kdtree / naive
+ * (photonmap_kdtree.cpp:123-165, photonmap_naive.cpp:45-56: every photon
+ * with |p - x|^2 < r^2), restated literally through TraceSubpath_
+ * (subpathsampler.cpp:42-203):
+ *  - photon subpath i: an emitter point and direction; the callback of
+ *    every vertex stores a photon at a diffuse or glossy vertex (position,
+ *    throughput at the vertex, direction to the previous vertex, vertex count)
+ *    and then plays roulette 0.5 (the endpoint included), the survivor's
+ *    throughput doubled.  Uniform row of the photon: 1-2 position, 3 emitter
+ *    selection, 4-5 direction, 6 roulette after the endpoint; vertex j >= 1
+ *    owns ORC_PT_SEG0 + 8 (j - 1) .. +7: 0-1 the direction sampled at vertex
+ *    j - 1 (j >= 2), 3 the roulette after vertex j;
+ *  - eye sample: the eye subpath of renderer::vcm's layout without roulette;
+ *    at every vertex past the camera an emitter adds throughput x Le, and the
+ *    first (finalgather 0) or second (finalgather 1) diffuse or glossy vertex
+ *    gathers the photons within r with the kernel 3/pi (1 - d^2/r^2)^2 over
+ *    r^2 N, times the BSDF towards the photon's incoming direction and the
+ *    photon's throughput, and ends the path.
+ * The photon set is found by a uniform grid (the same set as the kd-tree).
+ * max_num_vertices -1 is unbounded (subpaths cut at 32 vertices) and then
+ * gathers every photon (the reference compares the path length with -1 and
+ * so drops every photon). */
+#define PM_CAP 32
+
+typedef struct {
+    v3 p, wi;
+    float thr[3];
+    int nv;
+} pm_photon;
+
+static int pm_trace_photon(const bctx *b, int maxv, const usrc *us, uint64_t i, bvert *vs)
+{
+    const orc_scene *s = b->s;
+    int nv = 0;
+    v3 init_wo = mk(0, 0, 0);
+    float run[3] = {0.0f, 0.0f, 0.0f};
+    for (int step = 0; (maxv == -1 || step < maxv) && step < PM_CAP; step++) {
+        if (step == 0) {
+            if (s->d.nlight == 0) return 0;
+            bvert v;
+            memset(&v, 0, sizeof v);
+            const float uEP[2] = {U(us, i, 1), U(us, i, 2)};
+            const float uEPC = U(us, i, 3);
+            const float uED[2] = {U(us, i, 4), U(us, i, 5)};
+            int l = (int)(uEPC * (float)s->d.nlight);
+            l = l < 0 ? 0 : (l > (int)s->d.nlight - 1 ? (int)s->d.nlight - 1 : l);
+            sample_light_point(b, l, uEP, &v);
+            float lw[3];
+            orc_sample_cosine_hemisphere(uED, lw);
+            init_wo = to_world(&v.fr, mk(lw[0], lw[1], lw[2]));
+            const float a = 1.0f / pos_pdf(b, &v);
+            v.alpha[0] = v.alpha[1] = v.alpha[2] = a;
+            vs[nv++] = v;
+            if (U(us, i, 6) > 0.5f) break;                   /* renderer_pm.cpp:107-116 */
+            for (int c = 0; c < 3; c++) run[c] = v.alpha[c] / 0.5f;
+            continue;
+        }
+        const uint32_t sb = ORC_PT_SEG0 + 8u * (uint32_t)(step - 1);
+        bvert *pv = &vs[nv - 1];
+        v3 wi, wo;
+        if (step == 1) {
+            wi = mk(0, 0, 0);
+            wo = init_wo;
+        } else {
+            wi = dir_to(pv, &vs[nv - 2]);
+            const float uD[2] = {U(us, i, sb), U(us, i, sb + 1)};
+            if (pv->kind != BV_SURFACE || !bsdf_sample(pv->mat, &pv->fr, wi, uD, &wo)) break;
+        }
+        float fs[3];
+        v_eval(b, pv, wi, wo, 1, fs);
+        if (fs[0] == 0.0f && fs[1] == 0.0f && fs[2] == 0.0f) break;
+        const float pdf = v_pdf(b, pv, wi, wo);
+        orc_hit h;
+        if (!scene_intersect(s, pv->fr.p, wo, &h)) break;
+        bvert v;
+        memset(&v, 0, sizeof v);
+        surface_at(s, pv->fr.p, wo, &h, &v.fr);
+        v.kind = BV_SURFACE; v.degenerate = 0; v.tri = h.id;
+        v.mat = &s->d.mats[s->d.tri_mat[h.id]];
+        v.light = b->tri_light[h.id];
+        for (int c = 0; c < 3; c++) v.alpha[c] = run[c] * (fs[c] / pdf);
+        vs[nv++] = v;
+        if (U(us, i, sb + 3) > 0.5f) break;
+        for (int c = 0; c < 3; c++) run[c] = v.alpha[c] / 0.5f;
+    }
+    return nv;
+}
+
+static int pm_scatters(const orc_material *m)
+{
+    return m && (m->type == ORC_MAT_DIFFUSE || m->type == ORC_MAT_PHONG || m->type == ORC_MAT_GGX);
+}
+
+typedef struct {
+    const bctx *b;
+    const orc_pm_params *pp;
+    usrc us;
+    const pm_photon *ph;
+    float lo[3], cell;
+    int dim[3];
+    int *cstart, *items;
+    float *out;
+    uint64_t first;
+} pm_ctx;
+
+static void pm_sample(const pm_ctx *c, uint64_t sample, float *out)
+{
+    const bctx *b = c->b;
+    const orc_pm_params *pp = c->pp;
+    const float r = pp->radius;
+    memset(out, 0, sizeof(float) * 16);
+    int cap = pp->max_num_vertices == -1 ? PM_CAP : pp->max_num_vertices;
+    if (cap > 2 + pp->finalgather) cap = 2 + pp->finalgather;     /* the path ends at the gather vertex */
+    orc_bdpt_params bp = {cap, 0, pp->seed_lo, pp->seed_hi, pp->raster_mode};
+    bvert E[PM_CAP];
+    const int nE = trace_subpath(b, &bp, &c->us, sample, 0, 0, E);
+    float rx = 0.0f, ry = 0.0f;
+    if (pp->raster_mode == ORC_RASTER_STRATIFIED) {
+        const uint32_t npix = (uint32_t)(b->s->d.width * b->s->d.height);
+        const uint32_t pix = (uint32_t)(sample % (uint64_t)npix);
+        rx = ((float)(pix % (uint32_t)b->s->d.width) + 0.5f) / (float)b->s->d.width;
+        ry = ((float)(pix / (uint32_t)b->s->d.width) + 0.5f) / (float)b->s->d.height;
+    } else if (nE >= 2) {
+        if (!camera_raster(&b->cam, dir_to(&E[0], &E[1]), &rx, &ry)) return;
+    }
+    out[0] = rx; out[1] = ry;
+    int gather_next = !pp->finalgather;
+    for (int k = 1; k < nE; k++) {
+        const bvert *v = &E[k], *pv = &E[k - 1];
+        /* an emitter hit: throughput x Le towards the previous vertex (:170-180) */
+        if (v->light >= 0) {
+            bvert lv = *v;
+            lv.kind = BV_LIGHT;
+            float le[3];
+            v_eval(b, &lv, mk(0, 0, 0), dir_to(v, pv), 0, le);
+            for (int q = 0; q < 3; q++) out[2 + q] += v->alpha[q] * le[q];
+            out[5] += 1.0f;
+        }
+        if (!pm_scatters(v->mat)) continue;
+        if (!gather_next) { gather_next = 1; continue; }
+        /* density estimation (:183-208) */
+        out[10] = (float)(k + 1);
+        int lo[3], hi[3];
+        const float q3[3] = {v->fr.p.x, v->fr.p.y, v->fr.p.z};
+        for (int a = 0; a < 3; a++) {
+            lo[a] = (int)floorf((q3[a] - r - c->lo[a]) / c->cell);
+            hi[a] = (int)floorf((q3[a] + r - c->lo[a]) / c->cell);
+            if (lo[a] < 0) lo[a] = 0;
+            if (hi[a] > c->dim[a] - 1) hi[a] = c->dim[a] - 1;
+        }
+        const v3 wiE = dir_to(v, pv);
+        for (int gz = lo[2]; gz <= hi[2]; gz++)
+            for (int gy = lo[1]; gy <= hi[1]; gy++)
+                for (int gx = lo[0]; gx <= hi[0]; gx++) {
+                    const int cidx = (gz * c->dim[1] + gy) * c->dim[0] + gx;
+                    for (int it = c->cstart[cidx]; it < c->cstart[cidx + 1]; it++) {
+                        const pm_photon *ph = &c->ph[c->items[it]];
+                        const v3 d = vsub(ph->p, v->fr.p);
+                        const float d2 = vdot(d, d);
+                        if (!(d2 < r * r)) continue;
+                        if (pp->max_num_vertices != -1 && (k + 1) + ph->nv - 1 > pp->max_num_vertices) continue;
+                        const float sk = 1.0f - d2 / r / r;
+                        const float kern = 3.0f * INV_PI_F * sk * sk;
+                        const float p = kern / (r * r * (float)pp->num_photons);
+                        float f[3];
+                        v_eval(b, v, wiE, ph->wi, 0, f);
+                        for (int q = 0; q < 3; q++) out[6 + q] += v->alpha[q] * p * f[q] * ph->thr[q];
+                        out[9] += 1.0f;
+                    }
+                }
+        break;
+    }
+}
+
+typedef struct { pm_ctx *c; uint64_t b, e; } pm_job;
+
+static void *pm_run(void *p)
+{
+    pm_job *j = (pm_job *)p;
+    for (uint64_t i = j->b; i < j->e; i++) pm_sample(j->c, j->c->first + i, &j->c->out[16 * i]);
+    return NULL;
+}
+
+typedef struct {
+    const bctx *b;
+    int maxv;
+    const usrc *pu;
+    bvert *vs;              /* PM_CAP per photon subpath */
+    int *nv;
+    uint32_t lo, hi;
+} pm_pjob;
+
+static void *pm_photons(void *p)
+{
+    pm_pjob *j = (pm_pjob *)p;
+    for (uint32_t i = j->lo; i < j->hi; i++)
+        j->nv[i] = pm_trace_photon(j->b, j->maxv, j->pu, i, &j->vs[(size_t)i * PM_CAP]);
+    return NULL;
+}
+
+void orc_pm_samples(const orc_scene *s, const orc_pm_params *pp, const float *table, uint32_t table_width,
+                    const float *ptable, uint32_t ptable_width, uint64_t first_sample, uint64_t n, float *out,
+                    int nthreads)
+{
+    bctx b;
+    b.s = s;
+    camera_init(&s->d, &b.cam);
+    b.tri_light = (int *)malloc(sizeof(int) * (s->d.ntri ? s->d.ntri : 1));
+    for (uint32_t i = 0; i < s->d.ntri; i++) b.tri_light[i] = -1;
+    for (uint32_t l = 0; l < s->d.nlight; l++)
+        for (uint32_t k = 0; k < s->d.light_prims[2 * l + 1]; k++) b.tri_light[s->d.light_prims[2 * l] + k] = (int)l;
+    b.tri_cdf_base = NULL;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    const uint32_t np = pp->num_photons;
+    usrc pu;
+    pu.table = ptable; pu.width = ptable_width;
+    pu.seed = ((uint64_t)pp->pseed_hi << 32) | pp->pseed_lo;
+    pu.first = 0;
+    bvert *vs = (bvert *)malloc(sizeof(bvert) * ((size_t)np * PM_CAP + 1));
+    int *nv = (int *)calloc(np + 1, sizeof(int));
+    {
+        pthread_t th[256];
+        pm_pjob pj[256];
+        for (int t = 0; t < nthreads; t++) {
+            pj[t].b = &b; pj[t].maxv = pp->max_num_vertices; pj[t].pu = &pu; pj[t].vs = vs; pj[t].nv = nv;
+            pj[t].lo = (uint32_t)((uint64_t)np * (uint64_t)t / (uint64_t)nthreads);
+            pj[t].hi = (uint32_t)((uint64_t)np * (uint64_t)(t + 1) / (uint64_t)nthreads);
+            pthread_create(&th[t], NULL, pm_photons, &pj[t]);
+        }
+        for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    }
+    /* the photons: diffuse or glossy vertices past the light (:94-104) */
+    size_t count = 0;
+    for (uint32_t i = 0; i < np; i++)
+        for (int j = 1; j < nv[i]; j++) count += pm_scatters(vs[(size_t)i * PM_CAP + j].mat) ? 1 : 0;
+    pm_photon *ph = (pm_photon *)malloc(sizeof(pm_photon) * (count + 1));
+    size_t m = 0;
+    float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
+    for (uint32_t i = 0; i < np; i++)
+        for (int j = 1; j < nv[i]; j++) {
+            const bvert *v = &vs[(size_t)i * PM_CAP + j];
+            if (!pm_scatters(v->mat)) continue;
+            pm_photon *q = &ph[m++];
+            q->p = v->fr.p;
+            for (int c = 0; c < 3; c++) q->thr[c] = v->alpha[c];
+            q->wi = dir_to(v, &vs[(size_t)i * PM_CAP + j - 1]);
+            q->nv = j + 1;
+            const float a3[3] = {q->p.x, q->p.y, q->p.z};
+            for (int a = 0; a < 3; a++) { if (a3[a] < lo[a]) lo[a] = a3[a]; if (a3[a] > hi[a]) hi[a] = a3[a]; }
+        }
+    free(vs); free(nv);
+    pm_ctx c;
+    c.b = &b; c.pp = pp; c.ph = ph;
+    c.cell = pp->radius;
+    float span = 0.0f;
+    for (int a = 0; a < 3; a++) {
+        if (m == 0) { lo[a] = 0.0f; hi[a] = 0.0f; }
+        c.lo[a] = lo[a];
+        if (hi[a] - lo[a] > span) span = hi[a] - lo[a];
+    }
+    if (span / c.cell > 255.0f) c.cell = span / 255.0f;     /* a coarser grid finds the same set */
+    long long cells = 1;
+    for (int a = 0; a < 3; a++) { c.dim[a] = (int)floorf((hi[a] - lo[a]) / c.cell) + 1; cells *= c.dim[a]; }
+    c.cstart = (int *)calloc((size_t)cells + 1, sizeof(int));
+    c.items = (int *)malloc(sizeof(int) * (m + 1));
+    int *cellof = (int *)malloc(sizeof(int) * (m + 1));
+    for (size_t k = 0; k < m; k++) {
+        const float a3[3] = {ph[k].p.x, ph[k].p.y, ph[k].p.z};
+        int g[3];
+        for (int a = 0; a < 3; a++) {
+            g[a] = (int)floorf((a3[a] - c.lo[a]) / c.cell);
+            if (g[a] < 0) g[a] = 0;
+            if (g[a] > c.dim[a] - 1) g[a] = c.dim[a] - 1;
+        }
+        cellof[k] = (g[2] * c.dim[1] + g[1]) * c.dim[0] + g[0];
+        c.cstart[cellof[k] + 1]++;
+    }
+    for (long long k = 0; k < cells; k++) c.cstart[k + 1] += c.cstart[k];
+    int *fill = (int *)calloc((size_t)cells, sizeof(int));
+    for (size_t k = 0; k < m; k++) c.items[c.cstart[cellof[k]] + fill[cellof[k]]++] = (int)k;
+    free(fill); free(cellof);
+    c.us.table = table; c.us.width = table_width;
+    c.us.seed = ((uint64_t)pp->seed_hi << 32) | pp->seed_lo;
+    c.us.first = first_sample;
+    c.out = out;
+    c.first = first_sample;
+    pthread_t th[256];
+    pm_job jobs[256];
+    for (int t = 0; t < nthreads; t++) {
+        jobs[t].c = &c;
+        jobs[t].b = n * (uint64_t)t / (uint64_t)nthreads;
+        jobs[t].e = n * (uint64_t)(t + 1) / (uint64_t)nthreads;
+        pthread_create(&th[t], NULL, pm_run, &jobs[t]);
+    }
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    free(c.cstart); free(c.items); free(ph); free(b.tri_light);
+}
