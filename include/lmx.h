@@ -294,6 +294,39 @@ typedef struct lmx_vcm_params {
 int lmx_render_vcm(lmx_scene *scene, const lmx_render_params *params, const lmx_vcm_params *vcm, float *film_rgb,
                    void *stream, lmx_render_stats *stats);
 
+/* renderer::pm (renderer_pm.cpp:61-220) with photonmap::kdtree / naive
+ * (photonmap_kdtree.cpp:123-165: every photon within `radius`):
+ * num_photon_trace_samples photon (light) subpaths with roulette 0.5 after
+ * every vertex, the endpoint included, store a photon (position, throughput,
+ * direction to the previous vertex, vertex count) at every diffuse or glossy
+ * vertex; then params->num_samples eye samples, traced without roulette:
+ * every emitter vertex adds throughput x Le, and the first (finalgather 0)
+ * or second (finalgather 1, the reference default) diffuse or glossy vertex
+ * adds the density estimate -- kernel 3/pi (1 - d^2/r^2)^2 / (r^2 N) x BSDF
+ * towards the photon's direction x photon throughput over the photons whose
+ * merged path has at most max_num_vertices vertices -- and ends the path.
+ * max_num_vertices: -1 (unbounded, the reference default; subpaths cut at 32
+ * vertices, every photon gathered) or 2..32.  Film scaled by W H / N.
+ * Uniforms: eye sample i index i of `seed` in renderer::vcm's eye layout
+ * (table width >= 8 + 4 (min(M, 2 + finalgather) - 1)); photon i index i of
+ * photon_seed: 1-2 emitter position, 3 emitter selection, 4-5 direction,
+ * 6 roulette after the endpoint, vertex j >= 1 owns 8 + 8 (j - 1) .. +7: 0-1
+ * the direction sampled at vertex j - 1, 3 the roulette after vertex j (table
+ * width >= 8 + 8 (M - 1), M = max_num_vertices or 32).  sample_out: 16 floats
+ * per eye sample: raster x, y, emission C rgb, emitter vertices, density C
+ * rgb, photons gathered, vertex count at the gather vertex (0: none), 0 x 5
+ * (before the film scale). */
+typedef struct lmx_pm_params {
+    uint32_t num_photon_trace_samples;   /* 1 .. 2^26 (reference default 100000) */
+    int32_t finalgather;                 /* 0 or 1 (reference default 1) */
+    float radius;                        /* > 0 (reference default 0.01) */
+    uint64_t photon_seed;
+    const float *photon_table;           /* device, or NULL (Philox with photon_seed) */
+    uint32_t photon_table_width;
+} lmx_pm_params;
+int lmx_render_pm(lmx_scene *scene, const lmx_render_params *params, const lmx_pm_params *pm, float *film_rgb,
+                  void *stream, lmx_render_stats *stats);
+
 /* ---------------------------------------------------------------------------
  * Learned directional sampler.  No counterpart in the reference: a third
  * renderer beside renderer::pt and renderer::vcm, for the callers of

@@ -96,9 +96,10 @@ namespace {
 
 #define BD_VCM 0
 #define BD_BDPT 1
+#define BD_PM 2                // renderer::pm photon subpaths: roulette after every vertex, the endpoint included
 
 struct BParams {
-    int32_t variant;           // BD_VCM or BD_BDPT
+    int32_t variant;           // BD_VCM, BD_BDPT or BD_PM
     int32_t merging;           // vcm/bdpm: merge records, photon grid, merge kernel
     int32_t connect;           // connections and s = 0 strategies evaluated (all but bdpm)
     int32_t phase;             // merging: 0 eye samples, 1 photon subpaths
@@ -284,6 +285,12 @@ __global__ __launch_bounds__(256) void k_bd_gen(RScene s, BParams p, BCtl* __res
                 la = mk3(a0 * (m.Le[0] / lpdf), a0 * (m.Le[1] / lpdf), a0 * (m.Le[2] / lpdf));
                 light_ray = !(m.Le[0] == 0.0f && m.Le[1] == 0.0f && m.Le[2] == 0.0f);
             }
+            // renderer::pm: roulette after the endpoint (renderer_pm.cpp:107-116),
+            // the survivor's throughput doubled
+            if (p.variant == BD_PM && light_ray) {
+                if (U.one(b + 6) > 0.5f) light_ray = false;
+                else la = mk3(la.x * 2.0f, la.y * 2.0f, la.z * 2.0f);
+            }
         } else {
             vb.nsub[1][i] = 0;
         }
@@ -382,17 +389,17 @@ __global__ __launch_bounds__(256) void k_bd_shade(RScene s, BParams p, BCtl* __r
                 const uint32_t blk = kind ? p.wsub : 0u;
                 // bdpt: roulette after the vertex (renderer_bdpt.cpp:205-211),
                 // the survivors' throughput doubled (1 / SelectionPDF)
-                const bool rr_ok = p.variant != BD_BDPT || Un.one(blk + rb_step(k) + 3u) <= 0.5f;
+                const bool rr8 = p.variant == BD_BDPT || p.variant == BD_PM;
+                const bool rr_ok = !rr8 || Un.one(blk + rb_step(k) + 3u) <= 0.5f;
                 if ((int)k + 1 < p.maxv && is_scatterer(m) && rr_ok) {
                     const f3 wi = dir_to(fr.p, prev.p);
-                    const float4 u = Un.block(blk + (p.variant == BD_BDPT ? rb_step(k + 1u)
-                                                                           : LMX_PT_SLOT_SEG0 + 4u * (k - 1u)));
+                    const float4 u = Un.block(blk + (rr8 ? rb_step(k + 1u) : LMX_PT_SLOT_SEG0 + 4u * (k - 1u)));
                     f3 wo;
                     if (bsdf_sample(m, fr, wi, u.x, u.y, wo)) {
                         const f3 fs = bsdf_eval_t(m, fr, wi, wo, kind == 1);
                         if (!(fs.x == 0.0f && fs.y == 0.0f && fs.z == 0.0f)) {
                             const float pdf = bsdf_pdf(m, fr, wi, wo);
-                            const float sel = p.variant == BD_BDPT ? 2.0f : 1.0f;
+                            const float sel = rr8 ? 2.0f : 1.0f;
                             const f3 a2 = mk3(alpha.x * (fs.x / pdf) * sel, alpha.y * (fs.y / pdf) * sel,
                                               alpha.z * (fs.z / pdf) * sel);
                             o0 = make_float4(fr.p.x, fr.p.y, fr.p.z, LMX_EPS_ISECT);
@@ -830,7 +837,8 @@ __global__ void k_vc_grid_clear(Grid g)
 
 // one thread per (photon, vertex >= 1) slot; phase 0 counts, phase 1 scatters
 template <int PHASE>
-__global__ __launch_bounds__(256) void k_vc_grid_fill(BParams p, const BCtl* __restrict__ ctl, VB ph, Grid g)
+__global__ __launch_bounds__(256) void k_vc_grid_fill(RScene s, BParams p, const BCtl* __restrict__ ctl, VB ph,
+                                                      Grid g)
 {
     const uint32_t np = p.batch;
     const size_t n = (size_t)np * (size_t)(p.maxv - 1);
@@ -839,6 +847,8 @@ __global__ __launch_bounds__(256) void k_vc_grid_fill(BParams p, const BCtl* __r
         const uint32_t i = (uint32_t)(idx % np), j = 1u + (uint32_t)(idx / np);
         if (j >= ph.nsub[1][i]) continue;
         const float4 a = ph.v[1][((size_t)i * (size_t)p.maxv + j) * 4];
+        // renderer::pm stores photons at diffuse and glossy vertices only (renderer_pm.cpp:94-104)
+        if (p.variant == BD_PM && !is_scatterer(s.mats[__float_as_uint(a.w) >> 16])) continue;
         const uint32_t h = bucket_of(cell_of(a.x, cell), cell_of(a.y, cell), cell_of(a.z, cell), g.H);
         if (PHASE == 0) {
             atomicAdd(&g.start[h], 1u);
@@ -1007,6 +1017,114 @@ __global__ __launch_bounds__(256) void k_vc_merge(RScene s, BParams p, BCtl* __r
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) nmerge += __shfl_xor(nmerge, off);
     if ((threadIdx.x & 63u) == 0 && nmerge) atomicAdd(&ctl->merges, (unsigned long long)nmerge);
+}
+
+// renderer::pm's eye side (renderer_pm.cpp:150-210): one thread per eye
+// sample walks its subpath (traced without roulette up to the gather
+// vertex); an emitter vertex adds throughput x Le towards the previous vertex,
+// and the first (finalgather 0) or second (finalgather 1) diffuse or glossy
+// vertex gathers every photon within r: kernel 3/pi (1 - d^2/r^2)^2 over
+// r^2 N, times the BSDF towards the photon's incoming direction and the
+// photon's throughput, for photons whose merged path has at most
+// max_num_vertices vertices.  The gather alternates a list of up to
+// MERGE_LIST photons within r (distance tests only) with their evaluation.
+struct PMArgs {
+    int32_t pmaxv;             // photon record stride (vertices per photon subpath)
+    int32_t maxv;              // max_num_vertices, -1 unbounded
+    int32_t finalgather;
+    float nphoton;
+};
+
+__global__ __launch_bounds__(256) void k_pm_gather(RScene s, BParams p, BCtl* __restrict__ ctl, VB vb, VB ph,
+                                                   Grid g, PMArgs a)
+{
+    __shared__ uint32_t hl[MERGE_LIST * 256];
+    const float r = ctl->radius, cell = ctl->cell;
+    const float r2 = r * r;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.batch; i += gridDim.x * blockDim.x) {
+        const uint32_t sample = ctl->batch_first + i;
+        if (sample - ctl->pass_first >= p.total) continue;
+        const int nE = (int)vb.nsub[0][i];
+        const float2 eras = vb.eraster[i];
+        f3 Cem = mk3(0, 0, 0), Cde = mk3(0, 0, 0);
+        float nem = 0.0f, nph = 0.0f, gv = 0.0f;
+        bool gather_next = a.finalgather == 0;
+        for (int k = 1; k < nE; k++) {
+            const Vtx v = load_v(vb, p, 0, i, (uint32_t)k);
+            const float4 pq = vb.v[0][((size_t)i * (size_t)p.maxv + (size_t)(k - 1)) * 4];
+            const f3 toprev = dir_to(v.p, mk3(pq.x, pq.y, pq.z));
+            const RMat m = s.mats[v.mat];
+            if (v.light >= 0) {
+                if (emits_towards(v.fr, toprev))
+                    Cem = add(Cem, mk3(v.alpha.x * m.Le[0], v.alpha.y * m.Le[1], v.alpha.z * m.Le[2]));
+                nem += 1.0f;
+            }
+            if (!is_scatterer(m)) continue;
+            if (!gather_next) { gather_next = true; continue; }
+            gv = (float)(k + 1);
+            // range query: cells of edge 2r around v
+            int lo[3], nc[3];
+            const float q[3] = {v.p.x, v.p.y, v.p.z};
+#pragma unroll
+            for (int ax = 0; ax < 3; ax++) {
+                lo[ax] = cell_of(q[ax] - r, cell);
+                nc[ax] = cell_of(q[ax] + r, cell) - lo[ax] + 1;
+            }
+            const int ncell = nc[0] * nc[1] * nc[2];
+            // photon vertex j may join: (k + 1) + (j + 1) - 1 <= maxv
+            const int jmax = a.maxv == -1 ? 0x7fffffff : a.maxv - k - 1;
+            int c = 0;
+            uint32_t it = 0, end = 0;
+            bool fresh = true;
+            while (c < ncell) {
+                uint32_t nh = 0;
+                while (c < ncell && nh < MERGE_LIST) {
+                    const int cx = lo[0] + c % nc[0], cy = lo[1] + (c / nc[0]) % nc[1], cz = lo[2] + c / (nc[0] * nc[1]);
+                    if (fresh) {
+                        const uint32_t h = bucket_of(cx, cy, cz, g.H);
+                        it = g.start[h];
+                        end = g.start[h + 1];
+                        fresh = false;
+                    }
+                    for (; it < end && nh < MERGE_LIST; it++) {
+                        const float4 cd = g.items[it];
+                        const int j = (int)(__float_as_uint(cd.w) % (uint32_t)a.pmaxv);
+                        if (j > jmax) continue;
+                        if (cell_of(cd.x, cell) != cx || cell_of(cd.y, cell) != cy || cell_of(cd.z, cell) != cz) continue;
+                        const f3 d = sub(mk3(cd.x, cd.y, cd.z), v.p);
+                        if (!(dot(d, d) < r2)) continue;
+                        hl[nh * 256u + threadIdx.x] = __float_as_uint(cd.w);
+                        nh++;
+                    }
+                    if (it >= end) { c++; fresh = true; }
+                }
+                for (uint32_t h = 0; h < nh; h++) {
+                    const uint32_t id = hl[h * 256u + threadIdx.x];
+                    const float4* rec = ph.v[1] + (size_t)id * 4;          // photon vertex j of subpath id / pmaxv
+                    const float4 r0 = rec[0], r3 = rec[3], rp = rec[-4];   // its position, throughput; vertex j - 1
+                    const f3 pp = mk3(r0.x, r0.y, r0.z);
+                    const f3 d = sub(pp, v.p);
+                    const float sk = 1.0f - dot(d, d) / r / r;
+                    const float kern = 3.0f * LMX_INV_PI_F * sk * sk;
+                    const float pden = kern / (r * r * a.nphoton);
+                    const f3 f = bsdf_eval_t(m, v.fr, toprev, dir_to(pp, mk3(rp.x, rp.y, rp.z)), false);
+                    Cde = add(Cde, mk3(v.alpha.x * pden * f.x * r3.x, v.alpha.y * pden * f.y * r3.y,
+                                       v.alpha.z * pden * f.z * r3.z));
+                    nph += 1.0f;
+                }
+            }
+            break;
+        }
+        const f3 C = add(Cem, Cde);
+        if (!(C.x == 0.0f && C.y == 0.0f && C.z == 0.0f)) splat(p, vb.epix[i], C);
+        if (p.sample_out) {
+            float4* o = reinterpret_cast<float4*>(p.sample_out + 16 * (size_t)sample);
+            o[0] = make_float4(eras.x, eras.y, Cem.x, Cem.y);
+            o[1] = make_float4(Cem.z, nem, Cde.x, Cde.y);
+            o[2] = make_float4(Cde.z, nph, gv, 0.0f);
+            o[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    }
 }
 
 __global__ void k_bd_zero(float* a, size_t n)
@@ -1346,11 +1464,11 @@ int vc_enqueue_photons(lmx_scene* sc, const BParams& pp, hipStream_t st)
     const size_t nslot = (size_t)np * (size_t)(pp.maxv - 1);
     const int fgrid = std::max(1, (int)std::min<size_t>((nslot + 255) / 256, (size_t)cap_grid));
     k_vc_grid_clear<<<std::max(1, std::min((int)((n1 + 255) / 256), cap_grid)), 256, 0, st>>>(g);
-    k_vc_grid_fill<0><<<fgrid, 256, 0, st>>>(pp, ctl, ph, g);
+    k_vc_grid_fill<0><<<fgrid, 256, 0, st>>>(sc->rs, pp, ctl, ph, g);
     k_scan_tiles<<<ntile, 256, 0, st>>>(g.start, n1, g.part);
     k_scan_part<<<1, 256, 0, st>>>(g.part, ntile);
     k_scan_add<<<std::max(1, std::min((int)((n1 + 255) / 256), cap_grid)), 256, 0, st>>>(g.start, n1, g.part);
-    k_vc_grid_fill<1><<<fgrid, 256, 0, st>>>(pp, ctl, ph, g);
+    k_vc_grid_fill<1><<<fgrid, 256, 0, st>>>(sc->rs, pp, ctl, ph, g);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? LMX_OK : hip_error(e, "vcm photon launch");
 }
@@ -1489,6 +1607,146 @@ int render_vcm(lmx_scene* sc, const lmx_render_params* p, const lmx_vcm_params* 
     return rc;
 }
 
+// renderer::pm's eye batch: eye endpoints, subpaths up to the gather
+// vertex, the gather kernel, advance.
+int pm_enqueue_eye(lmx_scene* sc, const BParams& bp, const PMArgs& pa, hipStream_t st)
+{
+    const uint32_t B = bp.batch;
+    VB vb;
+    vb.v[0] = sc->bd_vert[0].p; vb.v[1] = sc->bd_vert[1].p;
+    vb.nsub[0] = sc->bd_nsub[0].p; vb.nsub[1] = sc->bd_nsub[1].p;
+    vb.epix = sc->bd_epix.p; vb.eraster = sc->bd_eraster.p;
+    vb.V = sc->bd_V.p;
+    vb.mq = nullptr;
+    VB ph = vb;
+    ph.v[1] = sc->vc_vert.p; ph.nsub[1] = sc->vc_nsub.p;
+    Queue q[2];
+    for (int k = 0; k < 2; k++) { q[k].ray = sc->bd_ray[k].p; q[k].thr = sc->bd_thr[k].p; q[k].meta = sc->bd_meta[k].p; }
+    BCtl* ctl = sc->bd_ctl.p;
+    const int cap_grid = sc->cus * 8;
+    k_bd_gen<<<(B + 255) / 256, 256, 0, st>>>(sc->rs, bp, ctl, vb, q[0]);
+    int a = 0;
+    for (int it = 0; it < bp.maxv - 1; it++) {
+        int rc = launch_trace(sc->accel, reinterpret_cast<const WRay*>(q[a].ray), 0, &ctl->n_trace, B,
+                              reinterpret_cast<WHit*>(sc->bd_hits.p), st, false);
+        if (rc != LMX_OK) return rc;
+        k_bd_shade<<<(B + 255) / 256, 256, 0, st>>>(sc->rs, bp, ctl, q[a], sc->bd_hits.p, q[1 - a], vb);
+        k_bd_commit<<<1, 64, 0, st>>>(ctl);
+        a = 1 - a;
+    }
+    const Grid g{sc->vc_items.p, sc->vc_start.p, sc->vc_cursor.p, sc->vc_part.p, sc->vc_H};
+    k_pm_gather<<<std::max(1, std::min((int)((B + 255) / 256), cap_grid * 2)), 256, 0, st>>>(sc->rs, bp, ctl, vb,
+                                                                                             ph, g, pa);
+    k_bd_next_batch<<<1, 64, 0, st>>>(ctl, B);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? LMX_OK : hip_error(e, "pm launch");
+}
+
+// renderer::pm: the photon graph (roulette light subpaths, the grid over
+// their diffuse and glossy vertices), then the eye batches; film scaled by
+// W H / N as the scheduler does (scheduler.cpp:281-288).
+int render_pm(lmx_scene* sc, const lmx_render_params* p, const lmx_pm_params* v, float* film_rgb, void* stream,
+              lmx_render_stats* stats)
+{
+    const char* name = "lmx_render_pm";
+    char msg[160];
+    if (!sc || !p || !v || !film_rgb) return set_error(LMX_EINVAL, "lmx_render_pm: NULL argument");
+    if (p->num_samples == 0 || p->num_samples > 0xffffffffull)
+        return set_error(LMX_EINVAL, "lmx_render_pm: num_samples must be in [1, 2^32)");
+    const bool bounded = p->max_num_vertices != -1;
+    if (bounded && (p->max_num_vertices < 2 || p->max_num_vertices > 32))
+        return set_error(LMX_EINVAL, "lmx_render_pm: max_num_vertices must be in [2, 32] or -1");
+    if (v->finalgather != 0 && v->finalgather != 1) return set_error(LMX_EINVAL, "lmx_render_pm: finalgather must be 0 or 1");
+    if (v->num_photon_trace_samples == 0 || v->num_photon_trace_samples > (1u << 26))
+        return set_error(LMX_EINVAL, "lmx_render_pm: num_photon_trace_samples must be in [1, 2^26]");
+    if (!(v->radius > 0.0f) || !(v->radius < 3.0e38f))
+        return set_error(LMX_EINVAL, "lmx_render_pm: radius must be positive and finite");
+    if (sc->mats.n > VB_MAX_MATERIALS || sc->lights.n > VB_MAX_LIGHTS)
+        return set_error(LMX_EINVAL, "lmx_render_pm: at most 65536 materials and 8190 lights");
+    if (p->raster_mode != LMX_RASTER_RANDOM && p->raster_mode != LMX_RASTER_STRATIFIED)
+        return set_error(LMX_EINVAL, "lmx_render_pm: unknown raster_mode");
+    const uint32_t pcap = bounded ? (uint32_t)p->max_num_vertices : 32u;
+    const uint32_t ecap = std::min<uint32_t>(pcap, 2u + (uint32_t)v->finalgather);
+    const uint32_t ewidth = LMX_PT_SLOT_SEG0 + 4u * (ecap - 1u);
+    const uint32_t pwidth = LMX_PT_SLOT_SEG0 + 8u * (pcap - 1u);
+    if ((p->uniform_table && p->table_width < ewidth) || (v->photon_table && v->photon_table_width < pwidth)) {
+        snprintf(msg, sizeof msg, "%s: uniform table narrower than %u (eye) / %u (photon)", name, ewidth, pwidth);
+        return set_error(LMX_EINVAL, msg);
+    }
+    hipStream_t st = (hipStream_t)stream;
+    hipStream_t own = nullptr;
+    hipError_t e;
+    if (!st) {
+        if ((e = hipStreamCreateWithFlags(&own, hipStreamNonBlocking)) != hipSuccess) return hip_error(e, "stream create");
+        st = own;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t N = (uint32_t)p->num_samples, np = v->num_photon_trace_samples;
+    BParams bp;                          // eye samples
+    bp.variant = BD_VCM;
+    bp.merging = 0; bp.connect = 0; bp.phase = 0; bp.c_conn = 1.0f;
+    bp.mis = LMX_MIS_POWER;
+    bp.bounded = 1;
+    bp.maxv = (int)ecap; bp.minv = 0;
+    bp.key = make_uint2((uint32_t)p->seed, (uint32_t)(p->seed >> 32));
+    bp.total = N;
+    bp.raster_mode = p->raster_mode;
+    bp.table = p->uniform_table; bp.table_width = p->table_width;
+    bp.wsub = ewidth;
+    bp.sample_out = p->sample_out;
+    bp.film = film_rgb;
+    uint32_t B = p->queue_capacity ? p->queue_capacity : (1u << 21);
+    B = std::min(B, N);
+    bp.batch = B;
+    BParams pp = bp;                     // photon subpaths
+    pp.variant = BD_PM;
+    pp.phase = 1;
+    pp.bounded = bounded ? 1 : 0;
+    pp.maxv = (int)pcap;
+    pp.batch = np; pp.total = np;
+    pp.key = make_uint2((uint32_t)v->photon_seed, (uint32_t)(v->photon_seed >> 32));
+    pp.table = v->photon_table; pp.table_width = v->photon_table_width;
+    pp.wsub = 0;
+    pp.sample_out = nullptr;
+    PMArgs pa{(int32_t)pcap, bounded ? p->max_num_vertices : -1, v->finalgather, (float)np};
+    int rc = bd_buffers(sc, std::max(B, (np + 1) / 2), ecap, 1);
+    if (rc == LMX_OK) rc = vc_buffers(sc, np, pcap, 1);
+    const size_t npix = (size_t)sc->width * (size_t)sc->height;
+    const int zgrid = std::max(1, std::min((int)((3 * npix + 255) / 256), sc->cus * 8));
+    hipGraphExec_t gph = nullptr, geye = nullptr;
+    if (rc == LMX_OK) {
+        k_bd_zero<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix);
+        k_bd_init<<<1, 64, 0, st>>>(sc->bd_ctl.p);
+        rc = capture(st, &gph, [&] { return vc_enqueue_photons(sc, pp, st); });
+    }
+    if (rc == LMX_OK) rc = capture(st, &geye, [&] { return pm_enqueue_eye(sc, bp, pa, st); });
+    const float r = v->radius;
+    if (rc == LMX_OK) {
+        k_bd_phase<<<1, 64, 0, st>>>(sc->bd_ctl.p, 0u, r, (float)(2.0 * (double)r), 0.0f);
+        if ((e = hipGraphLaunch(gph, st)) != hipSuccess) rc = hip_error(e, "graph launch");
+    }
+    const uint64_t nbatch = ((uint64_t)N + B - 1) / B;
+    if (rc == LMX_OK) k_bd_phase<<<1, 64, 0, st>>>(sc->bd_ctl.p, 0u, r, (float)(2.0 * (double)r), 0.0f);
+    for (uint64_t b = 0; rc == LMX_OK && b < nbatch; b++)
+        if ((e = hipGraphLaunch(geye, st)) != hipSuccess) rc = hip_error(e, "graph launch");
+    if (rc == LMX_OK) {
+        k_bd_scale<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix, (float)((double)npix / (double)N));
+        if ((e = hipMemcpyAsync(sc->bd_hctl, sc->bd_ctl.p, sizeof(BCtl), hipMemcpyDeviceToHost, st)) != hipSuccess ||
+            (e = hipStreamSynchronize(st)) != hipSuccess)
+            rc = hip_error(e, "lmx_render_pm: sync");
+    }
+    if (gph) (void)hipGraphExecDestroy(gph);
+    if (geye) (void)hipGraphExecDestroy(geye);
+    if (own) (void)hipStreamDestroy(own);
+    if (stats) {
+        stats->rays = rc == LMX_OK ? sc->bd_hctl->rays : 0;
+        stats->iterations = (uint32_t)nbatch;
+        stats->queue_capacity = B;
+        stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1508,6 +1766,12 @@ int lmx_render_vcm(lmx_scene* sc, const lmx_render_params* p, const lmx_vcm_para
                    lmx_render_stats* stats)
 {
     return render_vcm(sc, p, v, film_rgb, stream, stats);
+}
+
+int lmx_render_pm(lmx_scene* sc, const lmx_render_params* p, const lmx_pm_params* v, float* film_rgb, void* stream,
+                  lmx_render_stats* stats)
+{
+    return render_pm(sc, p, v, film_rgb, stream, stats);
 }
 
 }  // extern "C"

@@ -13,7 +13,7 @@ import ctypes
 import numpy as np
 
 from ._lib import (check, lib, lmx_camera, lmx_material, lmx_render_params, lmx_render_stats,
-                   lmx_scene_desc, lmx_vcm_params)
+                   lmx_scene_desc, lmx_pm_params, lmx_vcm_params)
 from .scenes import camera_frame
 
 RASTER_RANDOM = 0
@@ -39,6 +39,12 @@ def table_width(max_num_vertices, nee=False):
 def vcm_bdpt_table_width(max_num_vertices):
     """Uniform-table width of renderer::vcm's bdpt mode (eye + light block)."""
     return 2 * table_width(max_num_vertices)
+
+
+def pm_table_widths(max_num_vertices, finalgather=1):
+    """Uniform-table widths of renderer::pm: (eye, photon); -1 = 32 vertices."""
+    m = 32 if max_num_vertices == -1 else max_num_vertices
+    return table_width(min(m, 2 + finalgather)), table_width(m, nee=True)
 
 
 def bdpt_table_width(max_num_vertices):
@@ -172,6 +178,44 @@ class Scene:
         check(lib.lmx_render_vcm(self._h, ctypes.byref(p), ctypes.byref(v), ctypes.c_void_p(film.data_ptr()),
                                  ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st)))
         stats = {"rays": st.rays, "passes": st.iterations, "batch": st.queue_capacity, "ms": st.ms}
+        return film, stats, so
+
+    def render_pm(self, num_samples, num_photon_trace_samples=100000, finalgather=1, radius=0.01,
+                  max_num_vertices=-1, seed=42, photon_seed=None, raster_mode=RASTER_STRATIFIED, batch=0,
+                  table=None, photon_table=None, sample_out=False, stream=None):
+        """renderer::pm (defaults: renderer_pm.cpp:61-68).  Returns (film (H, W,
+        3), stats, per-sample records (n, 16) or None: raster x, y, emission C
+        rgb, emitter vertices, density C rgb, photons gathered, vertex count at
+        the gather vertex)."""
+        import torch
+        dev = torch.device("cuda")
+        film = torch.empty((self.height, self.width, 3), dtype=torch.float32, device=dev)
+        p = self._params(num_samples, max_num_vertices, 0, 1, 0.5, seed, raster_mode, batch)
+        v = lmx_pm_params()
+        v.num_photon_trace_samples = num_photon_trace_samples
+        v.finalgather = finalgather
+        v.radius = radius
+        v.photon_seed = (seed + 1) if photon_seed is None else photon_seed
+        for t, n in ((table, num_samples), (photon_table, num_photon_trace_samples)):
+            if t is not None:
+                assert t.is_cuda and t.dtype == torch.float32 and t.is_contiguous() and t.shape[0] == n
+        if table is not None:
+            p.uniform_table = ctypes.c_void_p(table.data_ptr())
+            p.table_width = table.shape[1]
+        if photon_table is not None:
+            v.photon_table = ctypes.c_void_p(photon_table.data_ptr())
+            v.photon_table_width = photon_table.shape[1]
+        so = None
+        if sample_out:
+            so = torch.zeros((num_samples, 16), dtype=torch.float32, device=dev)
+            p.sample_out = ctypes.c_void_p(so.data_ptr())
+        st = lmx_render_stats()
+        if stream is None:
+            stream = torch.cuda.Stream(dev)
+            stream.wait_stream(torch.cuda.current_stream(dev))
+        check(lib.lmx_render_pm(self._h, ctypes.byref(p), ctypes.byref(v), ctypes.c_void_p(film.data_ptr()),
+                                ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st)))
+        stats = {"rays": st.rays, "batches": st.iterations, "batch": st.queue_capacity, "ms": st.ms}
         return film, stats, so
 
     def _bidir(self, entry, nrec, num_samples, max_num_vertices, min_num_vertices, seed, raster_mode, batch,

@@ -356,8 +356,8 @@ def load_scene(source, base_dir=None):
 
 def render(loaded, seed=42, raster_mode=None):
     """Renders a load_scene result with its renderer node (pt, ptdirect,
-    ptmis, bdpt; vcm in its bdpt, vcm and bdpm modes) and the reference's
-    parameter defaults.
+    ptmis, bdpt, pm; vcm in its bdpt, vcm and bdpm modes) and the
+    reference's parameter defaults.
     Returns (film tensor, stats)."""
     from . import render as R
     rs = R.Scene(loaded["scene"])
@@ -392,6 +392,11 @@ def render(loaded, seed=42, raster_mode=None):
                                         alpha=float(p.get("alpha", 0.7)), **common)
         else:
             raise SceneFileError(f"renderer::vcm mode {mode!r} is unknown")
+    elif t == "pm":
+        # renderer_pm.cpp:61-68; the photon map type (kdtree, naive) finds the same photons
+        film, st, _ = rs.render_pm(n, num_photon_trace_samples=int(p.get("num_photon_trace_samples", 100000)),
+                                   finalgather=int(p.get("finalgather", 1)), radius=float(p.get("radius", 0.01)),
+                                   max_num_vertices=int(p.get("max_num_vertices", -1)), seed=seed, raster_mode=rm)
     else:
         raise SceneFileError(f"renderer::{t} is not supported here")
     return film, st
