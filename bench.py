@@ -224,8 +224,9 @@ def vcm_section(dev, cpu, threads):
     (connections and merges) and mode bdpm (merges alone).  Eye samples/s of
     the whole render (photon passes, grids, eye subpaths, connections, shadow
     rays and merges), beside the oracle's restatement of one pass on host
-    threads over a bounded prefix of eye samples, its photon pass (one host
-    thread) timed whole."""
+    threads over a bounded prefix of eye samples, its photon pass timed
+    whole.  renderer::pm on the same frame beside it (16 spp, 2^22 photon
+    subpaths, radius 0.05, final gather)."""
     import torch
     from lmx import render, scenes
     log("vcm: generating the sphere-grid scene")
@@ -274,6 +275,39 @@ def vcm_section(dev, cpu, threads):
             "sample": f"one pass on {threads} threads: photon pass of 2^20 subpaths with its grid ({tph:.2f}s) and "
                       f"{chunk} eye samples ({teye * 1e6:.1f} us each), scaled to the pass's {ne} eye samples"}
         out["vcm"]["gpu_over_cpu_samples"] = round(out["vcm"]["msamples_per_s"] * 1e6 / rate, 1)
+    # renderer::pm on the same frame: 16 spp, 2^22 photon subpaths, radius
+    # 0.05, final gather, at most 10 vertices
+    npm, nphm = 16 * npix, 1 << 22
+    kp = dict(num_photon_trace_samples=nphm, finalgather=1, radius=0.05, max_num_vertices=10, seed=42)
+    rs.render_pm(npix, **kp)                                                   # warm-up
+    torch.cuda.synchronize()
+    tp = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        fpm, spm, _ = rs.render_pm(npm, **kp)
+        torch.cuda.synchronize()
+        tp.append(time.perf_counter() - t0)
+    dp = min(tp)
+    out["pm"] = {"workload": "pm: same frame, 16 spp, 2^22 photon subpaths (roulette), radius 0.05, final gather, "
+                             "at most 10 vertices",
+                 "msamples_per_s": round(npm / dp / 1e6, 2), "mrays_per_s": round(spm["rays"] / dp / 1e6, 1),
+                 "s_per_frame": round(dp, 4), "image_mean": round(float(fpm.mean()), 5), "cpu_baseline": None}
+    if cpu:
+        log("vcm: cpu baseline (oracle renderer::pm)")
+        mp = dict(radius=0.05, num_photons=nphm, max_num_vertices=10, finalgather=1, seed=42, pseed=43)
+        t0 = time.perf_counter()
+        osc.pm_samples(0, 1, nthreads=threads, **mp)
+        tph = time.perf_counter() - t0
+        chunk = 1 << 20
+        t0 = time.perf_counter()
+        osc.pm_samples(0, chunk, nthreads=threads, **mp)
+        teye = max(time.perf_counter() - t0 - tph, 1e-6) / chunk
+        rate = npm / (tph + npm * teye)
+        out["pm"]["cpu_baseline"] = {
+            "msamples_per_s": round(rate / 1e6, 5), "cores": threads, "kind": "port",
+            "sample": f"photon pass of 2^22 subpaths with its grid ({tph:.2f}s) and {chunk} eye samples "
+                      f"({teye * 1e6:.2f} us each) on {threads} threads, scaled to the frame's {npm} samples"}
+        out["pm"]["gpu_over_cpu_samples"] = round(npm / dp / rate, 1)
     return out
 
 

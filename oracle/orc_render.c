@@ -2045,20 +2045,38 @@ static void *pm_run(void *p)
     return NULL;
 }
 
+/* photon subpaths [lo, hi): their photons, in subpath order, into a growing
+ * array of the job's own */
 typedef struct {
     const bctx *b;
     int maxv;
     const usrc *pu;
-    bvert *vs;              /* PM_CAP per photon subpath */
-    int *nv;
     uint32_t lo, hi;
+    pm_photon *ph;
+    size_t n, cap;
 } pm_pjob;
 
 static void *pm_photons(void *p)
 {
     pm_pjob *j = (pm_pjob *)p;
-    for (uint32_t i = j->lo; i < j->hi; i++)
-        j->nv[i] = pm_trace_photon(j->b, j->maxv, j->pu, i, &j->vs[(size_t)i * PM_CAP]);
+    bvert vs[PM_CAP];
+    j->ph = NULL; j->n = j->cap = 0;
+    for (uint32_t i = j->lo; i < j->hi; i++) {
+        const int nv = pm_trace_photon(j->b, j->maxv, j->pu, i, vs);
+        /* the photons: diffuse or glossy vertices past the light (:94-104) */
+        for (int k = 1; k < nv; k++) {
+            if (!pm_scatters(vs[k].mat)) continue;
+            if (j->n == j->cap) {
+                j->cap = j->cap ? 2 * j->cap : 4096;
+                j->ph = (pm_photon *)realloc(j->ph, sizeof(pm_photon) * j->cap);
+            }
+            pm_photon *q = &j->ph[j->n++];
+            q->p = vs[k].fr.p;
+            for (int c = 0; c < 3; c++) q->thr[c] = vs[k].alpha[c];
+            q->wi = dir_to(&vs[k], &vs[k - 1]);
+            q->nv = k + 1;
+        }
+    }
     return NULL;
 }
 
@@ -2081,39 +2099,31 @@ void orc_pm_samples(const orc_scene *s, const orc_pm_params *pp, const float *ta
     pu.table = ptable; pu.width = ptable_width;
     pu.seed = ((uint64_t)pp->pseed_hi << 32) | pp->pseed_lo;
     pu.first = 0;
-    bvert *vs = (bvert *)malloc(sizeof(bvert) * ((size_t)np * PM_CAP + 1));
-    int *nv = (int *)calloc(np + 1, sizeof(int));
+    pm_pjob pj[256];
     {
         pthread_t th[256];
-        pm_pjob pj[256];
         for (int t = 0; t < nthreads; t++) {
-            pj[t].b = &b; pj[t].maxv = pp->max_num_vertices; pj[t].pu = &pu; pj[t].vs = vs; pj[t].nv = nv;
+            pj[t].b = &b; pj[t].maxv = pp->max_num_vertices; pj[t].pu = &pu;
             pj[t].lo = (uint32_t)((uint64_t)np * (uint64_t)t / (uint64_t)nthreads);
             pj[t].hi = (uint32_t)((uint64_t)np * (uint64_t)(t + 1) / (uint64_t)nthreads);
             pthread_create(&th[t], NULL, pm_photons, &pj[t]);
         }
         for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
     }
-    /* the photons: diffuse or glossy vertices past the light (:94-104) */
-    size_t count = 0;
-    for (uint32_t i = 0; i < np; i++)
-        for (int j = 1; j < nv[i]; j++) count += pm_scatters(vs[(size_t)i * PM_CAP + j].mat) ? 1 : 0;
-    pm_photon *ph = (pm_photon *)malloc(sizeof(pm_photon) * (count + 1));
     size_t m = 0;
+    for (int t = 0; t < nthreads; t++) m += pj[t].n;
+    pm_photon *ph = (pm_photon *)malloc(sizeof(pm_photon) * (m + 1));
+    m = 0;
+    for (int t = 0; t < nthreads; t++) {
+        if (pj[t].n) memcpy(&ph[m], pj[t].ph, sizeof(pm_photon) * pj[t].n);
+        m += pj[t].n;
+        free(pj[t].ph);
+    }
     float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
-    for (uint32_t i = 0; i < np; i++)
-        for (int j = 1; j < nv[i]; j++) {
-            const bvert *v = &vs[(size_t)i * PM_CAP + j];
-            if (!pm_scatters(v->mat)) continue;
-            pm_photon *q = &ph[m++];
-            q->p = v->fr.p;
-            for (int c = 0; c < 3; c++) q->thr[c] = v->alpha[c];
-            q->wi = dir_to(v, &vs[(size_t)i * PM_CAP + j - 1]);
-            q->nv = j + 1;
-            const float a3[3] = {q->p.x, q->p.y, q->p.z};
-            for (int a = 0; a < 3; a++) { if (a3[a] < lo[a]) lo[a] = a3[a]; if (a3[a] > hi[a]) hi[a] = a3[a]; }
-        }
-    free(vs); free(nv);
+    for (size_t k = 0; k < m; k++) {
+        const float a3[3] = {ph[k].p.x, ph[k].p.y, ph[k].p.z};
+        for (int a = 0; a < 3; a++) { if (a3[a] < lo[a]) lo[a] = a3[a]; if (a3[a] > hi[a]) hi[a] = a3[a]; }
+    }
     pm_ctx c;
     c.b = &b; c.pp = pp; c.ph = ph;
     c.cell = pp->radius;
