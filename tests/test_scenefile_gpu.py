@@ -96,3 +96,18 @@ def test_scene_file_vcm_renderer_node(gpu, mode):
     assert a.mean() > 0
     # the same samples; splat order differs between the two renders
     np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-6 * float(b.max()))
+
+
+def test_scene_file_pm_renderer_node(gpu):
+    """renderer::pm's node (renderer_pm.cpp:61-68) renders as lmx_render_pm
+    with the same parameters."""
+    sc = scenes.cornell_box(32, 32)
+    node = ["  renderer:", "    type: pm", "    params:", "      num_samples: 16384",
+            "      num_photon_trace_samples: 20000", "      finalgather: 1", "      radius: 0.05",
+            "      max_num_vertices: 7", "      photonmap: kdtree"]
+    film, _ = scenefile.render(scenefile.load_scene(_cornell_yaml(sc, 1, node)), seed=13)
+    ref = render.Scene(sc).render_pm(16384, num_photon_trace_samples=20000, finalgather=1, radius=0.05,
+                                     max_num_vertices=7, seed=13, raster_mode=render.RASTER_RANDOM)[0]
+    a, b = film.cpu().numpy(), ref.cpu().numpy()
+    assert a.mean() > 0
+    np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-6 * float(b.max()))
