@@ -1406,8 +1406,10 @@ int render_bidir(lmx_scene* sc, const lmx_render_params* p, int variant, const c
     return rc;
 }
 
-// Photon-subpath and grid buffers for np photons of up to maxv vertices;
-// kept when large enough, else replaced (the old ones freed first).
+// Photon-subpath, photon-queue and grid buffers for np photons of up to
+// maxv vertices (per photon: maxv vertex records of 64 B, maxv - 1 grid
+// records of 16 B, 144 B of ray queues); kept when large enough, else
+// replaced (the old ones freed first).
 int vc_buffers(lmx_scene* sc, uint32_t np, uint32_t maxv, uint32_t B)
 {
     if ((size_t)B * (maxv - 1) > sc->vc_mq.n) {
@@ -1418,6 +1420,8 @@ int vc_buffers(lmx_scene* sc, uint32_t np, uint32_t maxv, uint32_t B)
     sc->vc_np = sc->vc_maxv = sc->vc_H = 0;
     sc->vc_vert.release(); sc->vc_nsub.release(); sc->vc_items.release();
     sc->vc_start.release(); sc->vc_cursor.release(); sc->vc_part.release();
+    for (int k = 0; k < 2; k++) { sc->vc_ray[k].release(); sc->vc_thr[k].release(); sc->vc_meta[k].release(); }
+    sc->vc_hits.release();
     const size_t items = (size_t)np * (maxv - 1);
     uint32_t H = 1024;
     while (H < items && H < (1u << 25)) H <<= 1;
@@ -1425,8 +1429,13 @@ int vc_buffers(lmx_scene* sc, uint32_t np, uint32_t maxv, uint32_t B)
     if ((e = sc->vc_vert.alloc((size_t)np * maxv * 4)) != hipSuccess || (e = sc->vc_nsub.alloc(np)) != hipSuccess ||
         (e = sc->vc_items.alloc(items)) != hipSuccess || (e = sc->vc_start.alloc((size_t)H + 1)) != hipSuccess ||
         (e = sc->vc_cursor.alloc(H)) != hipSuccess ||
-        (e = sc->vc_part.alloc((H + SCAN_TILE) / SCAN_TILE)) != hipSuccess)
-        return hip_error(e, "vcm photon buffers");
+        (e = sc->vc_part.alloc((H + SCAN_TILE) / SCAN_TILE)) != hipSuccess ||
+        (e = sc->vc_hits.alloc(np)) != hipSuccess)
+        return hip_error(e, "photon buffers");
+    for (int k = 0; k < 2; k++)
+        if ((e = sc->vc_ray[k].alloc(2 * (size_t)np)) != hipSuccess || (e = sc->vc_thr[k].alloc(np)) != hipSuccess ||
+            (e = sc->vc_meta[k].alloc(np)) != hipSuccess)
+            return hip_error(e, "photon queues");
     sc->vc_np = np;
     sc->vc_maxv = maxv;
     sc->vc_H = H;
@@ -1445,17 +1454,18 @@ int vc_enqueue_photons(lmx_scene* sc, const BParams& pp, hipStream_t st)
     ph.epix = sc->bd_epix.p; ph.eraster = sc->bd_eraster.p;
     ph.V = sc->bd_V.p;
     ph.mq = nullptr;
+    // the photon phase's own queues: one ray per photon subpath
     Queue q[2];
-    for (int k = 0; k < 2; k++) { q[k].ray = sc->bd_ray[k].p; q[k].thr = sc->bd_thr[k].p; q[k].meta = sc->bd_meta[k].p; }
+    for (int k = 0; k < 2; k++) { q[k].ray = sc->vc_ray[k].p; q[k].thr = sc->vc_thr[k].p; q[k].meta = sc->vc_meta[k].p; }
     BCtl* ctl = sc->bd_ctl.p;
     const int cap_grid = sc->cus * 8;
     k_bd_gen<<<(np + 255) / 256, 256, 0, st>>>(sc->rs, pp, ctl, ph, q[0]);
     int a = 0;
     for (int it = 0; it < pp.maxv - 1; it++) {
         int rc = launch_trace(sc->accel, reinterpret_cast<const WRay*>(q[a].ray), 0, &ctl->n_trace, np,
-                              reinterpret_cast<WHit*>(sc->bd_hits.p), st, false);
+                              reinterpret_cast<WHit*>(sc->vc_hits.p), st, false);
         if (rc != LMX_OK) return rc;
-        k_bd_shade<<<(np + 255) / 256, 256, 0, st>>>(sc->rs, pp, ctl, q[a], sc->bd_hits.p, q[1 - a], ph);
+        k_bd_shade<<<(np + 255) / 256, 256, 0, st>>>(sc->rs, pp, ctl, q[a], sc->vc_hits.p, q[1 - a], ph);
         k_bd_commit<<<1, 64, 0, st>>>(ctl);
         a = 1 - a;
     }
@@ -1559,8 +1569,7 @@ int render_vcm(lmx_scene* sc, const lmx_render_params* p, const lmx_vcm_params* 
     pp.key = make_uint2((uint32_t)v->photon_seed, (uint32_t)(v->photon_seed >> 32));
     pp.table = v->photon_table; pp.table_width = v->photon_table_width;
     pp.sample_out = nullptr;
-    // the photon phase runs in the same queues: one ray per photon
-    int rc = bd_buffers(sc, std::max(B, (np + 1) / 2), maxv, bd_conn_per_sample(maxv, BD_VCM, true, connect));
+    int rc = bd_buffers(sc, B, maxv, bd_conn_per_sample(maxv, BD_VCM, true, connect));
     if (rc == LMX_OK) rc = vc_buffers(sc, np, maxv, B);
     const size_t npix = (size_t)sc->width * (size_t)sc->height;
     const int zgrid = std::max(1, std::min((int)((3 * npix + 255) / 256), sc->cus * 8));
@@ -1709,7 +1718,7 @@ int render_pm(lmx_scene* sc, const lmx_render_params* p, const lmx_pm_params* v,
     pp.wsub = 0;
     pp.sample_out = nullptr;
     PMArgs pa{(int32_t)pcap, bounded ? p->max_num_vertices : -1, v->finalgather, (float)np};
-    int rc = bd_buffers(sc, std::max(B, (np + 1) / 2), ecap, 1);
+    int rc = bd_buffers(sc, B, ecap, 1);
     if (rc == LMX_OK) rc = vc_buffers(sc, np, pcap, 1);
     const size_t npix = (size_t)sc->width * (size_t)sc->height;
     const int zgrid = std::max(1, std::min((int)((3 * npix + 255) / 256), sc->cus * 8));

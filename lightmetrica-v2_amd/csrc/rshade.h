@@ -535,6 +535,8 @@ struct lmx_scene {
     DevBuf<float4> vc_items;            // position, photon * maxv + vertex
     DevBuf<uint32_t> vc_start, vc_cursor, vc_part;
     DevBuf<uint32_t> vc_mq;             // merge queue: eye sample * 32 + vertex, bd_batch x (bd_maxv - 1)
+    DevBuf<float4> vc_ray[2], vc_thr[2], vc_hits;   // photon-phase ray queues (vc_np rays each)
+    DevBuf<uint4> vc_meta[2];
     ~lmx_scene()
     {
         if (h_ctl) (void)hipHostFree(h_ctl);
