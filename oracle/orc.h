@@ -236,6 +236,26 @@ void orc_pm_samples(const orc_scene *s, const orc_pm_params *p, const float *tab
                     const float *ptable, uint32_t ptable_width, uint64_t first_sample, uint64_t n, float *out,
                     int nthreads);
 
+/* renderer::ppm (orc_render.c): n measurement points (eye uniforms: table
+ * or seed, renderer::vcm's eye layout), num_passes passes of num_photons
+ * photon subpaths (pass q uses photon rows q num_photons + i of ptable, or
+ * Philox with pseed).  max_num_vertices -1 (unbounded) or >= 2.  Output per
+ * point 16 floats: raster x, y, C rgb (throughput x tau / (pi r^2 P Np) +
+ * emission, before the W H / n film scale), emission rgb, tau rgb, N,
+ * radius, 1 if the sample made a point, 0 x 2. */
+typedef struct orc_ppm_params {
+    int32_t max_num_vertices;
+    uint32_t seed_lo, seed_hi;
+    uint32_t pseed_lo, pseed_hi;
+    int32_t raster_mode;
+    uint32_t num_passes;
+    uint32_t num_photons;
+    float initial_radius;
+    float alpha;
+} orc_ppm_params;
+void orc_ppm_points(const orc_scene *s, const orc_ppm_params *p, const float *table, uint32_t table_width,
+                    const float *ptable, uint32_t ptable_width, uint64_t n, float *out, int nthreads);
+
 /* renderer::bdpt (orc_render.c).  max_num_vertices: -1 (unbounded) or >= 2;
  * min_num_vertices applies only when bounded (renderer_bdpt.cpp:887);
  * mis: 0 power heuristic, 1 simple.  Output per sample: 2 (M+1)^2 records of

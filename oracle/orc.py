@@ -221,6 +221,13 @@ class orc_pm_params(ctypes.Structure):
                 ("finalgather", ctypes.c_int32), ("num_photons", ctypes.c_uint32), ("radius", ctypes.c_float)]
 
 
+class orc_ppm_params(ctypes.Structure):
+    _fields_ = [("max_num_vertices", ctypes.c_int32), ("seed_lo", ctypes.c_uint32), ("seed_hi", ctypes.c_uint32),
+                ("pseed_lo", ctypes.c_uint32), ("pseed_hi", ctypes.c_uint32), ("raster_mode", ctypes.c_int32),
+                ("num_passes", ctypes.c_uint32), ("num_photons", ctypes.c_uint32),
+                ("initial_radius", ctypes.c_float), ("alpha", ctypes.c_float)]
+
+
 VCM_VCM = 0
 VCM_BDPM = 2
 
@@ -252,6 +259,9 @@ def _render_sigs(so):
     so.orc_pm_samples.argtypes = [vp, ctypes.POINTER(orc_pm_params), vp, ctypes.c_uint32, vp, ctypes.c_uint32,
                                   ctypes.c_uint64, ctypes.c_uint64, vp, ctypes.c_int]
     so.orc_pm_samples.restype = None
+    so.orc_ppm_points.argtypes = [vp, ctypes.POINTER(orc_ppm_params), vp, ctypes.c_uint32, vp, ctypes.c_uint32,
+                                  ctypes.c_uint64, vp, ctypes.c_int]
+    so.orc_ppm_points.restype = None
     so.orc_film_rgbe.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
     so.orc_film_rgbe.restype = None
     so.orc_film_png.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
@@ -456,6 +466,24 @@ class Scene:
             ptable = np.ascontiguousarray(ptable, np.float32)
             ptw, pp = ptable.shape[1], ctypes.c_void_p(ptable.ctypes.data)
         lib().orc_pm_samples(self._h, ctypes.byref(p), tp, tw, pp, ptw, first, n, _p(out), nthreads)
+        return out
+
+    def ppm_points(self, n, num_passes, num_photons, max_num_vertices, initial_radius=0.1, alpha=0.7, seed=42,
+                   pseed=43, raster_mode=RASTER_STRATIFIED, table=None, ptable=None, nthreads=8):
+        """renderer::ppm: (n, 16) float32 per measurement point: raster x, y,
+        C rgb, emission rgb, tau rgb, N, radius, made-a-point flag, 0, 0."""
+        p = orc_ppm_params(max_num_vertices, seed & 0xFFFFFFFF, seed >> 32, pseed & 0xFFFFFFFF, pseed >> 32,
+                           raster_mode, num_passes, num_photons, initial_radius, alpha)
+        out = np.zeros((n, 16), np.float32)
+        tw = ptw = 0
+        tp = pp = None
+        if table is not None:
+            table = np.ascontiguousarray(table, np.float32)
+            tw, tp = table.shape[1], ctypes.c_void_p(table.ctypes.data)
+        if ptable is not None:
+            ptable = np.ascontiguousarray(ptable, np.float32)
+            ptw, pp = ptable.shape[1], ctypes.c_void_p(ptable.ctypes.data)
+        lib().orc_ppm_points(self._h, ctypes.byref(p), tp, tw, pp, ptw, n, _p(out), nthreads)
         return out
 
     def film(self, samples):

@@ -2170,3 +2170,332 @@ void orc_pm_samples(const orc_scene *s, const orc_pm_params *pp, const float *ta
     for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
     free(c.cstart); free(c.items); free(ph); free(b.tri_light);
 }
+
+/* ---------------------------------------------------------------------------
+ * renderer::ppm (renderer_ppm.cpp:84-320), restated literally:
+ *  - measurement points: eye subpath i (renderer::vcm's eye layout, no
+ *    roulette) stops at its first diffuse or glossy vertex, which becomes the
+ *    point (radius r0, N = 0, tau = 0, throughput, direction to the previous
+ *    vertex, vertex count), with the emission there if that surface emits
+ *    (a light-only surface is no point and ends the path: directly seen
+ *    emitters stay black, :118-143);
+ *  - pass q: photon subpaths q Np + i of the photon source, roulette 0.5
+ *    after every vertex but the endpoint (renderer::bdpt's light-block
+ *    layout: vertex j >= 1 owns ORC_PT_SEG0 + 8 (j - 1) .. +7, 0-1 the
+ *    direction sampled at vertex j - 1, 3 the roulette after vertex j), a
+ *    photon at every diffuse or glossy vertex past the light (:180-219);
+ *    every point gathers the pass's photons within its radius (M of them,
+ *    dtau = sum f x photon throughput) and updates ratio = (N + a M) /
+ *    (N + M), tau = (tau + dtau) ratio, r = r sqrt(ratio), N = N + a M
+ *    (:238-261);
+ *  - the film after the last pass: throughput x tau / (pi r^2 P Np) +
+ *    emission per point (:266-272).
+ * max_num_vertices -1 is unbounded (32-vertex subpaths, every photon). */
+typedef struct {
+    int valid;
+    bvert v;
+    v3 wi;
+    float thr[3], em[3], tau[3];
+    float N, r;
+    int nv;
+    float rx, ry;
+} ppm_point;
+
+static int ppm_trace_photon(const bctx *b, int maxv, const usrc *us, uint64_t i, bvert *vs)
+{
+    const orc_scene *s = b->s;
+    int nv = 0;
+    v3 init_wo = mk(0, 0, 0);
+    float run[3] = {0.0f, 0.0f, 0.0f};
+    for (int step = 0; (maxv == -1 || step < maxv) && step < PM_CAP; step++) {
+        if (step == 0) {
+            if (s->d.nlight == 0) return 0;
+            bvert v;
+            memset(&v, 0, sizeof v);
+            const float uEP[2] = {U(us, i, 1), U(us, i, 2)};
+            const float uEPC = U(us, i, 3);
+            const float uED[2] = {U(us, i, 4), U(us, i, 5)};
+            int l = (int)(uEPC * (float)s->d.nlight);
+            l = l < 0 ? 0 : (l > (int)s->d.nlight - 1 ? (int)s->d.nlight - 1 : l);
+            sample_light_point(b, l, uEP, &v);
+            float lw[3];
+            orc_sample_cosine_hemisphere(uED, lw);
+            init_wo = to_world(&v.fr, mk(lw[0], lw[1], lw[2]));
+            const float a = 1.0f / pos_pdf(b, &v);
+            v.alpha[0] = v.alpha[1] = v.alpha[2] = a;
+            vs[nv++] = v;
+            for (int c = 0; c < 3; c++) run[c] = a;              /* the endpoint skips the roulette (:183-186) */
+            continue;
+        }
+        const uint32_t sb = ORC_PT_SEG0 + 8u * (uint32_t)(step - 1);
+        bvert *pv = &vs[nv - 1];
+        v3 wi, wo;
+        if (step == 1) {
+            wi = mk(0, 0, 0);
+            wo = init_wo;
+        } else {
+            wi = dir_to(pv, &vs[nv - 2]);
+            const float uD[2] = {U(us, i, sb), U(us, i, sb + 1)};
+            if (pv->kind != BV_SURFACE || !bsdf_sample(pv->mat, &pv->fr, wi, uD, &wo)) break;
+        }
+        float fs[3];
+        v_eval(b, pv, wi, wo, 1, fs);
+        if (fs[0] == 0.0f && fs[1] == 0.0f && fs[2] == 0.0f) break;
+        const float pdf = v_pdf(b, pv, wi, wo);
+        orc_hit h;
+        if (!scene_intersect(s, pv->fr.p, wo, &h)) break;
+        bvert v;
+        memset(&v, 0, sizeof v);
+        surface_at(s, pv->fr.p, wo, &h, &v.fr);
+        v.kind = BV_SURFACE; v.degenerate = 0; v.tri = h.id;
+        v.mat = &s->d.mats[s->d.tri_mat[h.id]];
+        v.light = b->tri_light[h.id];
+        for (int c = 0; c < 3; c++) v.alpha[c] = run[c] * (fs[c] / pdf);
+        vs[nv++] = v;
+        if (U(us, i, sb + 3) > 0.5f) break;                      /* :205-214 */
+        for (int c = 0; c < 3; c++) run[c] = v.alpha[c] / 0.5f;
+    }
+    return nv;
+}
+
+typedef struct {
+    const bctx *b;
+    int maxv;
+    const usrc *pu;
+    uint64_t first;
+    uint32_t lo, hi;
+    pm_photon *ph;
+    size_t n, cap;
+} ppm_pjob;
+
+static void *ppm_photons(void *p)
+{
+    ppm_pjob *j = (ppm_pjob *)p;
+    bvert vs[PM_CAP];
+    j->ph = NULL; j->n = j->cap = 0;
+    for (uint32_t i = j->lo; i < j->hi; i++) {
+        const int nv = ppm_trace_photon(j->b, j->maxv, j->pu, j->first + i, vs);
+        for (int k = 1; k < nv; k++) {
+            if (!pm_scatters(vs[k].mat)) continue;
+            if (j->n == j->cap) {
+                j->cap = j->cap ? 2 * j->cap : 4096;
+                j->ph = (pm_photon *)realloc(j->ph, sizeof(pm_photon) * j->cap);
+            }
+            pm_photon *q = &j->ph[j->n++];
+            q->p = vs[k].fr.p;
+            for (int c = 0; c < 3; c++) q->thr[c] = vs[k].alpha[c];
+            q->wi = dir_to(&vs[k], &vs[k - 1]);
+            q->nv = k + 1;
+        }
+    }
+    return NULL;
+}
+
+typedef struct {
+    const bctx *b;
+    const orc_ppm_params *pp;
+    ppm_point *mp;
+    const pm_photon *ph;
+    size_t nph;
+    float lo[3], cell;
+    int dim[3];
+    const int *cstart, *items;
+    uint64_t b0, e0;
+} ppm_ujob;
+
+static void *ppm_update(void *p)
+{
+    ppm_ujob *j = (ppm_ujob *)p;
+    const orc_ppm_params *pp = j->pp;
+    for (uint64_t k = j->b0; k < j->e0; k++) {
+        ppm_point *m = &j->mp[k];
+        if (!m->valid) continue;
+        const float r = m->r;
+        float dtau[3] = {0.0f, 0.0f, 0.0f};
+        float M = 0.0f;
+        int lo[3], hi[3];
+        const float q3[3] = {m->v.fr.p.x, m->v.fr.p.y, m->v.fr.p.z};
+        for (int a = 0; a < 3; a++) {
+            lo[a] = (int)floorf((q3[a] - r - j->lo[a]) / j->cell);
+            hi[a] = (int)floorf((q3[a] + r - j->lo[a]) / j->cell);
+            if (lo[a] < 0) lo[a] = 0;
+            if (hi[a] > j->dim[a] - 1) hi[a] = j->dim[a] - 1;
+        }
+        for (int gz = lo[2]; gz <= hi[2]; gz++)
+            for (int gy = lo[1]; gy <= hi[1]; gy++)
+                for (int gx = lo[0]; gx <= hi[0]; gx++) {
+                    const int cidx = (gz * j->dim[1] + gy) * j->dim[0] + gx;
+                    for (int it = j->cstart[cidx]; it < j->cstart[cidx + 1]; it++) {
+                        const pm_photon *ph = &j->ph[j->items[it]];
+                        const v3 d = vsub(ph->p, m->v.fr.p);
+                        if (!(vdot(d, d) < r * r)) continue;
+                        if (pp->max_num_vertices != -1 && m->nv + ph->nv - 1 > pp->max_num_vertices) continue;
+                        float f[3];
+                        v_eval(j->b, &m->v, m->wi, ph->wi, 0, f);
+                        for (int c = 0; c < 3; c++) dtau[c] += f[c] * ph->thr[c];
+                        M += 1.0f;
+                    }
+                }
+        if (m->N + M == 0.0f) continue;
+        const float ratio = (m->N + pp->alpha * M) / (m->N + M);
+        for (int c = 0; c < 3; c++) m->tau[c] = (m->tau[c] + dtau[c]) * ratio;
+        m->r = m->r * sqrtf(ratio);
+        m->N = m->N + pp->alpha * M;
+    }
+    return NULL;
+}
+
+typedef struct { const bctx *b; const orc_ppm_params *pp; usrc us; ppm_point *mp; uint64_t lo, hi; } ppm_ejob;
+
+static void *ppm_points(void *p)
+{
+    ppm_ejob *j = (ppm_ejob *)p;
+    const bctx *b = j->b;
+    const orc_ppm_params *pp = j->pp;
+    orc_bdpt_params bp = {2, 0, pp->seed_lo, pp->seed_hi, pp->raster_mode};   /* the point is vertex 1 or none */
+    for (uint64_t i = j->lo; i < j->hi; i++) {
+        ppm_point *m = &j->mp[i];
+        memset(m, 0, sizeof *m);
+        bvert E[2];
+        const int nE = trace_subpath(b, &bp, &j->us, i, 0, 0, E);
+        if (nE < 2 || !pm_scatters(E[1].mat)) continue;
+        m->valid = 1;
+        m->v = E[1];
+        m->wi = dir_to(&E[1], &E[0]);
+        for (int c = 0; c < 3; c++) m->thr[c] = E[1].alpha[c];
+        m->nv = 2;
+        m->r = pp->initial_radius;
+        if (E[1].light >= 0) {
+            bvert lv = E[1];
+            lv.kind = BV_LIGHT;
+            float le[3];
+            v_eval(b, &lv, mk(0, 0, 0), m->wi, 0, le);
+            for (int c = 0; c < 3; c++) m->em[c] = m->thr[c] * le[c];
+        }
+        if (pp->raster_mode == ORC_RASTER_STRATIFIED) {
+            const uint32_t npix = (uint32_t)(b->s->d.width * b->s->d.height);
+            const uint32_t pix = (uint32_t)(i % (uint64_t)npix);
+            m->rx = ((float)(pix % (uint32_t)b->s->d.width) + 0.5f) / (float)b->s->d.width;
+            m->ry = ((float)(pix / (uint32_t)b->s->d.width) + 0.5f) / (float)b->s->d.height;
+        } else if (!camera_raster(&b->cam, dir_to(&E[0], &E[1]), &m->rx, &m->ry)) {
+            m->valid = 0;
+        }
+    }
+    return NULL;
+}
+
+void orc_ppm_points(const orc_scene *s, const orc_ppm_params *pp, const float *table, uint32_t table_width,
+                    const float *ptable, uint32_t ptable_width, uint64_t n, float *out, int nthreads)
+{
+    bctx b;
+    b.s = s;
+    camera_init(&s->d, &b.cam);
+    b.tri_light = (int *)malloc(sizeof(int) * (s->d.ntri ? s->d.ntri : 1));
+    for (uint32_t i = 0; i < s->d.ntri; i++) b.tri_light[i] = -1;
+    for (uint32_t l = 0; l < s->d.nlight; l++)
+        for (uint32_t k = 0; k < s->d.light_prims[2 * l + 1]; k++) b.tri_light[s->d.light_prims[2 * l] + k] = (int)l;
+    b.tri_cdf_base = NULL;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    ppm_point *mp = (ppm_point *)calloc(n + 1, sizeof(ppm_point));
+    pthread_t th[256];
+    {
+        ppm_ejob ej[256];
+        for (int t = 0; t < nthreads; t++) {
+            ej[t].b = &b; ej[t].pp = pp; ej[t].mp = mp;
+            ej[t].us.table = table; ej[t].us.width = table_width;
+            ej[t].us.seed = ((uint64_t)pp->seed_hi << 32) | pp->seed_lo;
+            ej[t].us.first = 0;
+            ej[t].lo = n * (uint64_t)t / (uint64_t)nthreads;
+            ej[t].hi = n * (uint64_t)(t + 1) / (uint64_t)nthreads;
+            pthread_create(&th[t], NULL, ppm_points, &ej[t]);
+        }
+        for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    }
+    usrc pu;
+    pu.table = ptable; pu.width = ptable_width;
+    pu.seed = ((uint64_t)pp->pseed_hi << 32) | pp->pseed_lo;
+    pu.first = 0;
+    const uint32_t np = pp->num_photons;
+    for (uint32_t pass = 0; pass < pp->num_passes; pass++) {
+        ppm_pjob pj[256];
+        for (int t = 0; t < nthreads; t++) {
+            pj[t].b = &b; pj[t].maxv = pp->max_num_vertices; pj[t].pu = &pu;
+            pj[t].first = (uint64_t)pass * np;
+            pj[t].lo = (uint32_t)((uint64_t)np * (uint64_t)t / (uint64_t)nthreads);
+            pj[t].hi = (uint32_t)((uint64_t)np * (uint64_t)(t + 1) / (uint64_t)nthreads);
+            pthread_create(&th[t], NULL, ppm_photons, &pj[t]);
+        }
+        for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+        size_t m = 0;
+        for (int t = 0; t < nthreads; t++) m += pj[t].n;
+        pm_photon *ph = (pm_photon *)malloc(sizeof(pm_photon) * (m + 1));
+        m = 0;
+        for (int t = 0; t < nthreads; t++) {
+            if (pj[t].n) memcpy(&ph[m], pj[t].ph, sizeof(pm_photon) * pj[t].n);
+            m += pj[t].n;
+            free(pj[t].ph);
+        }
+        /* a uniform grid of cell >= the largest radius (the initial one) */
+        float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
+        for (size_t k = 0; k < m; k++) {
+            const float a3[3] = {ph[k].p.x, ph[k].p.y, ph[k].p.z};
+            for (int a = 0; a < 3; a++) { if (a3[a] < lo[a]) lo[a] = a3[a]; if (a3[a] > hi[a]) hi[a] = a3[a]; }
+        }
+        float cell = pp->initial_radius, span = 0.0f;
+        for (int a = 0; a < 3; a++) {
+            if (m == 0) { lo[a] = 0.0f; hi[a] = 0.0f; }
+            if (hi[a] - lo[a] > span) span = hi[a] - lo[a];
+        }
+        if (span / cell > 255.0f) cell = span / 255.0f;
+        int dim[3];
+        long long cells = 1;
+        for (int a = 0; a < 3; a++) { dim[a] = (int)floorf((hi[a] - lo[a]) / cell) + 1; cells *= dim[a]; }
+        int *cstart = (int *)calloc((size_t)cells + 1, sizeof(int));
+        int *items = (int *)malloc(sizeof(int) * (m + 1));
+        int *cellof = (int *)malloc(sizeof(int) * (m + 1));
+        for (size_t k = 0; k < m; k++) {
+            const float a3[3] = {ph[k].p.x, ph[k].p.y, ph[k].p.z};
+            int g[3];
+            for (int a = 0; a < 3; a++) {
+                g[a] = (int)floorf((a3[a] - lo[a]) / cell);
+                if (g[a] < 0) g[a] = 0;
+                if (g[a] > dim[a] - 1) g[a] = dim[a] - 1;
+            }
+            cellof[k] = (g[2] * dim[1] + g[1]) * dim[0] + g[0];
+            cstart[cellof[k] + 1]++;
+        }
+        for (long long k = 0; k < cells; k++) cstart[k + 1] += cstart[k];
+        int *fill = (int *)calloc((size_t)cells, sizeof(int));
+        for (size_t k = 0; k < m; k++) items[cstart[cellof[k]] + fill[cellof[k]]++] = (int)k;
+        free(fill); free(cellof);
+        ppm_ujob uj[256];
+        for (int t = 0; t < nthreads; t++) {
+            uj[t].b = &b; uj[t].pp = pp; uj[t].mp = mp; uj[t].ph = ph; uj[t].nph = m;
+            for (int a = 0; a < 3; a++) { uj[t].lo[a] = lo[a]; uj[t].dim[a] = dim[a]; }
+            uj[t].cell = cell; uj[t].cstart = cstart; uj[t].items = items;
+            uj[t].b0 = n * (uint64_t)t / (uint64_t)nthreads;
+            uj[t].e0 = n * (uint64_t)(t + 1) / (uint64_t)nthreads;
+            pthread_create(&th[t], NULL, ppm_update, &uj[t]);
+        }
+        for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+        free(cstart); free(items); free(ph);
+    }
+    const double total = (double)pp->num_passes * (double)np;
+    for (uint64_t i = 0; i < n; i++) {
+        float *o = &out[16 * i];
+        memset(o, 0, sizeof(float) * 16);
+        const ppm_point *m = &mp[i];
+        if (!m->valid) continue;
+        /* :266-272 */
+        const float p = 1.0f / (m->r * m->r * PI_F * (float)total);
+        o[0] = m->rx; o[1] = m->ry;
+        for (int c = 0; c < 3; c++) {
+            o[2 + c] = m->thr[c] * p * m->tau[c] + m->em[c];
+            o[5 + c] = m->em[c];
+            o[8 + c] = m->tau[c];
+        }
+        o[11] = m->N; o[12] = m->r; o[13] = 1.0f;
+    }
+    free(mp); free(b.tri_light);
+}

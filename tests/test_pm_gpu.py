@@ -6,7 +6,7 @@
   (within 1e-4 relative for >= 97 % of the samples and 1e-3 for >= 99.5 %:
   near the radius the kernel (1 - d^2/r^2)^2 magnifies an ulp of a vertex
   position), for final gather on and off, bounded and unbounded path
-  lengths, Lambertian and Phong boxes;
+  lengths, Lambertian, Phong and Cook-Torrance boxes;
 * the furnace closed form (exact emitter hits, density within the kernel's
   bias) from Philox streams;
 * the argument checks.
@@ -26,10 +26,11 @@ def _tables(n, nph, M, fg, seed):
     return rng.random((n, we), dtype=np.float32), rng.random((nph, wp), dtype=np.float32)
 
 
-@pytest.mark.parametrize("fg,M,phong", [(1, 6, None), (0, 6, None), (1, -1, None), (1, 6, 30.0)])
-def test_pm_per_sample_parity_uniform_tables(gpu, fg, M, phong):
+@pytest.mark.parametrize("fg,M,boxes", [(1, 6, None), (0, 6, None), (1, -1, None), (1, 6, "phong"), (0, 6, "ggx")])
+def test_pm_per_sample_parity_uniform_tables(gpu, fg, M, boxes):
     import torch
-    sc = scenes.cornell_box(16, 16, phong_boxes=phong)
+    sc = scenes.cornell_box(16, 16, phong_boxes=30.0 if boxes == "phong" else None,
+                            ggx_boxes=0.3 if boxes == "ggx" else None)
     n, nph, r = 16 * 16 * 16, 50000, 0.03
     tab, ptab = _tables(n, nph, M, fg, 31)
     _, _, so = render.Scene(sc).render_pm(n, num_photon_trace_samples=nph, finalgather=fg, radius=r,

@@ -4,7 +4,8 @@ against the oracle's literal restatement (one pass, brute-force weights):
 * per (sample, t, s, connect / merge) from identical eye and photon uniform
   tables: the same connections and merges contribute (up to shadow rays and
   merge-radius tests that an ulp of a vertex position decides), connection
-  C and w within 1e-4 relative, merge sums of C, counts and weight sums;
+  C and w within 1e-4 relative, merge sums of C, counts and weight sums
+  (Lambertian, Phong and Cook-Torrance boxes);
 * the furnace closed form of both modes per path length;
 * a multi-pass render with the radius schedule agrees with the bdpt-mode
   image (both estimate the same image; merging's bias shrinks with r).
@@ -45,9 +46,10 @@ def _run(sc, n, m, nph, r, mode, tab, ptab, gpu):
 
 
 @pytest.mark.parametrize("mode", [render.VCM_MODE_VCM, render.VCM_MODE_BDPM])
-@pytest.mark.parametrize("phong", [None, 30.0])
-def test_vcm_per_record_parity_uniform_tables(gpu, mode, phong):
-    sc = scenes.cornell_box(16, 16, phong_boxes=phong)
+@pytest.mark.parametrize("boxes", [None, "phong", "ggx"])
+def test_vcm_per_record_parity_uniform_tables(gpu, mode, boxes):
+    sc = scenes.cornell_box(16, 16, phong_boxes=30.0 if boxes == "phong" else None,
+                            ggx_boxes=0.3 if boxes == "ggx" else None)
     n, m, nph, r = 16 * 16 * 16, 6, 20000, 0.04
     w = render.vcm_bdpt_table_width(m)
     rng = np.random.default_rng(21)
