@@ -327,6 +327,34 @@ typedef struct lmx_pm_params {
 int lmx_render_pm(lmx_scene *scene, const lmx_render_params *params, const lmx_pm_params *pm, float *film_rgb,
                   void *stream, lmx_render_stats *stats);
 
+/* renderer::ppm (renderer_ppm.cpp:84-320): params->num_samples (1 .. 2^24)
+ * measurement points, each the first vertex of an eye sample when it is
+ * diffuse or glossy (other surfaces end the path, so an emitter without a
+ * BSDF seen directly stays black, as in the reference); then
+ * num_iteration_pass passes of num_photon_trace_samples photon subpaths with
+ * roulette 0.5 after every vertex but the endpoint, each point gathering the
+ * pass's photons within its radius and updating N, tau and the radius
+ * progressively (alpha); the film is throughput x tau / (pi r^2 x all
+ * photons) + the point's emission, scaled by W H / num_samples.
+ * max_num_vertices -1 (unbounded: subpaths cut at 32 vertices, every photon
+ * gathered) or 2..32.  Uniforms: point i index i of `seed` (eye endpoint
+ * slots 4-5; table width >= 8); photon i of pass q index q N + i of
+ * photon_seed in lmx_render_pm's photon layout without slot 6 (table rows
+ * num_iteration_pass x N, width >= 8 + 8 (M - 1)).  sample_out: 16 floats per
+ * point: raster x, y, C rgb (before the film scale), emission rgb, tau rgb,
+ * N, radius, 1 if the sample made a point, 0 x 2. */
+typedef struct lmx_ppm_params {
+    uint32_t num_iteration_pass;         /* reference default 1000 */
+    uint32_t num_photon_trace_samples;   /* per pass, 1 .. 2^26 (reference default 100) */
+    float initial_radius;                /* > 0 (reference default 0.1) */
+    float alpha;                         /* (0, 1] (reference default 0.7) */
+    uint64_t photon_seed;
+    const float *photon_table;           /* device, or NULL (Philox with photon_seed) */
+    uint32_t photon_table_width;
+} lmx_ppm_params;
+int lmx_render_ppm(lmx_scene *scene, const lmx_render_params *params, const lmx_ppm_params *ppm, float *film_rgb,
+                   void *stream, lmx_render_stats *stats);
+
 /* ---------------------------------------------------------------------------
  * Learned directional sampler.  No counterpart in the reference: a third
  * renderer beside renderer::pt and renderer::vcm, for the callers of

@@ -97,9 +97,10 @@ namespace {
 #define BD_VCM 0
 #define BD_BDPT 1
 #define BD_PM 2                // renderer::pm photon subpaths: roulette after every vertex, the endpoint included
+#define BD_PPM 3               // renderer::ppm photon subpaths: roulette after every vertex but the endpoint
 
 struct BParams {
-    int32_t variant;           // BD_VCM, BD_BDPT or BD_PM
+    int32_t variant;           // BD_VCM, BD_BDPT, BD_PM or BD_PPM
     int32_t merging;           // vcm/bdpm: merge records, photon grid, merge kernel
     int32_t connect;           // connections and s = 0 strategies evaluated (all but bdpm)
     int32_t phase;             // merging: 0 eye samples, 1 photon subpaths
@@ -389,7 +390,7 @@ __global__ __launch_bounds__(256) void k_bd_shade(RScene s, BParams p, BCtl* __r
                 const uint32_t blk = kind ? p.wsub : 0u;
                 // bdpt: roulette after the vertex (renderer_bdpt.cpp:205-211),
                 // the survivors' throughput doubled (1 / SelectionPDF)
-                const bool rr8 = p.variant == BD_BDPT || p.variant == BD_PM;
+                const bool rr8 = p.variant == BD_BDPT || p.variant == BD_PM || p.variant == BD_PPM;
                 const bool rr_ok = !rr8 || Un.one(blk + rb_step(k) + 3u) <= 0.5f;
                 if ((int)k + 1 < p.maxv && is_scatterer(m) && rr_ok) {
                     const f3 wi = dir_to(fr.p, prev.p);
@@ -847,8 +848,9 @@ __global__ __launch_bounds__(256) void k_vc_grid_fill(RScene s, BParams p, const
         const uint32_t i = (uint32_t)(idx % np), j = 1u + (uint32_t)(idx / np);
         if (j >= ph.nsub[1][i]) continue;
         const float4 a = ph.v[1][((size_t)i * (size_t)p.maxv + j) * 4];
-        // renderer::pm stores photons at diffuse and glossy vertices only (renderer_pm.cpp:94-104)
-        if (p.variant == BD_PM && !is_scatterer(s.mats[__float_as_uint(a.w) >> 16])) continue;
+        // renderer::pm / ppm store photons at diffuse and glossy vertices only
+        // (renderer_pm.cpp:94-104, renderer_ppm.cpp:188-199)
+        if ((p.variant == BD_PM || p.variant == BD_PPM) && !is_scatterer(s.mats[__float_as_uint(a.w) >> 16])) continue;
         const uint32_t h = bucket_of(cell_of(a.x, cell), cell_of(a.y, cell), cell_of(a.z, cell), g.H);
         if (PHASE == 0) {
             atomicAdd(&g.start[h], 1u);
@@ -1126,6 +1128,130 @@ __global__ __launch_bounds__(256) void k_pm_gather(RScene s, BParams p, BCtl* __
         }
     }
 }
+
+// renderer::ppm (renderer_ppm.cpp:84-320).  Measurement point i is eye
+// vertex 1 of sample i when it is diffuse or glossy (an eye subpath reaches
+// no further point: other surfaces end it); its state is st[2 i] = (tau,
+// N) and st[2 i + 1] = (emission, radius), radius < 0 for no point.
+__global__ __launch_bounds__(256) void k_ppm_init(RScene s, BParams p, VB vb, float4* __restrict__ st, float r0)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.batch; i += gridDim.x * blockDim.x) {
+        float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+        if (vb.nsub[0][i] >= 2) {
+            const Vtx v = load_v(vb, p, 0, i, 1u);
+            const RMat m = s.mats[v.mat];
+            if (is_scatterer(m)) {
+                b.w = r0;
+                const float4 c = vb.v[0][(size_t)i * (size_t)p.maxv * 4];
+                // emission at the point (:134-142)
+                if (v.light >= 0 && emits_towards(v.fr, dir_to(v.p, mk3(c.x, c.y, c.z)))) {
+                    b.x = v.alpha.x * m.Le[0]; b.y = v.alpha.y * m.Le[1]; b.z = v.alpha.z * m.Le[2];
+                }
+            }
+        }
+        st[2 * (size_t)i] = a;
+        st[2 * (size_t)i + 1] = b;
+    }
+}
+
+// One progressive density-estimation step per measurement point
+// (:229-262): the pass's photons within the point's radius whose merged
+// path has at most max_num_vertices vertices give M and dtau = sum of
+// f x photon throughput; then ratio = (N + alpha M) / (N + M), tau = (tau +
+// dtau) ratio, r = r sqrt(ratio), N = N + alpha M.
+__global__ __launch_bounds__(256) void k_ppm_update(RScene s, BParams p, const BCtl* __restrict__ ctl, VB vb, VB ph,
+                                                    Grid g, PMArgs a, float4* __restrict__ st, float alpha)
+{
+    __shared__ uint32_t hl[MERGE_LIST * 256];
+    const float cell = ctl->cell;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.batch; i += gridDim.x * blockDim.x) {
+        float4 s0 = st[2 * (size_t)i];
+        const float4 s1 = st[2 * (size_t)i + 1];
+        const float r = s1.w;
+        if (!(r > 0.0f)) continue;
+        const Vtx v = load_v(vb, p, 0, i, 1u);
+        const float4 c0 = vb.v[0][(size_t)i * (size_t)p.maxv * 4];
+        const f3 wi = dir_to(v.p, mk3(c0.x, c0.y, c0.z));
+        const RMat m = s.mats[v.mat];
+        const float r2 = r * r;
+        int lo[3], nc[3];
+        const float q[3] = {v.p.x, v.p.y, v.p.z};
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++) {
+            lo[ax] = cell_of(q[ax] - r, cell);
+            nc[ax] = cell_of(q[ax] + r, cell) - lo[ax] + 1;
+        }
+        const int ncell = nc[0] * nc[1] * nc[2];
+        // photon vertex j joins when 2 + (j + 1) - 1 <= maxv
+        const int jmax = a.maxv == -1 ? 0x7fffffff : a.maxv - 2;
+        f3 dtau = mk3(0, 0, 0);
+        float M = 0.0f;
+        int c = 0;
+        uint32_t it = 0, end = 0;
+        bool fresh = true;
+        while (c < ncell) {
+            uint32_t nh = 0;
+            while (c < ncell && nh < MERGE_LIST) {
+                const int cx = lo[0] + c % nc[0], cy = lo[1] + (c / nc[0]) % nc[1], cz = lo[2] + c / (nc[0] * nc[1]);
+                if (fresh) {
+                    const uint32_t h = bucket_of(cx, cy, cz, g.H);
+                    it = g.start[h];
+                    end = g.start[h + 1];
+                    fresh = false;
+                }
+                for (; it < end && nh < MERGE_LIST; it++) {
+                    const float4 cd = g.items[it];
+                    const int j = (int)(__float_as_uint(cd.w) % (uint32_t)a.pmaxv);
+                    if (j > jmax) continue;
+                    if (cell_of(cd.x, cell) != cx || cell_of(cd.y, cell) != cy || cell_of(cd.z, cell) != cz) continue;
+                    const f3 d = sub(mk3(cd.x, cd.y, cd.z), v.p);
+                    if (!(dot(d, d) < r2)) continue;
+                    hl[nh * 256u + threadIdx.x] = __float_as_uint(cd.w);
+                    nh++;
+                }
+                if (it >= end) { c++; fresh = true; }
+            }
+            for (uint32_t h = 0; h < nh; h++) {
+                const uint32_t id = hl[h * 256u + threadIdx.x];
+                const float4* rec = ph.v[1] + (size_t)id * 4;
+                const float4 r0 = rec[0], r3 = rec[3], rp = rec[-4];
+                const f3 pp = mk3(r0.x, r0.y, r0.z);
+                const f3 f = bsdf_eval_t(m, v.fr, wi, dir_to(pp, mk3(rp.x, rp.y, rp.z)), false);
+                dtau = add(dtau, mk3(f.x * r3.x, f.y * r3.y, f.z * r3.z));
+                M += 1.0f;
+            }
+        }
+        if (s0.w + M == 0.0f) continue;
+        const float ratio = (s0.w + alpha * M) / (s0.w + M);
+        s0 = make_float4((s0.x + dtau.x) * ratio, (s0.y + dtau.y) * ratio, (s0.z + dtau.z) * ratio,
+                         s0.w + alpha * M);
+        st[2 * (size_t)i] = s0;
+        st[2 * (size_t)i + 1] = make_float4(s1.x, s1.y, s1.z, r * sqrtf(ratio));
+    }
+}
+
+// The film after the last pass (:264-272): throughput x tau / (pi r^2 N_total)
+// + emission per point.
+__global__ __launch_bounds__(256) void k_ppm_splat(BParams p, VB vb, const float4* __restrict__ st, float total)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.batch; i += gridDim.x * blockDim.x) {
+        const float4 s0 = st[2 * (size_t)i], s1 = st[2 * (size_t)i + 1];
+        if (!(s1.w > 0.0f)) continue;
+        const float4 d = vb.v[0][((size_t)i * (size_t)p.maxv + 1) * 4 + 3];          // throughput at the point
+        const float pd = 1.0f / (s1.w * s1.w * LMX_PI_F * total);
+        const f3 C = mk3(d.x * pd * s0.x + s1.x, d.y * pd * s0.y + s1.y, d.z * pd * s0.z + s1.z);
+        splat(p, vb.epix[i], C);
+        if (p.sample_out) {
+            const float2 e = vb.eraster[i];
+            float4* o = reinterpret_cast<float4*>(p.sample_out + 16 * (size_t)i);
+            o[0] = make_float4(e.x, e.y, C.x, C.y);
+            o[1] = make_float4(C.z, s1.x, s1.y, s1.z);
+            o[2] = make_float4(s0.x, s0.y, s0.z, s0.w);
+            o[3] = make_float4(s1.w, 1.0f, 0.0f, 0.0f);
+        }
+    }
+}
+
 
 __global__ void k_bd_zero(float* a, size_t n)
 {
@@ -1756,6 +1882,136 @@ int render_pm(lmx_scene* sc, const lmx_render_params* p, const lmx_pm_params* v,
     return rc;
 }
 
+// renderer::ppm: the measurement points (one eye batch of every sample,
+// subpaths of 2 vertices), then per pass the photon graph and the
+// progressive update, then the film.
+int render_ppm(lmx_scene* sc, const lmx_render_params* p, const lmx_ppm_params* v, float* film_rgb, void* stream,
+               lmx_render_stats* stats)
+{
+    const char* name = "lmx_render_ppm";
+    char msg[160];
+    if (!sc || !p || !v || !film_rgb) return set_error(LMX_EINVAL, "lmx_render_ppm: NULL argument");
+    if (p->num_samples == 0 || p->num_samples > (1ull << 24))
+        return set_error(LMX_EINVAL, "lmx_render_ppm: num_samples (measurement points) must be in [1, 2^24]");
+    const bool bounded = p->max_num_vertices != -1;
+    if (bounded && (p->max_num_vertices < 2 || p->max_num_vertices > 32))
+        return set_error(LMX_EINVAL, "lmx_render_ppm: max_num_vertices must be in [2, 32] or -1");
+    if (v->num_iteration_pass == 0) return set_error(LMX_EINVAL, "lmx_render_ppm: num_iteration_pass must be >= 1");
+    if (v->num_photon_trace_samples == 0 || v->num_photon_trace_samples > (1u << 26))
+        return set_error(LMX_EINVAL, "lmx_render_ppm: num_photon_trace_samples must be in [1, 2^26]");
+    if ((uint64_t)v->num_iteration_pass * v->num_photon_trace_samples > 0xffffffffull)
+        return set_error(LMX_EINVAL, "lmx_render_ppm: passes x photons must stay below 2^32");
+    if (!(v->initial_radius > 0.0f) || !(v->initial_radius < 3.0e38f) || !(v->alpha > 0.0f) || !(v->alpha <= 1.0f))
+        return set_error(LMX_EINVAL, "lmx_render_ppm: initial_radius must be positive and finite, alpha in (0, 1]");
+    if (sc->mats.n > VB_MAX_MATERIALS || sc->lights.n > VB_MAX_LIGHTS)
+        return set_error(LMX_EINVAL, "lmx_render_ppm: at most 65536 materials and 8190 lights");
+    if (p->raster_mode != LMX_RASTER_RANDOM && p->raster_mode != LMX_RASTER_STRATIFIED)
+        return set_error(LMX_EINVAL, "lmx_render_ppm: unknown raster_mode");
+    const uint32_t pcap = bounded ? (uint32_t)p->max_num_vertices : 32u;
+    const uint32_t ewidth = LMX_PT_SLOT_SEG0;                 // the eye endpoint's slots
+    const uint32_t pwidth = LMX_PT_SLOT_SEG0 + 8u * (pcap - 1u);
+    if ((p->uniform_table && p->table_width < ewidth) || (v->photon_table && v->photon_table_width < pwidth)) {
+        snprintf(msg, sizeof msg, "%s: uniform table narrower than %u (eye) / %u (photon)", name, ewidth, pwidth);
+        return set_error(LMX_EINVAL, msg);
+    }
+    hipStream_t st = (hipStream_t)stream;
+    hipStream_t own = nullptr;
+    hipError_t e;
+    if (!st) {
+        if ((e = hipStreamCreateWithFlags(&own, hipStreamNonBlocking)) != hipSuccess) return hip_error(e, "stream create");
+        st = own;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t N = (uint32_t)p->num_samples, np = v->num_photon_trace_samples;
+    BParams bp;                          // the measurement points: one batch of every sample
+    bp.variant = BD_VCM;
+    bp.merging = 0; bp.connect = 0; bp.phase = 0; bp.c_conn = 1.0f;
+    bp.mis = LMX_MIS_POWER;
+    bp.bounded = 1;
+    bp.maxv = 2; bp.minv = 0;
+    bp.key = make_uint2((uint32_t)p->seed, (uint32_t)(p->seed >> 32));
+    bp.total = N; bp.batch = N;
+    bp.raster_mode = p->raster_mode;
+    bp.table = p->uniform_table; bp.table_width = p->table_width;
+    bp.wsub = ewidth;
+    bp.sample_out = p->sample_out;
+    bp.film = film_rgb;
+    BParams pp = bp;                     // photon subpaths
+    pp.variant = BD_PPM;
+    pp.phase = 1;
+    pp.bounded = bounded ? 1 : 0;
+    pp.maxv = (int)pcap;
+    pp.batch = np; pp.total = np;
+    pp.key = make_uint2((uint32_t)v->photon_seed, (uint32_t)(v->photon_seed >> 32));
+    pp.table = v->photon_table; pp.table_width = v->photon_table_width;
+    pp.wsub = 0;
+    pp.sample_out = nullptr;
+    PMArgs pa{(int32_t)pcap, bounded ? p->max_num_vertices : -1, 1, (float)np};
+    int rc = bd_buffers(sc, N, 2, 1);
+    if (rc == LMX_OK) rc = vc_buffers(sc, np, pcap, 1);
+    if (rc == LMX_OK && sc->ppm_st.n < 2 * (size_t)N && (e = sc->ppm_st.alloc(2 * (size_t)N)) != hipSuccess)
+        rc = hip_error(e, "ppm measurement points");
+    const size_t npix = (size_t)sc->width * (size_t)sc->height;
+    const int zgrid = std::max(1, std::min((int)((3 * npix + 255) / 256), sc->cus * 8));
+    const int mgrid = std::max(1, std::min((int)((N + 255) / 256), sc->cus * 16));
+    const float r0 = v->initial_radius, cell = (float)(2.0 * (double)r0);
+    hipGraphExec_t gph = nullptr;
+    VB vb;
+    vb.v[0] = sc->bd_vert[0].p; vb.v[1] = sc->bd_vert[1].p;
+    vb.nsub[0] = sc->bd_nsub[0].p; vb.nsub[1] = sc->bd_nsub[1].p;
+    vb.epix = sc->bd_epix.p; vb.eraster = sc->bd_eraster.p;
+    vb.V = sc->bd_V.p;
+    vb.mq = nullptr;
+    VB ph = vb;
+    ph.v[1] = sc->vc_vert.p; ph.nsub[1] = sc->vc_nsub.p;
+    const Grid g{sc->vc_items.p, sc->vc_start.p, sc->vc_cursor.p, sc->vc_part.p, sc->vc_H};
+    if (rc == LMX_OK) {
+        k_bd_zero<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix);
+        if (p->sample_out)
+            k_bd_zero<<<std::max(1, (int)std::min<size_t>((16 * (size_t)N + 255) / 256, (size_t)sc->cus * 8)), 256,
+                        0, st>>>(p->sample_out, 16 * (size_t)N);
+        k_bd_init<<<1, 64, 0, st>>>(sc->bd_ctl.p);
+        // the measurement points: endpoints and the first hit of every sample
+        Queue q[2];
+        for (int k = 0; k < 2; k++) { q[k].ray = sc->bd_ray[k].p; q[k].thr = sc->bd_thr[k].p; q[k].meta = sc->bd_meta[k].p; }
+        BCtl* ctl = sc->bd_ctl.p;
+        k_bd_gen<<<(N + 255) / 256, 256, 0, st>>>(sc->rs, bp, ctl, vb, q[0]);
+        rc = launch_trace(sc->accel, reinterpret_cast<const WRay*>(q[0].ray), 0, &ctl->n_trace, N,
+                          reinterpret_cast<WHit*>(sc->bd_hits.p), st, false);
+        if (rc == LMX_OK) {
+            k_bd_shade<<<(N + 255) / 256, 256, 0, st>>>(sc->rs, bp, ctl, q[0], sc->bd_hits.p, q[1], vb);
+            k_bd_commit<<<1, 64, 0, st>>>(ctl);
+            k_ppm_init<<<mgrid, 256, 0, st>>>(sc->rs, bp, vb, sc->ppm_st.p, r0);
+            if ((e = hipGetLastError()) != hipSuccess) rc = hip_error(e, "ppm launch");
+        }
+    }
+    if (rc == LMX_OK) rc = capture(st, &gph, [&] { return vc_enqueue_photons(sc, pp, st); });
+    for (uint32_t q = 0; rc == LMX_OK && q < v->num_iteration_pass; q++) {
+        k_bd_phase<<<1, 64, 0, st>>>(sc->bd_ctl.p, q * np, r0, cell, 0.0f);
+        if ((e = hipGraphLaunch(gph, st)) != hipSuccess) { rc = hip_error(e, "graph launch"); break; }
+        k_ppm_update<<<mgrid, 256, 0, st>>>(sc->rs, bp, sc->bd_ctl.p, vb, ph, g, pa, sc->ppm_st.p, v->alpha);
+    }
+    if (rc == LMX_OK) {
+        const float total = (float)((double)v->num_iteration_pass * (double)np);
+        k_ppm_splat<<<mgrid, 256, 0, st>>>(bp, vb, sc->ppm_st.p, total);
+        k_bd_scale<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix, (float)((double)npix / (double)N));
+        if ((e = hipGetLastError()) != hipSuccess) rc = hip_error(e, "ppm launch");
+    }
+    if (rc == LMX_OK &&
+        ((e = hipMemcpyAsync(sc->bd_hctl, sc->bd_ctl.p, sizeof(BCtl), hipMemcpyDeviceToHost, st)) != hipSuccess ||
+         (e = hipStreamSynchronize(st)) != hipSuccess))
+        rc = hip_error(e, "lmx_render_ppm: sync");
+    if (gph) (void)hipGraphExecDestroy(gph);
+    if (own) (void)hipStreamDestroy(own);
+    if (stats) {
+        stats->rays = rc == LMX_OK ? sc->bd_hctl->rays : 0;
+        stats->iterations = v->num_iteration_pass;
+        stats->queue_capacity = N;
+        stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1781,6 +2037,12 @@ int lmx_render_pm(lmx_scene* sc, const lmx_render_params* p, const lmx_pm_params
                   lmx_render_stats* stats)
 {
     return render_pm(sc, p, v, film_rgb, stream, stats);
+}
+
+int lmx_render_ppm(lmx_scene* sc, const lmx_render_params* p, const lmx_ppm_params* v, float* film_rgb,
+                   void* stream, lmx_render_stats* stats)
+{
+    return render_ppm(sc, p, v, film_rgb, stream, stats);
 }
 
 }  // extern "C"

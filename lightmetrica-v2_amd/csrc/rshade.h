@@ -537,6 +537,7 @@ struct lmx_scene {
     DevBuf<uint32_t> vc_mq;             // merge queue: eye sample * 32 + vertex, bd_batch x (bd_maxv - 1)
     DevBuf<float4> vc_ray[2], vc_thr[2], vc_hits;   // photon-phase ray queues (vc_np rays each)
     DevBuf<uint4> vc_meta[2];
+    DevBuf<float4> ppm_st;              // renderer::ppm measurement-point state, 2 per point
     ~lmx_scene()
     {
         if (h_ctl) (void)hipHostFree(h_ctl);

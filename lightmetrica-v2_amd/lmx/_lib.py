@@ -83,6 +83,12 @@ class lmx_pm_params(ctypes.Structure):
                 ("photon_table_width", ctypes.c_uint32)]
 
 
+class lmx_ppm_params(ctypes.Structure):
+    _fields_ = [("num_iteration_pass", ctypes.c_uint32), ("num_photon_trace_samples", ctypes.c_uint32),
+                ("initial_radius", ctypes.c_float), ("alpha", ctypes.c_float), ("photon_seed", ctypes.c_uint64),
+                ("photon_table", ctypes.c_void_p), ("photon_table_width", ctypes.c_uint32)]
+
+
 # Every symbol include/lmx.h declares, with its ctypes signature.
 class lmx_guide_opts(ctypes.Structure):
     _fields_ = [("lr", ctypes.c_float), ("beta1", ctypes.c_float), ("beta2", ctypes.c_float),
@@ -123,6 +129,9 @@ SIGNATURES = {
     "lmx_render_pm": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(lmx_render_params),
                                      ctypes.POINTER(lmx_pm_params), ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.POINTER(lmx_render_stats)]),
+    "lmx_render_ppm": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(lmx_render_params),
+                                      ctypes.POINTER(lmx_ppm_params), ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.POINTER(lmx_render_stats)]),
     "lmx_film_save": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_char_p,
                                      ctypes.c_void_p]),
     "lmx_image_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),

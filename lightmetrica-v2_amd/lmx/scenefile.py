@@ -356,7 +356,7 @@ def load_scene(source, base_dir=None):
 
 def render(loaded, seed=42, raster_mode=None):
     """Renders a load_scene result with its renderer node (pt, ptdirect,
-    ptmis, bdpt, pm; vcm in its bdpt, vcm and bdpm modes) and the
+    ptmis, bdpt, pm, ppm; vcm in its bdpt, vcm and bdpm modes) and the
     reference's parameter defaults.
     Returns (film tensor, stats)."""
     from . import render as R
@@ -397,6 +397,15 @@ def render(loaded, seed=42, raster_mode=None):
         film, st, _ = rs.render_pm(n, num_photon_trace_samples=int(p.get("num_photon_trace_samples", 100000)),
                                    finalgather=int(p.get("finalgather", 1)), radius=float(p.get("radius", 0.01)),
                                    max_num_vertices=int(p.get("max_num_vertices", -1)), seed=seed, raster_mode=rm)
+    elif t == "ppm":
+        # renderer_ppm.cpp:72-83: max_num_vertices is required there
+        if "max_num_vertices" not in p:
+            raise SceneFileError("renderer::ppm needs max_num_vertices")
+        film, st, _ = rs.render_ppm(int(p.get("num_samples", 100000)), int(p["max_num_vertices"]),
+                                    num_iteration_pass=int(p.get("num_iteration_pass", 1000)),
+                                    num_photon_trace_samples=int(p.get("num_photon_trace_samples", 100)),
+                                    initial_radius=float(p.get("initial_radius", 0.1)),
+                                    alpha=float(p.get("alpha", 0.7)), seed=seed, raster_mode=rm)
     else:
         raise SceneFileError(f"renderer::{t} is not supported here")
     return film, st
