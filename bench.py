@@ -226,7 +226,8 @@ def vcm_section(dev, cpu, threads):
     rays and merges), beside the oracle's restatement of one pass on host
     threads over a bounded prefix of eye samples, its photon pass timed
     whole.  renderer::pm on the same frame beside it (16 spp, 2^22 photon
-    subpaths, radius 0.05, final gather)."""
+    subpaths, radius 0.05, final gather) and renderer::ppm (a point per
+    pixel, 32 passes of 2^20 photon subpaths)."""
     import torch
     from lmx import render, scenes
     log("vcm: generating the sphere-grid scene")
@@ -308,6 +309,38 @@ def vcm_section(dev, cpu, threads):
             "sample": f"photon pass of 2^22 subpaths with its grid ({tph:.2f}s) and {chunk} eye samples "
                       f"({teye * 1e6:.2f} us each) on {threads} threads, scaled to the frame's {npm} samples"}
         out["pm"]["gpu_over_cpu_samples"] = round(npm / dp / rate, 1)
+    # renderer::ppm on the same frame: one measurement point per pixel, 32
+    # passes of 2^20 photon subpaths, initial radius 0.05, alpha 0.7
+    passes, nphp = 32, 1 << 20
+    kq = dict(num_iteration_pass=passes, num_photon_trace_samples=nphp, initial_radius=0.05, alpha=0.7, seed=42)
+    rs.render_ppm(npix, 10, **dict(kq, num_iteration_pass=2))                 # warm-up
+    torch.cuda.synchronize()
+    tq = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        fpp, spp_, _ = rs.render_ppm(npix, 10, **kq)
+        torch.cuda.synchronize()
+        tq.append(time.perf_counter() - t0)
+    dq = min(tq)
+    out["ppm"] = {"workload": "ppm: same frame, one measurement point per pixel (1,048,576), 32 passes of 2^20 "
+                              "photon subpaths, initial radius 0.05, alpha 0.7, at most 10 vertices",
+                  "ms_per_pass": round(dq / passes * 1e3, 3), "mphotons_per_s": round(passes * nphp / dq / 1e6, 1),
+                  "s_per_frame": round(dq, 4), "image_mean": round(float(fpp.mean()), 5), "cpu_baseline": None}
+    if cpu:
+        log("vcm: cpu baseline (oracle renderer::ppm)")
+        mq = dict(initial_radius=0.05, alpha=0.7, seed=42, pseed=43)
+        t0 = time.perf_counter()
+        osc.ppm_points(npix, 1, nphp, 10, nthreads=threads, **mq)
+        t1 = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        osc.ppm_points(npix, 3, nphp, 10, nthreads=threads, **mq)
+        t3 = time.perf_counter() - t0
+        cpass = max(t3 - t1, 1e-6) / 2.0
+        out["ppm"]["cpu_baseline"] = {
+            "ms_per_pass": round(cpass * 1e3, 1), "mphotons_per_s": round(nphp / cpass / 1e6, 3), "cores": threads,
+            "kind": "port", "sample": f"the oracle at 3 passes minus 1 pass of the same workload on {threads} "
+                                      f"threads ({t1:.1f}s and {t3:.1f}s)"}
+        out["ppm"]["gpu_over_cpu_passes"] = round(cpass * passes / dq, 1)
     return out
 
 

@@ -286,7 +286,7 @@ __global__ __launch_bounds__(256) void k_bd_gen(RScene s, BParams p, BCtl* __res
                 la = mk3(a0 * (m.Le[0] / lpdf), a0 * (m.Le[1] / lpdf), a0 * (m.Le[2] / lpdf));
                 light_ray = !(m.Le[0] == 0.0f && m.Le[1] == 0.0f && m.Le[2] == 0.0f);
             }
-            // renderer::pm: roulette after the endpoint (renderer_pm.cpp:107-116),
+            // renderer::pm: roulette after the endpoint (renderer_pm.cpp:111-120),
             // the survivor's throughput doubled
             if (p.variant == BD_PM && light_ray) {
                 if (U.one(b + 6) > 0.5f) light_ray = false;
@@ -849,7 +849,7 @@ __global__ __launch_bounds__(256) void k_vc_grid_fill(RScene s, BParams p, const
         if (j >= ph.nsub[1][i]) continue;
         const float4 a = ph.v[1][((size_t)i * (size_t)p.maxv + j) * 4];
         // renderer::pm / ppm store photons at diffuse and glossy vertices only
-        // (renderer_pm.cpp:94-104, renderer_ppm.cpp:188-199)
+        // (renderer_pm.cpp:98-110, renderer_ppm.cpp:214-226)
         if ((p.variant == BD_PM || p.variant == BD_PPM) && !is_scatterer(s.mats[__float_as_uint(a.w) >> 16])) continue;
         const uint32_t h = bucket_of(cell_of(a.x, cell), cell_of(a.y, cell), cell_of(a.z, cell), g.H);
         if (PHASE == 0) {
@@ -1021,7 +1021,7 @@ __global__ __launch_bounds__(256) void k_vc_merge(RScene s, BParams p, BCtl* __r
     if ((threadIdx.x & 63u) == 0 && nmerge) atomicAdd(&ctl->merges, (unsigned long long)nmerge);
 }
 
-// renderer::pm's eye side (renderer_pm.cpp:150-210): one thread per eye
+// renderer::pm's eye side (renderer_pm.cpp:145-205): one thread per eye
 // sample walks its subpath (traced without roulette up to the gather
 // vertex); an emitter vertex adds throughput x Le towards the previous vertex,
 // and the first (finalgather 0) or second (finalgather 1) diffuse or glossy
@@ -1143,7 +1143,7 @@ __global__ __launch_bounds__(256) void k_ppm_init(RScene s, BParams p, VB vb, fl
             if (is_scatterer(m)) {
                 b.w = r0;
                 const float4 c = vb.v[0][(size_t)i * (size_t)p.maxv * 4];
-                // emission at the point (:134-142)
+                // emission at the point (:154-162)
                 if (v.light >= 0 && emits_towards(v.fr, dir_to(v.p, mk3(c.x, c.y, c.z)))) {
                     b.x = v.alpha.x * m.Le[0]; b.y = v.alpha.y * m.Le[1]; b.z = v.alpha.z * m.Le[2];
                 }
@@ -1155,7 +1155,7 @@ __global__ __launch_bounds__(256) void k_ppm_init(RScene s, BParams p, VB vb, fl
 }
 
 // One progressive density-estimation step per measurement point
-// (:229-262): the pass's photons within the point's radius whose merged
+// (:270-297): the pass's photons within the point's radius whose merged
 // path has at most max_num_vertices vertices give M and dtau = sum of
 // f x photon throughput; then ratio = (N + alpha M) / (N + M), tau = (tau +
 // dtau) ratio, r = r sqrt(ratio), N = N + alpha M.
@@ -1230,7 +1230,7 @@ __global__ __launch_bounds__(256) void k_ppm_update(RScene s, BParams p, const B
     }
 }
 
-// The film after the last pass (:264-272): throughput x tau / (pi r^2 N_total)
+// The film after the last pass (:300-310): throughput x tau / (pi r^2 N_total)
 // + emission per point.
 __global__ __launch_bounds__(256) void k_ppm_splat(BParams p, VB vb, const float4* __restrict__ st, float total)
 {

@@ -183,7 +183,7 @@ class Scene:
     def render_pm(self, num_samples, num_photon_trace_samples=100000, finalgather=1, radius=0.01,
                   max_num_vertices=-1, seed=42, photon_seed=None, raster_mode=RASTER_STRATIFIED, batch=0,
                   table=None, photon_table=None, sample_out=False, stream=None):
-        """renderer::pm (defaults: renderer_pm.cpp:61-68).  Returns (film (H, W,
+        """renderer::pm (defaults: renderer_pm.cpp:61-70).  Returns (film (H, W,
         3), stats, per-sample records (n, 16) or None: raster x, y, emission C
         rgb, emitter vertices, density C rgb, photons gathered, vertex count at
         the gather vertex)."""
@@ -221,7 +221,7 @@ class Scene:
     def render_ppm(self, num_samples, max_num_vertices, num_iteration_pass=1000, num_photon_trace_samples=100,
                    initial_radius=0.1, alpha=0.7, seed=42, photon_seed=None, raster_mode=RASTER_STRATIFIED,
                    table=None, photon_table=None, sample_out=False, stream=None):
-        """renderer::ppm (defaults: renderer_ppm.cpp:72-83; max_num_vertices
+        """renderer::ppm (defaults: renderer_ppm.cpp:76-89; max_num_vertices
         has none there either); num_samples = measurement points.  Returns
         (film (H, W, 3), stats, per-point records (n, 16) or None: raster x,
         y, C rgb, emission rgb, tau rgb, N, radius, made-a-point flag)."""

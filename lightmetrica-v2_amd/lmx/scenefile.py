@@ -393,12 +393,12 @@ def render(loaded, seed=42, raster_mode=None):
         else:
             raise SceneFileError(f"renderer::vcm mode {mode!r} is unknown")
     elif t == "pm":
-        # renderer_pm.cpp:61-68; the photon map type (kdtree, naive) finds the same photons
+        # renderer_pm.cpp:61-70; the photon map type (kdtree, naive) finds the same photons
         film, st, _ = rs.render_pm(n, num_photon_trace_samples=int(p.get("num_photon_trace_samples", 100000)),
                                    finalgather=int(p.get("finalgather", 1)), radius=float(p.get("radius", 0.01)),
                                    max_num_vertices=int(p.get("max_num_vertices", -1)), seed=seed, raster_mode=rm)
     elif t == "ppm":
-        # renderer_ppm.cpp:72-83: max_num_vertices is required there
+        # renderer_ppm.cpp:76-89: max_num_vertices is required there
         if "max_num_vertices" not in p:
             raise SceneFileError("renderer::ppm needs max_num_vertices")
         film, st, _ = rs.render_ppm(int(p.get("num_samples", 100000)), int(p["max_num_vertices"]),

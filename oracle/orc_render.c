@@ -1915,7 +1915,7 @@ static int pm_trace_photon(const bctx *b, int maxv, const usrc *us, uint64_t i, 
             const float a = 1.0f / pos_pdf(b, &v);
             v.alpha[0] = v.alpha[1] = v.alpha[2] = a;
             vs[nv++] = v;
-            if (U(us, i, 6) > 0.5f) break;                   /* renderer_pm.cpp:107-116 */
+            if (U(us, i, 6) > 0.5f) break;                   /* renderer_pm.cpp:111-120 */
             for (int c = 0; c < 3; c++) run[c] = v.alpha[c] / 0.5f;
             continue;
         }
@@ -1991,7 +1991,7 @@ static void pm_sample(const pm_ctx *c, uint64_t sample, float *out)
     int gather_next = !pp->finalgather;
     for (int k = 1; k < nE; k++) {
         const bvert *v = &E[k], *pv = &E[k - 1];
-        /* an emitter hit: throughput x Le towards the previous vertex (:170-180) */
+        /* an emitter hit: throughput x Le towards the previous vertex (:158-167) */
         if (v->light >= 0) {
             bvert lv = *v;
             lv.kind = BV_LIGHT;
@@ -2002,7 +2002,7 @@ static void pm_sample(const pm_ctx *c, uint64_t sample, float *out)
         }
         if (!pm_scatters(v->mat)) continue;
         if (!gather_next) { gather_next = 1; continue; }
-        /* density estimation (:183-208) */
+        /* density estimation (:169-197) */
         out[10] = (float)(k + 1);
         int lo[3], hi[3];
         const float q3[3] = {v->fr.p.x, v->fr.p.y, v->fr.p.z};
@@ -2063,7 +2063,7 @@ static void *pm_photons(void *p)
     j->ph = NULL; j->n = j->cap = 0;
     for (uint32_t i = j->lo; i < j->hi; i++) {
         const int nv = pm_trace_photon(j->b, j->maxv, j->pu, i, vs);
-        /* the photons: diffuse or glossy vertices past the light (:94-104) */
+        /* the photons: diffuse or glossy vertices past the light (:98-110) */
         for (int k = 1; k < nv; k++) {
             if (!pm_scatters(vs[k].mat)) continue;
             if (j->n == j->cap) {
@@ -2178,18 +2178,18 @@ void orc_pm_samples(const orc_scene *s, const orc_pm_params *pp, const float *ta
  *    point (radius r0, N = 0, tau = 0, throughput, direction to the previous
  *    vertex, vertex count), with the emission there if that surface emits
  *    (a light-only surface is no point and ends the path: directly seen
- *    emitters stay black, :118-143);
+ *    emitters stay black, :131-168);
  *  - pass q: photon subpaths q Np + i of the photon source, roulette 0.5
  *    after every vertex but the endpoint (renderer::bdpt's light-block
  *    layout: vertex j >= 1 owns ORC_PT_SEG0 + 8 (j - 1) .. +7, 0-1 the
  *    direction sampled at vertex j - 1, 3 the roulette after vertex j), a
- *    photon at every diffuse or glossy vertex past the light (:180-219);
+ *    photon at every diffuse or glossy vertex past the light (:204-240);
  *    every point gathers the pass's photons within its radius (M of them,
  *    dtau = sum f x photon throughput) and updates ratio = (N + a M) /
  *    (N + M), tau = (tau + dtau) ratio, r = r sqrt(ratio), N = N + a M
- *    (:238-261);
+ *    (:270-297);
  *  - the film after the last pass: throughput x tau / (pi r^2 P Np) +
- *    emission per point (:266-272).
+ *    emission per point (:300-310).
  * max_num_vertices -1 is unbounded (32-vertex subpaths, every photon). */
 typedef struct {
     int valid;
@@ -2224,7 +2224,7 @@ static int ppm_trace_photon(const bctx *b, int maxv, const usrc *us, uint64_t i,
             const float a = 1.0f / pos_pdf(b, &v);
             v.alpha[0] = v.alpha[1] = v.alpha[2] = a;
             vs[nv++] = v;
-            for (int c = 0; c < 3; c++) run[c] = a;              /* the endpoint skips the roulette (:183-186) */
+            for (int c = 0; c < 3; c++) run[c] = a;              /* the endpoint skips the roulette (:210-214) */
             continue;
         }
         const uint32_t sb = ORC_PT_SEG0 + 8u * (uint32_t)(step - 1);
@@ -2252,7 +2252,7 @@ static int ppm_trace_photon(const bctx *b, int maxv, const usrc *us, uint64_t i,
         v.light = b->tri_light[h.id];
         for (int c = 0; c < 3; c++) v.alpha[c] = run[c] * (fs[c] / pdf);
         vs[nv++] = v;
-        if (U(us, i, sb + 3) > 0.5f) break;                      /* :205-214 */
+        if (U(us, i, sb + 3) > 0.5f) break;                      /* :227-237 */
         for (int c = 0; c < 3; c++) run[c] = v.alpha[c] / 0.5f;
     }
     return nv;
@@ -2487,7 +2487,7 @@ void orc_ppm_points(const orc_scene *s, const orc_ppm_params *pp, const float *t
         memset(o, 0, sizeof(float) * 16);
         const ppm_point *m = &mp[i];
         if (!m->valid) continue;
-        /* :266-272 */
+        /* :303-307 */
         const float p = 1.0f / (m->r * m->r * PI_F * (float)total);
         o[0] = m->rx; o[1] = m->ry;
         for (int c = 0; c < 3; c++) {

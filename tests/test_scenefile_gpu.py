@@ -99,7 +99,7 @@ def test_scene_file_vcm_renderer_node(gpu, mode):
 
 
 def test_scene_file_pm_renderer_node(gpu):
-    """renderer::pm's node (renderer_pm.cpp:61-68) renders as lmx_render_pm
+    """renderer::pm's node (renderer_pm.cpp:61-70) renders as lmx_render_pm
     with the same parameters."""
     sc = scenes.cornell_box(32, 32)
     node = ["  renderer:", "    type: pm", "    params:", "      num_samples: 16384",
@@ -108,6 +108,21 @@ def test_scene_file_pm_renderer_node(gpu):
     film, _ = scenefile.render(scenefile.load_scene(_cornell_yaml(sc, 1, node)), seed=13)
     ref = render.Scene(sc).render_pm(16384, num_photon_trace_samples=20000, finalgather=1, radius=0.05,
                                      max_num_vertices=7, seed=13, raster_mode=render.RASTER_RANDOM)[0]
+    a, b = film.cpu().numpy(), ref.cpu().numpy()
+    assert a.mean() > 0
+    np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-6 * float(b.max()))
+
+
+def test_scene_file_ppm_renderer_node(gpu):
+    """renderer::ppm's node (renderer_ppm.cpp:76-89) renders as
+    lmx_render_ppm with the same parameters."""
+    sc = scenes.cornell_box(32, 32)
+    node = ["  renderer:", "    type: ppm", "    params:", "      num_samples: 4096", "      max_num_vertices: 7",
+            "      num_iteration_pass: 4", "      num_photon_trace_samples: 20000", "      initial_radius: 0.05",
+            "      alpha: 0.6"]
+    film, _ = scenefile.render(scenefile.load_scene(_cornell_yaml(sc, 1, node)), seed=17)
+    ref = render.Scene(sc).render_ppm(4096, 7, num_iteration_pass=4, num_photon_trace_samples=20000,
+                                      initial_radius=0.05, alpha=0.6, seed=17, raster_mode=render.RASTER_RANDOM)[0]
     a, b = film.cpu().numpy(), ref.cpu().numpy()
     assert a.mean() > 0
     np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-6 * float(b.max()))
