@@ -256,6 +256,14 @@ typedef struct orc_ppm_params {
 void orc_ppm_points(const orc_scene *s, const orc_ppm_params *p, const float *table, uint32_t table_width,
                     const float *ptable, uint32_t ptable_width, uint64_t n, float *out, int nthreads);
 
+/* renderer::sppm (orc_render.c): a point per pixel over num_passes passes;
+ * pass q's eye subpath of pixel i uses uniform index q W H + i (table rows
+ * P W H), its photons rows q num_photons + i as orc_ppm_points.  Output per
+ * pixel 16 floats: film C rgb, emission sum rgb, tau rgb, N, radius, passes
+ * with a point, 0 x 4. */
+void orc_sppm_pixels(const orc_scene *s, const orc_ppm_params *p, const float *table, uint32_t table_width,
+                     const float *ptable, uint32_t ptable_width, float *out, int nthreads);
+
 /* renderer::bdpt (orc_render.c).  max_num_vertices: -1 (unbounded) or >= 2;
  * min_num_vertices applies only when bounded (renderer_bdpt.cpp:887);
  * mis: 0 power heuristic, 1 simple.  Output per sample: 2 (M+1)^2 records of

@@ -262,6 +262,9 @@ def _render_sigs(so):
     so.orc_ppm_points.argtypes = [vp, ctypes.POINTER(orc_ppm_params), vp, ctypes.c_uint32, vp, ctypes.c_uint32,
                                   ctypes.c_uint64, vp, ctypes.c_int]
     so.orc_ppm_points.restype = None
+    so.orc_sppm_pixels.argtypes = [vp, ctypes.POINTER(orc_ppm_params), vp, ctypes.c_uint32, vp, ctypes.c_uint32,
+                                   vp, ctypes.c_int]
+    so.orc_sppm_pixels.restype = None
     so.orc_film_rgbe.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
     so.orc_film_rgbe.restype = None
     so.orc_film_png.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
@@ -484,6 +487,24 @@ class Scene:
             ptable = np.ascontiguousarray(ptable, np.float32)
             ptw, pp = ptable.shape[1], ctypes.c_void_p(ptable.ctypes.data)
         lib().orc_ppm_points(self._h, ctypes.byref(p), tp, tw, pp, ptw, n, _p(out), nthreads)
+        return out
+
+    def sppm_pixels(self, num_passes, num_photons, max_num_vertices, initial_radius=0.1, alpha=0.7, seed=42,
+                    pseed=43, table=None, ptable=None, nthreads=8):
+        """renderer::sppm: (H W, 16) float32 per pixel: film C rgb, emission
+        sum rgb, tau rgb, N, radius, passes with a point, 0 x 4."""
+        p = orc_ppm_params(max_num_vertices, seed & 0xFFFFFFFF, seed >> 32, pseed & 0xFFFFFFFF, pseed >> 32,
+                           RASTER_STRATIFIED, num_passes, num_photons, initial_radius, alpha)
+        out = np.zeros((self.height * self.width, 16), np.float32)
+        tw = ptw = 0
+        tp = pp = None
+        if table is not None:
+            table = np.ascontiguousarray(table, np.float32)
+            tw, tp = table.shape[1], ctypes.c_void_p(table.ctypes.data)
+        if ptable is not None:
+            ptable = np.ascontiguousarray(ptable, np.float32)
+            ptw, pp = ptable.shape[1], ctypes.c_void_p(ptable.ctypes.data)
+        lib().orc_sppm_pixels(self._h, ctypes.byref(p), tp, tw, pp, ptw, _p(out), nthreads)
         return out
 
     def film(self, samples):

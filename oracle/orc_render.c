@@ -2301,6 +2301,7 @@ typedef struct {
     int dim[3];
     const int *cstart, *items;
     uint64_t b0, e0;
+    int sppm;               /* renderer::sppm: tau gains throughput x dtau (renderer_sppm.cpp:314) */
 } ppm_ujob;
 
 static void *ppm_update(void *p)
@@ -2338,7 +2339,7 @@ static void *ppm_update(void *p)
                 }
         if (m->N + M == 0.0f) continue;
         const float ratio = (m->N + pp->alpha * M) / (m->N + M);
-        for (int c = 0; c < 3; c++) m->tau[c] = (m->tau[c] + dtau[c]) * ratio;
+        for (int c = 0; c < 3; c++) m->tau[c] = (m->tau[c] + (j->sppm ? m->thr[c] * dtau[c] : dtau[c])) * ratio;
         m->r = m->r * sqrtf(ratio);
         m->N = m->N + pp->alpha * M;
     }
@@ -2384,43 +2385,18 @@ static void *ppm_points(void *p)
     return NULL;
 }
 
-void orc_ppm_points(const orc_scene *s, const orc_ppm_params *pp, const float *table, uint32_t table_width,
-                    const float *ptable, uint32_t ptable_width, uint64_t n, float *out, int nthreads)
+/* one photon pass of renderer::ppm / sppm: the pass's photon subpaths
+ * (photon rows pass Np + i), a uniform grid of cell >= every radius, and the
+ * progressive update of the n points */
+static void ppm_pass(const bctx *b, const orc_ppm_params *pp, const usrc *pu, uint32_t pass, ppm_point *mp,
+                     uint64_t n, int sppm, int nthreads)
 {
-    bctx b;
-    b.s = s;
-    camera_init(&s->d, &b.cam);
-    b.tri_light = (int *)malloc(sizeof(int) * (s->d.ntri ? s->d.ntri : 1));
-    for (uint32_t i = 0; i < s->d.ntri; i++) b.tri_light[i] = -1;
-    for (uint32_t l = 0; l < s->d.nlight; l++)
-        for (uint32_t k = 0; k < s->d.light_prims[2 * l + 1]; k++) b.tri_light[s->d.light_prims[2 * l] + k] = (int)l;
-    b.tri_cdf_base = NULL;
-    if (nthreads < 1) nthreads = 1;
-    if (nthreads > 256) nthreads = 256;
-    ppm_point *mp = (ppm_point *)calloc(n + 1, sizeof(ppm_point));
     pthread_t th[256];
-    {
-        ppm_ejob ej[256];
-        for (int t = 0; t < nthreads; t++) {
-            ej[t].b = &b; ej[t].pp = pp; ej[t].mp = mp;
-            ej[t].us.table = table; ej[t].us.width = table_width;
-            ej[t].us.seed = ((uint64_t)pp->seed_hi << 32) | pp->seed_lo;
-            ej[t].us.first = 0;
-            ej[t].lo = n * (uint64_t)t / (uint64_t)nthreads;
-            ej[t].hi = n * (uint64_t)(t + 1) / (uint64_t)nthreads;
-            pthread_create(&th[t], NULL, ppm_points, &ej[t]);
-        }
-        for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
-    }
-    usrc pu;
-    pu.table = ptable; pu.width = ptable_width;
-    pu.seed = ((uint64_t)pp->pseed_hi << 32) | pp->pseed_lo;
-    pu.first = 0;
     const uint32_t np = pp->num_photons;
-    for (uint32_t pass = 0; pass < pp->num_passes; pass++) {
+    {
         ppm_pjob pj[256];
         for (int t = 0; t < nthreads; t++) {
-            pj[t].b = &b; pj[t].maxv = pp->max_num_vertices; pj[t].pu = &pu;
+            pj[t].b = b; pj[t].maxv = pp->max_num_vertices; pj[t].pu = pu;
             pj[t].first = (uint64_t)pass * np;
             pj[t].lo = (uint32_t)((uint64_t)np * (uint64_t)t / (uint64_t)nthreads);
             pj[t].hi = (uint32_t)((uint64_t)np * (uint64_t)(t + 1) / (uint64_t)nthreads);
@@ -2471,7 +2447,7 @@ void orc_ppm_points(const orc_scene *s, const orc_ppm_params *pp, const float *t
         free(fill); free(cellof);
         ppm_ujob uj[256];
         for (int t = 0; t < nthreads; t++) {
-            uj[t].b = &b; uj[t].pp = pp; uj[t].mp = mp; uj[t].ph = ph; uj[t].nph = m;
+            uj[t].b = b; uj[t].pp = pp; uj[t].mp = mp; uj[t].ph = ph; uj[t].nph = m; uj[t].sppm = sppm;
             for (int a = 0; a < 3; a++) { uj[t].lo[a] = lo[a]; uj[t].dim[a] = dim[a]; }
             uj[t].cell = cell; uj[t].cstart = cstart; uj[t].items = items;
             uj[t].b0 = n * (uint64_t)t / (uint64_t)nthreads;
@@ -2481,6 +2457,42 @@ void orc_ppm_points(const orc_scene *s, const orc_ppm_params *pp, const float *t
         for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
         free(cstart); free(items); free(ph);
     }
+}
+
+void orc_ppm_points(const orc_scene *s, const orc_ppm_params *pp, const float *table, uint32_t table_width,
+                    const float *ptable, uint32_t ptable_width, uint64_t n, float *out, int nthreads)
+{
+    bctx b;
+    b.s = s;
+    camera_init(&s->d, &b.cam);
+    b.tri_light = (int *)malloc(sizeof(int) * (s->d.ntri ? s->d.ntri : 1));
+    for (uint32_t i = 0; i < s->d.ntri; i++) b.tri_light[i] = -1;
+    for (uint32_t l = 0; l < s->d.nlight; l++)
+        for (uint32_t k = 0; k < s->d.light_prims[2 * l + 1]; k++) b.tri_light[s->d.light_prims[2 * l] + k] = (int)l;
+    b.tri_cdf_base = NULL;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    ppm_point *mp = (ppm_point *)calloc(n + 1, sizeof(ppm_point));
+    pthread_t th[256];
+    {
+        ppm_ejob ej[256];
+        for (int t = 0; t < nthreads; t++) {
+            ej[t].b = &b; ej[t].pp = pp; ej[t].mp = mp;
+            ej[t].us.table = table; ej[t].us.width = table_width;
+            ej[t].us.seed = ((uint64_t)pp->seed_hi << 32) | pp->seed_lo;
+            ej[t].us.first = 0;
+            ej[t].lo = n * (uint64_t)t / (uint64_t)nthreads;
+            ej[t].hi = n * (uint64_t)(t + 1) / (uint64_t)nthreads;
+            pthread_create(&th[t], NULL, ppm_points, &ej[t]);
+        }
+        for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    }
+    usrc pu;
+    pu.table = ptable; pu.width = ptable_width;
+    pu.seed = ((uint64_t)pp->pseed_hi << 32) | pp->pseed_lo;
+    pu.first = 0;
+    const uint32_t np = pp->num_photons;
+    for (uint32_t pass = 0; pass < pp->num_passes; pass++) ppm_pass(&b, pp, &pu, pass, mp, n, 0, nthreads);
     const double total = (double)pp->num_passes * (double)np;
     for (uint64_t i = 0; i < n; i++) {
         float *o = &out[16 * i];
@@ -2496,6 +2508,97 @@ void orc_ppm_points(const orc_scene *s, const orc_ppm_params *pp, const float *t
             o[8 + c] = m->tau[c];
         }
         o[11] = m->N; o[12] = m->r; o[13] = 1.0f;
+    }
+    free(mp); free(b.tri_light);
+}
+
+/* renderer::sppm (renderer_sppm.cpp:95-330): a measurement point per pixel
+ * keeps its radius, N and tau across passes; pass q re-traces pixel i's eye
+ * subpath (uniform index q W H + i, renderer::vcm's eye layout with the
+ * raster jittered inside the pixel, :160-165) to its first diffuse or
+ * glossy vertex, which becomes the point's position for the pass (no such
+ * vertex: the point skips the pass), adding its emission to the point's
+ * sum; then the pass's photons as renderer::ppm's and the update with tau
+ * gaining throughput x dtau; the film after the last pass is, per pixel,
+ * tau / (pi r^2 P Np) + emission / P (SetPixel, no rescale, :320-326). */
+typedef struct { const bctx *b; const orc_ppm_params *pp; usrc us; ppm_point *mp; uint32_t pass; uint64_t lo, hi; } sppm_ejob;
+
+static void *sppm_points(void *p)
+{
+    sppm_ejob *j = (sppm_ejob *)p;
+    const bctx *b = j->b;
+    const orc_ppm_params *pp = j->pp;
+    const uint64_t npix = (uint64_t)b->s->d.width * (uint64_t)b->s->d.height;
+    orc_bdpt_params bp = {2, 0, pp->seed_lo, pp->seed_hi, ORC_RASTER_STRATIFIED};
+    for (uint64_t i = j->lo; i < j->hi; i++) {
+        ppm_point *m = &j->mp[i];
+        m->valid = 0;
+        bvert E[2];
+        const int nE = trace_subpath(b, &bp, &j->us, (uint64_t)j->pass * npix + i, 0, 0, E);
+        if (nE < 2 || !pm_scatters(E[1].mat)) continue;
+        m->valid = 1;
+        m->v = E[1];
+        m->wi = dir_to(&E[1], &E[0]);
+        for (int c = 0; c < 3; c++) m->thr[c] = E[1].alpha[c];
+        m->nv = 2;
+        if (E[1].light >= 0) {
+            bvert lv = E[1];
+            lv.kind = BV_LIGHT;
+            float le[3];
+            v_eval(b, &lv, mk(0, 0, 0), m->wi, 0, le);
+            for (int c = 0; c < 3; c++) m->em[c] += m->thr[c] * le[c];
+        }
+        m->rx += 1.0f;                                   /* passes with a point */
+    }
+    return NULL;
+}
+
+void orc_sppm_pixels(const orc_scene *s, const orc_ppm_params *pp, const float *table, uint32_t table_width,
+                     const float *ptable, uint32_t ptable_width, float *out, int nthreads)
+{
+    bctx b;
+    b.s = s;
+    camera_init(&s->d, &b.cam);
+    b.tri_light = (int *)malloc(sizeof(int) * (s->d.ntri ? s->d.ntri : 1));
+    for (uint32_t i = 0; i < s->d.ntri; i++) b.tri_light[i] = -1;
+    for (uint32_t l = 0; l < s->d.nlight; l++)
+        for (uint32_t k = 0; k < s->d.light_prims[2 * l + 1]; k++) b.tri_light[s->d.light_prims[2 * l] + k] = (int)l;
+    b.tri_cdf_base = NULL;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    const uint64_t n = (uint64_t)s->d.width * (uint64_t)s->d.height;
+    ppm_point *mp = (ppm_point *)calloc(n + 1, sizeof(ppm_point));
+    for (uint64_t i = 0; i < n; i++) mp[i].r = pp->initial_radius;
+    usrc pu;
+    pu.table = ptable; pu.width = ptable_width;
+    pu.seed = ((uint64_t)pp->pseed_hi << 32) | pp->pseed_lo;
+    pu.first = 0;
+    pthread_t th[256];
+    for (uint32_t pass = 0; pass < pp->num_passes; pass++) {
+        sppm_ejob ej[256];
+        for (int t = 0; t < nthreads; t++) {
+            ej[t].b = &b; ej[t].pp = pp; ej[t].mp = mp; ej[t].pass = pass;
+            ej[t].us.table = table; ej[t].us.width = table_width;
+            ej[t].us.seed = ((uint64_t)pp->seed_hi << 32) | pp->seed_lo;
+            ej[t].us.first = 0;
+            ej[t].lo = n * (uint64_t)t / (uint64_t)nthreads;
+            ej[t].hi = n * (uint64_t)(t + 1) / (uint64_t)nthreads;
+            pthread_create(&th[t], NULL, sppm_points, &ej[t]);
+        }
+        for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+        ppm_pass(&b, pp, &pu, pass, mp, n, 1, nthreads);
+    }
+    const double total = (double)pp->num_passes * (double)pp->num_photons;
+    for (uint64_t i = 0; i < n; i++) {
+        float *o = &out[16 * i];
+        memset(o, 0, sizeof(float) * 16);
+        const ppm_point *m = &mp[i];
+        for (int c = 0; c < 3; c++) {
+            o[c] = m->tau[c] / (m->r * m->r * PI_F * (float)total) + m->em[c] / (float)pp->num_passes;
+            o[3 + c] = m->em[c];
+            o[6 + c] = m->tau[c];
+        }
+        o[9] = m->N; o[10] = m->r; o[11] = m->rx;
     }
     free(mp); free(b.tri_light);
 }
