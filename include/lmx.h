@@ -355,6 +355,23 @@ typedef struct lmx_ppm_params {
 int lmx_render_ppm(lmx_scene *scene, const lmx_render_params *params, const lmx_ppm_params *ppm, float *film_rgb,
                    void *stream, lmx_render_stats *stats);
 
+/* renderer::sppm (renderer_sppm.cpp:95-330): one measurement point per
+ * pixel with its own radius, N and tau; each of num_iteration_pass passes
+ * re-traces every pixel's eye sample (raster jittered inside the pixel) to
+ * its first diffuse or glossy vertex -- the point's position for that pass,
+ * whose emission adds to the pixel's sum -- then traces
+ * num_photon_trace_samples photon subpaths as lmx_render_ppm does and
+ * updates every point that has a position (tau gains throughput x dtau).
+ * film_rgb is written, not scaled: per pixel tau / (pi r^2 x all photons) +
+ * emission sum / passes.  params: max_num_vertices (-1 unbounded or 2..32),
+ * seed, uniform_table (rows passes x W x H: pass q's pixel i at row
+ * q W H + i, width >= 8), sample_out (16 floats per pixel: film C rgb,
+ * emission sum rgb, tau rgb, N, radius, passes with a point, 0 x 4);
+ * num_samples, raster_mode and queue_capacity are not used.  The photon
+ * parameters, tables and rows are lmx_render_ppm's. */
+int lmx_render_sppm(lmx_scene *scene, const lmx_render_params *params, const lmx_ppm_params *ppm, float *film_rgb,
+                    void *stream, lmx_render_stats *stats);
+
 /* ---------------------------------------------------------------------------
  * Learned directional sampler.  No counterpart in the reference: a third
  * renderer beside renderer::pt and renderer::vcm, for the callers of

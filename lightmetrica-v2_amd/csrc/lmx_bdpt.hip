@@ -1154,16 +1154,73 @@ __global__ __launch_bounds__(256) void k_ppm_init(RScene s, BParams p, VB vb, fl
     }
 }
 
+// The photons within r of a point v (direction wi to its previous vertex,
+// material m) whose vertex index is at most jmax: M and dtau = sum of f x
+// photon throughput (renderer_ppm.cpp:274-290), through the gather list of
+// k_vc_merge (hl: the workgroup's MERGE_LIST x 256 LDS list).
+RD void ppm_gather(const Vtx& v, f3 wi, const RMat& m, float r, float cell, const Grid& g, const VB& ph, int pmaxv,
+                   int jmax, uint32_t* hl, f3& dtau, float& M)
+{
+    const float r2 = r * r;
+    int lo[3], nc[3];
+    const float q[3] = {v.p.x, v.p.y, v.p.z};
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+        lo[ax] = cell_of(q[ax] - r, cell);
+        nc[ax] = cell_of(q[ax] + r, cell) - lo[ax] + 1;
+    }
+    const int ncell = nc[0] * nc[1] * nc[2];
+    dtau = mk3(0, 0, 0);
+    M = 0.0f;
+    int c = 0;
+    uint32_t it = 0, end = 0;
+    bool fresh = true;
+    while (c < ncell) {
+        uint32_t nh = 0;
+        while (c < ncell && nh < MERGE_LIST) {
+            const int cx = lo[0] + c % nc[0], cy = lo[1] + (c / nc[0]) % nc[1], cz = lo[2] + c / (nc[0] * nc[1]);
+            if (fresh) {
+                const uint32_t h = bucket_of(cx, cy, cz, g.H);
+                it = g.start[h];
+                end = g.start[h + 1];
+                fresh = false;
+            }
+            for (; it < end && nh < MERGE_LIST; it++) {
+                const float4 cd = g.items[it];
+                const int j = (int)(__float_as_uint(cd.w) % (uint32_t)pmaxv);
+                if (j > jmax) continue;
+                if (cell_of(cd.x, cell) != cx || cell_of(cd.y, cell) != cy || cell_of(cd.z, cell) != cz) continue;
+                const f3 d = sub(mk3(cd.x, cd.y, cd.z), v.p);
+                if (!(dot(d, d) < r2)) continue;
+                hl[nh * 256u + threadIdx.x] = __float_as_uint(cd.w);
+                nh++;
+            }
+            if (it >= end) { c++; fresh = true; }
+        }
+        for (uint32_t h = 0; h < nh; h++) {
+            const uint32_t id = hl[h * 256u + threadIdx.x];
+            const float4* rec = ph.v[1] + (size_t)id * 4;
+            const float4 r0 = rec[0], r3 = rec[3], rp = rec[-4];
+            const f3 pp = mk3(r0.x, r0.y, r0.z);
+            const f3 f = bsdf_eval_t(m, v.fr, wi, dir_to(pp, mk3(rp.x, rp.y, rp.z)), false);
+            dtau = add(dtau, mk3(f.x * r3.x, f.y * r3.y, f.z * r3.z));
+            M += 1.0f;
+        }
+    }
+}
+
 // One progressive density-estimation step per measurement point
 // (:270-297): the pass's photons within the point's radius whose merged
-// path has at most max_num_vertices vertices give M and dtau = sum of
-// f x photon throughput; then ratio = (N + alpha M) / (N + M), tau = (tau +
-// dtau) ratio, r = r sqrt(ratio), N = N + alpha M.
+// path has at most max_num_vertices vertices give M and dtau; then ratio =
+// (N + alpha M) / (N + M), tau = (tau + dtau) ratio, r = r sqrt(ratio),
+// N = N + alpha M.
 __global__ __launch_bounds__(256) void k_ppm_update(RScene s, BParams p, const BCtl* __restrict__ ctl, VB vb, VB ph,
                                                     Grid g, PMArgs a, float4* __restrict__ st, float alpha)
 {
     __shared__ uint32_t hl[MERGE_LIST * 256];
     const float cell = ctl->cell;
+    // photon vertex j joins when 2 + (j + 1) - 1 <= maxv
+    const int jmax = a.maxv == -1 ? 0x7fffffff : a.maxv - 2;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.batch; i += gridDim.x * blockDim.x) {
         float4 s0 = st[2 * (size_t)i];
         const float4 s1 = st[2 * (size_t)i + 1];
@@ -1171,62 +1228,85 @@ __global__ __launch_bounds__(256) void k_ppm_update(RScene s, BParams p, const B
         if (!(r > 0.0f)) continue;
         const Vtx v = load_v(vb, p, 0, i, 1u);
         const float4 c0 = vb.v[0][(size_t)i * (size_t)p.maxv * 4];
-        const f3 wi = dir_to(v.p, mk3(c0.x, c0.y, c0.z));
-        const RMat m = s.mats[v.mat];
-        const float r2 = r * r;
-        int lo[3], nc[3];
-        const float q[3] = {v.p.x, v.p.y, v.p.z};
-#pragma unroll
-        for (int ax = 0; ax < 3; ax++) {
-            lo[ax] = cell_of(q[ax] - r, cell);
-            nc[ax] = cell_of(q[ax] + r, cell) - lo[ax] + 1;
-        }
-        const int ncell = nc[0] * nc[1] * nc[2];
-        // photon vertex j joins when 2 + (j + 1) - 1 <= maxv
-        const int jmax = a.maxv == -1 ? 0x7fffffff : a.maxv - 2;
-        f3 dtau = mk3(0, 0, 0);
-        float M = 0.0f;
-        int c = 0;
-        uint32_t it = 0, end = 0;
-        bool fresh = true;
-        while (c < ncell) {
-            uint32_t nh = 0;
-            while (c < ncell && nh < MERGE_LIST) {
-                const int cx = lo[0] + c % nc[0], cy = lo[1] + (c / nc[0]) % nc[1], cz = lo[2] + c / (nc[0] * nc[1]);
-                if (fresh) {
-                    const uint32_t h = bucket_of(cx, cy, cz, g.H);
-                    it = g.start[h];
-                    end = g.start[h + 1];
-                    fresh = false;
-                }
-                for (; it < end && nh < MERGE_LIST; it++) {
-                    const float4 cd = g.items[it];
-                    const int j = (int)(__float_as_uint(cd.w) % (uint32_t)a.pmaxv);
-                    if (j > jmax) continue;
-                    if (cell_of(cd.x, cell) != cx || cell_of(cd.y, cell) != cy || cell_of(cd.z, cell) != cz) continue;
-                    const f3 d = sub(mk3(cd.x, cd.y, cd.z), v.p);
-                    if (!(dot(d, d) < r2)) continue;
-                    hl[nh * 256u + threadIdx.x] = __float_as_uint(cd.w);
-                    nh++;
-                }
-                if (it >= end) { c++; fresh = true; }
-            }
-            for (uint32_t h = 0; h < nh; h++) {
-                const uint32_t id = hl[h * 256u + threadIdx.x];
-                const float4* rec = ph.v[1] + (size_t)id * 4;
-                const float4 r0 = rec[0], r3 = rec[3], rp = rec[-4];
-                const f3 pp = mk3(r0.x, r0.y, r0.z);
-                const f3 f = bsdf_eval_t(m, v.fr, wi, dir_to(pp, mk3(rp.x, rp.y, rp.z)), false);
-                dtau = add(dtau, mk3(f.x * r3.x, f.y * r3.y, f.z * r3.z));
-                M += 1.0f;
-            }
-        }
+        f3 dtau;
+        float M;
+        ppm_gather(v, dir_to(v.p, mk3(c0.x, c0.y, c0.z)), s.mats[v.mat], r, cell, g, ph, a.pmaxv, jmax, hl, dtau, M);
         if (s0.w + M == 0.0f) continue;
         const float ratio = (s0.w + alpha * M) / (s0.w + M);
         s0 = make_float4((s0.x + dtau.x) * ratio, (s0.y + dtau.y) * ratio, (s0.z + dtau.z) * ratio,
                          s0.w + alpha * M);
         st[2 * (size_t)i] = s0;
         st[2 * (size_t)i + 1] = make_float4(s1.x, s1.y, s1.z, r * sqrtf(ratio));
+    }
+}
+
+// renderer::sppm's pass (renderer_sppm.cpp:140-316) per pixel i: this
+// pass's eye vertex 1 is the point when it is diffuse or glossy (else the
+// pixel skips the pass); its emission adds to the pixel's sum; then the
+// update of k_ppm_update with tau gaining throughput x dtau.  State per
+// pixel: st[3 i] = (tau, N), st[3 i + 1] = (emission sum, radius),
+// st[3 i + 2] = (passes with a point, 0, 0, 0).
+__global__ __launch_bounds__(256) void k_sppm_update(RScene s, BParams p, const BCtl* __restrict__ ctl, VB vb,
+                                                     VB ph, Grid g, PMArgs a, float4* __restrict__ st, float alpha)
+{
+    __shared__ uint32_t hl[MERGE_LIST * 256];
+    const float cell = ctl->cell;
+    const int jmax = a.maxv == -1 ? 0x7fffffff : a.maxv - 2;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.batch; i += gridDim.x * blockDim.x) {
+        if (vb.nsub[0][i] < 2) continue;
+        const Vtx v = load_v(vb, p, 0, i, 1u);
+        const RMat m = s.mats[v.mat];
+        if (!is_scatterer(m)) continue;
+        float4 s0 = st[3 * (size_t)i], s1 = st[3 * (size_t)i + 1], s2 = st[3 * (size_t)i + 2];
+        const float4 c0 = vb.v[0][(size_t)i * (size_t)p.maxv * 4];
+        const f3 wi = dir_to(v.p, mk3(c0.x, c0.y, c0.z));
+        // emission at the point (:180-188)
+        if (v.light >= 0 && emits_towards(v.fr, wi)) {
+            s1.x += v.alpha.x * m.Le[0]; s1.y += v.alpha.y * m.Le[1]; s1.z += v.alpha.z * m.Le[2];
+        }
+        s2.x += 1.0f;
+        f3 dtau;
+        float M;
+        ppm_gather(v, wi, m, s1.w, cell, g, ph, a.pmaxv, jmax, hl, dtau, M);
+        if (s0.w + M != 0.0f) {
+            const float ratio = (s0.w + alpha * M) / (s0.w + M);
+            s0 = make_float4((s0.x + v.alpha.x * dtau.x) * ratio, (s0.y + v.alpha.y * dtau.y) * ratio,
+                             (s0.z + v.alpha.z * dtau.z) * ratio, s0.w + alpha * M);
+            s1.w = s1.w * sqrtf(ratio);
+        }
+        st[3 * (size_t)i] = s0;
+        st[3 * (size_t)i + 1] = s1;
+        st[3 * (size_t)i + 2] = s2;
+    }
+}
+
+// renderer::sppm's film (:320-326): pixel i = tau / (pi r^2 N_total) +
+// emission / passes, written (not splatted).
+__global__ __launch_bounds__(256) void k_sppm_film(BParams p, const float4* __restrict__ st, float total,
+                                                   float passes)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.batch; i += gridDim.x * blockDim.x) {
+        const float4 s0 = st[3 * (size_t)i], s1 = st[3 * (size_t)i + 1], s2 = st[3 * (size_t)i + 2];
+        const float d = s1.w * s1.w * LMX_PI_F * total;
+        const f3 C = mk3(s0.x / d + s1.x / passes, s0.y / d + s1.y / passes, s0.z / d + s1.z / passes);
+        float* fp = p.film + 3 * (size_t)i;
+        fp[0] = C.x; fp[1] = C.y; fp[2] = C.z;
+        if (p.sample_out) {
+            float4* o = reinterpret_cast<float4*>(p.sample_out + 16 * (size_t)i);
+            o[0] = make_float4(C.x, C.y, C.z, s1.x);
+            o[1] = make_float4(s1.y, s1.z, s0.x, s0.y);
+            o[2] = make_float4(s0.z, s0.w, s1.w, s2.x);
+            o[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    }
+}
+
+__global__ void k_sppm_init(float4* __restrict__ st, uint32_t n, float r0)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        st[3 * (size_t)i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        st[3 * (size_t)i + 1] = make_float4(0.0f, 0.0f, 0.0f, r0);
+        st[3 * (size_t)i + 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
 }
 
@@ -2012,6 +2092,132 @@ int render_ppm(lmx_scene* sc, const lmx_render_params* p, const lmx_ppm_params* 
     return rc;
 }
 
+// renderer::sppm: per pass the pixels' points (one eye batch of W H
+// samples, 2-vertex subpaths), the photon graph, the per-pixel update; the
+// film written after the last pass.
+int render_sppm(lmx_scene* sc, const lmx_render_params* p, const lmx_ppm_params* v, float* film_rgb, void* stream,
+                lmx_render_stats* stats)
+{
+    const char* name = "lmx_render_sppm";
+    char msg[160];
+    if (!sc || !p || !v || !film_rgb) return set_error(LMX_EINVAL, "lmx_render_sppm: NULL argument");
+    const bool bounded = p->max_num_vertices != -1;
+    if (bounded && (p->max_num_vertices < 2 || p->max_num_vertices > 32))
+        return set_error(LMX_EINVAL, "lmx_render_sppm: max_num_vertices must be in [2, 32] or -1");
+    if (v->num_iteration_pass == 0) return set_error(LMX_EINVAL, "lmx_render_sppm: num_iteration_pass must be >= 1");
+    if (v->num_photon_trace_samples == 0 || v->num_photon_trace_samples > (1u << 26))
+        return set_error(LMX_EINVAL, "lmx_render_sppm: num_photon_trace_samples must be in [1, 2^26]");
+    const uint32_t npix = (uint32_t)sc->width * (uint32_t)sc->height;
+    if ((uint64_t)v->num_iteration_pass * v->num_photon_trace_samples > 0xffffffffull ||
+        (uint64_t)v->num_iteration_pass * npix > 0xffffffffull)
+        return set_error(LMX_EINVAL, "lmx_render_sppm: passes x photons and passes x pixels must stay below 2^32");
+    if (!(v->initial_radius > 0.0f) || !(v->initial_radius < 3.0e38f) || !(v->alpha > 0.0f) || !(v->alpha <= 1.0f))
+        return set_error(LMX_EINVAL, "lmx_render_sppm: initial_radius must be positive and finite, alpha in (0, 1]");
+    if (sc->mats.n > VB_MAX_MATERIALS || sc->lights.n > VB_MAX_LIGHTS)
+        return set_error(LMX_EINVAL, "lmx_render_sppm: at most 65536 materials and 8190 lights");
+    const uint32_t pcap = bounded ? (uint32_t)p->max_num_vertices : 32u;
+    const uint32_t ewidth = LMX_PT_SLOT_SEG0;
+    const uint32_t pwidth = LMX_PT_SLOT_SEG0 + 8u * (pcap - 1u);
+    if ((p->uniform_table && p->table_width < ewidth) || (v->photon_table && v->photon_table_width < pwidth)) {
+        snprintf(msg, sizeof msg, "%s: uniform table narrower than %u (eye) / %u (photon)", name, ewidth, pwidth);
+        return set_error(LMX_EINVAL, msg);
+    }
+    hipStream_t st = (hipStream_t)stream;
+    hipStream_t own = nullptr;
+    hipError_t e;
+    if (!st) {
+        if ((e = hipStreamCreateWithFlags(&own, hipStreamNonBlocking)) != hipSuccess) return hip_error(e, "stream create");
+        st = own;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t np = v->num_photon_trace_samples;
+    BParams bp;                          // the pixels' points of one pass
+    bp.variant = BD_VCM;
+    bp.merging = 0; bp.connect = 0; bp.phase = 0; bp.c_conn = 1.0f;
+    bp.mis = LMX_MIS_POWER;
+    bp.bounded = 1;
+    bp.maxv = 2; bp.minv = 0;
+    bp.key = make_uint2((uint32_t)p->seed, (uint32_t)(p->seed >> 32));
+    bp.total = npix; bp.batch = npix;
+    bp.raster_mode = LMX_RASTER_STRATIFIED;          // pixel i, jittered inside it (renderer_sppm.cpp:160)
+    bp.table = p->uniform_table; bp.table_width = p->table_width;
+    bp.wsub = ewidth;
+    bp.sample_out = p->sample_out;
+    bp.film = film_rgb;
+    BParams pp = bp;                     // photon subpaths
+    pp.variant = BD_PPM;
+    pp.phase = 1;
+    pp.bounded = bounded ? 1 : 0;
+    pp.maxv = (int)pcap;
+    pp.batch = np; pp.total = np;
+    pp.key = make_uint2((uint32_t)v->photon_seed, (uint32_t)(v->photon_seed >> 32));
+    pp.table = v->photon_table; pp.table_width = v->photon_table_width;
+    pp.wsub = 0;
+    pp.sample_out = nullptr;
+    PMArgs pa{(int32_t)pcap, bounded ? p->max_num_vertices : -1, 1, (float)np};
+    int rc = bd_buffers(sc, npix, 2, 1);
+    if (rc == LMX_OK) rc = vc_buffers(sc, np, pcap, 1);
+    if (rc == LMX_OK && sc->ppm_st.n < 3 * (size_t)npix && (e = sc->ppm_st.alloc(3 * (size_t)npix)) != hipSuccess)
+        rc = hip_error(e, "sppm pixel points");
+    const int mgrid = std::max(1, std::min((int)((npix + 255) / 256), sc->cus * 16));
+    const float r0 = v->initial_radius, cell = (float)(2.0 * (double)r0);
+    VB vb;
+    vb.v[0] = sc->bd_vert[0].p; vb.v[1] = sc->bd_vert[1].p;
+    vb.nsub[0] = sc->bd_nsub[0].p; vb.nsub[1] = sc->bd_nsub[1].p;
+    vb.epix = sc->bd_epix.p; vb.eraster = sc->bd_eraster.p;
+    vb.V = sc->bd_V.p;
+    vb.mq = nullptr;
+    VB ph = vb;
+    ph.v[1] = sc->vc_vert.p; ph.nsub[1] = sc->vc_nsub.p;
+    const Grid g{sc->vc_items.p, sc->vc_start.p, sc->vc_cursor.p, sc->vc_part.p, sc->vc_H};
+    hipGraphExec_t gph = nullptr, geye = nullptr;
+    if (rc == LMX_OK) {
+        k_bd_init<<<1, 64, 0, st>>>(sc->bd_ctl.p);
+        k_sppm_init<<<mgrid, 256, 0, st>>>(sc->ppm_st.p, npix, r0);
+        rc = capture(st, &gph, [&] { return vc_enqueue_photons(sc, pp, st); });
+    }
+    if (rc == LMX_OK)
+        rc = capture(st, &geye, [&] {
+            Queue q[2];
+            for (int k = 0; k < 2; k++) { q[k].ray = sc->bd_ray[k].p; q[k].thr = sc->bd_thr[k].p; q[k].meta = sc->bd_meta[k].p; }
+            BCtl* ctl = sc->bd_ctl.p;
+            k_bd_gen<<<(npix + 255) / 256, 256, 0, st>>>(sc->rs, bp, ctl, vb, q[0]);
+            int rc2 = launch_trace(sc->accel, reinterpret_cast<const WRay*>(q[0].ray), 0, &ctl->n_trace, npix,
+                                   reinterpret_cast<WHit*>(sc->bd_hits.p), st, false);
+            if (rc2 != LMX_OK) return rc2;
+            k_bd_shade<<<(npix + 255) / 256, 256, 0, st>>>(sc->rs, bp, ctl, q[0], sc->bd_hits.p, q[1], vb);
+            k_bd_commit<<<1, 64, 0, st>>>(ctl);
+            const hipError_t e2 = hipGetLastError();
+            return e2 == hipSuccess ? LMX_OK : hip_error(e2, "sppm launch");
+        });
+    for (uint32_t q = 0; rc == LMX_OK && q < v->num_iteration_pass; q++) {
+        k_bd_phase<<<1, 64, 0, st>>>(sc->bd_ctl.p, q * npix, r0, cell, 0.0f);
+        if ((e = hipGraphLaunch(geye, st)) != hipSuccess) { rc = hip_error(e, "graph launch"); break; }
+        k_bd_phase<<<1, 64, 0, st>>>(sc->bd_ctl.p, q * np, r0, cell, 0.0f);
+        if ((e = hipGraphLaunch(gph, st)) != hipSuccess) { rc = hip_error(e, "graph launch"); break; }
+        k_sppm_update<<<mgrid, 256, 0, st>>>(sc->rs, bp, sc->bd_ctl.p, vb, ph, g, pa, sc->ppm_st.p, v->alpha);
+    }
+    if (rc == LMX_OK) {
+        const float total = (float)((double)v->num_iteration_pass * (double)np);
+        k_sppm_film<<<mgrid, 256, 0, st>>>(bp, sc->ppm_st.p, total, (float)v->num_iteration_pass);
+        if ((e = hipGetLastError()) != hipSuccess) rc = hip_error(e, "sppm launch");
+    }
+    if (rc == LMX_OK &&
+        ((e = hipMemcpyAsync(sc->bd_hctl, sc->bd_ctl.p, sizeof(BCtl), hipMemcpyDeviceToHost, st)) != hipSuccess ||
+         (e = hipStreamSynchronize(st)) != hipSuccess))
+        rc = hip_error(e, "lmx_render_sppm: sync");
+    if (gph) (void)hipGraphExecDestroy(gph);
+    if (geye) (void)hipGraphExecDestroy(geye);
+    if (own) (void)hipStreamDestroy(own);
+    if (stats) {
+        stats->rays = rc == LMX_OK ? sc->bd_hctl->rays : 0;
+        stats->iterations = v->num_iteration_pass;
+        stats->queue_capacity = npix;
+        stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2043,6 +2249,12 @@ int lmx_render_ppm(lmx_scene* sc, const lmx_render_params* p, const lmx_ppm_para
                    void* stream, lmx_render_stats* stats)
 {
     return render_ppm(sc, p, v, film_rgb, stream, stats);
+}
+
+int lmx_render_sppm(lmx_scene* sc, const lmx_render_params* p, const lmx_ppm_params* v, float* film_rgb,
+                    void* stream, lmx_render_stats* stats)
+{
+    return render_sppm(sc, p, v, film_rgb, stream, stats);
 }
 
 }  // extern "C"

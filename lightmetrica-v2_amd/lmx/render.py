@@ -257,6 +257,46 @@ class Scene:
         stats = {"rays": st.rays, "passes": st.iterations, "ms": st.ms}
         return film, stats, so
 
+    def render_sppm(self, max_num_vertices, num_iteration_pass=1000, num_photon_trace_samples=100,
+                    initial_radius=0.1, alpha=0.7, seed=42, photon_seed=None, table=None, photon_table=None,
+                    sample_out=False, stream=None):
+        """renderer::sppm (defaults: renderer_sppm.cpp:84-96): a point per
+        pixel.  Returns (film (H, W, 3), stats, per-pixel records (W H, 16) or
+        None: film C rgb, emission sum rgb, tau rgb, N, radius, passes with a
+        point)."""
+        import torch
+        dev = torch.device("cuda")
+        npix = self.width * self.height
+        film = torch.empty((self.height, self.width, 3), dtype=torch.float32, device=dev)
+        p = self._params(npix, max_num_vertices, 0, 1, 0.5, seed, RASTER_STRATIFIED, 0)
+        v = lmx_ppm_params()
+        v.num_iteration_pass = num_iteration_pass
+        v.num_photon_trace_samples = num_photon_trace_samples
+        v.initial_radius = initial_radius
+        v.alpha = alpha
+        v.photon_seed = (seed + 1) if photon_seed is None else photon_seed
+        for t, n in ((table, num_iteration_pass * npix), (photon_table, num_iteration_pass * num_photon_trace_samples)):
+            if t is not None:
+                assert t.is_cuda and t.dtype == torch.float32 and t.is_contiguous() and t.shape[0] == n
+        if table is not None:
+            p.uniform_table = ctypes.c_void_p(table.data_ptr())
+            p.table_width = table.shape[1]
+        if photon_table is not None:
+            v.photon_table = ctypes.c_void_p(photon_table.data_ptr())
+            v.photon_table_width = photon_table.shape[1]
+        so = None
+        if sample_out:
+            so = torch.zeros((npix, 16), dtype=torch.float32, device=dev)
+            p.sample_out = ctypes.c_void_p(so.data_ptr())
+        st = lmx_render_stats()
+        if stream is None:
+            stream = torch.cuda.Stream(dev)
+            stream.wait_stream(torch.cuda.current_stream(dev))
+        check(lib.lmx_render_sppm(self._h, ctypes.byref(p), ctypes.byref(v), ctypes.c_void_p(film.data_ptr()),
+                                  ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st)))
+        stats = {"rays": st.rays, "passes": st.iterations, "ms": st.ms}
+        return film, stats, so
+
     def _bidir(self, entry, nrec, num_samples, max_num_vertices, min_num_vertices, seed, raster_mode, batch,
                table, sample_out, stream, mis):
         import torch

@@ -356,8 +356,8 @@ def load_scene(source, base_dir=None):
 
 def render(loaded, seed=42, raster_mode=None):
     """Renders a load_scene result with its renderer node (pt, ptdirect,
-    ptmis, bdpt, pm, ppm; vcm in its bdpt, vcm and bdpm modes) and the
-    reference's parameter defaults.
+    ptmis, bdpt, pm, ppm, sppm; vcm in its bdpt, vcm and bdpm modes) and
+    the reference's parameter defaults.
     Returns (film tensor, stats)."""
     from . import render as R
     rs = R.Scene(loaded["scene"])
@@ -406,6 +406,15 @@ def render(loaded, seed=42, raster_mode=None):
                                     num_photon_trace_samples=int(p.get("num_photon_trace_samples", 100)),
                                     initial_radius=float(p.get("initial_radius", 0.1)),
                                     alpha=float(p.get("alpha", 0.7)), seed=seed, raster_mode=rm)
+    elif t == "sppm":
+        # renderer_sppm.cpp:84-96: max_num_vertices is required there; a point per pixel
+        if "max_num_vertices" not in p:
+            raise SceneFileError("renderer::sppm needs max_num_vertices")
+        film, st, _ = rs.render_sppm(int(p["max_num_vertices"]),
+                                     num_iteration_pass=int(p.get("num_iteration_pass", 1000)),
+                                     num_photon_trace_samples=int(p.get("num_photon_trace_samples", 100)),
+                                     initial_radius=float(p.get("initial_radius", 0.1)),
+                                     alpha=float(p.get("alpha", 0.7)), seed=seed)
     else:
         raise SceneFileError(f"renderer::{t} is not supported here")
     return film, st

@@ -1,4 +1,5 @@
-"""GPU parity of renderer::ppm against the oracle's literal restatement:
+"""GPU parity of renderer::ppm and renderer::sppm against the oracle's literal
+restatements:
 
 * per measurement point from identical eye and photon uniform tables over
   several passes: the same points, emission within 1e-4, and the
@@ -73,3 +74,48 @@ def test_ppm_argument_errors(gpu):
     narrow = torch.zeros((2 * 100, 9), dtype=torch.float32, device=gpu)
     with pytest.raises(LmxError, match="narrower"):
         rs.render_ppm(64, 5, num_iteration_pass=2, num_photon_trace_samples=100, photon_table=narrow)
+
+
+@pytest.mark.parametrize("M,boxes", [(6, None), (-1, "phong"), (6, "ggx")])
+def test_sppm_per_pixel_parity_uniform_tables(gpu, M, boxes):
+    """renderer::sppm per pixel over 3 passes from identical eye and photon
+    tables: the same number of passes with a point, emission sums within
+    1e-4, film, tau, N and radius within 1e-4 for >= 97 % of the pixels."""
+    import torch
+    sc = scenes.cornell_box(16, 16, phong_boxes=30.0 if boxes == "phong" else None,
+                            ggx_boxes=0.3 if boxes == "ggx" else None)
+    P, nph, r0 = 3, 20000, 0.05
+    npix = 16 * 16
+    pw = render.table_width(32 if M == -1 else M, nee=True)
+    rng = np.random.default_rng(43)
+    tab = rng.random((P * npix, 8), dtype=np.float32)
+    ptab = rng.random((P * nph, pw), dtype=np.float32)
+    film, _, so = render.Scene(sc).render_sppm(M, num_iteration_pass=P, num_photon_trace_samples=nph,
+                                               initial_radius=r0, table=torch.from_numpy(tab).to(gpu),
+                                               photon_table=torch.from_numpy(ptab).to(gpu), sample_out=True)
+    g = so.cpu().numpy().astype(np.float64)
+    c = orc.Scene(sc).sppm_pixels(P, nph, M, initial_radius=r0, table=tab, ptable=ptab).astype(np.float64)
+    np.testing.assert_array_equal(g[:, 11], c[:, 11])
+    np.testing.assert_allclose(g[:, 3:6], c[:, 3:6], rtol=1e-4, atol=1e-7)
+    np.testing.assert_allclose(film.cpu().numpy().reshape(-1, 3), g[:, 0:3], rtol=0, atol=0)
+    gathered = c[:, 9] > 0
+    assert gathered.mean() > 0.5
+    rel = np.abs(g - c) / np.maximum(np.abs(c), 1e-12)
+    err = np.maximum(rel[:, 0:3].max(axis=1), np.maximum(rel[:, 6:9].max(axis=1), rel[:, 9:11].max(axis=1)))
+    q = np.quantile(err[gathered], [0.5, 0.9, 0.97, 0.99, 1.0])
+    print(f"sppm relative error quantiles (50/90/97/99/100 %): {q}")
+    assert (err[gathered] <= 1e-4).mean() >= 0.97, (err[gathered] > 1e-4).sum()
+    assert (err[gathered] <= 1e-2).mean() >= 0.99
+
+
+def test_sppm_furnace_closed_form(gpu):
+    rho, Le, M, P = 0.5, 1.0, 5, 16
+    sc = scenes.furnace_sphere(albedo=rho, Le=Le, subdiv=5, width=32, height=32)
+    film, _, so = render.Scene(sc).render_sppm(M, num_iteration_pass=P, num_photon_trace_samples=200000,
+                                               initial_radius=0.05, seed=9, sample_out=True)
+    rec = so.cpu().numpy().astype(np.float64)
+    assert (rec[:, 11] == P).all()
+    np.testing.assert_allclose(rec[:, 3:6] / P, Le, rtol=1e-5)
+    C = film.cpu().numpy().astype(np.float64).reshape(-1, 3).mean(axis=1)
+    want = sum(Le * rho ** (k - 2) for k in range(2, M + 1))
+    assert abs(C.mean() - want) < 4 * C.std() / np.sqrt(len(C)) + 0.015 * want, (C.mean(), want)
