@@ -226,8 +226,9 @@ def vcm_section(dev, cpu, threads):
     rays and merges), beside the oracle's restatement of one pass on host
     threads over a bounded prefix of eye samples, its photon pass timed
     whole.  renderer::pm on the same frame beside it (16 spp, 2^22 photon
-    subpaths, radius 0.05, final gather) and renderer::ppm (a point per
-    pixel, 32 passes of 2^20 photon subpaths)."""
+    subpaths, radius 0.05, final gather), renderer::ppm (a point per
+    pixel, 32 passes of 2^20 photon subpaths) and renderer::sppm (the same,
+    points re-traced every pass)."""
     import torch
     from lmx import render, scenes
     log("vcm: generating the sphere-grid scene")
@@ -341,6 +342,37 @@ def vcm_section(dev, cpu, threads):
             "kind": "port", "sample": f"the oracle at 3 passes minus 1 pass of the same workload on {threads} "
                                       f"threads ({t1:.1f}s and {t3:.1f}s)"}
         out["ppm"]["gpu_over_cpu_passes"] = round(cpass * passes / dq, 1)
+    # renderer::sppm on the same frame: a point per pixel re-traced every
+    # pass, 32 passes of 2^20 photon subpaths
+    ks = dict(num_iteration_pass=passes, num_photon_trace_samples=nphp, initial_radius=0.05, alpha=0.7, seed=42)
+    rs.render_sppm(10, **dict(ks, num_iteration_pass=2))                       # warm-up
+    torch.cuda.synchronize()
+    tsp = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        fsp, _, _ = rs.render_sppm(10, **ks)
+        torch.cuda.synchronize()
+        tsp.append(time.perf_counter() - t0)
+    dsp = min(tsp)
+    out["sppm"] = {"workload": "sppm: same frame, a point per pixel re-traced every pass, 32 passes of 2^20 "
+                               "photon subpaths, initial radius 0.05, alpha 0.7, at most 10 vertices",
+                   "ms_per_pass": round(dsp / passes * 1e3, 3),
+                   "mphotons_per_s": round(passes * nphp / dsp / 1e6, 1), "s_per_frame": round(dsp, 4),
+                   "image_mean": round(float(fsp.mean()), 5), "cpu_baseline": None}
+    if cpu:
+        log("vcm: cpu baseline (oracle renderer::sppm)")
+        t0 = time.perf_counter()
+        osc.sppm_pixels(1, nphp, 10, nthreads=threads, **mq)
+        t1 = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        osc.sppm_pixels(3, nphp, 10, nthreads=threads, **mq)
+        t3 = time.perf_counter() - t0
+        cpass = max(t3 - t1, 1e-6) / 2.0
+        out["sppm"]["cpu_baseline"] = {
+            "ms_per_pass": round(cpass * 1e3, 1), "mphotons_per_s": round(nphp / cpass / 1e6, 3), "cores": threads,
+            "kind": "port", "sample": f"the oracle at 3 passes minus 1 pass of the same workload on {threads} "
+                                      f"threads ({t1:.1f}s and {t3:.1f}s)"}
+        out["sppm"]["gpu_over_cpu_passes"] = round(cpass * passes / dsp, 1)
     return out
 
 

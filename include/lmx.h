@@ -355,7 +355,7 @@ typedef struct lmx_ppm_params {
 int lmx_render_ppm(lmx_scene *scene, const lmx_render_params *params, const lmx_ppm_params *ppm, float *film_rgb,
                    void *stream, lmx_render_stats *stats);
 
-/* renderer::sppm (renderer_sppm.cpp:95-330): one measurement point per
+/* renderer::sppm (renderer_sppm.cpp:98-378): one measurement point per
  * pixel with its own radius, N and tau; each of num_iteration_pass passes
  * re-traces every pixel's eye sample (raster jittered inside the pixel) to
  * its first diffuse or glossy vertex -- the point's position for that pass,

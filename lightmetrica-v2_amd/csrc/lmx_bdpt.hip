@@ -1240,7 +1240,7 @@ __global__ __launch_bounds__(256) void k_ppm_update(RScene s, BParams p, const B
     }
 }
 
-// renderer::sppm's pass (renderer_sppm.cpp:140-316) per pixel i: this
+// renderer::sppm's pass (renderer_sppm.cpp:159-316) per pixel i: this
 // pass's eye vertex 1 is the point when it is diffuse or glossy (else the
 // pixel skips the pass); its emission adds to the pixel's sum; then the
 // update of k_ppm_update with tau gaining throughput x dtau.  State per
@@ -1260,7 +1260,7 @@ __global__ __launch_bounds__(256) void k_sppm_update(RScene s, BParams p, const 
         float4 s0 = st[3 * (size_t)i], s1 = st[3 * (size_t)i + 1], s2 = st[3 * (size_t)i + 2];
         const float4 c0 = vb.v[0][(size_t)i * (size_t)p.maxv * 4];
         const f3 wi = dir_to(v.p, mk3(c0.x, c0.y, c0.z));
-        // emission at the point (:180-188)
+        // emission at the point (:183-191)
         if (v.light >= 0 && emits_towards(v.fr, wi)) {
             s1.x += v.alpha.x * m.Le[0]; s1.y += v.alpha.y * m.Le[1]; s1.z += v.alpha.z * m.Le[2];
         }
@@ -1280,7 +1280,7 @@ __global__ __launch_bounds__(256) void k_sppm_update(RScene s, BParams p, const 
     }
 }
 
-// renderer::sppm's film (:320-326): pixel i = tau / (pi r^2 N_total) +
+// renderer::sppm's film (:321-327): pixel i = tau / (pi r^2 N_total) +
 // emission / passes, written (not splatted).
 __global__ __launch_bounds__(256) void k_sppm_film(BParams p, const float4* __restrict__ st, float total,
                                                    float passes)
@@ -2139,7 +2139,7 @@ int render_sppm(lmx_scene* sc, const lmx_render_params* p, const lmx_ppm_params*
     bp.maxv = 2; bp.minv = 0;
     bp.key = make_uint2((uint32_t)p->seed, (uint32_t)(p->seed >> 32));
     bp.total = npix; bp.batch = npix;
-    bp.raster_mode = LMX_RASTER_STRATIFIED;          // pixel i, jittered inside it (renderer_sppm.cpp:160)
+    bp.raster_mode = LMX_RASTER_STRATIFIED;          // pixel i, jittered inside it (renderer_sppm.cpp:162)
     bp.table = p->uniform_table; bp.table_width = p->table_width;
     bp.wsub = ewidth;
     bp.sample_out = p->sample_out;

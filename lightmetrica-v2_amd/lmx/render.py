@@ -260,7 +260,7 @@ class Scene:
     def render_sppm(self, max_num_vertices, num_iteration_pass=1000, num_photon_trace_samples=100,
                     initial_radius=0.1, alpha=0.7, seed=42, photon_seed=None, table=None, photon_table=None,
                     sample_out=False, stream=None):
-        """renderer::sppm (defaults: renderer_sppm.cpp:84-96): a point per
+        """renderer::sppm (defaults: renderer_sppm.cpp:81-96): a point per
         pixel.  Returns (film (H, W, 3), stats, per-pixel records (W H, 16) or
         None: film C rgb, emission sum rgb, tau rgb, N, radius, passes with a
         point)."""

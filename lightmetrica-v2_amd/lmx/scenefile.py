@@ -407,7 +407,7 @@ def render(loaded, seed=42, raster_mode=None):
                                     initial_radius=float(p.get("initial_radius", 0.1)),
                                     alpha=float(p.get("alpha", 0.7)), seed=seed, raster_mode=rm)
     elif t == "sppm":
-        # renderer_sppm.cpp:84-96: max_num_vertices is required there; a point per pixel
+        # renderer_sppm.cpp:81-96: max_num_vertices is required there; a point per pixel
         if "max_num_vertices" not in p:
             raise SceneFileError("renderer::sppm needs max_num_vertices")
         film, st, _ = rs.render_sppm(int(p["max_num_vertices"]),

@@ -2301,7 +2301,7 @@ typedef struct {
     int dim[3];
     const int *cstart, *items;
     uint64_t b0, e0;
-    int sppm;               /* renderer::sppm: tau gains throughput x dtau (renderer_sppm.cpp:314) */
+    int sppm;               /* renderer::sppm: tau gains throughput x dtau (renderer_sppm.cpp:313) */
 } ppm_ujob;
 
 static void *ppm_update(void *p)
@@ -2512,15 +2512,15 @@ void orc_ppm_points(const orc_scene *s, const orc_ppm_params *pp, const float *t
     free(mp); free(b.tri_light);
 }
 
-/* renderer::sppm (renderer_sppm.cpp:95-330): a measurement point per pixel
+/* renderer::sppm (renderer_sppm.cpp:98-378): a measurement point per pixel
  * keeps its radius, N and tau across passes; pass q re-traces pixel i's eye
  * subpath (uniform index q W H + i, renderer::vcm's eye layout with the
- * raster jittered inside the pixel, :160-165) to its first diffuse or
+ * raster jittered inside the pixel, :159-197) to its first diffuse or
  * glossy vertex, which becomes the point's position for the pass (no such
  * vertex: the point skips the pass), adding its emission to the point's
  * sum; then the pass's photons as renderer::ppm's and the update with tau
  * gaining throughput x dtau; the film after the last pass is, per pixel,
- * tau / (pi r^2 P Np) + emission / P (SetPixel, no rescale, :320-326). */
+ * tau / (pi r^2 P Np) + emission / P (SetPixel, no rescale, :321-327). */
 typedef struct { const bctx *b; const orc_ppm_params *pp; usrc us; ppm_point *mp; uint32_t pass; uint64_t lo, hi; } sppm_ejob;
 
 static void *sppm_points(void *p)

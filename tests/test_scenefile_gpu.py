@@ -126,3 +126,17 @@ def test_scene_file_ppm_renderer_node(gpu):
     a, b = film.cpu().numpy(), ref.cpu().numpy()
     assert a.mean() > 0
     np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-6 * float(b.max()))
+
+
+def test_scene_file_sppm_renderer_node(gpu):
+    """renderer::sppm's node (renderer_sppm.cpp:81-96) renders as
+    lmx_render_sppm with the same parameters."""
+    sc = scenes.cornell_box(32, 32)
+    node = ["  renderer:", "    type: sppm", "    params:", "      max_num_vertices: 7",
+            "      num_iteration_pass: 3", "      num_photon_trace_samples: 20000", "      initial_radius: 0.05"]
+    film, _ = scenefile.render(scenefile.load_scene(_cornell_yaml(sc, 1, node)), seed=19)
+    ref = render.Scene(sc).render_sppm(7, num_iteration_pass=3, num_photon_trace_samples=20000, initial_radius=0.05,
+                                       seed=19)[0]
+    a, b = film.cpu().numpy(), ref.cpu().numpy()
+    assert a.mean() > 0
+    np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-6 * float(b.max()))
