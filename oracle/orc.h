@@ -264,6 +264,18 @@ void orc_ppm_points(const orc_scene *s, const orc_ppm_params *p, const float *ta
 void orc_sppm_pixels(const orc_scene *s, const orc_ppm_params *p, const float *table, uint32_t table_width,
                      const float *ptable, uint32_t ptable_width, float *out, int nthreads);
 
+/* renderer::ltdirect (orc_render.c): samples first_sample .. +n (table
+ * rows from first_sample, or Philox with seed).  max_num_vertices -1
+ * (unbounded) or >= 2.  Output per sample M = max_num_vertices (32 when
+ * unbounded) records of 8 floats, record k (light vertex k to the camera):
+ * raster x, y, C rgb, contributed flag, 0, 0. */
+typedef struct orc_lt_params {
+    int32_t max_num_vertices;
+    uint32_t seed_lo, seed_hi;
+} orc_lt_params;
+void orc_ltdirect_samples(const orc_scene *s, const orc_lt_params *p, const float *table, uint32_t table_width,
+                          uint64_t first_sample, uint64_t n, float *out, int nthreads);
+
 /* renderer::bdpt (orc_render.c).  max_num_vertices: -1 (unbounded) or >= 2;
  * min_num_vertices applies only when bounded (renderer_bdpt.cpp:887);
  * mis: 0 power heuristic, 1 simple.  Output per sample: 2 (M+1)^2 records of

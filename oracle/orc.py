@@ -228,6 +228,10 @@ class orc_ppm_params(ctypes.Structure):
                 ("initial_radius", ctypes.c_float), ("alpha", ctypes.c_float)]
 
 
+class orc_lt_params(ctypes.Structure):
+    _fields_ = [("max_num_vertices", ctypes.c_int32), ("seed_lo", ctypes.c_uint32), ("seed_hi", ctypes.c_uint32)]
+
+
 VCM_VCM = 0
 VCM_BDPM = 2
 
@@ -265,6 +269,9 @@ def _render_sigs(so):
     so.orc_sppm_pixels.argtypes = [vp, ctypes.POINTER(orc_ppm_params), vp, ctypes.c_uint32, vp, ctypes.c_uint32,
                                    vp, ctypes.c_int]
     so.orc_sppm_pixels.restype = None
+    so.orc_ltdirect_samples.argtypes = [vp, ctypes.POINTER(orc_lt_params), vp, ctypes.c_uint32, ctypes.c_uint64,
+                                        ctypes.c_uint64, vp, ctypes.c_int]
+    so.orc_ltdirect_samples.restype = None
     so.orc_film_rgbe.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
     so.orc_film_rgbe.restype = None
     so.orc_film_png.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
@@ -505,6 +512,19 @@ class Scene:
             ptable = np.ascontiguousarray(ptable, np.float32)
             ptw, pp = ptable.shape[1], ctypes.c_void_p(ptable.ctypes.data)
         lib().orc_sppm_pixels(self._h, ctypes.byref(p), tp, tw, pp, ptw, _p(out), nthreads)
+        return out
+
+    def ltdirect_samples(self, first, n, max_num_vertices=-1, seed=42, table=None, nthreads=8):
+        """renderer::ltdirect: (n, M, 8) float32, M = max_num_vertices (32
+        unbounded); record k: raster x, y, C rgb, contributed flag, 0, 0."""
+        m = 32 if max_num_vertices == -1 else max_num_vertices
+        p = orc_lt_params(max_num_vertices, seed & 0xFFFFFFFF, seed >> 32)
+        out = np.zeros((n, m, 8), np.float32)
+        tw, tp = 0, None
+        if table is not None:
+            table = np.ascontiguousarray(table, np.float32)
+            tw, tp = table.shape[1], ctypes.c_void_p(table.ctypes.data)
+        lib().orc_ltdirect_samples(self._h, ctypes.byref(p), tp, tw, first, n, _p(out), nthreads)
         return out
 
     def film(self, samples):

@@ -2602,3 +2602,92 @@ void orc_sppm_pixels(const orc_scene *s, const orc_ppm_params *pp, const float *
     }
     free(mp); free(b.tri_light);
 }
+
+/* ---------------------------------------------------------------------------
+ * renderer::ltdirect (renderer_ltdirect.cpp:72-250), restated literally: per
+ * sample a light subpath whose vertex j >= 1 is kept when the roulette after
+ * reaching it passes (its throughput doubled), and every kept vertex k
+ * (endpoint included, while k + 1 < max_num_vertices) connects to the
+ * pinhole: C = throughput x f_L(towards the camera) x G x V x We, splatted
+ * where the direction projects.  Uniform layout of lmx_render_pm's photons
+ * (slot 6: the roulette before vertex 1; vertex j owns ORC_PT_SEG0 +
+ * 8 (j - 1) .. +7: 0-1 its direction sampled at vertex j - 1, 3 the roulette
+ * before vertex j + 1).  Output per sample: max_num_vertices records of 8
+ * floats, record k: raster x, y, C rgb (before the W H / N film scale), 1
+ * if it contributes, 0, 0. */
+static void lt_sample(const bctx *b, const orc_lt_params *pp, const usrc *us, uint64_t i, float *out, int M)
+{
+    memset(out, 0, sizeof(float) * 8 * (size_t)M);
+    bvert vs[PM_CAP];
+    const int maxv = pp->max_num_vertices;
+    const int nv = pm_trace_photon(b, maxv, us, i, vs);
+    for (int k = 0; k < nv; k++) {
+        if (maxv != -1 && k + 1 >= maxv) break;          /* :100-103 */
+        const bvert *v = &vs[k];
+        bvert cam;
+        memset(&cam, 0, sizeof cam);
+        cam.kind = BV_CAMERA; cam.degenerate = 1; cam.light = -1;
+        cam.fr.p = b->cam.pos;
+        const v3 ppE = dir_to(v, &cam);
+        const v3 wi = k >= 1 ? dir_to(v, &vs[k - 1]) : mk(0, 0, 0);
+        float fsL[3];
+        v_eval(b, v, wi, ppE, 1, fsL);
+        const float fsE = importance(&b->cam, vneg(ppE));
+        const float G = gterm(v, &cam);
+        if (fsE == 0.0f || G == 0.0f || (fsL[0] == 0.0f && fsL[1] == 0.0f && fsL[2] == 0.0f)) continue;
+        /* Scene3::Visible from the vertex to the camera (scene3.h:107-116) */
+        v3 d = vsub(cam.fr.p, v->fr.p);
+        const float l = sqrtf(vdot(d, d));
+        d = mk(d.x / l, d.y / l, d.z / l);
+        orc_ray r;
+        r.o[0] = v->fr.p.x; r.o[1] = v->fr.p.y; r.o[2] = v->fr.p.z; r.tmin = EPS_ISECT;
+        r.d[0] = d.x; r.d[1] = d.y; r.d[2] = d.z; r.tmax = l * (1.0f - EPS_ISECT);
+        orc_hit h;
+        orc_qbvh_intersect_ex(b->s->q, &r, 1, &h, ORC_TEST_WATERTIGHT, ORC_TIE_MIN_ID, 1);
+        if (h.id != 0xffffffffu) continue;
+        float rx, ry;
+        if (!camera_raster(&b->cam, vneg(ppE), &rx, &ry)) continue;
+        float *o = &out[8 * k];
+        o[0] = rx; o[1] = ry;
+        for (int c = 0; c < 3; c++) o[2 + c] = v->alpha[c] * fsL[c] * G * fsE;
+        o[5] = 1.0f;
+    }
+}
+
+typedef struct { const bctx *b; const orc_lt_params *pp; usrc us; float *out; int M; uint64_t first, lo, hi; } lt_job;
+
+static void *lt_run(void *p)
+{
+    lt_job *j = (lt_job *)p;
+    for (uint64_t i = j->lo; i < j->hi; i++) lt_sample(j->b, j->pp, &j->us, j->first + i, &j->out[(size_t)8 * j->M * i], j->M);
+    return NULL;
+}
+
+void orc_ltdirect_samples(const orc_scene *s, const orc_lt_params *pp, const float *table, uint32_t table_width,
+                          uint64_t first, uint64_t n, float *out, int nthreads)
+{
+    bctx b;
+    b.s = s;
+    camera_init(&s->d, &b.cam);
+    b.tri_light = (int *)malloc(sizeof(int) * (s->d.ntri ? s->d.ntri : 1));
+    for (uint32_t i = 0; i < s->d.ntri; i++) b.tri_light[i] = -1;
+    for (uint32_t l = 0; l < s->d.nlight; l++)
+        for (uint32_t k = 0; k < s->d.light_prims[2 * l + 1]; k++) b.tri_light[s->d.light_prims[2 * l] + k] = (int)l;
+    b.tri_cdf_base = NULL;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    const int M = pp->max_num_vertices == -1 ? PM_CAP : pp->max_num_vertices;
+    pthread_t th[256];
+    lt_job jobs[256];
+    for (int t = 0; t < nthreads; t++) {
+        jobs[t].b = &b; jobs[t].pp = pp; jobs[t].out = out; jobs[t].M = M; jobs[t].first = first;
+        jobs[t].us.table = table; jobs[t].us.width = table_width;
+        jobs[t].us.seed = ((uint64_t)pp->seed_hi << 32) | pp->seed_lo;
+        jobs[t].us.first = first;
+        jobs[t].lo = n * (uint64_t)t / (uint64_t)nthreads;
+        jobs[t].hi = n * (uint64_t)(t + 1) / (uint64_t)nthreads;
+        pthread_create(&th[t], NULL, lt_run, &jobs[t]);
+    }
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    free(b.tri_light);
+}
