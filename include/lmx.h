@@ -372,6 +372,21 @@ int lmx_render_ppm(lmx_scene *scene, const lmx_render_params *params, const lmx_
 int lmx_render_sppm(lmx_scene *scene, const lmx_render_params *params, const lmx_ppm_params *ppm, float *film_rgb,
                     void *stream, lmx_render_stats *stats);
 
+/* renderer::ltdirect (renderer_ltdirect.cpp:72-250): num_samples light
+ * subpaths (a vertex j >= 1 is kept when the roulette 0.5 after reaching it
+ * passes, its throughput doubled), every kept vertex k with
+ * k + 1 < max_num_vertices (-1: unbounded, subpaths cut at 32 vertices)
+ * connected to the pinhole: throughput x f(towards the camera) x G x We,
+ * splatted where it projects when the shadow ray is clear; film scaled by
+ * W H / num_samples.  Uniforms: lmx_render_pm's photon layout (table width
+ * >= 8 + 8 (M - 1), M = max_num_vertices or 32).  queue_capacity: light
+ * subpaths per batch (0: 2^21).  sample_out: M records of 8 floats per
+ * sample, record k: raster x, y, C rgb (before the film scale), 1, 0, 0.
+ * renderer::lt (renderer_lt.cpp) only gathers light by hitting the sensor's
+ * geometry, which a pinhole has none of: its image is black. */
+int lmx_render_ltdirect(lmx_scene *scene, const lmx_render_params *params, float *film_rgb, void *stream,
+                        lmx_render_stats *stats);
+
 /* ---------------------------------------------------------------------------
  * Learned directional sampler.  No counterpart in the reference: a third
  * renderer beside renderer::pt and renderer::vcm, for the callers of

@@ -1301,6 +1301,105 @@ __global__ __launch_bounds__(256) void k_sppm_film(BParams p, const float4* __re
     }
 }
 
+// renderer::ltdirect (renderer_ltdirect.cpp:72-250): every kept vertex k of
+// light subpath i (k + 1 < max_num_vertices) connects to the pinhole:
+// C = throughput x f_L(towards the camera) x G x We, with a shadow ray to
+// the camera, appended to the shadow queue (one reservation per workgroup).
+__global__ __launch_bounds__(256) void k_lt_connect(RScene s, BParams p, BCtl* __restrict__ ctl, VB ph, ShadowQ sq,
+                                                    int32_t maxv_user)
+{
+    __shared__ uint32_t lds[5];
+    const size_t n = (size_t)p.batch * (size_t)p.maxv;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    // whole workgroups iterate together (the reservation synchronises them)
+    for (size_t base = (size_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+        const size_t idx = base + threadIdx.x;
+        bool emit = false;
+        float4 r0, r1, cc;
+        float rx = 0.0f, ry = 0.0f;
+        uint32_t i = 0, k = 0, pixel = 0;
+        if (idx < n) {
+            i = (uint32_t)(idx % p.batch);
+            k = (uint32_t)(idx / p.batch);
+            const uint32_t sample = ctl->batch_first + i;
+            if (sample < p.total && k < ph.nsub[1][i] && (maxv_user == -1 || (int)k + 1 < maxv_user)) {
+                const Vtx v = load_v(ph, p, 1, i, k);
+                const f3 cam = s.cam_pos;
+                const f3 ppE = dir_to(v.p, cam);
+                f3 fsL = mk3(0, 0, 0);
+                if (k == 0) {
+                    if (emits_towards(v.fr, ppE)) {
+                        const RMat m = s.mats[v.mat];
+                        fsL = mk3(m.Le[0], m.Le[1], m.Le[2]);
+                    }
+                } else {
+                    const RMat m = s.mats[v.mat];
+                    if (is_scatterer(m)) {
+                        const float4 q = ph.v[1][((size_t)i * (size_t)p.maxv + (size_t)(k - 1)) * 4];
+                        fsL = bsdf_eval_t(m, v.fr, dir_to(v.p, mk3(q.x, q.y, q.z)), ppE, true);
+                    }
+                }
+                const f3 toward = neg(ppE);
+                const float fsE = importance(s, toward);
+                const float G = gterm(v.p, v.sn, v.degenerate, cam, mk3(0, 0, 0), true);
+                if (fsE != 0.0f && G != 0.0f && !(fsL.x == 0.0f && fsL.y == 0.0f && fsL.z == 0.0f) &&
+                    camera_raster(s, toward, rx, ry)) {
+                    pixel = raster_pixel(s, rx, ry);
+                    cc = make_float4(v.alpha.x * fsL.x * G * fsE, v.alpha.y * fsL.y * G * fsE,
+                                     v.alpha.z * fsL.z * G * fsE, 0.0f);
+                    // Scene3::Visible (scene3.h:107-116)
+                    f3 d = sub(cam, v.p);
+                    const float l = sqrtf(dot(d, d));
+                    d = mk3(d.x / l, d.y / l, d.z / l);
+                    r0 = make_float4(v.p.x, v.p.y, v.p.z, LMX_EPS_ISECT);
+                    r1 = make_float4(d.x, d.y, d.z, l * (1.0f - LMX_EPS_ISECT));
+                    emit = true;
+                }
+            }
+        }
+        const uint32_t slot = block_reserve<4>(&ctl->n_shadow, emit ? 1u : 0u, lds);
+        if (emit) {
+            sq.ray[2 * slot] = r0;
+            sq.ray[2 * slot + 1] = r1;
+            sq.c[slot] = cc;
+            sq.aux[slot] = make_float4(rx, ry, 0.0f, 0.0f);
+            sq.meta[slot] = make_uint4(ctl->batch_first + i, k, 1u, pixel);
+        }
+    }
+}
+
+// Unoccluded camera connections to the film (and, when asked, record k of
+// the sample: raster x, y, C rgb, 1, 0, 0).
+__global__ __launch_bounds__(256) void k_lt_splat(BParams p, BCtl* __restrict__ ctl, ShadowQ sq,
+                                                  const float4* __restrict__ hits, int32_t M)
+{
+    const uint32_t n = ctl->n_shadow;
+    uint32_t live = 0;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        live++;
+        if (__float_as_uint(hits[j].w) != WBVH_MISS) continue;
+        const uint4 m = sq.meta[j];
+        const float4 c = sq.c[j], a = sq.aux[j];
+        splat(p, m.w, mk3(c.x, c.y, c.z));
+        if (p.sample_out) {
+            float4* o = reinterpret_cast<float4*>(p.sample_out + 8 * ((size_t)m.x * (size_t)M + m.y));
+            o[0] = make_float4(a.x, a.y, c.x, c.y);
+            o[1] = make_float4(c.z, 1.0f, 0.0f, 0.0f);
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) live += __shfl_xor(live, off);
+    if ((threadIdx.x & 63u) == 0 && live) atomicAdd(&ctl->shadow_rays, (unsigned long long)live);
+}
+
+__global__ void k_lt_batch(BCtl* ctl, uint32_t first)
+{
+    if (threadIdx.x == 0) {
+        ctl->n_trace = 0; ctl->n_next = 0; ctl->n_shadow = 0;
+        ctl->batch_first = first;
+    }
+}
+
 __global__ void k_sppm_init(float4* __restrict__ st, uint32_t n, float r0)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -1651,7 +1750,7 @@ int vc_buffers(lmx_scene* sc, uint32_t np, uint32_t maxv, uint32_t B)
 // Records one photon phase: the photon subpaths (light subpaths traced by the
 // gen / trace / shade kernels into the photon vertex buffer) and the hash
 // grid over their vertices.
-int vc_enqueue_photons(lmx_scene* sc, const BParams& pp, hipStream_t st)
+int vc_enqueue_photons(lmx_scene* sc, const BParams& pp, hipStream_t st, bool grid = true)
 {
     const uint32_t np = pp.batch;
     VB ph;
@@ -1674,6 +1773,10 @@ int vc_enqueue_photons(lmx_scene* sc, const BParams& pp, hipStream_t st)
         k_bd_shade<<<(np + 255) / 256, 256, 0, st>>>(sc->rs, pp, ctl, q[a], sc->vc_hits.p, q[1 - a], ph);
         k_bd_commit<<<1, 64, 0, st>>>(ctl);
         a = 1 - a;
+    }
+    if (!grid) {
+        const hipError_t e = hipGetLastError();
+        return e == hipSuccess ? LMX_OK : hip_error(e, "photon launch");
     }
     const Grid g{sc->vc_items.p, sc->vc_start.p, sc->vc_cursor.p, sc->vc_part.p, sc->vc_H};
     const uint32_t n1 = g.H + 1u, ntile = (n1 + SCAN_TILE - 1) / SCAN_TILE;
@@ -2218,6 +2321,109 @@ int render_sppm(lmx_scene* sc, const lmx_render_params* p, const lmx_ppm_params*
     return rc;
 }
 
+// renderer::ltdirect: batches of B light subpaths through the photon graph
+// (no grid), their camera connections, the any-hit trace, the splat; film
+// scaled by W H / N.
+int render_ltdirect(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, void* stream, lmx_render_stats* stats)
+{
+    const char* name = "lmx_render_ltdirect";
+    char msg[160];
+    if (!sc || !p || !film_rgb) return set_error(LMX_EINVAL, "lmx_render_ltdirect: NULL argument");
+    if (p->num_samples == 0 || p->num_samples > 0xffffffffull)
+        return set_error(LMX_EINVAL, "lmx_render_ltdirect: num_samples must be in [1, 2^32)");
+    const bool bounded = p->max_num_vertices != -1;
+    if (bounded && (p->max_num_vertices < 2 || p->max_num_vertices > 32))
+        return set_error(LMX_EINVAL, "lmx_render_ltdirect: max_num_vertices must be in [2, 32] or -1");
+    if (sc->mats.n > VB_MAX_MATERIALS || sc->lights.n > VB_MAX_LIGHTS)
+        return set_error(LMX_EINVAL, "lmx_render_ltdirect: at most 65536 materials and 8190 lights");
+    const uint32_t pcap = bounded ? (uint32_t)p->max_num_vertices : 32u;
+    const uint32_t width = LMX_PT_SLOT_SEG0 + 8u * (pcap - 1u);
+    if (p->uniform_table && p->table_width < width) {
+        snprintf(msg, sizeof msg, "%s: uniform table narrower than %u", name, width);
+        return set_error(LMX_EINVAL, msg);
+    }
+    hipStream_t st = (hipStream_t)stream;
+    hipStream_t own = nullptr;
+    hipError_t e;
+    if (!st) {
+        if ((e = hipStreamCreateWithFlags(&own, hipStreamNonBlocking)) != hipSuccess) return hip_error(e, "stream create");
+        st = own;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t N = (uint32_t)p->num_samples;
+    uint32_t B = p->queue_capacity ? p->queue_capacity : (1u << 21);
+    B = std::min(B, N);
+    BParams pp;
+    pp.variant = BD_PM;
+    pp.merging = 0; pp.connect = 0; pp.phase = 1; pp.c_conn = 1.0f;
+    pp.mis = LMX_MIS_POWER;
+    pp.bounded = bounded ? 1 : 0;
+    pp.maxv = (int)pcap; pp.minv = 0;
+    pp.key = make_uint2((uint32_t)p->seed, (uint32_t)(p->seed >> 32));
+    pp.total = N; pp.batch = B;
+    pp.raster_mode = LMX_RASTER_RANDOM;
+    pp.table = p->uniform_table; pp.table_width = p->table_width;
+    pp.wsub = 0;
+    pp.sample_out = p->sample_out;
+    pp.film = film_rgb;
+    int rc = bd_buffers(sc, B, 2, pcap);
+    if (rc == LMX_OK) rc = vc_buffers(sc, B, pcap, 1);
+    const size_t npix = (size_t)sc->width * (size_t)sc->height;
+    const int zgrid = std::max(1, std::min((int)((3 * npix + 255) / 256), sc->cus * 8));
+    const int cap_grid = sc->cus * 8;
+    hipGraphExec_t gb = nullptr;
+    if (rc == LMX_OK) {
+        k_bd_zero<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix);
+        if (p->sample_out) {
+            const size_t ns = (size_t)N * pcap * 8;
+            k_bd_zero<<<std::max(1, (int)std::min<size_t>((ns + 255) / 256, (size_t)cap_grid)), 256, 0, st>>>(
+                p->sample_out, ns);
+        }
+        k_bd_init<<<1, 64, 0, st>>>(sc->bd_ctl.p);
+        rc = capture(st, &gb, [&] {
+            int rc2 = vc_enqueue_photons(sc, pp, st, false);
+            if (rc2 != LMX_OK) return rc2;
+            VB ph;
+            ph.v[0] = sc->bd_vert[0].p; ph.v[1] = sc->vc_vert.p;
+            ph.nsub[0] = sc->bd_nsub[0].p; ph.nsub[1] = sc->vc_nsub.p;
+            ph.epix = sc->bd_epix.p; ph.eraster = sc->bd_eraster.p;
+            ph.V = sc->bd_V.p;
+            ph.mq = nullptr;
+            ShadowQ sq{sc->bd_sray.p, sc->bd_sc.p, sc->bd_saux.p, sc->bd_smeta.p};
+            BCtl* ctl = sc->bd_ctl.p;
+            const size_t nslot = (size_t)B * pcap;
+            const int g = std::max(1, (int)std::min<size_t>((nslot + 255) / 256, (size_t)cap_grid * 2));
+            k_lt_connect<<<g, 256, 0, st>>>(sc->rs, pp, ctl, ph, sq, bounded ? p->max_num_vertices : -1);
+            rc2 = launch_trace(sc->accel, reinterpret_cast<const WRay*>(sq.ray), 0, &ctl->n_shadow, (uint32_t)nslot,
+                               reinterpret_cast<WHit*>(sc->bd_shits.p), st, true);
+            if (rc2 != LMX_OK) return rc2;
+            k_lt_splat<<<g, 256, 0, st>>>(pp, ctl, sq, sc->bd_shits.p, (int32_t)pcap);
+            const hipError_t e2 = hipGetLastError();
+            return e2 == hipSuccess ? LMX_OK : hip_error(e2, "ltdirect launch");
+        });
+    }
+    const uint64_t nbatch = ((uint64_t)N + B - 1) / B;
+    for (uint64_t b = 0; rc == LMX_OK && b < nbatch; b++) {
+        k_lt_batch<<<1, 64, 0, st>>>(sc->bd_ctl.p, (uint32_t)(b * B));
+        if ((e = hipGraphLaunch(gb, st)) != hipSuccess) rc = hip_error(e, "graph launch");
+    }
+    if (rc == LMX_OK) {
+        k_bd_scale<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix, (float)((double)npix / (double)N));
+        if ((e = hipMemcpyAsync(sc->bd_hctl, sc->bd_ctl.p, sizeof(BCtl), hipMemcpyDeviceToHost, st)) != hipSuccess ||
+            (e = hipStreamSynchronize(st)) != hipSuccess)
+            rc = hip_error(e, "lmx_render_ltdirect: sync");
+    }
+    if (gb) (void)hipGraphExecDestroy(gb);
+    if (own) (void)hipStreamDestroy(own);
+    if (stats) {
+        stats->rays = rc == LMX_OK ? sc->bd_hctl->rays + sc->bd_hctl->shadow_rays : 0;
+        stats->iterations = (uint32_t)nbatch;
+        stats->queue_capacity = B;
+        stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2255,6 +2461,12 @@ int lmx_render_sppm(lmx_scene* sc, const lmx_render_params* p, const lmx_ppm_par
                     void* stream, lmx_render_stats* stats)
 {
     return render_sppm(sc, p, v, film_rgb, stream, stats);
+}
+
+int lmx_render_ltdirect(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, void* stream,
+                        lmx_render_stats* stats)
+{
+    return render_ltdirect(sc, p, film_rgb, stream, stats);
 }
 
 }  // extern "C"

@@ -297,6 +297,16 @@ class Scene:
         stats = {"rays": st.rays, "passes": st.iterations, "ms": st.ms}
         return film, stats, so
 
+    def render_ltdirect(self, num_samples, max_num_vertices=-1, seed=42, batch=0, table=None, sample_out=False,
+                        stream=None):
+        """renderer::ltdirect (renderer_ltdirect.cpp:63-68): light subpaths
+        connected to the pinhole at every vertex.  Returns (film (H, W, 3),
+        stats, per-sample records (n, M, 8) or None; M = max_num_vertices, 32
+        when unbounded)."""
+        m = 32 if max_num_vertices == -1 else max_num_vertices
+        return self._bidir("lmx_render_ltdirect", m, num_samples, max_num_vertices, 0, seed, RASTER_RANDOM, batch,
+                           table, sample_out, stream, MIS_POWER)
+
     def _bidir(self, entry, nrec, num_samples, max_num_vertices, min_num_vertices, seed, raster_mode, batch,
                table, sample_out, stream, mis):
         import torch
