@@ -136,7 +136,8 @@ def bdpt_section(dev, cpu, threads):
     strategy with power-heuristic MIS): samples/s and Mrays/s (subpath
     extension rays plus shadow rays), beside the oracle's per-sample
     restatement on host threads over a bounded prefix; renderer::bdpt on the
-    same frame (at most 10 vertices, and unbounded) beside it."""
+    same frame (at most 10 vertices, and unbounded) and renderer::ltdirect
+    (16 light subpaths per pixel) beside it."""
     import torch
     from lmx import render, scenes
     log("bdpt: generating the sphere-grid scene")
@@ -213,6 +214,33 @@ def bdpt_section(dev, cpu, threads):
         out["renderer_bdpt"]["cpu_baseline"] = {
             "msamples_per_s": round(done / rdt / 1e6, 5), "cores": threads, "kind": "port",
             "sample": f"first {done} samples of the same frame (about 5 s)"}
+    # renderer::ltdirect on the same frame: 16 light subpaths per pixel, at
+    # most 10 vertices, every vertex connected to the camera
+    kl = dict(max_num_vertices=10, seed=42)
+    rs.render_ltdirect(1 << 22, **kl)                                        # warm-up
+    torch.cuda.synchronize()
+    tl = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        fl, sl, _ = rs.render_ltdirect(n, **kl)
+        torch.cuda.synchronize()
+        tl.append(time.perf_counter() - t0)
+    dl = min(tl)
+    out["ltdirect"] = {"max_num_vertices": 10, "msamples_per_s": round(n / dl / 1e6, 2),
+                       "mrays_per_s": round(sl["rays"] / dl / 1e6, 1), "rays_per_sample": round(sl["rays"] / n, 4),
+                       "s_per_frame": round(dl, 4), "image_mean": round(float(fl.mean()), 5), "cpu_baseline": None}
+    if cpu:
+        log("bdpt: cpu baseline (oracle renderer::ltdirect)")
+        osc.ltdirect_samples(0, 1024, nthreads=threads, **kl)
+        done, t0 = 0, time.perf_counter()
+        while done < (1 << 21) and time.perf_counter() - t0 < 5.0:
+            osc.ltdirect_samples(done, 1 << 16, nthreads=threads, **kl)
+            done += 1 << 16
+        ldt = time.perf_counter() - t0
+        out["ltdirect"]["cpu_baseline"] = {
+            "msamples_per_s": round(done / ldt / 1e6, 5), "cores": threads, "kind": "port",
+            "sample": f"first {done} light subpaths of the same frame (about 5 s)"}
+        out["ltdirect"]["gpu_over_cpu_samples"] = round((n / dl) / (done / ldt), 1)
     return out
 
 
