@@ -387,6 +387,12 @@ int lmx_render_sppm(lmx_scene *scene, const lmx_render_params *params, const lmx
 int lmx_render_ltdirect(lmx_scene *scene, const lmx_render_params *params, float *film_rgb, void *stream,
                         lmx_render_stats *stats);
 
+/* renderer::raycast (renderer_raycast.cpp:59-111): one ray through each
+ * pixel centre; film_rgb (W x H x 3, row 0 = bottom) gets |sn . d| times the
+ * hit surface's reflectance R (1 for a surface without a BSDF), black on a
+ * miss.  Synchronous on `stream`. */
+int lmx_render_raycast(lmx_scene *scene, float *film_rgb, void *stream, lmx_render_stats *stats);
+
 /* ---------------------------------------------------------------------------
  * Learned directional sampler.  No counterpart in the reference: a third
  * renderer beside renderer::pt and renderer::vcm, for the callers of

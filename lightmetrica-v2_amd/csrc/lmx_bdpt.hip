@@ -1400,6 +1400,41 @@ __global__ void k_lt_batch(BCtl* ctl, uint32_t first)
     }
 }
 
+// renderer::raycast (renderer_raycast.cpp:59-111): a ray through every pixel
+// centre; a hit shows |sn . d| x R (the reflectance of diffuse, glossy and
+// cook-torrance surfaces, 1 for a surface without a BSDF), a miss black.
+__global__ __launch_bounds__(256) void k_rc_gen(RScene s, float4* __restrict__ ray, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t x = i % (uint32_t)s.width, y = i / (uint32_t)s.width;
+    const f3 d = camera_dir(s, ((float)x + 0.5f) / (float)s.width, ((float)y + 0.5f) / (float)s.height);
+    ray[2 * i] = make_float4(s.cam_pos.x, s.cam_pos.y, s.cam_pos.z, LMX_EPS_ISECT);
+    ray[2 * i + 1] = make_float4(d.x, d.y, d.z, LMX_FLT_MAX);
+}
+
+__global__ __launch_bounds__(256) void k_rc_shade(RScene s, const float4* __restrict__ ray,
+                                                  const float4* __restrict__ hits, float* __restrict__ film, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 h = hits[i];
+    const uint32_t id = __float_as_uint(h.w);
+    f3 c = mk3(0, 0, 0);
+    if (id != WBVH_MISS) {
+        const float4 r0 = ray[2 * i], r1 = ray[2 * i + 1];
+        const f3 d = mk3(r1.x, r1.y, r1.z);
+        Frame fr;
+        surface_at(s, mk3(r0.x, r0.y, r0.z), d, h.x, h.y, h.z, id, fr);
+        const RMat m = s.mats[s.tri_mat[id]];
+        const float k = fabsf(dot(fr.sn, neg(d)));
+        c = is_scatterer(m) ? mk3(k * m.R[0], k * m.R[1], k * m.R[2]) : mk3(k, k, k);
+    }
+    film[3 * (size_t)i] = c.x;
+    film[3 * (size_t)i + 1] = c.y;
+    film[3 * (size_t)i + 2] = c.z;
+}
+
 __global__ void k_sppm_init(float4* __restrict__ st, uint32_t n, float r0)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -2424,6 +2459,39 @@ int render_ltdirect(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, 
     return rc;
 }
 
+int render_raycast(lmx_scene* sc, float* film_rgb, void* stream, lmx_render_stats* stats)
+{
+    if (!sc || !film_rgb) return set_error(LMX_EINVAL, "lmx_render_raycast: NULL argument");
+    hipStream_t st = (hipStream_t)stream;
+    hipStream_t own = nullptr;
+    hipError_t e;
+    if (!st) {
+        if ((e = hipStreamCreateWithFlags(&own, hipStreamNonBlocking)) != hipSuccess) return hip_error(e, "stream create");
+        st = own;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t n = (uint32_t)sc->width * (uint32_t)sc->height;
+    int rc = bd_buffers(sc, (n + 1) / 2, 2, 1);           // queue 0 holds 2 (n + 1) / 2 >= n rays
+    if (rc == LMX_OK) {
+        k_rc_gen<<<(n + 255) / 256, 256, 0, st>>>(sc->rs, sc->bd_ray[0].p, n);
+        rc = launch_trace(sc->accel, reinterpret_cast<const WRay*>(sc->bd_ray[0].p), n, nullptr, n,
+                          reinterpret_cast<WHit*>(sc->bd_hits.p), st, false);
+    }
+    if (rc == LMX_OK) {
+        k_rc_shade<<<(n + 255) / 256, 256, 0, st>>>(sc->rs, sc->bd_ray[0].p, sc->bd_hits.p, film_rgb, n);
+        if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(st)) != hipSuccess)
+            rc = hip_error(e, "lmx_render_raycast");
+    }
+    if (own) (void)hipStreamDestroy(own);
+    if (stats) {
+        stats->rays = rc == LMX_OK ? n : 0;
+        stats->iterations = 1;
+        stats->queue_capacity = n;
+        stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2467,6 +2535,11 @@ int lmx_render_ltdirect(lmx_scene* sc, const lmx_render_params* p, float* film_r
                         lmx_render_stats* stats)
 {
     return render_ltdirect(sc, p, film_rgb, stream, stats);
+}
+
+int lmx_render_raycast(lmx_scene* sc, float* film_rgb, void* stream, lmx_render_stats* stats)
+{
+    return render_raycast(sc, film_rgb, stream, stats);
 }
 
 }  // extern "C"

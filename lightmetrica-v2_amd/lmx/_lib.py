@@ -137,6 +137,8 @@ SIGNATURES = {
                                        ctypes.POINTER(lmx_render_stats)]),
     "lmx_render_ltdirect": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(lmx_render_params), ctypes.c_void_p,
                                            ctypes.c_void_p, ctypes.POINTER(lmx_render_stats)]),
+    "lmx_render_raycast": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.POINTER(lmx_render_stats)]),
     "lmx_film_save": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_char_p,
                                      ctypes.c_void_p]),
     "lmx_image_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),

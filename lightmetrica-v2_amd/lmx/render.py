@@ -307,6 +307,20 @@ class Scene:
         return self._bidir("lmx_render_ltdirect", m, num_samples, max_num_vertices, 0, seed, RASTER_RANDOM, batch,
                            table, sample_out, stream, MIS_POWER)
 
+    def render_raycast(self, stream=None):
+        """renderer::raycast: (H, W, 3) film of |sn . d| x reflectance at the
+        pixel centres, and stats."""
+        import torch
+        dev = torch.device("cuda")
+        film = torch.empty((self.height, self.width, 3), dtype=torch.float32, device=dev)
+        st = lmx_render_stats()
+        if stream is None:
+            stream = torch.cuda.Stream(dev)
+            stream.wait_stream(torch.cuda.current_stream(dev))
+        check(lib.lmx_render_raycast(self._h, ctypes.c_void_p(film.data_ptr()), ctypes.c_void_p(stream.cuda_stream),
+                                     ctypes.byref(st)))
+        return film, {"rays": st.rays, "ms": st.ms}
+
     def _bidir(self, entry, nrec, num_samples, max_num_vertices, min_num_vertices, seed, raster_mode, batch,
                table, sample_out, stream, mis):
         import torch

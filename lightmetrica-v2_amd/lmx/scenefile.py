@@ -356,8 +356,8 @@ def load_scene(source, base_dir=None):
 
 def render(loaded, seed=42, raster_mode=None):
     """Renders a load_scene result with its renderer node (pt, ptdirect,
-    ptmis, bdpt, pm, ppm, sppm, ltdirect; vcm in its bdpt, vcm and bdpm
-    modes) and the reference's parameter defaults.
+    ptmis, bdpt, pm, ppm, sppm, ltdirect, raycast; vcm in its bdpt, vcm
+    and bdpm modes) and the reference's parameter defaults.
     Returns (film tensor, stats)."""
     from . import render as R
     rs = R.Scene(loaded["scene"])
@@ -415,6 +415,8 @@ def render(loaded, seed=42, raster_mode=None):
                                      num_photon_trace_samples=int(p.get("num_photon_trace_samples", 100)),
                                      initial_radius=float(p.get("initial_radius", 0.1)),
                                      alpha=float(p.get("alpha", 0.7)), seed=seed)
+    elif t == "raycast":
+        film, st = rs.render_raycast()
     elif t == "ltdirect":
         film, st, _ = rs.render_ltdirect(n, max_num_vertices=int(p.get("max_num_vertices", -1)), seed=seed)
     else:

@@ -276,6 +276,9 @@ typedef struct orc_lt_params {
 void orc_ltdirect_samples(const orc_scene *s, const orc_lt_params *p, const float *table, uint32_t table_width,
                           uint64_t first_sample, uint64_t n, float *out, int nthreads);
 
+/* renderer::raycast (orc_render.c): W H x 3 floats, pixel y W + x. */
+void orc_raycast(const orc_scene *s, float *out);
+
 /* renderer::bdpt (orc_render.c).  max_num_vertices: -1 (unbounded) or >= 2;
  * min_num_vertices applies only when bounded (renderer_bdpt.cpp:887);
  * mis: 0 power heuristic, 1 simple.  Output per sample: 2 (M+1)^2 records of

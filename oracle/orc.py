@@ -272,6 +272,8 @@ def _render_sigs(so):
     so.orc_ltdirect_samples.argtypes = [vp, ctypes.POINTER(orc_lt_params), vp, ctypes.c_uint32, ctypes.c_uint64,
                                         ctypes.c_uint64, vp, ctypes.c_int]
     so.orc_ltdirect_samples.restype = None
+    so.orc_raycast.argtypes = [vp, vp]
+    so.orc_raycast.restype = None
     so.orc_film_rgbe.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
     so.orc_film_rgbe.restype = None
     so.orc_film_png.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
@@ -525,6 +527,12 @@ class Scene:
             table = np.ascontiguousarray(table, np.float32)
             tw, tp = table.shape[1], ctypes.c_void_p(table.ctypes.data)
         lib().orc_ltdirect_samples(self._h, ctypes.byref(p), tp, tw, first, n, _p(out), nthreads)
+        return out
+
+    def raycast(self):
+        """renderer::raycast: (H, W, 3) float32, row 0 = bottom."""
+        out = np.zeros((self.height, self.width, 3), np.float32)
+        lib().orc_raycast(self._h, _p(out))
         return out
 
     def film(self, samples):

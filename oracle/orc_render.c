@@ -2691,3 +2691,28 @@ void orc_ltdirect_samples(const orc_scene *s, const orc_lt_params *pp, const flo
     for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
     free(b.tri_light);
 }
+
+/* renderer::raycast (renderer_raycast.cpp:59-111): one ray through each
+ * pixel centre; a hit shows |sn . d| times the surface's reflectance R
+ * (BSDF::Reflectance2 of diffuse and cook-torrance; 1 for a surface without
+ * a BSDF), a miss black.  out: W H x 3 floats, pixel y W + x. */
+void orc_raycast(const orc_scene *s, float *out)
+{
+    camera cam;
+    camera_init(&s->d, &cam);
+    const int w = (int)s->d.width, h = (int)s->d.height;
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            float *o = &out[3 * ((size_t)y * w + x)];
+            o[0] = o[1] = o[2] = 0.0f;
+            const v3 d = camera_dir(&cam, ((float)x + 0.5f) / (float)w, ((float)y + 0.5f) / (float)h);
+            orc_hit hh;
+            if (!scene_intersect(s, cam.pos, d, &hh)) continue;
+            frame fr;
+            surface_at(s, cam.pos, d, &hh, &fr);
+            const orc_material *m = &s->d.mats[s->d.tri_mat[hh.id]];
+            const float c = fabsf(vdot(fr.sn, vneg(d)));
+            const int refl = m->type == ORC_MAT_DIFFUSE || m->type == ORC_MAT_PHONG || m->type == ORC_MAT_GGX;
+            for (int k = 0; k < 3; k++) o[k] = c * (refl ? m->R[k] : 1.0f);
+        }
+}
