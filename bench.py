@@ -74,7 +74,9 @@ def pt_section(dev, cpu, threads):
     rs = render.Scene(sc)
     n = 1024 * 1024 * 64
     kw = dict(max_num_vertices=17, rr_depth=3, rr_prob=0.5, seed=42)
-    rs.render_pt(1024 * 1024 * 4, **kw)               # autotune + warm-up
+    # autotune + warm-up: the capacity autotuner probes on this render's
+    # first 2^24 samples, enough for its steady state at 2^23 paths in flight
+    rs.render_pt(1024 * 1024 * 16, **kw)
     torch.cuda.synchronize()
     times, rays = [], 0
     for _ in range(3):
