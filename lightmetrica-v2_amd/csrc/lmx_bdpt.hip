@@ -654,10 +654,13 @@ __global__ __launch_bounds__(256) void k_bd_plan(RScene s, BParams p, BCtl* __re
     }
 }
 
-// Connection evaluation: one thread per shadow-queue slot.  Writes the
+// Connection evaluation: one thread per shadow-queue slot.  The
+// waves-per-EU hints here and on the photon gathers below cap the register
+// budget (4 waves: 128 VGPRs, 5: 96) at the cost of a few spilled floats:
+// more waves hide more of their random gathers (measured 1-5 % faster).  Writes the
 // shadow ray with the weighted contribution, or a ray that cannot hit
 // (tmax < tmin) with s = 0 when the connection carries nothing.
-__global__ __launch_bounds__(256) void k_bd_connect(RScene s, BParams p, BCtl* __restrict__ ctl, VB vb, ShadowQ sq)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_bd_connect(RScene s, BParams p, BCtl* __restrict__ ctl, VB vb, ShadowQ sq)
 {
     const uint32_t nslot = ctl->n_shadow;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nslot; j += gridDim.x * blockDim.x) {
@@ -941,7 +944,7 @@ __global__ __launch_bounds__(256) void k_scan_add(uint32_t* a, uint32_t n, const
 // candidate tests.
 #define MERGE_LIST 16
 
-__global__ __launch_bounds__(256) void k_vc_merge(RScene s, BParams p, BCtl* __restrict__ ctl, VB vb, VB ph,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_vc_merge(RScene s, BParams p, BCtl* __restrict__ ctl, VB vb, VB ph,
                                                   Grid g)
 {
     __shared__ uint32_t hl[MERGE_LIST * 256];          // [hit][thread]
@@ -1037,7 +1040,7 @@ struct PMArgs {
     float nphoton;
 };
 
-__global__ __launch_bounds__(256) void k_pm_gather(RScene s, BParams p, BCtl* __restrict__ ctl, VB vb, VB ph,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_pm_gather(RScene s, BParams p, BCtl* __restrict__ ctl, VB vb, VB ph,
                                                    Grid g, PMArgs a)
 {
     __shared__ uint32_t hl[MERGE_LIST * 256];
@@ -1214,7 +1217,7 @@ RD void ppm_gather(const Vtx& v, f3 wi, const RMat& m, float r, float cell, cons
 // path has at most max_num_vertices vertices give M and dtau; then ratio =
 // (N + alpha M) / (N + M), tau = (tau + dtau) ratio, r = r sqrt(ratio),
 // N = N + alpha M.
-__global__ __launch_bounds__(256) void k_ppm_update(RScene s, BParams p, const BCtl* __restrict__ ctl, VB vb, VB ph,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_ppm_update(RScene s, BParams p, const BCtl* __restrict__ ctl, VB vb, VB ph,
                                                     Grid g, PMArgs a, float4* __restrict__ st, float alpha)
 {
     __shared__ uint32_t hl[MERGE_LIST * 256];
@@ -1246,7 +1249,7 @@ __global__ __launch_bounds__(256) void k_ppm_update(RScene s, BParams p, const B
 // update of k_ppm_update with tau gaining throughput x dtau.  State per
 // pixel: st[3 i] = (tau, N), st[3 i + 1] = (emission sum, radius),
 // st[3 i + 2] = (passes with a point, 0, 0, 0).
-__global__ __launch_bounds__(256) void k_sppm_update(RScene s, BParams p, const BCtl* __restrict__ ctl, VB vb,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_sppm_update(RScene s, BParams p, const BCtl* __restrict__ ctl, VB vb,
                                                      VB ph, Grid g, PMArgs a, float4* __restrict__ st, float alpha)
 {
     __shared__ uint32_t hl[MERGE_LIST * 256];
