@@ -886,23 +886,43 @@ RD uint32_t block_scan(uint32_t v, uint32_t* lds, uint32_t& tot)
 #define SCAN_PER 16u
 #define SCAN_TILE (256u * SCAN_PER)
 
-// tile-local exclusive scan of a[0, n), tile totals into part
+// tile-local exclusive scan of a[0, n), tile totals into part.  The tile is
+// staged through LDS so that global reads and writes are coalesced (lane l
+// of a pass touches word pass * 256 + l); each thread then scans its own 16
+// consecutive words there.  Word i sits at i + i / 32, so when a wave's 64
+// lanes read words 16 l + k no bank serves more than two of them.
+// (Lanes reading their 16 words straight from global memory took 4.7x the
+// time of k_scan_add over the same 2^24 counts.)
 __global__ __launch_bounds__(256) void k_scan_tiles(uint32_t* a, uint32_t n, uint32_t* part)
 {
+    __shared__ uint32_t tile[SCAN_TILE + SCAN_TILE / 32u];
     __shared__ uint32_t lds[4];
-    const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER;
+    const uint32_t t0 = blockIdx.x * SCAN_TILE;
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN_PER; k++) {
+        const uint32_t i = k * 256u + threadIdx.x;
+        tile[i + (i >> 5)] = t0 + i < n ? a[t0 + i] : 0u;
+    }
+    __syncthreads();
+    const uint32_t b = threadIdx.x * SCAN_PER;
     uint32_t v[SCAN_PER], sum = 0;
 #pragma unroll
     for (uint32_t k = 0; k < SCAN_PER; k++) {
-        v[k] = base + k < n ? a[base + k] : 0u;
+        v[k] = tile[b + k + ((b + k) >> 5)];
         sum += v[k];
     }
     uint32_t tot;
     uint32_t run = block_scan(sum, lds, tot);
 #pragma unroll
     for (uint32_t k = 0; k < SCAN_PER; k++) {
-        if (base + k < n) a[base + k] = run;
+        tile[b + k + ((b + k) >> 5)] = run;
         run += v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN_PER; k++) {
+        const uint32_t i = k * 256u + threadIdx.x;
+        if (t0 + i < n) a[t0 + i] = tile[i + (i >> 5)];
     }
     if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
