@@ -848,7 +848,10 @@ __global__ __launch_bounds__(256) void k_vc_grid_fill(RScene s, BParams p, const
     const size_t n = (size_t)np * (size_t)(p.maxv - 1);
     const float cell = ctl->cell;
     for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
-        const uint32_t i = (uint32_t)(idx % np), j = 1u + (uint32_t)(idx / np);
+        // neighbouring lanes take neighbouring vertices of one photon subpath
+        // (64 B apart, not maxv x 64 B): 8 % off both phases, which the bucket
+        // atomics bound
+        const uint32_t i = (uint32_t)(idx / (uint32_t)(p.maxv - 1)), j = 1u + (uint32_t)(idx % (uint32_t)(p.maxv - 1));
         if (j >= ph.nsub[1][i]) continue;
         const float4 a = ph.v[1][((size_t)i * (size_t)p.maxv + j) * 4];
         // renderer::pm / ppm store photons at diffuse and glossy vertices only
