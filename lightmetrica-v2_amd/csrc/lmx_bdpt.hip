@@ -1035,6 +1035,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 if (J.Cs.x == 0.0f && J.Cs.y == 0.0f && J.Cs.z == 0.0f) continue;
                 const float f = J.pLs * area / J.sum;
                 const f3 C = mk3(J.Cs.x * f, J.Cs.y * f, J.Cs.z * f);
+                // Squared pdfs can leave float range: without the connection
+                // terms (c_conn = 0, mode bdpm) J.sum may underflow to 0 or
+                // overflow with the numerator, and 0/0, inf/inf or 0 x inf
+                // would put a NaN into the film.  Such a merge is dropped
+                // (seen in an 8-pass bdpm frame of the 1024^2 sphere grid).
+                if (!isfinite(C.x) || !isfinite(C.y) || !isfinite(C.z)) continue;
                 const float pm = area * J.pLs;
                 Csum = add(Csum, C);
                 nmerge++;

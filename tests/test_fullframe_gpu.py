@@ -1,0 +1,71 @@
+"""Every renderer on the benchmark's full frame (16 x 16 displaced
+icospheres, 5,242,886 triangles, 1024 x 1024, paths of at most 10 vertices,
+the configurations the benchmark times) through size-independent
+properties:
+
+* every film is finite and non-negative;
+* the unbiased estimators (renderer::ptdirect, ptmis, vcm in bdpt mode,
+  bdpt, ltdirect) give renderer::pt's image mean within 1 %: a strategy
+  that is dropped, counted twice or mis-weighted moves the mean by its
+  share of the image;
+* the kernel estimators at radius 0.05 (renderer::vcm modes vcm and bdpm,
+  pm, ppm, sppm) give it within 2.5 %: their bias at that radius darkens
+  this frame by about 1 %.
+
+The statistical error of each mean is below 0.2 % at these sample counts.
+"""
+import numpy as np
+import pytest
+
+from lmx import render, scenes
+
+pytestmark = pytest.mark.gpu
+
+NPIX = 1024 * 1024
+M = 10
+
+
+@pytest.fixture(scope="module")
+def frame(gpu):
+    rs = render.Scene(scenes.sphere_grid())
+    film, _, _ = rs.render_pt(16 * NPIX, max_num_vertices=M, rr_depth=3, rr_prob=0.5, seed=5)
+    return rs, _checked(film)
+
+
+def _checked(film):
+    f = film.detach().cpu().numpy().astype(np.float64)
+    assert f.shape == (1024, 1024, 3) or f.size == 3 * NPIX, f.shape
+    assert np.isfinite(f).all()
+    assert (f >= 0.0).all()
+    return float(f.mean())
+
+
+RENDERS = {
+    "ptdirect": (0.01, lambda rs: rs.render_ptdirect(4 * NPIX, max_num_vertices=M, rr_depth=3, rr_prob=0.5, seed=6)),
+    "ptmis": (0.01, lambda rs: rs.render_ptmis(4 * NPIX, max_num_vertices=M, rr_depth=3, rr_prob=0.5, seed=7)),
+    "vcm_bdpt": (0.01, lambda rs: rs.render_vcm_bdpt(4 * NPIX, max_num_vertices=M, seed=8)),
+    "bdpt": (0.01, lambda rs: rs.render_bdpt(4 * NPIX, max_num_vertices=M, seed=9)),
+    "ltdirect": (0.01, lambda rs: rs.render_ltdirect(4 * NPIX, max_num_vertices=M, seed=10)),
+    "vcm": (0.025, lambda rs: rs.render_vcm(2 * NPIX, mode=render.VCM_MODE_VCM, num_iteration_pass=8,
+                                            num_photon_trace_samples=1 << 20, initial_radius=0.05, alpha=0.7,
+                                            max_num_vertices=M, seed=11)),
+    "bdpm": (0.025, lambda rs: rs.render_vcm(2 * NPIX, mode=render.VCM_MODE_BDPM, num_iteration_pass=8,
+                                             num_photon_trace_samples=1 << 20, initial_radius=0.05, alpha=0.7,
+                                             max_num_vertices=M, seed=12)),
+    "pm": (0.025, lambda rs: rs.render_pm(16 * NPIX, num_photon_trace_samples=1 << 22, finalgather=1, radius=0.05,
+                                          max_num_vertices=M, seed=13)),
+    "ppm": (0.025, lambda rs: rs.render_ppm(NPIX, M, num_iteration_pass=32, num_photon_trace_samples=1 << 20,
+                                            initial_radius=0.05, alpha=0.7, seed=14)),
+    "sppm": (0.025, lambda rs: rs.render_sppm(M, num_iteration_pass=32, num_photon_trace_samples=1 << 20,
+                                              initial_radius=0.05, alpha=0.7, seed=15)),
+}
+
+
+@pytest.mark.parametrize("name", list(RENDERS))
+def test_full_frame_image_mean_matches_pt(frame, name):
+    rs, m_pt = frame
+    tol, run = RENDERS[name]
+    film, _, _ = run(rs)
+    m = _checked(film)
+    print(f"{name}: image mean {m:.5f}, renderer::pt {m_pt:.5f} ({100 * (m / m_pt - 1):+.2f} %)")
+    assert abs(m - m_pt) <= tol * m_pt, (name, m, m_pt)
