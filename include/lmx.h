@@ -224,7 +224,7 @@ int lmx_render_ptmis(lmx_scene *scene, const lmx_render_params *params, float *f
  * (s, t) connection with s + t <= max_num_vertices, power-heuristic MIS over
  * all strategies.  Uses max_num_vertices (2..32), min_num_vertices, seed,
  * num_samples, raster_mode; queue_capacity is the batch size in samples
- * (0 = autotuned per scene and max_num_vertices: the fastest of 2^19..2^22
+ * (0 = autotuned per scene and max_num_vertices: the fastest of 2^19..2^24
  * that fits half of free device memory).  Uniform table width >= 2 (8 + 4 (max_num_vertices - 1)): an
  * eye block then a light block, each laid out as for lmx_render_pt except
  * 1-2 emitter position, 3 emitter selection, 4-5 emitter direction.
