@@ -64,6 +64,9 @@ typedef struct lmx_accel_info {
     int32_t isect;
     uint64_t node_bytes;
     uint64_t tri_bytes;
+    uint32_t stack_overflow; /* nonzero once a GPU traversal of this accel ran out of stack
+                                (never for a tree lmx_accel_build accepts); synchronises */
+    uint32_t reserved;
 } lmx_accel_info;
 
 /* Builds the compressed 8-wide BVH on the host over world-space triangles
@@ -470,6 +473,18 @@ int lmx_film_save(const float *film_rgb, int32_t width, int32_t height, int32_t 
  * W x H x 3 floats, row 0 = bottom); rgb NULL: only the size.  No reference
  * counterpart: for exchanging golden images. */
 int lmx_image_load(const char *path, float *rgb, int32_t *width, int32_t *height);
+
+/* Test hook, no reference counterpart: the FP32 functions the samplers are
+ * built with (lightmetrica-v2_amd/csrc/lmx_fmath.h), evaluated on the GPU
+ * over n device values -- out[i] = f(x[i]) or, for LMX_FMATH_POW,
+ * x[i]^y[i] -- so tests can check them bit for bit against the host build. */
+#define LMX_FMATH_SIN 0
+#define LMX_FMATH_COS 1
+#define LMX_FMATH_TAN 2
+#define LMX_FMATH_EXP 3
+#define LMX_FMATH_LOG 4
+#define LMX_FMATH_POW 5
+int lmx_fmath_eval(int32_t kind, const float *x, const float *y, uint32_t n, float *out, void *stream);
 
 /* Copies the host-side BVH out for inspection: num_nodes x 128-byte nodes and
  * num_tris x 48-byte triangle records (layouts in

@@ -63,7 +63,7 @@ struct lmx_accel {
     int device = -1;
     WNode* d_nodes = nullptr;
     WTri* d_tris = nullptr;
-    uint32_t* d_counters = nullptr;   // 8 region cursors + the ray count, CURSOR_STRIDE words apart
+    uint32_t* d_counters = nullptr;   // 8 region cursors, the ray count, the stack-overflow flag; CURSOR_STRIDE words apart
     int grid = 0;
     int stack_depth = 0;              // LDS stack entries per lane
     size_t lds_bytes = 0;
@@ -74,8 +74,10 @@ struct lmx_accel {
 
 namespace lmx {
 
+// Returns false when a node group did not fit the stack (a BVH deeper than
+// WBVH_STACK, which the builder refuses); the hit is then a miss.
 template <int ISECT>
-static void walk_impl(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* nv, uint64_t* nt)
+static bool walk_impl(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* nv, uint64_t* nt)
 {
     WRayPre r;
     wb_ray_setup(r, ray.o[0], ray.o[1], ray.o[2], ray.tmin, ray.d[0], ray.d[1], ray.d[2], ray.tmax);
@@ -98,6 +100,10 @@ static void walk_impl(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* n
         const uint32_t s = p ^ r.oct;
         const uint32_t child = g_base + (uint32_t)wb_popc(g_imask & ((1u << s) - 1u));
         if (g_hits) {
+            if (sp >= WBVH_STACK) {
+                hit.t = 0.0f; hit.b0 = 0.0f; hit.b1 = 0.0f; hit.id = WBVH_MISS;
+                return false;
+            }
             stack_base[sp] = g_base;
             stack_bits[sp] = g_hits | (g_imask << 8);
             sp++;
@@ -131,12 +137,13 @@ static void walk_impl(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* n
     }
     if (nv) *nv += visits;
     if (nt) *nt += tests;
+    return true;
 }
 
-void wbvh_walk(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* nv, uint64_t* nt)
+bool wbvh_walk(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* nv, uint64_t* nt)
 {
-    if (b.isect == WBVH_ISECT_WATERTIGHT) walk_impl<WBVH_ISECT_WATERTIGHT>(b, ray, hit, nv, nt);
-    else walk_impl<WBVH_ISECT_TRIACCEL>(b, ray, hit, nv, nt);
+    if (b.isect == WBVH_ISECT_WATERTIGHT) return walk_impl<WBVH_ISECT_WATERTIGHT>(b, ray, hit, nv, nt);
+    return walk_impl<WBVH_ISECT_TRIACCEL>(b, ray, hit, nv, nt);
 }
 
 }  // namespace lmx
@@ -145,6 +152,8 @@ void wbvh_walk(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* nv, uint
 // device kernels
 
 #define TRACE_WAVE 64
+// lmx_accel_intersect_host walks batches below this size on the calling thread
+#define LMX_HOST_INLINE_RAYS 1024u
 #ifndef LMX_REFILL_MIN
 #define LMX_REFILL_MIN 16      // refill when at least this many lanes of a wave are idle
 #endif
@@ -164,6 +173,9 @@ void wbvh_walk(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* nv, uint
 #define LMX_CURSOR_STRIDE 64
 #endif
 #define CURSOR_STRIDE LMX_CURSOR_STRIDE
+// counter block of one trace launch: 8 region cursors, the ray count, and a
+// sticky stack-overflow flag, each on a line of its own
+#define LMX_TRACE_COUNTER_WORDS (10 * CURSOR_STRIDE)
 
 // Resets the 8 region cursors.  The ray count comes from n_dev when given
 // (a queue length written by an earlier kernel), else from n; the count the
@@ -323,9 +335,15 @@ void k_trace(const WNode* __restrict__ nodes,
                 g_hits &= g_hits - 1u;
                 const uint32_t s = p ^ r.oct;
                 child = g_base + (uint32_t)__popc(g_imask & ((1u << s) - 1u));
-                if (g_hits && sp < stack_depth) {
-                    stack[sp * TRACE_WAVE + lane] = make_uint2(g_base, g_hits | (g_imask << 8));
-                    sp++;
+                if (g_hits) {
+                    if (sp < stack_depth) {
+                        stack[sp * TRACE_WAVE + lane] = make_uint2(g_base, g_hits | (g_imask << 8));
+                        sp++;
+                    } else {
+                        // unreachable for a tree the builder accepts (the
+                        // stack holds max_depth groups); flagged, never silent
+                        atomicOr(&cursors[9 * CURSOR_STRIDE], 1u);
+                    }
                 }
             }
         }
@@ -424,6 +442,17 @@ int lmx_accel_info_get(const lmx_accel* a, lmx_accel_info* info)
     info->isect = a->host.isect;
     info->node_bytes = (uint64_t)a->host.nodes.size() * sizeof(WNode);
     info->tri_bytes = (uint64_t)a->host.tris.size() * sizeof(WTri);
+    info->stack_overflow = 0;
+    info->reserved = 0;
+    if (a->device >= 0 && a->d_counters) {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        (void)hipSetDevice(a->device);
+        const hipError_t e = hipMemcpy(&info->stack_overflow, a->d_counters + 9 * CURSOR_STRIDE, sizeof(uint32_t),
+                                       hipMemcpyDeviceToHost);
+        (void)hipSetDevice(cur);
+        if (e != hipSuccess) return hip_error(e, "lmx_accel_info_get: overflow flag");
+    }
     return LMX_OK;
 }
 
@@ -449,12 +478,13 @@ int lmx_accel_upload(lmx_accel* a)
         (void)hipFree(a->d_nodes); a->d_nodes = nullptr;
         return hip_error(e, "lmx_accel_upload: hipMalloc triangles");
     }
-    if ((e = hipMalloc(&a->d_counters, (9 * CURSOR_STRIDE) * sizeof(uint32_t))) != hipSuccess) {
+    if ((e = hipMalloc(&a->d_counters, LMX_TRACE_COUNTER_WORDS * sizeof(uint32_t))) != hipSuccess) {
         (void)hipFree(a->d_nodes); (void)hipFree(a->d_tris); a->d_nodes = nullptr; a->d_tris = nullptr;
         return hip_error(e, "lmx_accel_upload: hipMalloc counters");
     }
     if ((e = hipMemcpy(a->d_nodes, a->host.nodes.data(), nb, hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemcpy(a->d_tris, a->host.tris.data(), tb, hipMemcpyHostToDevice)) != hipSuccess) {
+        (e = hipMemcpy(a->d_tris, a->host.tris.data(), tb, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemset(a->d_counters, 0, LMX_TRACE_COUNTER_WORDS * sizeof(uint32_t))) != hipSuccess) {
         (void)hipFree(a->d_nodes); (void)hipFree(a->d_tris); (void)hipFree(a->d_counters);
         a->d_nodes = nullptr; a->d_tris = nullptr; a->d_counters = nullptr;
         return hip_error(e, "lmx_accel_upload: hipMemcpy");
@@ -481,29 +511,38 @@ int lmx_accel_upload(lmx_accel* a)
 
 namespace lmx {
 
+// Enqueues one traversal launch of an accel already resident on the current
+// device, with its counter block at `counters` (LMX_TRACE_COUNTER_WORDS
+// words of device memory).
+static int enqueue_trace(const lmx_accel* a, uint32_t* counters, const WRay* rays, uint32_t n,
+                         const uint32_t* n_dev, uint32_t n_max, WHit* hits, hipStream_t s, bool anyhit)
+{
+    const int grid = std::max(1, std::min(a->grid, (int)((n_max + TRACE_WAVE - 1) / TRACE_WAVE)));
+    k_init_cursors<<<1, 64, 0, s>>>(counters, n, n_dev);
+    const bool wt = a->host.isect == WBVH_ISECT_WATERTIGHT;
+    if (wt && !anyhit)
+        k_trace<WBVH_ISECT_WATERTIGHT, false><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
+            a->d_nodes, a->d_tris, rays, hits, counters, a->stack_depth, REFILL_MIN);
+    else if (wt)
+        k_trace<WBVH_ISECT_WATERTIGHT, true><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
+            a->d_nodes, a->d_tris, rays, hits, counters, a->stack_depth, REFILL_MIN);
+    else if (!anyhit)
+        k_trace<WBVH_ISECT_TRIACCEL, false><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
+            a->d_nodes, a->d_tris, rays, hits, counters, a->stack_depth, REFILL_MIN);
+    else
+        k_trace<WBVH_ISECT_TRIACCEL, true><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
+            a->d_nodes, a->d_tris, rays, hits, counters, a->stack_depth, REFILL_MIN);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(e, "trace launch");
+    return LMX_OK;
+}
+
 int launch_trace(lmx_accel* a, const WRay* rays, uint32_t n, const uint32_t* n_dev, uint32_t n_max,
                  WHit* hits, hipStream_t s, bool anyhit)
 {
     int rc = lmx_accel_upload(a);
     if (rc != LMX_OK) return rc;
-    const int grid = std::max(1, std::min(a->grid, (int)((n_max + TRACE_WAVE - 1) / TRACE_WAVE)));
-    k_init_cursors<<<1, 64, 0, s>>>(a->d_counters, n, n_dev);
-    const bool wt = a->host.isect == WBVH_ISECT_WATERTIGHT;
-    if (wt && !anyhit)
-        k_trace<WBVH_ISECT_WATERTIGHT, false><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
-            a->d_nodes, a->d_tris, rays, hits, a->d_counters, a->stack_depth, REFILL_MIN);
-    else if (wt)
-        k_trace<WBVH_ISECT_WATERTIGHT, true><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
-            a->d_nodes, a->d_tris, rays, hits, a->d_counters, a->stack_depth, REFILL_MIN);
-    else if (!anyhit)
-        k_trace<WBVH_ISECT_TRIACCEL, false><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
-            a->d_nodes, a->d_tris, rays, hits, a->d_counters, a->stack_depth, REFILL_MIN);
-    else
-        k_trace<WBVH_ISECT_TRIACCEL, true><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
-            a->d_nodes, a->d_tris, rays, hits, a->d_counters, a->stack_depth, REFILL_MIN);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_error(e, "trace launch");
-    return LMX_OK;
+    return enqueue_trace(a, a->d_counters, rays, n, n_dev, n_max, hits, s, anyhit);
 }
 
 }  // namespace lmx
@@ -536,17 +575,32 @@ int lmx_accel_intersect_host(const lmx_accel* a, const lmx_ray* rays, uint32_t n
     if (!a) return set_error(LMX_EINVAL, "lmx_accel_intersect_host: NULL accel");
     if (n == 0) return LMX_OK;
     if (!rays || !hits) return set_error(LMX_EINVAL, "lmx_accel_intersect_host: NULL array");
+    const WRay* r = reinterpret_cast<const WRay*>(rays);
+    WHit* h = reinterpret_cast<WHit*>(hits);
+    // The per-ray path (Accel3::Intersect from every scheduler thread at
+    // once): walked on the calling thread, no allocation, no lock; reads of
+    // the accel only, so any number of threads may call it concurrently.
+    if (nthreads == 1 || n < LMX_HOST_INLINE_RAYS) {
+        bool ok = true;
+        for (uint32_t i = 0; i < n; i++) ok &= wbvh_walk(a->host, r[i], h[i], nullptr, nullptr);
+        return ok ? LMX_OK : set_error(LMX_EINVAL, "lmx_accel_intersect_host: traversal stack overflow");
+    }
     if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
-    auto work = [&](uint32_t b, uint32_t e) {
+    nthreads = (int)std::min<uint32_t>((uint32_t)nthreads, (n + LMX_HOST_INLINE_RAYS - 1) / LMX_HOST_INLINE_RAYS);
+    std::vector<uint8_t> ok(nthreads, 1);
+    auto work = [&](int t, uint32_t b, uint32_t e) {
         for (uint32_t i = b; i < e; i++)
-            wbvh_walk(a->host, reinterpret_cast<const WRay*>(rays)[i], reinterpret_cast<WHit*>(hits)[i], nullptr, nullptr);
+            if (!wbvh_walk(a->host, r[i], h[i], nullptr, nullptr)) ok[t] = 0;
     };
     std::vector<std::thread> th;
-    for (int t = 0; t < nthreads; t++) {
+    for (int t = 1; t < nthreads; t++) {
         const uint32_t b = (uint32_t)((uint64_t)n * t / nthreads), e = (uint32_t)((uint64_t)n * (t + 1) / nthreads);
-        th.emplace_back(work, b, e);
+        th.emplace_back(work, t, b, e);
     }
+    work(0, 0, (uint32_t)((uint64_t)n / nthreads));
     for (auto& x : th) x.join();
+    for (int t = 0; t < nthreads; t++)
+        if (!ok[t]) return set_error(LMX_EINVAL, "lmx_accel_intersect_host: traversal stack overflow");
     return LMX_OK;
 }
 

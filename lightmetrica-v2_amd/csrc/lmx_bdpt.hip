@@ -1531,9 +1531,8 @@ size_t bd_bytes_per_sample(uint32_t maxv, uint32_t conn)
 // Buffers for batches of B samples of up to maxv vertices and conn
 // connections each; kept when the current ones are large enough, else
 // replaced by buffers of exactly this shape (the old ones are freed first).
-int bd_buffers(lmx_scene* sc, uint32_t B, uint32_t maxv, uint32_t conn)
+void bd_release(lmx_scene* sc)
 {
-    if (B <= sc->bd_batch && maxv <= sc->bd_maxv && conn <= sc->bd_conn) return LMX_OK;
     sc->bd_batch = sc->bd_maxv = sc->bd_conn = 0;
     for (int k = 0; k < 2; k++) {
         sc->bd_vert[k].release(); sc->bd_nsub[k].release(); sc->bd_ray[k].release();
@@ -1542,6 +1541,12 @@ int bd_buffers(lmx_scene* sc, uint32_t B, uint32_t maxv, uint32_t conn)
     sc->bd_V.release(); sc->bd_epix.release(); sc->bd_eraster.release(); sc->bd_hits.release();
     sc->bd_sray.release(); sc->bd_sc.release(); sc->bd_saux.release(); sc->bd_shits.release();
     sc->bd_smeta.release();
+}
+
+int bd_buffers(lmx_scene* sc, uint32_t B, uint32_t maxv, uint32_t conn)
+{
+    if (B <= sc->bd_batch && maxv <= sc->bd_maxv && conn <= sc->bd_conn) return LMX_OK;
+    bd_release(sc);
     hipError_t e;
     const size_t nv = (size_t)B * maxv * 4;
     const size_t nconn = (size_t)B * conn;
@@ -1638,10 +1643,13 @@ uint32_t bd_key(const BParams& bp)
 // (after one untimed batch that warms the graph and the caches) and keeps the
 // size with the highest samples/s for this max_num_vertices.  Candidates are
 // bounded by half of the free device memory (what this scene's batch buffers
-// hold counts as free).  A half rather than a quarter lets unbounded
-// renderer::bdpt (105 KB per sample) reach 2^20 samples per batch: +49 %.
+// hold counts as free) and by the render's own sample count rounded up to a
+// power of two.  A half rather than a quarter lets unbounded renderer::bdpt
+// (105 KB per sample) reach 2^20 samples per batch: +49 %.  A candidate whose
+// buffers or graph cannot be made ends the probing and the best size so far
+// is kept.  Afterwards the scene holds buffers for the chosen size only.
 // The probe batches splat into `film`, which the caller clears afterwards.
-int bd_autotune(lmx_scene* sc, BParams bp, float* film, hipStream_t st)
+int bd_autotune(lmx_scene* sc, BParams bp, uint64_t num_samples, float* film, hipStream_t st)
 {
     // 2^23 and 2^24 (110 GB at max_num_vertices 10) lift renderer::bdpt on a
     // 1024^2 x 16 spp frame from 342 to 384 and 403 Msamples/s
@@ -1661,13 +1669,20 @@ int bd_autotune(lmx_scene* sc, BParams bp, float* film, hipStream_t st)
     uint32_t best = cands[0];
     double best_rate = -1.0;
     int rc = LMX_OK;
+    uint64_t want = 1;
+    while (want < num_samples) want <<= 1;
     for (uint32_t B : cands) {
-        if (B != cands[0] && (size_t)B * per > budget) break;
+        if (B != cands[0] && ((size_t)B * per > budget || B > want)) break;
         bp.batch = B;
         bp.total = 2u * B;
-        if ((rc = bd_buffers(sc, B, (uint32_t)bp.maxv, conn)) != LMX_OK) break;
         hipGraphExec_t exec = nullptr;
-        if ((rc = bd_graph(sc, bp, st, &exec)) != LMX_OK) break;
+        if ((rc = bd_buffers(sc, B, (uint32_t)bp.maxv, conn)) != LMX_OK ||
+            (rc = bd_graph(sc, bp, st, &exec)) != LMX_OK) {
+            if (best_rate < 0.0) break;        // not even the smallest size fits: report it
+            (void)hipGetLastError();           // a larger candidate failed: keep the best so far
+            rc = LMX_OK;
+            break;
+        }
         k_bd_init<<<1, 64, 0, st>>>(sc->bd_ctl.p);
         (void)hipGraphLaunch(exec, st);
         (void)hipEventRecord(e0, st);
@@ -1686,6 +1701,8 @@ int bd_autotune(lmx_scene* sc, BParams bp, float* film, hipStream_t st)
     if (rc == LMX_OK) {
         sc->bd_tuned_batch = best;
         sc->bd_tuned_key = bd_key(bp);
+        sc->bd_tuned_limit = want;
+        if (sc->bd_batch > best) bd_release(sc);   // the render allocates the chosen size
     }
     return rc;
 }
@@ -1748,7 +1765,10 @@ int render_bidir(lmx_scene* sc, const lmx_render_params* p, int variant, const c
     if (!B) {
         if (p->num_samples <= (1u << 19)) B = (uint32_t)p->num_samples;
         else {
-            if (sc->bd_tuned_key != bd_key(bp)) rc = bd_autotune(sc, bp, film_rgb, st);
+            // tuned again when an earlier, smaller render capped the candidates
+            if (sc->bd_tuned_key != bd_key(bp) ||
+                (sc->bd_tuned_limit < (1ull << 24) && sc->bd_tuned_limit < p->num_samples))
+                rc = bd_autotune(sc, bp, p->num_samples, film_rgb, st);
             B = sc->bd_tuned_batch;
         }
     }

@@ -59,9 +59,13 @@ __global__ void k_enc_rgbe(const float* __restrict__ film, int w, int h, uint8_t
     if (c2 > v) v = c2;
     uchar4 o = make_uchar4(0, 0, 0, 0);
     if (v >= 1e-32f) {
+        // Ward's scale frexp(v) 256 / v is exactly 2^(8 - e) for v = m 2^e,
+        // a normal float for every v in [1e-32, FLT_MAX]: computed in FP32
+        // as that power of two, it equals the double-precision formula bit
+        // for bit
         int e;
-        const double m = frexp((double)v, &e);
-        const float sc = (float)(m * 256.0 / (double)v);
+        (void)frexpf(v, &e);
+        const float sc = ldexpf(1.0f, 8 - e);
         o.x = (uint8_t)fminf(c0 * sc, 255.0f);
         o.y = (uint8_t)fminf(c1 * sc, 255.0f);
         o.z = (uint8_t)fminf(c2 * sc, 255.0f);
@@ -70,22 +74,60 @@ __global__ void k_enc_rgbe(const float* __restrict__ film, int w, int h, uint8_t
     reinterpret_cast<uchar4*>(out)[i] = o;
 }
 
-__device__ __forceinline__ uint8_t tonemap(float v)
+// The PNG tone map byte(v) = clamp((int)(pow((double)v, 1/2.2) 255), 0, 255)
+// (film_hdr.cpp:135-137) is a non-decreasing step function of v >= 0; the
+// host finds its 255 steps (t[b] = the smallest float whose byte is >= b) in
+// double precision once, and the kernel, in FP32 only, counts the steps at
+// or below v by binary search.  NaN and negative v (pow gives NaN, taken as
+// 0) map to 0, +inf to 255.
+struct ToneSteps { float t[256]; };
+
+__device__ __forceinline__ uint8_t tonemap(float v, const ToneSteps& ts)
 {
-    const double t = pow((double)v, 1.0 / 2.2) * 255.0;
-    // (int) of NaN (negative v) is 0, beyond the int range it saturates
-    const int k = t != t ? 0 : (t >= 2147483647.0 ? 2147483647 : (int)t);
-    return (uint8_t)(k < 0 ? 0 : (k > 255 ? 255 : k));
+    if (!(v >= 0.0f)) return 0;
+    uint32_t lo = 0, hi = 255;             // largest b in [0, 255] with t[b] <= v (t[0] = 0)
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (ts.t[mid] <= v) lo = mid; else hi = mid - 1;
+    }
+    return (uint8_t)lo;
 }
 
-__global__ void k_enc_png(const float* __restrict__ film, int w, int h, uint8_t* __restrict__ out)
+__global__ void k_enc_png(const float* __restrict__ film, int w, int h, uint8_t* __restrict__ out, ToneSteps ts)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)w * (size_t)h) return;
     const float* p = film_px(film, w, h, i, true);
-    out[3 * i] = tonemap(p[0]);
-    out[3 * i + 1] = tonemap(p[1]);
-    out[3 * i + 2] = tonemap(p[2]);
+    out[3 * i] = tonemap(p[0], ts);
+    out[3 * i + 1] = tonemap(p[1], ts);
+    out[3 * i + 2] = tonemap(p[2], ts);
+}
+
+// host: the byte of the reference's formula, and its steps
+int tone_byte_host(float v)
+{
+    const double t = std::pow((double)v, 1.0 / 2.2) * 255.0;
+    const int k = t != t ? 0 : (t >= 2147483647.0 ? 2147483647 : (int)t);
+    return k < 0 ? 0 : (k > 255 ? 255 : k);
+}
+
+ToneSteps tone_steps()
+{
+    ToneSteps ts;
+    ts.t[0] = 0.0f;
+    for (int b = 1; b < 256; b++) {
+        // non-negative floats order as their bit patterns: bisect over
+        // [0, +inf] for the first pattern whose byte reaches b
+        uint32_t lo = 0, hi = 0x7f800000u;
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            float v;
+            std::memcpy(&v, &mid, 4);
+            if (tone_byte_host(v) >= b) hi = mid; else lo = mid + 1;
+        }
+        std::memcpy(&ts.t[b], &lo, 4);
+    }
+    return ts;
 }
 
 // openexr: per file row, the B, G, R planes (clamped at zero); pfm: rows
@@ -458,7 +500,10 @@ int lmx_film_save(const float* film_rgb, int32_t width, int32_t height, int32_t 
     if (e != hipSuccess) return hip_error(e, "lmx_film_save: device buffer");
     const int grid = (int)((n + 255) / 256);
     if (type == LMX_IMAGE_RADIANCEHDR) k_enc_rgbe<<<grid, 256, 0, st>>>(film_rgb, width, height, (uint8_t*)dev);
-    else if (type == LMX_IMAGE_PNG) k_enc_png<<<grid, 256, 0, st>>>(film_rgb, width, height, (uint8_t*)dev);
+    else if (type == LMX_IMAGE_PNG) {
+        static const ToneSteps ts = tone_steps();
+        k_enc_png<<<grid, 256, 0, st>>>(film_rgb, width, height, (uint8_t*)dev, ts);
+    }
     else k_enc_f32<<<grid, 256, 0, st>>>(film_rgb, width, height, type == LMX_IMAGE_OPENEXR, (float*)dev);
     std::vector<uint8_t> host(n * per);
     if ((e = hipGetLastError()) == hipSuccess &&

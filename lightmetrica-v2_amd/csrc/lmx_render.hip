@@ -343,6 +343,7 @@ namespace {
 int ensure_queues(lmx_scene* sc, uint32_t cap)
 {
     if (sc->cap == cap) return LMX_OK;
+    sc->cap = 0;                       // set again only once every buffer is in place
     hipError_t e;
     for (int q = 0; q < 2; q++) {
         if ((e = sc->ray[q].alloc(2 * (size_t)cap)) != hipSuccess) return hip_error(e, "queue alloc");
@@ -357,6 +358,7 @@ int ensure_queues(lmx_scene* sc, uint32_t cap)
 int ensure_nee(lmx_scene* sc, uint32_t cap)
 {
     if (sc->nee_cap >= cap) return LMX_OK;
+    sc->nee_cap = 0;
     hipError_t e;
     const size_t n = 2 * (size_t)cap;
     if ((e = sc->nee_ray.alloc(2 * n)) != hipSuccess || (e = sc->nee_c.alloc(n)) != hipSuccess ||
@@ -653,7 +655,7 @@ int lmx_scene_create(const lmx_scene_desc* d, lmx_scene** out)
     rs.cam_vy = mk3(c.vy[0], c.vy[1], c.vy[2]);
     rs.cam_vz = mk3(c.vz[0], c.vz[1], c.vz[2]);
     const float fov = c.fov_deg * LMX_PI_F / 180.0f;          // Math::Radians (math.h:1674)
-    rs.tan_fov = tanf(fov * 0.5f);
+    rs.tan_fov = lmx_tanf(fov * 0.5f);
     rs.aspect = (float)c.width / (float)c.height;
     rs.width = c.width; rs.height = c.height;
     sc->width = c.width; sc->height = c.height;
@@ -687,3 +689,35 @@ int lmx_render_ptmis(lmx_scene* sc, const lmx_render_params* p, float* film_rgb,
 }
 
 }  // extern "C"
+
+namespace {
+
+__global__ void k_fmath(int kind, const float* __restrict__ x, const float* __restrict__ y, uint32_t n,
+                        float* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float a = x[i];
+    float r;
+    switch (kind) {
+    case LMX_FMATH_SIN: r = lmx_sinf(a); break;
+    case LMX_FMATH_COS: r = lmx_cosf(a); break;
+    case LMX_FMATH_TAN: r = lmx_tanf(a); break;
+    case LMX_FMATH_EXP: r = lmx_expf(a); break;
+    case LMX_FMATH_LOG: r = a > 0.0f ? lmx_logf_pos(a) : lmx_u2f(0x7fc00000u); break;
+    default: r = lmx_powf(a, y[i]); break;
+    }
+    out[i] = r;
+}
+
+}  // namespace
+
+extern "C" int lmx_fmath_eval(int32_t kind, const float* x, const float* y, uint32_t n, float* out, void* stream)
+{
+    if (kind < LMX_FMATH_SIN || kind > LMX_FMATH_POW) return set_error(LMX_EINVAL, "lmx_fmath_eval: unknown kind");
+    if (n == 0) return LMX_OK;
+    if (!x || !out || (kind == LMX_FMATH_POW && !y)) return set_error(LMX_EINVAL, "lmx_fmath_eval: NULL array");
+    k_fmath<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(kind, x, y, n, out);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? LMX_OK : hip_error(e, "lmx_fmath_eval");
+}

@@ -8,6 +8,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "lmx_fmath.h"
+
 #define RD __device__ __forceinline__
 
 namespace lmx {
@@ -64,8 +66,10 @@ RD void concentric_disk(float u0, float u1, float& ox, float& oy)
         if (vx < vy) { r = -vx; theta = pi4 * (4.0f + vy / vx); }
         else { r = -vy; theta = pi4 * (6.0f - vx / vy); }
     }
-    ox = r * cosf(theta);
-    oy = r * sinf(theta);
+    float st, ct;
+    lmx_sincosf(theta, &st, &ct);
+    ox = r * ct;
+    oy = r * st;
 }
 
 RD f3 cosine_hemisphere(float u0, float u1)

@@ -210,7 +210,9 @@ RD f3 ggx_sample_H(float r, float u0, float u1)
     const float cosT = sqrtf(1.0f - a) / den;
     const float sinT = r * (sqrtf(a) / den);
     const float phi = LMX_PI_F * (2.0f * b - 1.0f);
-    return mk3(sinT * cosf(phi), sinT * sinf(phi), cosT);
+    float sp, cp;
+    lmx_sincosf(phi, &sp, &cp);
+    return mk3(sinT * cp, sinT * sp, cosT);
 }
 RD float ggx_G(f3 li, f3 lo, f3 H)
 {
@@ -251,10 +253,12 @@ RD bool bsdf_sample(const RMat& m, const Frame& f, f3 wi, float u0, float u1, f3
         const f3 r = reflect_sn(f, wi);
         f3 ru, rv;
         onb(r, ru, rv);
-        const float ca = powf(u0, 1.0f / (m.exponent + 1.0f));
+        const float ca = lmx_powf(u0, 1.0f / (m.exponent + 1.0f));
         const float sa = sqrtf(fmaxf(0.0f, 1.0f - ca * ca));
         const float phi = 2.0f * LMX_PI_F * u1;
-        const f3 l = mk3(sa * cosf(phi), sa * sinf(phi), ca);
+        float sp, cp;
+        lmx_sincosf(phi, &sp, &cp);
+        const f3 l = mk3(sa * cp, sa * sp, ca);
         wo = add(add(mul(ru, l.x), mul(rv, l.y)), mul(r, l.z));
         return true;
     }
@@ -276,7 +280,7 @@ RD float bsdf_pdf(const RMat& m, const Frame& f, f3 wi, f3 wo)
     if (m.type == LMX_MAT_DIFFUSE) return LMX_INV_PI_F;
     if (m.type == LMX_MAT_PHONG) {
         const float ca = fmaxf(0.0f, dot(reflect_sn(f, wi), wo));
-        const float pdf_sa = (m.exponent + 1.0f) * (0.5f * LMX_INV_PI_F) * powf(ca, m.exponent);
+        const float pdf_sa = (m.exponent + 1.0f) * (0.5f * LMX_INV_PI_F) * lmx_powf(ca, m.exponent);
         return pdf_sa / lo.z;
     }
     if (m.type == LMX_MAT_GGX) {
@@ -305,7 +309,7 @@ RD f3 bsdf_eval(const RMat& m, const Frame& f, f3 wi, f3 wo)
         return mk3(m.R[0] * LMX_INV_PI_F * corr, m.R[1] * LMX_INV_PI_F * corr, m.R[2] * LMX_INV_PI_F * corr);
     if (m.type == LMX_MAT_PHONG) {
         const float ca = fmaxf(0.0f, dot(reflect_sn(f, wi), wo));
-        const float k = (m.exponent + 2.0f) * (0.5f * LMX_INV_PI_F) * powf(ca, m.exponent);
+        const float k = (m.exponent + 2.0f) * (0.5f * LMX_INV_PI_F) * lmx_powf(ca, m.exponent);
         return mk3(m.R[0] * k * corr, m.R[1] * k * corr, m.R[2] * k * corr);
     }
     if (m.type == LMX_MAT_GGX) {
@@ -332,7 +336,7 @@ RD f3 bsdf_eval_t(const RMat& m, const Frame& f, f3 wi, f3 wo, bool le)
         return mk3(m.R[0] * LMX_INV_PI_F * corr, m.R[1] * LMX_INV_PI_F * corr, m.R[2] * LMX_INV_PI_F * corr);
     if (m.type == LMX_MAT_PHONG) {
         const float ca = fmaxf(0.0f, dot(reflect_sn(f, wi), wo));
-        const float k = (m.exponent + 2.0f) * (0.5f * LMX_INV_PI_F) * powf(ca, m.exponent);
+        const float k = (m.exponent + 2.0f) * (0.5f * LMX_INV_PI_F) * lmx_powf(ca, m.exponent);
         return mk3(m.R[0] * k * corr, m.R[1] * k * corr, m.R[2] * k * corr);
     }
     if (m.type == LMX_MAT_GGX) {
@@ -454,11 +458,18 @@ struct DevBuf {
         p = nullptr;
         n = 0;
     }
+    // n records the size only once the allocation succeeded, so a reuse
+    // check on n never hands a null buffer to a kernel after a failure
     hipError_t alloc(size_t count)
     {
         if (p) { (void)hipFree(p); p = nullptr; }
+        n = 0;
+        T* q = nullptr;
+        const hipError_t e = hipMalloc(&q, std::max<size_t>(1, count) * sizeof(T));
+        if (e != hipSuccess) return e;
+        p = q;
         n = count;
-        return hipMalloc(&p, std::max<size_t>(1, count) * sizeof(T));
+        return e;
     }
 };
 
@@ -516,6 +527,7 @@ struct lmx_scene {
     // bidirectional sampler buffers (sized for bd_batch samples, bd_maxv vertices)
     uint32_t bd_batch = 0, bd_maxv = 0, bd_conn = 0;   // bd_conn: shadow-queue slots per sample
     uint32_t bd_tuned_batch = 0, bd_tuned_key = 0;      // autotuned batch for one (variant, max_num_vertices)
+    uint64_t bd_tuned_limit = 0;        // largest candidate the tuning run allowed (render size rounded up)
     DevBuf<float4> bd_vert[2];          // per subpath vertex: 4 float4 (see lmx_bdpt.hip)
     DevBuf<float> bd_V;                 // per light subpath vertex (bdpt variant)
     DevBuf<uint32_t> bd_nsub[2];

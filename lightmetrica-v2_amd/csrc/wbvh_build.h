@@ -24,8 +24,9 @@ WTri make_tri(const float* A, const float* B, const float* C, uint32_t id, int i
 
 // Closest hit of one ray by walking the wide BVH on the host in exactly the
 // order the traversal kernel uses.  Optionally counts node visits and
-// triangle tests.
-void wbvh_walk(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* nodes_visited,
+// triangle tests.  Returns false (and a miss) if a node group did not fit
+// the WBVH_STACK-entry stack, which a tree the builder accepts never needs.
+bool wbvh_walk(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* nodes_visited,
                uint64_t* tris_tested);
 
 }  // namespace lmx

@@ -32,7 +32,8 @@ class lmx_build_opts(ctypes.Structure):
 class lmx_accel_info(ctypes.Structure):
     _fields_ = [("num_nodes", ctypes.c_uint32), ("num_tris", ctypes.c_uint32),
                 ("max_depth", ctypes.c_uint32), ("isect", ctypes.c_int32),
-                ("node_bytes", ctypes.c_uint64), ("tri_bytes", ctypes.c_uint64)]
+                ("node_bytes", ctypes.c_uint64), ("tri_bytes", ctypes.c_uint64),
+                ("stack_overflow", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class lmx_material(ctypes.Structure):
@@ -162,6 +163,8 @@ SIGNATURES = {
     "lmx_guide_mix_sample": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
                                             ctypes.c_void_p]),
     "lmx_accel_export": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "lmx_fmath_eval": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                      ctypes.c_void_p, ctypes.c_void_p]),
     "lmx_accel_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                        ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                        ctypes.c_int32]),

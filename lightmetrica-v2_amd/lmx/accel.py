@@ -60,7 +60,8 @@ class Accel:
         i = lmx_accel_info()
         check(lib.lmx_accel_info_get(self._h, ctypes.byref(i)))
         return {"num_nodes": i.num_nodes, "num_tris": i.num_tris, "max_depth": i.max_depth,
-                "isect": i.isect, "node_bytes": i.node_bytes, "tri_bytes": i.tri_bytes}
+                "isect": i.isect, "node_bytes": i.node_bytes, "tri_bytes": i.tri_bytes,
+                "stack_overflow": i.stack_overflow}
 
     def export(self):
         """(nodes, tris) as structured numpy arrays (layouts of csrc/wbvh.h)."""

@@ -24,6 +24,10 @@
  *   orc_qbvh_*            hit index equal to brute force on meshes of up to
  *                         1e5 triangles (tests/test_oracle.py).
  *   orc_sample_*          bit-exact against the reference's sampler.h via _ref.
+ *   orc_fmath_eval        the shared FP32 math of lmx_fmath.h, which the
+ *                         render oracle uses: against double precision
+ *                         (ulp and relative bounds, special values) in
+ *                         tests/test_fmath.py.
  */
 #ifndef LMX_ORC_H
 #define LMX_ORC_H
@@ -111,6 +115,15 @@ void orc_sample_concentric_disk(const float u[2], float out[2]);
 void orc_sample_cosine_hemisphere(const float u[2], float out[3]);
 void orc_sample_uniform_sphere(const float u[2], float out[3]);
 void orc_sample_uniform_triangle(const float u[2], float out[2]);
+
+/* Host build of lmx_fmath.h (orc_fmath.c): kind ORC_FM_*, y used by POW */
+#define ORC_FM_SIN 0
+#define ORC_FM_COS 1
+#define ORC_FM_TAN 2
+#define ORC_FM_EXP 3
+#define ORC_FM_LOG 4
+#define ORC_FM_POW 5
+void orc_fmath_eval(int kind, const float *x, const float *y, size_t n, float *out);
 
 /* ---------------------------------------------------------------------- */
 /* Path samplers (orc_render.c) */

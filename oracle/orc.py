@@ -87,6 +87,21 @@ def ref():
     return _ref
 
 
+FM_SIN, FM_COS, FM_TAN, FM_EXP, FM_LOG, FM_POW = range(6)
+
+
+def fmath(kind, x, y=None):
+    """The host build of lmx_fmath.h (orc_fmath.c) over float32 arrays."""
+    so = lib()
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros_like(x) if y is None else np.ascontiguousarray(np.broadcast_to(y, x.shape), dtype=np.float32)
+    out = np.empty_like(x)
+    so.orc_fmath_eval.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    so.orc_fmath_eval.restype = None
+    so.orc_fmath_eval(kind, _p(x), _p(y), x.size, _p(out))
+    return out
+
+
 def closest_hit_naive(verts, tris, rays, test=TEST_WATERTIGHT, tie=TIE_MIN_ID, nthreads=8):
     v = np.ascontiguousarray(verts, dtype=np.float32)
     t = np.ascontiguousarray(tris, dtype=np.uint32)

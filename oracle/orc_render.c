@@ -14,8 +14,17 @@
  * same consumption order: the reference's own order of two Next2D() calls in
  * one argument list is unspecified (renderer_pt.cpp:84,136); here it is left
  * to right.
+ *
+ * Sines, cosines, the tangent and powers come from lmx_fmath.h, the FP32
+ * functions the GPU kernels are compiled with (the header is pinned against
+ * double precision by tests/test_fmath.py), so that oracle and kernels round
+ * every operation of a sample alike; the vendor libraries (glibc here, the
+ * ROCm device library there) may differ by an ulp.  The reference's own
+ * sampler mappings, which tests pin against vectors from its headers, stay
+ * in orc_isect.c with the C library's functions.
  */
 #include "orc.h"
+#include "lmx_fmath.h"
 
 #include <float.h>
 #include <math.h>
@@ -190,8 +199,30 @@ static void surface_at(const orc_scene *s, v3 o, v3 d, const orc_hit *h, frame *
  * evaluate pdf (projected solid angle) and f separately
  * (renderer_pt.cpp:129-160). */
 
-/* Sampler::CosineSampleHemisphere (sampler.h:44-76) */
-void orc_sample_cosine_hemisphere(const float u[2], float out[3]);
+/* Sampler::UniformConcentricDiskSample and CosineSampleHemisphere
+ * (sampler.h:44-76), with the sine and cosine of lmx_fmath.h */
+static void cos_hemisphere(const float u[2], float out[3])
+{
+    const float vx = 2.0f * u[0] - 1.0f, vy = 2.0f * u[1] - 1.0f;
+    float x = 0.0f, y = 0.0f;
+    if (!(vx == 0.0f && vy == 0.0f)) {
+        const float pi4 = PI_F * 0.25f;
+        float r, theta;
+        if (vx > -vy) {
+            if (vx > vy) { r = vx; theta = pi4 * vy / vx; }
+            else { r = vy; theta = pi4 * (2.0f - vx / vy); }
+        } else {
+            if (vx < vy) { r = -vx; theta = pi4 * (4.0f + vy / vx); }
+            else { r = -vy; theta = pi4 * (6.0f - vx / vy); }
+        }
+        float st, ct;
+        lmx_sincosf(theta, &st, &ct);
+        x = r * ct;
+        y = r * st;
+    }
+    out[0] = x; out[1] = y;
+    out[2] = sqrtf(fmaxf(0.0f, 1.0f - x * x - y * y));
+}
 
 /* BSDFUtils::ShadingNormalCorrection for transport EL (bsdfutils.h:55-66) */
 static inline float shading_correction(const frame *f, v3 wi, v3 wo)
@@ -233,7 +264,9 @@ static v3 ggx_sample_H(float r, const float u[2])
     const float den = sqrtf(1.0f - (1.0f - r * r) * a);
     const float cosT = sqrtf(1.0f - a) / den, sinT = r * (sqrtf(a) / den);
     const float phi = PI_F * (2.0f * b - 1.0f);
-    return mk(sinT * cosf(phi), sinT * sinf(phi), cosT);
+    float sp, cp;
+    lmx_sincosf(phi, &sp, &cp);
+    return mk(sinT * cp, sinT * sp, cosT);
 }
 static float ggx_G(v3 li, v3 lo, v3 H)
 {
@@ -264,7 +297,7 @@ static int bsdf_sample(const orc_material *m, const frame *f, v3 wi, const float
     if (li.z <= 0.0f) return 0;
     if (m->type == ORC_MAT_DIFFUSE) {
         float l[3];
-        orc_sample_cosine_hemisphere(u, l);
+        cos_hemisphere(u, l);
         *wo = to_world(f, mk(l[0], l[1], l[2]));
         return 1;
     }
@@ -273,10 +306,12 @@ static int bsdf_sample(const orc_material *m, const frame *f, v3 wi, const float
         const v3 r = reflect_sn(f, wi);
         v3 ru, rv;
         onb(r, &ru, &rv);
-        const float ca = powf(u[0], 1.0f / (m->exponent + 1.0f));
+        const float ca = lmx_powf(u[0], 1.0f / (m->exponent + 1.0f));
         const float sa = sqrtf(fmaxf(0.0f, 1.0f - ca * ca));
         const float phi = 2.0f * PI_F * u[1];
-        const v3 l = mk(sa * cosf(phi), sa * sinf(phi), ca);
+        float sp, cp;
+        lmx_sincosf(phi, &sp, &cp);
+        const v3 l = mk(sa * cp, sa * sp, ca);
         *wo = vadd(vadd(vmul(ru, l.x), vmul(rv, l.y)), vmul(r, l.z));
         return 1;
     }
@@ -299,7 +334,7 @@ static float bsdf_pdf(const orc_material *m, const frame *f, v3 wi, v3 wo)
     if (m->type == ORC_MAT_DIFFUSE) return INV_PI_F;           /* sampler.h:70-76 */
     if (m->type == ORC_MAT_PHONG) {
         const float ca = fmaxf(0.0f, vdot(reflect_sn(f, wi), wo));
-        const float pdf_sa = (m->exponent + 1.0f) * (0.5f * INV_PI_F) * powf(ca, m->exponent);
+        const float pdf_sa = (m->exponent + 1.0f) * (0.5f * INV_PI_F) * lmx_powf(ca, m->exponent);
         return pdf_sa / lo.z;
     }
     if (m->type == ORC_MAT_GGX) {
@@ -319,7 +354,7 @@ static void bsdf_eval(const orc_material *m, const frame *f, v3 wi, v3 wo, float
         for (int c = 0; c < 3; c++) out[c] = m->R[c] * INV_PI_F * corr;
     } else if (m->type == ORC_MAT_PHONG) {
         const float ca = fmaxf(0.0f, vdot(reflect_sn(f, wi), wo));
-        const float k = (m->exponent + 2.0f) * (0.5f * INV_PI_F) * powf(ca, m->exponent);
+        const float k = (m->exponent + 2.0f) * (0.5f * INV_PI_F) * lmx_powf(ca, m->exponent);
         for (int c = 0; c < 3; c++) out[c] = m->R[c] * k * corr;
     } else if (m->type == ORC_MAT_GGX) {
         ggx_f(m, li, lo, out);
@@ -339,7 +374,7 @@ static void camera_init(const orc_scene_desc *d, camera *c)
 {
     c->pos = ld3(d->cam_pos); c->vx = ld3(d->cam_vx); c->vy = ld3(d->cam_vy); c->vz = ld3(d->cam_vz);
     const float fov = d->fov_deg * PI_F / 180.0f;                 /* Math::Radians, math.h:1674 */
-    c->tan_fov = tanf(fov * 0.5f);
+    c->tan_fov = lmx_tanf(fov * 0.5f);
     c->aspect = (float)d->width / (float)d->height;
 }
 
@@ -600,7 +635,7 @@ static void bsdf_eval_dir(const orc_material *m, const frame *f, v3 wi, v3 wo, i
         for (int c = 0; c < 3; c++) out[c] = m->R[c] * INV_PI_F * corr;
     } else if (m->type == ORC_MAT_PHONG) {
         const float ca = fmaxf(0.0f, vdot(reflect_sn(f, wi), wo));
-        const float k = (m->exponent + 2.0f) * (0.5f * INV_PI_F) * powf(ca, m->exponent);
+        const float k = (m->exponent + 2.0f) * (0.5f * INV_PI_F) * lmx_powf(ca, m->exponent);
         for (int c = 0; c < 3; c++) out[c] = m->R[c] * k * corr;
     } else if (m->type == ORC_MAT_GGX) {
         ggx_f(m, li, lo, out);
@@ -703,7 +738,7 @@ static int trace_subpath(const bctx *b, const orc_bdpt_params *pp, const usrc *u
                 l = l < 0 ? 0 : (l > (int)s->d.nlight - 1 ? (int)s->d.nlight - 1 : l);
                 sample_light_point(b, l, uEP, &v);
                 float lw[3];
-                orc_sample_cosine_hemisphere(uED, lw);
+                cos_hemisphere(uED, lw);
                 init_wo = to_world(&v.fr, mk(lw[0], lw[1], lw[2]));
                 const float a = 1.0f / pos_pdf(b, &v);
                 v.alpha[0] = v.alpha[1] = v.alpha[2] = a;
@@ -1351,7 +1386,7 @@ static int rb_subpath(const bctx *b, const orc_bdptref_params *pp, const usrc *u
                 l = l < 0 ? 0 : (l > (int)s->d.nlight - 1 ? (int)s->d.nlight - 1 : l);
                 sample_light_point(b, l, uP, v);
                 float lw[3];
-                orc_sample_cosine_hemisphere(uD, lw);
+                cos_hemisphere(uD, lw);
                 init_wo = to_world(&v->fr, mk(lw[0], lw[1], lw[2]));
             } else {
                 v->kind = BV_CAMERA; v->degenerate = 1; v->light = -1;
@@ -1910,7 +1945,7 @@ static int pm_trace_photon(const bctx *b, int maxv, const usrc *us, uint64_t i, 
             l = l < 0 ? 0 : (l > (int)s->d.nlight - 1 ? (int)s->d.nlight - 1 : l);
             sample_light_point(b, l, uEP, &v);
             float lw[3];
-            orc_sample_cosine_hemisphere(uED, lw);
+            cos_hemisphere(uED, lw);
             init_wo = to_world(&v.fr, mk(lw[0], lw[1], lw[2]));
             const float a = 1.0f / pos_pdf(b, &v);
             v.alpha[0] = v.alpha[1] = v.alpha[2] = a;
@@ -2219,7 +2254,7 @@ static int ppm_trace_photon(const bctx *b, int maxv, const usrc *us, uint64_t i,
             l = l < 0 ? 0 : (l > (int)s->d.nlight - 1 ? (int)s->d.nlight - 1 : l);
             sample_light_point(b, l, uEP, &v);
             float lw[3];
-            orc_sample_cosine_hemisphere(uED, lw);
+            cos_hemisphere(uED, lw);
             init_wo = to_world(&v.fr, mk(lw[0], lw[1], lw[2]));
             const float a = 1.0f / pos_pdf(b, &v);
             v.alpha[0] = v.alpha[1] = v.alpha[2] = a;

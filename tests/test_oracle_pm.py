@@ -26,8 +26,11 @@ def test_pm_furnace_closed_form(fg, M):
     rec = orc.Scene(sc).pm_samples(0, n, radius=0.02, num_photons=200000, max_num_vertices=M, finalgather=fg,
                                    seed=3, pseed=4).astype(np.float64)
     em, de = rec[:, 2:5].mean(axis=1), rec[:, 6:9].mean(axis=1)
-    # every eye path sees the emitter at vertex 1 (and 2 with final gather)
-    np.testing.assert_allclose(em, Le * (1.0 + rho * fg), rtol=1e-5)
+    # every eye path sees the emitter at vertex 1 (and 2 with final gather),
+    # except where the sampled direction is tangent to the facet (a disk
+    # sample on the unit circle: cos = 0 in float, so f = 0 at vertex 1)
+    exact = np.abs(em - Le * (1.0 + rho * fg)) <= 1e-5 * Le * (1.0 + rho * fg)
+    assert exact.mean() >= 0.999 and np.all(np.abs(em[~exact] - Le) <= 1e-5 * Le), em[~exact]
     kmax = M if M > 0 else 200
     total = sum(Le * rho ** (k - 2) for k in range(2, kmax + 1))
     want = total - Le * (1.0 + rho * fg)
