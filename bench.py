@@ -35,6 +35,59 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def median(xs):
+    xs = sorted(xs)
+    k = len(xs)
+    return xs[k // 2] if k % 2 else 0.5 * (xs[k // 2 - 1] + xs[k // 2])
+
+
+# One GPU's job on the MI355X box gets a 16-core share of the host, whatever
+# the machine's core count; the CPU baselines use every core of that share.
+HOST_SHARE_CORES = 16
+
+
+def host_cores(requested=0):
+    """(threads for the CPU baselines, what the host reports)."""
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = nproc
+    threads = requested or min(affinity, HOST_SHARE_CORES)
+    return threads, {"threads": threads, "host_nproc": nproc, "affinity": affinity,
+                     "share": HOST_SHARE_CORES, "cpu": _cpu_model()}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_native():
+    """Builds the oracle for this host's CPU (-march=native, as the
+    reference's own build, cmake/LMPlatformOptions.cmake:95) into a temporary
+    directory and points the oracle's loader at it; the results are those of
+    the portable build (no contraction, no fast-math), only the instruction
+    selection changes.  Returns the flags used, or None if the build failed
+    (the portable build is then timed)."""
+    import subprocess
+    import tempfile
+    out = os.path.join(tempfile.mkdtemp(prefix="lmx_orc_native_"), "liborc.so")
+    r = subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle"), "native", "NATIVE_OUT=" + out],
+                       capture_output=True, text=True)
+    if r.returncode != 0 or not os.path.exists(out):
+        log(f"native oracle build failed, timing the portable build: {r.stderr[-300:]}")
+        return None
+    os.environ["LMX_ORC_LIB"] = out
+    return "-O3 -march=native -ffp-contract=off"
+
+
 def traffic_from_profiles():
     """HBM bytes per traversal launch measured with rocprofv3 PMC counters,
     if a summary with a `traffic_bytes_per_launch:` line is committed."""
@@ -48,18 +101,90 @@ def traffic_from_profiles():
     return None
 
 
-def cpu_baseline(verts, tris, rays, threads):
-    """Reference-algorithm QBVH on the host (oracle restatement), timed."""
+def l2_from_profiles():
+    """L2 (TCC) hit rate of the traversal kernel from a committed
+    rocprofv3 counter summary with an `l2_hit_rate:` line, if present."""
+    path = os.path.join(ROOT, "profiles", "traversal_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("l2_hit_rate:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(verts, tris, rays, threads, reps=3):
+    """Reference-algorithm QBVH on the host (oracle restatement): the median
+    of `reps` passes over the rays."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
     t0 = time.perf_counter()
     q = orc.QBVH(verts, tris)
     build_s = time.perf_counter() - t0
-    q.intersect(rays[:4096], nthreads=threads)      # warm-up
-    t0 = time.perf_counter()
-    q.intersect(rays, nthreads=threads)
-    dt = time.perf_counter() - t0
+    q.intersect(rays[:65536], nthreads=threads)     # warm-up
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        q.intersect(rays, nthreads=threads)
+        ts.append(time.perf_counter() - t0)
+    dt = median(ts)
     return rays.shape[0] / dt / 1e6, build_s, dt
+
+
+def pt_cornell_section(dev, cpu, threads):
+    """renderer::pt on the Cornell box (5 wall quads, 2 boxes rotated by
+    +-17 degrees, 0.25 x 0.25 emitter of radiance (17, 12, 4); 256x256,
+    16 spp, paths of at most 8 segments, roulette 0.5 from depth 3, seed 42),
+    with the oracle's per-sample renderer::pt over the whole frame beside it
+    (renderer_pt.cpp:64-242)."""
+    import torch
+    from lmx import render, scenes
+    sc = scenes.cornell_box(256, 256)
+    rs = render.Scene(sc)
+    n = 256 * 256 * 16
+    kw = dict(max_num_vertices=9, rr_depth=3, rr_prob=0.5, seed=42)
+    rs.render_pt(n, **kw)                                   # capacity autotune + warm-up
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        film, st, _ = rs.render_pt(n, **kw)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    dt = median(times)
+    out = {
+        "workload": f"pt_cornell: Cornell box ({sc['tris'].shape[0]} tris incl. the emitter), 256x256, 16 spp, "
+                    "max 8 segments, RR 0.5 from depth 3, seed 42",
+        "msamples_per_s": round(n / dt / 1e6, 2),
+        "mrays_per_s": round(st["rays"] / dt / 1e6, 1),
+        "rays_per_sample": round(st["rays"] / n, 4),
+        "ms_per_frame": round(dt * 1e3, 3),
+        "queue_capacity": st["queue_capacity"],
+        "image_mean": round(float(film.mean()), 5),
+        "cpu_baseline": None,
+    }
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import orc
+        osc = orc.Scene(sc)
+        osc.pt_samples(0, 4096, nthreads=threads, **kw)
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            c = osc.pt_samples(0, n, nthreads=threads, **kw)
+            ts.append(time.perf_counter() - t0)
+        cdt = median(ts)
+        img = osc.film(c) * (256 * 256 / n)
+        out["cpu_baseline"] = {
+            "msamples_per_s": round(n / cdt / 1e6, 3),
+            "mrays_per_s_est": round(n / cdt * st["rays"] / n / 1e6, 2),
+            "cores": threads, "kind": "port",
+            "sample": f"the whole frame ({n} samples), median of 3 runs of the oracle's renderer::pt; "
+                      "Mrays/s with the GPU's rays per sample",
+            "image_mean": round(float(img.mean()), 5)}
+        out["gpu_over_cpu_samples"] = round((n / dt) / (n / cdt), 1)
+    return out
 
 
 def pt_section(dev, cpu, threads):
@@ -79,13 +204,13 @@ def pt_section(dev, cpu, threads):
     rs.render_pt(1024 * 1024 * 16, **kw)
     torch.cuda.synchronize()
     times, rays = [], 0
-    for _ in range(3):
+    for _ in range(5):
         t0 = time.perf_counter()
         film, st, _ = rs.render_pt(n, **kw)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
         rays = st["rays"]
-    dt = min(times)
+    dt = median(times)
     out = {
         "workload": "pt: 16x16 displaced icospheres (5,242,886 tris), 1024x1024, 64 spp, depth 16, RR from depth 3",
         "samples_per_s": round(n / dt, 1),
@@ -150,12 +275,12 @@ def bdpt_section(dev, cpu, threads):
     rs.render_vcm_bdpt(1 << 22, **kw)                     # autotune + warm-up
     torch.cuda.synchronize()
     times = []
-    for _ in range(3):
+    for _ in range(5):
         t0 = time.perf_counter()
         film, st, _ = rs.render_vcm_bdpt(n, **kw)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
-    dt = min(times)
+    dt = median(times)
     rays = st["rays"]
     out = {
         "workload": "bdpt: 16x16 displaced icospheres (5,242,886 tris), 1024x1024, 16 spp, "
@@ -182,7 +307,7 @@ def bdpt_section(dev, cpu, threads):
             fb, sb, _ = rs.render_bdpt(n, **kb)
             torch.cuda.synchronize()
             tb.append(time.perf_counter() - t0)
-        db = min(tb)
+        db = median(tb)
         out[key] = {"max_num_vertices": m, "msamples_per_s": round(n / db / 1e6, 2),
                     "mrays_per_s": round(sb["rays"] / db / 1e6, 1), "rays_per_sample": round(sb["rays"] / n, 4),
                     "s_per_frame": round(db, 4), "batch": sb["batch"], "image_mean": round(float(fb.mean()), 5)}
@@ -227,7 +352,7 @@ def bdpt_section(dev, cpu, threads):
         fl, sl, _ = rs.render_ltdirect(n, **kl)
         torch.cuda.synchronize()
         tl.append(time.perf_counter() - t0)
-    dl = min(tl)
+    dl = median(tl)
     out["ltdirect"] = {"max_num_vertices": 10, "msamples_per_s": round(n / dl / 1e6, 2),
                        "mrays_per_s": round(sl["rays"] / dl / 1e6, 1), "rays_per_sample": round(sl["rays"] / n, 4),
                        "s_per_frame": round(dl, 4), "image_mean": round(float(fl.mean()), 5), "cpu_baseline": None}
@@ -279,7 +404,7 @@ def vcm_section(dev, cpu, threads):
             film, st, _ = rs.render_vcm(ne, mode=mode, **kw)
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t0)
-        dt = min(times)
+        dt = median(times)
         out[key] = {"msamples_per_s": round(passes * ne / dt / 1e6, 2),
                     "mrays_per_s": round(st["rays"] / dt / 1e6, 1),
                     "rays_per_sample": round(st["rays"] / (passes * ne), 4),
@@ -296,7 +421,7 @@ def vcm_section(dev, cpu, threads):
             t0 = time.perf_counter()
             osc.vcm_samples(0, 1, nthreads=threads, **m)
             tph.append(time.perf_counter() - t0)
-        tph = min(tph)
+        tph = median(tph)
         chunk = 65536
         t0 = time.perf_counter()
         osc.vcm_samples(0, chunk, nthreads=threads, **m)
@@ -319,7 +444,7 @@ def vcm_section(dev, cpu, threads):
         fpm, spm, _ = rs.render_pm(npm, **kp)
         torch.cuda.synchronize()
         tp.append(time.perf_counter() - t0)
-    dp = min(tp)
+    dp = median(tp)
     out["pm"] = {"workload": "pm: same frame, 16 spp, 2^22 photon subpaths (roulette), radius 0.05, final gather, "
                              "at most 10 vertices",
                  "msamples_per_s": round(npm / dp / 1e6, 2), "mrays_per_s": round(spm["rays"] / dp / 1e6, 1),
@@ -352,7 +477,7 @@ def vcm_section(dev, cpu, threads):
         fpp, spp_, _ = rs.render_ppm(npix, 10, **kq)
         torch.cuda.synchronize()
         tq.append(time.perf_counter() - t0)
-    dq = min(tq)
+    dq = median(tq)
     out["ppm"] = {"workload": "ppm: same frame, one measurement point per pixel (1,048,576), 32 passes of 2^20 "
                               "photon subpaths, initial radius 0.05, alpha 0.7, at most 10 vertices",
                   "ms_per_pass": round(dq / passes * 1e3, 3), "mphotons_per_s": round(passes * nphp / dq / 1e6, 1),
@@ -383,7 +508,7 @@ def vcm_section(dev, cpu, threads):
         fsp, _, _ = rs.render_sppm(10, **ks)
         torch.cuda.synchronize()
         tsp.append(time.perf_counter() - t0)
-    dsp = min(tsp)
+    dsp = median(tsp)
     out["sppm"] = {"workload": "sppm: same frame, a point per pixel re-traced every pass, 32 passes of 2^20 "
                                "photon subpaths, initial radius 0.05, alpha 0.7, at most 10 vertices",
                    "ms_per_pass": round(dsp / passes * 1e3, 3),
@@ -445,8 +570,8 @@ MLP_FLOP_FWD = 2 * (168 * 64 + 3 * 64 * 64 + 64 * 41)     # 51,328 algorithmic F
 MFMA_BF16_PEAK = 2.5e15
 
 
-def guide_section(dev, ref_spp=16384, spp=256, train_passes=16, steps=16, batch=1 << 15, lr=5e-3, cache=None,
-                  trials_n=3):
+def guide_section(dev, ref_spp=16384, spp=256, train_passes=16, steps=16, batch=1 << 15, lr=1e-3, cache=None,
+                  trials_n=3, mfma=True):
     """Learned path guiding on the occluded-room scene (12 quads + 96
     displaced spheres = 1,966,104 tris, 40% Phong e=200, emitter behind a wall
     with a 0.1-wide gap), 1024x1024: train online on the first 16 spp (1 spp
@@ -531,6 +656,9 @@ def guide_section(dev, ref_spp=16384, spp=256, train_passes=16, steps=16, batch=
                        "plain_s": round(t_plain, 3), "guided_msamples_per_s": round(npix * spp / t_render / 1e6, 1),
                        "train_records": recs, "train_records_nonzero": nz})
     med = sorted(trials, key=lambda t: t["mse_ratio_equal_time"])[len(trials) // 2]
+    if not mfma:
+        return {"lr": lr, **med, "trials_equal_time": [t["mse_ratio_equal_time"] for t in trials],
+                "trials_equal_spp": [t["mse_ratio_equal_spp"] for t in trials]}
     # MFMA utilisation: inference over 2^22 queries, training over 2^16 records
     x = torch.rand((1 << 22, 7), device=dev)
     g.eval(x)
@@ -617,10 +745,14 @@ def main():
     from lmx import accel, scenes
 
     dev = torch.device("cuda:0")
+    threads, host = host_cores(args.cpu_threads)
+    native = None if args.no_cpu_baseline else oracle_native()
+    host["oracle_build"] = native or "-O3 -march=x86-64-v3 -ffp-contract=off (portable build)"
     if args.only:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         if args.only == "guide":
-            out = guide_section(dev, args.guide_ref_spp)
+            cache = {}
+            out = guide_section(dev, args.guide_ref_spp, lr=1e-3, cache=cache)
+            out["alt_lr"] = guide_section(dev, args.guide_ref_spp, lr=5e-3, cache=cache, mfma=False)
         elif args.only == "image":
             out = image_section(dev, not args.no_cpu_baseline)
         elif args.only == "vcm":
@@ -689,6 +821,7 @@ def main():
     alg_bytes = RAY_BYTES * nray + NODE_BYTES * nodes_v + TRI_BYTES * tris_t
     achieved = alg_bytes / avg_kernel_s / 1e9
     traffic = traffic_from_profiles()
+    l2 = l2_from_profiles()
 
     hits_np = d_hits.cpu().numpy()
     hit_frac = float((accel.hit_ids(hits_np) != accel.MISS).mean())
@@ -725,6 +858,13 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this benchmark, scaled by a copy "
+                              "kernel's counts (profiles/traversal_traffic.txt); not measured in this run",
+            "dram_gbs": None if traffic is None else round(traffic / avg_kernel_s / 1e9, 1),
+            "dram_frac": None if traffic is None else round(traffic / avg_kernel_s / 1e9 / HBM_PEAK_GBS, 4),
+            "l2_hit_rate": l2,
+            "limiter": "node-fetch latency: the nodes and triangles (98 MB) stay mostly in L2 and the Infinity "
+                       "Cache, so DRAM traffic is a fraction of the algorithmic bytes",
             "alg_bytes_per_launch": int(alg_bytes),
             "node_visits_per_ray": round(nodes_v / nray, 3),
             "tri_tests_per_ray": round(tris_t / nray, 3),
@@ -734,26 +874,31 @@ def main():
     }
 
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        sample = np.concatenate([coh[::4], inc[::4]])
-        log(f"cpu baseline: oracle QBVH, {threads} threads, {sample.shape[0]} rays")
-        v, bsec, tsec = cpu_baseline(verts, tris, sample, threads)
+        log(f"cpu baseline: oracle QBVH, {threads} threads, {nray} rays x 3")
+        v, bsec, tsec = cpu_baseline(verts, tris, rays, threads)
         out["cpu_baseline"] = {
             "value": round(v, 2),
             "unit": "Mrays/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"every 4th ray of both sets ({sample.shape[0]} rays), same mesh; "
-                      f"QBVH build {bsec:.2f}s (not timed), trace {tsec:.2f}s",
+            "sample": f"all {nray} rays of both sets, same mesh, median of 3 passes of {tsec:.2f}s "
+                      f"({threads} threads, {threads * 3 * tsec:.0f} core-seconds); QBVH build {bsec:.2f}s not timed",
+            "host": host,
         }
+        out["gpu_over_cpu"] = round(mrays / v, 1)
     if not args.no_pt:
-        out["pt"] = pt_section(dev, not args.no_cpu_baseline, args.cpu_threads or min(os.cpu_count() or 1, 16))
+        out["pt_cornell"] = pt_cornell_section(dev, not args.no_cpu_baseline, threads)
+        out["pt"] = pt_section(dev, not args.no_cpu_baseline, threads)
     if not args.no_bdpt:
-        out["bdpt"] = bdpt_section(dev, not args.no_cpu_baseline, args.cpu_threads or min(os.cpu_count() or 1, 16))
+        out["bdpt"] = bdpt_section(dev, not args.no_cpu_baseline, threads)
     if not args.no_vcm:
-        out["vcm"] = vcm_section(dev, not args.no_cpu_baseline, args.cpu_threads or min(os.cpu_count() or 1, 16))
+        out["vcm"] = vcm_section(dev, not args.no_cpu_baseline, threads)
     if not args.no_guide:
-        out["guide"] = guide_section(dev, args.guide_ref_spp)
+        cache = {}
+        out["guide"] = guide_section(dev, args.guide_ref_spp, lr=1e-3, cache=cache)
+        # the same trials at a larger learning rate, reported beside the
+        # workload's 1e-3
+        out["guide"]["alt_lr"] = guide_section(dev, args.guide_ref_spp, lr=5e-3, cache=cache, mfma=False)
     if not args.no_image:
         out["image"] = image_section(dev, not args.no_cpu_baseline)
     print(json.dumps(out), flush=True)

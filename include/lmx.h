@@ -185,6 +185,16 @@ typedef struct lmx_render_params {
     float *sample_out;           /* device, num_samples x 8 floats (raster x, y, C rgb, #splats, 0, 0), or NULL */
     int32_t rr_mode;             /* LMX_RR_FIXED (reference) or LMX_RR_THROUGHPUT */
     int32_t mis;                 /* renderer::bdpt: LMX_MIS_POWER (reference default) or LMX_MIS_SIMPLE */
+    /* Time mode of the reference's scheduler (render_time, scheduler.cpp:57,
+     * 108,191-199): > 0 renders for this many seconds instead of num_samples
+     * -- new camera samples (the unidirectional samplers) or batches (the
+     * bidirectional ones) start until the deadline, every started sample
+     * completes, and the film is scaled by W H / (samples taken), reported
+     * in lmx_render_stats.samples.  The clock starts after the one-time
+     * autotuning of the scene.  <= 0: sample-count mode.  Not with
+     * uniform_table or sample_out.  lmx_render_pt, _ptdirect, _ptmis,
+     * _vcm_bdpt and _bdpt. */
+    double render_time;
 } lmx_render_params;
 
 typedef struct lmx_render_stats {
@@ -192,6 +202,7 @@ typedef struct lmx_render_stats {
     uint32_t iterations;         /* wavefront iterations */
     uint32_t queue_capacity;     /* capacity used */
     double ms;                   /* wall time of the render call */
+    uint64_t samples;            /* camera samples taken (num_samples unless render_time > 0) */
 } lmx_render_stats;
 
 /* Builds the accel (watertight, 8-wide) and uploads shading data to the

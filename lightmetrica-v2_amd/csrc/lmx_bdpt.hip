@@ -1736,6 +1736,9 @@ int render_bidir(lmx_scene* sc, const lmx_render_params* p, int variant, const c
     }
     if (p->raster_mode != LMX_RASTER_RANDOM && p->raster_mode != LMX_RASTER_STRATIFIED)
         return set_error(LMX_EINVAL, (std::string(name) + ": unknown raster_mode").c_str());
+    const bool timed = p->render_time > 0.0;
+    if (timed && (p->uniform_table || p->sample_out))
+        return set_error(LMX_EINVAL, (std::string(name) + ": render_time takes neither a uniform table nor sample_out").c_str());
     hipStream_t st = (hipStream_t)stream;
     hipStream_t own = nullptr;
     hipError_t e;
@@ -1744,6 +1747,9 @@ int render_bidir(lmx_scene* sc, const lmx_render_params* p, int variant, const c
         st = own;
     }
     const auto t0 = std::chrono::steady_clock::now();
+    // time mode: batches of the autotuned size for the largest candidate,
+    // started until the deadline (at most 2^32 - 1 samples)
+    const uint64_t num_samples = timed ? 0xffffffffull : p->num_samples;
     BParams bp;
     bp.variant = variant;
     bp.merging = 0; bp.connect = 1; bp.phase = 0; bp.c_conn = 1.0f;
@@ -1751,7 +1757,7 @@ int render_bidir(lmx_scene* sc, const lmx_render_params* p, int variant, const c
     bp.bounded = bounded ? 1 : 0;
     bp.maxv = (int)maxv; bp.minv = p->min_num_vertices;
     bp.key = make_uint2((uint32_t)p->seed, (uint32_t)(p->seed >> 32));
-    bp.total = (uint32_t)p->num_samples;
+    bp.total = (uint32_t)num_samples;
     bp.raster_mode = p->raster_mode;
     bp.table = p->uniform_table; bp.table_width = p->table_width;
     bp.wsub = wsub;
@@ -1763,16 +1769,16 @@ int render_bidir(lmx_scene* sc, const lmx_render_params* p, int variant, const c
     // candidate)
     uint32_t B = p->queue_capacity;
     if (!B) {
-        if (p->num_samples <= (1u << 19)) B = (uint32_t)p->num_samples;
+        if (num_samples <= (1u << 19)) B = (uint32_t)num_samples;
         else {
             // tuned again when an earlier, smaller render capped the candidates
             if (sc->bd_tuned_key != bd_key(bp) ||
-                (sc->bd_tuned_limit < (1ull << 24) && sc->bd_tuned_limit < p->num_samples))
-                rc = bd_autotune(sc, bp, p->num_samples, film_rgb, st);
+                (sc->bd_tuned_limit < (1ull << 24) && sc->bd_tuned_limit < num_samples))
+                rc = bd_autotune(sc, bp, num_samples, film_rgb, st);
             B = sc->bd_tuned_batch;
         }
     }
-    B = (uint32_t)std::min<uint64_t>(B, p->num_samples);
+    B = (uint32_t)std::min<uint64_t>(B, num_samples);
     bp.batch = B;
     if (rc == LMX_OK) rc = bd_buffers(sc, B, maxv, bd_conn_per_sample(maxv, variant, bounded));
     const size_t npix = (size_t)sc->width * (size_t)sc->height;
@@ -1783,11 +1789,29 @@ int render_bidir(lmx_scene* sc, const lmx_render_params* p, int variant, const c
         k_bd_init<<<1, 64, 0, st>>>(sc->bd_ctl.p);
         rc = bd_graph(sc, bp, st, &exec);
     }
-    const uint64_t nbatch = (p->num_samples + B - 1) / B;
-    for (uint64_t b = 0; rc == LMX_OK && b < nbatch; b++)
-        if ((e = hipGraphLaunch(exec, st)) != hipSuccess) rc = hip_error(e, "graph launch");
+    uint64_t nbatch = (num_samples + B - 1) / B;
+    uint64_t taken = p->num_samples;
+    if (!timed) {
+        for (uint64_t b = 0; rc == LMX_OK && b < nbatch; b++)
+            if ((e = hipGraphLaunch(exec, st)) != hipSuccess) rc = hip_error(e, "graph launch");
+    } else {
+        // the scheduler's time mode: whole batches until the deadline, each
+        // finished before the clock is read (scheduler.cpp:108,191-199)
+        const auto ts = std::chrono::steady_clock::now();
+        uint64_t b = 0;
+        while (rc == LMX_OK && b < nbatch) {
+            if ((e = hipGraphLaunch(exec, st)) != hipSuccess || (e = hipStreamSynchronize(st)) != hipSuccess) {
+                rc = hip_error(e, "graph launch");
+                break;
+            }
+            b++;
+            if (std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count() >= p->render_time) break;
+        }
+        nbatch = b;
+        taken = b * (uint64_t)B;
+    }
     if (rc == LMX_OK) {
-        k_bd_scale<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix, (float)((double)npix / (double)p->num_samples));
+        k_bd_scale<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix, (float)((double)npix / (double)std::max<uint64_t>(taken, 1)));
         if ((e = hipMemcpyAsync(sc->bd_hctl, sc->bd_ctl.p, sizeof(BCtl), hipMemcpyDeviceToHost, st)) != hipSuccess ||
             (e = hipStreamSynchronize(st)) != hipSuccess)
             rc = hip_error(e, (std::string(name) + ": sync").c_str());
@@ -1799,6 +1823,7 @@ int render_bidir(lmx_scene* sc, const lmx_render_params* p, int variant, const c
         stats->iterations = (uint32_t)nbatch;
         stats->queue_capacity = B;
         stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        stats->samples = rc == LMX_OK ? taken : 0;
     }
     return rc;
 }
@@ -2013,6 +2038,7 @@ int render_vcm(lmx_scene* sc, const lmx_render_params* p, const lmx_vcm_params* 
         stats->iterations = v->num_iteration_pass;
         stats->queue_capacity = B;
         stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        stats->samples = rc == LMX_OK ? p->num_samples * v->num_iteration_pass : 0;
     }
     return rc;
 }
@@ -2153,6 +2179,7 @@ int render_pm(lmx_scene* sc, const lmx_render_params* p, const lmx_pm_params* v,
         stats->iterations = (uint32_t)nbatch;
         stats->queue_capacity = B;
         stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        stats->samples = rc == LMX_OK ? p->num_samples : 0;
     }
     return rc;
 }
@@ -2283,6 +2310,7 @@ int render_ppm(lmx_scene* sc, const lmx_render_params* p, const lmx_ppm_params* 
         stats->iterations = v->num_iteration_pass;
         stats->queue_capacity = N;
         stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        stats->samples = rc == LMX_OK ? p->num_samples : 0;
     }
     return rc;
 }
@@ -2409,6 +2437,7 @@ int render_sppm(lmx_scene* sc, const lmx_render_params* p, const lmx_ppm_params*
         stats->iterations = v->num_iteration_pass;
         stats->queue_capacity = npix;
         stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        stats->samples = rc == LMX_OK ? (uint64_t)npix * v->num_iteration_pass : 0;
     }
     return rc;
 }
@@ -2512,6 +2541,7 @@ int render_ltdirect(lmx_scene* sc, const lmx_render_params* p, float* film_rgb, 
         stats->iterations = (uint32_t)nbatch;
         stats->queue_capacity = B;
         stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        stats->samples = rc == LMX_OK ? p->num_samples : 0;
     }
     return rc;
 }
@@ -2545,6 +2575,7 @@ int render_raycast(lmx_scene* sc, float* film_rgb, void* stream, lmx_render_stat
         stats->iterations = 1;
         stats->queue_capacity = n;
         stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        stats->samples = rc == LMX_OK ? n : 0;
     }
     return rc;
 }

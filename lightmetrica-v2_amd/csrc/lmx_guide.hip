@@ -1207,6 +1207,7 @@ int lmx_render_guided(lmx_scene* sc, lmx_guide* g, const lmx_render_params* p, i
 {
     if (!sc || !g || !p || !film_rgb) return set_error(LMX_EINVAL, "lmx_render_guided: NULL argument");
     if (g->scene != sc) return set_error(LMX_EINVAL, "lmx_render_guided: guide belongs to another scene");
+    if (p->render_time > 0.0) return set_error(LMX_EINVAL, "lmx_render_guided: render_time is not supported");
     if (p->num_samples == 0 || p->num_samples > 0xffffffffull)
         return set_error(LMX_EINVAL, "lmx_render_guided: num_samples must be in [1, 2^32)");
     if (p->max_num_vertices != -1 && p->max_num_vertices < 2)
@@ -1324,6 +1325,7 @@ int lmx_render_guided(lmx_scene* sc, lmx_guide* g, const lmx_render_params* p, i
         stats->iterations = iters;
         stats->queue_capacity = cap;
         stats->ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
+        stats->samples = rc == LMX_OK ? p->num_samples : 0;
     }
     if (gs) {
         gs->records = nrec;

@@ -216,6 +216,7 @@ __global__ __launch_bounds__(256) void k_shade(RScene s, RParams p, Ctl* __restr
 
 RD uint32_t gen_count(const Ctl* ctl, uint32_t cap, uint32_t total)
 {
+    if (ctl->stop) return 0;
     const uint32_t free_slots = cap - ctl->n_next;
     const uint32_t left = total - ctl->issued;
     return free_slots < left ? free_slots : left;
@@ -321,8 +322,14 @@ __global__ void k_init_ctl(Ctl* ctl)
 {
     if (threadIdx.x == 0) {
         ctl->n_trace = 0; ctl->n_next = 0; ctl->issued = 0; ctl->iterations = 0; ctl->rays = 0;
-        ctl->n_shadow = 0; ctl->shadow_rays = 0;
+        ctl->n_shadow = 0; ctl->shadow_rays = 0; ctl->stop = 0;
     }
+}
+
+// Time mode: camera samples stop starting from the next iteration on.
+__global__ void k_stop(Ctl* ctl)
+{
+    if (threadIdx.x == 0) ctl->stop = 1;
 }
 
 __global__ void k_zero(float* a, size_t n)
@@ -403,7 +410,7 @@ int enqueue_iteration(lmx_scene* sc, const RParams& rp, int a, int b, hipStream_
 }  // namespace
 
 int wavefront_render(lmx_scene* sc, const RParams& rp_in, uint32_t cap, hipStream_t st, uint64_t* rays,
-                     uint32_t* iters)
+                     uint32_t* iters, double render_time, uint64_t* samples)
 {
     int rc = ensure_queues(sc, cap);
     if (rc != LMX_OK) return rc;
@@ -436,8 +443,12 @@ int wavefront_render(lmx_scene* sc, const RParams& rp_in, uint32_t cap, hipStrea
         return hip_error(e, "graph instantiate");
     }
     // replay until every sample is issued and every queue is empty; the
-    // control block is read back every `batch` replays
+    // control block is read back every `batch` replays.  Time mode: once the
+    // deadline has passed no new camera sample starts, and the replays go on
+    // until the samples already started are finished.
     const int batch = 4;
+    const auto t_start = std::chrono::steady_clock::now();
+    bool stopped = false;
     rc = LMX_OK;
     for (int guard = 0; guard < 1000000; guard++) {
         for (int k = 0; k < batch; k++) {
@@ -449,12 +460,18 @@ int wavefront_render(lmx_scene* sc, const RParams& rp_in, uint32_t cap, hipStrea
             rc = hip_error(e, "control readback");
             break;
         }
-        if (sc->h_ctl->n_trace == 0 && sc->h_ctl->issued >= rp.total) break;
+        if (sc->h_ctl->n_trace == 0 && (sc->h_ctl->issued >= rp.total || stopped)) break;
+        if (render_time > 0.0 && !stopped &&
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count() >= render_time) {
+            k_stop<<<1, 64, 0, st>>>(sc->ctl.p);
+            stopped = true;
+        }
     }
     (void)hipGraphExecDestroy(exec);
     (void)hipGraphDestroy(graph);
     if (rays) *rays = sc->h_ctl->rays + sc->h_ctl->shadow_rays;
     if (iters) *iters = sc->h_ctl->iterations;
+    if (samples) *samples = sc->h_ctl->issued;
     return rc;
 }
 
@@ -518,6 +535,9 @@ static int render_unidir(lmx_scene* sc, const lmx_render_params* p, int nee, con
         return set_error(LMX_EINVAL, std::string(name) + ": unknown rr_mode");
     if (p->raster_mode != LMX_RASTER_RANDOM && p->raster_mode != LMX_RASTER_STRATIFIED)
         return set_error(LMX_EINVAL, std::string(name) + ": unknown raster_mode");
+    const bool timed = p->render_time > 0.0;
+    if (timed && (p->uniform_table || p->sample_out))
+        return set_error(LMX_EINVAL, std::string(name) + ": render_time takes neither a uniform table nor sample_out");
     hipStream_t st = (hipStream_t)stream;
     hipStream_t own = nullptr;
     hipError_t e;
@@ -531,7 +551,7 @@ static int render_unidir(lmx_scene* sc, const lmx_render_params* p, int nee, con
     rp.max_nv = p->max_num_vertices; rp.min_nv = p->min_num_vertices; rp.rr_depth = p->rr_depth;
     rp.rr_prob = p->rr_prob;
     rp.key = make_uint2((uint32_t)p->seed, (uint32_t)(p->seed >> 32));
-    rp.total = (uint32_t)p->num_samples;
+    rp.total = timed ? 0xffffffffu : (uint32_t)p->num_samples;
     rp.raster_mode = p->raster_mode;
     rp.table = p->uniform_table; rp.table_width = p->table_width;
     rp.sample_out = p->sample_out;
@@ -545,15 +565,16 @@ static int render_unidir(lmx_scene* sc, const lmx_render_params* p, int nee, con
     int rc = LMX_OK;
     if (p->queue_capacity == 0 && sc->tuned_cap == 0) rc = autotune_capacity(sc, rp, st);
     uint32_t cap = p->queue_capacity ? p->queue_capacity : sc->tuned_cap;
-    cap = (uint32_t)std::min<uint64_t>(cap, std::max<uint64_t>(p->num_samples, 64));
-    uint64_t rays = 0;
+    if (!timed) cap = (uint32_t)std::min<uint64_t>(cap, std::max<uint64_t>(p->num_samples, 64));
+    uint64_t rays = 0, taken = 0;
     uint32_t iters = 0;
     if (rc == LMX_OK) {
         k_zero<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix);
-        rc = wavefront_render(sc, rp, cap, st, &rays, &iters);
+        rc = wavefront_render(sc, rp, cap, st, &rays, &iters, timed ? p->render_time : 0.0, &taken);
     }
     if (rc == LMX_OK) {
-        k_scale<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix, (float)((double)npix / (double)p->num_samples));
+        // Film::Rescale by W H / the samples taken (scheduler.cpp:281-288)
+        k_scale<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix, (float)((double)npix / (double)std::max<uint64_t>(taken, 1)));
         if ((e = hipStreamSynchronize(st)) != hipSuccess) rc = hip_error(e, "render sync");
     }
     if (own) (void)hipStreamDestroy(own);
@@ -562,6 +583,7 @@ static int render_unidir(lmx_scene* sc, const lmx_render_params* p, int nee, con
         stats->iterations = iters;
         stats->queue_capacity = cap;
         stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        stats->samples = rc == LMX_OK ? taken : 0;
     }
     return rc;
 }

@@ -56,7 +56,7 @@ struct Ctl {
     uint32_t iterations;
     unsigned long long rays;   // extension rays traced so far
     uint32_t n_shadow;         // next-event shadow rays of the iteration
-    uint32_t pad;
+    uint32_t stop;             // time mode: no new camera samples once set
     unsigned long long shadow_rays;
 };
 
@@ -414,7 +414,7 @@ struct lmx_scene;
 // shade (guided when rp.guide is set), graph-replayed until every sample is
 // done.  `cap` paths in flight.
 int wavefront_render(lmx_scene* sc, const lmx::RParams& rp, uint32_t cap, hipStream_t st, uint64_t* rays,
-                     uint32_t* iters);
+                     uint32_t* iters, double render_time = 0.0, uint64_t* samples = nullptr);
 // The guided shading step of one iteration (lmx_guide.hip).
 int launch_guided_shade(lmx_scene* sc, const lmx::RParams& rp, lmx::Queue in, lmx::Queue out, hipStream_t st);
 

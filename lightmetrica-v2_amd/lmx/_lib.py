@@ -63,12 +63,13 @@ class lmx_render_params(ctypes.Structure):
                 ("seed", ctypes.c_uint64), ("num_samples", ctypes.c_uint64),
                 ("raster_mode", ctypes.c_int32), ("queue_capacity", ctypes.c_uint32),
                 ("uniform_table", ctypes.c_void_p), ("table_width", ctypes.c_uint32),
-                ("sample_out", ctypes.c_void_p), ("rr_mode", ctypes.c_int32), ("mis", ctypes.c_int32)]
+                ("sample_out", ctypes.c_void_p), ("rr_mode", ctypes.c_int32), ("mis", ctypes.c_int32),
+                ("render_time", ctypes.c_double)]
 
 
 class lmx_render_stats(ctypes.Structure):
     _fields_ = [("rays", ctypes.c_uint64), ("iterations", ctypes.c_uint32),
-                ("queue_capacity", ctypes.c_uint32), ("ms", ctypes.c_double)]
+                ("queue_capacity", ctypes.c_uint32), ("ms", ctypes.c_double), ("samples", ctypes.c_uint64)]
 
 
 class lmx_vcm_params(ctypes.Structure):

@@ -122,22 +122,26 @@ class Scene:
         return p
 
     def render_vcm_bdpt(self, num_samples, max_num_vertices=10, min_num_vertices=0, seed=42,
-                        raster_mode=RASTER_STRATIFIED, batch=0, table=None, sample_out=False, stream=None):
+                        raster_mode=RASTER_STRATIFIED, batch=0, table=None, sample_out=False, stream=None,
+                        render_time=0.0):
         """renderer::vcm in bdpt mode.  Returns (film (H, W, 3), stats,
-        per-sample records (n, (m+1)^2, 8) or None)."""
+        per-sample records (n, (m+1)^2, 8) or None).  render_time > 0: the
+        scheduler's time mode (num_samples is then ignored; stats["samples"]
+        says how many were taken)."""
         return self._bidir("lmx_render_vcm_bdpt", (max_num_vertices + 1) ** 2, num_samples, max_num_vertices,
-                           min_num_vertices, seed, raster_mode, batch, table, sample_out, stream, MIS_POWER)
+                           min_num_vertices, seed, raster_mode, batch, table, sample_out, stream, MIS_POWER,
+                           render_time)
 
     def render_bdpt(self, num_samples, max_num_vertices=-1, min_num_vertices=0, seed=42,
                     raster_mode=RASTER_STRATIFIED, batch=0, table=None, sample_out=False, stream=None,
-                    mis=MIS_POWER):
+                    mis=MIS_POWER, render_time=0.0):
         """renderer::bdpt (roulette subpaths, direct light vertices, power or
         simple weights).  Returns (film (H, W, 3), stats, per-sample records
         (n, 2 (M+1)^2, 8) at 2 (t (M+1) + s) + direct, or None); M =
         max_num_vertices, 32 when unbounded (-1)."""
         m = 32 if max_num_vertices == -1 else max_num_vertices
         return self._bidir("lmx_render_bdpt", 2 * (m + 1) ** 2, num_samples, max_num_vertices,
-                           min_num_vertices, seed, raster_mode, batch, table, sample_out, stream, mis)
+                           min_num_vertices, seed, raster_mode, batch, table, sample_out, stream, mis, render_time)
 
     def render_vcm(self, num_samples, mode=VCM_MODE_VCM, num_iteration_pass=100, num_photon_trace_samples=10000,
                    initial_radius=0.1, alpha=0.7, max_num_vertices=10, min_num_vertices=0, seed=42,
@@ -322,12 +326,13 @@ class Scene:
         return film, {"rays": st.rays, "ms": st.ms}
 
     def _bidir(self, entry, nrec, num_samples, max_num_vertices, min_num_vertices, seed, raster_mode, batch,
-               table, sample_out, stream, mis):
+               table, sample_out, stream, mis, render_time=0.0):
         import torch
         dev = torch.device("cuda")
         film = torch.empty((self.height, self.width, 3), dtype=torch.float32, device=dev)
         p = self._params(num_samples, max_num_vertices, min_num_vertices, 1, 0.5, seed, raster_mode, batch)
         p.mis = mis
+        p.render_time = render_time
         so = None
         if table is not None:
             assert table.is_cuda and table.dtype == torch.float32 and table.is_contiguous()
@@ -343,7 +348,8 @@ class Scene:
             stream.wait_stream(torch.cuda.current_stream(dev))
         check(getattr(lib, entry)(self._h, ctypes.byref(p), ctypes.c_void_p(film.data_ptr()),
                                   ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st)))
-        stats = {"rays": st.rays, "batches": st.iterations, "batch": st.queue_capacity, "ms": st.ms}
+        stats = {"rays": st.rays, "batches": st.iterations, "batch": st.queue_capacity, "ms": st.ms,
+                 "samples": st.samples}
         return film, stats, so
 
     def render_ptdirect(self, num_samples, **kw):
@@ -357,10 +363,12 @@ class Scene:
 
     def render_pt(self, num_samples, max_num_vertices=9, min_num_vertices=0, rr_depth=1, rr_prob=0.5,
                   seed=42, raster_mode=RASTER_STRATIFIED, queue_capacity=0, table=None, sample_out=False,
-                  stream=None, rr_mode=RR_FIXED, _entry="lmx_render_pt"):
+                  stream=None, rr_mode=RR_FIXED, render_time=0.0, _entry="lmx_render_pt"):
         """Returns (film as a torch tensor (H, W, 3) on the GPU, stats dict,
         per-sample records (n, 8) or None).  `table`: torch float32 tensor
-        (num_samples, width) on the GPU, or None for Philox."""
+        (num_samples, width) on the GPU, or None for Philox.  render_time > 0:
+        the scheduler's time mode (num_samples ignored; stats["samples"] is the
+        number taken)."""
         import torch
         dev = torch.device("cuda")
         film = torch.empty((self.height, self.width, 3), dtype=torch.float32, device=dev)
@@ -375,6 +383,7 @@ class Scene:
         p.raster_mode = raster_mode
         p.queue_capacity = queue_capacity
         p.rr_mode = rr_mode
+        p.render_time = render_time
         if table is not None:
             assert table.is_cuda and table.dtype == torch.float32 and table.is_contiguous()
             assert table.shape[0] == num_samples
@@ -391,5 +400,6 @@ class Scene:
             stream.wait_stream(torch.cuda.current_stream(dev))
         check(getattr(lib, _entry)(self._h, ctypes.byref(p), ctypes.c_void_p(film.data_ptr()),
                                    ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st)))
-        stats = {"rays": st.rays, "iterations": st.iterations, "queue_capacity": st.queue_capacity, "ms": st.ms}
+        stats = {"rays": st.rays, "iterations": st.iterations, "queue_capacity": st.queue_capacity, "ms": st.ms,
+                 "samples": st.samples}
         return film, stats, so

@@ -24,6 +24,9 @@
  *   orc_qbvh_*            hit index equal to brute force on meshes of up to
  *                         1e5 triangles (tests/test_oracle.py).
  *   orc_sample_*          bit-exact against the reference's sampler.h via _ref.
+ *   orc_dsfmt_*           the reference's dSFMT-19937 stream: equal to the
+ *                         golden streams of seeds 1 and 42 written by its
+ *                         vendored dSFMT.c (tests/test_oracle.py).
  *   orc_fmath_eval        the shared FP32 math of lmx_fmath.h, which the
  *                         render oracle uses: against double precision
  *                         (ulp and relative bounds, special values) in
@@ -124,6 +127,14 @@ void orc_sample_uniform_triangle(const float u[2], float out[2]);
 #define ORC_FM_LOG 4
 #define ORC_FM_POW 5
 void orc_fmath_eval(int kind, const float *x, const float *y, size_t n, float *out);
+
+/* dSFMT-19937, the reference's Random (orc_dsfmt.c) */
+typedef struct orc_dsfmt orc_dsfmt;
+size_t orc_dsfmt_size(void);
+void orc_dsfmt_seed(orc_dsfmt *g, uint32_t seed);
+double orc_dsfmt_next(orc_dsfmt *g);
+uint32_t orc_dsfmt_uint32(orc_dsfmt *g);
+void orc_dsfmt_stream(uint32_t seed, size_t n, double *next, uint32_t *uints);
 
 /* ---------------------------------------------------------------------- */
 /* Path samplers (orc_render.c) */

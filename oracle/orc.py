@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORC_PATH = os.path.join(HERE, "liborc.so")
+# LMX_ORC_LIB names another build of the same sources (bench.py's
+# -march=native build for its CPU baseline)
+ORC_PATH = os.environ.get("LMX_ORC_LIB") or os.path.join(HERE, "liborc.so")
 REF_PATH = os.path.join(HERE, "_ref", "libref.so")
 
 TEST_TRIACCEL = 0
@@ -99,6 +101,21 @@ def fmath(kind, x, y=None):
     so.orc_fmath_eval.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
     so.orc_fmath_eval.restype = None
     so.orc_fmath_eval(kind, _p(x), _p(y), x.size, _p(out))
+    return out
+
+
+def dsfmt_stream(seed, n, uint32=False):
+    """n values of the reference's Random seeded with `seed`: Next() as
+    float64 in [0, 1), or NextUInt() words (orc_dsfmt.c)."""
+    so = lib()
+    so.orc_dsfmt_stream.argtypes = [ctypes.c_uint32, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p]
+    so.orc_dsfmt_stream.restype = None
+    if uint32:
+        out = np.zeros(n, np.uint32)
+        so.orc_dsfmt_stream(seed, n, None, _p(out))
+    else:
+        out = np.zeros(n, np.float64)
+        so.orc_dsfmt_stream(seed, n, _p(out), None)
     return out
 
 
