@@ -187,6 +187,94 @@ def pt_cornell_section(dev, cpu, threads):
     return out
 
 
+# Algorithmic bytes of one path segment of the wavefront path tracer beyond
+# its traversal: the shading kernel reads the queued ray (32 B), throughput
+# (16 B), path record (16 B) and hit (16 B), the triangle's vertex indices,
+# vertices and vertex normals (12 + 36 + 36 B), its material index and
+# material (4 + 64 B); every traced ray was written once to a queue (ray,
+# throughput, path record: 64 B) by the generation or the shading kernel.
+PT_SEG_STATE_BYTES = 80 + 152 + 64
+
+
+def pt_roofline(sc, rs, kw, n, dt, rays):
+    """B_seg roofline of the wavefront path tracer: per extension ray the
+    traversal's algorithmic bytes (48 B ray and hit + 128 B per node visit +
+    48 B per triangle test, the visits counted by the host walker over the
+    same BVH for every ray of a captured 1-spp frame) plus the segment's
+    queue and shading bytes, times the rays of the timed frame, over its
+    time, against 8 TB/s."""
+    import torch
+    from lmx import accel
+    npix = sc["camera"]["width"] * sc["camera"]["height"]
+    cap = 1 << 22
+    buf = torch.empty((cap, 8), dtype=torch.float32, device="cuda:0")
+    _, st, _ = rs.render_pt(npix, ray_out=buf, **kw)
+    k = min(cap, st["rays"])
+    captured = buf[:k].cpu().numpy()
+    a = accel.Accel(sc["verts"], sc["tris"])
+    nv, nt = a.count(captured)
+    a.close()
+    live = captured[:, 7] > 0.0                 # camera paths that end at once carry tmax < 0
+    b_ray = RAY_BYTES + (NODE_BYTES * nv + TRI_BYTES * nt) / k
+    b_seg = b_ray + PT_SEG_STATE_BYTES
+    achieved = b_seg * rays / dt / 1e9
+    return {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS, "achieved": round(achieved, 1),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "b_seg_bytes": round(b_seg, 1),
+            "b_ray_bytes": round(b_ray, 1), "state_bytes_per_segment": PT_SEG_STATE_BYTES,
+            "node_visits_per_ray": round(nv / k, 3), "tri_tests_per_ray": round(nt / k, 3),
+            "rays_counted": int(k), "live_fraction": round(float(live.mean()), 4),
+            "method": "visits counted by the host walker on every extension ray of a captured 1-spp frame "
+                      "(lmx_render_params.ray_out); segments = the timed frame's extension rays"}
+
+
+# Algorithmic bytes of the batched bidirectional sampler beyond traversal.
+# A subpath extension: the queued ray, throughput, path record and hit
+# (32 + 16 + 16 + 16 B), the hit triangle's shading data (152 B, as for the
+# path tracer), the new vertex record (64 B) and the next queue entry (64 B).
+# A connection: its four vertex records (4 x 64 B) and two materials
+# (2 x 64 B), the shadow-queue record it writes and the splat kernel reads
+# back (ray 32 B, contribution, aux and path records 3 x 16 B, read back
+# with the hit: 16 + 16 + 16 B) and its film splat (12 B).
+BD_EXT_STATE_BYTES = 80 + 152 + 64 + 64
+BD_CONN_STATE_BYTES = 4 * 64 + 2 * 64 + 32 + 48 + 48 + 12
+
+
+def bd_roofline(sc, rs, render, n, dt, label):
+    """B_seg roofline of a bidirectional render: every subpath extension ray
+    and every shadow ray of a captured random-raster render of 2^17 samples
+    is counted by the host walkers (closest hit and occlusion) over the same
+    BVH; per-sample ray counts and bytes scale to the timed frame."""
+    import numpy as np
+    import torch
+    from lmx import accel
+    m = 1 << 17
+    cap = 1 << 25
+    buf = torch.full((cap, 8), float("nan"), dtype=torch.float32, device="cuda:0")
+    _, st, _ = render(m, buf)
+    half = cap // 2
+    host = buf.cpu().numpy()
+    ext, sh = host[:half], host[half:]
+    ne = int((~np.isnan(ext[:, 0])).sum())               # captured records fill each half from its start
+    ns = int((~np.isnan(sh[:, 0])).sum())
+    ext, sh = ext[:ne], sh[:ns]
+    a = accel.Accel(sc["verts"], sc["tris"])
+    nve, nte = a.count(ext)
+    nvs, nts = a.count(sh, anyhit=True)
+    a.close()
+    b_ext = RAY_BYTES + (NODE_BYTES * nve + TRI_BYTES * nte) / max(ne, 1) + BD_EXT_STATE_BYTES
+    b_sh = RAY_BYTES + (NODE_BYTES * nvs + TRI_BYTES * nts) / max(ns, 1) + BD_CONN_STATE_BYTES
+    per_sample = (ne * b_ext + ns * b_sh) / m
+    achieved = per_sample * n / dt / 1e9
+    return {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS, "achieved": round(achieved, 1),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_sample": round(per_sample, 1),
+            "ext_rays_per_sample": round(ne / m, 3), "shadow_rays_per_sample": round(ns / m, 3),
+            "b_seg_ext_bytes": round(b_ext, 1), "b_seg_shadow_bytes": round(b_sh, 1),
+            "ext_node_visits_per_ray": round(nve / max(ne, 1), 3), "shadow_node_visits_per_ray": round(nvs / max(ns, 1), 3),
+            "captured_rays_match_stats": bool(ne + int((sh[:, 7] >= sh[:, 3]).sum()) == st["rays"]),
+            "method": f"{label}: every ray of a captured render of {m} random-raster samples counted by the host "
+                      "walkers; bytes per sample times the timed frame's samples"}
+
+
 def pt_section(dev, cpu, threads):
     """Wavefront renderer::pt on the sphere-grid scene (5,242,886 triangles,
     1024x1024, 64 spp, max 17 vertices = depth 16, roulette from depth 3):
@@ -223,6 +311,7 @@ def pt_section(dev, cpu, threads):
         "image_mean": round(float(film.mean()), 5),
         "cpu_baseline": None,
     }
+    out["roofline"] = pt_roofline(sc, rs, kw, n, dt, rays)
     # renderer::ptdirect / ptmis on the same frame: next-event estimation
     # adds one shadow ray per vertex
     for entry in ("ptdirect", "ptmis"):
@@ -282,7 +371,11 @@ def bdpt_section(dev, cpu, threads):
         times.append(time.perf_counter() - t0)
     dt = median(times)
     rays = st["rays"]
+    from lmx.render import RASTER_RANDOM
+    roof = bd_roofline(sc, rs, lambda m, buf: rs.render_vcm_bdpt(m, raster_mode=RASTER_RANDOM, ray_out=buf, **kw),
+                       n, dt, "renderer::vcm bdpt")
     out = {
+        "roofline": roof,
         "workload": "bdpt: 16x16 displaced icospheres (5,242,886 tris), 1024x1024, 16 spp, "
                     "subpaths of up to 10 vertices, all (s,t) strategies, power heuristic",
         "samples_per_s": round(n / dt, 1),
@@ -310,7 +403,10 @@ def bdpt_section(dev, cpu, threads):
         db = median(tb)
         out[key] = {"max_num_vertices": m, "msamples_per_s": round(n / db / 1e6, 2),
                     "mrays_per_s": round(sb["rays"] / db / 1e6, 1), "rays_per_sample": round(sb["rays"] / n, 4),
-                    "s_per_frame": round(db, 4), "batch": sb["batch"], "image_mean": round(float(fb.mean()), 5)}
+                    "s_per_frame": round(db, 4), "batch": sb["batch"], "image_mean": round(float(fb.mean()), 5),
+                    "roofline": bd_roofline(sc, rs, lambda k, buf: rs.render_bdpt(k, raster_mode=RASTER_RANDOM,
+                                                                                   ray_out=buf, **kb),
+                                            n, db, "renderer::bdpt")}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import orc

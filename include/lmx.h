@@ -109,6 +109,10 @@ int lmx_accel_intersect_host(const lmx_accel *accel, const lmx_ray *rays, uint32
  * algorithmic byte count. */
 int lmx_accel_count(const lmx_accel *accel, const lmx_ray *rays, uint32_t n,
                     uint64_t *node_visits, uint64_t *tri_tests, int32_t nthreads);
+/* The same counts for the occlusion query (lmx_accel_occluded): the walk
+ * stops after the node in which it found a triangle in range. */
+int lmx_accel_count_occluded(const lmx_accel *accel, const lmx_ray *rays, uint32_t n,
+                             uint64_t *node_visits, uint64_t *tri_tests, int32_t nthreads);
 
 /* ------------------------------------------------------------------------ */
 /* Path samplers: scene and renderers (Renderer::Render, renderer.h:81)       */
@@ -195,7 +199,26 @@ typedef struct lmx_render_params {
      * uniform_table or sample_out.  lmx_render_pt, _ptdirect, _ptmis,
      * _vcm_bdpt and _bdpt. */
     double render_time;
+    /* lmx_render_pt / _ptdirect / _ptmis: LMX_SHADE_QUEUE shades the traced
+     * paths in queue order; LMX_SHADE_BY_MATERIAL first bins them by the
+     * BSDF of the surface they hit (one pass over the hits), so each wave of
+     * the shading kernel runs one material's code.  Results are the same
+     * either way up to the order of the film's float additions. */
+    int32_t shade_order;
+    /* Measurement hook: when ray_out is a device array of ray_out_cap x 8
+     * floats, the first ray_out_cap extension rays traced (lmx_ray records,
+     * in launch order) are copied there (lmx_render_pt / _ptdirect / _ptmis),
+     * or the first ray_out_cap / 2 subpath extension rays to the first half
+     * and the first ray_out_cap / 2 shadow rays to the second
+     * (lmx_render_vcm_bdpt / _bdpt), so that the counting walkers
+     * (lmx_accel_count, lmx_accel_count_occluded) can price the render's
+     * traversal in algorithmic bytes.  NULL: off. */
+    uint32_t ray_out_cap;
+    float *ray_out;
 } lmx_render_params;
+
+#define LMX_SHADE_QUEUE 0
+#define LMX_SHADE_BY_MATERIAL 1
 
 typedef struct lmx_render_stats {
     uint64_t rays;               /* extension rays traced */
@@ -203,6 +226,8 @@ typedef struct lmx_render_stats {
     uint32_t queue_capacity;     /* capacity used */
     double ms;                   /* wall time of the render call */
     uint64_t samples;            /* camera samples taken (num_samples unless render_time > 0) */
+    uint64_t dropped;            /* lmx_render_vcm: merges dropped because their contribution was not
+                                    finite in FP32 (reported so that a test can require none) */
 } lmx_render_stats;
 
 /* Builds the accel (watertight, 8-wide) and uploads shading data to the

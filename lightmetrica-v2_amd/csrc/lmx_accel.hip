@@ -76,7 +76,7 @@ namespace lmx {
 
 // Returns false when a node group did not fit the stack (a BVH deeper than
 // WBVH_STACK, which the builder refuses); the hit is then a miss.
-template <int ISECT>
+template <int ISECT, bool ANYHIT>
 static bool walk_impl(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* nv, uint64_t* nt)
 {
     WRayPre r;
@@ -127,6 +127,8 @@ static bool walk_impl(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* n
                 best_t = t; best_id = id; h0 = a0; h1 = a1; hdet = dt;
             }
         }
+        // any hit: the kernel tests the rest of this node's triangles, then stops
+        if (ANYHIT && best_id != WBVH_MISS) break;
     }
     if (best_id == WBVH_MISS) {
         hit.t = 0.0f; hit.b0 = 0.0f; hit.b1 = 0.0f; hit.id = WBVH_MISS;
@@ -142,8 +144,14 @@ static bool walk_impl(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* n
 
 bool wbvh_walk(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* nv, uint64_t* nt)
 {
-    if (b.isect == WBVH_ISECT_WATERTIGHT) return walk_impl<WBVH_ISECT_WATERTIGHT>(b, ray, hit, nv, nt);
-    return walk_impl<WBVH_ISECT_TRIACCEL>(b, ray, hit, nv, nt);
+    if (b.isect == WBVH_ISECT_WATERTIGHT) return walk_impl<WBVH_ISECT_WATERTIGHT, false>(b, ray, hit, nv, nt);
+    return walk_impl<WBVH_ISECT_TRIACCEL, false>(b, ray, hit, nv, nt);
+}
+
+bool wbvh_walk_anyhit(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* nv, uint64_t* nt)
+{
+    if (b.isect == WBVH_ISECT_WATERTIGHT) return walk_impl<WBVH_ISECT_WATERTIGHT, true>(b, ray, hit, nv, nt);
+    return walk_impl<WBVH_ISECT_TRIACCEL, true>(b, ray, hit, nv, nt);
 }
 
 }  // namespace lmx
@@ -604,8 +612,25 @@ int lmx_accel_intersect_host(const lmx_accel* a, const lmx_ray* rays, uint32_t n
     return LMX_OK;
 }
 
+static int count_impl(const lmx_accel* a, const lmx_ray* rays, uint32_t n, uint64_t* node_visits,
+                      uint64_t* tri_tests, int32_t nthreads, bool anyhit);
+
 int lmx_accel_count(const lmx_accel* a, const lmx_ray* rays, uint32_t n, uint64_t* node_visits,
                     uint64_t* tri_tests, int32_t nthreads)
+{
+    return count_impl(a, rays, n, node_visits, tri_tests, nthreads, false);
+}
+
+int lmx_accel_count_occluded(const lmx_accel* a, const lmx_ray* rays, uint32_t n, uint64_t* node_visits,
+                             uint64_t* tri_tests, int32_t nthreads)
+{
+    return count_impl(a, rays, n, node_visits, tri_tests, nthreads, true);
+}
+
+}  // extern "C"
+
+static int count_impl(const lmx_accel* a, const lmx_ray* rays, uint32_t n, uint64_t* node_visits,
+                      uint64_t* tri_tests, int32_t nthreads, bool anyhit)
 {
     if (!a || !node_visits || !tri_tests) return set_error(LMX_EINVAL, "lmx_accel_count: NULL argument");
     *node_visits = 0; *tri_tests = 0;
@@ -615,8 +640,10 @@ int lmx_accel_count(const lmx_accel* a, const lmx_ray* rays, uint32_t n, uint64_
     std::vector<uint64_t> nv(nthreads, 0), nt(nthreads, 0);
     auto work = [&](int t, uint32_t b, uint32_t e) {
         WHit h;
-        for (uint32_t i = b; i < e; i++)
-            wbvh_walk(a->host, reinterpret_cast<const WRay*>(rays)[i], h, &nv[t], &nt[t]);
+        for (uint32_t i = b; i < e; i++) {
+            if (anyhit) wbvh_walk_anyhit(a->host, reinterpret_cast<const WRay*>(rays)[i], h, &nv[t], &nt[t]);
+            else wbvh_walk(a->host, reinterpret_cast<const WRay*>(rays)[i], h, &nv[t], &nt[t]);
+        }
     };
     std::vector<std::thread> th;
     for (int t = 0; t < nthreads; t++) {
@@ -628,4 +655,4 @@ int lmx_accel_count(const lmx_accel* a, const lmx_ray* rays, uint32_t n, uint64_
     return LMX_OK;
 }
 
-}  // extern "C"
+

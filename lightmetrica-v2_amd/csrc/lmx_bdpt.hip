@@ -117,7 +117,28 @@ struct BParams {
     uint32_t batch;            // samples per batch
     float* sample_out;         // (maxv+1)^2 x 8 floats per sample, or null
     float* film;
+    float4* ray_out = nullptr; // measurement hook: extension rays at [0, cap/2), shadow rays at [cap/2, cap)
+    uint32_t ray_out_cap = 0;
 };
+
+// Measurement hook: copies the n rays about to be traced to out[at, ...),
+// within [lo, hi) (the slot of BCtl::cap_n[kind] counts them).
+__global__ void k_bd_capture(BCtl* __restrict__ ctl, int kind, const float4* __restrict__ ray, float4* __restrict__ out,
+                             uint32_t lo, uint32_t hi)
+{
+    const uint32_t n = kind == 0 ? ctl->n_trace : ctl->n_shadow;
+    const uint32_t at = lo + ctl->cap_n[kind];
+    const uint32_t m = at >= hi ? 0u : min(n, hi - at);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+        out[2 * (size_t)(at + i)] = ray[2 * (size_t)i];
+        out[2 * (size_t)(at + i) + 1] = ray[2 * (size_t)i + 1];
+    }
+}
+
+__global__ void k_bd_capture_advance(BCtl* ctl, int kind, uint32_t room)
+{
+    if (threadIdx.x == 0) ctl->cap_n[kind] = min(room, ctl->cap_n[kind] + (kind == 0 ? ctl->n_trace : ctl->n_shadow));
+}
 
 struct VB {
     float4* v[2];              // eye, light
@@ -505,6 +526,8 @@ struct Junction {
     f3 Cs;
     float pLs;
     float aL, bL;      // the light side's ratios at vertices sv - 1, sv - 2
+    float aE, bE;      // the eye side's ratios at vertices t - 2, t - 3 (t >= 2)
+    float pLsp1;       // pdf of eye vertex t - 2 from the junction (t >= 3)
     float sum;
     bool ok;           // every pdf the sum divides by is nonzero
 };
@@ -554,11 +577,16 @@ RD Junction junction(const RScene& s, const BParams& p, float cm, const Vtx& vL,
     J.ok = true;
     if (vL.pf == 0.0f || (sv >= 2 && (vLp.pf == 0.0f || vL.U < 0.0f))) J.ok = false;
     if (t >= 2 && (vE.pf == 0.0f || (t >= 3 && (vEp.pf == 0.0f || vE.U < 0.0f)))) J.ok = false;
-    J.aL = J.bL = 0.0f;
+    J.aL = J.bL = J.aE = J.bE = 0.0f;
+    J.pLsp1 = pLsp1;
     J.sum = p.c_conn;
     if (J.ok) {
         J.aL = pEsm1 / vL.pf;
         J.bL = sv >= 2 ? pEsm2 / vLp.pf : 0.0f;
+        if (t >= 2) {
+            J.aE = pLs / vE.pf;
+            J.bE = t >= 3 ? pLsp1 / vEp.pf : 0.0f;
+        }
         // this strategy's own term: mu of vE (no merge at the pinhole)
         J.sum = mu_of(p.c_conn, cm, t >= 2, pLs);
         // light side: mu of light vertices sv - 1, sv - 2 (the endpoint is no merge)
@@ -566,10 +594,33 @@ RD Junction junction(const RScene& s, const BParams& p, float cm, const Vtx& vL,
                           mu_of(p.c_conn, cm, sv >= 3, sv >= 3 ? vLp.pf : 0.0f), vL.U, p.mis);
         // eye side: mu of eye vertices t - 2, t - 3 (the camera is no merge)
         if (t >= 2)
-            J.sum += chain_mu(pLs / vE.pf, t >= 3 ? pLsp1 / vEp.pf : 0.0f, mu_of(p.c_conn, cm, t >= 3, pLsp1),
+            J.sum += chain_mu(J.aE, J.bE, mu_of(p.c_conn, cm, t >= 3, pLsp1),
                               mu_of(p.c_conn, cm, t >= 4, prE3), vE.U, p.mis);
     }
     return J;
+}
+
+// The weight denominator of a merge with the connection terms off (mode
+// bdpm, c_conn = 0) divided by the merge's own term N (pi r^2 pLs)^2: every
+// mu = N (pi r^2 pf)^2 becomes (pf / pLs)^2 and the stored chains U are
+// divided by the same constant.  Unlike the plain sum, whose terms all scale
+// with the squared pdfs and can underflow together (to 0, giving 0/0 for the
+// contribution), this is >= 1; it overflows only where the merge's weight
+// truly vanishes (the contribution then goes to 0).
+RD float merge_den_bdpm(const Junction& J, float cm, const Vtx& vL, const Vtx& vLp, int sv, const Vtx& vE, int t,
+                        float prE3, int mis)
+{
+    const float inv = 1.0f / J.pLs;
+    auto mn = [&](bool mergeable, float pf) -> float {
+        if (!mergeable || pf == 0.0f) return 0.0f;
+        const float q = pf * inv;
+        return q * q;
+    };
+    auto un = [&](float U) -> float { return U == 0.0f ? 0.0f : (U / cm) * inv * inv; };
+    float d = 1.0f;                                          // the merge's own term
+    d += chain_mu(J.aL, J.bL, mn(sv >= 2, vL.pf), mn(sv >= 3, sv >= 3 ? vLp.pf : 0.0f), un(vL.U), mis);
+    if (t >= 2) d += chain_mu(J.aE, J.bE, mn(t >= 3, J.pLsp1), mn(t >= 4, prE3), un(vE.U), mis);
+    return d;
 }
 
 // Connection plan: one thread per sample.  Strategies s = 0 (the eye
@@ -788,7 +839,7 @@ __global__ void k_bd_init(BCtl* ctl)
     if (threadIdx.x == 0) {
         ctl->n_trace = 0; ctl->n_next = 0; ctl->n_shadow = 0; ctl->batch_first = 0;
         ctl->rays = 0; ctl->shadow_rays = 0;
-        ctl->pass_first = 0; ctl->radius = 0.0f; ctl->cell = 0.0f; ctl->c_merge = 0.0f; ctl->merges = 0;
+        ctl->pass_first = 0; ctl->radius = 0.0f; ctl->cell = 0.0f; ctl->c_merge = 0.0f; ctl->merges = 0; ctl->dropped = 0; ctl->cap_n[0] = 0; ctl->cap_n[1] = 0;
         ctl->n_mq = 0;
     }
 }
@@ -975,7 +1026,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const uint32_t nq = ctl->n_mq;
     const float r = ctl->radius, cell = ctl->cell, cm = ctl->c_merge;
     const float r2 = r * r, area = LMX_PI_F * r * r;
-    uint32_t nmerge = 0;
+    uint32_t nmerge = 0, ndrop = 0;
     for (uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x; qi < nq; qi += gridDim.x * blockDim.x) {
         const uint32_t e = vb.mq[qi];
         const uint32_t i = e >> 5, k = e & 31u;
@@ -1033,25 +1084,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 const Junction J = junction(s, p, cm, vL, vLp, j, vE, vEp, t, prE3);
                 if (!J.ok || J.pLs == 0.0f) continue;
                 if (J.Cs.x == 0.0f && J.Cs.y == 0.0f && J.Cs.z == 0.0f) continue;
-                const float f = J.pLs * area / J.sum;
-                const f3 C = mk3(J.Cs.x * f, J.Cs.y * f, J.Cs.z * f);
-                // Squared pdfs can leave float range: without the connection
-                // terms (c_conn = 0, mode bdpm) J.sum may underflow to 0 or
-                // overflow with the numerator, and 0/0, inf/inf or 0 x inf
-                // would put a NaN into the film.  Such a merge is dropped
-                // (seen in an 8-pass bdpm frame of the 1024^2 sphere grid).
-                if (!isfinite(C.x) || !isfinite(C.y) || !isfinite(C.z)) continue;
-                const float pm = area * J.pLs;
+                // C = Cs pLs pi r^2 / sum and w = (pi r^2 pLs)^2 / sum.  With
+                // the connection terms on, sum >= c_conn keeps both in range;
+                // in mode bdpm every term of sum scales with squared pdfs, so
+                // it is taken relative to the merge's own term instead.
+                f3 C;
+                float w;
+                if (p.c_conn != 0.0f) {
+                    const float f = J.pLs * area / J.sum;
+                    C = mk3(J.Cs.x * f, J.Cs.y * f, J.Cs.z * f);
+                    const float pm = area * J.pLs;
+                    w = pm * pm / J.sum;
+                } else {
+                    const float d = merge_den_bdpm(J, cm, vL, vLp, j, vE, t, prE3, p.mis);
+                    const float f = (area / cm) / (J.pLs * d);
+                    C = mk3(J.Cs.x * f, J.Cs.y * f, J.Cs.z * f);
+                    w = (area / cm) * area / d;
+                }
+                if (!isfinite(C.x) || !isfinite(C.y) || !isfinite(C.z)) {
+                    ndrop++;                                     // counted and reported, never silent
+                    continue;
+                }
                 Csum = add(Csum, C);
                 nmerge++;
-                record_merge(p, sample, j + 1, t, eras.x, eras.y, C, pm * pm / J.sum);
+                record_merge(p, sample, j + 1, t, eras.x, eras.y, C, w);
             }
         }
         if (!(Csum.x == 0.0f && Csum.y == 0.0f && Csum.z == 0.0f)) splat(p, vb.epix[i], Csum);
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) nmerge += __shfl_xor(nmerge, off);
+    for (int off = 32; off > 0; off >>= 1) {
+        nmerge += __shfl_xor(nmerge, off);
+        ndrop += __shfl_xor(ndrop, off);
+    }
     if ((threadIdx.x & 63u) == 0 && nmerge) atomicAdd(&ctl->merges, (unsigned long long)nmerge);
+    if ((threadIdx.x & 63u) == 0 && ndrop) atomicAdd(&ctl->dropped, (unsigned long long)ndrop);
 }
 
 // renderer::pm's eye side (renderer_pm.cpp:145-205): one thread per eye
@@ -1589,7 +1656,12 @@ int bd_enqueue_batch(lmx_scene* sc, const BParams& bp, hipStream_t st)
     const int cap_grid = sc->cus * 8;
     k_bd_gen<<<(B + 255) / 256, 256, 0, st>>>(sc->rs, bp, ctl, vb, q[0]);
     int a = 0;
+    const uint32_t half = bp.ray_out_cap / 2;
     for (int it = 0; it < bp.maxv - 1; it++) {
+        if (bp.ray_out) {
+            k_bd_capture<<<cap_grid, 256, 0, st>>>(ctl, 0, q[a].ray, bp.ray_out, 0u, half);
+            k_bd_capture_advance<<<1, 64, 0, st>>>(ctl, 0, half);
+        }
         int rc = launch_trace(sc->accel, reinterpret_cast<const WRay*>(q[a].ray), 0, &ctl->n_trace, 2 * B,
                               reinterpret_cast<WHit*>(sc->bd_hits.p), st, false);
         if (rc != LMX_OK) return rc;
@@ -1601,6 +1673,10 @@ int bd_enqueue_batch(lmx_scene* sc, const BParams& bp, hipStream_t st)
     k_bd_plan<<<(B + 255) / 256, 256, 0, st>>>(sc->rs, bp, ctl, vb, sq);
     k_bd_connect<<<cap_grid, 256, 0, st>>>(sc->rs, bp, ctl, vb, sq);
     const uint32_t nconn = B * bd_conn_per_sample((uint32_t)bp.maxv, bp.variant, bp.bounded != 0, bp.connect != 0);
+    if (bp.ray_out) {
+        k_bd_capture<<<cap_grid, 256, 0, st>>>(ctl, 1, sq.ray, bp.ray_out, half, bp.ray_out_cap);
+        k_bd_capture_advance<<<1, 64, 0, st>>>(ctl, 1, bp.ray_out_cap - half);
+    }
     int rc = launch_trace(sc->accel, reinterpret_cast<const WRay*>(sq.ray), 0, &ctl->n_shadow, nconn,
                           reinterpret_cast<WHit*>(sc->bd_shits.p), st, true);
     if (rc != LMX_OK) return rc;
@@ -1661,6 +1737,8 @@ int bd_autotune(lmx_scene* sc, BParams bp, uint64_t num_samples, float* film, hi
     const uint32_t conn = bd_conn_per_sample((uint32_t)bp.maxv, bp.variant, bp.bounded != 0);
     const size_t per = bd_bytes_per_sample((uint32_t)bp.maxv, conn);
     bp.sample_out = nullptr;
+    bp.ray_out = nullptr;
+    bp.ray_out_cap = 0;
     bp.table = nullptr;                        // probes draw from the generator
     bp.film = film;
     hipEvent_t e0, e1;
@@ -1763,6 +1841,8 @@ int render_bidir(lmx_scene* sc, const lmx_render_params* p, int variant, const c
     bp.wsub = wsub;
     bp.sample_out = p->sample_out;
     bp.film = film_rgb;
+    bp.ray_out = p->ray_out && p->ray_out_cap >= 2 ? reinterpret_cast<float4*>(p->ray_out) : nullptr;
+    bp.ray_out_cap = bp.ray_out ? p->ray_out_cap : 0u;
     int rc = LMX_OK;
     // batch size: the caller's, else the autotuned one (tuned once per scene,
     // variant and max_num_vertices, for renders of more than the smallest
@@ -2039,6 +2119,7 @@ int render_vcm(lmx_scene* sc, const lmx_render_params* p, const lmx_vcm_params* 
         stats->queue_capacity = B;
         stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         stats->samples = rc == LMX_OK ? p->num_samples * v->num_iteration_pass : 0;
+        stats->dropped = rc == LMX_OK ? sc->bd_hctl->dropped : 0;
     }
     return rc;
 }

@@ -147,15 +147,34 @@ __global__ void k_pack(const float* __restrict__ P, __bf16* __restrict__ out, fl
     out[t] = (__bf16)v;
 }
 
-// Sums the per-workgroup gradient partials, then one Adam step (bias-corrected)
-__global__ void k_adam(float* __restrict__ P, float* __restrict__ m, float* __restrict__ v,
-                       const float* __restrict__ partial, int nparts, float scale, float lr, float b1, float b2,
-                       float c1, float c2, float eps, float* __restrict__ grad_out)
+// Sums the per-workgroup gradient partials, then one Adam step
+// (bias-corrected).  A workgroup of 256 threads owns 64 parameters: its four
+// waves each sum every fourth partial (64 consecutive parameters per load,
+// four loads in flight per lane), and the first wave adds the four sums.
+constexpr int ADAM_PARAMS = 64;
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ P, float* __restrict__ m, float* __restrict__ v,
+                                              const float* __restrict__ partial, int nparts, float scale, float lr,
+                                              float b1, float b2, float c1, float c2, float eps,
+                                              float* __restrict__ grad_out)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= GP_N) return;
-    float g = 0.0f;
-    for (int k = 0; k < nparts; k++) g += partial[(size_t)k * GP_N + i];
+    __shared__ float red[4][ADAM_PARAMS];
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int i = blockIdx.x * ADAM_PARAMS + lane;
+    float g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, g3 = 0.0f;
+    if (i < GP_N) {
+        int k = q;
+        for (; k + 12 < nparts; k += 16) {
+            g0 += partial[(size_t)k * GP_N + i];
+            g1 += partial[(size_t)(k + 4) * GP_N + i];
+            g2 += partial[(size_t)(k + 8) * GP_N + i];
+            g3 += partial[(size_t)(k + 12) * GP_N + i];
+        }
+        for (; k < nparts; k += 4) g0 += partial[(size_t)k * GP_N + i];
+    }
+    red[q][lane] = (g0 + g1) + (g2 + g3);
+    __syncthreads();
+    if (q != 0 || i >= GP_N) return;
+    float g = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
     g *= scale;
     if (grad_out) grad_out[i] = g;
     if (lr == 0.0f) return;
@@ -1063,7 +1082,7 @@ int guide_train_step(lmx_guide* g, const float4* recs, const uint32_t* idx, uint
         c1 = 1.0f - powf(g->opts.beta1, (float)g->step);
         c2 = 1.0f - powf(g->opts.beta2, (float)g->step);
     }
-    k_adam<<<(GP_N + 255) / 256, 256, 0, st>>>(g->P.p, g->m.p, g->v.p, g->partial.p, nwg, 1.0f, lr, g->opts.beta1,
+    k_adam<<<(GP_N + ADAM_PARAMS - 1) / ADAM_PARAMS, 256, 0, st>>>(g->P.p, g->m.p, g->v.p, g->partial.p, nwg, 1.0f, lr, g->opts.beta1,
                                                g->opts.beta2, c1, c2, g->opts.eps, grad_out);
     if ((e = hipGetLastError()) != hipSuccess) return hip_error(e, "guide train launch");
     return adam ? guide_pack(g, st) : LMX_OK;

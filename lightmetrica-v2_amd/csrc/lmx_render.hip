@@ -118,14 +118,74 @@ RD void nee_append(const RParams& p, Ctl* ctl, uint32_t* lds, const NeeOut& o, u
     }
 }
 
+// Material-binned shading: the traced paths grouped by the class of the
+// surface they hit, so that a wave of the shading kernel runs one BSDF's
+// code.  One thread per queued path; per wave a ballot per class ranks the
+// lanes, per workgroup one atomic per class reserves the class's range of
+// perm (class c owns perm[c cap, (c + 1) cap)).
+RD uint32_t shade_class(const RScene& s, uint32_t id, bool dead)
+{
+    if (dead || id == WBVH_MISS) return 0;
+    const int32_t t = s.mats[s.tri_mat[id]].type;
+    return t == LMX_MAT_DIFFUSE ? 1u : (t == LMX_MAT_PHONG ? 2u : (t == LMX_MAT_GGX ? 3u : 0u));
+}
+
+__global__ __launch_bounds__(256) void k_bin(RScene s, Ctl* __restrict__ ctl, const uint4* __restrict__ meta,
+                                             const float4* __restrict__ hits, uint32_t* __restrict__ perm,
+                                             uint32_t cap)
+{
+    __shared__ uint32_t wcount[SHADE_CLASSES][4];
+    __shared__ uint32_t wbase[SHADE_CLASSES][4];
+    const uint32_t n = ctl->n_trace;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t c = SHADE_CLASSES;                  // no path
+        if (i < n) c = shade_class(s, __float_as_uint(hits[i].w), (meta[i].w & META_DEAD) != 0);
+        uint32_t rank = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < SHADE_CLASSES; k++) {
+            const uint64_t m = __ballot(c == k);
+            if (lane == 0) wcount[k][wave] = (uint32_t)__popcll(m);
+            if (c == k) rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        }
+        __syncthreads();
+        if (threadIdx.x < SHADE_CLASSES) {
+            const uint32_t k = threadIdx.x;
+            const uint32_t tot = wcount[k][0] + wcount[k][1] + wcount[k][2] + wcount[k][3];
+            uint32_t b = tot ? atomicAdd(&ctl->bin[k], tot) : 0u;
+            for (int w = 0; w < 4; w++) { wbase[k][w] = b; b += wcount[k][w]; }
+        }
+        __syncthreads();
+        if (c < SHADE_CLASSES) perm[(size_t)c * cap + wbase[c][wave] + rank] = i;
+        __syncthreads();
+    }
+}
+
+__global__ void k_bin_clear(Ctl* ctl)
+{
+    if (threadIdx.x < SHADE_CLASSES) ctl->bin[threadIdx.x] = 0;
+}
+
 __global__ __launch_bounds__(256) void k_shade(RScene s, RParams p, Ctl* __restrict__ ctl, Queue in,
                                                const float4* __restrict__ hits, Queue out)
 {
     __shared__ uint32_t lds[5];
     const uint32_t n = ctl->n_trace;
     const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t bend[SHADE_CLASSES];                    // class ranges in the binned order
+    if (p.perm) {
+        uint32_t acc = 0;
+#pragma unroll
+        for (int k = 0; k < SHADE_CLASSES; k++) { acc += ctl->bin[k]; bend[k] = acc; }
+    }
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
-        const uint32_t i = base + threadIdx.x;
+        uint32_t i = base + threadIdx.x;
+        if (p.perm && i < n) {
+            uint32_t k = 0, start = 0;
+            while (k + 1 < SHADE_CLASSES && i >= bend[k]) { start = bend[k]; k++; }
+            i = p.perm[(size_t)k * p.perm_cap + (i - start)];
+        }
         bool cont = false;
         float4 o_ray0, o_ray1, o_thr;
         uint4 o_meta;
@@ -322,8 +382,26 @@ __global__ void k_init_ctl(Ctl* ctl)
 {
     if (threadIdx.x == 0) {
         ctl->n_trace = 0; ctl->n_next = 0; ctl->issued = 0; ctl->iterations = 0; ctl->rays = 0;
-        ctl->n_shadow = 0; ctl->shadow_rays = 0; ctl->stop = 0;
+        ctl->n_shadow = 0; ctl->shadow_rays = 0; ctl->stop = 0; ctl->ray_out_n = 0;
     }
+}
+
+// Measurement hook: copies the queue about to be traced (the first
+// ray_out_cap rays of the render, in launch order) out, then advances.
+__global__ void k_capture(const Ctl* __restrict__ ctl, const float4* __restrict__ ray, float4* __restrict__ out,
+                          uint32_t cap)
+{
+    const uint32_t n = ctl->n_trace, at = ctl->ray_out_n;
+    const uint32_t m = at >= cap ? 0u : min(n, cap - at);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+        out[2 * (size_t)(at + i)] = ray[2 * (size_t)i];
+        out[2 * (size_t)(at + i) + 1] = ray[2 * (size_t)i + 1];
+    }
+}
+
+__global__ void k_capture_advance(Ctl* ctl, uint32_t cap)
+{
+    if (threadIdx.x == 0) ctl->ray_out_n = min(cap, ctl->ray_out_n + ctl->n_trace);
 }
 
 // Time mode: camera samples stop starting from the next iteration on.
@@ -391,6 +469,11 @@ int enqueue_iteration(lmx_scene* sc, const RParams& rp, int a, int b, hipStream_
 {
     Queue qa{sc->ray[a].p, sc->thr[a].p, sc->meta[a].p};
     Queue qb{sc->ray[b].p, sc->thr[b].p, sc->meta[b].p};
+    if (rp.ray_out) {
+        k_capture<<<std::max(1, std::min((int)((sc->cap + 255) / 256), sc->cus * 4)), 256, 0, st>>>(
+            sc->ctl.p, qa.ray, rp.ray_out, rp.ray_out_cap);
+        k_capture_advance<<<1, 64, 0, st>>>(sc->ctl.p, rp.ray_out_cap);
+    }
     int rc = launch_trace(sc->accel, reinterpret_cast<const WRay*>(qa.ray), 0, &sc->ctl.p->n_trace, sc->cap,
                           reinterpret_cast<WHit*>(sc->hits.p), st);
     if (rc != LMX_OK) return rc;
@@ -398,6 +481,10 @@ int enqueue_iteration(lmx_scene* sc, const RParams& rp, int a, int b, hipStream_
     if (rp.guide) {
         if ((rc = launch_guided_shade(sc, rp, qa, qb, st)) != LMX_OK) return rc;
     } else {
+        if (rp.perm) {
+            k_bin_clear<<<1, 64, 0, st>>>(sc->ctl.p);
+            k_bin<<<grid, 256, 0, st>>>(sc->rs, sc->ctl.p, qa.meta, sc->hits.p, sc->perm.p, sc->cap);
+        }
         k_shade<<<grid, 256, 0, st>>>(sc->rs, rp, sc->ctl.p, qa, sc->hits.p, qb);
     }
     k_gen<<<grid, 256, 0, st>>>(sc->rs, rp, sc->ctl.p, sc->cap, qb);
@@ -415,6 +502,14 @@ int wavefront_render(lmx_scene* sc, const RParams& rp_in, uint32_t cap, hipStrea
     int rc = ensure_queues(sc, cap);
     if (rc != LMX_OK) return rc;
     RParams rp = rp_in;
+    if (rp.perm_cap) {                       // material-binned shading requested
+        if (sc->perm.n < (size_t)SHADE_CLASSES * cap) {
+            const hipError_t ea = sc->perm.alloc((size_t)SHADE_CLASSES * cap);
+            if (ea != hipSuccess) return hip_error(ea, "shading bins");
+        }
+        rp.perm = sc->perm.p;
+        rp.perm_cap = cap;
+    }
     if (rp.nee != NEE_NONE) {
         if ((rc = ensure_nee(sc, cap)) != LMX_OK) return rc;
         rp.nq = NeeQ{sc->nee_ray.p, sc->nee_c.p, sc->nee_meta.p, sc->nee_ry.p, sc->nee_hits.p};
@@ -488,6 +583,8 @@ int autotune_capacity(lmx_scene* sc, const RParams& rp, hipStream_t st)
     static const uint32_t cands[] = {1u << 20, 1u << 21, 1u << 22, 1u << 23};
     RParams probe = rp;
     probe.sample_out = nullptr;
+    probe.ray_out = nullptr;
+    probe.ray_out_cap = 0;
     probe.total = (uint32_t)std::min<uint64_t>(rp.total, 1ull << 24);
     uint32_t best = cands[1];
     double best_rate = -1.0;
@@ -538,6 +635,8 @@ static int render_unidir(lmx_scene* sc, const lmx_render_params* p, int nee, con
     const bool timed = p->render_time > 0.0;
     if (timed && (p->uniform_table || p->sample_out))
         return set_error(LMX_EINVAL, std::string(name) + ": render_time takes neither a uniform table nor sample_out");
+    if (p->shade_order != LMX_SHADE_QUEUE && p->shade_order != LMX_SHADE_BY_MATERIAL)
+        return set_error(LMX_EINVAL, std::string(name) + ": unknown shade_order");
     hipStream_t st = (hipStream_t)stream;
     hipStream_t own = nullptr;
     hipError_t e;
@@ -561,6 +660,10 @@ static int render_unidir(lmx_scene* sc, const lmx_render_params* p, int nee, con
     rp.nee = nee;
     rp.seg_stride = stride;
     rp.so_stride = nee == NEE_NONE ? 8u : 16u;
+    rp.perm = nullptr;
+    rp.perm_cap = p->shade_order == LMX_SHADE_BY_MATERIAL ? 1u : 0u;   // set to the capacity when the queues exist
+    rp.ray_out = p->ray_out && p->ray_out_cap ? reinterpret_cast<float4*>(p->ray_out) : nullptr;
+    rp.ray_out_cap = rp.ray_out ? p->ray_out_cap : 0u;
     const int zgrid = std::max(1, std::min((int)((3 * npix + 255) / 256), sc->cus * 8));
     int rc = LMX_OK;
     if (p->queue_capacity == 0 && sc->tuned_cap == 0) rc = autotune_capacity(sc, rp, st);

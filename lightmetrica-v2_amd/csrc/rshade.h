@@ -58,7 +58,13 @@ struct Ctl {
     uint32_t n_shadow;         // next-event shadow rays of the iteration
     uint32_t stop;             // time mode: no new camera samples once set
     unsigned long long shadow_rays;
+    uint32_t bin[4];           // material-binned shading: paths per class this iteration
+    uint32_t ray_out_n;        // rays copied to RParams::ray_out so far
 };
+
+// Shading classes of material-binned shading: paths that end at their hit
+// (a miss, an emitter without a BSDF) and the three BSDFs.
+#define SHADE_CLASSES 4
 
 struct Queue {
     float4* ray;               // 2 per path: (o, tmin), (d, tmax)
@@ -100,6 +106,10 @@ struct RParams {
     uint32_t seg_stride;       // uniform slots per vertex (4; 8 with next-event estimation)
     uint32_t so_stride;        // floats per sample_out record (8; 16 with next-event estimation)
     NeeQ nq;
+    const uint32_t* perm = nullptr;   // material-binned shading: class c's paths at perm[c cap ...], or null
+    uint32_t perm_cap = 0;
+    float4* ray_out = nullptr;        // measurement hook: the first ray_out_cap rays traced
+    uint32_t ray_out_cap = 0;
 };
 
 // Russian-roulette continuation probability of a path with throughput thr
@@ -495,6 +505,9 @@ struct BCtl {
     float c_merge;             // vertex merging: N (pi radius^2)^2
     unsigned long long merges; // vertex merging: merged paths evaluated
     uint32_t n_mq;             // vertex merging: eye vertices queued for merging in the batch
+    uint32_t pad;
+    unsigned long long dropped;   // vertex merging: merges whose contribution was not finite (reported)
+    uint32_t cap_n[2];         // measurement hook: extension and shadow rays copied out so far
 };
 
 // The scene object behind the C ABI: accel, shading data on the device and
@@ -521,6 +534,7 @@ struct lmx_scene {
     DevBuf<uint4> nee_meta;
     DevBuf<float> nee_ry;
     uint32_t nee_cap = 0;
+    DevBuf<uint32_t> perm;              // material-binned shading, SHADE_CLASSES x cap
     DevBuf<Ctl> ctl;
     Ctl* h_ctl = nullptr;      // pinned
     uint32_t tuned_cap = 0;

@@ -28,5 +28,9 @@ WTri make_tri(const float* A, const float* B, const float* C, uint32_t id, int i
 // the WBVH_STACK-entry stack, which a tree the builder accepts never needs.
 bool wbvh_walk(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* nodes_visited,
                uint64_t* tris_tested);
+// The any-hit walk of the occlusion kernel: stops after the node in which it
+// found a triangle in range (whose other triangles it still tests).
+bool wbvh_walk_anyhit(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* nodes_visited,
+                      uint64_t* tris_tested);
 
 }  // namespace lmx

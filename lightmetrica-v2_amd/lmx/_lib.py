@@ -64,12 +64,14 @@ class lmx_render_params(ctypes.Structure):
                 ("raster_mode", ctypes.c_int32), ("queue_capacity", ctypes.c_uint32),
                 ("uniform_table", ctypes.c_void_p), ("table_width", ctypes.c_uint32),
                 ("sample_out", ctypes.c_void_p), ("rr_mode", ctypes.c_int32), ("mis", ctypes.c_int32),
-                ("render_time", ctypes.c_double)]
+                ("render_time", ctypes.c_double), ("shade_order", ctypes.c_int32),
+                ("ray_out_cap", ctypes.c_uint32), ("ray_out", ctypes.c_void_p)]
 
 
 class lmx_render_stats(ctypes.Structure):
     _fields_ = [("rays", ctypes.c_uint64), ("iterations", ctypes.c_uint32),
-                ("queue_capacity", ctypes.c_uint32), ("ms", ctypes.c_double), ("samples", ctypes.c_uint64)]
+                ("queue_capacity", ctypes.c_uint32), ("ms", ctypes.c_double), ("samples", ctypes.c_uint64),
+                ("dropped", ctypes.c_uint64)]
 
 
 class lmx_vcm_params(ctypes.Structure):
@@ -169,6 +171,9 @@ SIGNATURES = {
     "lmx_accel_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                        ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                        ctypes.c_int32]),
+    "lmx_accel_count_occluded": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                                ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                                ctypes.c_int32]),
 }
 
 

@@ -112,12 +112,14 @@ class Accel:
         check(lib.lmx_accel_intersect_host(self._h, _c(rays), rays.shape[0], _c(hits), nthreads))
         return hits
 
-    def count(self, rays, nthreads=0):
-        """(node visits, triangle tests) the traversal kernel performs."""
+    def count(self, rays, nthreads=0, anyhit=False):
+        """(node visits, triangle tests) the traversal kernel performs (the
+        occlusion kernel with anyhit=True)."""
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
         nv = ctypes.c_uint64()
         nt = ctypes.c_uint64()
-        check(lib.lmx_accel_count(self._h, _c(rays), rays.shape[0], ctypes.byref(nv), ctypes.byref(nt), nthreads))
+        fn = lib.lmx_accel_count_occluded if anyhit else lib.lmx_accel_count
+        check(fn(self._h, _c(rays), rays.shape[0], ctypes.byref(nv), ctypes.byref(nt), nthreads))
         return nv.value, nt.value
 
 

@@ -22,6 +22,8 @@ RR_FIXED = 0
 RR_THROUGHPUT = 1
 MIS_POWER = 0
 MIS_SIMPLE = 1
+SHADE_QUEUE = 0
+SHADE_BY_MATERIAL = 1
 # renderer::vcm modes with merging (renderer_vcm.cpp:563-570)
 VCM_MODE_VCM = 0
 VCM_MODE_BDPM = 2
@@ -123,25 +125,26 @@ class Scene:
 
     def render_vcm_bdpt(self, num_samples, max_num_vertices=10, min_num_vertices=0, seed=42,
                         raster_mode=RASTER_STRATIFIED, batch=0, table=None, sample_out=False, stream=None,
-                        render_time=0.0):
+                        render_time=0.0, ray_out=None):
         """renderer::vcm in bdpt mode.  Returns (film (H, W, 3), stats,
         per-sample records (n, (m+1)^2, 8) or None).  render_time > 0: the
         scheduler's time mode (num_samples is then ignored; stats["samples"]
         says how many were taken)."""
         return self._bidir("lmx_render_vcm_bdpt", (max_num_vertices + 1) ** 2, num_samples, max_num_vertices,
                            min_num_vertices, seed, raster_mode, batch, table, sample_out, stream, MIS_POWER,
-                           render_time)
+                           render_time, ray_out)
 
     def render_bdpt(self, num_samples, max_num_vertices=-1, min_num_vertices=0, seed=42,
                     raster_mode=RASTER_STRATIFIED, batch=0, table=None, sample_out=False, stream=None,
-                    mis=MIS_POWER, render_time=0.0):
+                    mis=MIS_POWER, render_time=0.0, ray_out=None):
         """renderer::bdpt (roulette subpaths, direct light vertices, power or
         simple weights).  Returns (film (H, W, 3), stats, per-sample records
         (n, 2 (M+1)^2, 8) at 2 (t (M+1) + s) + direct, or None); M =
         max_num_vertices, 32 when unbounded (-1)."""
         m = 32 if max_num_vertices == -1 else max_num_vertices
         return self._bidir("lmx_render_bdpt", 2 * (m + 1) ** 2, num_samples, max_num_vertices,
-                           min_num_vertices, seed, raster_mode, batch, table, sample_out, stream, mis, render_time)
+                           min_num_vertices, seed, raster_mode, batch, table, sample_out, stream, mis, render_time,
+                           ray_out)
 
     def render_vcm(self, num_samples, mode=VCM_MODE_VCM, num_iteration_pass=100, num_photon_trace_samples=10000,
                    initial_radius=0.1, alpha=0.7, max_num_vertices=10, min_num_vertices=0, seed=42,
@@ -181,7 +184,8 @@ class Scene:
             stream.wait_stream(torch.cuda.current_stream(dev))
         check(lib.lmx_render_vcm(self._h, ctypes.byref(p), ctypes.byref(v), ctypes.c_void_p(film.data_ptr()),
                                  ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st)))
-        stats = {"rays": st.rays, "passes": st.iterations, "batch": st.queue_capacity, "ms": st.ms}
+        stats = {"rays": st.rays, "passes": st.iterations, "batch": st.queue_capacity, "ms": st.ms,
+                 "dropped": st.dropped}
         return film, stats, so
 
     def render_pm(self, num_samples, num_photon_trace_samples=100000, finalgather=1, radius=0.01,
@@ -326,13 +330,18 @@ class Scene:
         return film, {"rays": st.rays, "ms": st.ms}
 
     def _bidir(self, entry, nrec, num_samples, max_num_vertices, min_num_vertices, seed, raster_mode, batch,
-               table, sample_out, stream, mis, render_time=0.0):
+               table, sample_out, stream, mis, render_time=0.0, ray_out=None):
         import torch
         dev = torch.device("cuda")
         film = torch.empty((self.height, self.width, 3), dtype=torch.float32, device=dev)
         p = self._params(num_samples, max_num_vertices, min_num_vertices, 1, 0.5, seed, raster_mode, batch)
         p.mis = mis
         p.render_time = render_time
+        if ray_out is not None:
+            # torch float32 (k, 8): subpath rays in the first half, shadow rays in the second
+            assert ray_out.is_cuda and ray_out.dtype == torch.float32 and ray_out.is_contiguous()
+            p.ray_out = ctypes.c_void_p(ray_out.data_ptr())
+            p.ray_out_cap = ray_out.shape[0]
         so = None
         if table is not None:
             assert table.is_cuda and table.dtype == torch.float32 and table.is_contiguous()
@@ -363,7 +372,8 @@ class Scene:
 
     def render_pt(self, num_samples, max_num_vertices=9, min_num_vertices=0, rr_depth=1, rr_prob=0.5,
                   seed=42, raster_mode=RASTER_STRATIFIED, queue_capacity=0, table=None, sample_out=False,
-                  stream=None, rr_mode=RR_FIXED, render_time=0.0, _entry="lmx_render_pt"):
+                  stream=None, rr_mode=RR_FIXED, render_time=0.0, shade_order=0, ray_out=None,
+                  _entry="lmx_render_pt"):
         """Returns (film as a torch tensor (H, W, 3) on the GPU, stats dict,
         per-sample records (n, 8) or None).  `table`: torch float32 tensor
         (num_samples, width) on the GPU, or None for Philox.  render_time > 0:
@@ -384,6 +394,13 @@ class Scene:
         p.queue_capacity = queue_capacity
         p.rr_mode = rr_mode
         p.render_time = render_time
+        p.shade_order = shade_order
+        if ray_out is not None:
+            # torch float32 (k, 8) on the GPU: the first k extension rays traced
+            assert ray_out.is_cuda and ray_out.dtype == torch.float32 and ray_out.is_contiguous()
+            assert ray_out.dim() == 2 and ray_out.shape[1] == 8
+            p.ray_out = ctypes.c_void_p(ray_out.data_ptr())
+            p.ray_out_cap = ray_out.shape[0]
         if table is not None:
             assert table.is_cuda and table.dtype == torch.float32 and table.is_contiguous()
             assert table.shape[0] == num_samples

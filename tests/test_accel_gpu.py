@@ -5,8 +5,9 @@ brute force under the same triangle test and the canonical tie rule
 (smallest id among equal t), for both the reference-compatible TriAccel test
 and the watertight test.  At the benchmark's full size (1,310,720 triangles,
 2048^2 coherent + 2048^2 incoherent rays) the kernel is compared bit-exactly
-with the host walker over the same BVH on every ray, and with the oracle's
-brute force on a seeded sample of rays.
+with the host walker over the same BVH on every ray and with the oracle's
+brute force on a seeded sample of rays (watertight mode), and in TriAccel
+mode with the oracle's restatement of the reference's QBVH on every ray.
 """
 import numpy as np
 import pytest
@@ -71,6 +72,40 @@ def test_full_size_traversal_benchmark(gpu):
     ids = accel.hit_ids(hg)
     assert (ids[: coh.shape[0]] != accel.MISS).mean() > 0.2
     assert (ids[coh.shape[0]:] != accel.MISS).mean() > 0.3
+
+
+def test_full_size_triaccel_vs_reference_qbvh_every_ray(gpu):
+    """The benchmark's full size in the reference's TriAccel mode
+    (triaccel.h:93-151) against the oracle's restatement of the reference's
+    own accel, the 4-wide QBVH (accel_qbvh.cpp:392-481, a later equal-t hit
+    replaces an earlier one), on all 8,388,608 rays: the same triangle on
+    every ray except exact t ties (two triangles hit at the same t, where the
+    kernel keeps the smallest id and the QBVH its traversal order), and where
+    the triangle agrees t and barycentrics bit for bit (the same FP32
+    operations in the same order).  Ties are checked with the FP64 solve:
+    both triangles are hit at that t."""
+    from test_oracle import _fp64_solve
+    v, f = scenes.traversal_scene()
+    coh, inc = scenes.traversal_rays(v)
+    rays = np.concatenate([coh, inc])
+    a = accel.Accel(v, f, isect=accel.ISECT_TRIACCEL)
+    hg = _gpu_hits(a, rays, gpu)
+    ho = orc.QBVH(v, f).intersect(rays, nthreads=16)
+    ig, io = accel.hit_ids(hg), orc.hit_ids(ho)
+    same = ig == io
+    assert (ig != accel.MISS).mean() > 0.5
+    np.testing.assert_array_equal(hg[same].view(np.uint32), ho[same].view(np.uint32))
+    bad = np.nonzero(~same)[0]
+    print(f"TriAccel mode, {rays.shape[0]} rays: {len(bad)} with another triangle")
+    assert len(bad) <= 1e-5 * rays.shape[0]
+    for k in bad:
+        assert ig[k] != accel.MISS and io[k] != accel.MISS, (k, ig[k], io[k])
+        assert hg[k, 0] == ho[k, 0], (k, hg[k, 0], ho[k, 0])          # an exact t tie
+        for tri in (ig[k], io[k]):
+            t, b0, b1 = _fp64_solve(v, f, rays[k:k + 1], np.array([tri]))
+            assert abs(t[0] - hg[k, 0]) <= 1e-5 * abs(t[0])
+            assert min(b0[0], b1[0], 1.0 - b0[0] - b1[0]) > -1e-5
+    assert a.info()["stack_overflow"] == 0
 
 
 @pytest.mark.parametrize("simple2", [False, True])

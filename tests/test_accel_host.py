@@ -99,6 +99,28 @@ def test_counting_walker_matches_host_walk_cost():
     assert (nva + nvb, nta + ntb) == (nv1, nt1)
 
 
+def test_occlusion_counting_walker():
+    """The any-hit count (lmx_accel_count_occluded): per ray never more node
+    visits than the closest-hit walk, the same for rays that hit nothing,
+    and exactly one visit more than the nodes opened before the first hit
+    is impossible to undercut: at least the root per ray."""
+    v, f = mesh("sphere6")
+    rays = rays_for(v, n=3000)
+    a = accel.Accel(v, f)
+    hits = a.intersect_host(rays)
+    miss = accel.hit_ids(hits) == accel.MISS
+    assert 0 < miss.sum() < rays.shape[0]
+    nvc, ntc = a.count(rays)
+    nva, nta = a.count(rays, anyhit=True)
+    assert nva <= nvc and nta <= ntc and nva >= rays.shape[0]
+    assert a.count(rays[miss]) == a.count(rays[miss], anyhit=True)
+    # per ray: the any-hit walk stops early only on rays that hit
+    for i in np.nonzero(~miss)[0][:200]:
+        c1, _ = a.count(rays[i:i + 1])
+        c2, _ = a.count(rays[i:i + 1], anyhit=True)
+        assert 1 <= c2 <= c1
+
+
 def decode_boxes(node):
     o = node["o"].astype(np.float64)
     sc = np.ldexp(1.0, node["e"].astype(np.int64) - 127)

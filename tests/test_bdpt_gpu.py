@@ -38,20 +38,23 @@ def _connections(g, c, floor=1e-9):
     return one, int(both.sum()), rel, dw
 
 
-def _check(g, c, tight, loose, edge_frac=5e-4, min_both=5000):
+def _check(g, c, worst=5e-4, edge_frac=5e-4, min_both=5000):
     """Same strategies contribute, except for shadow rays that graze an edge
     (a visibility decided by the last ulp of a vertex position, at most
     `edge_frac` of the connections); contributions and MIS weights agree
-    within 1e-4 for a `tight` fraction of the common connections and within
-    2e-3 for a `loose` fraction.  The tails are long specular chains, where
-    FP32 position differences of a few ulps are raised to the Phong
-    exponent."""
+    within 1e-4 for at least 99.99 % of the common connections and within
+    `worst` for all of them.  The vertices, BSDFs and contributions are
+    computed with the same FP32 operations on both sides (shared sine,
+    cosine and power); what differs is the MIS weight, an O(1) ratio chain in
+    FP32 on the GPU against the oracle's re-evaluation of every strategy's
+    path pdf, whose rounding differences grow along long glossy chains."""
     one, both, rel, dw = _connections(g, c)
     assert both > min_both
     assert one <= edge_frac * both, f"{one} of {both} connections contribute on one side only"
     err = np.maximum(rel, dw)
-    assert (err <= 1e-4).mean() >= tight, f"{(err > 1e-4).sum()} of {both} beyond 1e-4"
-    assert (err <= 2e-3).mean() >= loose, f"{(err > 2e-3).sum()} of {both} beyond 2e-3"
+    print(f"connections {both}: 99.99 % quantile {np.quantile(err, 0.9999):.2e}, worst {err.max():.2e}")
+    assert (err <= 1e-4).mean() >= 0.9999, f"{(err > 1e-4).sum()} of {both} beyond 1e-4"
+    assert err.max() <= worst, f"worst error {err.max():.3e}"
 
 
 @pytest.mark.parametrize("raster_mode", [render.RASTER_STRATIFIED, render.RASTER_RANDOM])
@@ -66,10 +69,7 @@ def test_bdpt_per_strategy_parity_uniform_table(gpu, raster_mode, phong):
                                          table=torch.from_numpy(tab).to(gpu), sample_out=True, batch=4096)
     g = so.cpu().numpy()
     c = orc.Scene(sc).bdpt_samples(0, n, max_num_vertices=m, raster_mode=raster_mode, table=tab)
-    if phong is None:
-        _check(g, c, tight=0.999, loose=0.9999)
-    else:
-        _check(g, c, tight=0.99, loose=0.999)
+    _check(g, c)
     # raster positions of light-tracing (t = 1) contributions
     t1 = (c[:, m + 1:2 * (m + 1), 5] > 0) & (g[:, m + 1:2 * (m + 1), 5] > 0)
     np.testing.assert_allclose(g[:, m + 1:2 * (m + 1), 0:2][t1], c[:, m + 1:2 * (m + 1), 0:2][t1], atol=1e-4)
@@ -80,7 +80,7 @@ def test_bdpt_per_strategy_parity_philox(gpu):
     n, m = 48 * 48 * 4, 10
     _, _, so = render.Scene(sc).render_vcm_bdpt(n, max_num_vertices=m, seed=7, sample_out=True, batch=3000)
     c = orc.Scene(sc).bdpt_samples(0, n, max_num_vertices=m, seed=7)
-    _check(so.cpu().numpy(), c, tight=0.999, loose=0.9999, min_both=1000)
+    _check(so.cpu().numpy(), c, min_both=1000)
 
 
 @pytest.mark.parametrize("maxv", [3, 5])
@@ -129,4 +129,4 @@ def test_bdpt_per_strategy_parity_ggx(gpu):
     _, _, so = render.Scene(sc).render_vcm_bdpt(n, max_num_vertices=m, table=torch.from_numpy(tab).to(gpu),
                                                 sample_out=True, batch=4096)
     c = orc.Scene(sc).bdpt_samples(0, n, max_num_vertices=m, table=tab)
-    _check(so.cpu().numpy(), c, tight=0.99, loose=0.999)
+    _check(so.cpu().numpy(), c)

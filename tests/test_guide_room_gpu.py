@@ -2,8 +2,10 @@
 12 quads plus 96 displaced spheres, 1,966,104 triangles, 40 % normalized
 Phong e = 200, the only emitter behind a wall with a 0.1-wide gap), at 256^2
 with the same generator, the same MLP (168-64x4-41, BF16, 8 vMF lobes + BSDF
-mix) and the workload's Adam learning rate 1e-3, trained online on the first
-16 spp (one pass per spp):
+mix) and the workload's Adam learning rate 1e-3, trained online in 16 passes
+of 16 Adam steps on the records of 16 spp each -- the training data of the
+1024^2 workload's first 16 spp (16 passes of 2^20 samples) at a quarter of
+its resolution:
 
 * unbiasedness of one-sample MIS on this scene: the guided image agrees with
   the plain path tracer's per 8x8-pixel cell within 3 sigma (batch means) and
@@ -47,7 +49,7 @@ def reference(room):
 def _trained(room, seed, passes=16):
     g = guide.Guide(room, lr=1e-3, seed=seed)
     for k in range(passes):
-        g.render(W * H, seed=100 * seed + k, train=True, **KW)
+        g.render(W * H * 16, seed=100 * seed + k, train=True, **KW)
     return g
 
 

@@ -4,8 +4,9 @@ restatements:
 * per measurement point from identical eye and photon uniform tables over
   several passes: the same points, emission within 1e-4, and the
   progressive state (N, tau, radius) and final contribution within 1e-4
-  relative for >= 97 % of the points (a photon within an ulp of a radius
-  changes one pass's count M, and every later pass inherits the difference);
+  relative for >= 99.99 % of the points, none beyond 1e-2 (a photon within
+  an ulp of a radius would change one pass's count M, and every later pass
+  inherits the difference);
 * the furnace closed form from Philox streams;
 * the argument checks.
 """
@@ -43,8 +44,8 @@ def test_ppm_per_point_parity_uniform_tables(gpu, M, boxes):
     err = np.maximum(rel[:, 2:5].max(axis=1), np.maximum(rel[:, 8:11].max(axis=1), rel[:, 11:13].max(axis=1)))
     q = np.quantile(err[gathered], [0.5, 0.9, 0.97, 0.99, 1.0])
     print(f"ppm relative error quantiles (50/90/97/99/100 %): {q}")
-    assert (err[gathered] <= 1e-4).mean() >= 0.97, (err[gathered] > 1e-4).sum()
-    assert (err[gathered] <= 1e-2).mean() >= 0.995
+    assert (err[gathered] <= 1e-4).mean() >= 0.9999, (err[gathered] > 1e-4).sum()
+    assert err[gathered].max() <= 1e-2, err[gathered].max()
 
 
 def test_ppm_furnace_closed_form(gpu):
@@ -104,8 +105,8 @@ def test_sppm_per_pixel_parity_uniform_tables(gpu, M, boxes):
     err = np.maximum(rel[:, 0:3].max(axis=1), np.maximum(rel[:, 6:9].max(axis=1), rel[:, 9:11].max(axis=1)))
     q = np.quantile(err[gathered], [0.5, 0.9, 0.97, 0.99, 1.0])
     print(f"sppm relative error quantiles (50/90/97/99/100 %): {q}")
-    assert (err[gathered] <= 1e-4).mean() >= 0.97, (err[gathered] > 1e-4).sum()
-    assert (err[gathered] <= 1e-2).mean() >= 0.99
+    assert (err[gathered] <= 1e-4).mean() >= 0.9999, (err[gathered] > 1e-4).sum()
+    assert err[gathered].max() <= 1e-2, err[gathered].max()
 
 
 def test_sppm_furnace_closed_form(gpu):

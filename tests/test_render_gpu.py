@@ -1,10 +1,11 @@
 """GPU parity of the wavefront renderer::pt against the CPU oracle.
 
 * per-sample contributions from identical uniform tables: the same splat
-  raster position and C within 1e-4 relative for every sample whose path
-  takes the same triangles (paths that diverge are counted; FP32 sin/cos/pow
-  of GPU and host may round differently by an ulp, which can move a grazing
-  bounce onto another triangle);
+  raster position, the same number of splats and C within 1e-4 relative for
+  at least 99.99 % of the samples, and no sample beyond 1e-3 among those
+  with the same splats (kernels and oracle share the FP32 sine, cosine and
+  power of lmx_fmath.h and both round every operation as written, so the
+  records agree bit for bit in practice);
 * the same with Philox-generated uniforms (the generator itself is pinned
   in the CPU tests by the published Random123 known-answer vectors);
 * a closed form: radiance inside an emitting, reflecting closed sphere;
@@ -25,15 +26,17 @@ def _table(n, width, seed):
     return rng.random((n, width), dtype=np.float32)
 
 
-def _compare(gpu_s, cpu_s, rtol=1e-4):
-    """per-sample records: raster x, y, C rgb, #splats"""
+def _compare(gpu_s, cpu_s, rtol=1e-4, worst=1e-3):
+    """per-sample records: raster x, y, C rgb, #splats.  Returns the mask of
+    samples that agree within rtol; asserts the worst case among samples
+    with the same splats."""
     g = gpu_s[:, :6]
     c = cpu_s
-    same_splats = g[:, 5] == c[:, 5]
-    close = np.all(np.abs(g[:, 2:5] - c[:, 2:5]) <= rtol * np.maximum(np.abs(c[:, 2:5]), 1e-3), axis=1)
-    raster = np.all(g[:, 0:2] == c[:, 0:2], axis=1)
-    ok = same_splats & close & raster
-    return ok
+    same = (g[:, 5] == c[:, 5]) & np.all(g[:, 0:2] == c[:, 0:2], axis=1)
+    rel = (np.abs(g[:, 2:5] - c[:, 2:5]) / np.maximum(np.abs(c[:, 2:5]), 1e-3)).max(axis=1)
+    if same.any():
+        assert rel[same].max() <= worst, f"worst relative error {rel[same].max():.3e}"
+    return same & (rel <= rtol)
 
 
 @pytest.mark.parametrize("raster_mode", [render.RASTER_STRATIFIED, render.RASTER_RANDOM])
@@ -50,7 +53,7 @@ def test_pt_per_sample_parity_uniform_table(gpu, raster_mode):
     c = orc.Scene(sc).pt_samples(0, n, max_num_vertices=maxv, raster_mode=raster_mode, table=tab)
     ok = _compare(g, c)
     assert (c[:, 5] > 0).sum() > 200                     # paths that reach the light
-    assert ok.mean() > 0.999, f"{(~ok).sum()} of {n} samples differ"
+    assert ok.mean() >= 0.9999, f"{(~ok).sum()} of {n} samples differ"
     # the film is the sum of the per-sample splats, rescaled by W H / N
     f_cpu = orc.Scene(sc).film(c) * (64 * 64 / n)
     np.testing.assert_allclose(film.cpu().numpy(), f_cpu, rtol=2e-3, atol=2e-3)
@@ -65,7 +68,7 @@ def test_pt_per_sample_parity_phong(gpu, exponent):
     _, _, so = render.Scene(sc).render_pt(n, max_num_vertices=9, seed=77, sample_out=True)
     c = orc.Scene(sc).pt_samples(0, n, max_num_vertices=9, seed=77)
     ok = _compare(so.cpu().numpy(), c)
-    assert ok.mean() > 0.999, f"{(~ok).sum()} of {n} samples differ"
+    assert ok.mean() >= 0.9999, f"{(~ok).sum()} of {n} samples differ"
 
 
 def test_pt_per_sample_parity_sphere_grid_small(gpu):
@@ -75,7 +78,7 @@ def test_pt_per_sample_parity_sphere_grid_small(gpu):
     c = orc.Scene(sc).pt_samples(0, n, max_num_vertices=17, rr_depth=3, seed=5)
     ok = _compare(so.cpu().numpy(), c)
     assert (c[:, 5] > 0).sum() > 100
-    assert ok.mean() > 0.999, f"{(~ok).sum()} of {n} samples differ"
+    assert ok.mean() >= 0.9999, f"{(~ok).sum()} of {n} samples differ"
 
 
 @pytest.mark.parametrize("rr_mode", [render.RR_FIXED, render.RR_THROUGHPUT])
@@ -86,7 +89,7 @@ def test_pt_per_sample_parity_philox(gpu, rr_mode):
     _, _, so = rs.render_pt(n, max_num_vertices=9, seed=1234, sample_out=True, rr_mode=rr_mode, rr_depth=2)
     c = orc.Scene(sc).pt_samples(0, n, max_num_vertices=9, seed=1234, rr_mode=rr_mode, rr_depth=2)
     ok = _compare(so.cpu().numpy(), c)
-    assert ok.mean() > 0.999, f"{(~ok).sum()} of {n} samples differ"
+    assert ok.mean() >= 0.9999, f"{(~ok).sum()} of {n} samples differ"
 
 
 @pytest.mark.parametrize("maxv,rr_depth", [(4, 1), (7, 3), (-1, 2)])
@@ -202,4 +205,4 @@ def test_ggx_per_sample_parity(gpu, entry, mode):
     else:
         c = orc.Scene(sc).nee_samples(0, n, mode, max_num_vertices=m, table=tab)
         ok = _compare_nee(so.cpu().numpy(), c)
-    assert ok.mean() > 0.995, f"{(~ok).sum()} of {n} samples differ"
+    assert ok.mean() >= 0.9999, f"{(~ok).sum()} of {n} samples differ"

@@ -41,10 +41,7 @@ def _run(sc, n, m, gpu, seed=None, table_seed=None, batch=0, **kw):
 def test_renderer_bdpt_per_strategy_parity_uniform_table(gpu, raster_mode, phong):
     sc = scenes.cornell_box(32, 32, phong_boxes=phong)
     g, c = _run(sc, 32 * 32 * 8, 8, gpu, table_seed=13, raster_mode=raster_mode, batch=4096)
-    if phong is None:
-        _check(g, c, tight=0.999, loose=0.9999, min_both=3000)
-    else:
-        _check(g, c, tight=0.99, loose=0.999, min_both=3000)
+    _check(g, c, min_both=3000)
     # every kind of strategy is exercised: direct (s = 0 with a light point),
     # light hits, connections, and t = 1 (light subpath to the camera)
     M = 8
@@ -57,21 +54,21 @@ def test_renderer_bdpt_per_strategy_parity_uniform_table(gpu, raster_mode, phong
 def test_renderer_bdpt_per_strategy_parity_philox(gpu, mis):
     sc = scenes.cornell_box(32, 32, phong_boxes=20.0)
     g, c = _run(sc, 32 * 32 * 8, 9, gpu, seed=77, mis=mis, batch=3000)
-    _check(g, c, tight=0.99, loose=0.999, min_both=3000)
+    _check(g, c, min_both=3000)
 
 
 def test_renderer_bdpt_unbounded_parity(gpu):
     """max_num_vertices = -1: records of 2 x 33^2 strategies per sample."""
     sc = scenes.cornell_box(16, 16)
     g, c = _run(sc, 16 * 16 * 8, -1, gpu, seed=5)
-    _check(g, c, tight=0.999, loose=0.9999, min_both=500)
+    _check(g, c, min_both=500)
     assert (c[:, :, 5] > 0).sum(axis=1).max() > 10
 
 
 def test_renderer_bdpt_min_num_vertices(gpu):
     sc = scenes.cornell_box(16, 16)
     g, c = _run(sc, 16 * 16 * 8, 7, gpu, seed=9, min_num_vertices=4)
-    _check(g, c, tight=0.999, loose=0.9999, min_both=300)
+    _check(g, c, min_both=300)
     M = 7
     for t in range(M + 1):
         for s in range(M + 1 - t):
@@ -113,4 +110,4 @@ def test_renderer_bdpt_per_strategy_parity_ggx(gpu):
     sides and the direct strategies at glossy eye vertices."""
     sc = scenes.cornell_box(32, 32, ggx_boxes=0.3)
     g, c = _run(sc, 32 * 32 * 8, 8, gpu, table_seed=17, batch=4096)
-    _check(g, c, tight=0.99, loose=0.999, min_both=3000)
+    _check(g, c, min_both=3000)

@@ -4,7 +4,8 @@ against the oracle's literal restatement (one pass, brute-force weights):
 * per (sample, t, s, connect / merge) from identical eye and photon uniform
   tables: the same connections and merges contribute (up to shadow rays and
   merge-radius tests that an ulp of a vertex position decides), connection
-  C and w within 1e-4 relative, merge sums of C, counts and weight sums
+  C and w and merge sums of C and of weights within 1e-4 relative for at
+  least 99.99 % of the records (counts equal) with bounded worst cases
   (Lambertian, Phong and Cook-Torrance boxes);
 * the furnace closed form of both modes per path length;
 * a multi-pass render with the radius schedule agrees with the bdpt-mode
@@ -30,7 +31,9 @@ def _records(g, c, merge, floor=1e-9):
     wg, wc = g[both][:, 6].astype(np.float64), c[both][:, 6].astype(np.float64)
     relw = np.abs(wg - wc) / np.maximum(np.abs(wc), 1e-30)
     cnt = g[both][:, 5] != c[both][:, 5]
-    return one, int(both.sum()), np.maximum(rel, relw), int(cnt.sum())
+    # a record whose photon count differs (a photon within an ulp of the
+    # radius) is counted, not compared
+    return one, int(both.sum()), np.maximum(rel, relw)[~cnt], int(cnt.sum())
 
 
 def _run(sc, n, m, nph, r, mode, tab, ptab, gpu):
@@ -62,16 +65,18 @@ def test_vcm_per_record_parity_uniform_tables(gpu, mode, boxes):
     assert both > 2000, both
     assert one <= 2e-3 * both, f"{one} of {both} merge records on one side only"
     assert cnt <= 2e-3 * both, f"{cnt} of {both} merge records with another photon count"
-    assert (err <= 1e-4).mean() >= 0.995, f"{(err > 1e-4).sum()} of {both} merge records beyond 1e-4"
-    assert (err <= 2e-3).mean() >= 0.999, f"{(err > 2e-3).sum()} of {both} merge records beyond 2e-3"
+    print(f"merge records {both}: 99.99 % quantile {np.quantile(err, 0.9999):.2e}, worst {err.max():.2e}")
+    assert (err <= 1e-4).mean() >= 0.9999, f"{(err > 1e-4).sum()} of {both} merge records beyond 1e-4"
+    assert err.max() <= 1e-3, f"worst merge record error {err.max():.3e}"
     if mode == render.VCM_MODE_BDPM:
         assert not np.any(g[:, 0::2, 5] > 0)
         return
     one, both, err, _ = _records(g, c, 0)
     assert both > 5000
     assert one <= 5e-4 * both, f"{one} of {both} connections on one side only"
-    assert (err <= 1e-4).mean() >= 0.995, f"{(err > 1e-4).sum()} of {both} connections beyond 1e-4"
-    assert (err <= 2e-3).mean() >= 0.999
+    print(f"connections {both}: 99.99 % quantile {np.quantile(err, 0.9999):.2e}, worst {err.max():.2e}")
+    assert (err <= 1e-4).mean() >= 0.9999, f"{(err > 1e-4).sum()} of {both} connections beyond 1e-4"
+    assert err.max() <= 5e-4, f"worst connection error {err.max():.3e}"
 
 
 @pytest.mark.parametrize("mode", [render.VCM_MODE_VCM, render.VCM_MODE_BDPM])
