@@ -312,6 +312,19 @@ def pt_section(dev, cpu, threads):
         "cpu_baseline": None,
     }
     out["roofline"] = pt_roofline(sc, rs, kw, n, dt, rays)
+    # the same frame with the traced paths regrouped by material before
+    # shading (70 % Lambertian / 30 % Phong), timed the same way
+    rs.render_pt(1024 * 1024 * 4, shade_order=render.SHADE_BY_MATERIAL, **kw)
+    torch.cuda.synchronize()
+    tm = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        rs.render_pt(n, shade_order=render.SHADE_BY_MATERIAL, **kw)
+        torch.cuda.synchronize()
+        tm.append(time.perf_counter() - t0)
+    out["shade_by_material"] = {"msamples_per_s": round(n / median(tm) / 1e6, 2),
+                                "queue_order_msamples_per_s": out["msamples_per_s"],
+                                "speedup": round(dt / median(tm), 4)}
     # renderer::ptdirect / ptmis on the same frame: next-event estimation
     # adds one shadow ray per vertex
     for entry in ("ptdirect", "ptmis"):

@@ -466,7 +466,7 @@ typedef struct lmx_guide_stats {
     float loss;                    /* mean loss of the last step */
     double render_ms, train_ms;
 } lmx_guide_stats;
-void lmx_guide_default_opts(lmx_guide_opts *opts);   /* lr 5e-3, 0.9, 0.999, 1e-8, 2^15, 16, 50, 7 */
+void lmx_guide_default_opts(lmx_guide_opts *opts);   /* lr 1e-3, 0.9, 0.999, 1e-8, 2^15, 16, 50, 7 */
 int lmx_guide_create(lmx_scene *scene, const lmx_guide_opts *opts, lmx_guide **out);
 void lmx_guide_free(lmx_guide *guide);
 /* host copy of the FP32 parameters in (set != 0) or out */

@@ -147,6 +147,8 @@ struct VB {
     float2* eraster;
     float* V;                  // bdpt: V_k of the light vertices
     uint32_t* mq;              // merging: queue of eye vertices to merge at (sample * 32 + vertex)
+    uint32_t* off;             // connection slots per sample, scanned into slot offsets (batch + 1 words)
+    uint32_t* dmask;           // bdpt: eye vertices k with a direct strategy (bit k), per sample
 };
 
 struct Vtx {
@@ -231,8 +233,11 @@ RD float chain_mu(float a, float b, float m1, float m2, float U, int mis)
 {
     if (a == 0.0f) return 0.0f;
     const float a2 = sqm(a, mis);
-    if (b == 0.0f) return a2 * m1;
-    return a2 * (m1 + sqm(b, mis) * (m2 + U));
+    // a strategy that does not exist (mu 0: no merge there and no connection
+    // term) adds 0 even when its ratio overflowed to infinity
+    const float rest = (b == 0.0f || m2 + U == 0.0f) ? 0.0f : sqm(b, mis) * (m2 + U);
+    const float inner = m1 + rest;
+    return inner == 0.0f ? 0.0f : a2 * inner;
 }
 
 // uniform slots of the bdpt variant (within a subpath block): step j >= 1
@@ -528,6 +533,7 @@ struct Junction {
     float aL, bL;      // the light side's ratios at vertices sv - 1, sv - 2
     float aE, bE;      // the eye side's ratios at vertices t - 2, t - 3 (t >= 2)
     float pLsp1;       // pdf of eye vertex t - 2 from the junction (t >= 3)
+    float pEsm1, pEsm2;  // pdfs of light vertices sv - 1, sv - 2 from the eye side
     float sum;
     bool ok;           // every pdf the sum divides by is nonzero
 };
@@ -579,6 +585,8 @@ RD Junction junction(const RScene& s, const BParams& p, float cm, const Vtx& vL,
     if (t >= 2 && (vE.pf == 0.0f || (t >= 3 && (vEp.pf == 0.0f || vE.U < 0.0f)))) J.ok = false;
     J.aL = J.bL = J.aE = J.bE = 0.0f;
     J.pLsp1 = pLsp1;
+    J.pEsm1 = pEsm1;
+    J.pEsm2 = pEsm2;
     J.sum = p.c_conn;
     if (J.ok) {
         J.aL = pEsm1 / vL.pf;
@@ -600,35 +608,80 @@ RD Junction junction(const RScene& s, const BParams& p, float cm, const Vtx& vL,
     return J;
 }
 
-// The weight denominator of a merge with the connection terms off (mode
-// bdpm, c_conn = 0) divided by the merge's own term N (pi r^2 pLs)^2: every
-// mu = N (pi r^2 pf)^2 becomes (pf / pLs)^2 and the stored chains U are
-// divided by the same constant.  Unlike the plain sum, whose terms all scale
-// with the squared pdfs and can underflow together (to 0, giving 0/0 for the
-// contribution), this is >= 1; it overflows only where the merge's weight
-// truly vanishes (the contribution then goes to 0).
-RD float merge_den_bdpm(const Junction& J, float cm, const Vtx& vL, const Vtx& vLp, int sv, const Vtx& vE, int t,
-                        float prE3, int mis)
+// A float with a separate exponent, x = m 2^e (m from frexpf, in [0.5, 1) or
+// 0, inf, nan): products and quotients of pdfs whose factors would overflow
+// or underflow FP32 on their own, while the result is in range.
+struct XF {
+    float m;
+    int e;
+};
+RD XF xf(float v) { XF r; r.m = frexpf(v, &r.e); return r; }
+RD XF xmul(XF a, XF b) { XF r = xf(a.m * b.m); r.e += a.e + b.e; return r; }
+RD XF xdiv(XF a, XF b) { XF r = xf(a.m / b.m); r.e += a.e - b.e; return r; }
+// x^2 as an FP32 value: +inf above FLT_MAX, 0 below the denormals
+RD float xsq(XF a)
 {
-    const float inv = 1.0f / J.pLs;
-    auto mn = [&](bool mergeable, float pf) -> float {
-        if (!mergeable || pf == 0.0f) return 0.0f;
-        const float q = pf * inv;
-        return q * q;
-    };
-    auto un = [&](float U) -> float { return U == 0.0f ? 0.0f : (U / cm) * inv * inv; };
+    const int e = 2 * a.e;
+    return e > 256 ? INFINITY : (e < -320 ? 0.0f : ldexpf(a.m * a.m, e));
+}
+
+// The weight denominator of a merge with the connection terms off (mode
+// bdpm, c_conn = 0), divided by the merge's own term N (pi r^2 pLs)^2: every
+// mu = N (pi r^2 pf)^2 becomes (pf / pLs)^2 and the stored chains U are
+// divided by N (pi r^2 pLs)^2 as well.  The power-heuristic terms are written
+// out as products of the pdfs they consist of, so that factors that cancel
+// (the light side's pdfs against pLs, which carry the same glossy lobe; pLs
+// against the eye side's ratio a_E = pLs / pf) never pass through FP32 on
+// their own:
+//   light side  (pEsm1 / pLs)^2 [sv >= 2],
+//               (pEsm1 pEsm2 / (pf_{sv-1} pLs))^2 [sv >= 3],
+//               (pEsm1 pEsm2 / (pf_{sv-1} pf_{sv-2} pLs))^2 U_L / N(pi r^2)^2;
+//   eye side    (pLsp1 / pf_{t-1})^2 [t >= 3],
+//               (pLsp1 prE3 / (pf_{t-1} pf_{t-2}))^2 [t >= 4],
+//               (pLsp1 / (pf_{t-1} pf_{t-2}))^2 U_E / N(pi r^2)^2
+// (a_L a_L m1 and so on of the chains, the zero-factor ends included).  The
+// sum is >= 1 and overflows only where the merge's weight truly vanishes.
+RD float merge_den_bdpm(const Junction& J, float cm, const Vtx& vL, const Vtx& vLp, int sv, const Vtx& vE,
+                        const Vtx& vEp, int t, float prE3)
+{
+    const XF pLs = xf(J.pLs);
     float d = 1.0f;                                          // the merge's own term
-    d += chain_mu(J.aL, J.bL, mn(sv >= 2, vL.pf), mn(sv >= 3, sv >= 3 ? vLp.pf : 0.0f), un(vL.U), mis);
-    if (t >= 2) d += chain_mu(J.aE, J.bE, mn(t >= 3, J.pLsp1), mn(t >= 4, prE3), un(vE.U), mis);
+    if (J.pEsm1 != 0.0f) {
+        const XF e1 = xf(J.pEsm1);
+        if (sv >= 2) d += xsq(xdiv(e1, pLs));
+        if (J.pEsm2 != 0.0f) {
+            const XF e12 = xdiv(xmul(e1, xf(J.pEsm2)), xmul(xf(vL.pf), pLs));
+            if (sv >= 3) d += xsq(e12);
+            if (vL.U > 0.0f) d += xsq(xdiv(xmul(e12, xf(sqrtf(vL.U / cm))), xf(vLp.pf)));
+        }
+    }
+    if (t >= 3 && J.pLsp1 != 0.0f) {
+        const XF e0 = xdiv(xf(J.pLsp1), xf(vE.pf));
+        d += xsq(e0);
+        const XF e01 = xdiv(e0, xf(vEp.pf));
+        if (t >= 4 && prE3 != 0.0f) d += xsq(xmul(e01, xf(prE3)));
+        if (vE.U > 0.0f) d += xsq(xmul(e01, xf(sqrtf(vE.U / cm))));
+    }
     return d;
+}
+
+// s range [s1, s2] of the connections with eye vertex t - 1:
+// [max(2 - t, minv - t, 1), min(maxv - t, nL)]; the unbounded bdpt variant
+// takes every s <= nL
+RD int conn_s1(int t, int minv) { return max(max(1, 2 - t), minv - t); }
+RD int conn_s2(const BParams& p, int t, int nL)
+{
+    return (p.variant == BD_BDPT && !p.bounded) ? nL : min(p.maxv - t, nL);
 }
 
 // Connection plan: one thread per sample.  Strategies s = 0 (the eye
 // subpath hit a light) are evaluated and splatted here; every structurally
-// possible s >= 1 gets a shadow-queue slot, and so does (bdpt) every direct
-// strategy, reserved with one atomic per workgroup, whose (sample, s, t,
-// direct) the evaluation step reads.
-__global__ __launch_bounds__(256) void k_bd_plan(RScene s, BParams p, BCtl* __restrict__ ctl, VB vb, ShadowQ sq)
+// possible s >= 1 needs a shadow-queue slot, and so does (bdpt) every
+// direct strategy: their number goes to off[i], which a scan turns into the
+// sample's first slot.  The slots themselves are written round by round
+// (k_bd_round, k_bd_slots), so the shadow queue holds what one round
+// connects rather than the worst case of every sample.
+__global__ __launch_bounds__(256) void k_bd_plan(RScene s, BParams p, BCtl* __restrict__ ctl, VB vb)
 {
     __shared__ uint32_t lds[5];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -638,28 +691,20 @@ __global__ __launch_bounds__(256) void k_bd_plan(RScene s, BParams p, BCtl* __re
     const int nL = valid ? (int)vb.nsub[1][i] : 0;
     const int minv = min_len(p);
     const bool bdpt = p.variant == BD_BDPT;
-    // s range of each t: [max(2 - t, minv - t, 1), min(maxv - t, nL)]; the
-    // unbounded bdpt variant takes every s <= nL
     uint32_t cnt = 0;
     for (int t = 1; t <= nE; t++) {
-        const int s1 = max(max(1, 2 - t), minv - t);
-        const int s2 = (bdpt && !p.bounded) ? nL : min(p.maxv - t, nL);
+        const int s1 = conn_s1(t, minv), s2 = conn_s2(p, t, nL);
         if (s2 >= s1) cnt += (uint32_t)(s2 - s1 + 1);
     }
-    uint32_t ndirect = 0;
+    uint32_t dm = 0;
     if (bdpt)
-        for (int k = 0; k < nE; k++) ndirect += eye_direct(s, p, vb, i, sample, k) ? 1u : 0u;
-    uint32_t slot = block_reserve<4>(&ctl->n_shadow, cnt + ndirect, lds);
-    for (int t = 1; t <= nE; t++) {
-        const int s1 = max(max(1, 2 - t), minv - t);
-        const int s2 = (bdpt && !p.bounded) ? nL : min(p.maxv - t, nL);
-        for (int sv = s1; sv <= s2; sv++) sq.meta[slot++] = make_uint4(i, (uint32_t)sv, (uint32_t)t, 0u);
+        for (int k = 0; k < nE; k++) dm |= eye_direct(s, p, vb, i, sample, k) ? 1u << k : 0u;
+    cnt += (uint32_t)__popc(dm);
+    if (i < p.batch) {
+        vb.off[i] = cnt;
+        if (bdpt) vb.dmask[i] = dm;
     }
-    // a direct strategy is evaluated as the s = 1 connection of a fresh light
-    // point with eye vertex k (its pdf equals that of s = 1); flag in .w
-    if (ndirect)
-        for (int k = 0; k < nE; k++)
-            if (eye_direct(s, p, vb, i, sample, k)) sq.meta[slot++] = make_uint4(i, 1u, (uint32_t)k + 1u, 1u);
+    if (i == 0) vb.off[p.batch] = 0u;          // becomes the batch's total
     if (p.merging) {
         // eye vertices k >= 1 that scatter and leave room for a merged path
         // of at most maxv vertices (photon vertex j >= 1, length j + k + 1)
@@ -702,6 +747,62 @@ __global__ __launch_bounds__(256) void k_bd_plan(RScene s, BParams p, BCtl* __re
         const f3 C = mk3(Cs.x * w, Cs.y * w, Cs.z * w);
         splat(p, pixel, C);
         record(p, sample, 0, t, 0, eras.x, eras.y, C, w);
+    }
+}
+
+// Start of the next connection round: the samples [lo, hi) whose slots fit
+// the shadow queue's `cap` slots from lo's first slot on, hi as large as
+// possible.  The offsets are non-decreasing and no sample needs more than
+// cap slots, so every round takes at least one sample while any remain.
+// One wave: a 64-way search, each level probing 64 indices at once.
+__global__ void k_bd_round(BCtl* ctl, const uint32_t* __restrict__ off, uint32_t batch, uint32_t cap)
+{
+    const uint32_t lane = threadIdx.x;
+    const uint32_t lo = ctl->conn_next;
+    if (lo >= batch) {
+        if (lane == 0) { ctl->n_shadow = 0; ctl->conn_lo = batch; ctl->conn_hi = batch; }
+        return;
+    }
+    const uint32_t base = off[lo];
+    // the answer lies in [a, b]; off[a] - base <= cap holds throughout
+    uint32_t a = lo, b = batch;
+    while (a < b) {
+        const uint32_t step = (b - a + 63u) / 64u;
+        const uint32_t x = min(b, a + (lane + 1u) * step);
+        const uint32_t k = (uint32_t)__popcll(__ballot(off[x] - base <= cap));   // lanes 0 .. k-1 fit
+        const uint32_t na = a + k * step;
+        b = k == 64u ? b : min(b, a + (k + 1u) * step) - 1u;
+        a = min(na, b);
+    }
+    if (lane == 0) {
+        ctl->conn_lo = lo;
+        ctl->conn_hi = a;
+        ctl->conn_base = base;
+        ctl->n_shadow = off[a] - base;
+        ctl->conn_next = a;
+    }
+}
+
+// The shadow-queue slots of the round's samples: (sample, s, t, direct) of
+// every connection counted by k_bd_plan, from the sample's own offset on.
+__global__ __launch_bounds__(256) void k_bd_slots(RScene s, BParams p, const BCtl* __restrict__ ctl, VB vb, ShadowQ sq)
+{
+    const uint32_t lo = ctl->conn_lo, hi = ctl->conn_hi, base = ctl->conn_base;
+    const int minv = min_len(p);
+    for (uint32_t i = lo + blockIdx.x * blockDim.x + threadIdx.x; i < hi; i += gridDim.x * blockDim.x) {
+        const uint32_t sample = ctl->batch_first + i;
+        if (!(sample - ctl->pass_first < p.total)) continue;
+        const int nE = (int)vb.nsub[0][i], nL = (int)vb.nsub[1][i];
+        uint32_t slot = vb.off[i] - base;
+        for (int t = 1; t <= nE; t++) {
+            const int s1 = conn_s1(t, minv), s2 = conn_s2(p, t, nL);
+            for (int sv = s1; sv <= s2; sv++) sq.meta[slot++] = make_uint4(i, (uint32_t)sv, (uint32_t)t, 0u);
+        }
+        // a direct strategy is evaluated as the s = 1 connection of a fresh
+        // light point with eye vertex k (its pdf equals that of s = 1); flag in .w
+        if (p.variant == BD_BDPT)
+            for (uint32_t dm = vb.dmask[i]; dm; dm &= dm - 1u)
+                sq.meta[slot++] = make_uint4(i, 1u, (uint32_t)__ffs(dm), 1u);
     }
 }
 
@@ -834,9 +935,15 @@ __global__ void k_bd_next_batch(BCtl* ctl, uint32_t batch)
     }
 }
 
+__global__ void k_bd_conn_start(BCtl* ctl)
+{
+    if (threadIdx.x == 0) ctl->conn_next = 0;
+}
+
 __global__ void k_bd_init(BCtl* ctl)
 {
     if (threadIdx.x == 0) {
+        ctl->conn_next = 0; ctl->conn_lo = 0; ctl->conn_hi = 0; ctl->conn_base = 0;
         ctl->n_trace = 0; ctl->n_next = 0; ctl->n_shadow = 0; ctl->batch_first = 0;
         ctl->rays = 0; ctl->shadow_rays = 0;
         ctl->pass_first = 0; ctl->radius = 0.0f; ctl->cell = 0.0f; ctl->c_merge = 0.0f; ctl->merges = 0; ctl->dropped = 0; ctl->cap_n[0] = 0; ctl->cap_n[1] = 0;
@@ -1096,10 +1203,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                     const float pm = area * J.pLs;
                     w = pm * pm / J.sum;
                 } else {
-                    const float d = merge_den_bdpm(J, cm, vL, vLp, j, vE, t, prE3, p.mis);
-                    const float f = (area / cm) / (J.pLs * d);
-                    C = mk3(J.Cs.x * f, J.Cs.y * f, J.Cs.z * f);
-                    w = (area / cm) * area / d;
+                    // C = Cs / (N pi r^2 pLs d), with separate exponents:
+                    // Cs and pLs share the geometry term and the light
+                    // side's glossy lobe, so their quotient is in range
+                    // where each of them underflows
+                    const float d = merge_den_bdpm(J, cm, vL, vLp, j, vE, vEp, t, prE3);
+                    const float ac = area / cm;
+                    const XF pd = xmul(xf(J.pLs), xf(d));
+                    auto c1 = [&](float c) {
+                        const XF q = xdiv(xmul(xf(c), xf(ac)), pd);
+                        return ldexpf(q.m, q.e);
+                    };
+                    C = mk3(c1(J.Cs.x), c1(J.Cs.y), c1(J.Cs.z));
+                    w = ac * area / d;
                 }
                 if (!isfinite(C.x) || !isfinite(C.y) || !isfinite(C.z)) {
                     ndrop++;                                     // counted and reported, never silent
@@ -1587,12 +1703,21 @@ uint32_t bd_conn_per_sample(uint32_t maxv, int variant, bool bounded, bool conne
     return (bounded ? maxv * (maxv - 1) / 2 : maxv * maxv) + maxv;
 }
 
+// Shadow-queue slots per sample of one connection round: a sample whose
+// subpaths hold more connections than this spills into further rounds
+// instead of every sample reserving its worst case (1,056 slots, 101 KB,
+// for unbounded renderer::bdpt).
+#define BD_ROUND_SLOTS 24u
+
+uint32_t bd_round_slots(uint32_t worst) { return std::min(worst, BD_ROUND_SLOTS); }
+
 // Device bytes of one batch slot: two subpaths of vertex records, their
-// queues and hits, and the shadow-ray records of every connection.
-size_t bd_bytes_per_sample(uint32_t maxv, uint32_t conn)
+// queues and hits, the connection count, and the shadow-ray records of one
+// round's connections.
+size_t bd_bytes_per_sample(uint32_t maxv, uint32_t worst)
 {
-    return 2 * ((size_t)maxv * 64 + 4 + 4 * 16 + 2 * 16 + 2 * 16) + (size_t)maxv * 4 + 4 + 8 + 2 * 16 +
-           (size_t)conn * (2 * 16 + 4 * 16);
+    return 2 * ((size_t)maxv * 64 + 4 + 4 * 16 + 2 * 16 + 2 * 16) + (size_t)maxv * 4 + 4 + 8 + 2 * 16 + 8 +
+           (size_t)bd_round_slots(worst) * (2 * 16 + 4 * 16);
 }
 
 // Buffers for batches of B samples of up to maxv vertices and conn
@@ -1607,16 +1732,30 @@ void bd_release(lmx_scene* sc)
     }
     sc->bd_V.release(); sc->bd_epix.release(); sc->bd_eraster.release(); sc->bd_hits.release();
     sc->bd_sray.release(); sc->bd_sc.release(); sc->bd_saux.release(); sc->bd_shits.release();
-    sc->bd_smeta.release();
+    sc->bd_smeta.release(); sc->bd_off.release(); sc->bd_part.release(); sc->bd_dmask.release();
+    sc->bd_nconn = 0;
 }
 
-int bd_buffers(lmx_scene* sc, uint32_t B, uint32_t maxv, uint32_t conn)
+// Device bytes the scene's batch buffers hold.
+size_t bd_held_bytes(const lmx_scene* sc)
 {
-    if (B <= sc->bd_batch && maxv <= sc->bd_maxv && conn <= sc->bd_conn) return LMX_OK;
+    const size_t B = sc->bd_batch, nv = B * sc->bd_maxv * 4;
+    return 2 * (nv * 16 + B * 4 + 4 * B * 16 + 2 * B * 16 + 2 * B * 16) + B * sc->bd_maxv * 4 + B * 4 + B * 8 +
+           2 * B * 16 + (B + 1) * 4 + B * 4 + (size_t)sc->bd_nconn * (2 * 16 + 4 * 16);
+}
+
+// Buffers for batches of B samples of up to maxv vertices and `conn`
+// shadow-queue slots per sample.  With `rounds` (the samplers that connect
+// in rounds), conn is a sample's worst case and the queue holds one round:
+// B x bd_round_slots(conn) slots, never fewer than one sample's worst case.
+int bd_buffers(lmx_scene* sc, uint32_t B, uint32_t maxv, uint32_t conn, bool rounds = false)
+{
+    const size_t nconn = rounds ? std::max((size_t)B * bd_round_slots(conn), (size_t)conn) : (size_t)B * conn;
+    if (nconn > 0xffffffffull) return set_error(LMX_EINVAL, "bdpt buffers: more than 2^32 - 1 connection slots");
+    if (B <= sc->bd_batch && maxv <= sc->bd_maxv && nconn <= sc->bd_nconn) return LMX_OK;
     bd_release(sc);
     hipError_t e;
     const size_t nv = (size_t)B * maxv * 4;
-    const size_t nconn = (size_t)B * conn;
     for (int k = 0; k < 2; k++) {
         if ((e = sc->bd_vert[k].alloc(nv)) != hipSuccess) return hip_error(e, "bdpt vertex buffers");
         if ((e = sc->bd_nsub[k].alloc(B)) != hipSuccess) return hip_error(e, "bdpt buffers");
@@ -1628,7 +1767,9 @@ int bd_buffers(lmx_scene* sc, uint32_t B, uint32_t maxv, uint32_t conn)
         (e = sc->bd_eraster.alloc(B)) != hipSuccess ||
         (e = sc->bd_hits.alloc(2 * (size_t)B)) != hipSuccess || (e = sc->bd_sray.alloc(2 * nconn)) != hipSuccess ||
         (e = sc->bd_sc.alloc(nconn)) != hipSuccess || (e = sc->bd_saux.alloc(nconn)) != hipSuccess ||
-        (e = sc->bd_shits.alloc(nconn)) != hipSuccess || (e = sc->bd_smeta.alloc(nconn)) != hipSuccess)
+        (e = sc->bd_shits.alloc(nconn)) != hipSuccess || (e = sc->bd_smeta.alloc(nconn)) != hipSuccess ||
+        (e = sc->bd_off.alloc((size_t)B + 1)) != hipSuccess || (e = sc->bd_dmask.alloc(B)) != hipSuccess ||
+        (e = sc->bd_part.alloc(((size_t)B + SCAN_TILE) / SCAN_TILE)) != hipSuccess)
         return hip_error(e, "bdpt buffers");
     if (!sc->bd_ctl.p && (e = sc->bd_ctl.alloc(1)) != hipSuccess) return hip_error(e, "bdpt control block");
     if (!sc->bd_hctl && (e = hipHostMalloc((void**)&sc->bd_hctl, sizeof(BCtl), hipHostMallocDefault)) != hipSuccess)
@@ -1636,7 +1777,42 @@ int bd_buffers(lmx_scene* sc, uint32_t B, uint32_t maxv, uint32_t conn)
     sc->bd_batch = B;
     sc->bd_maxv = maxv;
     sc->bd_conn = conn;
+    sc->bd_nconn = (uint32_t)nconn;
     return LMX_OK;
+}
+
+// Records one connection round: the next range of samples whose slots fit
+// the shadow queue, their slots, the connection terms, the any-hit trace of
+// the shadow rays, and the splat of the unoccluded ones.  A round with no
+// sample left launches kernels that find nothing to do.
+int bd_enqueue_round(lmx_scene* sc, const BParams& bp, const VB& vb, hipStream_t st)
+{
+    BCtl* ctl = sc->bd_ctl.p;
+    const int cap_grid = sc->cus * 8;
+    const uint32_t nconn = sc->bd_nconn;
+    ShadowQ sq{sc->bd_sray.p, sc->bd_sc.p, sc->bd_saux.p, sc->bd_smeta.p};
+    k_bd_round<<<1, 64, 0, st>>>(ctl, vb.off, bp.batch, nconn);
+    k_bd_slots<<<cap_grid, 256, 0, st>>>(sc->rs, bp, ctl, vb, sq);
+    k_bd_connect<<<cap_grid, 256, 0, st>>>(sc->rs, bp, ctl, vb, sq);
+    if (bp.ray_out) {
+        const uint32_t half = bp.ray_out_cap / 2;
+        k_bd_capture<<<cap_grid, 256, 0, st>>>(ctl, 1, sq.ray, bp.ray_out, half, bp.ray_out_cap);
+        k_bd_capture_advance<<<1, 64, 0, st>>>(ctl, 1, bp.ray_out_cap - half);
+    }
+    int rc = launch_trace(sc->accel, reinterpret_cast<const WRay*>(sq.ray), 0, &ctl->n_shadow, nconn,
+                          reinterpret_cast<WHit*>(sc->bd_shits.p), st, true);
+    if (rc != LMX_OK) return rc;
+    k_bd_splat<<<std::max(1, std::min((int)((nconn + 255) / 256), cap_grid)), 256, 0, st>>>(bp, ctl, sq, sc->bd_shits.p);
+    return LMX_OK;
+}
+
+// Connection rounds a batch graph holds: one when a round can take every
+// sample at its worst case, else two, after which the host checks for
+// samples left over (rarely any: the round slots exceed the typical count).
+int bd_graph_rounds(const lmx_scene* sc, const BParams& bp)
+{
+    const uint32_t worst = bd_conn_per_sample((uint32_t)bp.maxv, bp.variant, bp.bounded != 0, bp.connect != 0);
+    return (uint64_t)sc->bd_nconn >= (uint64_t)bp.batch * worst ? 1 : 2;
 }
 
 // Records one batch: generation, subpath iterations, connections, shadow
@@ -1650,6 +1826,8 @@ int bd_enqueue_batch(lmx_scene* sc, const BParams& bp, hipStream_t st)
     vb.epix = sc->bd_epix.p; vb.eraster = sc->bd_eraster.p;
     vb.V = sc->bd_V.p;
     vb.mq = sc->vc_mq.p;
+    vb.off = sc->bd_off.p;
+    vb.dmask = sc->bd_dmask.p;
     Queue q[2];
     for (int k = 0; k < 2; k++) { q[k].ray = sc->bd_ray[k].p; q[k].thr = sc->bd_thr[k].p; q[k].meta = sc->bd_meta[k].p; }
     BCtl* ctl = sc->bd_ctl.p;
@@ -1669,18 +1847,20 @@ int bd_enqueue_batch(lmx_scene* sc, const BParams& bp, hipStream_t st)
         k_bd_commit<<<1, 64, 0, st>>>(ctl);
         a = 1 - a;
     }
-    ShadowQ sq{sc->bd_sray.p, sc->bd_sc.p, sc->bd_saux.p, sc->bd_smeta.p};
-    k_bd_plan<<<(B + 255) / 256, 256, 0, st>>>(sc->rs, bp, ctl, vb, sq);
-    k_bd_connect<<<cap_grid, 256, 0, st>>>(sc->rs, bp, ctl, vb, sq);
-    const uint32_t nconn = B * bd_conn_per_sample((uint32_t)bp.maxv, bp.variant, bp.bounded != 0, bp.connect != 0);
-    if (bp.ray_out) {
-        k_bd_capture<<<cap_grid, 256, 0, st>>>(ctl, 1, sq.ray, bp.ray_out, half, bp.ray_out_cap);
-        k_bd_capture_advance<<<1, 64, 0, st>>>(ctl, 1, bp.ray_out_cap - half);
+    k_bd_plan<<<(B + 255) / 256, 256, 0, st>>>(sc->rs, bp, ctl, vb);
+    // connection counts -> slot offsets; off[B] becomes the batch's total
+    const uint32_t nt = (B + 1 + SCAN_TILE - 1) / SCAN_TILE;
+    k_scan_tiles<<<nt, 256, 0, st>>>(vb.off, B + 1, sc->bd_part.p);
+    if (nt > 1) {
+        k_scan_part<<<1, 256, 0, st>>>(sc->bd_part.p, nt);
+        k_scan_add<<<std::max(1, std::min((int)((B + 1 + 255) / 256), cap_grid)), 256, 0, st>>>(vb.off, B + 1, sc->bd_part.p);
     }
-    int rc = launch_trace(sc->accel, reinterpret_cast<const WRay*>(sq.ray), 0, &ctl->n_shadow, nconn,
-                          reinterpret_cast<WHit*>(sc->bd_shits.p), st, true);
-    if (rc != LMX_OK) return rc;
-    k_bd_splat<<<std::max(1, std::min((int)((nconn + 255) / 256), cap_grid)), 256, 0, st>>>(bp, ctl, sq, sc->bd_shits.p);
+    k_bd_conn_start<<<1, 64, 0, st>>>(ctl);
+    const int rounds = bd_graph_rounds(sc, bp);
+    for (int k = 0; k < rounds; k++) {
+        const int rc = bd_enqueue_round(sc, bp, vb, st);
+        if (rc != LMX_OK) return rc;
+    }
     if (bp.merging) {
         VB ph = vb;
         ph.v[1] = sc->vc_vert.p; ph.nsub[1] = sc->vc_nsub.p;
@@ -1689,13 +1869,49 @@ int bd_enqueue_batch(lmx_scene* sc, const BParams& bp, hipStream_t st)
         k_vc_merge<<<std::max(1, (int)std::min<size_t>((nm + 255) / 256, (size_t)cap_grid * 2)), 256, 0, st>>>(
             sc->rs, bp, ctl, vb, ph, g);
     }
-    k_bd_next_batch<<<1, 64, 0, st>>>(ctl, B);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? LMX_OK : hip_error(e, "bdpt launch");
 }
 
-// Captures one batch as a graph.
-int bd_graph(lmx_scene* sc, const BParams& bp, hipStream_t st, hipGraphExec_t* exec)
+// Captures one connection round as a graph.
+int bd_round_graph(lmx_scene* sc, const BParams& bp, hipStream_t st, hipGraphExec_t* exec)
+{
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_error(e, "capture");
+    VB vb;
+    vb.v[0] = sc->bd_vert[0].p; vb.v[1] = sc->bd_vert[1].p;
+    vb.nsub[0] = sc->bd_nsub[0].p; vb.nsub[1] = sc->bd_nsub[1].p;
+    vb.epix = sc->bd_epix.p; vb.eraster = sc->bd_eraster.p;
+    vb.V = sc->bd_V.p;
+    vb.mq = sc->vc_mq.p;
+    vb.off = sc->bd_off.p;
+    vb.dmask = sc->bd_dmask.p;
+    int rc = bd_enqueue_round(sc, bp, vb, st);
+    if (rc == LMX_OK && (e = hipGetLastError()) != hipSuccess) rc = hip_error(e, "bdpt launch");
+    e = hipStreamEndCapture(st, &graph);
+    if (rc == LMX_OK && e != hipSuccess) rc = hip_error(e, "capture end");
+    if (rc == LMX_OK && (e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0)) != hipSuccess)
+        rc = hip_error(e, "graph instantiate");
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+}
+
+// The graphs of one batch shape: the batch itself (with its in-graph
+// connection rounds) and, when those rounds may not be enough, one more
+// round to replay until every sample is connected.
+struct BGraphs {
+    hipGraphExec_t batch = nullptr, round = nullptr;
+    uint32_t B = 0;
+    ~BGraphs()
+    {
+        if (batch) (void)hipGraphExecDestroy(batch);
+        if (round) (void)hipGraphExecDestroy(round);
+    }
+};
+
+// Captures one batch as a graph (and the extra round when needed).
+int bd_graph(lmx_scene* sc, const BParams& bp, hipStream_t st, BGraphs& g)
 {
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
@@ -1703,10 +1919,32 @@ int bd_graph(lmx_scene* sc, const BParams& bp, hipStream_t st, hipGraphExec_t* e
     int rc = bd_enqueue_batch(sc, bp, st);
     e = hipStreamEndCapture(st, &graph);
     if (rc == LMX_OK && e != hipSuccess) rc = hip_error(e, "capture end");
-    if (rc == LMX_OK && (e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0)) != hipSuccess)
+    if (rc == LMX_OK && (e = hipGraphInstantiate(&g.batch, graph, nullptr, nullptr, 0)) != hipSuccess)
         rc = hip_error(e, "graph instantiate");
     if (graph) (void)hipGraphDestroy(graph);
+    g.B = bp.batch;
+    if (rc == LMX_OK && bd_graph_rounds(sc, bp) > 1) rc = bd_round_graph(sc, bp, st, &g.round);
     return rc;
+}
+
+// Runs one batch: its graph; then, when its rounds may have left samples
+// unconnected, a read of the round cursor and further rounds until none is
+// left; then the advance to the next batch.
+int bd_run_batch(lmx_scene* sc, const BGraphs& g, hipStream_t st)
+{
+    hipError_t e = hipGraphLaunch(g.batch, st);
+    if (e != hipSuccess) return hip_error(e, "graph launch");
+    while (g.round) {
+        if ((e = hipMemcpyAsync(&sc->bd_hctl->conn_next, &sc->bd_ctl.p->conn_next, sizeof(uint32_t),
+                                hipMemcpyDeviceToHost, st)) != hipSuccess ||
+            (e = hipStreamSynchronize(st)) != hipSuccess)
+            return hip_error(e, "connection rounds");
+        if (sc->bd_hctl->conn_next >= g.B) break;
+        if ((e = hipGraphLaunch(g.round, st)) != hipSuccess) return hip_error(e, "graph launch");
+    }
+    k_bd_next_batch<<<1, 64, 0, st>>>(sc->bd_ctl.p, g.B);
+    e = hipGetLastError();
+    return e == hipSuccess ? LMX_OK : hip_error(e, "bdpt launch");
 }
 
 // autotuner key: variant, max_num_vertices, bounded
@@ -1733,7 +1971,7 @@ int bd_autotune(lmx_scene* sc, BParams bp, uint64_t num_samples, float* film, hi
     size_t free_b = 0, total_b = 0;
     hipError_t e = hipMemGetInfo(&free_b, &total_b);
     if (e != hipSuccess) return hip_error(e, "bdpt autotune: memory info");
-    const size_t budget = (free_b + (size_t)sc->bd_batch * bd_bytes_per_sample(sc->bd_maxv, sc->bd_conn)) / 2;
+    const size_t budget = (free_b + bd_held_bytes(sc)) / 2;
     const uint32_t conn = bd_conn_per_sample((uint32_t)bp.maxv, bp.variant, bp.bounded != 0);
     const size_t per = bd_bytes_per_sample((uint32_t)bp.maxv, conn);
     bp.sample_out = nullptr;
@@ -1753,22 +1991,22 @@ int bd_autotune(lmx_scene* sc, BParams bp, uint64_t num_samples, float* film, hi
         if (B != cands[0] && ((size_t)B * per > budget || B > want)) break;
         bp.batch = B;
         bp.total = 2u * B;
-        hipGraphExec_t exec = nullptr;
-        if ((rc = bd_buffers(sc, B, (uint32_t)bp.maxv, conn)) != LMX_OK ||
-            (rc = bd_graph(sc, bp, st, &exec)) != LMX_OK) {
+        BGraphs g;
+        if ((rc = bd_buffers(sc, B, (uint32_t)bp.maxv, conn, true)) != LMX_OK ||
+            (rc = bd_graph(sc, bp, st, g)) != LMX_OK) {
             if (best_rate < 0.0) break;        // not even the smallest size fits: report it
             (void)hipGetLastError();           // a larger candidate failed: keep the best so far
             rc = LMX_OK;
             break;
         }
         k_bd_init<<<1, 64, 0, st>>>(sc->bd_ctl.p);
-        (void)hipGraphLaunch(exec, st);
+        if ((rc = bd_run_batch(sc, g, st)) != LMX_OK) break;
         (void)hipEventRecord(e0, st);
-        e = hipGraphLaunch(exec, st);
+        rc = bd_run_batch(sc, g, st);
         (void)hipEventRecord(e1, st);
         const hipError_t es = hipEventSynchronize(e1);
-        (void)hipGraphExecDestroy(exec);
-        if (e != hipSuccess || es != hipSuccess) { rc = hip_error(e != hipSuccess ? e : es, "bdpt autotune"); break; }
+        if (rc != LMX_OK) break;
+        if (es != hipSuccess) { rc = hip_error(es, "bdpt autotune"); break; }
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, e0, e1);
         const double rate = B / (double)std::max(ms, 1e-3f);
@@ -1860,27 +2098,27 @@ int render_bidir(lmx_scene* sc, const lmx_render_params* p, int variant, const c
     }
     B = (uint32_t)std::min<uint64_t>(B, num_samples);
     bp.batch = B;
-    if (rc == LMX_OK) rc = bd_buffers(sc, B, maxv, bd_conn_per_sample(maxv, variant, bounded));
+    if (rc == LMX_OK) rc = bd_buffers(sc, B, maxv, bd_conn_per_sample(maxv, variant, bounded), true);
     const size_t npix = (size_t)sc->width * (size_t)sc->height;
     const int zgrid = std::max(1, std::min((int)((3 * npix + 255) / 256), sc->cus * 8));
-    hipGraphExec_t exec = nullptr;
+    BGraphs g;
     if (rc == LMX_OK) {
         k_bd_zero<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix);
         k_bd_init<<<1, 64, 0, st>>>(sc->bd_ctl.p);
-        rc = bd_graph(sc, bp, st, &exec);
+        rc = bd_graph(sc, bp, st, g);
     }
     uint64_t nbatch = (num_samples + B - 1) / B;
     uint64_t taken = p->num_samples;
     if (!timed) {
-        for (uint64_t b = 0; rc == LMX_OK && b < nbatch; b++)
-            if ((e = hipGraphLaunch(exec, st)) != hipSuccess) rc = hip_error(e, "graph launch");
+        for (uint64_t b = 0; rc == LMX_OK && b < nbatch; b++) rc = bd_run_batch(sc, g, st);
     } else {
         // the scheduler's time mode: whole batches until the deadline, each
         // finished before the clock is read (scheduler.cpp:108,191-199)
         const auto ts = std::chrono::steady_clock::now();
         uint64_t b = 0;
         while (rc == LMX_OK && b < nbatch) {
-            if ((e = hipGraphLaunch(exec, st)) != hipSuccess || (e = hipStreamSynchronize(st)) != hipSuccess) {
+            if ((rc = bd_run_batch(sc, g, st)) != LMX_OK) break;
+            if ((e = hipStreamSynchronize(st)) != hipSuccess) {
                 rc = hip_error(e, "graph launch");
                 break;
             }
@@ -1896,7 +2134,6 @@ int render_bidir(lmx_scene* sc, const lmx_render_params* p, int variant, const c
             (e = hipStreamSynchronize(st)) != hipSuccess)
             rc = hip_error(e, (std::string(name) + ": sync").c_str());
     }
-    if (exec) (void)hipGraphExecDestroy(exec);
     if (own) (void)hipStreamDestroy(own);
     if (stats) {
         stats->rays = rc == LMX_OK ? sc->bd_hctl->rays + sc->bd_hctl->shadow_rays : 0;
@@ -2075,11 +2312,12 @@ int render_vcm(lmx_scene* sc, const lmx_render_params* p, const lmx_vcm_params* 
     pp.key = make_uint2((uint32_t)v->photon_seed, (uint32_t)(v->photon_seed >> 32));
     pp.table = v->photon_table; pp.table_width = v->photon_table_width;
     pp.sample_out = nullptr;
-    int rc = bd_buffers(sc, B, maxv, bd_conn_per_sample(maxv, BD_VCM, true, connect));
+    int rc = bd_buffers(sc, B, maxv, bd_conn_per_sample(maxv, BD_VCM, true, connect), true);
     if (rc == LMX_OK) rc = vc_buffers(sc, np, maxv, B);
     const size_t npix = (size_t)sc->width * (size_t)sc->height;
     const int zgrid = std::max(1, std::min((int)((3 * npix + 255) / 256), sc->cus * 8));
-    hipGraphExec_t gph = nullptr, geye = nullptr;
+    hipGraphExec_t gph = nullptr;
+    BGraphs geye;
     if (rc == LMX_OK) {
         k_bd_zero<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix);
         if (p->sample_out) {
@@ -2090,7 +2328,7 @@ int render_vcm(lmx_scene* sc, const lmx_render_params* p, const lmx_vcm_params* 
         k_bd_init<<<1, 64, 0, st>>>(sc->bd_ctl.p);
         rc = capture(st, &gph, [&] { return vc_enqueue_photons(sc, pp, st); });
     }
-    if (rc == LMX_OK) rc = capture(st, &geye, [&] { return bd_enqueue_batch(sc, bp, st); });
+    if (rc == LMX_OK) rc = bd_graph(sc, bp, st, geye);
     const uint64_t nbatch = ((uint64_t)Ne + B - 1) / B;
     double r = v->initial_radius;
     for (uint32_t q = 0; rc == LMX_OK && q < v->num_iteration_pass; q++) {
@@ -2100,8 +2338,7 @@ int render_vcm(lmx_scene* sc, const lmx_render_params* p, const lmx_vcm_params* 
         k_bd_phase<<<1, 64, 0, st>>>(sc->bd_ctl.p, q * np, rf, cell, cm);
         if ((e = hipGraphLaunch(gph, st)) != hipSuccess) { rc = hip_error(e, "graph launch"); break; }
         k_bd_phase<<<1, 64, 0, st>>>(sc->bd_ctl.p, q * Ne, rf, cell, cm);
-        for (uint64_t b = 0; rc == LMX_OK && b < nbatch; b++)
-            if ((e = hipGraphLaunch(geye, st)) != hipSuccess) rc = hip_error(e, "graph launch");
+        for (uint64_t b = 0; rc == LMX_OK && b < nbatch; b++) rc = bd_run_batch(sc, geye, st);
     }
     if (rc == LMX_OK) {
         k_bd_scale<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix,
@@ -2111,7 +2348,6 @@ int render_vcm(lmx_scene* sc, const lmx_render_params* p, const lmx_vcm_params* 
             rc = hip_error(e, "lmx_render_vcm: sync");
     }
     if (gph) (void)hipGraphExecDestroy(gph);
-    if (geye) (void)hipGraphExecDestroy(geye);
     if (own) (void)hipStreamDestroy(own);
     if (stats) {
         stats->rays = rc == LMX_OK ? sc->bd_hctl->rays + sc->bd_hctl->shadow_rays : 0;

@@ -148,38 +148,42 @@ __global__ void k_pack(const float* __restrict__ P, __bf16* __restrict__ out, fl
 }
 
 // Sums the per-workgroup gradient partials, then one Adam step
-// (bias-corrected).  A workgroup of 256 threads owns 64 parameters: its four
-// waves each sum every fourth partial (64 consecutive parameters per load,
-// four loads in flight per lane), and the first wave adds the four sums.
+// (bias-corrected).  A workgroup of 16 waves owns 64 consecutive
+// parameters: wave q sums the partials q, q + 16, q + 32, ... with eight
+// loads in flight per lane (a 128-partial sum is one round of loads), and
+// the first wave adds the 16 sums.  The reduction is latency-bound: with 4
+// waves and 4 loads in flight a 2^15-record step spent 31 us here.
 constexpr int ADAM_PARAMS = 64;
-__global__ __launch_bounds__(256) void k_adam(float* __restrict__ P, float* __restrict__ m, float* __restrict__ v,
-                                              const float* __restrict__ partial, int nparts, float scale, float lr,
-                                              float b1, float b2, float c1, float c2, float eps,
-                                              float* __restrict__ grad_out)
+constexpr int ADAM_WAVES = 16;
+__global__ __launch_bounds__(ADAM_WAVES * 64) void k_adam(float* __restrict__ P, float* __restrict__ m,
+                                                          float* __restrict__ v, const float* __restrict__ partial,
+                                                          int nparts, float scale, float lr, float b1, float b2,
+                                                          float c1, float c2, float eps, float* __restrict__ grad_out)
 {
-    __shared__ float red[4][ADAM_PARAMS];
+    __shared__ float red[ADAM_WAVES][ADAM_PARAMS];
     const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
     const int i = blockIdx.x * ADAM_PARAMS + lane;
-    float g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, g3 = 0.0f;
+    float g[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     if (i < GP_N) {
         int k = q;
-        for (; k + 12 < nparts; k += 16) {
-            g0 += partial[(size_t)k * GP_N + i];
-            g1 += partial[(size_t)(k + 4) * GP_N + i];
-            g2 += partial[(size_t)(k + 8) * GP_N + i];
-            g3 += partial[(size_t)(k + 12) * GP_N + i];
+        for (; k + 7 * ADAM_WAVES < nparts; k += 8 * ADAM_WAVES) {
+#pragma unroll
+            for (int u = 0; u < 8; u++) g[u] += partial[(size_t)(k + u * ADAM_WAVES) * GP_N + i];
         }
-        for (; k < nparts; k += 4) g0 += partial[(size_t)k * GP_N + i];
+        for (; k < nparts; k += ADAM_WAVES) g[0] += partial[(size_t)k * GP_N + i];
     }
-    red[q][lane] = (g0 + g1) + (g2 + g3);
+    red[q][lane] = ((g[0] + g[1]) + (g[2] + g[3])) + ((g[4] + g[5]) + (g[6] + g[7]));
     __syncthreads();
     if (q != 0 || i >= GP_N) return;
-    float g = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
-    g *= scale;
-    if (grad_out) grad_out[i] = g;
+    float h[4];
+#pragma unroll
+    for (int w = 0; w < 4; w++)
+        h[w] = (red[4 * w][lane] + red[4 * w + 1][lane]) + (red[4 * w + 2][lane] + red[4 * w + 3][lane]);
+    float gs = ((h[0] + h[1]) + (h[2] + h[3])) * scale;
+    if (grad_out) grad_out[i] = gs;
     if (lr == 0.0f) return;
-    const float mi = b1 * m[i] + (1.0f - b1) * g;
-    const float vi = b2 * v[i] + (1.0f - b2) * g * g;
+    const float mi = b1 * m[i] + (1.0f - b1) * gs;
+    const float vi = b2 * v[i] + (1.0f - b2) * gs * gs;
     m[i] = mi;
     v[i] = vi;
     P[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
@@ -747,15 +751,29 @@ __global__ __launch_bounds__(WG_WAVES * 64) void k_wgrad(const __bf16* __restric
     float bsum = 0.0f;                                   // thread t < 240: bias row t of D2..D5
     const uint32_t p_begin = blockIdx.x * chunk;
     const uint32_t p_end = min(npass, p_begin + chunk);
+    // software pipeline: the next pass block is loaded into registers while
+    // the MFMAs of the current one run (752 rows x 64 B = 12 x 16 B per thread)
+    constexpr int NPRE = (NROWS * 4 + WG_WAVES * 64 - 1) / (WG_WAVES * 64);
+    uint4 pre[NPRE];
+    auto fetch = [&](uint32_t pass) {
+        const uint4* src = reinterpret_cast<const uint4*>(act + (size_t)pass * NROWS * 32);
+#pragma unroll
+        for (int u = 0; u < NPRE; u++) {
+            const int t = (int)threadIdx.x + u * WG_WAVES * 64;
+            if (t < NROWS * 4) pre[u] = src[t];
+        }
+    };
+    if (p_begin < p_end) fetch(p_begin);
     for (uint32_t pass = p_begin; pass < p_end; pass++) {
         __syncthreads();
         // stage one pass block: 752 rows x 64 B, 16 B per thread per step
-        const uint4* src = reinterpret_cast<const uint4*>(act + (size_t)pass * NROWS * 32);
-        for (int t = threadIdx.x; t < NROWS * 4; t += blockDim.x) {
-            const int row = t >> 2, part = t & 3;
-            *reinterpret_cast<uint4*>(&st[row * SROW + part * 8]) = src[t];
+#pragma unroll
+        for (int u = 0; u < NPRE; u++) {
+            const int t = (int)threadIdx.x + u * WG_WAVES * 64;
+            if (t < NROWS * 4) *reinterpret_cast<uint4*>(&st[(t >> 2) * SROW + (t & 3) * 8]) = pre[u];
         }
         __syncthreads();
+        if (pass + 1 < p_end) fetch(pass + 1);
 #pragma unroll
         for (int i = 0; i < TILES_PER_WAVE; i++) {
             const WTile T = tile_of(wave * TILES_PER_WAVE + i);
@@ -1068,8 +1086,9 @@ int guide_train_step(lmx_guide* g, const float4* recs, const uint32_t* idx, uint
     }
     // forward/backward: one wave per 32-record pass, up to 8 waves per CU
     const int fb_wg = std::max(1, std::min((int)((npass + FB_WAVES - 1) / FB_WAVES), 2 * g->nparts));
-    // weight gradients: at least 8 passes per workgroup
-    const int nwg = std::max(1, std::min((int)((npass + 7) / 8), g->nparts));
+    // weight gradients: at least 4 passes per workgroup (a 2^15-record step
+    // fills the CUs with one workgroup each)
+    const int nwg = std::max(1, std::min((int)((npass + 3) / 4), g->nparts));
     const uint32_t chunk = (npass + nwg - 1) / nwg;
     k_fill<<<1, 64, 0, st>>>(g->loss.p, 1, 0.0f);
     k_train_fb<<<fb_wg, FB_WAVES * 64, 0, st>>>(g->dev(), recs, idx, n, wnorm, wclamp, 1.0f / (float)n, g->act.p,
@@ -1082,7 +1101,7 @@ int guide_train_step(lmx_guide* g, const float4* recs, const uint32_t* idx, uint
         c1 = 1.0f - powf(g->opts.beta1, (float)g->step);
         c2 = 1.0f - powf(g->opts.beta2, (float)g->step);
     }
-    k_adam<<<(GP_N + ADAM_PARAMS - 1) / ADAM_PARAMS, 256, 0, st>>>(g->P.p, g->m.p, g->v.p, g->partial.p, nwg, 1.0f, lr, g->opts.beta1,
+    k_adam<<<(GP_N + ADAM_PARAMS - 1) / ADAM_PARAMS, ADAM_WAVES * 64, 0, st>>>(g->P.p, g->m.p, g->v.p, g->partial.p, nwg, 1.0f, lr, g->opts.beta1,
                                                g->opts.beta2, c1, c2, g->opts.eps, grad_out);
     if ((e = hipGetLastError()) != hipSuccess) return hip_error(e, "guide train launch");
     return adam ? guide_pack(g, st) : LMX_OK;
@@ -1159,7 +1178,7 @@ void lmx_guide_free(lmx_guide* g) { delete g; }
 void lmx_guide_default_opts(lmx_guide_opts* o)
 {
     if (!o) return;
-    o->lr = 5e-3f;
+    o->lr = 1e-3f;
     o->beta1 = 0.9f;
     o->beta2 = 0.999f;
     o->eps = 1e-8f;

@@ -508,6 +508,9 @@ struct BCtl {
     uint32_t pad;
     unsigned long long dropped;   // vertex merging: merges whose contribution was not finite (reported)
     uint32_t cap_n[2];         // measurement hook: extension and shadow rays copied out so far
+    uint32_t conn_next;        // connection rounds: first sample of the batch not yet connected
+    uint32_t conn_lo, conn_hi; // connection rounds: the samples [lo, hi) of the current round
+    uint32_t conn_base;        // connection rounds: slot offset of sample lo
 };
 
 // The scene object behind the C ABI: accel, shading data on the device and
@@ -539,7 +542,8 @@ struct lmx_scene {
     Ctl* h_ctl = nullptr;      // pinned
     uint32_t tuned_cap = 0;
     // bidirectional sampler buffers (sized for bd_batch samples, bd_maxv vertices)
-    uint32_t bd_batch = 0, bd_maxv = 0, bd_conn = 0;   // bd_conn: shadow-queue slots per sample
+    uint32_t bd_batch = 0, bd_maxv = 0, bd_conn = 0;   // bd_conn: shadow-queue slots per sample and round
+    uint32_t bd_nconn = 0;              // shadow-queue slots of one connection round
     uint32_t bd_tuned_batch = 0, bd_tuned_key = 0;      // autotuned batch for one (variant, max_num_vertices)
     uint64_t bd_tuned_limit = 0;        // largest candidate the tuning run allowed (render size rounded up)
     DevBuf<float4> bd_vert[2];          // per subpath vertex: 4 float4 (see lmx_bdpt.hip)
@@ -552,6 +556,8 @@ struct lmx_scene {
     DevBuf<float4> bd_hits;
     DevBuf<float4> bd_sray, bd_sc, bd_saux, bd_shits;
     DevBuf<uint4> bd_smeta;
+    DevBuf<uint32_t> bd_off, bd_part;   // per-sample connection counts -> slot offsets, and scan partials
+    DevBuf<uint32_t> bd_dmask;          // bdpt: per-sample mask of the eye vertices with a direct strategy
     DevBuf<BCtl> bd_ctl;
     BCtl* bd_hctl = nullptr;      // pinned
     // vertex merging: photon subpaths and the hash grid over their vertices

@@ -106,21 +106,18 @@ WB_HD uint32_t wb_node_hits(const WNode& n, const WRayPre& r, float tmax, uint32
 #pragma unroll
 #endif
     for (int i = 0; i < 8; i++) {
-        const uint32_t meta = n.meta[i];
-        const bool internal = (n.imask >> i) & 1u;
-        if (!internal && meta == 0) continue;
+        // every slot is tested (no branch per slot); an empty slot (neither
+        // internal nor a leaf with triangles) contributes nothing
         const float x0 = fmaf((float)n.qlo[0][i], sx, bx), x1 = fmaf((float)n.qhi[0][i], sx, bx);
         const float y0 = fmaf((float)n.qlo[1][i], sy, by), y1 = fmaf((float)n.qhi[1][i], sy, by);
         const float z0 = fmaf((float)n.qlo[2][i], sz, bz), z1 = fmaf((float)n.qhi[2][i], sz, bz);
         const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
         const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
-        if (!(tn <= tf * WBVH_FAR_SCALE)) continue;
-        if (internal) {
-            inner |= 1u << (i ^ r.oct);
-        } else {
-            const uint32_t cnt = meta >> 5, off = meta & 31u;
-            tris |= ((1u << cnt) - 1u) << off;
-        }
+        const bool hit = tn <= tf * WBVH_FAR_SCALE;
+        const uint32_t meta = n.meta[i];
+        const bool internal = (n.imask >> i) & 1u;
+        inner |= (hit && internal) ? 1u << (i ^ r.oct) : 0u;
+        tris |= (hit && !internal) ? ((1u << (meta >> 5)) - 1u) << (meta & 31u) : 0u;
     }
     tri_mask = tris;
     return inner;

@@ -83,7 +83,22 @@ def test_bdpt_per_strategy_parity_philox(gpu):
     _check(so.cpu().numpy(), c, min_both=1000)
 
 
-@pytest.mark.parametrize("maxv", [3, 5])
+def test_bdpt_connection_rounds_parity(gpu):
+    """A closed box at max_num_vertices 12: nearly every sample holds all 67
+    of its connections, more than the 24 shadow-queue slots per sample of
+    one connection round, so each batch needs the graph's two rounds and
+    more from the host; every connection must still be made exactly once."""
+    import torch
+    sc = scenes.cornell_box(32, 32)
+    n, m = 32 * 32 * 2, 12
+    tab = np.random.default_rng(23).random((n, render.vcm_bdpt_table_width(m)), dtype=np.float32)
+    _, st, so = render.Scene(sc).render_vcm_bdpt(n, max_num_vertices=m, table=torch.from_numpy(tab).to(gpu),
+                                                  sample_out=True, batch=1024)
+    c = orc.Scene(sc).bdpt_samples(0, n, max_num_vertices=m, table=tab)
+    _check(so.cpu().numpy(), c, min_both=20000)
+
+
+@pytest.mark.parametrize("maxv", [3, 5, 12])
 def test_bdpt_furnace_closed_form(gpu, maxv):
     rho, Le = 0.5, 1.0
     sc = scenes.furnace_sphere(albedo=rho, Le=Le, subdiv=2, width=16, height=16)
