@@ -99,6 +99,19 @@ int lmx_accel_intersect(lmx_accel *accel, const lmx_ray *rays, uint32_t n, lmx_h
  * there (not necessarily the closest; t, b0, b1 belong to that triangle). */
 int lmx_accel_occluded(lmx_accel *accel, const lmx_ray *rays, uint32_t n, lmx_hit *hits, void *stream);
 
+/* lmx_accel_intersect without any device allocation, for callers that cannot
+ * allocate: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  No counterpart in the reference.  Same arguments, same
+ * hits, same kernel.  The accel must already be resident on the current
+ * device (lmx_accel_upload), else LMX_EINVAL.  The launch's scratch (region
+ * cursors and ray count, 2,304 bytes for any n; n up to 2^31 - 1, else
+ * LMX_EINVAL) is one static device array per GPU, held until the process
+ * ends and shared by every call on that GPU: calls that overlap on one GPU
+ * (different streams, or different threads) race and must be serialised by
+ * the caller.  Enqueued on `stream`, asynchronous. */
+int lmx_accel_intersect_noalloc(lmx_accel *accel, const lmx_ray *rays, uint32_t n, lmx_hit *hits,
+                                void *stream);
+
 /* The same query on the host (the scalar Accel3::Intersect path), walking
  * the same wide BVH in the same order as the kernel.  Host arrays. */
 int lmx_accel_intersect_host(const lmx_accel *accel, const lmx_ray *rays, uint32_t n,

@@ -25,8 +25,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -242,6 +244,7 @@ void k_trace(const WNode* __restrict__ nodes,
                                                       const WRay* __restrict__ rays,
                                                       WHit* __restrict__ hits,
                                                       uint32_t* __restrict__ cursors,
+                                                      uint32_t* __restrict__ overflow,
                                                       int stack_depth, uint32_t refill_min)
 {
     extern __shared__ uint2 stack[];     // [stack_depth][TRACE_WAVE]
@@ -350,7 +353,7 @@ void k_trace(const WNode* __restrict__ nodes,
                     } else {
                         // unreachable for a tree the builder accepts (the
                         // stack holds max_depth groups); flagged, never silent
-                        atomicOr(&cursors[9 * CURSOR_STRIDE], 1u);
+                        atomicOr(overflow, 1u);
                     }
                 }
             }
@@ -520,26 +523,28 @@ int lmx_accel_upload(lmx_accel* a)
 namespace lmx {
 
 // Enqueues one traversal launch of an accel already resident on the current
-// device, with its counter block at `counters` (LMX_TRACE_COUNTER_WORDS
-// words of device memory).
+// device, with its region cursors and ray count at `counters` (9 words,
+// CURSOR_STRIDE apart, of device memory); the stack-overflow flag is always
+// the accel's own.
 static int enqueue_trace(const lmx_accel* a, uint32_t* counters, const WRay* rays, uint32_t n,
                          const uint32_t* n_dev, uint32_t n_max, WHit* hits, hipStream_t s, bool anyhit)
 {
+    uint32_t* overflow = a->d_counters + 9 * CURSOR_STRIDE;
     const int grid = std::max(1, std::min(a->grid, (int)((n_max + TRACE_WAVE - 1) / TRACE_WAVE)));
     k_init_cursors<<<1, 64, 0, s>>>(counters, n, n_dev);
     const bool wt = a->host.isect == WBVH_ISECT_WATERTIGHT;
     if (wt && !anyhit)
         k_trace<WBVH_ISECT_WATERTIGHT, false><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
-            a->d_nodes, a->d_tris, rays, hits, counters, a->stack_depth, REFILL_MIN);
+            a->d_nodes, a->d_tris, rays, hits, counters, overflow, a->stack_depth, REFILL_MIN);
     else if (wt)
         k_trace<WBVH_ISECT_WATERTIGHT, true><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
-            a->d_nodes, a->d_tris, rays, hits, counters, a->stack_depth, REFILL_MIN);
+            a->d_nodes, a->d_tris, rays, hits, counters, overflow, a->stack_depth, REFILL_MIN);
     else if (!anyhit)
         k_trace<WBVH_ISECT_TRIACCEL, false><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
-            a->d_nodes, a->d_tris, rays, hits, counters, a->stack_depth, REFILL_MIN);
+            a->d_nodes, a->d_tris, rays, hits, counters, overflow, a->stack_depth, REFILL_MIN);
     else
         k_trace<WBVH_ISECT_TRIACCEL, true><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
-            a->d_nodes, a->d_tris, rays, hits, counters, a->stack_depth, REFILL_MIN);
+            a->d_nodes, a->d_tris, rays, hits, counters, overflow, a->stack_depth, REFILL_MIN);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "trace launch");
     return LMX_OK;
@@ -555,7 +560,75 @@ int launch_trace(lmx_accel* a, const WRay* rays, uint32_t n, const uint32_t* n_d
 
 }  // namespace lmx
 
+// ---------------------------------------------------------------------------
+// Allocation-free closest hit (lmx_accel_intersect_noalloc)
+//
+// The same launch as lmx_accel_intersect, with the launch's scratch -- the 8
+// region cursors and the ray count, each on a 256-byte line of its own,
+// 2,304 bytes, whatever the number of rays -- in this file-scope array
+// instead of the accel's counter block.  Every GPU the library runs on holds
+// its own copy until the process ends, shared by every call on that GPU;
+// k_init_cursors rewrites all 9 words at the start of each launch, so no
+// value carries over from the previous call (two calls that overlap on one
+// GPU, on different streams, race on it).  On the host, `scratch` only
+// stands in for the device symbol: its device address comes from
+// hipGetSymbolAddress, taken once per device with that device current, and
+// only that address is handed to kernels.
+#define LMX_NOALLOC_SCRATCH_WORDS (9 * CURSOR_STRIDE)
+static __device__ __align__(256) uint32_t scratch[LMX_NOALLOC_SCRATCH_WORDS];
+
+namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::atomic<uint32_t*> g_scratch_dev[SCRATCH_MAX_DEVICES];
+std::mutex g_scratch_mu;
+
+// Device address of `scratch` on device `dev`, which is current.
+int scratch_on(int dev, uint32_t** out)
+{
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES)
+        return set_error(LMX_EINVAL, "lmx_accel_intersect_noalloc: device index beyond 63");
+    uint32_t* p = g_scratch_dev[dev].load(std::memory_order_acquire);
+    if (!p) {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        p = g_scratch_dev[dev].load(std::memory_order_relaxed);
+        if (!p) {
+            void* q = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+            if (e != hipSuccess) return hip_error(e, "lmx_accel_intersect_noalloc: hipGetSymbolAddress(scratch)");
+            if (!q) return set_error(LMX_EHIP, "lmx_accel_intersect_noalloc: no device address for scratch");
+            p = static_cast<uint32_t*>(q);
+            g_scratch_dev[dev].store(p, std::memory_order_release);
+        }
+    }
+    *out = p;
+    return LMX_OK;
+}
+
+}  // namespace
+
 extern "C" {
+
+int lmx_accel_intersect_noalloc(lmx_accel* a, const lmx_ray* rays, uint32_t n, lmx_hit* hits, void* stream)
+{
+    static_assert(LMX_NOALLOC_SCRATCH_WORDS * sizeof(uint32_t) == 2304, "scratch size stated in lmx.h");
+    if (!a) return set_error(LMX_EINVAL, "lmx_accel_intersect_noalloc: NULL accel");
+    if (n == 0) return LMX_OK;
+    if (!rays || !hits) return set_error(LMX_EINVAL, "lmx_accel_intersect_noalloc: NULL ray or hit array");
+    if (n > 0x7fffffffu)
+        return set_error(LMX_EINVAL, "lmx_accel_intersect_noalloc: too many rays (at most 2^31 - 1 per call)");
+    int dev = 0;
+    const hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_error(e, "lmx_accel_intersect_noalloc: hipGetDevice");
+    if (a->device != dev || !a->d_nodes)
+        return set_error(LMX_EINVAL, "lmx_accel_intersect_noalloc: the accel is not resident on the current "
+                                     "device; call lmx_accel_upload first (this entry point allocates nothing)");
+    uint32_t* cur = nullptr;
+    const int rc = scratch_on(dev, &cur);
+    if (rc != LMX_OK) return rc;
+    return enqueue_trace(a, cur, reinterpret_cast<const WRay*>(rays), n, nullptr, n, reinterpret_cast<WHit*>(hits),
+                         (hipStream_t)stream, false);
+}
 
 int lmx_accel_intersect(lmx_accel* a, const lmx_ray* rays, uint32_t n, lmx_hit* hits, void* stream)
 {

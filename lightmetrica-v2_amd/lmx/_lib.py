@@ -117,6 +117,8 @@ SIGNATURES = {
                                            ctypes.c_void_p, ctypes.c_void_p]),
     "lmx_accel_occluded": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                           ctypes.c_void_p, ctypes.c_void_p]),
+    "lmx_accel_intersect_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                                   ctypes.c_void_p, ctypes.c_void_p]),
     "lmx_accel_intersect_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                                 ctypes.c_void_p, ctypes.c_int32]),
     "lmx_scene_create": (ctypes.c_int, [ctypes.POINTER(lmx_scene_desc), ctypes.POINTER(ctypes.c_void_p)]),

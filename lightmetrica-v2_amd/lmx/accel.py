@@ -91,6 +91,21 @@ class Accel:
                                       ctypes.c_void_p(hits.data_ptr()), ctypes.c_void_p(stream.cuda_stream)))
         return hits
 
+    def intersect_noalloc(self, rays, hits, stream=None):
+        """lmx_accel_intersect_noalloc: the same query with no device
+        allocation; the accel must have been uploaded and `hits` (n, 4) is
+        the caller's."""
+        import torch
+        assert rays.is_cuda and rays.dtype == torch.float32 and rays.is_contiguous()
+        assert rays.dim() == 2 and rays.shape[1] == 8
+        n = rays.shape[0]
+        assert hits.is_cuda and hits.dtype == torch.float32 and hits.is_contiguous() and hits.shape == (n, 4)
+        if stream is None:
+            stream = torch.cuda.current_stream(rays.device)
+        check(lib.lmx_accel_intersect_noalloc(self._h, ctypes.c_void_p(rays.data_ptr()), n,
+                                              ctypes.c_void_p(hits.data_ptr()), ctypes.c_void_p(stream.cuda_stream)))
+        return hits
+
     def occluded(self, rays, hits=None, stream=None):
         """GPU any-hit query: returns (and fills) a torch float32 tensor
         (n, 4) whose id column is MISS exactly for unoccluded rays."""
