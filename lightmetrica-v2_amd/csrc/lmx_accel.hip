@@ -266,6 +266,13 @@ void k_trace(const WNode* __restrict__ nodes,
     uint32_t best_id = WBVH_MISS;
     uint32_t g_base = 0, g_hits = 0, g_imask = 0;
     int sp = 0;
+    // Triangles of the last tested node still to test (bit i: triangle
+    // tbase + i), one per iteration, and whether a next node was chosen:
+    // a lane tests its node's triangles over the following iterations while
+    // the other lanes of the wave go on testing nodes, instead of the whole
+    // wave looping as many times as its lane with the most triangles.
+    uint32_t pend = 0, tbase = 0, cur = 0;
+    bool more = false;
     WNode nd;                           // node to test in the next iteration
     load_node(nd, nodes);               // idle lanes always hold the root
 
@@ -313,6 +320,7 @@ void k_trace(const WNode* __restrict__ nodes,
                 wb_ray_setup(r, a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w);
                 best_t = b.w; best_id = WBVH_MISS; h0 = h1 = 0.0f; hdet = 1.0f;
                 sp = 0;                        // nd already holds the root
+                pend = 0; more = true;
                 valid = true;
             }
         }
@@ -320,18 +328,19 @@ void k_trace(const WNode* __restrict__ nodes,
         // body below does nothing but reload the root
         if (__ballot(valid) == 0 && regions_left == 0 && pool_cnt == 0) break;
 
-        // 1. box test of the node loaded in the previous iteration, and the
-        //    choice of the next node.  Every lane then loads exactly one node
-        //    per iteration at a single site (lanes without a next node reload
-        //    the root, which is what a refilled lane needs first), so the
-        //    node registers are carried around the loop without copies.
-        uint32_t tmask = 0, tbase = 0, child = 0;
-        bool more = false;
-        if (valid && !(ANYHIT && best_id != WBVH_MISS)) {
+        // 1. lanes with no triangle pending: box test of the node loaded in
+        //    the previous iteration and the choice of the next node; lanes
+        //    with triangles pending keep their node (it is loaded again).
+        //    Every lane loads exactly one node per iteration at one site
+        //    (lanes without a next node load the root, which is what a
+        //    refilled lane needs first).
+        uint32_t child = 0;
+        if (valid && pend == 0 && more && !(ANYHIT && best_id != WBVH_MISS)) {
+            uint32_t tmask = 0;
             const uint32_t inner = wb_node_hits(nd, r, best_t, tmask);
+            pend = tmask;
             tbase = nd.tri_base;
             g_base = nd.child_base; g_hits = inner; g_imask = nd.imask;
-            more = true;
             if (g_hits == 0) {
                 if (sp == 0) {
                     more = false;
@@ -357,35 +366,26 @@ void k_trace(const WNode* __restrict__ nodes,
                     }
                 }
             }
+        } else if (valid && pend != 0 && more) {
+            child = cur;                       // the node chosen before these triangles
         }
-        // 2. start loading the next node; its latency overlaps the
-        //    triangle tests below
+        // 2. load the next node; its latency overlaps the triangle test below
+        cur = child;
         load_node(nd, nodes + child);
         if (valid) {
-            // 3. triangles of the tested node
-            while (tmask) {
-                const uint32_t k1 = (uint32_t)(__ffs(tmask) - 1);
-                tmask &= tmask - 1u;
-#if LMX_TRI_BATCH == 2
-                const bool two = tmask != 0u;
-                const uint32_t k2 = two ? (uint32_t)(__ffs(tmask) - 1) : k1;
-                if (two) tmask &= tmask - 1u;
-                WTri t1, t2;
-                load_tri(t1, tris + tbase + k1);
-                load_tri(t2, tris + tbase + k2);
-                test_tri<ISECT>(t1, r, best_t, best_id, h0, h1, hdet);
-                if (two) test_tri<ISECT>(t2, r, best_t, best_id, h0, h1, hdet);
-#else
+            // 3. one triangle of the tested node
+            if (pend) {
+                const uint32_t k1 = (uint32_t)(__ffs(pend) - 1);
+                pend &= pend - 1u;
                 WTri t1;
                 load_tri(t1, tris + tbase + k1);
                 test_tri<ISECT>(t1, r, best_t, best_id, h0, h1, hdet);
-#endif
+                if (ANYHIT && best_id != WBVH_MISS) { pend = 0; more = false; }
             }
             // 4. ray finished: write its hit and free the lane.  An any-hit
-            //    ray that found a triangle stops one iteration later, after
-            //    the root has been loaded for its lane (the node loaded this
-            //    iteration is not the root).
-            if (!more) {
+            //    ray that found a triangle with a next node already loaded
+            //    stops one iteration later, after the root has been loaded.
+            if (pend == 0 && !more && cur == 0) {
                 float4 o;
                 if (best_id == WBVH_MISS) {
                     o = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(WBVH_MISS));

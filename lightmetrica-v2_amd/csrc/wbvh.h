@@ -6,8 +6,11 @@
 //  src/liblightmetrica/accel/accel_qbvh.cpp:54-137).
 //
 // Node: 128 bytes = two 64-byte blocks, 128-byte aligned, so one node is one
-// L2 line on CDNA.  Child boxes are quantised to 16 bits per axis relative to
-// the node origin with a power-of-two scale per axis, rounded outward.  Up to
+// L2 line on CDNA.  Child boxes are quantised per axis to a grid of 2047
+// steps from the node origin, the step a power of two, rounded outward; the
+// grid coordinates (integers 0..2047, exact in binary16) are stored as
+// half-precision values, which the box test's FMAs take as operands
+// directly (v_fma_mix_f32), with no integer-to-float conversion.  Up to
 // eight children per node; internal children of a node are stored
 // contiguously (child_base + rank), and the triangles of all leaf children of
 // a node are stored contiguously (tri_base + offset).  Each leaf holds 1..3
@@ -24,6 +27,7 @@
 #define WBVH_WIDTH 8
 #define WBVH_MAX_LEAF_TRIS 3
 #define WBVH_STACK 32         // traversal stack entries per ray (node groups)
+#define WBVH_QMAX 2047        // quantisation grid steps per axis (exact in binary16)
 
 // Triangle test selector (stored in the accel; picks the kernel variant).
 #define WBVH_ISECT_TRIACCEL 0     // Wald projection test, reference semantics
@@ -36,8 +40,8 @@ struct alignas(128) WNode {
     uint32_t child_base;       // index of the first internal child
     uint32_t tri_base;         // index of the first triangle of leaf children
     uint8_t meta[8];           // leaf slot: (count << 5) | offset; empty: 0
-    uint16_t qlo[3][8];        // quantised child lower bounds, per axis
-    uint16_t qhi[3][8];        // quantised child upper bounds, per axis
+    _Float16 qlo[3][8];        // quantised child lower bounds, per axis (grid units)
+    _Float16 qhi[3][8];        // quantised child upper bounds, per axis (grid units)
 };
 static_assert(sizeof(WNode) == 128, "WNode must be 128 bytes");
 

@@ -314,8 +314,8 @@ const char* wbvh_build(const float* verts, uint32_t nvert, const uint32_t* tris,
             const double ext = (double)hi[a] - (double)lo[a];
             int e = -100;
             if (ext > 0.0) {
-                e = (int)std::ceil(std::log2(ext / 65535.0));
-                while (std::ldexp(65535.0, e) < ext) e++;
+                e = (int)std::ceil(std::log2(ext / (double)WBVH_QMAX));
+                while (std::ldexp((double)WBVH_QMAX, e) < ext) e++;
                 if (e < -100) e = -100;
             }
             if (e > 127) return "scene extent too large to quantise";
@@ -337,10 +337,10 @@ const char* wbvh_build(const float* verts, uint32_t nvert, const uint32_t* tris,
                 const double sc = std::ldexp(1.0, ex[a]);
                 double ql = std::floor(((double)c.lo[a] - (double)org[a]) / sc);
                 double qh = std::ceil(((double)c.hi[a] - (double)org[a]) / sc);
-                ql = std::max(0.0, std::min(65535.0, ql));
-                qh = std::max(0.0, std::min(65535.0, qh));
-                wn.qlo[a][s] = (uint16_t)ql;
-                wn.qhi[a][s] = (uint16_t)qh;
+                ql = std::max(0.0, std::min((double)WBVH_QMAX, ql));
+                qh = std::max(0.0, std::min((double)WBVH_QMAX, qh));
+                wn.qlo[a][s] = (_Float16)ql;
+                wn.qhi[a][s] = (_Float16)qh;
             }
             if (c.left >= 0) {
                 wn.imask |= (uint8_t)(1u << s);

@@ -47,6 +47,9 @@ WB_HD int wb_ctz(uint32_t x)
 #endif
 }
 
+// two binary16 grid coordinates (children 2w and 2w + 1 of one axis)
+typedef _Float16 wb_h2 __attribute__((ext_vector_type(2)));
+
 struct WRayPre {
     float ox, oy, oz, tmin;
     float dx, dy, dz, tmax;
@@ -101,6 +104,24 @@ WB_HD uint32_t wb_node_hits(const WNode& n, const WRayPre& r, float tmax, uint32
     const float bx = (n.ox - r.ox) * r.ix;
     const float by = (n.oy - r.oy) * r.iy;
     const float bz = (n.oz - r.oz) * r.iz;
+    // Near and far planes per axis chosen once per node by the sign of the
+    // slab scale (for lo <= hi, fma(lo, s, b) <= fma(hi, s, b) exactly when
+    // s >= 0), instead of a min and a max per plane pair and child.
+    wb_h2 nq[3][4], fq[3][4];
+    const wb_h2* qlo = reinterpret_cast<const wb_h2*>(&n.qlo[0][0]);
+    const wb_h2* qhi = reinterpret_cast<const wb_h2*>(&n.qhi[0][0]);
+    const bool neg[3] = {sx < 0.0f, sy < 0.0f, sz < 0.0f};
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (int a = 0; a < 3; a++)
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+        for (int w = 0; w < 4; w++) {
+            nq[a][w] = neg[a] ? qhi[4 * a + w] : qlo[4 * a + w];
+            fq[a][w] = neg[a] ? qlo[4 * a + w] : qhi[4 * a + w];
+        }
     uint32_t inner = 0, tris = 0;
 #if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll
@@ -108,11 +129,16 @@ WB_HD uint32_t wb_node_hits(const WNode& n, const WRayPre& r, float tmax, uint32
     for (int i = 0; i < 8; i++) {
         // every slot is tested (no branch per slot); an empty slot (neither
         // internal nor a leaf with triangles) contributes nothing
-        const float x0 = fmaf((float)n.qlo[0][i], sx, bx), x1 = fmaf((float)n.qhi[0][i], sx, bx);
-        const float y0 = fmaf((float)n.qlo[1][i], sy, by), y1 = fmaf((float)n.qhi[1][i], sy, by);
-        const float z0 = fmaf((float)n.qlo[2][i], sz, bz), z1 = fmaf((float)n.qhi[2][i], sz, bz);
-        const float tn = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), r.tmin));
-        const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+        const int w = i >> 1;
+        const bool odd = (i & 1) != 0;
+        const float xn = fmaf((float)(odd ? nq[0][w].y : nq[0][w].x), sx, bx);
+        const float xf = fmaf((float)(odd ? fq[0][w].y : fq[0][w].x), sx, bx);
+        const float yn = fmaf((float)(odd ? nq[1][w].y : nq[1][w].x), sy, by);
+        const float yf = fmaf((float)(odd ? fq[1][w].y : fq[1][w].x), sy, by);
+        const float zn = fmaf((float)(odd ? nq[2][w].y : nq[2][w].x), sz, bz);
+        const float zf = fmaf((float)(odd ? fq[2][w].y : fq[2][w].x), sz, bz);
+        const float tn = fmaxf(fmaxf(xn, yn), fmaxf(zn, r.tmin));
+        const float tf = fminf(fminf(xf, yf), fminf(zf, tmax));
         const bool hit = tn <= tf * WBVH_FAR_SCALE;
         const uint32_t meta = n.meta[i];
         const bool internal = (n.imask >> i) & 1u;

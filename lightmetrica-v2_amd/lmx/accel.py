@@ -25,7 +25,7 @@ FLT_MAX = np.float32(np.finfo(np.float32).max)
 WNODE_DTYPE = np.dtype([
     ("o", np.float32, 3), ("e", np.uint8, 3), ("imask", np.uint8),
     ("child_base", np.uint32), ("tri_base", np.uint32), ("meta", np.uint8, 8),
-    ("qlo", np.uint16, (3, 8)), ("qhi", np.uint16, (3, 8))])
+    ("qlo", np.float16, (3, 8)), ("qhi", np.float16, (3, 8))])
 assert WNODE_DTYPE.itemsize == 128
 
 
