@@ -26,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "lightmetrica-v2_amd"))
 
 HBM_PEAK_GBS = 8000.0
+VALU_CLOCK_HZ = 2.4e9        # MI355X peak engine clock
 NODE_BYTES = 128
 TRI_BYTES = 48
 RAY_BYTES = 32 + 16
@@ -88,30 +89,23 @@ def oracle_native():
     return "-O3 -march=native -ffp-contract=off"
 
 
-def traffic_from_profiles():
-    """HBM bytes per traversal launch measured with rocprofv3 PMC counters,
-    if a summary with a `traffic_bytes_per_launch:` line is committed."""
+def traversal_profile():
+    """Figures of the committed rocprofv3 counter summary of the traversal
+    kernel (profiles/traversal_traffic.txt: `key: value` lines): HBM-side
+    bytes per launch from FETCH_SIZE / WRITE_SIZE, the L2 hit rate, VALU
+    instructions per launch and the profiled kernel time."""
     path = os.path.join(ROOT, "profiles", "traversal_traffic.txt")
-    if not os.path.exists(path):
-        return None
-    with open(path) as fh:
-        for line in fh:
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":", 1)[1])
-    return None
-
-
-def l2_from_profiles():
-    """L2 (TCC) hit rate of the traversal kernel from a committed
-    rocprofv3 counter summary with an `l2_hit_rate:` line, if present."""
-    path = os.path.join(ROOT, "profiles", "traversal_traffic.txt")
-    if not os.path.exists(path):
-        return None
-    with open(path) as fh:
-        for line in fh:
-            if line.startswith("l2_hit_rate:"):
-                return float(line.split(":", 1)[1])
-    return None
+    out = {}
+    if os.path.exists(path):
+        with open(path) as fh:
+            for line in fh:
+                if not line.startswith("#") and ":" in line:
+                    k, v = line.split(":", 1)
+                    try:
+                        out[k.strip()] = float(v)
+                    except ValueError:
+                        pass
+    return out
 
 
 def cpu_baseline(verts, tris, rays, threads, reps=3):
@@ -929,8 +923,10 @@ def main():
     nodes_v, tris_t = a.count(rays)
     alg_bytes = RAY_BYTES * nray + NODE_BYTES * nodes_v + TRI_BYTES * tris_t
     achieved = alg_bytes / avg_kernel_s / 1e9
-    traffic = traffic_from_profiles()
-    l2 = l2_from_profiles()
+    prof = traversal_profile()
+    traffic = prof.get("traffic_bytes_per_launch")
+    l2 = prof.get("l2_hit_rate")
+    valu = prof.get("valu_insts_per_launch")
 
     hits_np = d_hits.cpu().numpy()
     hit_frac = float((accel.hit_ids(hits_np) != accel.MISS).mean())
@@ -967,13 +963,18 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this benchmark, scaled by a copy "
-                              "kernel's counts (profiles/traversal_traffic.txt); not measured in this run",
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this benchmark's command, scaled by a "
+                              "copy kernel's counts (profiles/traversal_traffic.txt, the committed profile run; "
+                              "the L2 hit rate and VALU count come from the same runs); not measured in this run",
             "dram_gbs": None if traffic is None else round(traffic / avg_kernel_s / 1e9, 1),
             "dram_frac": None if traffic is None else round(traffic / avg_kernel_s / 1e9 / HBM_PEAK_GBS, 4),
             "l2_hit_rate": l2,
-            "limiter": "node-fetch latency: the nodes and triangles (98 MB) stay mostly in L2 and the Infinity "
-                       "Cache, so DRAM traffic is a fraction of the algorithmic bytes",
+            "valu_insts_per_launch": valu,
+            "valu_issue_frac": None if valu is None else round(valu * 4 / 1024 / (avg_kernel_s * VALU_CLOCK_HZ), 4),
+            "limiter": "VALU issue: the wave-instruction count of the profiled launch (4 cycles each on one of "
+                       "1,024 SIMDs) against the launch's cycles at 2.4 GHz; the nodes and triangles (98 MB) are "
+                       "served mostly by L2 and the Infinity Cache, so fabric traffic is a fraction of the "
+                       "algorithmic bytes, whose rate exceeds nothing physical",
             "alg_bytes_per_launch": int(alg_bytes),
             "node_visits_per_ray": round(nodes_v / nray, 3),
             "tri_tests_per_ray": round(tris_t / nray, 3),

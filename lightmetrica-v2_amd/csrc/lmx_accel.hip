@@ -118,7 +118,7 @@ static bool walk_impl(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* n
         while (tmask) {
             const uint32_t k = (uint32_t)wb_ctz(tmask);
             tmask &= tmask - 1u;
-            const WTri& tri = b.tris[nd.tri_base + k];
+            const WTri& tri = b.tris[wb_tri_index(nd.tri_base, nd.tvalid, k)];
             const uint32_t id = wb_tri_id(tri, ISECT);
             tests++;
             float t, a0, a1, dt = 1.0f;
@@ -271,7 +271,7 @@ void k_trace(const WNode* __restrict__ nodes,
     // a lane tests its node's triangles over the following iterations while
     // the other lanes of the wave go on testing nodes, instead of the whole
     // wave looping as many times as its lane with the most triangles.
-    uint32_t pend = 0, tbase = 0, cur = 0;
+    uint32_t pend = 0, tbase = 0, tvalid = 0, cur = 0;
     bool more = false;
     WNode nd;                           // node to test in the next iteration
     load_node(nd, nodes);               // idle lanes always hold the root
@@ -340,6 +340,7 @@ void k_trace(const WNode* __restrict__ nodes,
             const uint32_t inner = wb_node_hits(nd, r, best_t, tmask);
             pend = tmask;
             tbase = nd.tri_base;
+            tvalid = nd.tvalid;
             g_base = nd.child_base; g_hits = inner; g_imask = nd.imask;
             if (g_hits == 0) {
                 if (sp == 0) {
@@ -378,7 +379,7 @@ void k_trace(const WNode* __restrict__ nodes,
                 const uint32_t k1 = (uint32_t)(__ffs(pend) - 1);
                 pend &= pend - 1u;
                 WTri t1;
-                load_tri(t1, tris + tbase + k1);
+                load_tri(t1, tris + wb_tri_index(tbase, tvalid, k1));
                 test_tri<ISECT>(t1, r, best_t, best_id, h0, h1, hdet);
                 if (ANYHIT && best_id != WBVH_MISS) { pend = 0; more = false; }
             }

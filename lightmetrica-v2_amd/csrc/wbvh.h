@@ -13,8 +13,11 @@
 // directly (v_fma_mix_f32), with no integer-to-float conversion.  Up to
 // eight children per node; internal children of a node are stored
 // contiguously (child_base + rank), and the triangles of all leaf children of
-// a node are stored contiguously (tri_base + offset).  Each leaf holds 1..3
-// triangles.
+// a node are stored contiguously in slot order.  Each leaf holds 1..3
+// triangles; leaf slot s owns bits 3 s .. 3 s + 2 of a 24-bit triangle mask,
+// tvalid marks the bits in use, and the triangle of bit b is stored at
+// tri_base + popcount(tvalid below b), so a node's hit leaves become a
+// triangle mask by spreading the 8 hit bits to 3-bit groups.
 //
 // Child slots are assigned per node by octant (after Ylitie, Karras and
 // Laine, "Efficient Incoherent Ray Traversal on GPUs Through Compressed Wide
@@ -39,7 +42,8 @@ struct alignas(128) WNode {
     uint8_t imask;             // bit s set: slot s is an internal child
     uint32_t child_base;       // index of the first internal child
     uint32_t tri_base;         // index of the first triangle of leaf children
-    uint8_t meta[8];           // leaf slot: (count << 5) | offset; empty: 0
+    uint32_t tvalid;           // leaf triangles: bit 3 s + k set for the k-th triangle of leaf slot s
+    uint32_t reserved;
     _Float16 qlo[3][8];        // quantised child lower bounds, per axis (grid units)
     _Float16 qhi[3][8];        // quantised child upper bounds, per axis (grid units)
 };

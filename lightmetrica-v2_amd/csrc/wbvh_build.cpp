@@ -328,7 +328,6 @@ const char* wbvh_build(const float* verts, uint32_t nvert, const uint32_t* tris,
         // leaf triangles are appended contiguously
         wn.child_base = (uint32_t)queue.size();
         wn.tri_base = (uint32_t)out.tris.size();
-        uint32_t tri_off = 0;
         for (int s = 0; s < 8; s++) {
             const int i = kid_in_slot[s];
             if (i < 0) continue;
@@ -347,7 +346,7 @@ const char* wbvh_build(const float* verts, uint32_t nvert, const uint32_t* tris,
                 queue.push_back({kids[i], pd.depth + 1});
                 out.nodes.emplace_back();
             } else {
-                wn.meta[s] = (uint8_t)((c.count << 5) | tri_off);
+                wn.tvalid |= ((1u << c.count) - 1u) << (3 * s);
                 for (uint32_t k = 0; k < c.count; k++) {
                     const uint32_t t = idx[c.first + k];
                     const float* p0 = &verts[3 * tris[3 * t]];
@@ -355,7 +354,6 @@ const char* wbvh_build(const float* verts, uint32_t nvert, const uint32_t* tris,
                     const float* p2 = &verts[3 * tris[3 * t + 2]];
                     out.tris.push_back(make_tri(p0, p1, p2, t, isect));
                 }
-                tri_off += c.count;
             }
         }
         out.nodes[wi] = wn;

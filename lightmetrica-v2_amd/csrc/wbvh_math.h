@@ -95,7 +95,7 @@ WB_HD void wb_ray_setup(WRayPre& r, float ox, float oy, float oz, float tmin,
 // Box test of all eight child slots of a node against [tmin, tmax].
 // Returns the hit mask of internal children permuted by the ray octant
 // (bit p <-> slot p ^ oct) and, through tri_mask, the triangles of hit leaf
-// children (bit i <-> triangle tri_base + i).
+// children (bit b <-> triangle wb_tri_index(node, b)).
 WB_HD uint32_t wb_node_hits(const WNode& n, const WRayPre& r, float tmax, uint32_t& tri_mask)
 {
     const float sx = wb_as_float((uint32_t)n.ex << 23) * r.ix;
@@ -122,13 +122,13 @@ WB_HD uint32_t wb_node_hits(const WNode& n, const WRayPre& r, float tmax, uint32
             nq[a][w] = neg[a] ? qhi[4 * a + w] : qlo[4 * a + w];
             fq[a][w] = neg[a] ? qlo[4 * a + w] : qhi[4 * a + w];
         }
-    uint32_t inner = 0, tris = 0;
+    uint32_t hb = 0;                   // hit slots, bit i <-> slot i
 #if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll
 #endif
     for (int i = 0; i < 8; i++) {
-        // every slot is tested (no branch per slot); an empty slot (neither
-        // internal nor a leaf with triangles) contributes nothing
+        // every slot is tested (no branch per slot); empty slots are masked
+        // out below (neither internal nor holding triangles)
         const int w = i >> 1;
         const bool odd = (i & 1) != 0;
         const float xn = fmaf((float)(odd ? nq[0][w].y : nq[0][w].x), sx, bx);
@@ -139,14 +139,31 @@ WB_HD uint32_t wb_node_hits(const WNode& n, const WRayPre& r, float tmax, uint32
         const float zf = fmaf((float)(odd ? fq[2][w].y : fq[2][w].x), sz, bz);
         const float tn = fmaxf(fmaxf(xn, yn), fmaxf(zn, r.tmin));
         const float tf = fminf(fminf(xf, yf), fminf(zf, tmax));
-        const bool hit = tn <= tf * WBVH_FAR_SCALE;
-        const uint32_t meta = n.meta[i];
-        const bool internal = (n.imask >> i) & 1u;
-        inner |= (hit && internal) ? 1u << (i ^ r.oct) : 0u;
-        tris |= (hit && !internal) ? ((1u << (meta >> 5)) - 1u) << (meta & 31u) : 0u;
+        hb |= (tn <= tf * WBVH_FAR_SCALE) ? 1u << i : 0u;
     }
+    // internal children in traversal order: bit p <-> slot p ^ octant, by
+    // swapping bits, bit pairs and nibbles as the octant's bits say
+    uint32_t inner = hb & n.imask;
+    if (r.oct & 1u) inner = ((inner & 0x55u) << 1) | ((inner >> 1) & 0x55u);
+    if (r.oct & 2u) inner = ((inner & 0x33u) << 2) | ((inner >> 2) & 0x33u);
+    if (r.oct & 4u) inner = ((inner & 0x0fu) << 4) | ((inner >> 4) & 0x0fu);
+    // leaf slots hit -> their triangle bits: slot bit i spread to bits
+    // 3 i .. 3 i + 2, restricted to the triangles present
+    uint32_t leaf = hb & ~(uint32_t)n.imask;
+    leaf = (leaf | (leaf << 8)) & 0x00f00fu;
+    leaf = (leaf | (leaf << 4)) & 0x0c30c3u;
+    leaf = (leaf | (leaf << 2)) & 0x249249u;
+    const uint32_t tris = (leaf * 7u) & n.tvalid;
     tri_mask = tris;
     return inner;
+}
+
+// Triangle of bit b of a node's triangle mask: the node's leaf triangles are
+// stored contiguously in slot order, so it is preceded by the valid bits
+// below b.
+WB_HD uint32_t wb_tri_index(uint32_t tri_base, uint32_t tvalid, uint32_t b)
+{
+    return tri_base + (uint32_t)wb_popc(tvalid & ((1u << b) - 1u));
 }
 
 // Kahan's difference of products a*b - c*d with two FMAs: exactly zero iff

@@ -24,7 +24,7 @@ FLT_MAX = np.float32(np.finfo(np.float32).max)
 # struct WNode (csrc/wbvh.h), 128 bytes
 WNODE_DTYPE = np.dtype([
     ("o", np.float32, 3), ("e", np.uint8, 3), ("imask", np.uint8),
-    ("child_base", np.uint32), ("tri_base", np.uint32), ("meta", np.uint8, 8),
+    ("child_base", np.uint32), ("tri_base", np.uint32), ("tvalid", np.uint32), ("reserved", np.uint32),
     ("qlo", np.float16, (3, 8)), ("qhi", np.float16, (3, 8))])
 assert WNODE_DTYPE.itemsize == 128
 

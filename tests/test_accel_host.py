@@ -153,16 +153,17 @@ def test_bvh_structure_invariants(name):
         rank = 0
         for s in range(8):
             internal = (node["imask"] >> s) & 1
-            meta = node["meta"][s]
+            tv = int(node["tvalid"])
+            c = bin((tv >> (3 * s)) & 7).count("1")                 # triangles of leaf slot s
+            off = bin(tv & ((1 << (3 * s)) - 1)).count("1")        # triangles of the leaf slots before it
             if internal:
                 child = node["child_base"] + rank
                 rank += 1
                 assert child > ni
                 seen_children.append(child)
                 glo, ghi = sub_lo[child], sub_hi[child]
-            elif meta:
-                c, off = meta >> 5, meta & 31
-                assert 1 <= c <= 3
+            elif c:
+                assert (tv >> (3 * s)) & 7 in (1, 3, 7)                 # the first c bits of the group
                 p = tris[node["tri_base"] + off: node["tri_base"] + off + c].reshape(-1, 3, 4)[:, :, 0:3]
                 p = p.reshape(-1, 3).astype(np.float64)
                 glo, ghi = p.min(axis=0), p.max(axis=0)
