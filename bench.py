@@ -949,7 +949,7 @@ def main():
             "triangles": int(tris.shape[0]),
             "rays": int(nray),
             "isect": "watertight",
-            "bvh": "8-wide, 16-bit quantised, 128 B nodes",
+            "bvh": "8-wide, child boxes on a 2047-step binary16 grid, 128 B nodes",
             "bvh_nodes": info["num_nodes"],
             "bvh_depth": info["max_depth"],
             "bvh_build_s": round(build_s, 3),
@@ -975,6 +975,7 @@ def main():
                        "1,024 SIMDs) against the launch's cycles at 2.4 GHz; the nodes and triangles (98 MB) are "
                        "served mostly by L2 and the Infinity Cache, so fabric traffic is a fraction of the "
                        "algorithmic bytes, whose rate exceeds nothing physical",
+            "above_hbm_peak_by_cache_reuse": bool(achieved > HBM_PEAK_GBS),
             "alg_bytes_per_launch": int(alg_bytes),
             "node_visits_per_ray": round(nodes_v / nray, 3),
             "tri_tests_per_ray": round(tris_t / nray, 3),
