@@ -10,8 +10,8 @@
 //      subtree over primitive range [b,e) owns node slots [2b, 2e-1) and its
 //      root sits at slot 2m-1 for split point m), so the result does not
 //      depend on thread timing;
-//   2. greedy collapse into 8-wide nodes (repeatedly open the internal child
-//      with the largest surface area);
+//   2. collapse into 8-wide nodes by the cost-optimal dynamic program over
+//      the binary tree (Ylitie, Karras and Laine, HPG 2017);
 //   3. breadth-first layout (internal children of a node contiguous, leaf
 //      triangles of a node contiguous), octant slot assignment, 16-bit
 //      outward quantisation of child boxes.
@@ -173,6 +173,91 @@ inline float bn_area(const BNode& n)
     return 2.0f * (dx * dy + dy * dz + dz * dx);
 }
 
+// Cost-optimal collapse of the binary tree into 8-wide nodes (the dynamic
+// program of Ylitie, Karras and Laine, HPG 2017, with the binary leaves kept
+// as they are).  cost[n][i-1] is the least surface-area cost of the subtree
+// at n represented as a forest of at most i trees, a tree being a binary
+// leaf (area x triangles x kTri) or a wide node (area x kNode plus the best
+// forest of at most 8 trees for its slots).  Because a forest of at most i
+// trees is also one of at most i + 1, the cost of splitting i slots between
+// the two children is best with all i of them, so choice[n][i-1] is 0 when
+// the subtree is best kept as one tree, else the number of the i slots its
+// left child gets.  kNode / kTri is the cost of one wide-node visit (8 slot
+// boxes) over one triangle test in the traversal kernel.
+struct Collapse {
+    static constexpr float kNode = 3.0f, kTri = 1.0f;
+    const std::vector<BNode>& bn;
+    std::vector<float> cost;       // [node][8]
+    std::vector<uint8_t> choice;   // [node][8]
+    std::vector<uint8_t> split8;   // [node]: left slots of the node's own 8
+
+    explicit Collapse(const std::vector<BNode>& b)
+        : bn(b), cost(b.size() * 8, 0.0f), choice(b.size() * 8, 0), split8(b.size(), 1) {}
+
+    void run(int32_t root)
+    {
+        // post-order over the reachable nodes
+        std::vector<std::pair<int32_t, bool>> st;
+        st.push_back({root, false});
+        while (!st.empty()) {
+            const auto [n, expanded] = st.back();
+            st.pop_back();
+            const BNode& b = bn[n];
+            if (b.left >= 0 && !expanded) {
+                st.push_back({n, true});
+                st.push_back({b.left, false});
+                st.push_back({b.right, false});
+                continue;
+            }
+            float* c = &cost[(size_t)n * 8];
+            uint8_t* ch = &choice[(size_t)n * 8];
+            const float a = bn_area(b);
+            if (b.left < 0) {
+                for (int i = 0; i < 8; i++) { c[i] = a * (float)b.count * kTri; ch[i] = 0; }
+                continue;
+            }
+            const float* cl = &cost[(size_t)b.left * 8];
+            const float* cr = &cost[(size_t)b.right * 8];
+            // dist[j]: j slots split between the children, k of them to the left
+            float dist[9];
+            uint8_t dk[9];
+            for (int j = 2; j <= 8; j++) {
+                dist[j] = FLT_MAX; dk[j] = 1;
+                for (int k = 1; k < j; k++) {
+                    const float v = cl[k - 1] + cr[j - k - 1];
+                    if (v < dist[j]) { dist[j] = v; dk[j] = (uint8_t)k; }
+                }
+            }
+            const float one = a * kNode + dist[8];
+            split8[n] = dk[8];
+            c[0] = one; ch[0] = 0;
+            for (int i = 2; i <= 8; i++) {
+                if (dist[i] < one) { c[i - 1] = dist[i]; ch[i - 1] = dk[i]; }
+                else { c[i - 1] = one; ch[i - 1] = 0; }
+            }
+        }
+    }
+
+    // the trees of the best forest of at most i trees for the subtree at n
+    void forest(int32_t n, int i, int32_t* out, int& nk) const
+    {
+        const uint8_t k = bn[n].left < 0 ? 0 : choice[(size_t)n * 8 + (i - 1)];
+        if (k == 0) { out[nk++] = n; return; }
+        forest(bn[n].left, k, out, nk);
+        forest(bn[n].right, i - k, out, nk);
+    }
+
+    // the slots of the wide node made from binary node n (at most 8)
+    int children(int32_t n, int32_t* out) const
+    {
+        int nk = 0;
+        if (bn[n].left < 0) { out[nk++] = n; return nk; }
+        forest(bn[n].left, split8[n], out, nk);
+        forest(bn[n].right, 8 - split8[n], out, nk);
+        return nk;
+    }
+};
+
 }  // namespace
 
 const char* wbvh_build(const float* verts, uint32_t nvert, const uint32_t* tris, uint32_t ntri,
@@ -216,7 +301,10 @@ const char* wbvh_build(const float* verts, uint32_t nvert, const uint32_t* tris,
     Builder bld(prims, idx, bn, nthreads);
     const int32_t root = bld.build(0, ntri);
 
-    // Collapse into wide nodes, breadth first.
+    // Collapse into wide nodes, breadth first, the slots of each chosen by
+    // the cost-optimal dynamic program.
+    Collapse col(bn);
+    col.run(root);
     struct Pending { int32_t bnode; uint32_t depth; };
     std::vector<Pending> queue;
     queue.push_back({root, 1});
@@ -234,26 +322,7 @@ const char* wbvh_build(const float* verts, uint32_t nvert, const uint32_t* tris,
 
         // children of this wide node (binary node ids)
         int32_t kids[WBVH_WIDTH];
-        int nk = 0;
-        const BNode& b0 = bn[pd.bnode];
-        if (b0.left < 0) {
-            kids[nk++] = pd.bnode;            // a leaf at the root
-        } else {
-            kids[nk++] = b0.left;
-            kids[nk++] = b0.right;
-            while (nk < WBVH_WIDTH) {
-                int best = -1;
-                float ba = -1.0f;
-                for (int i = 0; i < nk; i++) {
-                    const BNode& c = bn[kids[i]];
-                    if (c.left >= 0 && bn_area(c) > ba) { ba = bn_area(c); best = i; }
-                }
-                if (best < 0) break;
-                const BNode& c = bn[kids[best]];
-                kids[best] = c.left;
-                kids[nk++] = c.right;
-            }
-        }
+        const int nk = col.children(pd.bnode, kids);   // a leaf at the root stays one slot
 
         // octant slot assignment: slot s holds the child that comes first
         // along direction octant s (bit a of s set: negative direction on axis a)

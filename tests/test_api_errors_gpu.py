@@ -46,6 +46,21 @@ def test_render_argument_errors(rs, entry):
     assert _call(entry, rs, _params()) == 0
 
 
+@pytest.mark.parametrize("entry", ["lmx_render_bdpt", "lmx_render_vcm_bdpt"])
+def test_failed_batch_allocation_leaves_the_scene_usable(rs, entry):
+    """A batch whose buffers cannot all be allocated (2^27 samples of 10
+    vertices: 172 GB of vertex records and 310 GB of shadow-queue records) is
+    an error, and the next render on the same scene allocates its own buffers
+    again instead of taking the failed ones for allocated."""
+    big = _params(max_num_vertices=10, num_samples=1 << 27, queue_capacity=1 << 27)
+    assert _call(entry, rs, big) == -3 and "bdpt" in lib.lmx_last_error().decode()
+    film = torch.zeros((16, 16, 3), device="cuda")
+    assert _call(entry, rs, _params(max_num_vertices=10, num_samples=16 * 16 * 64), film) == 0
+    torch.cuda.synchronize()
+    f = film.cpu().numpy()
+    assert np.isfinite(f).all() and f.mean() > 0.0
+
+
 def test_bdpt_specific_errors(rs):
     assert _call("lmx_render_bdpt", rs, _params(mis=5)) == -1 and "mis" in lib.lmx_last_error().decode()
     assert _call("lmx_render_bdpt", rs, _params(max_num_vertices=40)) == -1
