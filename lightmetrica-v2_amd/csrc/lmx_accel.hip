@@ -51,8 +51,12 @@ int set_error(int code, const std::string& msg)
     return code;
 }
 
+// A failed HIP call is reported through the library's own error string;
+// the runtime's last-error slot is cleared so that the caller's next launch
+// check does not report it a second time.
 int hip_error(hipError_t e, const char* what)
 {
+    (void)hipGetLastError();
     return set_error(LMX_EHIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 

@@ -1756,12 +1756,15 @@ int bd_buffers(lmx_scene* sc, uint32_t B, uint32_t maxv, uint32_t conn, bool rou
     bd_release(sc);
     hipError_t e;
     const size_t nv = (size_t)B * maxv * 4;
+    // a failure frees whatever was allocated so far: the scene then holds no
+    // batch buffers (and reallocates them on its next render)
+    auto fail = [&](hipError_t err, const char* what) { bd_release(sc); return hip_error(err, what); };
     for (int k = 0; k < 2; k++) {
-        if ((e = sc->bd_vert[k].alloc(nv)) != hipSuccess) return hip_error(e, "bdpt vertex buffers");
-        if ((e = sc->bd_nsub[k].alloc(B)) != hipSuccess) return hip_error(e, "bdpt buffers");
-        if ((e = sc->bd_ray[k].alloc(4 * (size_t)B)) != hipSuccess) return hip_error(e, "bdpt queues");
-        if ((e = sc->bd_thr[k].alloc(2 * (size_t)B)) != hipSuccess) return hip_error(e, "bdpt queues");
-        if ((e = sc->bd_meta[k].alloc(2 * (size_t)B)) != hipSuccess) return hip_error(e, "bdpt queues");
+        if ((e = sc->bd_vert[k].alloc(nv)) != hipSuccess) return fail(e, "bdpt vertex buffers");
+        if ((e = sc->bd_nsub[k].alloc(B)) != hipSuccess) return fail(e, "bdpt buffers");
+        if ((e = sc->bd_ray[k].alloc(4 * (size_t)B)) != hipSuccess) return fail(e, "bdpt queues");
+        if ((e = sc->bd_thr[k].alloc(2 * (size_t)B)) != hipSuccess) return fail(e, "bdpt queues");
+        if ((e = sc->bd_meta[k].alloc(2 * (size_t)B)) != hipSuccess) return fail(e, "bdpt queues");
     }
     if ((e = sc->bd_V.alloc((size_t)B * maxv)) != hipSuccess || (e = sc->bd_epix.alloc(B)) != hipSuccess ||
         (e = sc->bd_eraster.alloc(B)) != hipSuccess ||
@@ -1770,7 +1773,7 @@ int bd_buffers(lmx_scene* sc, uint32_t B, uint32_t maxv, uint32_t conn, bool rou
         (e = sc->bd_shits.alloc(nconn)) != hipSuccess || (e = sc->bd_smeta.alloc(nconn)) != hipSuccess ||
         (e = sc->bd_off.alloc((size_t)B + 1)) != hipSuccess || (e = sc->bd_dmask.alloc(B)) != hipSuccess ||
         (e = sc->bd_part.alloc(((size_t)B + SCAN_TILE) / SCAN_TILE)) != hipSuccess)
-        return hip_error(e, "bdpt buffers");
+        return fail(e, "bdpt buffers");
     if (!sc->bd_ctl.p && (e = sc->bd_ctl.alloc(1)) != hipSuccess) return hip_error(e, "bdpt control block");
     if (!sc->bd_hctl && (e = hipHostMalloc((void**)&sc->bd_hctl, sizeof(BCtl), hipHostMallocDefault)) != hipSuccess)
         return hip_error(e, "bdpt pinned control block");
@@ -2165,16 +2168,23 @@ int vc_buffers(lmx_scene* sc, uint32_t np, uint32_t maxv, uint32_t B)
     uint32_t H = 1024;
     while (H < items && H < (1u << 25)) H <<= 1;
     hipError_t e;
+    auto fail = [&](hipError_t err, const char* what) {
+        sc->vc_vert.release(); sc->vc_nsub.release(); sc->vc_items.release();
+        sc->vc_start.release(); sc->vc_cursor.release(); sc->vc_part.release();
+        for (int k = 0; k < 2; k++) { sc->vc_ray[k].release(); sc->vc_thr[k].release(); sc->vc_meta[k].release(); }
+        sc->vc_hits.release();
+        return hip_error(err, what);
+    };
     if ((e = sc->vc_vert.alloc((size_t)np * maxv * 4)) != hipSuccess || (e = sc->vc_nsub.alloc(np)) != hipSuccess ||
         (e = sc->vc_items.alloc(items)) != hipSuccess || (e = sc->vc_start.alloc((size_t)H + 1)) != hipSuccess ||
         (e = sc->vc_cursor.alloc(H)) != hipSuccess ||
         (e = sc->vc_part.alloc((H + SCAN_TILE) / SCAN_TILE)) != hipSuccess ||
         (e = sc->vc_hits.alloc(np)) != hipSuccess)
-        return hip_error(e, "photon buffers");
+        return fail(e, "photon buffers");
     for (int k = 0; k < 2; k++)
         if ((e = sc->vc_ray[k].alloc(2 * (size_t)np)) != hipSuccess || (e = sc->vc_thr[k].alloc(np)) != hipSuccess ||
             (e = sc->vc_meta[k].alloc(np)) != hipSuccess)
-            return hip_error(e, "photon queues");
+            return fail(e, "photon queues");
     sc->vc_np = np;
     sc->vc_maxv = maxv;
     sc->vc_H = H;

@@ -430,12 +430,18 @@ int ensure_queues(lmx_scene* sc, uint32_t cap)
     if (sc->cap == cap) return LMX_OK;
     sc->cap = 0;                       // set again only once every buffer is in place
     hipError_t e;
+    // a failure frees what was allocated so far
+    auto fail = [&](hipError_t err, const char* what) {
+        for (int q = 0; q < 2; q++) { sc->ray[q].release(); sc->thr[q].release(); sc->meta[q].release(); }
+        sc->hits.release();
+        return hip_error(err, what);
+    };
     for (int q = 0; q < 2; q++) {
-        if ((e = sc->ray[q].alloc(2 * (size_t)cap)) != hipSuccess) return hip_error(e, "queue alloc");
-        if ((e = sc->thr[q].alloc(cap)) != hipSuccess) return hip_error(e, "queue alloc");
-        if ((e = sc->meta[q].alloc(cap)) != hipSuccess) return hip_error(e, "queue alloc");
+        if ((e = sc->ray[q].alloc(2 * (size_t)cap)) != hipSuccess) return fail(e, "queue alloc");
+        if ((e = sc->thr[q].alloc(cap)) != hipSuccess) return fail(e, "queue alloc");
+        if ((e = sc->meta[q].alloc(cap)) != hipSuccess) return fail(e, "queue alloc");
     }
-    if ((e = sc->hits.alloc(cap)) != hipSuccess) return hip_error(e, "hit buffer alloc");
+    if ((e = sc->hits.alloc(cap)) != hipSuccess) return fail(e, "hit buffer alloc");
     sc->cap = cap;
     return LMX_OK;
 }
@@ -448,8 +454,10 @@ int ensure_nee(lmx_scene* sc, uint32_t cap)
     const size_t n = 2 * (size_t)cap;
     if ((e = sc->nee_ray.alloc(2 * n)) != hipSuccess || (e = sc->nee_c.alloc(n)) != hipSuccess ||
         (e = sc->nee_hits.alloc(n)) != hipSuccess || (e = sc->nee_meta.alloc(n)) != hipSuccess ||
-        (e = sc->nee_ry.alloc(n)) != hipSuccess)
+        (e = sc->nee_ry.alloc(n)) != hipSuccess) {
+        sc->nee_ray.release(); sc->nee_c.release(); sc->nee_hits.release(); sc->nee_meta.release(); sc->nee_ry.release();
         return hip_error(e, "shadow queue alloc");
+    }
     sc->nee_cap = cap;
     return LMX_OK;
 }

@@ -469,14 +469,16 @@ struct DevBuf {
         n = 0;
     }
     // n records the size only once the allocation succeeded, so a reuse
-    // check on n never hands a null buffer to a kernel after a failure
+    // check on n never hands a null buffer to a kernel after a failure; a
+    // failed hipMalloc's error is also taken off the runtime's last-error
+    // slot, which the caller's next launch check would otherwise report
     hipError_t alloc(size_t count)
     {
         if (p) { (void)hipFree(p); p = nullptr; }
         n = 0;
         T* q = nullptr;
         const hipError_t e = hipMalloc(&q, std::max<size_t>(1, count) * sizeof(T));
-        if (e != hipSuccess) return e;
+        if (e != hipSuccess) { (void)hipGetLastError(); return e; }
         p = q;
         n = count;
         return e;
