@@ -673,12 +673,12 @@ MLP_FLOP_FWD = 2 * (168 * 64 + 3 * 64 * 64 + 64 * 41)     # 51,328 algorithmic F
 MFMA_BF16_PEAK = 2.5e15
 
 
-def guide_section(dev, ref_spp=16384, spp=256, train_passes=16, steps=16, batch=1 << 15, lr=1e-3, cache=None,
-                  trials_n=3, mfma=True):
+def guide_section(dev, ref_spp=16384, spp=256, train_passes=8, steps=32, batch=1 << 15, lr=1e-3, cache=None,
+                  trials_n=3, mfma=True, train_spp=2):
     """Learned path guiding on the occluded-room scene (12 quads + 96
     displaced spheres = 1,966,104 tris, 40% Phong e=200, emitter behind a wall
-    with a 0.1-wide gap), 1024x1024: train online on the first 16 spp (1 spp
-    per pass, Adam steps after each), render 256 spp guided; plain PT gets the
+    with a 0.1-wide gap), 1024x1024: train online on the first 16 spp (8 passes
+    of 2 spp, 32 Adam steps after each), render 256 spp guided; plain PT gets the
     same wall-clock time; both MSEs against a ref_spp plain reference.  Both
     samplers use the throughput roulette from depth 3 and at most 17
     vertices.  `trials_n` independent trials (fresh network, other seeds):
@@ -736,7 +736,7 @@ def guide_section(dev, ref_spp=16384, spp=256, train_passes=16, steps=16, batch=
         t0 = time.perf_counter()
         recs = nz = 0
         for k in range(train_passes):
-            _, _, gs = g.render(npix, seed=100 + 1000 * tr + k, train=True, **kw)
+            _, _, gs = g.render(npix * train_spp, seed=100 + 1000 * tr + k, train=True, **kw)
             recs += gs["records"]
             nz += gs["records_nonzero"]
         torch.cuda.synchronize()
@@ -796,7 +796,8 @@ def guide_section(dev, ref_spp=16384, spp=256, train_passes=16, steps=16, batch=
         "trials_equal_time": [t["mse_ratio_equal_time"] for t in trials],
         "trials_equal_spp": [t["mse_ratio_equal_spp"] for t in trials],
         "reference": {"spp": ref_spp, "s": round(t_ref, 1), "mean": round(float(ref.mean()), 6)},
-        "training": {"passes": train_passes, "steps_per_pass": steps, "batch": batch, "lr": lr},
+        "training": {"passes": train_passes, "spp_per_pass": train_spp, "steps_per_pass": steps, "batch": batch,
+                     "lr": lr},
         "mfma": {"peak_tflops_bf16": MFMA_BF16_PEAK / 1e12,
                  "inference_tflops": round(inf_tflops, 1), "inference_frac": round(inf_tflops * 1e12 / MFMA_BF16_PEAK, 4),
                  "inference_mqueries_per_s": round((1 << 22) / t_eval / 1e6, 1),
