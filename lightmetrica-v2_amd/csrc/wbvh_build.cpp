@@ -13,8 +13,8 @@
 //   2. collapse into 8-wide nodes by the cost-optimal dynamic program over
 //      the binary tree (Ylitie, Karras and Laine, HPG 2017);
 //   3. breadth-first layout (internal children of a node contiguous, leaf
-//      triangles of a node contiguous), octant slot assignment, 16-bit
-//      outward quantisation of child boxes.
+//      triangles of a node contiguous), octant slot assignment, outward
+//      quantisation of child boxes to a 2047-step grid (stored as binary16).
 // Triangle ids are the caller's enumeration order (primitive order x face
 // order, accel_bvh_sahbin.cpp:79-111).
 #include "wbvh_build.h"
