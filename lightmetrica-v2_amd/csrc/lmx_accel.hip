@@ -81,7 +81,11 @@ struct lmx_accel {
 namespace lmx {
 
 // Returns false when a node group did not fit the stack (a BVH deeper than
-// WBVH_STACK, which the builder refuses); the hit is then a miss.
+// WBVH_STACK, which the builder refuses); the hit is then a miss.  The walk
+// is the kernel's per-lane state machine, step for step (node test while the
+// two-group triangle queue has room, one triangle per step), so the node
+// visits and triangle tests it counts, and an any-hit ray's reported
+// triangle, are the kernel's.
 template <int ISECT, bool ANYHIT>
 static bool walk_impl(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* nv, uint64_t* nt)
 {
@@ -91,38 +95,57 @@ static bool walk_impl(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* n
     uint32_t best_id = WBVH_MISS;
     uint32_t stack_base[WBVH_STACK], stack_bits[WBVH_STACK];
     int sp = 0;
-    uint32_t g_base = 0, g_hits = 1u << (0 ^ r.oct), g_imask = 1u;
+    uint32_t g_base = 0, g_hits = 0, g_imask = 0;
+    uint32_t pend = 0, tbase = 0, tvalid = 0, pend2 = 0, tbase2 = 0, tvalid2 = 0;
+    uint32_t cur = 0;                   // the node to test next (the root first)
+    bool more = true;
     uint64_t visits = 0, tests = 0;
     while (true) {
-        if (g_hits == 0) {
-            if (sp == 0) break;
-            sp--;
-            g_base = stack_base[sp];
-            g_hits = stack_bits[sp] & 0xffu;
-            g_imask = stack_bits[sp] >> 8;
-        }
-        const uint32_t p = (uint32_t)wb_ctz(g_hits);
-        g_hits &= g_hits - 1u;
-        const uint32_t s = p ^ r.oct;
-        const uint32_t child = g_base + (uint32_t)wb_popc(g_imask & ((1u << s) - 1u));
-        if (g_hits) {
-            if (sp >= WBVH_STACK) {
-                hit.t = 0.0f; hit.b0 = 0.0f; hit.b1 = 0.0f; hit.id = WBVH_MISS;
-                return false;
+        uint32_t child = 0;
+        if (pend2 == 0 && more && !(ANYHIT && best_id != WBVH_MISS)) {
+            const WNode& nd = b.nodes[cur];
+            visits++;
+            uint32_t tmask = 0;
+            const uint32_t inner = wb_node_hits(nd, r, best_t, tmask);
+            if (pend == 0) {
+                pend = tmask; tbase = nd.tri_base; tvalid = nd.tvalid;
+            } else {
+                pend2 = tmask; tbase2 = nd.tri_base; tvalid2 = nd.tvalid;
             }
-            stack_base[sp] = g_base;
-            stack_bits[sp] = g_hits | (g_imask << 8);
-            sp++;
+            g_base = nd.child_base; g_hits = inner; g_imask = nd.imask;
+            if (g_hits == 0) {
+                if (sp == 0) {
+                    more = false;
+                } else {
+                    sp--;
+                    g_base = stack_base[sp];
+                    g_hits = stack_bits[sp] & 0xffu;
+                    g_imask = stack_bits[sp] >> 8;
+                }
+            }
+            if (more) {
+                const uint32_t p = (uint32_t)wb_ctz(g_hits);
+                g_hits &= g_hits - 1u;
+                const uint32_t s = p ^ r.oct;
+                child = g_base + (uint32_t)wb_popc(g_imask & ((1u << s) - 1u));
+                if (g_hits) {
+                    if (sp >= WBVH_STACK) {
+                        hit.t = 0.0f; hit.b0 = 0.0f; hit.b1 = 0.0f; hit.id = WBVH_MISS;
+                        return false;
+                    }
+                    stack_base[sp] = g_base;
+                    stack_bits[sp] = g_hits | (g_imask << 8);
+                    sp++;
+                }
+            }
+        } else if (pend2 != 0 && more) {
+            child = cur;
         }
-        const WNode& nd = b.nodes[child];
-        visits++;
-        uint32_t tmask = 0;
-        const uint32_t inner = wb_node_hits(nd, r, best_t, tmask);
-        g_base = nd.child_base; g_hits = inner; g_imask = nd.imask;
-        while (tmask) {
-            const uint32_t k = (uint32_t)wb_ctz(tmask);
-            tmask &= tmask - 1u;
-            const WTri& tri = b.tris[wb_tri_index(nd.tri_base, nd.tvalid, k)];
+        cur = child;
+        if (pend) {
+            const uint32_t k = (uint32_t)wb_ctz(pend);
+            pend &= pend - 1u;
+            const WTri& tri = b.tris[wb_tri_index(tbase, tvalid, k)];
             const uint32_t id = wb_tri_id(tri, ISECT);
             tests++;
             float t, a0, a1, dt = 1.0f;
@@ -132,9 +155,13 @@ static bool walk_impl(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* n
             if (h && (t < best_t || id < best_id)) {
                 best_t = t; best_id = id; h0 = a0; h1 = a1; hdet = dt;
             }
+            if (ANYHIT && best_id != WBVH_MISS) { pend = 0; pend2 = 0; more = false; }
         }
-        // any hit: the kernel tests the rest of this node's triangles, then stops
-        if (ANYHIT && best_id != WBVH_MISS) break;
+        if (pend == 0) {
+            pend = pend2; tbase = tbase2; tvalid = tvalid2;
+            pend2 = 0;
+        }
+        if (pend == 0 && !more && cur == 0) break;
     }
     if (best_id == WBVH_MISS) {
         hit.t = 0.0f; hit.b0 = 0.0f; hit.b1 = 0.0f; hit.id = WBVH_MISS;
@@ -270,12 +297,15 @@ void k_trace(const WNode* __restrict__ nodes,
     uint32_t best_id = WBVH_MISS;
     uint32_t g_base = 0, g_hits = 0, g_imask = 0;
     int sp = 0;
-    // Triangles of the last tested node still to test (bit i: triangle
-    // tbase + i), one per iteration, and whether a next node was chosen:
-    // a lane tests its node's triangles over the following iterations while
-    // the other lanes of the wave go on testing nodes, instead of the whole
-    // wave looping as many times as its lane with the most triangles.
+    // Leaf triangles still to test, one per iteration, in a queue of two
+    // node groups (bits of a node's triangle mask, with the node's triangle
+    // base and valid mask): a lane goes on testing nodes while one group is
+    // pending and stops only when a second one is queued, and the wave
+    // tests nodes and triangles in the same iteration instead of looping as
+    // many times as its lane with the most triangles.  `more`: a next node
+    // was chosen (it is `cur`, loaded into nd).
     uint32_t pend = 0, tbase = 0, tvalid = 0, cur = 0;
+    uint32_t pend2 = 0, tbase2 = 0, tvalid2 = 0;
     bool more = false;
     WNode nd;                           // node to test in the next iteration
     load_node(nd, nodes);               // idle lanes always hold the root
@@ -324,7 +354,7 @@ void k_trace(const WNode* __restrict__ nodes,
                 wb_ray_setup(r, a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w);
                 best_t = b.w; best_id = WBVH_MISS; h0 = h1 = 0.0f; hdet = 1.0f;
                 sp = 0;                        // nd already holds the root
-                pend = 0; more = true;
+                pend = 0; pend2 = 0; more = true;
                 valid = true;
             }
         }
@@ -332,19 +362,21 @@ void k_trace(const WNode* __restrict__ nodes,
         // body below does nothing but reload the root
         if (__ballot(valid) == 0 && regions_left == 0 && pool_cnt == 0) break;
 
-        // 1. lanes with no triangle pending: box test of the node loaded in
-        //    the previous iteration and the choice of the next node; lanes
-        //    with triangles pending keep their node (it is loaded again).
-        //    Every lane loads exactly one node per iteration at one site
-        //    (lanes without a next node load the root, which is what a
-        //    refilled lane needs first).
+        // 1. lanes with room in their triangle queue: box test of the node
+        //    loaded in the previous iteration, its leaf triangles queued, and
+        //    the choice of the next node; lanes with a full queue keep their
+        //    node (it is loaded again).  Every lane loads exactly one node
+        //    per iteration at one site (lanes without a next node load the
+        //    root, which is what a refilled lane needs first).
         uint32_t child = 0;
-        if (valid && pend == 0 && more && !(ANYHIT && best_id != WBVH_MISS)) {
+        if (valid && pend2 == 0 && more && !(ANYHIT && best_id != WBVH_MISS)) {
             uint32_t tmask = 0;
             const uint32_t inner = wb_node_hits(nd, r, best_t, tmask);
-            pend = tmask;
-            tbase = nd.tri_base;
-            tvalid = nd.tvalid;
+            if (pend == 0) {
+                pend = tmask; tbase = nd.tri_base; tvalid = nd.tvalid;
+            } else {
+                pend2 = tmask; tbase2 = nd.tri_base; tvalid2 = nd.tvalid;
+            }
             g_base = nd.child_base; g_hits = inner; g_imask = nd.imask;
             if (g_hits == 0) {
                 if (sp == 0) {
@@ -371,8 +403,8 @@ void k_trace(const WNode* __restrict__ nodes,
                     }
                 }
             }
-        } else if (valid && pend != 0 && more) {
-            child = cur;                       // the node chosen before these triangles
+        } else if (valid && pend2 != 0 && more) {
+            child = cur;                       // the node chosen before the queued triangles
         }
         // 2. load the next node; its latency overlaps the triangle test below
         cur = child;
@@ -385,7 +417,11 @@ void k_trace(const WNode* __restrict__ nodes,
                 WTri t1;
                 load_tri(t1, tris + wb_tri_index(tbase, tvalid, k1));
                 test_tri<ISECT>(t1, r, best_t, best_id, h0, h1, hdet);
-                if (ANYHIT && best_id != WBVH_MISS) { pend = 0; more = false; }
+                if (ANYHIT && best_id != WBVH_MISS) { pend = 0; pend2 = 0; more = false; }
+            }
+            if (pend == 0) {                   // the queued group moves up
+                pend = pend2; tbase = tbase2; tvalid = tvalid2;
+                pend2 = 0;
             }
             // 4. ray finished: write its hit and free the lane.  An any-hit
             //    ray that found a triangle with a next node already loaded
