@@ -204,9 +204,6 @@ bool wbvh_walk_anyhit(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* n
 #ifndef LMX_CLAIM_CHUNK
 #define LMX_CLAIM_CHUNK 64     // rays claimed from a region cursor per atomic
 #endif
-#ifndef LMX_TRI_BATCH
-#define LMX_TRI_BATCH 1        // triangle loads kept in flight together (1 or 2)
-#endif
 #define REFILL_MIN LMX_REFILL_MIN
 // words between two region cursors: each cursor sits in a 256-byte line of
 // its own, so the atomics of different regions do not serialise on one line
@@ -286,6 +283,13 @@ void k_trace(const WNode* __restrict__ nodes,
     // a wavefront render) are spread over every wave of the grid, so that
     // their time is that of a few rays per wave, not of whole chunks in turn
     const uint32_t claim = min((uint32_t)LMX_CLAIM_CHUNK, max(1u, (nray + gridDim.x - 1u) / gridDim.x));
+    // the node test's bit shuffles as tables: the internal-child hit mask in
+    // each octant's order, and leaf slot bits spread to triangle bits
+    __shared__ uint8_t s_order[8 * 256];
+    __shared__ uint32_t s_spread[256];
+    for (uint32_t i = lane; i < 8u * 256u; i += TRACE_WAVE) s_order[i] = (uint8_t)wb_octant_order(i & 255u, i >> 8);
+    for (uint32_t i = lane; i < 256u; i += TRACE_WAVE) s_spread[i] = wb_leaf_spread(i);
+    __syncthreads();
 
     uint32_t region = blockIdx.x & 7u;
     uint32_t regions_left = 8;          // regions not yet found exhausted
@@ -370,8 +374,9 @@ void k_trace(const WNode* __restrict__ nodes,
         //    root, which is what a refilled lane needs first).
         uint32_t child = 0;
         if (valid && pend2 == 0 && more && !(ANYHIT && best_id != WBVH_MISS)) {
-            uint32_t tmask = 0;
-            const uint32_t inner = wb_node_hits(nd, r, best_t, tmask);
+            const uint32_t hb = wb_slot_hits(nd, r, best_t);
+            const uint32_t inner = s_order[(r.oct << 8) | (hb & (uint32_t)nd.imask)];
+            const uint32_t tmask = s_spread[hb & ~(uint32_t)nd.imask & 0xffu] & nd.tvalid;
             if (pend == 0) {
                 pend = tmask; tbase = nd.tri_base; tvalid = nd.tvalid;
             } else {

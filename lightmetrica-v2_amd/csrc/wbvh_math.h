@@ -92,11 +92,10 @@ WB_HD void wb_ray_setup(WRayPre& r, float ox, float oy, float oz, float tmin,
     r.Sz = 1.0f / dkz;
 }
 
-// Box test of all eight child slots of a node against [tmin, tmax].
-// Returns the hit mask of internal children permuted by the ray octant
-// (bit p <-> slot p ^ oct) and, through tri_mask, the triangles of hit leaf
-// children (bit b <-> triangle wb_tri_index(node, b)).
-WB_HD uint32_t wb_node_hits(const WNode& n, const WRayPre& r, float tmax, uint32_t& tri_mask)
+// Box test of all eight child slots of a node against [tmin, tmax]: bit i
+// of the result set when slot i's box is hit (empty slots included; they
+// are neither internal nor hold triangles).
+WB_HD uint32_t wb_slot_hits(const WNode& n, const WRayPre& r, float tmax)
 {
     const float sx = wb_as_float((uint32_t)n.ex << 23) * r.ix;
     const float sy = wb_as_float((uint32_t)n.ey << 23) * r.iy;
@@ -141,23 +140,38 @@ WB_HD uint32_t wb_node_hits(const WNode& n, const WRayPre& r, float tmax, uint32
         const float tf = fminf(fminf(xf, yf), fminf(zf, tmax));
         hb |= (tn <= tf * WBVH_FAR_SCALE) ? 1u << i : 0u;
     }
-    // internal children in traversal order: bit p <-> slot p ^ octant, by
-    // swapping bits, bit pairs and nibbles as the octant's bits say
-    uint32_t inner = hb & n.imask;
-    if (r.oct & 1u) inner = ((inner & 0x55u) << 1) | ((inner >> 1) & 0x55u);
-    if (r.oct & 2u) inner = ((inner & 0x33u) << 2) | ((inner >> 2) & 0x33u);
-    if (r.oct & 4u) inner = ((inner & 0x0fu) << 4) | ((inner >> 4) & 0x0fu);
-    // leaf slots hit -> their triangle bits: slot bit i spread to bits
-    // 3 i .. 3 i + 2, restricted to the triangles present
-    uint32_t leaf = hb & ~(uint32_t)n.imask;
-    leaf = (leaf | (leaf << 8)) & 0x00f00fu;
-    leaf = (leaf | (leaf << 4)) & 0x0c30c3u;
-    leaf = (leaf | (leaf << 2)) & 0x249249u;
-    const uint32_t tris = (leaf * 7u) & n.tvalid;
-    tri_mask = tris;
+    return hb;
+}
+
+// Internal children of a slot hit mask in traversal order: bit p <-> slot
+// p ^ oct, by swapping bits, bit pairs and nibbles as the octant's bits say.
+WB_HD uint32_t wb_octant_order(uint32_t inner, uint32_t oct)
+{
+    if (oct & 1u) inner = ((inner & 0x55u) << 1) | ((inner >> 1) & 0x55u);
+    if (oct & 2u) inner = ((inner & 0x33u) << 2) | ((inner >> 2) & 0x33u);
+    if (oct & 4u) inner = ((inner & 0x0fu) << 4) | ((inner >> 4) & 0x0fu);
     return inner;
 }
 
+// Leaf slots hit -> their triangle bits: slot bit i spread to bits 3 i ..
+// 3 i + 2 (to be restricted to the node's valid triangles).
+WB_HD uint32_t wb_leaf_spread(uint32_t leaf)
+{
+    leaf = (leaf | (leaf << 8)) & 0x00f00fu;
+    leaf = (leaf | (leaf << 4)) & 0x0c30c3u;
+    leaf = (leaf | (leaf << 2)) & 0x249249u;
+    return leaf * 7u;
+}
+
+// Returns the hit mask of internal children permuted by the ray octant
+// (bit p <-> slot p ^ oct) and, through tri_mask, the triangles of hit leaf
+// children (bit b <-> triangle wb_tri_index(node, b)).
+WB_HD uint32_t wb_node_hits(const WNode& n, const WRayPre& r, float tmax, uint32_t& tri_mask)
+{
+    const uint32_t hb = wb_slot_hits(n, r, tmax);
+    tri_mask = wb_leaf_spread(hb & ~(uint32_t)n.imask) & n.tvalid;
+    return wb_octant_order(hb & n.imask, r.oct);
+}
 // Triangle of bit b of a node's triangle mask: the node's leaf triangles are
 // stored contiguously in slot order, so it is preceded by the valid bits
 // below b.
