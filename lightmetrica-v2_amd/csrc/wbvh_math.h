@@ -145,7 +145,7 @@ WB_HD uint32_t wb_slot_hits(const WNode& n, const WRayPre& r, float tmax)
 
 // Internal children of a slot hit mask in traversal order: bit p <-> slot
 // p ^ oct, by swapping bits, bit pairs and nibbles as the octant's bits say.
-WB_HD uint32_t wb_octant_order(uint32_t inner, uint32_t oct)
+WB_HD constexpr uint32_t wb_octant_order(uint32_t inner, uint32_t oct)
 {
     if (oct & 1u) inner = ((inner & 0x55u) << 1) | ((inner >> 1) & 0x55u);
     if (oct & 2u) inner = ((inner & 0x33u) << 2) | ((inner >> 2) & 0x33u);
@@ -155,7 +155,7 @@ WB_HD uint32_t wb_octant_order(uint32_t inner, uint32_t oct)
 
 // Leaf slots hit -> their triangle bits: slot bit i spread to bits 3 i ..
 // 3 i + 2 (to be restricted to the node's valid triangles).
-WB_HD uint32_t wb_leaf_spread(uint32_t leaf)
+WB_HD constexpr uint32_t wb_leaf_spread(uint32_t leaf)
 {
     leaf = (leaf | (leaf << 8)) & 0x00f00fu;
     leaf = (leaf | (leaf << 4)) & 0x0c30c3u;
