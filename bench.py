@@ -93,7 +93,8 @@ def traversal_profile():
     """Figures of the committed rocprofv3 counter summary of the traversal
     kernel (profiles/traversal_traffic.txt: `key: value` lines): HBM-side
     bytes per launch from FETCH_SIZE / WRITE_SIZE, the L2 hit rate, VALU
-    instructions per launch and the profiled kernel time."""
+    instructions per launch, the busy fractions of the texture address and
+    data units, the L1 hit rate and the profiled kernel time."""
     path = os.path.join(ROOT, "profiles", "traversal_traffic.txt")
     out = {}
     if os.path.exists(path):
@@ -972,10 +973,16 @@ def main():
             "l2_hit_rate": l2,
             "valu_insts_per_launch": valu,
             "valu_issue_frac": None if valu is None else round(valu * 4 / 1024 / (avg_kernel_s * VALU_CLOCK_HZ), 4),
-            "limiter": "VALU issue: the wave-instruction count of the profiled launch (4 cycles each on one of "
-                       "1,024 SIMDs) against the launch's cycles at 2.4 GHz; the nodes and triangles (98 MB) are "
-                       "served mostly by L2 and the Infinity Cache, so fabric traffic is a fraction of the "
-                       "algorithmic bytes, whose rate exceeds nothing physical",
+            "ta_busy_frac": prof.get("ta_busy_frac"),
+            "td_busy_frac": prof.get("td_busy_frac"),
+            "td_l1_stall_frac": prof.get("td_l1_stall_frac"),
+            "l1_hit_rate": prof.get("l1_hit_rate"),
+            "limiter": "the vector-memory pipeline, then VALU issue: in the profiled launch the texture "
+                       "address (TA) and data (TD) units are busy most cycles, the data unit waiting on the "
+                       "L1 for much of it (L1 misses to L2), while VALU issue (wave instructions x 4 cycles "
+                       "on 1,024 SIMDs against the launch's cycles at 2.4 GHz) comes next; the nodes and "
+                       "triangles (98 MB) are served mostly by L2 and the Infinity Cache, so fabric traffic "
+                       "is a fraction of the algorithmic bytes, whose rate exceeds nothing physical",
             "above_hbm_peak_by_cache_reuse": bool(achieved > HBM_PEAK_GBS),
             "alg_bytes_per_launch": int(alg_bytes),
             "node_visits_per_ray": round(nodes_v / nray, 3),
