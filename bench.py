@@ -8,7 +8,7 @@ uniform on the sphere); closest hit, t in [1e-4, FLT_MAX].  One step = one
 lmx_accel_intersect launch over all 8,388,608 rays, inputs resident in HBM.
 
 Prints one JSON line.  `roofline` prices the traversal kernel's algorithmic
-bytes -- 32 B ray read + 16 B hit write per ray, plus 128 B per wide-node
+bytes -- 32 B ray read + 16 B hit write per ray, plus 80 B per wide-node
 visit and 48 B per triangle test as counted by the host walker over the same
 BVH in the same order -- against 8 TB/s HBM, with the kernel's duration
 measured by HIP events on the stream it runs on.  `cpu_baseline` is the
@@ -27,7 +27,7 @@ sys.path.insert(0, os.path.join(ROOT, "lightmetrica-v2_amd"))
 
 HBM_PEAK_GBS = 8000.0
 VALU_CLOCK_HZ = 2.4e9        # MI355X peak engine clock
-NODE_BYTES = 128
+NODE_BYTES = 80
 TRI_BYTES = 48
 RAY_BYTES = 32 + 16
 
@@ -193,7 +193,7 @@ PT_SEG_STATE_BYTES = 80 + 152 + 64
 
 def pt_roofline(sc, rs, kw, n, dt, rays):
     """B_seg roofline of the wavefront path tracer: per extension ray the
-    traversal's algorithmic bytes (48 B ray and hit + 128 B per node visit +
+    traversal's algorithmic bytes (48 B ray and hit + 80 B per node visit +
     48 B per triangle test, the visits counted by the host walker over the
     same BVH for every ray of a captured 1-spp frame) plus the segment's
     queue and shading bytes, times the rays of the timed frame, over its
@@ -951,7 +951,7 @@ def main():
             "triangles": int(tris.shape[0]),
             "rays": int(nray),
             "isect": "watertight",
-            "bvh": "8-wide, child boxes on a 2047-step binary16 grid, 128 B nodes",
+            "bvh": "8-wide, child boxes on a 255-step grid (bytes), 80 B nodes",
             "bvh_nodes": info["num_nodes"],
             "bvh_depth": info["max_depth"],
             "bvh_build_s": round(build_s, 3),

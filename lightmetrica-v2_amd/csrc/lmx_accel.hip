@@ -12,7 +12,7 @@
 //   * the next node is chosen right after a node's box test and its load is
 //     issued before that node's triangles are tested, so node latency
 //     overlaps triangle work; each lane loads exactly one node per loop
-//     iteration at one site, which keeps the 128-byte node in registers
+//     iteration at one site, which keeps the 80-byte node in registers
 //     across iterations without copies;
 //   * lanes whose ray finished are refilled as a batch once at least
 //     LMX_REFILL_MIN lanes are idle: one ballot counts them, mbcnt ranks give
@@ -254,7 +254,7 @@ __device__ __forceinline__ void load_node(WNode& nd, const WNode* src)
     const uint4* s = reinterpret_cast<const uint4*>(src);
     uint4* d = reinterpret_cast<uint4*>(&nd);
 #pragma unroll
-    for (int i = 0; i < 8; i++) d[i] = s[i];
+    for (int i = 0; i < (int)(sizeof(WNode) / 16); i++) d[i] = s[i];
 }
 
 __device__ __forceinline__ void load_tri(WTri& t, const WTri* src)

@@ -5,15 +5,16 @@
 // (src/liblightmetrica/accel/accel_bvh_sahbin.cpp:36-55,
 //  src/liblightmetrica/accel/accel_qbvh.cpp:54-137).
 //
-// Node: 128 bytes = two 64-byte blocks, 128-byte aligned, so one node is one
-// L2 line on CDNA.  Child boxes are quantised per axis to a grid of 2047
-// steps from the node origin, the step a power of two, rounded outward; the
-// grid coordinates (integers 0..2047, exact in binary16) are stored as
-// half-precision values, which the box test's FMAs take as operands
-// directly (v_fma_mix_f32), with no integer-to-float conversion.  Up to
-// eight children per node; internal children of a node are stored
-// contiguously (child_base + rank), and the triangles of all leaf children of
-// a node are stored contiguously in slot order.  Each leaf holds 1..3
+// Node: 80 bytes = five 16-byte loads.  Child boxes are quantised per axis
+// to a grid of 255 steps from the node origin, the step a power of two,
+// rounded outward, and stored as bytes.  The box test turns a pair of bytes
+// into a pair of binary16 subnormals q * 2^-24 with one byte permute
+// (v_perm_b32: each byte below a zero byte), which its FMAs take as
+// operands directly (v_fma_mix_f32, exact conversion); the node's scale
+// exponent carries the 2^24.  Up to eight children per node; internal
+// children of a node are stored contiguously (child_base + rank), and the
+// triangles of all leaf children of a node are stored contiguously in slot
+// order.  Each leaf holds 1..3
 // triangles; leaf slot s owns bits 3 s .. 3 s + 2 of a 24-bit triangle mask,
 // tvalid marks the bits in use, and the triangle of bit b is stored at
 // tri_base + popcount(tvalid below b), so a node's hit leaves become a
@@ -30,24 +31,25 @@
 #define WBVH_WIDTH 8
 #define WBVH_MAX_LEAF_TRIS 3
 #define WBVH_STACK 32         // traversal stack entries per ray (node groups)
-#define WBVH_QMAX 2047        // quantisation grid steps per axis (exact in binary16)
+#define WBVH_QMAX 255         // quantisation grid steps per axis (one byte)
+#define WBVH_QBIAS 151        // scale exponent bias: grid step = 2^(e - 151), q * 2^-24 * 2^(e - 127)
 
 // Triangle test selector (stored in the accel; picks the kernel variant).
 #define WBVH_ISECT_TRIACCEL 0     // Wald projection test, reference semantics
 #define WBVH_ISECT_WATERTIGHT 1   // Woop/Benthin/Wald watertight test
 
-struct alignas(128) WNode {
+struct alignas(16) WNode {
     float ox, oy, oz;          // quantisation origin (lower corner)
-    uint8_t ex, ey, ez;        // per-axis scale = 2^(e - 127)
+    uint8_t ex, ey, ez;        // per-axis grid step = 2^(e - WBVH_QBIAS)
     uint8_t imask;             // bit s set: slot s is an internal child
     uint32_t child_base;       // index of the first internal child
     uint32_t tri_base;         // index of the first triangle of leaf children
     uint32_t tvalid;           // leaf triangles: bit 3 s + k set for the k-th triangle of leaf slot s
     uint32_t reserved;
-    _Float16 qlo[3][8];        // quantised child lower bounds, per axis (grid units)
-    _Float16 qhi[3][8];        // quantised child upper bounds, per axis (grid units)
+    uint8_t qlo[3][8];         // quantised child lower bounds, per axis (grid units)
+    uint8_t qhi[3][8];         // quantised child upper bounds, per axis (grid units)
 };
-static_assert(sizeof(WNode) == 128, "WNode must be 128 bytes");
+static_assert(sizeof(WNode) == 80, "WNode must be 80 bytes");
 
 // Triangle record, 48 bytes = 3 x float4.
 //   watertight: {v0.xyz, id} {v1.xyz, 0} {v2.xyz, 0}

@@ -14,7 +14,7 @@
 //      the binary tree (Ylitie, Karras and Laine, HPG 2017);
 //   3. breadth-first layout (internal children of a node contiguous, leaf
 //      triangles of a node contiguous), octant slot assignment, outward
-//      quantisation of child boxes to a 2047-step grid (stored as binary16).
+//      quantisation of child boxes to a 255-step grid (stored as bytes).
 // Triangle ids are the caller's enumeration order (primitive order x face
 // order, accel_bvh_sahbin.cpp:79-111).
 #include "wbvh_build.h"
@@ -387,10 +387,12 @@ const char* wbvh_build(const float* verts, uint32_t nvert, const uint32_t* tris,
                 while (std::ldexp((double)WBVH_QMAX, e) < ext) e++;
                 if (e < -100) e = -100;
             }
-            if (e > 127) return "scene extent too large to quantise";
+            if (e + WBVH_QBIAS > 254) return "scene extent too large to quantise";
             ex[a] = e;
         }
-        wn.ex = (uint8_t)(ex[0] + 127); wn.ey = (uint8_t)(ex[1] + 127); wn.ez = (uint8_t)(ex[2] + 127);
+        wn.ex = (uint8_t)(ex[0] + WBVH_QBIAS);
+        wn.ey = (uint8_t)(ex[1] + WBVH_QBIAS);
+        wn.ez = (uint8_t)(ex[2] + WBVH_QBIAS);
         const float org[3] = {lo[0], lo[1], lo[2]};
 
         // children in slot order: internal ones get consecutive wide slots,
@@ -407,8 +409,8 @@ const char* wbvh_build(const float* verts, uint32_t nvert, const uint32_t* tris,
                 double qh = std::ceil(((double)c.hi[a] - (double)org[a]) / sc);
                 ql = std::max(0.0, std::min((double)WBVH_QMAX, ql));
                 qh = std::max(0.0, std::min((double)WBVH_QMAX, qh));
-                wn.qlo[a][s] = (_Float16)ql;
-                wn.qhi[a][s] = (_Float16)qh;
+                wn.qlo[a][s] = (uint8_t)ql;
+                wn.qhi[a][s] = (uint8_t)qh;
             }
             if (c.left >= 0) {
                 wn.imask |= (uint8_t)(1u << s);

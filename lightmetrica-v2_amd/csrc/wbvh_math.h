@@ -92,6 +92,20 @@ WB_HD void wb_ray_setup(WRayPre& r, float ox, float oy, float oz, float tmin,
     r.Sz = 1.0f / dkz;
 }
 
+// Grid coordinates of children 2p and 2p + 1 of one axis (bytes 2p and
+// 2p + 1 of a word holding children 0..3 or 4..7, p = 0 or 1) as a pair of
+// binary16 subnormals q * 2^-24: each byte below a zero byte, one v_perm_b32
+// on the device.  The conversion to FP32 is exact.
+WB_HD wb_h2 wb_q_pair(uint32_t word, int p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t v = __builtin_amdgcn_perm(0u, word, p ? 0x0c030c02u : 0x0c010c00u);
+#else
+    const uint32_t v = p ? (((word >> 16) & 0xffu) | ((word >> 8) & 0xff0000u)) : ((word & 0xffu) | ((word & 0xff00u) << 8));
+#endif
+    return __builtin_bit_cast(wb_h2, v);
+}
+
 // Box test of all eight child slots of a node against [tmin, tmax]: bit i
 // of the result set when slot i's box is hit (empty slots included; they
 // are neither internal nor hold triangles).
@@ -105,11 +119,12 @@ WB_HD uint32_t wb_slot_hits(const WNode& n, const WRayPre& r, float tmax)
     const float bz = (n.oz - r.oz) * r.iz;
     // Near and far planes per axis chosen once per node by the sign of the
     // slab scale (for lo <= hi, fma(lo, s, b) <= fma(hi, s, b) exactly when
-    // s >= 0), instead of a min and a max per plane pair and child.
-    wb_h2 nq[3][4], fq[3][4];
-    const wb_h2* qlo = reinterpret_cast<const wb_h2*>(&n.qlo[0][0]);
-    const wb_h2* qhi = reinterpret_cast<const wb_h2*>(&n.qhi[0][0]);
+    // s >= 0), instead of a min and a max per plane pair and child: the
+    // byte words are selected, then each byte pair becomes a binary16 pair.
+    const uint32_t* lw = reinterpret_cast<const uint32_t*>(&n.qlo[0][0]);
+    const uint32_t* hw = reinterpret_cast<const uint32_t*>(&n.qhi[0][0]);
     const bool neg[3] = {sx < 0.0f, sy < 0.0f, sz < 0.0f};
+    wb_h2 nq[3][4], fq[3][4];
 #if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll
 #endif
@@ -117,9 +132,13 @@ WB_HD uint32_t wb_slot_hits(const WNode& n, const WRayPre& r, float tmax)
 #if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll
 #endif
-        for (int w = 0; w < 4; w++) {
-            nq[a][w] = neg[a] ? qhi[4 * a + w] : qlo[4 * a + w];
-            fq[a][w] = neg[a] ? qlo[4 * a + w] : qhi[4 * a + w];
+        for (int w = 0; w < 2; w++) {
+            const uint32_t nw = neg[a] ? hw[2 * a + w] : lw[2 * a + w];
+            const uint32_t fw = neg[a] ? lw[2 * a + w] : hw[2 * a + w];
+            nq[a][2 * w] = wb_q_pair(nw, 0);
+            nq[a][2 * w + 1] = wb_q_pair(nw, 1);
+            fq[a][2 * w] = wb_q_pair(fw, 0);
+            fq[a][2 * w + 1] = wb_q_pair(fw, 1);
         }
     uint32_t hb = 0;                   // hit slots, bit i <-> slot i
 #if defined(__HIP_DEVICE_COMPILE__)

@@ -123,7 +123,7 @@ def test_occlusion_counting_walker():
 
 def decode_boxes(node):
     o = node["o"].astype(np.float64)
-    sc = np.ldexp(1.0, node["e"].astype(np.int64) - 127)
+    sc = np.ldexp(1.0, node["e"].astype(np.int64) - accel.WNODE_QBIAS)
     lo = o[:, None] + node["qlo"].astype(np.float64) * sc[:, None]
     hi = o[:, None] + node["qhi"].astype(np.float64) * sc[:, None]
     return lo, hi      # (3, 8) each
