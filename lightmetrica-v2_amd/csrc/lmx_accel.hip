@@ -17,7 +17,11 @@
 //   * lanes whose ray finished are refilled as a batch once at least
 //     LMX_REFILL_MIN lanes are idle: one ballot counts them, mbcnt ranks give
 //     every idle lane its ray, and the rays come from a per-wave pool that
-//     one atomic refills up to LMX_CLAIM_CHUNK rays at a time;
+//     one atomic refills up to LMX_CLAIM_CHUNK rays at a time; the ray
+//     set-up that depends on the direction alone (reciprocals, octant, the
+//     watertight shear with its divisions) is computed when the pool is
+//     filled, a ray per lane at the full width of the wave, and handed to
+//     the refilled lanes by shuffles;
 //   * XCD-aware work split: workgroup b runs on XCD (b mod 8); the rays are
 //     cut into 8 contiguous regions and a wave first drains the region of
 //     its own XCD (neighbouring rays share BVH nodes in that XCD's L2), then
@@ -196,7 +200,7 @@ bool wbvh_walk_anyhit(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* n
 // lmx_accel_intersect_host walks batches below this size on the calling thread
 #define LMX_HOST_INLINE_RAYS 1024u
 #ifndef LMX_REFILL_MIN
-#define LMX_REFILL_MIN 16      // refill when at least this many lanes of a wave are idle
+#define LMX_REFILL_MIN 8       // refill when at least this many lanes of a wave are idle
 #endif
 #ifndef LMX_TRACE_WAVES_PER_EU
 #define LMX_TRACE_WAVES_PER_EU 1   // occupancy hint for the register allocator (5 measured 2 % slower)
@@ -312,6 +316,11 @@ void k_trace(const WNode* __restrict__ nodes,
     uint32_t region = blockIdx.x & 7u;
     uint32_t regions_left = 8;          // regions not yet found exhausted
     uint32_t pool_base = 0, pool_cnt = 0;   // claimed, not yet started rays
+    // direction set-up of the pool's rays: lane pool_lane + k holds that of
+    // ray pool_base + k (octant | kx << 3 | ky << 5 | kz << 7 in p_meta)
+    uint32_t pool_lane = 0;
+    float p_ix = 0.0f, p_iy = 0.0f, p_iz = 0.0f, p_sx = 0.0f, p_sy = 0.0f, p_sz = 0.0f;
+    uint32_t p_meta = 0;
     bool valid = false;
     uint32_t ray_id = 0;
     WRayPre r;
@@ -333,17 +342,16 @@ void k_trace(const WNode* __restrict__ nodes,
     load_node(nd, nodes);               // idle lanes always hold the root
 
     while (true) {
-        // ---- batch refill of idle lanes, from this wave's pool of claimed
-        // rays first, then from a freshly claimed chunk
+        // ---- batch refill of idle lanes from this wave's pool of claimed
+        // rays; an empty pool is refilled first with a fresh chunk, whose
+        // direction set-up (reciprocals, octant, shear: the divisions) the
+        // whole wave computes at once, a ray per lane, so that a refill
+        // costs a few lane shuffles instead of the set-up at the width of
+        // the lanes refilled
         const uint64_t idle = __ballot(!valid);
         const uint32_t nidle = (uint32_t)__popcll(idle);
         if ((regions_left != 0 || pool_cnt != 0) && (nidle >= refill_min || nidle == TRACE_WAVE)) {
-            const uint32_t take0 = min(nidle, pool_cnt);
-            const uint32_t base0 = pool_base;
-            pool_base += take0;
-            pool_cnt -= take0;
-            uint32_t base1 = 0, got1 = 0;
-            if (take0 < nidle && regions_left != 0) {
+            if (pool_cnt == 0) {
                 uint32_t cb = 0, cg = 0;
                 if (lane == 0) {
                     while (regions_left != 0) {
@@ -362,22 +370,44 @@ void k_trace(const WNode* __restrict__ nodes,
                 cg = __shfl(cg, 0);
                 regions_left = __shfl(regions_left, 0);
                 region = __shfl(region, 0);
-                got1 = min(nidle - take0, cg);
-                base1 = cb;
-                pool_base = cb + got1;
-                pool_cnt = cg - got1;
+                pool_base = cb;
+                pool_cnt = cg;
+                pool_lane = 0;
+                if (lane < cg) {
+                    const float4 b = reinterpret_cast<const float4*>(rays)[2 * (cb + lane) + 1];
+                    WRayPre q;
+                    wb_ray_setup(q, 0.0f, 0.0f, 0.0f, 0.0f, b.x, b.y, b.z, 0.0f);
+                    p_ix = q.ix; p_iy = q.iy; p_iz = q.iz;
+                    p_sx = q.Sx; p_sy = q.Sy; p_sz = q.Sz;
+                    p_meta = q.oct | ((uint32_t)q.kx << 3) | ((uint32_t)q.ky << 5) | ((uint32_t)q.kz << 7);
+                }
             }
-            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
-                                   __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
-            if (!valid && rank < take0 + got1) {
-                ray_id = rank < take0 ? base0 + rank : base1 + (rank - take0);
-                const float4 a = reinterpret_cast<const float4*>(rays)[2 * ray_id];
-                const float4 b = reinterpret_cast<const float4*>(rays)[2 * ray_id + 1];
-                wb_ray_setup(r, a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w);
-                best_t = b.w; best_id = WBVH_MISS; h0 = h1 = 0.0f; hdet = 1.0f;
-                sp = 0;                        // nd already holds the root
-                pend = 0; pend2 = 0; more = true;
-                valid = true;
+            const uint32_t take = min(nidle, pool_cnt);
+            if (take != 0) {
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                       __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                const int src = (int)(pool_lane + min(rank, take - 1u));
+                const float ix = __shfl(p_ix, src), iy = __shfl(p_iy, src), iz = __shfl(p_iz, src);
+                const float sx = __shfl(p_sx, src), sy = __shfl(p_sy, src), sz = __shfl(p_sz, src);
+                const uint32_t meta = (uint32_t)__shfl((int)p_meta, src);
+                if (!valid && rank < take) {
+                    ray_id = pool_base + rank;
+                    const float4 a = reinterpret_cast<const float4*>(rays)[2 * ray_id];
+                    const float4 b = reinterpret_cast<const float4*>(rays)[2 * ray_id + 1];
+                    r.ox = a.x; r.oy = a.y; r.oz = a.z; r.tmin = a.w;
+                    r.dx = b.x; r.dy = b.y; r.dz = b.z; r.tmax = b.w;
+                    r.ix = ix; r.iy = iy; r.iz = iz;
+                    r.Sx = sx; r.Sy = sy; r.Sz = sz;
+                    r.oct = meta & 7u;
+                    r.kx = (int)((meta >> 3) & 3u); r.ky = (int)((meta >> 5) & 3u); r.kz = (int)((meta >> 7) & 3u);
+                    best_t = b.w; best_id = WBVH_MISS; h0 = h1 = 0.0f; hdet = 1.0f;
+                    sp = 0;                        // nd already holds the root
+                    pend = 0; pend2 = 0; more = true;
+                    valid = true;
+                }
+                pool_base += take;
+                pool_cnt -= take;
+                pool_lane += take;
             }
         }
         // single loop back-edge: with no ray in any lane and work left, the
