@@ -7,11 +7,14 @@
 //
 // Node: 80 bytes = five 16-byte loads.  Child boxes are quantised per axis
 // to a grid of 255 steps from the node origin, the step a power of two,
-// rounded outward, and stored as bytes.  The box test turns a pair of bytes
-// into a pair of binary16 subnormals q * 2^-24 with one byte permute
-// (v_perm_b32: each byte below a zero byte), which its FMAs take as
-// operands directly (v_fma_mix_f32, exact conversion); the node's scale
-// exponent carries the 2^24.  Up to eight children per node; internal
+// rounded outward, and stored as bytes, a slot's lower and upper bound
+// side by side, so one 4-byte word holds both bounds of two slots on one
+// axis.  The box test turns the near (or far) bounds of such a word into a
+// pair of binary16 subnormals q * 2^-24 with one byte permute (v_perm_b32:
+// each chosen byte below a zero byte; the choice of lower or upper bound by
+// the ray's direction sign is in the permute's selector), which its FMAs
+// take as operands directly (v_fma_mix_f32, exact conversion); the node's
+// scale exponent carries the 2^24.  Up to eight children per node; internal
 // children of a node are stored contiguously (child_base + rank), and the
 // triangles of all leaf children of a node are stored contiguously in slot
 // order.  Each leaf holds 1..3
@@ -46,8 +49,7 @@ struct alignas(16) WNode {
     uint32_t tri_base;         // index of the first triangle of leaf children
     uint32_t tvalid;           // leaf triangles: bit 3 s + k set for the k-th triangle of leaf slot s
     uint32_t reserved;
-    uint8_t qlo[3][8];         // quantised child lower bounds, per axis (grid units)
-    uint8_t qhi[3][8];         // quantised child upper bounds, per axis (grid units)
+    uint8_t q[3][8][2];        // quantised child bounds per axis and slot: lower, upper (grid units)
 };
 static_assert(sizeof(WNode) == 80, "WNode must be 80 bytes");
 

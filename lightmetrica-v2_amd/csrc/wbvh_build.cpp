@@ -409,8 +409,8 @@ const char* wbvh_build(const float* verts, uint32_t nvert, const uint32_t* tris,
                 double qh = std::ceil(((double)c.hi[a] - (double)org[a]) / sc);
                 ql = std::max(0.0, std::min((double)WBVH_QMAX, ql));
                 qh = std::max(0.0, std::min((double)WBVH_QMAX, qh));
-                wn.qlo[a][s] = (uint8_t)ql;
-                wn.qhi[a][s] = (uint8_t)qh;
+                wn.q[a][s][0] = (uint8_t)ql;
+                wn.q[a][s][1] = (uint8_t)qh;
             }
             if (c.left >= 0) {
                 wn.imask |= (uint8_t)(1u << s);

@@ -92,16 +92,21 @@ WB_HD void wb_ray_setup(WRayPre& r, float ox, float oy, float oz, float tmin,
     r.Sz = 1.0f / dkz;
 }
 
-// Grid coordinates of children 2p and 2p + 1 of one axis (bytes 2p and
-// 2p + 1 of a word holding children 0..3 or 4..7, p = 0 or 1) as a pair of
-// binary16 subnormals q * 2^-24: each byte below a zero byte, one v_perm_b32
-// on the device.  The conversion to FP32 is exact.
-WB_HD wb_h2 wb_q_pair(uint32_t word, int p)
+// Two grid coordinates of a node word (bytes: lower and upper bound of
+// slot 2w, then of slot 2w + 1, on one axis) as a pair of binary16
+// subnormals q * 2^-24: the selector takes byte 0 or 1 and byte 2 or 3,
+// each below a zero byte (selector byte 0x0c), one v_perm_b32 on the
+// device.  The conversion to FP32 is exact.
+WB_HD wb_h2 wb_q_pair(uint32_t word, uint32_t sel)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    const uint32_t v = __builtin_amdgcn_perm(0u, word, p ? 0x0c030c02u : 0x0c010c00u);
+    const uint32_t v = __builtin_amdgcn_perm(0u, word, sel);
 #else
-    const uint32_t v = p ? (((word >> 16) & 0xffu) | ((word >> 8) & 0xff0000u)) : ((word & 0xffu) | ((word & 0xff00u) << 8));
+    uint32_t v = 0;
+    for (int i = 0; i < 4; i++) {
+        const uint32_t s = (sel >> (8 * i)) & 0xffu;
+        if (s < 4u) v |= ((word >> (8 * s)) & 0xffu) << (8 * i);
+    }
 #endif
     return __builtin_bit_cast(wb_h2, v);
 }
@@ -120,26 +125,24 @@ WB_HD uint32_t wb_slot_hits(const WNode& n, const WRayPre& r, float tmax)
     // Near and far planes per axis chosen once per node by the sign of the
     // slab scale (for lo <= hi, fma(lo, s, b) <= fma(hi, s, b) exactly when
     // s >= 0), instead of a min and a max per plane pair and child: the
-    // byte words are selected, then each byte pair becomes a binary16 pair.
-    const uint32_t* lw = reinterpret_cast<const uint32_t*>(&n.qlo[0][0]);
-    const uint32_t* hw = reinterpret_cast<const uint32_t*>(&n.qhi[0][0]);
+    // choice is the byte permute's selector.
+    const uint32_t* qw = reinterpret_cast<const uint32_t*>(&n.q[0][0][0]);
     const bool neg[3] = {sx < 0.0f, sy < 0.0f, sz < 0.0f};
     wb_h2 nq[3][4], fq[3][4];
 #if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll
 #endif
-    for (int a = 0; a < 3; a++)
+    for (int a = 0; a < 3; a++) {
+        const uint32_t nsel = neg[a] ? 0x0c030c01u : 0x0c020c00u;
+        const uint32_t fsel = nsel ^ 0x00010001u;
 #if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll
 #endif
-        for (int w = 0; w < 2; w++) {
-            const uint32_t nw = neg[a] ? hw[2 * a + w] : lw[2 * a + w];
-            const uint32_t fw = neg[a] ? lw[2 * a + w] : hw[2 * a + w];
-            nq[a][2 * w] = wb_q_pair(nw, 0);
-            nq[a][2 * w + 1] = wb_q_pair(nw, 1);
-            fq[a][2 * w] = wb_q_pair(fw, 0);
-            fq[a][2 * w + 1] = wb_q_pair(fw, 1);
+        for (int w = 0; w < 4; w++) {
+            nq[a][w] = wb_q_pair(qw[4 * a + w], nsel);
+            fq[a][w] = wb_q_pair(qw[4 * a + w], fsel);
         }
+    }
     uint32_t hb = 0;                   // hit slots, bit i <-> slot i
 #if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll

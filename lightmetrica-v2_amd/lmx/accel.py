@@ -25,7 +25,7 @@ FLT_MAX = np.float32(np.finfo(np.float32).max)
 WNODE_DTYPE = np.dtype([
     ("o", np.float32, 3), ("e", np.uint8, 3), ("imask", np.uint8),
     ("child_base", np.uint32), ("tri_base", np.uint32), ("tvalid", np.uint32), ("reserved", np.uint32),
-    ("qlo", np.uint8, (3, 8)), ("qhi", np.uint8, (3, 8))])
+    ("q", np.uint8, (3, 8, 2))])
 assert WNODE_DTYPE.itemsize == 80
 WNODE_QBIAS = 151     # grid step of a node axis = 2^(e - WNODE_QBIAS)
 

@@ -124,8 +124,8 @@ def test_occlusion_counting_walker():
 def decode_boxes(node):
     o = node["o"].astype(np.float64)
     sc = np.ldexp(1.0, node["e"].astype(np.int64) - accel.WNODE_QBIAS)
-    lo = o[:, None] + node["qlo"].astype(np.float64) * sc[:, None]
-    hi = o[:, None] + node["qhi"].astype(np.float64) * sc[:, None]
+    lo = o[:, None] + node["q"][:, :, 0].astype(np.float64) * sc[:, None]
+    hi = o[:, None] + node["q"][:, :, 1].astype(np.float64) * sc[:, None]
     return lo, hi      # (3, 8) each
 
 
