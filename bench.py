@@ -981,7 +981,7 @@ def main():
                        "address (TA) and data (TD) units are busy most cycles, the data unit waiting on the "
                        "L1 for much of it (L1 misses to L2), while VALU issue (wave instructions x 4 cycles "
                        "on 1,024 SIMDs against the launch's cycles at 2.4 GHz) comes next; the nodes and "
-                       "triangles (98 MB) are served mostly by L2 and the Infinity Cache, so fabric traffic "
+                       "triangles (75 MB) are served mostly by L2 and the Infinity Cache, so fabric traffic "
                        "is a fraction of the algorithmic bytes, whose rate exceeds nothing physical",
             "above_hbm_peak_by_cache_reuse": bool(achieved > HBM_PEAK_GBS),
             "alg_bytes_per_launch": int(alg_bytes),
