@@ -370,34 +370,48 @@ struct Mix {
     bool kclamp[8];
 };
 
+// The mixture's arithmetic uses the hardware's exp / log / reciprocal /
+// square root (a few ulp, against the correctly rounded library calls):
+// it runs for every guided vertex, and its densities only need to agree
+// with the directions actually drawn to well within the Monte Carlo noise
+// (relative 1e-5 at most here; the tests bound pdfs at 1e-4).
+RD float mx_exp(float x) { return __expf(x); }
+RD float mx_log(float x) { return __logf(x); }
+RD float mx_div(float a, float b) { return __fdividef(a, b); }
+RD float mx_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 RD void mix_decode(const float* y, Mix& m)
 {
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         const f3 r = mk3(y[3 * k], y[3 * k + 1], y[3 * k + 2]);
-        const float l = sqrtf(dot(r, r));
+        const float l = mx_sqrt(dot(r, r));
         m.mlen[k] = l;
-        m.mu[k] = l > 0.0f ? mk3(r.x / l, r.y / l, r.z / l) : mk3(0.0f, 0.0f, 1.0f);
+        const float il = mx_div(1.0f, l);
+        m.mu[k] = l > 0.0f ? mk3(r.x * il, r.y * il, r.z * il) : mk3(0.0f, 0.0f, 1.0f);
         float lk = y[24 + k];
         m.kclamp[k] = lk < LOGK_MIN || lk > LOGK_MAX;
         lk = fminf(fmaxf(lk, LOGK_MIN), LOGK_MAX);
-        m.kap[k] = expf(lk);
+        m.kap[k] = mx_exp(lk);
     }
     float mx = y[32];
 #pragma unroll
     for (int k = 1; k < 8; k++) mx = fmaxf(mx, y[32 + k]);
     float s = 0.0f;
 #pragma unroll
-    for (int k = 0; k < 8; k++) { m.pi[k] = expf(y[32 + k] - mx); s += m.pi[k]; }
+    for (int k = 0; k < 8; k++) { m.pi[k] = mx_exp(y[32 + k] - mx); s += m.pi[k]; }
+    const float is = mx_div(1.0f, s);
 #pragma unroll
-    for (int k = 0; k < 8; k++) m.pi[k] /= s;
-    m.sig = 1.0f / (1.0f + expf(-y[40]));
+    for (int k = 0; k < 8; k++) m.pi[k] *= is;
+    m.sig = mx_div(1.0f, 1.0f + mx_exp(-y[40]));
     m.alpha = ALPHA_MIN + ALPHA_SPAN * m.sig;
 }
 
+// kappa >= e^-4, so 1 - e^{-2 kappa} >= 0.036 keeps its relative error
+// below 1e-5 without expm1
 RD float vmf_pdf(f3 mu, float k, f3 w)
 {
-    return k / (2.0f * LMX_PI_F * -expm1f(-2.0f * k)) * expf(k * (dot(mu, w) - 1.0f));
+    return mx_div(k, 2.0f * LMX_PI_F * (1.0f - mx_exp(-2.0f * k))) * mx_exp(k * (dot(mu, w) - 1.0f));
 }
 
 RD float mix_pdf(const Mix& m, f3 w, float* v = nullptr)
@@ -415,8 +429,8 @@ RD float mix_pdf(const Mix& m, f3 w, float* v = nullptr)
 // Jakob 2012: cos theta = 1 + log(u1 + (1 - u1) e^{-2 kappa}) / kappa
 RD f3 vmf_sample(f3 mu, float k, float u1, float u2)
 {
-    const float W = fmaxf(1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * k)) / k, -1.0f);
-    const float rr = sqrtf(fmaxf(0.0f, 1.0f - W * W));
+    const float W = fmaxf(1.0f + mx_div(mx_log(u1 + (1.0f - u1) * mx_exp(-2.0f * k)), k), -1.0f);
+    const float rr = mx_sqrt(fmaxf(0.0f, 1.0f - W * W));
     float sn, cs;
     sincospif(2.0f * u2, &sn, &cs);
     f3 b, c;
