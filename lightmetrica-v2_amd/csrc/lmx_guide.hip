@@ -231,7 +231,14 @@ RD bf16x8 encode_chunk(const float* xs, int kc, int g, int r, int mt, float& sx,
     bf16x8 b;
     const int d = kc < 3 ? g : g + 4;
     if (d < 7) {
-        if (kc == 0 || kc == 3) sincospif(xs[(mt * 16 + r) * XS + d], &sx, &cx);
+        // sin / cos of pi x by the hardware's sine and cosine (argument in
+        // revolutions: pi x = 2 pi (x / 2)), ~1e-6 absolute for x in [0, 1],
+        // far inside the bf16 rounding of every operand
+        if (kc == 0 || kc == 3) {
+            const float h = 0.5f * xs[(mt * 16 + r) * XS + d];
+            sx = __builtin_amdgcn_sinf(h);
+            cx = __builtin_amdgcn_cosf(h);
+        }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             b[2 * q] = (__bf16)sx;
