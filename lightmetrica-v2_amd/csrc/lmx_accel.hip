@@ -283,6 +283,23 @@ __device__ __forceinline__ void test_tri(const WTri& tri, const WRayPre& r, floa
     }
 }
 
+// Hit record of a finished ray (b0 = V/det, b1 = W/det for the watertight
+// test; a miss is all zeros but the id).
+template <int ISECT>
+__device__ __forceinline__ void write_hit(WHit* __restrict__ hits, uint32_t ray_id, float best_t, uint32_t best_id,
+                                          float h0, float h1, float hdet)
+{
+    float4 o;
+    if (best_id == WBVH_MISS) {
+        o = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(WBVH_MISS));
+    } else if (ISECT == WBVH_ISECT_WATERTIGHT) {
+        o = make_float4(best_t, h0 / hdet, h1 / hdet, __uint_as_float(best_id));
+    } else {
+        o = make_float4(best_t, h0, h1, __uint_as_float(best_id));
+    }
+    reinterpret_cast<float4*>(hits)[ray_id] = o;
+}
+
 // The LDS stack is sized per launch to the BVH's depth (stack_depth entries
 // of 8 bytes per lane), which is what sets the kernel's occupancy.
 // ANYHIT: stop at the first accepted triangle (shadow rays); the hit record
@@ -322,6 +339,7 @@ void k_trace(const WNode* __restrict__ nodes,
     float p_ix = 0.0f, p_iy = 0.0f, p_iz = 0.0f, p_sx = 0.0f, p_sy = 0.0f, p_sz = 0.0f;
     uint32_t p_meta = 0;
     bool valid = false;
+    bool done = false;                  // finished, hit not yet written
     uint32_t ray_id = 0;
     WRayPre r;
     float best_t = 0.0f, h0 = 0.0f, h1 = 0.0f, hdet = 1.0f;
@@ -351,6 +369,10 @@ void k_trace(const WNode* __restrict__ nodes,
         const uint64_t idle = __ballot(!valid);
         const uint32_t nidle = (uint32_t)__popcll(idle);
         if ((regions_left != 0 || pool_cnt != 0) && (nidle >= refill_min || nidle == TRACE_WAVE)) {
+            if (done) {
+                write_hit<ISECT>(hits, ray_id, best_t, best_id, h0, h1, hdet);
+                done = false;
+            }
             if (pool_cnt == 0) {
                 uint32_t cb = 0, cg = 0;
                 if (lane == 0) {
@@ -476,23 +498,20 @@ void k_trace(const WNode* __restrict__ nodes,
                 pend = pend2; tbase = tbase2; tvalid = tvalid2;
                 pend2 = 0;
             }
-            // 4. ray finished: write its hit and free the lane.  An any-hit
-            //    ray that found a triangle with a next node already loaded
-            //    stops one iteration later, after the root has been loaded.
+            // 4. ray finished: free the lane; its hit is written with those
+            //    of the other lanes finished by then at the next refill (or
+            //    at the end), so the wave runs the divisions of the hit
+            //    record once per refill rather than once per iteration.  An
+            //    any-hit ray that found a triangle with a next node already
+            //    loaded stops one iteration later, after the root has been
+            //    loaded.
             if (pend == 0 && !more && cur == 0) {
-                float4 o;
-                if (best_id == WBVH_MISS) {
-                    o = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(WBVH_MISS));
-                } else if (ISECT == WBVH_ISECT_WATERTIGHT) {
-                    o = make_float4(best_t, h0 / hdet, h1 / hdet, __uint_as_float(best_id));
-                } else {
-                    o = make_float4(best_t, h0, h1, __uint_as_float(best_id));
-                }
-                reinterpret_cast<float4*>(hits)[ray_id] = o;
                 valid = false;
+                done = true;
             }
         }
     }
+    if (done) write_hit<ISECT>(hits, ray_id, best_t, best_id, h0, h1, hdet);
 }
 
 // ---------------------------------------------------------------------------
