@@ -301,7 +301,8 @@ __device__ __forceinline__ void write_hit(WHit* __restrict__ hits, uint32_t ray_
 }
 
 // The LDS stack is sized per launch to the BVH's depth (stack_depth entries
-// of 8 bytes per lane), which is what sets the kernel's occupancy.
+// of 8 bytes per lane); with the 3 KB of node-test tables it bounds the
+// workgroups per CU (22 at depth 8, where the registers allow 20).
 // ANYHIT: stop at the first accepted triangle (shadow rays); the hit record
 // then names that triangle, not necessarily the closest.
 template <int ISECT, bool ANYHIT>
