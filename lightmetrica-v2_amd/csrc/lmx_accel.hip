@@ -219,20 +219,45 @@ bool wbvh_walk_anyhit(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* n
 // sticky stack-overflow flag, each on a line of its own
 #define LMX_TRACE_COUNTER_WORDS (10 * CURSOR_STRIDE)
 
-// Resets the 8 region cursors.  The ray count comes from n_dev when given
-// (a queue length written by an earlier kernel), else from n; the count the
-// trace kernel uses is stored after the cursors.
-__global__ void k_init_cursors(uint32_t* cursors, uint32_t n, const uint32_t* n_dev)
+__device__ __forceinline__ uint32_t region_start(uint32_t j, uint32_t n)
 {
-    const uint32_t j = threadIdx.x;
-    const uint32_t nn = n_dev ? *n_dev : n;
-    if (j < 8) cursors[j * CURSOR_STRIDE] = (uint32_t)(((uint64_t)nn * j) / 8u);
-    if (j == 8) cursors[8 * CURSOR_STRIDE] = nn;
+    return (uint32_t)(((uint64_t)n * j) / 8u);
 }
 
 __device__ __forceinline__ uint32_t region_end(uint32_t j, uint32_t n)
 {
     return (uint32_t)(((uint64_t)n * (j + 1u)) / 8u);
+}
+
+// Rays per claim: LMX_CLAIM_CHUNK for full queues; short queues (the tail of
+// a wavefront render) are spread over every wave of the grid, so that their
+// time is that of a few rays per wave, not of whole chunks in turn.
+__device__ __forceinline__ uint32_t trace_claim(uint32_t n, uint32_t grid)
+{
+    return min((uint32_t)LMX_CLAIM_CHUNK, max(1u, (n + grid - 1u) / grid));
+}
+
+// Every wave's first chunk is fixed by its index (wave b, of XCD region
+// r = b mod 8, takes chunk b / 8 of its region); the region's cursor starts
+// after those chunks, so a launch claims with atomics only what is left,
+// and one with few rays (the tail of a wavefront render) with none.
+__device__ __forceinline__ uint32_t region_dyn_start(uint32_t r, uint32_t n, uint32_t grid, uint32_t claim)
+{
+    const uint64_t waves = (grid + 7u - r) / 8u;          // waves b < grid with b mod 8 = r
+    const uint64_t s = (uint64_t)region_start(r, n) + waves * claim;
+    return (uint32_t)min(s, (uint64_t)region_end(r, n));
+}
+
+// Sets the 8 region cursors for a launch of `grid` waves.  The ray count
+// comes from n_dev when given (a queue length written by an earlier
+// kernel), else from n; the count the trace kernel uses is stored after the
+// cursors.
+__global__ void k_init_cursors(uint32_t* cursors, uint32_t n, const uint32_t* n_dev, uint32_t grid)
+{
+    const uint32_t j = threadIdx.x;
+    const uint32_t nn = n_dev ? *n_dev : n;
+    if (j < 8) cursors[j * CURSOR_STRIDE] = region_dyn_start(j, nn, grid, trace_claim(nn, grid));
+    if (j == 8) cursors[8 * CURSOR_STRIDE] = nn;
 }
 
 // The node test's bit shuffles as tables (the internal-child hit mask in
@@ -319,10 +344,7 @@ void k_trace(const WNode* __restrict__ nodes,
     const uint32_t lane = threadIdx.x;
     const uint32_t nray = cursors[8 * CURSOR_STRIDE];
     if (nray == 0) return;                  // uniform over the grid
-    // claim size: LMX_CLAIM_CHUNK for full queues; short queues (the tail of
-    // a wavefront render) are spread over every wave of the grid, so that
-    // their time is that of a few rays per wave, not of whole chunks in turn
-    const uint32_t claim = min((uint32_t)LMX_CLAIM_CHUNK, max(1u, (nray + gridDim.x - 1u) / gridDim.x));
+    const uint32_t claim = trace_claim(nray, gridDim.x);
     __shared__ TraceTables s_tab;
     {
         const uint4* src = reinterpret_cast<const uint4*>(&g_trace_tables);
@@ -339,6 +361,7 @@ void k_trace(const WNode* __restrict__ nodes,
     uint32_t pool_lane = 0;
     float p_ix = 0.0f, p_iy = 0.0f, p_iz = 0.0f, p_sx = 0.0f, p_sy = 0.0f, p_sz = 0.0f;
     uint32_t p_meta = 0;
+    bool first = true;                  // the fixed first chunk is still to take
     bool valid = false;
     bool done = false;                  // finished, hit not yet written
     uint32_t ray_id = 0;
@@ -376,23 +399,39 @@ void k_trace(const WNode* __restrict__ nodes,
             }
             if (pool_cnt == 0) {
                 uint32_t cb = 0, cg = 0;
-                if (lane == 0) {
-                    while (regions_left != 0) {
-                        const uint32_t end = region_end(region, nray);
-                        const uint32_t old = atomicAdd(&cursors[region * CURSOR_STRIDE], claim);
-                        if (old < end) {
-                            cb = old;
-                            cg = min(claim, end - old);
-                            break;
-                        }
-                        region = (region + 1u) & 7u;
-                        regions_left--;
+                if (first) {                       // this wave's fixed chunk
+                    first = false;
+                    const uint32_t r0 = blockIdx.x & 7u;
+                    const uint64_t s0 = (uint64_t)region_start(r0, nray) + (uint64_t)(blockIdx.x >> 3) * claim;
+                    const uint32_t e0 = region_end(r0, nray);
+                    if (s0 < e0) {
+                        cb = (uint32_t)s0;
+                        cg = min(claim, e0 - cb);
                     }
                 }
-                cb = __shfl(cb, 0);
-                cg = __shfl(cg, 0);
-                regions_left = __shfl(regions_left, 0);
-                region = __shfl(region, 0);
+                if (cg == 0) {
+                    if (lane == 0) {
+                        while (regions_left != 0) {
+                            const uint32_t end = region_end(region, nray);
+                            // a region whose fixed chunks cover it has
+                            // nothing for the cursor
+                            if (region_dyn_start(region, nray, gridDim.x, claim) < end) {
+                                const uint32_t old = atomicAdd(&cursors[region * CURSOR_STRIDE], claim);
+                                if (old < end) {
+                                    cb = old;
+                                    cg = min(claim, end - old);
+                                    break;
+                                }
+                            }
+                            region = (region + 1u) & 7u;
+                            regions_left--;
+                        }
+                    }
+                    cb = __shfl(cb, 0);
+                    cg = __shfl(cg, 0);
+                    regions_left = __shfl(regions_left, 0);
+                    region = __shfl(region, 0);
+                }
                 pool_base = cb;
                 pool_cnt = cg;
                 pool_lane = 0;
@@ -645,7 +684,7 @@ static int enqueue_trace(const lmx_accel* a, uint32_t* counters, const WRay* ray
 {
     uint32_t* overflow = a->d_counters + 9 * CURSOR_STRIDE;
     const int grid = std::max(1, std::min(a->grid, (int)((n_max + TRACE_WAVE - 1) / TRACE_WAVE)));
-    k_init_cursors<<<1, 64, 0, s>>>(counters, n, n_dev);
+    k_init_cursors<<<1, 64, 0, s>>>(counters, n, n_dev, (uint32_t)grid);
     const bool wt = a->host.isect == WBVH_ISECT_WATERTIGHT;
     if (wt && !anyhit)
         k_trace<WBVH_ISECT_WATERTIGHT, false><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
