@@ -54,6 +54,21 @@ def test_ragged_ray_counts(gpu, n):
     np.testing.assert_array_equal(hg.view(np.uint32), ho.view(np.uint32))
 
 
+@pytest.mark.parametrize("n", [327679, 327680, 327681, 400003])
+def test_fixed_and_claimed_chunks(gpu, n):
+    """Ray counts around 5,120 waves x 64 rays, where the waves' fixed first
+    chunks stop covering the batch and the rest is claimed through the
+    region cursors: every ray is traced exactly once (bit-exact against the
+    host walker over the same BVH)."""
+    v, f = scenes.displaced_icosphere(4, seed=6)
+    _, rays = scenes.traversal_rays(v, res=4, n_incoherent=n, seed=10)
+    a = accel.Accel(v, f)
+    hg = _gpu_hits(a, rays, gpu)
+    hh = a.intersect_host(rays)
+    assert (accel.hit_ids(hh) != accel.MISS).sum() > n // 10
+    np.testing.assert_array_equal(hg.view(np.uint32), hh.view(np.uint32))
+
+
 def test_full_size_traversal_benchmark(gpu):
     v, f = scenes.traversal_scene()
     assert f.shape[0] == 1310720
