@@ -531,9 +531,11 @@ int wavefront_render(lmx_scene* sc, const RParams& rp_in, uint32_t cap, hipStrea
     k_commit<<<1, 64, 0, st>>>(sc->ctl.p, cap, rp.total);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "wavefront start");
-    // capture two iterations (0 -> 1, 1 -> 0) in a graph
+    // capture two iterations (0 -> 1, 1 -> 0) in a graph; the scene keeps
+    // the executable graph of its last render and updates it in place with
+    // this capture (same kernels, new arguments), which costs a fraction of
+    // an instantiation -- a few hundred microseconds on a small frame
     hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
     if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess)
         return hip_error(e, "graph capture begin");
     rc = enqueue_iteration(sc, rp, 0, 1, st);
@@ -541,10 +543,21 @@ int wavefront_render(lmx_scene* sc, const RParams& rp_in, uint32_t cap, hipStrea
     e = hipStreamEndCapture(st, &graph);
     if (rc != LMX_OK) { if (graph) (void)hipGraphDestroy(graph); return rc; }
     if (e != hipSuccess) return hip_error(e, "graph capture end");
-    if ((e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0)) != hipSuccess) {
+    if (sc->wf_exec) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        if (hipGraphExecUpdate(sc->wf_exec, graph, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(sc->wf_exec);
+            sc->wf_exec = nullptr;
+        }
+    }
+    if (!sc->wf_exec && (e = hipGraphInstantiate(&sc->wf_exec, graph, nullptr, nullptr, 0)) != hipSuccess) {
+        sc->wf_exec = nullptr;
         (void)hipGraphDestroy(graph);
         return hip_error(e, "graph instantiate");
     }
+    hipGraphExec_t exec = sc->wf_exec;
     // replay until every sample is issued and every queue is empty; the
     // control block is read back every `batch` replays.  Time mode: once the
     // deadline has passed no new camera sample starts, and the replays go on
@@ -570,7 +583,6 @@ int wavefront_render(lmx_scene* sc, const RParams& rp_in, uint32_t cap, hipStrea
             stopped = true;
         }
     }
-    (void)hipGraphExecDestroy(exec);
     (void)hipGraphDestroy(graph);
     if (rays) *rays = sc->h_ctl->rays + sc->h_ctl->shadow_rays;
     if (iters) *iters = sc->h_ctl->iterations;

@@ -542,6 +542,7 @@ struct lmx_scene {
     DevBuf<uint32_t> perm;              // material-binned shading, SHADE_CLASSES x cap
     DevBuf<Ctl> ctl;
     Ctl* h_ctl = nullptr;      // pinned
+    hipGraphExec_t wf_exec = nullptr;   // the wavefront iterations' graph, updated in place per render
     uint32_t tuned_cap = 0;
     // bidirectional sampler buffers (sized for bd_batch samples, bd_maxv vertices)
     uint32_t bd_batch = 0, bd_maxv = 0, bd_conn = 0;   // bd_conn: shadow-queue slots per sample and round
@@ -574,6 +575,7 @@ struct lmx_scene {
     DevBuf<float4> ppm_st;              // renderer::ppm measurement-point state, 2 per point
     ~lmx_scene()
     {
+        if (wf_exec) (void)hipGraphExecDestroy(wf_exec);
         if (h_ctl) (void)hipHostFree(h_ctl);
         if (bd_hctl) (void)hipHostFree(bd_hctl);
         lmx_accel_free(accel);
