@@ -25,7 +25,9 @@
 //   * XCD-aware work split: workgroup b runs on XCD (b mod 8); the rays are
 //     cut into 8 contiguous regions and a wave first drains the region of
 //     its own XCD (neighbouring rays share BVH nodes in that XCD's L2), then
-//     helps the others.
+//     helps the others; its first chunk is fixed by its index, and only
+//     what the fixed chunks leave is claimed through the regions' atomic
+//     cursors.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
