@@ -43,6 +43,7 @@
 #include "lmx_internal.h"
 #include "wbvh_build.h"
 #include "wbvh_math.h"
+#include "trace_cursors.h"
 
 namespace lmx {
 
@@ -207,48 +208,7 @@ bool wbvh_walk_anyhit(const WbvhHost& b, const WRay& ray, WHit& hit, uint64_t* n
 #ifndef LMX_TRACE_WAVES_PER_EU
 #define LMX_TRACE_WAVES_PER_EU 1   // occupancy hint for the register allocator (5 measured 2 % slower)
 #endif
-#ifndef LMX_CLAIM_CHUNK
-#define LMX_CLAIM_CHUNK 64     // rays claimed from a region cursor per atomic
-#endif
 #define REFILL_MIN LMX_REFILL_MIN
-// words between two region cursors: each cursor sits in a 256-byte line of
-// its own, so the atomics of different regions do not serialise on one line
-#ifndef LMX_CURSOR_STRIDE
-#define LMX_CURSOR_STRIDE 64
-#endif
-#define CURSOR_STRIDE LMX_CURSOR_STRIDE
-// counter block of one trace launch: 8 region cursors, the ray count, and a
-// sticky stack-overflow flag, each on a line of its own
-#define LMX_TRACE_COUNTER_WORDS (10 * CURSOR_STRIDE)
-
-__device__ __forceinline__ uint32_t region_start(uint32_t j, uint32_t n)
-{
-    return (uint32_t)(((uint64_t)n * j) / 8u);
-}
-
-__device__ __forceinline__ uint32_t region_end(uint32_t j, uint32_t n)
-{
-    return (uint32_t)(((uint64_t)n * (j + 1u)) / 8u);
-}
-
-// Rays per claim: LMX_CLAIM_CHUNK for full queues; short queues (the tail of
-// a wavefront render) are spread over every wave of the grid, so that their
-// time is that of a few rays per wave, not of whole chunks in turn.
-__device__ __forceinline__ uint32_t trace_claim(uint32_t n, uint32_t grid)
-{
-    return min((uint32_t)LMX_CLAIM_CHUNK, max(1u, (n + grid - 1u) / grid));
-}
-
-// Every wave's first chunk is fixed by its index (wave b, of XCD region
-// r = b mod 8, takes chunk b / 8 of its region); the region's cursor starts
-// after those chunks, so a launch claims with atomics only what is left,
-// and one with few rays (the tail of a wavefront render) with none.
-__device__ __forceinline__ uint32_t region_dyn_start(uint32_t r, uint32_t n, uint32_t grid, uint32_t claim)
-{
-    const uint64_t waves = (grid + 7u - r) / 8u;          // waves b < grid with b mod 8 = r
-    const uint64_t s = (uint64_t)region_start(r, n) + waves * claim;
-    return (uint32_t)min(s, (uint64_t)region_end(r, n));
-}
 
 // Sets the 8 region cursors for a launch of `grid` waves.  The ray count
 // comes from n_dev when given (a queue length written by an earlier
@@ -256,10 +216,7 @@ __device__ __forceinline__ uint32_t region_dyn_start(uint32_t r, uint32_t n, uin
 // cursors.
 __global__ void k_init_cursors(uint32_t* cursors, uint32_t n, const uint32_t* n_dev, uint32_t grid)
 {
-    const uint32_t j = threadIdx.x;
-    const uint32_t nn = n_dev ? *n_dev : n;
-    if (j < 8) cursors[j * CURSOR_STRIDE] = region_dyn_start(j, nn, grid, trace_claim(nn, grid));
-    if (j == 8) cursors[8 * CURSOR_STRIDE] = nn;
+    trace_cursors_set(cursors, n_dev ? *n_dev : n, grid, threadIdx.x);
 }
 
 // The node test's bit shuffles as tables (the internal-child hit mask in
@@ -681,12 +638,18 @@ namespace lmx {
 // device, with its region cursors and ray count at `counters` (9 words,
 // CURSOR_STRIDE apart, of device memory); the stack-overflow flag is always
 // the accel's own.
+static int trace_grid(const lmx_accel* a, uint32_t n_max)
+{
+    return std::max(1, std::min(a->grid, (int)((n_max + TRACE_WAVE - 1) / TRACE_WAVE)));
+}
+
 static int enqueue_trace(const lmx_accel* a, uint32_t* counters, const WRay* rays, uint32_t n,
-                         const uint32_t* n_dev, uint32_t n_max, WHit* hits, hipStream_t s, bool anyhit)
+                         const uint32_t* n_dev, uint32_t n_max, WHit* hits, hipStream_t s, bool anyhit,
+                         bool cursors_set = false)
 {
     uint32_t* overflow = a->d_counters + 9 * CURSOR_STRIDE;
-    const int grid = std::max(1, std::min(a->grid, (int)((n_max + TRACE_WAVE - 1) / TRACE_WAVE)));
-    k_init_cursors<<<1, 64, 0, s>>>(counters, n, n_dev, (uint32_t)grid);
+    const int grid = trace_grid(a, n_max);
+    if (!cursors_set) k_init_cursors<<<1, 64, 0, s>>>(counters, n, n_dev, (uint32_t)grid);
     const bool wt = a->host.isect == WBVH_ISECT_WATERTIGHT;
     if (wt && !anyhit)
         k_trace<WBVH_ISECT_WATERTIGHT, false><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
@@ -706,11 +669,20 @@ static int enqueue_trace(const lmx_accel* a, uint32_t* counters, const WRay* ray
 }
 
 int launch_trace(lmx_accel* a, const WRay* rays, uint32_t n, const uint32_t* n_dev, uint32_t n_max,
-                 WHit* hits, hipStream_t s, bool anyhit)
+                 WHit* hits, hipStream_t s, bool anyhit, bool cursors_set)
 {
     int rc = lmx_accel_upload(a);
     if (rc != LMX_OK) return rc;
-    return enqueue_trace(a, a->d_counters, rays, n, n_dev, n_max, hits, s, anyhit);
+    return enqueue_trace(a, a->d_counters, rays, n, n_dev, n_max, hits, s, anyhit, cursors_set);
+}
+
+int trace_shape(lmx_accel* a, uint32_t n_max, uint32_t** counters, uint32_t* grid)
+{
+    int rc = lmx_accel_upload(a);
+    if (rc != LMX_OK) return rc;
+    *counters = a->d_counters;
+    *grid = (uint32_t)trace_grid(a, n_max);
+    return LMX_OK;
 }
 
 }  // namespace lmx

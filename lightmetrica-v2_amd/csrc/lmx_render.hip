@@ -29,6 +29,7 @@
 
 #include "lmx.h"
 #include "lmx_internal.h"
+#include "trace_cursors.h"
 #include "rmath.h"
 #include "rshade.h"
 #include "wbvh.h"
@@ -365,17 +366,25 @@ __global__ __launch_bounds__(256) void k_nee_splat(RParams p, Ctl* __restrict__ 
     }
 }
 
-__global__ void k_commit(Ctl* ctl, uint32_t cap, uint32_t total)
+// Commits the generation: the queue just written becomes the one to trace.
+// Also sets the next trace launch's cursors (so that launch needs no cursor
+// kernel of its own), for `grid` workgroups.
+__global__ void k_commit(Ctl* ctl, uint32_t cap, uint32_t total, uint32_t* cursors, uint32_t grid)
 {
-    if (threadIdx.x != 0) return;
-    const uint32_t want = gen_count(ctl, cap, total);
-    ctl->shadow_rays += ctl->n_shadow;
-    ctl->n_shadow = 0;
-    ctl->rays += ctl->n_trace;
-    ctl->n_trace = ctl->n_next + want;
-    ctl->n_next = 0;
-    ctl->issued += want;
-    ctl->iterations += 1;
+    __shared__ uint32_t n_next;
+    if (threadIdx.x == 0) {
+        const uint32_t want = gen_count(ctl, cap, total);
+        ctl->shadow_rays += ctl->n_shadow;
+        ctl->n_shadow = 0;
+        ctl->rays += ctl->n_trace;
+        ctl->n_trace = ctl->n_next + want;
+        n_next = ctl->n_trace;
+        ctl->n_next = 0;
+        ctl->issued += want;
+        ctl->iterations += 1;
+    }
+    __syncthreads();
+    trace_cursors_set(cursors, n_next, grid, threadIdx.x);
 }
 
 __global__ void k_init_ctl(Ctl* ctl)
@@ -483,7 +492,7 @@ int enqueue_iteration(lmx_scene* sc, const RParams& rp, int a, int b, hipStream_
         k_capture_advance<<<1, 64, 0, st>>>(sc->ctl.p, rp.ray_out_cap);
     }
     int rc = launch_trace(sc->accel, reinterpret_cast<const WRay*>(qa.ray), 0, &sc->ctl.p->n_trace, sc->cap,
-                          reinterpret_cast<WHit*>(sc->hits.p), st);
+                          reinterpret_cast<WHit*>(sc->hits.p), st, false, true);   // cursors set by k_commit
     if (rc != LMX_OK) return rc;
     const int grid = std::max(1, std::min((int)((sc->cap + 255) / 256), sc->cus * 8));
     if (rp.guide) {
@@ -497,7 +506,10 @@ int enqueue_iteration(lmx_scene* sc, const RParams& rp, int a, int b, hipStream_
     }
     k_gen<<<grid, 256, 0, st>>>(sc->rs, rp, sc->ctl.p, sc->cap, qb);
     if (rp.nee != NEE_NONE && (rc = enqueue_nee(sc, rp, st)) != LMX_OK) return rc;
-    k_commit<<<1, 64, 0, st>>>(sc->ctl.p, sc->cap, rp.total);
+    uint32_t* cursors = nullptr;
+    uint32_t tgrid = 0;
+    if ((rc = trace_shape(sc->accel, sc->cap, &cursors, &tgrid)) != LMX_OK) return rc;
+    k_commit<<<1, 64, 0, st>>>(sc->ctl.p, sc->cap, rp.total, cursors, tgrid);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? LMX_OK : hip_error(e, "wavefront launch");
 }
@@ -528,7 +540,10 @@ int wavefront_render(lmx_scene* sc, const RParams& rp_in, uint32_t cap, hipStrea
     k_gen<<<std::max(1, std::min((int)((cap + 255) / 256), sc->cus * 8)), 256, 0, st>>>(
         sc->rs, rp, sc->ctl.p, cap, Queue{sc->ray[0].p, sc->thr[0].p, sc->meta[0].p});
     if (rp.nee != NEE_NONE && (rc = enqueue_nee(sc, rp, st)) != LMX_OK) return rc;
-    k_commit<<<1, 64, 0, st>>>(sc->ctl.p, cap, rp.total);
+    uint32_t* cursors = nullptr;
+    uint32_t tgrid = 0;
+    if ((rc = trace_shape(sc->accel, cap, &cursors, &tgrid)) != LMX_OK) return rc;
+    k_commit<<<1, 64, 0, st>>>(sc->ctl.p, cap, rp.total, cursors, tgrid);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "wavefront start");
     // capture two iterations (0 -> 1, 1 -> 0) in a graph; the scene keeps
