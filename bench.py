@@ -821,7 +821,7 @@ def main():
     ap.add_argument("--no-vcm", action="store_true", help="skip the vertex-merging section")
     ap.add_argument("--guide-ref-spp", type=int, default=16384,
                     help="spp of the plain reference image of the path-guiding section")
-    ap.add_argument("--only", choices=["pt", "bdpt", "vcm", "guide", "image"], default=None,
+    ap.add_argument("--only", choices=["pt", "pt_cornell", "bdpt", "vcm", "guide", "image"], default=None,
                     help="run one renderer section alone and print its JSON (for profiling runs)")
     ap.add_argument("--no-per-set", action="store_true",
                     help="skip the separate coherent/incoherent timings (for profiling runs)")
@@ -862,6 +862,8 @@ def main():
             out = image_section(dev, not args.no_cpu_baseline)
         elif args.only == "vcm":
             out = vcm_section(dev, not args.no_cpu_baseline, threads)
+        elif args.only == "pt_cornell":
+            out = pt_cornell_section(dev, not args.no_cpu_baseline, threads)
         else:
             fn = pt_section if args.only == "pt" else bdpt_section
             out = fn(dev, not args.no_cpu_baseline, threads)
