@@ -237,12 +237,19 @@ constexpr TraceTables make_trace_tables()
 
 __constant__ TraceTables g_trace_tables = make_trace_tables();
 
-__device__ __forceinline__ void load_node(WNode& nd, const WNode* src)
+// Node idx through a raw buffer resource over the node array: one 32-bit
+// offset per lane instead of a 64-bit address (3 VGPRs and an address
+// computation less per node, measured +1 %), and loads past the array's
+// bytes read zero.
+__device__ __forceinline__ void load_node(WNode& nd, __amdgpu_buffer_rsrc_t rs, uint32_t idx)
 {
-    const uint4* s = reinterpret_cast<const uint4*>(src);
     uint4* d = reinterpret_cast<uint4*>(&nd);
+    const int off = (int)(idx * (uint32_t)sizeof(WNode));
 #pragma unroll
-    for (int i = 0; i < (int)(sizeof(WNode) / 16); i++) d[i] = s[i];
+    for (int i = 0; i < (int)(sizeof(WNode) / 16); i++) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, off + 16 * i, 0, 0);
+        d[i] = make_uint4(v[0], v[1], v[2], v[3]);
+    }
 }
 
 __device__ __forceinline__ void load_tri(WTri& t, const WTri* src)
@@ -297,7 +304,7 @@ void k_trace(const WNode* __restrict__ nodes,
                                                       WHit* __restrict__ hits,
                                                       uint32_t* __restrict__ cursors,
                                                       uint32_t* __restrict__ overflow,
-                                                      int stack_depth, uint32_t refill_min)
+                                                      int stack_depth, uint32_t refill_min, uint32_t node_bytes)
 {
     extern __shared__ uint2 stack[];     // [stack_depth][TRACE_WAVE]
     const uint32_t lane = threadIdx.x;
@@ -339,8 +346,12 @@ void k_trace(const WNode* __restrict__ nodes,
     uint32_t pend = 0, tbase = 0, tvalid = 0, cur = 0;
     uint32_t pend2 = 0, tbase2 = 0, tvalid2 = 0;
     bool more = false;
+    // raw buffer (stride 0) over the node_bytes of the nodes; dword 3 of the
+    // resource is the gfx9-family raw-buffer format
+    const __amdgpu_buffer_rsrc_t nrs = __builtin_amdgcn_make_buffer_rsrc((void*)nodes, (short)0, (int)node_bytes,
+                                                                         0x00020000);
     WNode nd;                           // node to test in the next iteration
-    load_node(nd, nodes);               // idle lanes always hold the root
+    load_node(nd, nrs, 0u);             // idle lanes always hold the root
 
     while (true) {
         // ---- batch refill of idle lanes from this wave's pool of claimed
@@ -482,7 +493,7 @@ void k_trace(const WNode* __restrict__ nodes,
         }
         // 2. load the next node; its latency overlaps the triangle test below
         cur = child;
-        load_node(nd, nodes + child);
+        load_node(nd, nrs, child);
         if (valid) {
             // 3. one triangle of the tested node
             if (pend) {
@@ -649,20 +660,21 @@ static int enqueue_trace(const lmx_accel* a, uint32_t* counters, const WRay* ray
 {
     uint32_t* overflow = a->d_counters + 9 * CURSOR_STRIDE;
     const int grid = trace_grid(a, n_max);
+    const uint32_t node_bytes = (uint32_t)(a->host.nodes.size() * sizeof(WNode));   // < 4 GB, the builder's bound
     if (!cursors_set) k_init_cursors<<<1, 64, 0, s>>>(counters, n, n_dev, (uint32_t)grid);
     const bool wt = a->host.isect == WBVH_ISECT_WATERTIGHT;
     if (wt && !anyhit)
         k_trace<WBVH_ISECT_WATERTIGHT, false><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
-            a->d_nodes, a->d_tris, rays, hits, counters, overflow, a->stack_depth, REFILL_MIN);
+            a->d_nodes, a->d_tris, rays, hits, counters, overflow, a->stack_depth, REFILL_MIN, node_bytes);
     else if (wt)
         k_trace<WBVH_ISECT_WATERTIGHT, true><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
-            a->d_nodes, a->d_tris, rays, hits, counters, overflow, a->stack_depth, REFILL_MIN);
+            a->d_nodes, a->d_tris, rays, hits, counters, overflow, a->stack_depth, REFILL_MIN, node_bytes);
     else if (!anyhit)
         k_trace<WBVH_ISECT_TRIACCEL, false><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
-            a->d_nodes, a->d_tris, rays, hits, counters, overflow, a->stack_depth, REFILL_MIN);
+            a->d_nodes, a->d_tris, rays, hits, counters, overflow, a->stack_depth, REFILL_MIN, node_bytes);
     else
         k_trace<WBVH_ISECT_TRIACCEL, true><<<grid, TRACE_WAVE, a->lds_bytes, s>>>(
-            a->d_nodes, a->d_tris, rays, hits, counters, overflow, a->stack_depth, REFILL_MIN);
+            a->d_nodes, a->d_tris, rays, hits, counters, overflow, a->stack_depth, REFILL_MIN, node_bytes);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_error(e, "trace launch");
     return LMX_OK;

@@ -432,6 +432,8 @@ const char* wbvh_build(const float* verts, uint32_t nvert, const uint32_t* tris,
     out.max_depth = max_depth;
     out.ntri = ntri;
     if (max_depth > WBVH_STACK) return "BVH deeper than the traversal stack";
+    // the traversal kernel reads nodes through 32-bit byte offsets
+    if ((uint64_t)out.nodes.size() * sizeof(WNode) > 0xffffffffull) return "BVH nodes exceed 4 GB";
     return nullptr;
 }
 
