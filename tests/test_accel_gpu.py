@@ -54,6 +54,25 @@ def test_ragged_ray_counts(gpu, n):
     np.testing.assert_array_equal(hg.view(np.uint32), ho.view(np.uint32))
 
 
+@pytest.mark.parametrize("scale,shift", [(1e-12, 0.0), (1e12, 0.0), (1.0, 1e6), (1e-6, 1.0)])
+def test_extreme_scales_bitexact_vs_bruteforce(gpu, scale, shift):
+    """Scene sizes from 1e-12 to 1e12 and far from the origin (ray origins
+    and tmin scaled with the scene): the kernel's byte-grid box test, with
+    its binary16 subnormal operands, still finds exactly the brute force's
+    hits."""
+    v, f = scenes.displaced_icosphere(4, seed=3)
+    coh, inc = scenes.traversal_rays(v, res=48, n_incoherent=4096)
+    rays = np.concatenate([coh, inc]).copy()
+    v = (v.astype(np.float64) * scale + shift).astype(np.float32)
+    rays[:, 0:3] = (rays[:, 0:3].astype(np.float64) * scale + shift).astype(np.float32)
+    rays[:, 3] = np.float32(rays[0, 3] * scale)
+    a = accel.Accel(v, f)
+    hg = _gpu_hits(a, rays, gpu)
+    ho = orc.closest_hit_naive(v, f, rays, test=orc.TEST_WATERTIGHT, tie=orc.TIE_MIN_ID)
+    assert (orc.hit_ids(ho) != orc.MISS).sum() > 1000
+    np.testing.assert_array_equal(hg.view(np.uint32), ho.view(np.uint32))
+
+
 @pytest.mark.parametrize("n", [327679, 327680, 327681, 400003])
 def test_fixed_and_claimed_chunks(gpu, n):
     """Ray counts around 5,120 waves x 64 rays, where the waves' fixed first

@@ -85,6 +85,25 @@ def test_host_walker_bitexact_vs_bruteforce(name, mode):
     np.testing.assert_array_equal(hh.view(np.uint32), ho.view(np.uint32))
 
 
+@pytest.mark.parametrize("scale,shift", [(1e-12, 0.0), (1e12, 0.0), (1.0, 1e6), (1e-6, 1.0)])
+def test_host_walker_bitexact_at_extreme_scales(scale, shift):
+    """The byte grid's power-of-two steps and the binary16 subnormals of the
+    box test at scene sizes from 1e-12 to 1e12 (beyond them the watertight
+    test's own FP32 products leave range) and far from the origin: the
+    walker still finds exactly the brute force's hits (ray origins and tmin
+    scaled with the scene)."""
+    v, f = mesh("sphere4")
+    v = (v.astype(np.float64) * scale + shift).astype(np.float32)
+    rays = rays_for(mesh("sphere4")[0], n=3000).copy()
+    rays[:, 0:3] = (rays[:, 0:3].astype(np.float64) * scale + shift).astype(np.float32)
+    rays[:, 3] = np.float32(rays[0, 3] * scale)
+    a = accel.Accel(v, f)
+    hh = a.intersect_host(rays)
+    ho = orc.closest_hit_naive(v, f, rays, test=orc.TEST_WATERTIGHT, tie=orc.TIE_MIN_ID)
+    assert (orc.hit_ids(ho) != orc.MISS).sum() > 500
+    np.testing.assert_array_equal(hh.view(np.uint32), ho.view(np.uint32))
+
+
 def test_counting_walker_matches_host_walk_cost():
     v, f = mesh("sphere6")
     rays = rays_for(v, n=3000)
