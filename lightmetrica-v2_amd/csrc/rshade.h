@@ -543,6 +543,7 @@ struct lmx_scene {
     DevBuf<Ctl> ctl;
     Ctl* h_ctl = nullptr;      // pinned
     hipGraphExec_t wf_exec = nullptr;   // the wavefront iterations' graph, updated in place per render
+    hipStream_t cap_stream = nullptr;   // records that graph and runs nothing
     uint32_t tuned_cap = 0;
     // bidirectional sampler buffers (sized for bd_batch samples, bd_maxv vertices)
     uint32_t bd_batch = 0, bd_maxv = 0, bd_conn = 0;   // bd_conn: shadow-queue slots per sample and round
@@ -576,6 +577,7 @@ struct lmx_scene {
     ~lmx_scene()
     {
         if (wf_exec) (void)hipGraphExecDestroy(wf_exec);
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
         if (h_ctl) (void)hipHostFree(h_ctl);
         if (bd_hctl) (void)hipHostFree(bd_hctl);
         lmx_accel_free(accel);
