@@ -391,7 +391,7 @@ __global__ void k_init_ctl(Ctl* ctl)
 {
     if (threadIdx.x == 0) {
         ctl->n_trace = 0; ctl->n_next = 0; ctl->issued = 0; ctl->iterations = 0; ctl->rays = 0;
-        ctl->n_shadow = 0; ctl->shadow_rays = 0; ctl->stop = 0; ctl->ray_out_n = 0;
+        ctl->n_shadow = 0; ctl->shadow_rays = 0; ctl->stop = 0; ctl->ray_out_n = 0; ctl->film_scaled = 0;
     }
 }
 
@@ -426,6 +426,18 @@ __global__ void k_zero(float* a, size_t n)
 
 __global__ void k_scale(float* a, size_t n, float w)
 {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] *= w;
+}
+
+// Film::Rescale on the device, queued before each control readback of a
+// render whose sample count is fixed: it scales only once the render is
+// complete (every sample issued, no path left), which that readback then
+// finds, so the host launches no scaling after it.
+__global__ void k_scale_done(Ctl* __restrict__ ctl, float* a, size_t n, uint32_t total, float w)
+{
+    const bool done = ctl->n_trace == 0 && ctl->issued >= total;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->film_scaled = done ? 1u : 0u;
+    if (!done) return;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] *= w;
 }
 
@@ -517,7 +529,7 @@ int enqueue_iteration(lmx_scene* sc, const RParams& rp, int a, int b, hipStream_
 }  // namespace
 
 int wavefront_render(lmx_scene* sc, const RParams& rp_in, uint32_t cap, hipStream_t st, uint64_t* rays,
-                     uint32_t* iters, double render_time, uint64_t* samples)
+                     uint32_t* iters, double render_time, uint64_t* samples, size_t film_n, float film_w)
 {
     int rc = ensure_queues(sc, cap);
     if (rc != LMX_OK) return rc;
@@ -597,6 +609,9 @@ int wavefront_render(lmx_scene* sc, const RParams& rp_in, uint32_t cap, hipStrea
         }
         if (rc != LMX_OK) break;
         batch = 4;
+        if (film_n && render_time <= 0.0)
+            k_scale_done<<<std::max(1, std::min((int)((film_n + 255) / 256), sc->cus * 8)), 256, 0, st>>>(
+                sc->ctl.p, rp.film, film_n, rp.total, film_w);
         if ((e = hipMemcpyAsync(sc->h_ctl, sc->ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, st)) != hipSuccess ||
             (e = hipStreamSynchronize(st)) != hipSuccess) {
             rc = hip_error(e, "control readback");
@@ -719,11 +734,14 @@ static int render_unidir(lmx_scene* sc, const lmx_render_params* p, int nee, con
     uint32_t iters = 0;
     if (rc == LMX_OK) {
         k_zero<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix);
-        rc = wavefront_render(sc, rp, cap, st, &rays, &iters, timed ? p->render_time : 0.0, &taken);
+        // a fixed sample count is rescaled on the device as the render ends
+        rc = wavefront_render(sc, rp, cap, st, &rays, &iters, timed ? p->render_time : 0.0, &taken,
+                              timed ? 0 : 3 * npix, (float)((double)npix / (double)p->num_samples));
     }
     if (rc == LMX_OK) {
         // Film::Rescale by W H / the samples taken (scheduler.cpp:281-288)
-        k_scale<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix, (float)((double)npix / (double)std::max<uint64_t>(taken, 1)));
+        if (!sc->h_ctl->film_scaled)
+            k_scale<<<zgrid, 256, 0, st>>>(film_rgb, 3 * npix, (float)((double)npix / (double)std::max<uint64_t>(taken, 1)));
         if ((e = hipStreamSynchronize(st)) != hipSuccess) rc = hip_error(e, "render sync");
     }
     if (own) (void)hipStreamDestroy(own);

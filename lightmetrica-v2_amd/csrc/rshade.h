@@ -60,6 +60,7 @@ struct Ctl {
     unsigned long long shadow_rays;
     uint32_t bin[4];           // material-binned shading: paths per class this iteration
     uint32_t ray_out_n;        // rays copied to RParams::ray_out so far
+    uint32_t film_scaled;      // the film was rescaled on the device at the render's end
 };
 
 // Shading classes of material-binned shading: paths that end at their hit
@@ -423,8 +424,12 @@ struct lmx_scene;
 // The unidirectional wavefront loop (lmx_render.hip): generation, trace,
 // shade (guided when rp.guide is set), graph-replayed until every sample is
 // done.  `cap` paths in flight.
+// film_n > 0: once every sample is issued and every path is done, the
+// film's first film_n floats are scaled by film_w on the device (sets
+// Ctl::film_scaled; the readback after it ends the loop).
 int wavefront_render(lmx_scene* sc, const lmx::RParams& rp, uint32_t cap, hipStream_t st, uint64_t* rays,
-                     uint32_t* iters, double render_time = 0.0, uint64_t* samples = nullptr);
+                     uint32_t* iters, double render_time = 0.0, uint64_t* samples = nullptr, size_t film_n = 0,
+                     float film_w = 0.0f);
 // The guided shading step of one iteration (lmx_guide.hip).
 int launch_guided_shade(lmx_scene* sc, const lmx::RParams& rp, lmx::Queue in, lmx::Queue out, hipStream_t st);
 
