@@ -135,8 +135,7 @@ class Guide:
             p.table_width = table.shape[1]
         st, gs = lmx_render_stats(), lmx_guide_stats()
         if stream is None:
-            stream = torch.cuda.Stream(dev)
-            stream.wait_stream(torch.cuda.current_stream(dev))
+            stream = sc._render_stream()
         check(lib.lmx_render_guided(sc._h, self._h, ctypes.byref(p), int(train), ctypes.c_void_p(film.data_ptr()),
                                     ctypes.c_void_p(stream.cuda_stream), ctypes.byref(gs), ctypes.byref(st)))
         stats = {"rays": st.rays, "iterations": st.iterations, "queue_capacity": st.queue_capacity, "ms": st.ms}

@@ -56,8 +56,23 @@ def bdpt_table_width(max_num_vertices):
 
 
 class Scene:
+    def _render_stream(self):
+        """The stream a render runs on when the caller names none: one
+        non-default stream per scene and device, made once (a new torch
+        stream costs ~130 us at its first use), ordered after the work
+        already queued on torch's current stream.  Renders return only once
+        their stream is idle, so the next render may reuse it."""
+        import torch
+        idx = torch.cuda.current_device()
+        s = self._streams.get(idx)
+        if s is None:
+            s = self._streams[idx] = torch.cuda.Stream(idx)
+        s.wait_stream(torch.cuda.current_stream(idx))
+        return s
+
     def __init__(self, sc):
         self._keep = []
+        self._streams = {}
 
         def keep(a, dt):
             a = np.ascontiguousarray(a, dtype=dt)
@@ -180,8 +195,7 @@ class Scene:
             p.sample_out = ctypes.c_void_p(so.data_ptr())
         st = lmx_render_stats()
         if stream is None:
-            stream = torch.cuda.Stream(dev)
-            stream.wait_stream(torch.cuda.current_stream(dev))
+            stream = self._render_stream()
         check(lib.lmx_render_vcm(self._h, ctypes.byref(p), ctypes.byref(v), ctypes.c_void_p(film.data_ptr()),
                                  ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st)))
         stats = {"rays": st.rays, "passes": st.iterations, "batch": st.queue_capacity, "ms": st.ms,
@@ -219,8 +233,7 @@ class Scene:
             p.sample_out = ctypes.c_void_p(so.data_ptr())
         st = lmx_render_stats()
         if stream is None:
-            stream = torch.cuda.Stream(dev)
-            stream.wait_stream(torch.cuda.current_stream(dev))
+            stream = self._render_stream()
         check(lib.lmx_render_pm(self._h, ctypes.byref(p), ctypes.byref(v), ctypes.c_void_p(film.data_ptr()),
                                 ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st)))
         stats = {"rays": st.rays, "batches": st.iterations, "batch": st.queue_capacity, "ms": st.ms}
@@ -258,8 +271,7 @@ class Scene:
             p.sample_out = ctypes.c_void_p(so.data_ptr())
         st = lmx_render_stats()
         if stream is None:
-            stream = torch.cuda.Stream(dev)
-            stream.wait_stream(torch.cuda.current_stream(dev))
+            stream = self._render_stream()
         check(lib.lmx_render_ppm(self._h, ctypes.byref(p), ctypes.byref(v), ctypes.c_void_p(film.data_ptr()),
                                  ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st)))
         stats = {"rays": st.rays, "passes": st.iterations, "ms": st.ms}
@@ -298,8 +310,7 @@ class Scene:
             p.sample_out = ctypes.c_void_p(so.data_ptr())
         st = lmx_render_stats()
         if stream is None:
-            stream = torch.cuda.Stream(dev)
-            stream.wait_stream(torch.cuda.current_stream(dev))
+            stream = self._render_stream()
         check(lib.lmx_render_sppm(self._h, ctypes.byref(p), ctypes.byref(v), ctypes.c_void_p(film.data_ptr()),
                                   ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st)))
         stats = {"rays": st.rays, "passes": st.iterations, "ms": st.ms}
@@ -323,8 +334,7 @@ class Scene:
         film = torch.empty((self.height, self.width, 3), dtype=torch.float32, device=dev)
         st = lmx_render_stats()
         if stream is None:
-            stream = torch.cuda.Stream(dev)
-            stream.wait_stream(torch.cuda.current_stream(dev))
+            stream = self._render_stream()
         check(lib.lmx_render_raycast(self._h, ctypes.c_void_p(film.data_ptr()), ctypes.c_void_p(stream.cuda_stream),
                                      ctypes.byref(st)))
         return film, {"rays": st.rays, "ms": st.ms}
@@ -353,8 +363,7 @@ class Scene:
             p.sample_out = ctypes.c_void_p(so.data_ptr())
         st = lmx_render_stats()
         if stream is None:
-            stream = torch.cuda.Stream(dev)
-            stream.wait_stream(torch.cuda.current_stream(dev))
+            stream = self._render_stream()
         check(getattr(lib, entry)(self._h, ctypes.byref(p), ctypes.c_void_p(film.data_ptr()),
                                   ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st)))
         stats = {"rays": st.rays, "batches": st.iterations, "batch": st.queue_capacity, "ms": st.ms,
@@ -411,10 +420,8 @@ class Scene:
             p.sample_out = ctypes.c_void_p(so.data_ptr())
         st = lmx_render_stats()
         if stream is None:
-            # the renderer captures a hipGraph, which needs a non-legacy
-            # stream; order it after the work already queued on torch's
-            stream = torch.cuda.Stream(dev)
-            stream.wait_stream(torch.cuda.current_stream(dev))
+            # a non-legacy stream: the renderer replays hipGraphs on it
+            stream = self._render_stream()
         check(getattr(lib, _entry)(self._h, ctypes.byref(p), ctypes.c_void_p(film.data_ptr()),
                                    ctypes.c_void_p(stream.cuda_stream), ctypes.byref(st)))
         stats = {"rays": st.rays, "iterations": st.iterations, "queue_capacity": st.queue_capacity, "ms": st.ms,
